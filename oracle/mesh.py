@@ -1,0 +1,151 @@
+"""Structured Q1 meshes and boundary-condition index sets (oracle, test-only).
+
+Follows jax_fem_cfd/setup/regular_mesh.py and jax_fem_cfd/setup/simulation.py.
+"""
+import numpy as np
+
+
+def create_mesh_2d(nex, ney, lx, ly):
+    """Rectangular Q1 mesh.  regular_mesh.py:5-43.
+
+    Node (i, j) -> j*nnx + i at (i*dx, j*dy); element (ie, je) -> je*nex + ie
+    with counter-clockwise nodes [BL, BR, TR, TL].  Faces [down, right, top,
+    left]; bconns are consecutive node pairs along each face; outward normals.
+    """
+    nnx, nny = nex + 1, ney + 1
+    dx, dy = lx / nex, ly / ney
+    i = np.arange(nnx)
+    j = np.arange(nny)
+    coords = np.stack(np.broadcast_arrays(i[None, :] * dx, j[:, None] * dy), axis=-1).reshape(-1, 2)
+    ie = np.arange(nex)[None, :]
+    je = np.arange(ney)[:, None]
+    n0 = je * nnx + ie
+    nconn = np.stack([n0, n0 + 1, n0 + nnx + 1, n0 + nnx], axis=-1).reshape(-1, 4).astype(np.int64)
+    surf = [np.arange(nnx, dtype=np.int64),
+            (nnx * np.arange(1, nny + 1) - 1).astype(np.int64),
+            (nnx * (nny - 1) + np.arange(nnx)).astype(np.int64),
+            (nnx * np.arange(nny)).astype(np.int64)]
+    bconns = [np.stack([s[:-1], s[1:]], axis=1) for s in surf]
+    normals = np.array([[0, -1], [1, 0], [0, 1], [-1, 0]], dtype=np.int64)
+    nvecs = [np.tile(normals[f], (bconns[f].shape[0], 1)) for f in range(4)]
+    return coords, nconn, surf, bconns, nvecs
+
+
+def create_periodic_mesh_2d(nex, ney, lx, ly):
+    """Doubly periodic Q1 mesh, no duplicated nodes.  regular_mesh.py:149-180."""
+    nnx, nny = nex, ney
+    dx, dy = lx / nex, ly / ney
+    i = np.arange(nnx)
+    j = np.arange(nny)
+    coords = np.stack(np.broadcast_arrays(i[None, :] * dx, j[:, None] * dy), axis=-1).reshape(-1, 2)
+    e = np.arange(nex * ney)
+    ie, je = e % nex, e // nex
+    n0 = ie + je * nnx
+    n1 = (ie + 1) % nnx + je * nnx
+    n2 = (ie + 1) % nnx + ((je + 1) % nny) * nnx
+    n3 = ie + ((je + 1) % nny) * nnx
+    nconn = np.stack([n0, n1, n2, n3], axis=1).astype(np.int64)
+    return coords, nconn, None, None, None
+
+
+def create_mesh_3d(nex, ney, nez, lx, ly, lz):
+    """Box Q1 hex mesh.  regular_mesh.py:45-105 (node k*nnx*nny + j*nnx + i;
+    hex nodes = quad at k ++ quad at k+1; faces [D, R, T, L, F, B])."""
+    nnx, nny, nnz = nex + 1, ney + 1, nez + 1
+    dx, dy, dz = lx / nex, ly / ney, lz / nez
+    k, j, i = np.meshgrid(np.arange(nnz), np.arange(nny), np.arange(nnx), indexing="ij")
+    coords = np.stack([i * dx, j * dy, k * dz], axis=-1).reshape(-1, 3)
+    ke, je, ie = np.meshgrid(np.arange(nez), np.arange(ney), np.arange(nex), indexing="ij")
+    q = je * nnx + ie
+    quad = np.stack([q, q + 1, q + nnx + 1, q + nnx], axis=-1)
+    lay = nnx * nny
+    nconn = np.concatenate([ke[..., None] * lay + quad, (ke[..., None] + 1) * lay + quad],
+                           axis=-1).reshape(-1, 8).astype(np.int64)
+    n3 = nconn.reshape(nez, ney, nex, 8)
+    # Same mixed basic/advanced indexing as regular_mesh.py:77-82.  Under
+    # NumPy (and JAX) rules the face-local axis moves to the FRONT when the
+    # two advanced indices are separated by a slice (faces D, T, F, B), so the
+    # reshaped rows of those faces are not per-face-element node quadruples.
+    # The quirk is restated as is.
+    bD = n3[:, 0, :, [0, 1, 5, 4]].reshape(-1, 4)
+    bR = n3[:, :, -1, [1, 5, 6, 2]].reshape(-1, 4)
+    bT = n3[:, -1, :, [3, 2, 6, 7]].reshape(-1, 4)
+    bL = n3[:, :, 0, [0, 4, 7, 3]].reshape(-1, 4)
+    bF = n3[0, :, :, [0, 1, 2, 3]].reshape(-1, 4)
+    bB = n3[-1, :, :, [4, 5, 6, 7]].reshape(-1, 4)
+    bconns = [bD, bR, bT, bL, bF, bB]
+    zoff = np.repeat(np.arange(nnz) * lay, nnx)
+    zoffy = np.repeat(np.arange(nnz) * lay, nny)
+    surf = [np.tile(np.arange(nnx), nnz) + zoff,
+            np.tile(nnx * np.arange(1, nny + 1) - 1, nnz) + zoffy,
+            np.tile(nnx * (nny - 1) + np.arange(nnx), nnz) + zoff,
+            np.tile(nnx * np.arange(nny), nnz) + zoffy,
+            np.arange(lay),
+            lay * (nnz - 1) + np.arange(lay)]
+    surf = [s.astype(np.int64) for s in surf]
+    normals = np.array([[0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, -1], [0, 0, 1]])
+    nvecs = [np.tile(normals[f], (bconns[f].shape[0], 1)) for f in range(6)]
+    return coords, nconn, surf, bconns, nvecs
+
+
+def mesh_func_for(ndim, periodic):
+    if ndim == 2:
+        return create_periodic_mesh_2d if periodic else create_mesh_2d
+    return create_mesh_3d
+
+
+def simple_mesh(num_elem, domain_size, periodic, dirichlet_faces, outlet_faces,
+                partial_dirichlet=None):
+    """Mesh + BC index sets.  simulation.py:96-183.
+
+    gamma_d: sorted unique Dirichlet nodes tiled per component (+nn*i);
+    U_d zeros; bconn/nvec concatenated over Dirichlet faces; gamma_p: unique
+    outlet nodes, else the scalar 0.
+    """
+    ndim = len(num_elem)
+    mesh_func = mesh_func_for(ndim, periodic)
+    coords, nconn, surf, bconns, nvecs = mesh_func(*num_elem, *domain_size)
+    nn = coords.shape[0]
+    h = domain_size[0] / num_elem[0]
+    bconn = nvec = gamma_d = U_d = None
+    dirichlet_nodes, outlet_nodes, bconn_list, nvec_list = [], [], [], []
+    if surf is not None:
+        surf = list(surf)
+    for f in dirichlet_faces or []:
+        dirichlet_nodes.append(surf[f])
+        bconn_list.append(bconns[f])
+        nvec_list.append(nvecs[f])
+    for f in outlet_faces or []:
+        outlet_nodes.append(surf[f])
+    if partial_dirichlet:
+        for f, (s_frac, e_frac, rem) in partial_dirichlet.items():
+            nodes = surf[f]
+            nfe = bconns[f].shape[0]
+            s_idx = round(s_frac * nfe)
+            e_idx = round(e_frac * nfe)
+            bconn_list.append(bconns[f][s_idx:e_idx])
+            nvec_list.append(nvecs[f][s_idx:e_idx])
+            surf[f] = nodes[s_idx:e_idx + 1]
+            dirichlet_nodes.append(nodes[s_idx:e_idx + 1])
+            if rem == "outlet":
+                outlet_nodes.append(nodes[:s_idx])
+                outlet_nodes.append(nodes[e_idx + 1:])
+    if dirichlet_nodes:
+        g = np.unique(np.concatenate(dirichlet_nodes))
+        gamma_d = np.concatenate([g + nn * i for i in range(ndim)]).astype(np.int64)
+        U_d = np.zeros(gamma_d.shape[0])
+        bconn = np.concatenate(bconn_list, axis=0)
+        nvec = np.concatenate(nvec_list, axis=0)
+    if outlet_nodes:
+        gamma_p = np.unique(np.concatenate(outlet_nodes)).astype(np.int64)
+    else:
+        gamma_p = 0
+    return coords, nconn, surf, nn, h, bconn, nvec, gamma_d, U_d, gamma_p, mesh_func
+
+
+def set_face_velocity(U_d, nodes, gamma_d, values, nn, var):
+    """simulation.py:219-226 (searchsorted into the sorted gamma_d)."""
+    idx = np.searchsorted(gamma_d, np.asarray(nodes) + nn * (var - 1))
+    out = np.array(U_d, dtype=float, copy=True)
+    out[idx] = values
+    return out
