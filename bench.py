@@ -1,0 +1,206 @@
+"""Benchmark: full 4-step timesteps/s on the 1024x1024 Q1 lid-driven cavity.
+
+Workload (one "step" = one four_step_timestep): 2D walled unit box, 1024x1024
+Q1 elements, lid u = 1 (corners excluded), no-slip elsewhere, Re = 1000
+(rho = 1, mu = 1e-3), U0 = N(0, 1e-3) noise from seed 1 with the Dirichlet
+values applied, P0 = 0, dt = 5e-4; BiCGSTAB momentum + geometric-multigrid
+PCG pressure (1024^2 -> 16^2, pinned pressure), both to 1e-8.
+
+`value` is the mixed-precision form (FP32 multigrid inside an FP64 flexible
+CG, FP32 stored convection matrices, FP64 Krylov vectors, dots and
+projection).  The FP64 form (the reference's algorithm) runs on the same GPU
+in the same job and is reported beside it, with the per-field difference of
+the two after the timed steps.  Also reported: the pure-Neumann 1024^2
+Poisson solve alone (solves/s, FP64 and mixed, 1e-8), the roofline of the
+dominant kernel (momentum operator apply) and the CPU oracle timed on this
+host on a bounded sample.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "jax-fem-cfd_amd"))
+
+import numpy as np  # noqa: E402
+
+N_EL = 1024
+RHO, MU, DT, TOL = 1.0, 1e-3, 5e-4, 1e-8
+HBM_PEAK_GBS = 8000.0
+
+
+def _cavity(precision, ce_format, n=N_EL):
+    import jax_fem_cfd.setup.simulation as sim
+    cfg = sim.Config(mesh="simple", solver="standard",
+                     mesh_config=sim.SimpleMeshConfig([n, n], [1.0, 1.0], 1, False),
+                     solver_config=sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, TOL, True, "multigrid",
+                                                            [16, 16], precision=precision, ce_format=ce_format))
+    compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
+    U_d = sim.set_face_velocity(U_d, surf[2][1:-1], gamma_d, 1.0, nn, 1)
+    U0 = np.random.default_rng(1).normal(0, 1e-3, 2 * nn)
+    U0[gamma_d] = U_d
+    return compute, timestep, U_d, U0, np.zeros(nn), nn
+
+
+def run_form(precision, ce_format, steps, warmup):
+    import torch
+    compute, timestep, U_d, U0, P0, nn = _cavity(precision, ce_format)
+    F_p, F = compute(U_d, None, 0, 0)
+    U = torch.as_tensor(U0, device="cuda")
+    P = torch.as_tensor(P0, device="cuda")
+    Ud = torch.as_tensor(U_d, device="cuda")
+    for _ in range(warmup):
+        U, P, ui, pi = timestep(U, P, DT, RHO, MU, Ud, F_p, F)
+    torch.cuda.synchronize()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        U, P, ui, pi = timestep(U, P, DT, RHO, MU, Ud, F_p, F)
+        its.append((ui, pi))
+    torch.cuda.synchronize()
+    dt_s = time.perf_counter() - t0
+    ctx = timestep.context
+    return dict(sec=dt_s, its=its, U=U, P=P, ctx=ctx, nn=nn)
+
+
+def poisson_alone(precision, solves=3):
+    """Pure-Neumann 1024^2 Poisson: rhs N(0,1) seed 0 projected to zero mean, x0 = 0,
+    MG 1024^2 -> 16^2, PCG to 1e-8 relative residual."""
+    import torch
+    from jax_fem_cfd._context import Context
+    n = N_EL
+    c = Context(n, n, 1.0, 1.0, False, tol=TOL, set_zeroP=False, precond="multigrid", mg_levels=7,
+                precision=precision)
+    b = np.random.default_rng(0).standard_normal(c.nn)
+    b -= b.mean()
+    bt = torch.as_tensor(b, device="cuda")
+    x, k, conv = c.pressure_cg(bt, tol=TOL)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(solves):
+        x, k, conv = c.pressure_cg(bt, tol=TOL)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / solves
+    # true FP64 residual through the native Laplacian
+    Le = c.element_constant("Le")
+    Ax = c.elem_mult_const(Le, x)
+    res = float(torch.linalg.norm(bt - Ax) / torch.linalg.norm(bt))
+    return dict(solves_per_s=1.0 / sec, ms_per_solve=sec * 1e3, iters=k, converged=conv, true_rel_residual=res)
+
+
+def cpu_baseline(steps=2):
+    """Oracle (NumPy FP64 restatement) on this host, one thread, same workload,
+    bounded sample of `steps` timesteps after set-up."""
+    code = f"""
+import sys, time, json
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from oracle import ns
+s, surf, h, U_d = ns.build([{N_EL}, {N_EL}], [1.0, 1.0], False, [0, 1, 2, 3], [], {TOL}, True, "multigrid", [16, 16])
+U_d = ns.cavity_lid(s, surf, U_d)
+U = np.random.default_rng(1).normal(0, 1e-3, 2 * s.nn); U[s.gamma_d] = U_d
+P = np.zeros(s.nn)
+t0 = time.perf_counter()
+its = []
+for _ in range({steps}):
+    U, P, ui, pi = ns.four_step_timestep(s, U, P, {DT}, {RHO}, {MU}, U_d, 0, 0)
+    its.append((ui, pi))
+sec = time.perf_counter() - t0
+print(json.dumps(dict(sec=sec, steps={steps}, its=its)))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        return dict(error=out.stderr[-400:])
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    r["value"] = r["steps"] / r["sec"]
+    return r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the timestep is a single-GPU path; "
+              "multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+
+    f64 = run_form("fp64", "f64", args.steps, args.warmup)
+    mix = run_form("mixed", "f32", args.steps, args.warmup)
+    nn = mix["nn"]
+    U64, P64, Um, Pm = (f64["U"].cpu().numpy(), f64["P"].cpu().numpy(), mix["U"].cpu().numpy(),
+                        mix["P"].cpu().numpy())
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    # dominant kernel: momentum operator apply (BiCGSTAB s = r - alpha q, t = A s, 3 dots)
+    ne = N_EL * N_EL
+    reps = 50
+    ms_m = mix["ctx"].bench_momentum_apply(reps)
+    ms_d = f64["ctx"].bench_momentum_apply(reps)
+    bytes_m = 16 * ne * 4 + 8 * nn * 8      # FP32 Ce + r, q read, s, t written (2 components, FP64)
+    bytes_d = 16 * ne * 8 + 8 * nn * 8
+    ach_m = bytes_m / (ms_m * 1e-3) / 1e9
+    ach_d = bytes_d / (ms_d * 1e-3) / 1e9
+
+    pois64 = poisson_alone("fp64")
+    poism = poisson_alone("mixed")
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps)
+
+    steps_s_m = args.steps / mix["sec"]
+    steps_s_d = args.steps / f64["sec"]
+    traffic = None
+    line = {
+        "metric": "timesteps/s 1024² Q1: mixed-prec vs FP64 MG-PCG at 1e-8 residual",
+        "value": steps_s_m,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mix["sec"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 (mixed)",
+        "data": "synthetic (seeded N(0,1e-3) initial velocity)",
+        "config": {"workload": "lid-driven cavity 1024x1024 Q1, Re=1000, dt=5e-4, BiCGSTAB + MG-PCG "
+                               "(1024->16, pinned p), tol 1e-8, Ce FP32, hipStream host loop",
+                   "elements": ne, "nodes": nn, "mg_levels": 7},
+        "fp64": {"timesteps_per_s": steps_s_d, "ms_per_step": 1e3 * f64["sec"] / args.steps,
+                 "mean_iters": [float(np.mean([i[0] for i in f64["its"]])),
+                                float(np.mean([i[1] for i in f64["its"]]))]},
+        "mixed": {"mean_iters": [float(np.mean([i[0] for i in mix["its"]])),
+                                 float(np.mean([i[1] for i in mix["its"]]))]},
+        "speedup_mixed_vs_fp64": steps_s_m / steps_s_d,
+        "mixed_vs_fp64_rel_l2": {"u": rel(Um[:nn], U64[:nn]), "v": rel(Um[nn:], U64[nn:]), "p": rel(Pm, P64)},
+        "poisson_1024_neumann": {"fp64": pois64, "mixed": poism},
+        "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_m / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "node_kernel<3, MomK2<CeF32>> (BiCGSTAB s/t: momentum operator apply + 3 dots)",
+                     "bytes_per_launch": bytes_m, "us_per_launch": ms_m * 1e3,
+                     "fp64_ce": {"achieved": ach_d, "bytes_per_launch": bytes_d, "us_per_launch": ms_d * 1e3}},
+        "cpu_baseline": None if cpu is None else (
+            {"error": cpu["error"]} if "error" in cpu else
+            {"value": cpu["value"], "unit": "timesteps/s", "cores": 1, "kind": "port",
+             "sample": f"{cpu['steps']} timesteps of the same 1024^2 cavity (FP64 oracle, NumPy, 1 thread), "
+                       f"iters {cpu['its']}"}),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,195 @@
+/*
+ * fcfd.h -- C ABI of the MI355X (gfx950) matrix-free Navier-Stokes timestep.
+ *
+ * Drop-in boundary for the hot path of jax-fem-cfd: the 4-step pressure
+ * projection (per-element convection matrices, element-by-element matvecs,
+ * BiCGSTAB momentum solve, multigrid-preconditioned CG pressure solve).
+ * The reference has no FFI of its own -- its boundary is the Python call
+ * surface -- so each entry point below names the reference function it
+ * replaces (paths relative to the reference repository).
+ *
+ * Conventions
+ *   - All vector arguments are DEVICE pointers (hipMalloc / torch CUDA
+ *     tensors), FP64, in the reference layout: nodal vectors (nn,), velocity
+ *     component-major [u; v] (2*nn,), element matrices (ne, 4, 4) row-major.
+ *   - Index arrays passed to fcfd_ctx_create are HOST pointers (int64).
+ *   - Every call is asynchronous on the context stream unless it returns
+ *     iteration counts (those synchronise the stream once at the end).
+ *   - Return value: FCFD_OK (0) or a negative FCFD_ERR_*; fcfd_last_error()
+ *     gives the message.  Solver iteration counts mirror the reference:
+ *     BiCGSTAB returns -10 (rho = 0) or -11 (omega or alpha = 0) on
+ *     breakdown; *converged tells maxiter exhaustion from success.
+ *   - A context is not thread safe; one stream per context.
+ */
+#ifndef FCFD_H
+#define FCFD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fcfd_ctx fcfd_ctx;
+
+enum {
+    FCFD_OK = 0,
+    FCFD_ERR_ARG = -1,
+    FCFD_ERR_HIP = -2,
+    FCFD_ERR_UNSUPPORTED = -3,
+    FCFD_ERR_STATE = -4
+};
+
+/* pressure preconditioner (StandardSolverConfig.pressure_precond) */
+enum { FCFD_PC_NONE = 0, FCFD_PC_JACOBI = 1, FCFD_PC_APPROX_INV = 2, FCFD_PC_MULTIGRID = 3 };
+
+/* solver precision policy */
+enum {
+    FCFD_PREC_FP64 = 0,   /* the reference's algorithm, FP64 throughout            */
+    FCFD_PREC_MIXED = 1   /* FP64 outer flexible CG / FP64 Krylov vectors, FP32
+                             multigrid, reduced-precision stored element matrices  */
+};
+
+/* storage format of the per-element convection matrices */
+enum { FCFD_CE_F64 = 0, FCFD_CE_F32 = 1, FCFD_CE_F16S = 2, FCFD_CE_BF16S = 3 };
+
+typedef struct fcfd_mesh_desc {
+    int32_t ndim;          /* 2 */
+    int32_t nex, ney;      /* elements per direction */
+    double lx, ly;         /* domain size */
+    int32_t periodic;      /* 1: create_periodic_mesh_2d, 0: create_mesh_2d */
+} fcfd_mesh_desc;
+
+typedef struct fcfd_solver_desc {
+    double tol;                 /* solver_tol: BiCGSTAB, PCG and the coarse CG   */
+    int32_t set_zeroP;          /* pin gamma_p (1) or pure-Neumann mean form (0) */
+    int32_t precond;            /* FCFD_PC_*                                     */
+    int32_t mg_levels;          /* multigrid levels incl. the fine one (>= 1)    */
+    int32_t precision;          /* FCFD_PREC_*                                   */
+    int32_t ce_format;          /* FCFD_CE_* used by the timestep                */
+    int32_t maxiter_momentum;   /* <= 0: 10 * size, as the reference             */
+    int32_t maxiter_pressure;   /* <= 0: 1000, as the reference                  */
+    const int64_t* gamma_d;     /* sorted velocity Dirichlet DOF ids or NULL     */
+    int64_t n_gamma_d;
+    const int64_t* gamma_p;     /* fine-level pressure pins (pinned form)        */
+    int64_t n_gamma_p;
+    const int64_t* mg_gamma_p;      /* per-level pins, concatenated              */
+    const int64_t* mg_gamma_p_off;  /* mg_levels + 1 offsets into mg_gamma_p     */
+    const int64_t* bconn;       /* (n_bedge, 2) Dirichlet boundary edges         */
+    const int64_t* nvec;        /* (n_bedge, 2) outward normals                  */
+    int64_t n_bedge;
+} fcfd_solver_desc;
+
+/* ---- context --------------------------------------------------------------------------- */
+
+/* replaces setup_solver (jax_fem_cfd/solvers/standard.py:191-247): element
+ * constants from element 0, lumped mass, multigrid hierarchy
+ * (jax_fem_cfd/core/multigrid.py:9-88), Dirichlet masks; allocates every
+ * workspace the hot calls use.  stream: hipStream_t (NULL = default). */
+int fcfd_ctx_create(const fcfd_mesh_desc* mesh, const fcfd_solver_desc* solver, void* stream,
+                    fcfd_ctx** out);
+int fcfd_ctx_destroy(fcfd_ctx* ctx);
+int fcfd_set_stream(fcfd_ctx* ctx, void* stream);
+const char* fcfd_last_error(void);
+/* nn, ne, number of multigrid levels, nodes of level l (l = -1: returns levels) */
+int fcfd_ctx_info(const fcfd_ctx* ctx, int64_t* nn, int64_t* ne, int32_t* levels);
+int fcfd_level_nodes(const fcfd_ctx* ctx, int32_t level, int64_t* nn, int32_t* nnx, int32_t* nny);
+
+/* ---- mesh --------------------------------------------------------------------------- */
+
+/* node coordinates (nn, 2) and connectivity (ne, 4) from the device index
+ * math the kernels use.  replaces create_mesh_2d / create_periodic_mesh_2d
+ * (jax_fem_cfd/setup/regular_mesh.py:5-43, :149-180). */
+int fcfd_mesh_fill(fcfd_ctx* ctx, double* coords, int64_t* nconn);
+
+/* host copies of the FP64 element constants (16 doubles each, row-major):
+ * which = 0 Me, 1 Le, 2 Ge[0], 3 Ge[1], 4 Le_off; plus lumped mass (2*nn, device).
+ * replaces lumped_mass_matrix / laplacian_element_calc / gradient_element_calc
+ * (jax_fem_cfd/core/element_calc.py:9-40). */
+int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16);
+int fcfd_lumped_mass(fcfd_ctx* ctx, double* M);
+/* L_diag of level l (device out).  replaces laplacian_diag (element_calc.py:99-109) */
+int fcfd_laplacian_diag(fcfd_ctx* ctx, int32_t level, double* out);
+
+/* ---- element calculations ----------------------------------------------------------- */
+
+/* per-element convection matrices (ne, 4, 4) in the reference layout.
+ * replaces conservative_convec_elem_calc (jax_fem_cfd/core/element_calc.py:165-185) */
+int fcfd_conv_elem_calc(fcfd_ctx* ctx, const double* U, double beta, double* Ce);
+
+/* F_p = int_Gamma q (U_d . n).  replaces u_dot_n_boundary_integral_1d
+ * (element_calc.py:111-135) as called by compute (standard.py:167-188).
+ * U_d: device (n_gamma_d,). */
+int fcfd_boundary_integral(fcfd_ctx* ctx, const double* U_d, double* F_p);
+
+/* ---- element-by-element matvecs (jax_fem_cfd/core/element_mult.py) ------------------ */
+
+/* y = A x per component with one shared element matrix A (16 doubles, host).
+ * Covers consistent_mass_mult (:11-36), laplacian_element_mult (:38-49,
+ * also with Le_off), gradient_element_mult (:51-77, one call per G_i) and
+ * divergence_element_mult (:133-157, A = G_i^T, accumulate = 1).
+ * level selects the multigrid grid (0 = fine).  accumulate: y += A x. */
+int fcfd_elem_mult_const(fcfd_ctx* ctx, int32_t level, const double* A16, const double* x,
+                         double* y, int32_t accumulate);
+
+/* y_c = sum_e (K0 + c1 A1_e + c2 A2_e) x_c per component (ncomp 1..3);
+ * K0: shared element matrix (16 doubles, host, NULL = 0); A1, A2: per-element
+ * matrices (ne,4,4) device or NULL.  Replaces visc_conv_mult (:79-104;
+ * K0 = mu Le, A1 = Ce, c1 = rho), scalar_visc_conv_mult (:159-172),
+ * scalar_stabilized_visc_conv_mult (:174-187; A2 = Se) and
+ * scalar_consistent_mass_SUPG_mult (:190-201; K0 = Me, A1 = Me_tau). */
+int fcfd_elem_mult_stored(fcfd_ctx* ctx, const double* K0_16, const double* A1, double c1,
+                          const double* A2, double c2, int32_t ncomp, const double* x, double* y);
+
+/* ---- multigrid and solvers --------------------------------------------------------- */
+
+/* z = V(r), one V-cycle from zero.  replaces v_cycle (jax_fem_cfd/core/multigrid.py:231-330) */
+int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z);
+
+/* PCG on the (pinned or Neumann) pressure Laplacian with the context's
+ * preconditioner; x holds x0 on entry.  Mirrors the cg call inside
+ * solve_pressure (standard.py:114-117) and _cg_solve
+ * (jax_fem_cfd/utils/jax_iterative_solvers.py:87-120): stops on the recursive
+ * residual r.r <= max(tol^2 b.b, 0) or maxiter. */
+int fcfd_pressure_cg(fcfd_ctx* ctx, const double* b, double* x, double tol, int32_t maxiter,
+                     int32_t* iters, int32_t* converged);
+
+/* solve_pressure (standard.py:85-125), gamma = 0: rhs = rho/dt (D U* - F_p),
+ * pin or mean-project, x0 = P_n, PCG, post-process.  F_p may be NULL (zero). */
+int fcfd_solve_pressure(fcfd_ctx* ctx, const double* U_star, const double* P_n,
+                        const double* F_p, double rho, double dt, double* P, int32_t* iters);
+
+/* solve_momentum (standard.py:14-43), theta = 1/2, gamma = 1: BiCGSTAB
+ * (jax_iterative_solvers.py:123-167) on (rho/dt) M_c + theta (rho Ce + mu Le)
+ * with Dirichlet rows -> identity.  Ce: (ne,4,4) device (NULL: the context
+ * builds it from U_n with beta = 1/2).  F may be NULL. */
+int fcfd_solve_momentum(fcfd_ctx* ctx, const double* U_n, const double* P_n, const double* Ce,
+                        double rho, double mu, double dt, const double* U_d, const double* F,
+                        double* U_hat, int32_t* iters);
+
+/* four_step_timestep (standard.py:148-165): Ce(U_n), momentum, U*, pressure,
+ * projection, Dirichlet values.  Uses the context's precision policy.
+ * U_d: device (n_gamma_d,) or NULL; F_p, F: device or NULL (zero).
+ * iters: host int32[2] = (U_iters, P_iters), written after one stream sync. */
+int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, double dt,
+                            double rho, double mu, const double* U_d, const double* F_p,
+                            const double* F, double* U, double* P, int32_t* iters);
+
+/* ---- BLAS-1 used by the generic cg / bicgstab wrappers ----------------------------- */
+
+/* *out (host) = x . y, FP64, deterministic two-stage reduction */
+int fcfd_dot(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out);
+/* y = a x + b y */
+int fcfd_axpby(fcfd_ctx* ctx, double a, const double* x, double b, double* y, int64_t n);
+
+/* ---- timing of the dominant kernel (momentum operator apply) ----------------------- */
+
+/* Average device time (ms) of `reps` momentum-operator applications
+ * (mass + theta(rho Ce + mu Le), both components, Dirichlet rows) on the
+ * context's stored Ce, measured with HIP events on the context stream. */
+int fcfd_bench_momentum_apply(fcfd_ctx* ctx, int32_t reps, float* ms_per_apply);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FCFD_H */

@@ -1,0 +1,478 @@
+// Context construction and the extern "C" entry points declared in include/fcfd.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include "fcfd_internal.hpp"
+
+using namespace fcfd;
+
+struct fcfd_ctx {
+    Ctx c;
+};
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guard(F&& f) {
+    try {
+        f();
+        return FCFD_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return FCFD_ERR_ARG;
+    }
+}
+
+namespace {
+
+// L_diag and lumped mass on a level: per node, the sum over its incident elements (in element
+// order) of diag(Le)[a] and of the row sums of Me (element_calc.py:9-25, :99-109).
+void node_sums(const Grid2D& g, const double* per_row, std::vector<double>& out) {
+    out.assign(g.nn, 0.0);
+    for (int e = 0; e < g.ne; ++e) {
+        int je = e / g.nex, ie = e - je * g.nex;
+        int c[4];
+        if (g.periodic) {
+            int ip = (ie + 1) % g.nnx, jp = (je + 1) % g.nny;
+            c[0] = je * g.nnx + ie; c[1] = je * g.nnx + ip; c[2] = jp * g.nnx + ip; c[3] = jp * g.nnx + ie;
+        } else {
+            int n0 = je * g.nnx + ie;
+            c[0] = n0; c[1] = n0 + 1; c[2] = n0 + g.nnx + 1; c[3] = n0 + g.nnx;
+        }
+        for (int a = 0; a < 4; ++a) out[c[a]] += per_row[a];
+    }
+}
+
+template <class T>
+void upload(const std::vector<T>& h, T* d) {
+    FCFD_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void* stream) {
+    FCFD_REQUIRE(m && s, "null descriptor");
+    FCFD_REQUIRE(m->ndim == 2, "only 2D meshes are supported by this build");
+    FCFD_REQUIRE(m->nex >= 1 && m->ney >= 1, "bad element counts");
+    c.stream = (hipStream_t)stream;
+    c.g = make_grid(m->nex, m->ney, m->periodic);
+    c.lx = m->lx; c.ly = m->ly;
+    c.tol = s->tol;
+    c.set_zeroP = s->set_zeroP;
+    c.precond = s->precond;
+    c.precision = s->precision;
+    c.ce_format = s->ce_format;
+    c.maxit_mom = s->maxiter_momentum;
+    c.maxit_p = s->maxiter_pressure;
+    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_MULTIGRID,
+                 "pressure preconditioner not supported (approx_inv is not built)");
+    const Grid2D& g = c.g;
+    const int nn = g.nn;
+    quad4_constants(c.lx / g.nex, c.ly / g.ney, c.ec);
+    c.ec_dev = dalloc<ElemConst>(c, 1);
+    FCFD_HIP(hipMemcpy(c.ec_dev, &c.ec, sizeof(ElemConst), hipMemcpyHostToDevice));
+
+    // lumped mass, tiled per component
+    {
+        double rs[4];
+        for (int a = 0; a < 4; ++a) {
+            rs[a] = 0.0;
+            for (int b = 0; b < 4; ++b) rs[a] += c.ec.Me[a * 4 + b];
+        }
+        std::vector<double> M;
+        node_sums(g, rs, M);
+        M.resize(2 * nn);
+        std::copy(M.begin(), M.begin() + nn, M.begin() + nn);
+        c.Mlump = dalloc<double>(c, 2 * nn);
+        upload(M, c.Mlump);
+    }
+
+    // multigrid levels (level 0 always exists: L_diag and scratch of the fine grid)
+    int nlev = c.precond == FCFD_PC_MULTIGRID ? s->mg_levels : 1;
+    FCFD_REQUIRE(nlev >= 1 && nlev <= MAX_LEVELS, "bad number of multigrid levels");
+    if (nlev > 1) {
+        FCFD_REQUIRE((g.nex >> (nlev - 1)) >= 1 && (g.ney >> (nlev - 1)) >= 1, "too many multigrid levels");
+        FCFD_REQUIRE(g.nex % (1 << (nlev - 1)) == 0 && g.ney % (1 << (nlev - 1)) == 0,
+                     "element counts must be divisible by 2^(levels-1)");
+    }
+    c.nlev = nlev;
+    c.lv.resize(nlev);
+    const bool mixed = c.precision == FCFD_PREC_MIXED;
+    for (int l = 0; l < nlev; ++l) {
+        Level& L = c.lv[l];
+        L.g = make_grid(g.nex >> l, g.ney >> l, g.periodic);
+        quad4_constants(c.lx / L.g.nex, c.ly / L.g.ney, L.ec);
+        L.ec_dev = dalloc<ElemConst>(c, 1);
+        FCFD_HIP(hipMemcpy(L.ec_dev, &L.ec, sizeof(ElemConst), hipMemcpyHostToDevice));
+        double dg[4] = {L.ec.Le[0], L.ec.Le[5], L.ec.Le[10], L.ec.Le[15]};
+        std::vector<double> Ld;
+        node_sums(L.g, dg, Ld);
+        L.Ld = dalloc<double>(c, L.g.nn);
+        upload(Ld, L.Ld);
+        std::vector<float> Ld32(Ld.begin(), Ld.end());
+        L.Ld32 = dalloc<float>(c, L.g.nn);
+        upload(Ld32, L.Ld32);
+        const int n = L.g.nn;
+        if (l > 0) L.r = dalloc<double>(c, n);
+        L.x = dalloc<double>(c, n);
+        L.y = dalloc<double>(c, n);
+        L.f = dalloc<double>(c, n);
+        if (mixed) {
+            if (l > 0) L.r32 = dalloc<float>(c, n);
+            L.x32 = dalloc<float>(c, n);
+            L.y32 = dalloc<float>(c, n);
+            L.f32 = dalloc<float>(c, n);
+        }
+        if (c.set_zeroP) {
+            std::vector<uint8_t> pin(n, 0);
+            if (s->mg_gamma_p && s->mg_gamma_p_off) {
+                for (int64_t k = s->mg_gamma_p_off[l]; k < s->mg_gamma_p_off[l + 1]; ++k) {
+                    int64_t v = s->mg_gamma_p[k];
+                    FCFD_REQUIRE(v >= 0 && v < n, "multigrid pin out of range");
+                    pin[v] = 1;
+                    L.pins.push_back(v);
+                }
+            } else {
+                pin[0] = 1;
+                L.pins.push_back(0);
+            }
+            L.pin = dalloc<uint8_t>(c, n);
+            upload(pin, L.pin);
+        }
+    }
+    {
+        const int ncoarse = c.lv[nlev - 1].g.nn;
+        c.cc_r = dalloc<double>(c, ncoarse);
+        c.cc_p = dalloc<double>(c, ncoarse);
+        c.cc_Ap = dalloc<double>(c, ncoarse);
+        if (mixed) {
+            c.cc_r32 = dalloc<float>(c, ncoarse);
+            c.cc_p32 = dalloc<float>(c, ncoarse);
+            c.cc_Ap32 = dalloc<float>(c, ncoarse);
+        }
+    }
+
+    // fine-level pressure pins
+    if (c.set_zeroP) {
+        std::vector<uint8_t> pin(nn, 0);
+        if (s->gamma_p && s->n_gamma_p > 0) {
+            for (int64_t k = 0; k < s->n_gamma_p; ++k) {
+                FCFD_REQUIRE(s->gamma_p[k] >= 0 && s->gamma_p[k] < nn, "gamma_p out of range");
+                pin[s->gamma_p[k]] = 1;
+            }
+        } else {
+            pin[0] = 1;
+        }
+        c.ppin = dalloc<uint8_t>(c, nn);
+        upload(pin, c.ppin);
+    }
+
+    // Dirichlet DOFs
+    std::vector<int64_t> gd;
+    if (s->gamma_d && s->n_gamma_d > 0) {
+        gd.assign(s->gamma_d, s->gamma_d + s->n_gamma_d);
+        for (size_t k = 0; k < gd.size(); ++k) {
+            FCFD_REQUIRE(gd[k] >= 0 && gd[k] < 2 * nn, "gamma_d out of range");
+            FCFD_REQUIRE(k == 0 || gd[k] > gd[k - 1], "gamma_d must be sorted and unique");
+        }
+        c.n_gamma_d = (int)gd.size();
+        std::vector<uint8_t> mask(2 * nn, 0);
+        for (auto v : gd) mask[v] = 1;
+        c.dmask = dalloc<uint8_t>(c, 2 * nn);
+        upload(mask, c.dmask);
+        c.gamma_d_dev = dalloc<int64_t>(c, gd.size());
+        upload(gd, c.gamma_d_dev);
+        c.ud_full = dalloc<double>(c, 2 * nn);
+        FCFD_HIP(hipMemset(c.ud_full, 0, sizeof(double) * 2 * nn));
+    }
+
+    // boundary edges -> per-node gather lists in bconn order (segment_sum order)
+    if (s->bconn && s->nvec && s->n_bedge > 0 && !gd.empty()) {
+        const int ne_b = (int)s->n_bedge;
+        c.n_bedge = ne_b;
+        std::vector<std::vector<int>> lists(nn);
+        std::vector<double> geom(3 * ne_b);
+        std::vector<int> ui(4 * ne_b);
+        const double dx = c.lx / g.nex, dy = c.ly / g.ney;
+        auto sorted_index = [&](int64_t v) {
+            return (int)(std::lower_bound(gd.begin(), gd.end(), v) - gd.begin());
+        };
+        for (int ed = 0; ed < ne_b; ++ed) {
+            int64_t a = s->bconn[2 * ed], b = s->bconn[2 * ed + 1];
+            FCFD_REQUIRE(a >= 0 && a < nn && b >= 0 && b < nn, "bconn out of range");
+            double xa = (double)(a % g.nnx) * dx, ya = (double)(a / g.nnx) * dy;
+            double xb = (double)(b % g.nnx) * dx, yb = (double)(b / g.nnx) * dy;
+            geom[3 * ed + 0] = (double)s->nvec[2 * ed];
+            geom[3 * ed + 1] = (double)s->nvec[2 * ed + 1];
+            geom[3 * ed + 2] = std::sqrt((xb - xa) * (xb - xa) + (yb - ya) * (yb - ya));
+            int ia[4] = {sorted_index(a), sorted_index(a + nn), sorted_index(b), sorted_index(b + nn)};
+            for (int k = 0; k < 4; ++k) {
+                FCFD_REQUIRE(ia[k] < (int)gd.size(), "boundary node is not a Dirichlet DOF");
+                ui[4 * ed + k] = ia[k];
+            }
+            lists[a].push_back(ed * 2 + 0);
+            lists[b].push_back(ed * 2 + 1);
+        }
+        std::vector<int> off(nn + 1, 0), slots;
+        for (int n = 0; n < nn; ++n) {
+            off[n + 1] = off[n] + (int)lists[n].size();
+            slots.insert(slots.end(), lists[n].begin(), lists[n].end());
+        }
+        c.fb_off = dalloc<int>(c, nn + 1);
+        upload(off, c.fb_off);
+        c.fb_edge = dalloc<int>(c, slots.size());
+        upload(slots, c.fb_edge);
+        c.fb_geom = dalloc<double>(c, geom.size());
+        upload(geom, c.fb_geom);
+        c.fb_ui = dalloc<int>(c, ui.size());
+        upload(ui, c.fb_ui);
+    }
+
+    // element matrix storage for the timestep
+    {
+        const size_t ne = g.ne;
+        switch (c.ce_format) {
+            case FCFD_CE_F64: c.ce = dalloc<double>(c, 16 * ne); break;
+            case FCFD_CE_F32: c.ce = dalloc<float>(c, 16 * ne); break;
+            case FCFD_CE_F16S:
+            case FCFD_CE_BF16S:
+                c.ce = dalloc<uint16_t>(c, 16 * ne);
+                c.ce_scale = dalloc<float>(c, ne);
+                break;
+            default: throw Error(FCFD_ERR_ARG, "unknown Ce format");
+        }
+    }
+    // solver work
+    c.m_r = dalloc<double>(c, 2 * nn);
+    c.m_rhat = dalloc<double>(c, 2 * nn);
+    for (int k = 0; k < 2; ++k) {
+        c.m_p[k] = dalloc<double>(c, 2 * nn);
+        c.m_q[k] = dalloc<double>(c, 2 * nn);
+        c.c_p[k] = dalloc<double>(c, nn);
+        c.c_z[k] = dalloc<double>(c, nn);
+    }
+    c.m_s = dalloc<double>(c, 2 * nn);
+    c.m_t = dalloc<double>(c, 2 * nn);
+    c.m_x = dalloc<double>(c, 2 * nn);
+    c.c_r = dalloc<double>(c, nn);
+    c.c_Ap = dalloc<double>(c, nn);
+    c.c_x = dalloc<double>(c, nn);
+    c.c_rhs = dalloc<double>(c, nn);
+    c.ustar = dalloc<double>(c, 2 * nn);
+    c.partials = dalloc<double>(c, (size_t)MAX_RED * MAX_BLOCKS);
+    c.counter = dalloc<unsigned>(c, 1);
+    FCFD_HIP(hipMemset(c.counter, 0, sizeof(unsigned)));
+    c.st = dalloc<DevState>(c, 1);
+    FCFD_HIP(hipMemset(c.st, 0, sizeof(DevState)));
+    FCFD_HIP(hipHostMalloc((void**)&c.h_pinned, 64, hipHostMallocDefault));
+    FCFD_HIP(hipDeviceSynchronize());
+}
+
+void destroy_ctx(Ctx& c) {
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    else (void)hipDeviceSynchronize();
+    for (void* p : c.allocs) (void)hipFree(p);
+    c.allocs.clear();
+    if (c.h_pinned) (void)hipHostFree(c.h_pinned);
+    c.h_pinned = nullptr;
+}
+
+double* api_ce_buffer(Ctx& c) {
+    if (!c.ce64_api) c.ce64_api = dalloc<double>(c, 16 * (size_t)c.g.ne);
+    return c.ce64_api;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fcfd_last_error(void) { return g_last_error.c_str(); }
+
+int fcfd_ctx_create(const fcfd_mesh_desc* mesh, const fcfd_solver_desc* solver, void* stream, fcfd_ctx** out) {
+    if (!out) {
+        g_last_error = "out is null";
+        return FCFD_ERR_ARG;
+    }
+    *out = nullptr;
+    fcfd_ctx* h = new fcfd_ctx();
+    int rc = guard([&] { build_ctx(h->c, mesh, solver, stream); });
+    if (rc != FCFD_OK) {
+        destroy_ctx(h->c);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return FCFD_OK;
+}
+
+int fcfd_ctx_destroy(fcfd_ctx* ctx) {
+    if (!ctx) return FCFD_OK;
+    destroy_ctx(ctx->c);
+    delete ctx;
+    return FCFD_OK;
+}
+
+int fcfd_set_stream(fcfd_ctx* ctx, void* stream) {
+    if (!ctx) return FCFD_ERR_ARG;
+    ctx->c.stream = (hipStream_t)stream;
+    return FCFD_OK;
+}
+
+int fcfd_ctx_info(const fcfd_ctx* ctx, int64_t* nn, int64_t* ne, int32_t* levels) {
+    if (!ctx) return FCFD_ERR_ARG;
+    if (nn) *nn = ctx->c.g.nn;
+    if (ne) *ne = ctx->c.g.ne;
+    if (levels) *levels = ctx->c.nlev;
+    return FCFD_OK;
+}
+
+int fcfd_level_nodes(const fcfd_ctx* ctx, int32_t level, int64_t* nn, int32_t* nnx, int32_t* nny) {
+    if (!ctx || level < 0 || level >= ctx->c.nlev) return FCFD_ERR_ARG;
+    const Grid2D& g = ctx->c.lv[level].g;
+    if (nn) *nn = g.nn;
+    if (nnx) *nnx = g.nnx;
+    if (nny) *nny = g.nny;
+    return FCFD_OK;
+}
+
+int fcfd_mesh_fill(fcfd_ctx* ctx, double* coords, int64_t* nconn) {
+    if (!ctx) return FCFD_ERR_ARG;
+    return guard([&] { mesh_fill(ctx->c, coords, nconn); });
+}
+
+int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16) {
+    if (!ctx || !out16) return FCFD_ERR_ARG;
+    const ElemConst& e = ctx->c.ec;
+    const double* src = which == 0 ? e.Me : which == 1 ? e.Le : which == 2 ? e.Gx : which == 3 ? e.Gy
+                      : which == 4 ? e.Leoff : nullptr;
+    if (!src) {
+        g_last_error = "unknown element constant";
+        return FCFD_ERR_ARG;
+    }
+    std::memcpy(out16, src, 16 * sizeof(double));
+    return FCFD_OK;
+}
+
+int fcfd_lumped_mass(fcfd_ctx* ctx, double* M) {
+    if (!ctx || !M) return FCFD_ERR_ARG;
+    return guard([&] {
+        FCFD_HIP(hipMemcpyAsync(M, ctx->c.Mlump, sizeof(double) * 2 * ctx->c.g.nn, hipMemcpyDeviceToDevice,
+                                ctx->c.stream));
+    });
+}
+
+int fcfd_laplacian_diag(fcfd_ctx* ctx, int32_t level, double* out) {
+    if (!ctx || !out || level < 0 || level >= ctx->c.nlev) return FCFD_ERR_ARG;
+    return guard([&] {
+        const Level& L = ctx->c.lv[level];
+        FCFD_HIP(hipMemcpyAsync(out, L.Ld, sizeof(double) * L.g.nn, hipMemcpyDeviceToDevice, ctx->c.stream));
+    });
+}
+
+int fcfd_conv_elem_calc(fcfd_ctx* ctx, const double* U, double beta, double* Ce) {
+    if (!ctx || !U || !Ce) return FCFD_ERR_ARG;
+    return guard([&] {
+        build_ce_api(ctx->c, U, beta, Ce);
+    });
+}
+
+int fcfd_boundary_integral(fcfd_ctx* ctx, const double* U_d, double* F_p) {
+    if (!ctx || !F_p) return FCFD_ERR_ARG;
+    return guard([&] { boundary_integral(ctx->c, U_d, F_p); });
+}
+
+int fcfd_elem_mult_const(fcfd_ctx* ctx, int32_t level, const double* A16, const double* x, double* y,
+                         int32_t accumulate) {
+    if (!ctx || !A16 || !x || !y) return FCFD_ERR_ARG;
+    return guard([&] { elem_mult_const(ctx->c, level, A16, x, y, accumulate); });
+}
+
+int fcfd_elem_mult_stored(fcfd_ctx* ctx, const double* K0_16, const double* A1, double c1, const double* A2,
+                          double c2, int32_t ncomp, const double* x, double* y) {
+    if (!ctx || !x || !y || ncomp < 1 || ncomp > 3) return FCFD_ERR_ARG;
+    return guard([&] { elem_mult_general(ctx->c, K0_16, A1, c1, A2, c2, ncomp, x, y); });
+}
+
+int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z) {
+    if (!ctx || !r || !z) return FCFD_ERR_ARG;
+    return guard([&] { vcycle(ctx->c, r, z); });
+}
+
+int fcfd_pressure_cg(fcfd_ctx* ctx, const double* b, double* x, double tol, int32_t maxiter, int32_t* iters,
+                     int32_t* converged) {
+    if (!ctx || !b || !x) return FCFD_ERR_ARG;
+    return guard([&] {
+        bool conv = false;
+        int it = pressure_cg(ctx->c, b, x, tol, maxiter > 0 ? maxiter : 1000, &conv);
+        FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
+        if (iters) *iters = it;
+        if (converged) *converged = conv;
+    });
+}
+
+int fcfd_solve_pressure(fcfd_ctx* ctx, const double* U_star, const double* P_n, const double* F_p, double rho,
+                        double dt, double* P, int32_t* iters) {
+    if (!ctx || !U_star || !P_n || !P) return FCFD_ERR_ARG;
+    return guard([&] {
+        int it = solve_pressure(ctx->c, U_star, P_n, F_p, rho, dt, P);
+        FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
+        if (iters) *iters = it;
+    });
+}
+
+int fcfd_solve_momentum(fcfd_ctx* ctx, const double* U_n, const double* P_n, const double* Ce, double rho, double mu,
+                        double dt, const double* U_d, const double* F, double* U_hat, int32_t* iters) {
+    if (!ctx || !U_n || !P_n || !U_hat) return FCFD_ERR_ARG;
+    return guard([&] {
+        Ctx& c = ctx->c;
+        scatter_ud(c, U_d);
+        int it;
+        if (Ce) {
+            double* em = api_ce_buffer(c);
+            ce_from_api(c, Ce, em);
+            it = solve_momentum(c, FCFD_CE_F64, em, nullptr, U_n, P_n, rho, mu, dt, F, U_hat);
+        } else {
+            build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
+            it = solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, U_hat);
+        }
+        FCFD_HIP(hipStreamSynchronize(c.stream));
+        if (iters) *iters = it;
+    });
+}
+
+int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, double dt, double rho, double mu,
+                            const double* U_d, const double* F_p, const double* F, double* U, double* P,
+                            int32_t* iters) {
+    if (!ctx || !U_n || !P_n || !U || !P) return FCFD_ERR_ARG;
+    return guard([&] {
+        Ctx& c = ctx->c;
+        scatter_ud(c, U_d);
+        build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
+        int uit = solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x);
+        projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
+        int pit = solve_pressure(c, c.ustar, P_n, F_p, rho, dt, P);
+        projection_final(c, c.ustar, P, rho, dt, U);
+        if (iters) {
+            iters[0] = uit;
+            iters[1] = pit;
+        }
+    });
+}
+
+int fcfd_dot(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out) {
+    if (!ctx || !x || !y || !out || n < 0) return FCFD_ERR_ARG;
+    return guard([&] { *out = dot(ctx->c, x, y, (int)n); });
+}
+
+int fcfd_axpby(fcfd_ctx* ctx, double a, const double* x, double b, double* y, int64_t n) {
+    if (!ctx || !x || !y || n < 0) return FCFD_ERR_ARG;
+    return guard([&] { axpby(ctx->c, a, x, b, y, (int)n); });
+}
+
+int fcfd_bench_momentum_apply(fcfd_ctx* ctx, int32_t reps, float* ms_per_apply) {
+    if (!ctx || reps <= 0 || !ms_per_apply) return FCFD_ERR_ARG;
+    return guard([&] { *ms_per_apply = bench_momentum_apply(ctx->c, reps); });
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+// Context, device solver state and host-side launch helpers shared by the
+// translation units of libfcfd.
+#pragma once
+#include <vector>
+#include <cstring>
+#include <type_traits>
+#include <string>
+#include <stdexcept>
+#include "fcfd_common.hpp"
+#include "../../include/fcfd.h"
+
+namespace fcfd {
+
+constexpr int MAX_LEVELS = 24;
+constexpr int MAX_RED = 4;  // reductions per kernel
+
+// Scalars that solver kernels read and the finalize steps of reducing kernels
+// write.  Lives in device memory; never read by the host inside a solve except
+// for the active flags and iteration counts.
+struct DevState {
+    // BiCGSTAB (jax_iterative_solvers.py:123-167)
+    double b_atol2, b_rho, b_rho_new, b_alpha, b_omega, b_beta;
+    int b_k, b_active, b_ee, b_maxit;
+    // PCG (jax_iterative_solvers.py:87-120)
+    double c_atol2, c_gamma, c_alpha, c_beta, c_rr, c_mean_r;
+    int c_k, c_active, c_maxit, c_flex;
+    // pressure rhs / solution means (pure Neumann form)
+    double mean_rhs, mean_p;
+    // last coarse-CG iteration count (diagnostic)
+    int coarse_k;
+    int pad_;
+    // generic reduction output
+    double dot;
+    // multigrid lazy means per level: see MG_* slots
+    double mg[MAX_LEVELS][6];
+};
+enum { MG_R = 0, MG_X = 1, MG_F = 2, MG_C = 3, MG_P = 4, MG_Y = 5 };
+
+struct Level {
+    Grid2D g;
+    ElemConst ec;
+    ElemConst* ec_dev = nullptr;
+    double* Ld = nullptr;       // L_diag
+    float* Ld32 = nullptr;
+    uint8_t* pin = nullptr;     // pinned form: 1 at gamma_p of this level
+    std::vector<int64_t> pins;
+    // V-cycle vectors, FP64 form
+    double *r = nullptr, *x = nullptr, *y = nullptr, *f = nullptr, *y2 = nullptr;
+    // FP32 form (mixed precision)
+    float *r32 = nullptr, *x32 = nullptr, *y32 = nullptr, *f32 = nullptr, *y32b = nullptr;
+};
+
+struct Ctx {
+    hipStream_t stream = nullptr;
+    Grid2D g;
+    double lx, ly;
+    ElemConst ec;
+    ElemConst* ec_dev = nullptr;
+    double tol;
+    int set_zeroP, precond, precision, ce_format, maxit_mom, maxit_p;
+    int nlev = 0;
+    std::vector<Level> lv;
+    // Dirichlet data
+    int n_gamma_d = 0;
+    int64_t* gamma_d_dev = nullptr;
+    uint8_t* dmask = nullptr;   // (2nn) 1 on Dirichlet DOFs
+    double* ud_full = nullptr;  // (2nn) dense U_d
+    double* Mlump = nullptr;    // (2nn) lumped mass
+    // pressure pins at the fine level
+    uint8_t* ppin = nullptr;    // (nn) or null (Neumann)
+    // boundary edges for F_p: node-owner gather lists
+    int n_bedge = 0;
+    int* fb_off = nullptr;      // (nn+1) CSR offsets node -> incident boundary edge slots
+    int* fb_edge = nullptr;     // edge id * 2 + local node
+    double* fb_geom = nullptr;  // per edge: nx, ny, length
+    int* fb_ui = nullptr;       // per edge slot: index into U_d of u / v at the 2 nodes
+    // element matrices (timestep storage)
+    void* ce = nullptr;
+    float* ce_scale = nullptr;
+    double* ce64_api = nullptr;  // FP64 entry-major buffer for API calls taking (ne,4,4)
+    // momentum work (2nn)
+    double *m_r = nullptr, *m_rhat = nullptr, *m_p[2] = {nullptr, nullptr}, *m_q[2] = {nullptr, nullptr},
+           *m_s = nullptr, *m_t = nullptr, *m_x = nullptr;
+    // pressure work (nn)
+    double *c_r = nullptr, *c_p[2] = {nullptr, nullptr}, *c_Ap = nullptr, *c_x = nullptr, *c_rhs = nullptr;
+    double* c_z[2] = {nullptr, nullptr};
+    double* ustar = nullptr;  // (2nn)
+    // coarse CG scratch (coarsest level)
+    double *cc_r = nullptr, *cc_p = nullptr, *cc_Ap = nullptr;
+    float *cc_r32 = nullptr, *cc_p32 = nullptr, *cc_Ap32 = nullptr;
+    // reductions
+    double* partials = nullptr;
+    unsigned* counter = nullptr;
+    DevState* st = nullptr;
+    int* h_pinned = nullptr;   // pinned host mirror for flags / counts
+    // iteration predictions for chunked launching
+    int pred_mom = 4, pred_p = 8;
+    std::vector<void*> allocs;
+};
+
+// error handling --------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+#define FCFD_HIP(call)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::fcfd::Error(FCFD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define FCFD_CHECK_LAUNCH() FCFD_HIP(hipGetLastError())
+#define FCFD_REQUIRE(cond, msg)                                   \
+    do {                                                          \
+        if (!(cond)) throw ::fcfd::Error(FCFD_ERR_ARG, msg);      \
+    } while (0)
+
+template <class T>
+T* dalloc(Ctx& c, size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    FCFD_HIP(hipMalloc(&p, n * sizeof(T)));
+    c.allocs.push_back(p);
+    return (T*)p;
+}
+
+// host element constants --------------------------------------------------------------------
+void quad4_constants(double hx, double hy, ElemConst& ec);
+
+// launch helpers (defined in fcfd_ops.hip) ------------------------------------------------
+void launch_fill(Ctx& c, double* x, double v, int n);
+void launch_fill32(Ctx& c, float* x, float v, int n);
+void build_ce(Ctx& c, const double* U, double beta, int fmt, void* store, float* scale);
+void ce_to_api(Ctx& c, const double* entry_major, double* api);
+void ce_from_api(Ctx& c, const double* api, double* entry_major);
+void elem_mult_const(Ctx& c, int level, const double* A16, const double* x, double* y, int acc);
+void build_ce_api(Ctx& c, const double* U, double beta, double* Ce);
+void elem_mult_general(Ctx& c, const double* K0_16, const double* A1, double c1, const double* A2, double c2,
+                       int ncomp, const double* x, double* y);
+void boundary_integral(Ctx& c, const double* U_d, double* F_p);
+void mesh_fill(Ctx& c, double* coords, int64_t* nconn);
+void scatter_ud(Ctx& c, const double* U_d);
+double dot(Ctx& c, const double* x, const double* y, int n);
+void axpby(Ctx& c, double a, const double* x, double b, double* y, int n);
+float bench_momentum_apply(Ctx& c, int reps);
+
+// momentum (fcfd_momentum.hip)
+int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n, const double* P_n, double rho,
+                   double mu, double dt, const double* F, double* U_hat);
+
+// pressure / multigrid (fcfd_mg.hip)
+void vcycle(Ctx& c, const double* r, double* z);
+int pressure_cg(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged);
+int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho,
+                   double dt, double* P_out);
+void projection_ustar(Ctx& c, const double* U_hat, const double* P_n, double rho, double dt, double* U_star);
+void projection_final(Ctx& c, const double* U_star, const double* P, double rho, double dt, double* U);
+
+// host sync helper: read device ints into the pinned mirror
+void read_state_ints(Ctx& c, const int* dev, int n, int* host);
+
+}  // namespace fcfd

@@ -1,0 +1,599 @@
+// Pressure solve: geometric multigrid V-cycle (jax_fem_cfd/core/multigrid.py:231-330) as a
+// preconditioner of CG (jax_fem_cfd/utils/jax_iterative_solvers.py:87-120), and
+// solve_pressure (jax_fem_cfd/solvers/standard.py:85-125).
+//
+// V-cycle kernels per level (pinned form / pure-Neumann form):
+//   down   [DownX (Neumann)], DownRes, Restrict
+//   coarse one single-workgroup kernel: CG to tol (pinned) or one Jacobi (Neumann)
+//   up     [Prolong (Neumann)], Jacobi (prolongation formed on the fly in the pinned form)
+// The pure-Neumann mean projections are applied lazily: every producing kernel reduces the sum
+// of what it writes (fused, deterministic) and consumers subtract the mean when they read.
+// Level vectors are templated on T: double (plain FP64 form) or float (mixed form).
+#include "fcfd_internal.hpp"
+#include "fcfd_ops.hpp"
+
+namespace fcfd {
+
+constexpr double OMEGA = 2.0 / 3.0;
+
+template <class T> struct LvBufs;
+template <> struct LvBufs<double> {
+    static double* r(Level& L) { return L.r; }
+    static double* x(Level& L) { return L.x; }
+    static double* y(Level& L) { return L.y; }
+    static double* f(Level& L) { return L.f; }
+    static const double* Ld(Level& L) { return L.Ld; }
+    static double* cr(Ctx& c) { return c.cc_r; }
+    static double* cp(Ctx& c) { return c.cc_p; }
+    static double* cAp(Ctx& c) { return c.cc_Ap; }
+};
+template <> struct LvBufs<float> {
+    static float* r(Level& L) { return L.r32; }
+    static float* x(Level& L) { return L.x32; }
+    static float* y(Level& L) { return L.y32; }
+    static float* f(Level& L) { return L.f32; }
+    static const float* Ld(Level& L) { return L.Ld32; }
+    static float* cr(Ctx& c) { return c.cc_r32; }
+    static float* cp(Ctx& c) { return c.cc_p32; }
+    static float* cAp(Ctx& c) { return c.cc_Ap32; }
+};
+
+// ------------------------------------------------------------------------------ down leg
+// xh = 2/3 (r - mean r) / L_diag   (multigrid.py:257-261), sum -> mean(xh)
+template <class T, class TR>
+struct DownX {
+    const TR* r; const T* Ld; T* x; const DevState* st; int l;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const T mr = (T)st->mg[l][MG_R];
+        const T v = (T(2.0 / 3.0) * ((T)r[n] - mr)) / Ld[n];
+        x[n] = v;
+        acc[0] += (double)v;
+    }
+};
+struct MeanFin {
+    double* slot; double* slot2; int n;
+    __device__ void operator()(const double (&t)[1]) const {
+        double m = t[0] / n;
+        *slot = m;
+        if (slot2) *slot2 = m;
+    }
+};
+
+// rH = rh - L xh   (Neumann: multigrid.py:263-264; pinned: :246-251)
+template <class T, class TR>
+struct DownRes {
+    Grid2D g; const double* Le; const TR* r; const T* Ld; const uint8_t* pin; T* x; T* f;
+    const DevState* st; int l; int neumann;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        NodeNbr s;
+        node_nbr(g, n, s);
+        T X[3][3];
+        T fv;
+        if (neumann) {
+            const T mx = (T)st->mg[l][MG_X], mr = (T)st->mg[l][MG_R];
+            load3x3(s, [&](int k) { return x[k] - mx; }, X);
+            fv = ((T)r[n] - mr) - ebe_const(s, Le, X);
+        } else {
+            load3x3(s, [&](int k) { return (pin && pin[k]) ? T(0) : (T(2.0 / 3.0) * (T)r[k]) / Ld[k]; }, X);
+            x[n] = X[1][1];
+            const bool pn = pin && pin[n];
+            const T rh = pn ? T(0) : (T)r[n];
+            const T lx = pn ? T(0) : ebe_const(s, Le, X);
+            fv = rh - lx;
+        }
+        f[n] = fv;
+        acc[0] += (double)fv;
+    }
+};
+
+// coarse[I] = R (f - mean f): R = I^T without the 1/4 factor (multigrid.py:142-212);
+// missing neighbours (domain edges) contribute zero.  Periodic grids restrict the padded fine
+// grid with the same formulas and drop the wrap row/column, so only negative fine indices can
+// be missing there.
+template <class T>
+struct Restrict {
+    Grid2D gf, gc; const T* f; T* rc; const DevState* st; int l; int neumann;
+    __device__ void operator()(int nc, double (&acc)[1]) const {
+        const int I = nc / gc.nnx, J = nc - I * gc.nnx;
+        const int fi = 2 * I, fj = 2 * J;
+        const T mf = neumann ? (T)st->mg[l][MG_F] : T(0);
+        auto val = [&](int dy, int dx) -> T {
+            int a = fi + dy, b = fj + dx;
+            bool ok = a >= 0 && b >= 0 && (gf.periodic || (a < gf.nny && b < gf.nnx));
+            return ok ? f[a * gf.nnx + b] - mf : T(0);
+        };
+        const T E = ((val(1, 0) + val(-1, 0)) + val(0, 1)) + val(0, -1);
+        const T D = ((val(1, 1) + val(1, -1)) + val(-1, 1)) + val(-1, -1);
+        const T v = (val(0, 0) + T(0.5) * E) + T(0.25) * D;
+        rc[nc] = v;
+        acc[0] += (double)v;
+    }
+};
+
+// ------------------------------------------------------------------------------ up leg
+// bilinear prolongation of the coarse vector (minus its mean) at fine node n
+// (multigrid.py:90-140; periodic coarse grids wrap)
+template <class T>
+__device__ __forceinline__ T interp_at(const Grid2D& gf, const Grid2D& gc, const T* yc, T my, int n) {
+    const int j = n / gf.nnx, i = n - j * gf.nnx;
+    auto H = [&](int I, int J) -> T {
+        if (gc.periodic) {
+            if (I >= gc.nny) I -= gc.nny;
+            if (J >= gc.nnx) J -= gc.nnx;
+        }
+        return yc[I * gc.nnx + J] - my;
+    };
+    const bool io = i & 1, jo = j & 1;
+    if (!io && !jo) return H(j >> 1, i >> 1);
+    if (io && !jo) return T(0.5) * H(j >> 1, i >> 1) + T(0.5) * H(j >> 1, (i >> 1) + 1);
+    if (!io && jo) return T(0.5) * H(j >> 1, i >> 1) + T(0.5) * H((j >> 1) + 1, i >> 1);
+    const int jb = j >> 1, il = i >> 1;
+    return ((T(0.25) * H(jb, il) + T(0.25) * H(jb, il + 1)) + T(0.25) * H(jb + 1, il)) + T(0.25) * H(jb + 1, il + 1);
+}
+
+template <class T>
+struct Prolong {
+    Grid2D gf, gc; const T* yc; T* p; const DevState* st; int l;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const T v = interp_at(gf, gc, yc, (T)st->mg[l + 1][MG_Y], n);
+        p[n] = v;
+        acc[0] += (double)v;
+    }
+};
+
+// xh <- xh + I xH; one damped Jacobi (multigrid.py:214-228, :306-326)
+template <class T, class TR>
+struct Jacobi {
+    Grid2D gf, gc; const double* Leoff; const T* Ld; const uint8_t* pin; const TR* r; const T* x; const T* p;
+    const T* yc; T* y; const DevState* st; int l; int neumann;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        NodeNbr s;
+        node_nbr(gf, n, s);
+        T X[3][3];
+        T rh;
+        if (neumann) {
+            const T mx = (T)st->mg[l][MG_X], mp = (T)st->mg[l][MG_P];
+            load3x3(s, [&](int k) { return (x[k] - mx) + (p[k] - mp); }, X);
+            const T oup = l == 0 ? T(0) : (T)st->mg[l][MG_R];
+            rh = (T)r[n] - oup;
+        } else {
+            load3x3(s, [&](int k) { return (pin && pin[k]) ? T(0) : x[k] + interp_at(gf, gc, yc, T(0), k); }, X);
+            rh = (T)r[n];
+        }
+        const T off = ebe_const(s, Leoff, X);
+        T v = (T(OMEGA) * (rh - off)) / Ld[n] + T(1.0 - OMEGA) * X[1][1];
+        if (!neumann && pin && pin[n]) v = T(0);
+        y[n] = v;
+        acc[0] += (double)v;
+    }
+};
+
+// ------------------------------------------------------------------------------ coarse level
+// One workgroup.  Pinned: CG (M = identity) to tol with maxiter 1000 on L with pinned rows
+// zeroed (multigrid.py:278-287).  Neumann: one Jacobi from zero (:289-301).
+constexpr int COARSE_BLOCK = 1024;
+
+template <class T, class TR>
+__global__ __launch_bounds__(COARSE_BLOCK) void k_coarse(Grid2D g, const ElemConst* __restrict__ ec, const T* Ld,
+                                                         const uint8_t* pin, const TR* r, T* x, T* cr, T* cp,
+                                                         T* cAp, DevState* st, int l, int neumann, double tol,
+                                                         int maxit, const int* active) {
+    if (active && *active == 0) return;
+    const int nn = g.nn;
+    if (neumann) {
+        const T mr = (T)st->mg[l][MG_R];
+        double acc[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+            // single_jacobi_iter(0, rN): omega (rN - Le_off 0) / L_diag + (1 - omega) 0
+            const T v = (T(OMEGA) * (((T)r[n] - mr) - T(0))) / Ld[n] + T(1.0 - OMEGA) * T(0);
+            x[n] = v;
+            acc[0] += (double)v;
+        }
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) st->mg[l][MG_Y] = acc[0] / nn;
+        return;
+    }
+    double g2[1] = {0.0};
+    for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+        const T b = (pin && pin[n]) ? T(0) : (T)r[n];
+        cr[n] = b; cp[n] = b; x[n] = T(0);
+        g2[0] += (double)b * (double)b;
+    }
+    block_sum<1>(g2);  // includes a barrier: cr / cp visible to the block
+    double gamma = g2[0];
+    const double atol2 = fmax(tol * tol * gamma, 0.0);
+    int k = 0;
+    while (gamma > atol2 && k < maxit) {
+        double pap[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+            NodeNbr s;
+            node_nbr(g, n, s);
+            T X[3][3];
+            load3x3(s, [&](int m) { return cp[m]; }, X);
+            const T a = (pin && pin[n]) ? T(0) : ebe_const(s, ec->Le, X);
+            cAp[n] = a;
+            pap[0] += (double)X[1][1] * (double)a;
+        }
+        block_sum<1>(pap);
+        const double alpha = gamma / pap[0];
+        double rr[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+            x[n] = x[n] + (T)alpha * cp[n];
+            const T rn = cr[n] - (T)alpha * cAp[n];
+            cr[n] = rn;
+            rr[0] += (double)rn * (double)rn;
+        }
+        block_sum<1>(rr);
+        const double beta = rr[0] / gamma;
+        for (int n = threadIdx.x; n < nn; n += blockDim.x) cp[n] = cr[n] + (T)beta * cp[n];
+        __syncthreads();
+        gamma = rr[0];
+        ++k;
+    }
+    for (int n = threadIdx.x; n < nn; n += blockDim.x)
+        if (pin && pin[n]) x[n] = T(0);
+    if (threadIdx.x == 0) st->coarse_k = k;
+}
+
+// ------------------------------------------------------------------------------ V-cycle driver
+template <class T>
+static void vcycle_t(Ctx& c, const double* r0, const int* active) {
+    using B = LvBufs<T>;
+    const int N = c.nlev;
+    const int neu = !c.set_zeroP;
+    for (int l = 0; l < N - 1; ++l) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        const int nb = grid_for(L.g.nn);
+        if (l == 0) {
+            if (neu)
+                node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
+                    L.g.nn, active, DownX<T, double>{r0, B::Ld(L), B::x(L), c.st, l},
+                    MeanFin{&c.st->mg[l][MG_X], nullptr, L.g.nn}, c.partials, c.counter);
+            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
+                L.g.nn, active,
+                DownRes<T, double>{L.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::f(L), c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_F], nullptr, L.g.nn}, c.partials, c.counter);
+        } else {
+            if (neu)
+                node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
+                    L.g.nn, active, DownX<T, T>{B::r(L), B::Ld(L), B::x(L), c.st, l},
+                    MeanFin{&c.st->mg[l][MG_X], nullptr, L.g.nn}, c.partials, c.counter);
+            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
+                L.g.nn, active,
+                DownRes<T, T>{L.g, L.ec_dev->Le, B::r(L), B::Ld(L), L.pin, B::x(L), B::f(L), c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_F], nullptr, L.g.nn}, c.partials, c.counter);
+        }
+        node_kernel<1><<<grid_for(Lc.g.nn), BLOCK, 0, c.stream>>>(
+            Lc.g.nn, active, Restrict<T>{L.g, Lc.g, B::f(L), B::r(Lc), c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g.nn}, c.partials, c.counter);
+    }
+    {
+        Level& C = c.lv[N - 1];
+        const double ctol = std::is_same<T, float>::value ? (c.tol > 1e-5 ? c.tol : 1e-5) : c.tol;
+        if (N == 1)
+            k_coarse<T, double><<<1, COARSE_BLOCK, 0, c.stream>>>(C.g, C.ec_dev, B::Ld(C), C.pin, r0, B::y(C),
+                                                                 B::cr(c), B::cp(c), B::cAp(c), c.st, N - 1, neu,
+                                                                 ctol, 1000, active);
+        else
+            k_coarse<T, T><<<1, COARSE_BLOCK, 0, c.stream>>>(C.g, C.ec_dev, B::Ld(C), C.pin, B::r(C), B::y(C),
+                                                            B::cr(c), B::cp(c), B::cAp(c), c.st, N - 1, neu, ctol,
+                                                            1000, active);
+        FCFD_CHECK_LAUNCH();
+    }
+    for (int l = N - 2; l >= 0; --l) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        const int nb = grid_for(L.g.nn);
+        if (neu)
+            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(L.g.nn, active,
+                                                       Prolong<T>{L.g, Lc.g, B::y(Lc), B::f(L), c.st, l},
+                                                       MeanFin{&c.st->mg[l][MG_P], nullptr, L.g.nn}, c.partials,
+                                                       c.counter);
+        if (l == 0)
+            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
+                L.g.nn, active,
+                Jacobi<T, double>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, r0, B::x(L), B::f(L), B::y(Lc),
+                                  B::y(L), c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_Y], nullptr, L.g.nn}, c.partials, c.counter);
+        else
+            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
+                L.g.nn, active,
+                Jacobi<T, T>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, B::r(L), B::x(L), B::f(L), B::y(Lc),
+                             B::y(L), c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_Y], nullptr, L.g.nn}, c.partials, c.counter);
+    }
+    FCFD_CHECK_LAUNCH();
+}
+
+// z = y0 - mean (Neumann) as FP64
+template <class T>
+struct ZOut {
+    const T* y; double* z; const DevState* st; int neumann;
+    __device__ void operator()(int k, double (&)[1]) const {
+        z[k] = (double)y[k] - (neumann ? st->mg[0][MG_Y] : 0.0);
+    }
+};
+
+void vcycle(Ctx& c, const double* r, double* z) {
+    FCFD_REQUIRE(c.precond == FCFD_PC_MULTIGRID, "context was not created with the multigrid preconditioner");
+    const int nn = c.g.nn;
+    // mean of r for the first down-leg projection (pure Neumann)
+    struct SumBody {
+        const double* r;
+        __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += r[k]; }
+    };
+    node_kernel<1><<<grid_for(nn), BLOCK, 0, c.stream>>>(nn, nullptr, SumBody{r},
+                                                         MeanFin{&c.st->mg[0][MG_R], nullptr, nn}, c.partials,
+                                                         c.counter);
+    if (c.precision == FCFD_PREC_MIXED) {
+        vcycle_t<float>(c, r, nullptr);
+        node_kernel<0><<<grid_for(nn), BLOCK, 0, c.stream>>>(nn, nullptr,
+                                                             ZOut<float>{c.lv[0].y32, z, c.st, !c.set_zeroP},
+                                                             NoFin{}, c.partials, c.counter);
+    } else {
+        vcycle_t<double>(c, r, nullptr);
+        node_kernel<0><<<grid_for(nn), BLOCK, 0, c.stream>>>(nn, nullptr,
+                                                             ZOut<double>{c.lv[0].y, z, c.st, !c.set_zeroP},
+                                                             NoFin{}, c.partials, c.counter);
+    }
+    FCFD_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------------------ PCG
+struct CgInit {
+    Grid2D g; const double* Le; const uint8_t* pin; const double* b; const double* bmean; const double* x0in;
+    int zero_pins; double* x; double* r;
+    __device__ void operator()(int n, double (&acc)[3]) const {
+        NodeNbr s;
+        node_nbr(g, n, s);
+        double X[3][3];
+        load3x3(s, [&](int k) { return (zero_pins && pin && pin[k]) ? 0.0 : x0in[k]; }, X);
+        const bool pn = pin && pin[n];
+        const double ax = pn ? 0.0 : ebe_const(s, Le, X);
+        const double bn = b[n] - (bmean ? *bmean : 0.0);
+        const double rn = bn - ax;
+        x[n] = X[1][1];
+        r[n] = rn;
+        acc[0] += bn * bn;
+        acc[1] += rn * rn;
+        acc[2] += rn;
+    }
+};
+struct CgInitFin {
+    DevState* st; double tol; int maxit; int nn;
+    __device__ void operator()(const double (&t)[3]) const {
+        st->c_atol2 = fmax(tol * tol * t[0], 0.0);
+        st->c_rr = t[1];
+        st->mg[0][MG_R] = t[2] / nn;
+        st->c_k = 0;
+        st->c_maxit = maxit;
+        st->c_active = (t[1] > st->c_atol2) && (0 < maxit);
+    }
+};
+
+// z = y0 - mean ; gamma0 = r . z ; p_old = 0 ; beta = 0
+template <class T>
+struct CgGamma0 {
+    const double* r; const T* y; double* z; double* pold; const DevState* st; int neumann;
+    __device__ void operator()(int k, double (&acc)[1]) const {
+        const double zk = (double)y[k] - (neumann ? st->mg[0][MG_Y] : 0.0);
+        z[k] = zk;
+        pold[k] = 0.0;
+        acc[0] += r[k] * zk;
+    }
+};
+struct CgGamma0Fin {
+    DevState* st;
+    __device__ void operator()(const double (&t)[1]) const {
+        st->c_gamma = t[0];
+        st->c_beta = 0.0;
+    }
+};
+
+// p = z + beta p_old (on the fly), Ap = L p (pinned rows 0), finalize alpha = gamma / p.Ap
+struct CgK1 {
+    Grid2D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const double beta = st->c_beta;
+        NodeNbr s;
+        node_nbr(g, n, s);
+        double X[3][3];
+        load3x3(s, [&](int k) { return z[k] + beta * pold[k]; }, X);
+        const double a = (pin && pin[n]) ? 0.0 : ebe_const(s, Le, X);
+        pnew[n] = X[1][1];
+        Ap[n] = a;
+        acc[0] += X[1][1] * a;
+    }
+};
+struct CgK1Fin {
+    DevState* st;
+    __device__ void operator()(const double (&t)[1]) const { st->c_alpha = st->c_gamma / t[0]; }
+};
+
+// x += alpha p ; r -= alpha Ap ; finalize r.r and mean r
+struct CgK2 {
+    const double* p; const double* Ap; double* x; double* r; const DevState* st;
+    __device__ void operator()(int k, double (&acc)[2]) const {
+        const double a = st->c_alpha;
+        x[k] = x[k] + a * p[k];
+        const double rk = r[k] - a * Ap[k];
+        r[k] = rk;
+        acc[0] += rk * rk;
+        acc[1] += rk;
+    }
+};
+struct CgK2Fin {
+    DevState* st; int nn;
+    __device__ void operator()(const double (&t)[2]) const {
+        st->c_rr = t[0];
+        st->mg[0][MG_R] = t[1] / nn;
+    }
+};
+
+// z_new = y0 - mean ; gamma' = r.z_new ; (flexible: r.z_old) ; beta ; k++ ; convergence flag
+template <class T>
+struct CgK3 {
+    const double* r; const T* y; double* znew; const double* zold; const DevState* st; int neumann; int flex;
+    __device__ void operator()(int k, double (&acc)[2]) const {
+        const double zk = (double)y[k] - (neumann ? st->mg[0][MG_Y] : 0.0);
+        znew[k] = zk;
+        acc[0] += r[k] * zk;
+        if (flex) acc[1] += r[k] * zold[k];
+    }
+};
+struct CgK3Fin {
+    DevState* st; int flex;
+    __device__ void operator()(const double (&t)[2]) const {
+        const double gnew = t[0];
+        st->c_beta = flex ? (gnew - t[1]) / st->c_gamma : gnew / st->c_gamma;
+        st->c_gamma = gnew;
+        const int k = st->c_k + 1;
+        st->c_k = k;
+        st->c_active = (st->c_rr > st->c_atol2) && (k < st->c_maxit);
+    }
+};
+
+struct JacobiPrec {
+    const double* r; const double* Ld; double* y;
+    __device__ void operator()(int k, double (&)[1]) const { y[k] = r[k] / Ld[k]; }
+};
+
+// applies the preconditioner to c.c_r; returns pointer + type of y0
+static const void* precondition(Ctx& c, const int* active, bool& is_f32) {
+    const int nn = c.g.nn;
+    is_f32 = false;
+    switch (c.precond) {
+        case FCFD_PC_NONE:
+            return c.c_r;
+        case FCFD_PC_JACOBI:
+            node_kernel<0><<<grid_for(nn), BLOCK, 0, c.stream>>>(nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y},
+                                                                 NoFin{}, c.partials, c.counter);
+            return c.lv[0].y;
+        case FCFD_PC_MULTIGRID:
+            if (c.precision == FCFD_PREC_MIXED) {
+                vcycle_t<float>(c, c.c_r, active);
+                is_f32 = true;
+                return c.lv[0].y32;
+            }
+            vcycle_t<double>(c, c.c_r, active);
+            return c.lv[0].y;
+    }
+    throw Error(FCFD_ERR_UNSUPPORTED, "pressure preconditioner not supported");
+}
+
+// PCG: returns iterations; x0 in x0in (pins zeroed if zero_pins), solution in c.c_x
+static int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
+               bool* converged) {
+    const Grid2D& g = c.g;
+    const int nn = g.nn, nb = grid_for(nn);
+    const int neu = (c.precond == FCFD_PC_MULTIGRID) && !c.set_zeroP;
+    const int flex = c.precision == FCFD_PREC_MIXED && c.precond == FCFD_PC_MULTIGRID;
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    const double* Le = c.lv[0].ec_dev->Le;
+    node_kernel<3><<<nb, BLOCK, 0, c.stream>>>(nn, nullptr, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
+                                               CgInitFin{c.st, tol, maxit, nn}, c.partials, c.counter);
+    const int* act = &c.st->c_active;
+    bool f32;
+    const void* y0 = precondition(c, act, f32);
+    if (f32)
+        node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(nn, act,
+                                                   CgGamma0<float>{c.c_r, (const float*)y0, c.c_z[0], c.c_p[0], c.st, neu},
+                                                   CgGamma0Fin{c.st}, c.partials, c.counter);
+    else
+        node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
+            nn, act, CgGamma0<double>{c.c_r, (const double*)y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st},
+            c.partials, c.counter);
+    FCFD_CHECK_LAUNCH();
+    int launched = 0;
+    int chunk = c.pred_p > 0 ? c.pred_p : 1;
+    int h[2] = {0, 0};
+    // an already converged start needs no iteration
+    while (true) {
+        for (int t = 0; t < chunk && launched < maxit; ++t, ++launched) {
+            const int o = launched & 1, nw = o ^ 1;
+            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(nn, act,
+                                                       CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                                                       CgK1Fin{c.st}, c.partials, c.counter);
+            node_kernel<2><<<nb, BLOCK, 0, c.stream>>>(nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st},
+                                                       CgK2Fin{c.st, nn}, c.partials, c.counter);
+            bool f;
+            const void* y = precondition(c, act, f);
+            if (f)
+                node_kernel<2><<<nb, BLOCK, 0, c.stream>>>(
+                    nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                    CgK3Fin{c.st, flex}, c.partials, c.counter);
+            else
+                node_kernel<2><<<nb, BLOCK, 0, c.stream>>>(
+                    nn, act, CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                    CgK3Fin{c.st, flex}, c.partials, c.counter);
+        }
+        FCFD_CHECK_LAUNCH();
+        read_state_ints(c, &c.st->c_k, 2, h);  // c_k, c_active
+        if (!h[1] || launched >= maxit) break;
+        chunk = 2;
+    }
+    if (converged) *converged = !h[1] && h[0] <= maxit;
+    if (h[0] > 0) c.pred_p = h[0];
+    return h[0];
+}
+
+int pressure_cg(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged) {
+    int it = pcg(c, b, nullptr, x, 0, tol, maxit, converged);
+    FCFD_HIP(hipMemcpyAsync(x, c.c_x, sizeof(double) * c.g.nn, hipMemcpyDeviceToDevice, c.stream));
+    return it;
+}
+
+// ------------------------------------------------------------------------------ solve_pressure
+// rhs = rho/dt (D U* - F_p), pinned rows zeroed; sum for the mean projection
+struct PrhsBody {
+    Grid2D g; const ElemConst* ec; const double* Us; const double* Fp; const uint8_t* pin; double* rhs; double c;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        NodeNbr s;
+        node_nbr(g, n, s);
+        double Xu[3][3], Xv[3][3];
+        load3x3(s, [&](int k) { return Us[k]; }, Xu);
+        load3x3(s, [&](int k) { return Us[k + g.nn]; }, Xv);
+        const double div = ebe_const_T(s, ec->Gx, Xu) + ebe_const_T(s, ec->Gy, Xv);
+        double v = c * (div - (Fp ? Fp[n] : 0.0));
+        if (pin && pin[n]) v = 0.0;
+        rhs[n] = v;
+        acc[0] += v;
+    }
+};
+
+// P = x - mean(x) (Neumann) or x with pins zeroed
+struct PoutBody {
+    const double* x; const uint8_t* pin; double* P; const DevState* st; int neumann;
+    __device__ void operator()(int k, double (&)[1]) const {
+        double v = neumann ? x[k] - st->mean_p : x[k];
+        if (pin && pin[k]) v = 0.0;
+        P[k] = v;
+    }
+};
+struct SumXBody {
+    const double* x;
+    __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += x[k]; }
+};
+
+int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                   double* P_out) {
+    const int nn = c.g.nn, nb = grid_for(nn);
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(nn, nullptr,
+                                               PrhsBody{c.g, c.ec_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
+                                               MeanFin{&c.st->mean_rhs, nullptr, nn}, c.partials, c.counter);
+    const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
+    bool conv;
+    int it = pcg(c, c.c_rhs, c.set_zeroP ? nullptr : &c.st->mean_rhs, P_n, c.set_zeroP, c.tol, maxit, &conv);
+    if (!c.set_zeroP)
+        node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(nn, nullptr, SumXBody{c.c_x},
+                                                   MeanFin{&c.st->mean_p, nullptr, nn}, c.partials, c.counter);
+    node_kernel<0><<<nb, BLOCK, 0, c.stream>>>(nn, nullptr, PoutBody{c.c_x, pin, P_out, c.st, !c.set_zeroP}, NoFin{},
+                                               c.partials, c.counter);
+    FCFD_CHECK_LAUNCH();
+    return it;
+}
+
+}  // namespace fcfd

@@ -1,0 +1,295 @@
+"""Owning wrapper of one native context (fcfd_ctx) plus the grid registry the
+element-operator functions use to find the structured grid behind an
+``nconn`` array."""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _native as nat
+
+
+class Context:
+    """One structured grid + solver configuration on the GPU.
+
+    Mirrors what setup_solver (jax_fem_cfd/solvers/standard.py:191-247) closes
+    over: element constants, lumped mass, Dirichlet / pin index sets, the
+    multigrid hierarchy and every solver workspace (allocated once here, never
+    in the hot calls)."""
+
+    def __init__(self, nex, ney, lx, ly, periodic, *, tol=1e-8, set_zeroP=False, precond=None, mg_levels=1,
+                 precision="fp64", ce_format="f64", gamma_d=None, gamma_p=None, mg_gamma_p=None, bconn=None,
+                 nvec=None, maxiter_momentum=0, maxiter_pressure=0):
+        lib = nat.load_library()
+        t = nat.require_gpu()
+        self.nex, self.ney, self.lx, self.ly, self.periodic = int(nex), int(ney), float(lx), float(ly), bool(periodic)
+        self.nnx = self.nex if periodic else self.nex + 1
+        self.nny = self.ney if periodic else self.ney + 1
+        self.nn = self.nnx * self.nny
+        self.ne = self.nex * self.ney
+        self.tol, self.set_zeroP, self.precision, self.ce_format = tol, bool(set_zeroP), precision, ce_format
+        self.precond = precond
+        md = nat.MeshDesc(2, self.nex, self.ney, self.lx, self.ly, int(self.periodic))
+        keep = []
+
+        def arr(a):
+            p, k = nat.i64_array(a)
+            keep.append(k)
+            return p
+
+        sd = nat.SolverDesc()
+        sd.tol = float(tol)
+        sd.set_zeroP = int(bool(set_zeroP))
+        sd.precond = nat.PC[precond]
+        sd.mg_levels = int(mg_levels)
+        sd.precision = nat.PREC[precision]
+        sd.ce_format = nat.CE_FMT[ce_format]
+        sd.maxiter_momentum = int(maxiter_momentum)
+        sd.maxiter_pressure = int(maxiter_pressure)
+        if gamma_d is not None and len(gamma_d):
+            sd.gamma_d, sd.n_gamma_d = arr(gamma_d), len(gamma_d)
+        if gamma_p is not None and np.size(gamma_p):
+            g = np.atleast_1d(np.asarray(gamma_p, dtype=np.int64))
+            sd.gamma_p, sd.n_gamma_p = arr(g), len(g)
+        if mg_gamma_p is not None:
+            flat = np.concatenate([np.atleast_1d(np.asarray(g, dtype=np.int64)) for g in mg_gamma_p])
+            off = np.concatenate([[0], np.cumsum([np.size(g) for g in mg_gamma_p])]).astype(np.int64)
+            sd.mg_gamma_p, sd.mg_gamma_p_off = arr(flat), arr(off)
+        if bconn is not None and len(bconn):
+            sd.bconn, sd.nvec, sd.n_bedge = arr(bconn), arr(nvec), len(bconn)
+        h = ctypes.c_void_p()
+        nat.check(lib.fcfd_ctx_create(ctypes.byref(md), ctypes.byref(sd), nat.stream_handle(), ctypes.byref(h)),
+                  "fcfd_ctx_create")
+        self._h = h
+        self._lib = lib
+        self._t = t
+        self.n_gamma_d = 0 if gamma_d is None else len(gamma_d)
+        lv = ctypes.c_int32()
+        nat.check(lib.fcfd_ctx_info(h, None, None, ctypes.byref(lv)), "fcfd_ctx_info")
+        self.levels = lv.value
+
+    # ---------------------------------------------------------------------------------- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.fcfd_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _sync_stream(self):
+        nat.check(self._lib.fcfd_set_stream(self._h, nat.stream_handle()), "fcfd_set_stream")
+
+    def _empty(self, n, dtype=None):
+        return self._t.empty(n, dtype=dtype or self._t.float64, device=nat.device())
+
+    # ---------------------------------------------------------------------------------- mesh / constants
+    def mesh(self):
+        coords = self._empty(2 * self.nn)
+        nconn = self._empty(4 * self.ne, self._t.int64)
+        self._sync_stream()
+        nat.check(self._lib.fcfd_mesh_fill(self._h, nat.ptr(coords), nat.ptr(nconn)), "fcfd_mesh_fill")
+        return coords.view(self.nn, 2), nconn.view(self.ne, 4)
+
+    def element_constant(self, which):
+        out = (ctypes.c_double * 16)()
+        nat.check(self._lib.fcfd_element_constant(self._h, {"Me": 0, "Le": 1, "Gx": 2, "Gy": 3, "Le_off": 4}[which],
+                                                  out), "fcfd_element_constant")
+        return np.array(out[:]).reshape(4, 4)
+
+    def lumped_mass(self):
+        M = self._empty(2 * self.nn)
+        self._sync_stream()
+        nat.check(self._lib.fcfd_lumped_mass(self._h, nat.ptr(M)), "fcfd_lumped_mass")
+        return M
+
+    def laplacian_diag(self, level=0):
+        n = ctypes.c_int64()
+        nat.check(self._lib.fcfd_level_nodes(self._h, level, ctypes.byref(n), None, None), "fcfd_level_nodes")
+        out = self._empty(n.value)
+        self._sync_stream()
+        nat.check(self._lib.fcfd_laplacian_diag(self._h, level, nat.ptr(out)), "fcfd_laplacian_diag")
+        return out
+
+    def level_shape(self, level):
+        n, nx, ny = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        nat.check(self._lib.fcfd_level_nodes(self._h, level, ctypes.byref(n), ctypes.byref(nx), ctypes.byref(ny)),
+                  "fcfd_level_nodes")
+        return n.value, nx.value, ny.value
+
+    # ---------------------------------------------------------------------------------- element ops
+    def conv_elem_calc(self, U, beta):
+        U = nat.as_dev(U)
+        Ce = self._empty(16 * self.ne)
+        self._sync_stream()
+        nat.check(self._lib.fcfd_conv_elem_calc(self._h, nat.ptr(U), float(beta), nat.ptr(Ce)), "fcfd_conv_elem_calc")
+        return Ce.view(self.ne, 4, 4)
+
+    def boundary_integral(self, U_d):
+        Fp = self._empty(self.nn)
+        self._sync_stream()
+        Ud = None if U_d is None else nat.as_dev(U_d)
+        nat.check(self._lib.fcfd_boundary_integral(self._h, nat.ptr(Ud), nat.ptr(Fp)), "fcfd_boundary_integral")
+        return Fp
+
+    def elem_mult_const(self, A, x, level=0, out=None, accumulate=False):
+        x = nat.as_dev(x)
+        if out is None:
+            out = self._empty(x.numel()) if not accumulate else None
+        self._sync_stream()
+        nat.check(self._lib.fcfd_elem_mult_const(self._h, int(level), nat.c16(A), nat.ptr(x), nat.ptr(out),
+                                                 int(bool(accumulate))), "fcfd_elem_mult_const")
+        return out
+
+    def elem_mult_stored(self, x, K0=None, A1=None, c1=0.0, A2=None, c2=0.0, ncomp=1):
+        x = nat.as_dev(x)
+        A1 = None if A1 is None else nat.as_dev(A1)
+        A2 = None if A2 is None else nat.as_dev(A2)
+        y = self._empty(self.nn * ncomp)
+        self._sync_stream()
+        k0 = None if K0 is None else nat.c16(K0)
+        nat.check(self._lib.fcfd_elem_mult_stored(self._h, k0, nat.ptr(A1), float(c1), nat.ptr(A2), float(c2),
+                                                  int(ncomp), nat.ptr(x), nat.ptr(y)), "fcfd_elem_mult_stored")
+        return y
+
+    # ---------------------------------------------------------------------------------- solvers
+    def vcycle(self, r):
+        r = nat.as_dev(r)
+        z = self._empty(self.nn)
+        self._sync_stream()
+        nat.check(self._lib.fcfd_vcycle(self._h, nat.ptr(r), nat.ptr(z)), "fcfd_vcycle")
+        return z
+
+    def pressure_cg(self, b, x0=None, tol=None, maxiter=1000):
+        b = nat.as_dev(b)
+        x = self._t.zeros(self.nn, dtype=self._t.float64, device=nat.device()) if x0 is None else nat.as_dev(x0).clone()
+        it, conv = ctypes.c_int32(), ctypes.c_int32()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_pressure_cg(self._h, nat.ptr(b), nat.ptr(x), float(self.tol if tol is None else tol),
+                                             int(maxiter), ctypes.byref(it), ctypes.byref(conv)), "fcfd_pressure_cg")
+        return x, it.value, bool(conv.value)
+
+    def solve_pressure(self, U_star, P_n, F_p, rho, dt):
+        P = self._empty(self.nn)
+        it = ctypes.c_int32()
+        U_star, P_n = nat.as_dev(U_star), nat.as_dev(P_n)
+        Fp = _opt_vec(F_p, self.nn)
+        self._sync_stream()
+        nat.check(self._lib.fcfd_solve_pressure(self._h, nat.ptr(U_star), nat.ptr(P_n), nat.ptr(Fp), float(rho),
+                                                float(dt), nat.ptr(P), ctypes.byref(it)), "fcfd_solve_pressure")
+        return P, it.value
+
+    def solve_momentum(self, U_n, P_n, rho, mu, dt, U_d=None, F=None, Ce=None):
+        U_n, P_n = nat.as_dev(U_n), nat.as_dev(P_n)
+        Ud = None if U_d is None else nat.as_dev(U_d)
+        Fv = _opt_vec(F, 2 * self.nn)
+        Ce = None if Ce is None else nat.as_dev(Ce)
+        out = self._empty(2 * self.nn)
+        it = ctypes.c_int32()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_solve_momentum(self._h, nat.ptr(U_n), nat.ptr(P_n), nat.ptr(Ce), float(rho), float(mu),
+                                                float(dt), nat.ptr(Ud), nat.ptr(Fv), nat.ptr(out), ctypes.byref(it)),
+                  "fcfd_solve_momentum")
+        return out, it.value
+
+    def four_step_timestep(self, U_n, P_n, dt, rho, mu, U_d=None, F_p=None, F=None, out=None):
+        U_n, P_n = nat.as_dev(U_n), nat.as_dev(P_n)
+        Ud = None if U_d is None else nat.as_dev(U_d)
+        Fp = _opt_vec(F_p, self.nn)
+        Fv = _opt_vec(F, 2 * self.nn)
+        if out is None:
+            U, P = self._empty(2 * self.nn), self._empty(self.nn)
+        else:
+            U, P = out
+        iters = (ctypes.c_int32 * 2)()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_four_step_timestep(self._h, nat.ptr(U_n), nat.ptr(P_n), float(dt), float(rho),
+                                                    float(mu), nat.ptr(Ud), nat.ptr(Fp), nat.ptr(Fv), nat.ptr(U),
+                                                    nat.ptr(P), iters), "fcfd_four_step_timestep")
+        return U, P, int(iters[0]), int(iters[1])
+
+    def dot(self, x, y):
+        out = ctypes.c_double()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_dot(self._h, nat.ptr(x), nat.ptr(y), x.numel(), ctypes.byref(out)), "fcfd_dot")
+        return out.value
+
+    def axpby(self, a, x, b, y):
+        self._sync_stream()
+        nat.check(self._lib.fcfd_axpby(self._h, float(a), nat.ptr(x), float(b), nat.ptr(y), y.numel()), "fcfd_axpby")
+        return y
+
+    def bench_momentum_apply(self, reps=20):
+        ms = ctypes.c_float()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_bench_momentum_apply(self._h, int(reps), ctypes.byref(ms)), "fcfd_bench_momentum_apply")
+        return ms.value
+
+
+def _opt_vec(v, n):
+    """None for an absent or scalar-zero vector (the reference passes 0)."""
+    if v is None:
+        return None
+    if np.isscalar(v) or (hasattr(v, "ndim") and v.ndim == 0):
+        if float(v) == 0.0:
+            return None
+        return nat.as_dev(np.full(n, float(v)))
+    return nat.as_dev(v)
+
+
+# ------------------------------------------------------------------------------------ grid registry
+_grid_cache = {}
+
+
+def _h_from_shape(wq, Nderiv):
+    """Element sizes (hx, hy) of a rectangular quad4 element from its 2x2
+    Gauss data (dN1/dx and dN3/dy at the first point)."""
+    s = 1.0 / math.sqrt(3.0)
+    Nd = nat.to_numpy(Nderiv)
+    hx = (1 + s) / (2 * Nd[0, 0, 1])
+    hy = (1 + s) / (2 * Nd[0, 1, 3])
+    return float(hx), float(hy)
+
+
+def infer_grid(nconn, nn):
+    """(nex, ney, periodic) of the structured Q1 grid behind ``nconn``;
+    raises for connectivity that is not one of the reference's regular meshes."""
+    nc = nconn
+    ne = int(nc.shape[0])
+    first = nat.to_numpy(nc[0]).astype(np.int64)
+    last = nat.to_numpy(nc[ne - 1]).astype(np.int64)
+    if ne == 1:
+        raise ValueError("cannot infer a grid from a single element")
+    nnx = int(first[3])
+    if nn == ne:
+        nex, periodic = nnx, True
+    else:
+        nex, periodic = nnx - 1, False
+    if nex < 1 or ne % nex:
+        raise ValueError("nconn is not a structured Q1 grid of the reference layout")
+    ney = ne // nex
+    if (not periodic and nn != (nex + 1) * (ney + 1)) or (periodic and nn != nex * ney):
+        raise ValueError("nconn / nn mismatch for a structured Q1 grid")
+    ie, je = nex - 1, ney - 1
+    if periodic:
+        exp = [ie + je * nnx, 0 + je * nnx, 0, ie]
+    else:
+        n0 = je * nnx + ie
+        exp = [n0, n0 + 1, n0 + nnx + 1, n0 + nnx]
+    if list(last) != exp or list(first[:3]) != [0, 1, nnx + 1]:
+        raise ValueError("nconn is not a structured Q1 grid of the reference layout")
+    return nex, ney, periodic
+
+
+def grid_context(nconn, nn, hx=1.0, hy=1.0):
+    """Cached operator-only context for the grid behind nconn."""
+    nex, ney, periodic = infer_grid(nconn, nn)
+    key = (nex, ney, periodic, round(hx, 15), round(hy, 15))
+    ctx = _grid_cache.get(key)
+    if ctx is None:
+        ctx = Context(nex, ney, hx * nex, hy * ney, periodic, precond=None)
+        _grid_cache[key] = ctx
+    return ctx

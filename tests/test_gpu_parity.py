@@ -1,0 +1,301 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Tolerances (FP64 form): integer / index work bit-exact; single element and
+matvec kernels 1e-13 relative (different summation order than the oracle's
+BLAS); solves and timesteps 1e-10 relative L2, which is the accuracy the
+north-star sets for the FP64 form, with identical iteration counts.
+"""
+import numpy as np
+import pytest
+
+from oracle import elements as oel
+from oracle import emult as oem
+from oracle import mesh as oms
+from oracle import multigrid as omg
+from oracle import ns
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(20241203)
+
+
+def _np(x):
+    return x.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return ns.rel_l2(a, b)
+
+
+def _ctx(nex, ney, lx, ly, periodic, **kw):
+    from jax_fem_cfd._context import Context
+    print(f"\n[case] grid {nex}x{ney} periodic={periodic} {kw.get('precond')}", flush=True)
+    return Context(nex, ney, lx, ly, periodic, **kw)
+
+
+# ----------------------------------------------------------------------------------------- mesh
+@pytest.mark.parametrize("nex,ney,periodic", [(7, 5, False), (16, 16, False), (8, 6, True), (64, 64, True)])
+def test_device_mesh_bit_exact(nex, ney, periodic):
+    c = _ctx(nex, ney, 2.0, 1.5, periodic)
+    coords, nconn = c.mesh()
+    f = oms.create_periodic_mesh_2d if periodic else oms.create_mesh_2d
+    oc, on, *_ = f(nex, ney, 2.0, 1.5)
+    assert np.array_equal(_np(nconn), on)
+    assert np.array_equal(_np(coords), oc)
+
+
+def test_element_constants_and_node_sums():
+    c = _ctx(8, 8, 1.0, 1.0, False)
+    coords, nconn, *_ = oms.create_mesh_2d(8, 8, 1.0, 1.0)
+    wq, N, Nd = oel.quad4_shape_functions(coords[nconn[0]])
+    assert np.allclose(c.element_constant("Me"), oel.mass_element(wq, N), rtol=0, atol=1e-17)
+    assert np.allclose(c.element_constant("Le"), oel.laplacian_element_calc(wq, Nd), rtol=0, atol=1e-15)
+    Ge = oel.gradient_element_calc(wq, N, Nd, 2)
+    assert np.allclose(c.element_constant("Gx"), Ge[0], rtol=0, atol=1e-16)
+    assert np.allclose(c.element_constant("Gy"), Ge[1], rtol=0, atol=1e-16)
+    M, _ = oel.lumped_mass_matrix(nconn, wq, N, 81, 2)
+    assert np.allclose(_np(c.lumped_mass()), M, rtol=1e-15, atol=0)
+    Ld = oel.laplacian_diag(oel.laplacian_element_calc(wq, Nd), nconn, 81)
+    assert np.allclose(_np(c.laplacian_diag()), Ld, rtol=1e-15, atol=0)
+
+
+# ----------------------------------------------------------------------------------------- Ce
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("beta", [0.0, 0.5, 1.0])
+def test_convection_matrices(periodic, beta):
+    n = 12
+    c = _ctx(n, n + 2, 1.0, 1.3, periodic)
+    f = oms.create_periodic_mesh_2d if periodic else oms.create_mesh_2d
+    coords, nconn, *_ = f(n, n + 2, 1.0, 1.3)
+    nn = coords.shape[0]
+    U = RNG.standard_normal(2 * nn)
+    wq, N, Nd = oel.quad4_shape_functions(coords[nconn[0]])
+    ref = oel.conservative_convec_elem_calc(nconn, U, wq, N, Nd, nn, 2, beta)
+    got = _np(c.conv_elem_calc(U, beta))
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+# ----------------------------------------------------------------------------------------- matvecs
+@pytest.mark.parametrize("periodic", [False, True])
+def test_element_mult_api(periodic):
+    from jax_fem_cfd.core import element_mult as em
+    n = 10
+    f = oms.create_periodic_mesh_2d if periodic else oms.create_mesh_2d
+    coords, nconn, *_ = f(n, n + 3, 1.0, 1.3)
+    nn = coords.shape[0]
+    print(f"\n[case] element_mult periodic={periodic}", flush=True)
+    wq, N, Nd = oel.quad4_shape_functions(coords[nconn[0]])
+    Me, Le, Ge = oel.mass_element(wq, N), oel.laplacian_element_calc(wq, Nd), oel.gradient_element_calc(wq, N, Nd, 2)
+    U, P = RNG.standard_normal(2 * nn), RNG.standard_normal(nn)
+    Ce = oel.conservative_convec_elem_calc(nconn, U, wq, N, Nd, nn, 2, 0.5)
+    Se = RNG.standard_normal(Ce.shape)
+    tol = 1e-13
+    assert _rel(_np(em.consistent_mass_mult(U, Me, nconn, nn, 2)), oem.consistent_mass_mult(U, Me, nconn, nn, 2)) < tol
+    assert _rel(_np(em.laplacian_element_mult(P, Le, nconn, nn)), oem.laplacian_element_mult(P, Le, nconn, nn)) < tol
+    Leoff = Le - np.diag(np.diag(Le))
+    assert _rel(_np(em.laplacian_element_mult(P, Leoff, nconn, nn)), oem.laplacian_element_mult(P, Leoff, nconn, nn)) < tol
+    assert _rel(_np(em.gradient_element_mult(P, Ge, nconn, nn, 2)), oem.gradient_element_mult(P, Ge, nconn, nn, 2)) < tol
+    assert _rel(_np(em.divergence_element_mult(U, Ge, nconn, nn, 2)), oem.divergence_element_mult(U, Ge, nconn, nn, 2)) < tol
+    assert _rel(_np(em.visc_conv_mult(U, Ce, Le, 0.01, 1.3, nconn, nn, 2)),
+                oem.visc_conv_mult(U, Ce, Le, 0.01, 1.3, nconn, nn, 2)) < tol
+    assert _rel(_np(em.scalar_visc_conv_mult(P, Ce, Le, 1e-3, nconn, nn)),
+                oem.scalar_visc_conv_mult(P, Ce, Le, 1e-3, nconn, nn)) < tol
+    assert _rel(_np(em.scalar_stabilized_visc_conv_mult(P, Ce, Se, Le, 1e-3, nconn, nn)),
+                oem.scalar_stabilized_visc_conv_mult(P, Ce, Se, Le, 1e-3, nconn, nn)) < tol
+    assert _rel(_np(em.scalar_consistent_mass_SUPG_mult(P, Se, Me, nconn, nn)),
+                oem.scalar_consistent_mass_SUPG_mult(P, Se, Me, nconn, nn)) < tol
+
+
+def test_element_calc_api():
+    from jax_fem_cfd.core import element_calc as ec
+    from jax_fem_cfd.core import shape_functions as sf
+    coords, nconn, *_ = oms.create_mesh_2d(9, 7, 0.9, 0.7)
+    nn = coords.shape[0]
+    wq, N, Nd = sf.quad4_shape_functions(coords[nconn[0]])
+    U = RNG.standard_normal(2 * nn)
+    got = _np(ec.conservative_convec_elem_calc(nconn, U, wq, N, Nd, nn, 2, 0.5))
+    ref = oel.conservative_convec_elem_calc(nconn, U, wq, N, Nd, nn, 2, 0.5)
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+    M, Me = ec.lumped_mass_matrix(nconn, wq, N, nn, 2)
+    Mo, _ = oel.lumped_mass_matrix(nconn, wq, N, nn, 2)
+    assert np.allclose(_np(M), Mo, rtol=1e-15, atol=0)
+    assert np.allclose(_np(ec.laplacian_diag(ec.laplacian_element_calc(wq, Nd), nconn, nn)),
+                       oel.laplacian_diag(oel.laplacian_element_calc(wq, Nd), nconn, nn), rtol=1e-15, atol=0)
+
+
+def test_boundary_integral():
+    from jax_fem_cfd.core import element_calc as ec
+    s, surf, h, U_d = ns.build([8, 6], [1.0, 0.75], False, [0, 1, 3], [], 1e-8, True, None, None)
+    U_d = U_d + RNG.standard_normal(U_d.shape)
+    ref = oel.u_dot_n_boundary_integral_1d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords)
+    got = _np(ec.u_dot_n_boundary_integral_1d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords))
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+# ----------------------------------------------------------------------------------------- multigrid
+@pytest.mark.parametrize("periodic,set_zeroP", [(False, True), (False, False), (True, False)])
+def test_vcycle(periodic, set_zeroP):
+    n, nf = 32, 4
+    s, *_ = ns.build([n, n], [1.0, 1.0], periodic, [], [], 1e-10, set_zeroP, "multigrid", [nf, nf])
+    c = _ctx(n, n, 1.0, 1.0, periodic, tol=1e-10, set_zeroP=set_zeroP, precond="multigrid",
+             mg_levels=len(s.levels), gamma_p=[0] if set_zeroP else None, mg_gamma_p=[L["gamma_p"] for L in s.levels])
+    r = RNG.standard_normal(s.nn)
+    if set_zeroP:
+        r[0] = 0
+    else:
+        r -= r.mean()
+    ref = s.M_op(r)
+    got = _np(c.vcycle(r))
+    assert _rel(got, ref) < 1e-12
+
+
+def test_vcycle_api_tuple():
+    from jax_fem_cfd.core import multigrid as mg
+    from jax_fem_cfd.core import shape_functions as sf
+    from jax_fem_cfd.setup import regular_mesh as rm
+    data = mg.multigrid_setup([32, 32], [4, 4], [1, 1], [], rm.create_mesh_2d, sf.quad4_shape_functions, 1, None)
+    s, *_ = ns.build([32, 32], [1.0, 1.0], False, [], [], 1e-8, True, "multigrid", [4, 4])
+    r = RNG.standard_normal(s.nn)
+    r[0] = 0
+    got = _np(mg.v_cycle(r, *data, 1e-8, False, True))
+    assert _rel(got, s.M_op(r)) < 1e-10
+
+
+# ----------------------------------------------------------------------------------------- solvers
+@pytest.mark.parametrize("set_zeroP", [True, False])
+def test_pressure_cg_multigrid(set_zeroP):
+    n = 64
+    s, *_ = ns.build([n, n], [1.0, 1.0], False, [], [], 1e-10, set_zeroP, "multigrid", [8, 8])
+    c = _ctx(n, n, 1.0, 1.0, False, tol=1e-10, set_zeroP=set_zeroP, precond="multigrid", mg_levels=len(s.levels),
+             gamma_p=[0] if set_zeroP else None, mg_gamma_p=[L["gamma_p"] for L in s.levels])
+    b = RNG.standard_normal(s.nn)
+    if set_zeroP:
+        b[0] = 0
+    else:
+        b -= b.mean()
+
+    def A(x):
+        y = oem.laplacian_element_mult(x, s.Le, s.nconn, s.nn)
+        if set_zeroP:
+            y[0] = 0
+        return y
+
+    from oracle.solvers import cg
+    xr, kr = cg(A, b, tol=1e-10, maxiter=1000, M=s.M_op)
+    x, k, conv = c.pressure_cg(b, tol=1e-10, maxiter=1000)
+    assert conv and k == kr
+    assert _rel(_np(x), xr) < 1e-10
+
+
+def test_generic_solver_wrappers():
+    from jax_fem_cfd.utils import jax_iterative_solvers as jis
+    from oracle import solvers as osv
+    import torch
+    n = 50
+    A = np.diag(np.linspace(1, 10, n)) + 0.1 * RNG.standard_normal((n, n))
+    S = A + A.T + 20 * np.eye(n)
+    b = RNG.standard_normal(n)
+    At, St = torch.tensor(A, device="cuda"), torch.tensor(S, device="cuda")
+    x, k = jis.cg(lambda v: St @ v, b, tol=1e-10)
+    xr, kr = osv.cg(lambda v: S @ v, b, tol=1e-10)
+    assert k == kr and _rel(_np(x), xr) < 1e-9
+    x, k = jis.bicgstab(lambda v: At @ v, b, tol=1e-10)
+    xr, kr = osv.bicgstab(lambda v: A @ v, b, tol=1e-10)
+    assert k == kr and _rel(_np(x), xr) < 1e-8
+
+
+def _cavity(n, tol, precision="fp64", ce_format=None, mu=1e-3):
+    import jax_fem_cfd.setup.simulation as sim
+    cfg = sim.Config(mesh="simple", solver="standard",
+                     mesh_config=sim.SimpleMeshConfig([n, n], [1.0, 1.0], 1, False),
+                     solver_config=sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, tol, True, "multigrid",
+                                                            [4, 4], precision=precision, ce_format=ce_format))
+    compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
+    U_d = sim.set_face_velocity(U_d, surf[2][1:-1], gamma_d, 1.0, nn, 1)
+    U0 = np.random.default_rng(1).normal(0, 1e-3, 2 * nn)
+    U0[gamma_d] = U_d
+    return compute, timestep, U_d, U0, np.zeros(nn), nn
+
+
+def test_momentum_solve_matches_oracle():
+    n = 32
+    compute, timestep, U_d, U0, P0, nn = _cavity(n, 1e-10)
+    ctx = timestep.context
+    s, surf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-10, True, "multigrid", [4, 4])
+    oUd = ns.cavity_lid(s, surf, oUd)
+    P0 = RNG.standard_normal(nn) * 0.1
+    Ce = oel.conservative_convec_elem_calc(s.nconn, U0, s.wq, s.N, s.Nderiv, nn, 2, 0.5)
+    ref, kr = ns.solve_momentum(U0, P0, s.Me, s.Le, s.Ge, Ce, 1.0, 1e-3, 5e-4, s.nconn, nn, 2, s.gamma_d, oUd, 0,
+                                1e-10, 0.5, 1)
+    got, k = ctx.solve_momentum(U0, P0, 1.0, 1e-3, 5e-4, U_d=U_d)
+    assert k == kr and _rel(_np(got), ref) < 1e-10
+    got2, k2 = ctx.solve_momentum(U0, P0, 1.0, 1e-3, 5e-4, U_d=U_d, Ce=Ce)
+    assert k2 == kr and _rel(_np(got2), ref) < 1e-10
+
+
+# ----------------------------------------------------------------------------------------- timestep
+def test_taylor_green_64_100_steps():
+    """Config 1: 2D periodic Taylor-Green, 64x64 Q1, nu = 0.01, dt = 1e-3, 100 steps,
+    FP64, MG-PCG and BiCGSTAB tol 1e-10 -- per-field error vs the oracle."""
+    import jax_fem_cfd.setup.simulation as sim
+    rho, mu, dt, steps = 1.0, 0.01, 1e-3, 100
+    cfg = sim.Config(mesh="simple", solver="standard",
+                     mesh_config=sim.SimpleMeshConfig([64, 64], [2 * np.pi, 2 * np.pi], 1, True),
+                     solver_config=sim.StandardSolverConfig([], [], [], False, 1e-10, False, "multigrid", [2, 2]))
+    compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
+    u, v, p = ns.taylor_green(coords, 0, rho, mu)
+    U0, P0 = np.concatenate([u, v]), p - p.mean()
+    s, *_ = ns.build([64, 64], [2 * np.pi, 2 * np.pi], True, [], [], 1e-10, False, "multigrid", [2, 2])
+    U, P = U0, P0
+    Uo, Po = U0.copy(), P0.copy()
+    its, itso = [], []
+    for _ in range(steps):
+        F_p, F = compute(U_d, None, 0, 0)
+        U, P, ui, pi = timestep(U, P, dt, rho, mu, U_d, F_p, F)
+        Uo, Po, uio, pio = ns.four_step_timestep(s, Uo, Po, dt, rho, mu, None, 0, 0)
+        its.append((ui, pi))
+        itso.append((uio, pio))
+    Ug, Pg = _np(U), _np(P)
+    eu = _rel(Ug[:nn], Uo[:nn])
+    ev = _rel(Ug[nn:], Uo[nn:])
+    ep = _rel(Pg, Po)
+    print(f"TG64 100 steps: relL2 u {eu:.2e} v {ev:.2e} p {ep:.2e}; iters equal: {its == itso}")
+    assert its == itso
+    assert eu < 1e-10 and ev < 1e-10 and ep < 1e-10
+    ue, ve, pe = ns.taylor_green(coords, steps * dt, rho, mu)
+    assert _rel(Ug, np.concatenate([ue, ve])) < 1e-4
+
+
+def test_cavity_steps_match_oracle_and_are_deterministic():
+    n = 64
+    compute, timestep, U_d, U0, P0, nn = _cavity(n, 1e-8)
+    s, surf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-8, True, "multigrid", [4, 4])
+    oUd = ns.cavity_lid(s, surf, oUd)
+    U, P, Uo, Po = U0, P0, U0.copy(), P0.copy()
+    for _ in range(5):
+        F_p, F = compute(U_d, None, 0, 0)
+        U, P, ui, pi = timestep(U, P, 5e-4, 1.0, 1e-3, U_d, F_p, F)
+        Uo, Po, uio, pio = ns.four_step_timestep(s, Uo, Po, 5e-4, 1.0, 1e-3, oUd, 0, 0)
+        assert (ui, pi) == (uio, pio)
+    assert _rel(_np(U), Uo) < 1e-10 and _rel(_np(P), Po) < 1e-10
+    # bitwise run-to-run reproducibility (no float atomics)
+    Ua, Pa, *_ = timestep(U0, P0, 5e-4, 1.0, 1e-3, U_d, 0, 0)
+    Ub, Pb, *_ = timestep(U0, P0, 5e-4, 1.0, 1e-3, U_d, 0, 0)
+    assert np.array_equal(_np(Ua), _np(Ub)) and np.array_equal(_np(Pa), _np(Pb))
+
+
+@pytest.mark.parametrize("ce_format", ["f32", "f16s", "bf16s"])
+def test_mixed_precision_cavity(ce_format):
+    """Mixed form (FP32 V-cycle in FP64 flexible CG, reduced-precision Ce):
+    after 5 steps at tol 1e-8 the fields stay within 1e-6 relative L2 of the
+    FP64 form (velocity) and 1e-5 (pressure)."""
+    n = 64
+    c64, t64, U_d, U0, P0, nn = _cavity(n, 1e-8)
+    cm, tm, *_ = _cavity(n, 1e-8, precision="mixed", ce_format=ce_format)
+    U, P, Um, Pm = U0, P0, U0, P0
+    for _ in range(5):
+        U, P, *_ = t64(U, P, 5e-4, 1.0, 1e-3, U_d, 0, 0)
+        Um, Pm, ui, pi = tm(Um, Pm, 5e-4, 1.0, 1e-3, U_d, 0, 0)
+    eu, ep = _rel(_np(Um), _np(U)), _rel(_np(Pm), _np(P))
+    print(f"mixed {ce_format}: relL2 U {eu:.2e} P {ep:.2e} iters {ui},{pi}")
+    assert eu < 1e-6 and ep < 1e-5
