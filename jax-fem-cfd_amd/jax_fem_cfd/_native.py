@@ -85,6 +85,10 @@ def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise NativeError(f"libfcfd.so not built at {path}: run __graft_entry__.build() "
                           "(or make -C jax-fem-cfd_amd)")
+    # torch ships its own HIP runtime (SONAME libamdhip64.so.7).  Loading it first makes
+    # libfcfd bind to that same runtime instead of pulling a second copy from /opt/rocm
+    # into the process (two HIP runtimes in one process do not share the device).
+    import torch as _t  # noqa: F401
     lib = ctypes.CDLL(path)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)

@@ -284,11 +284,16 @@ def test_cavity_steps_match_oracle_and_are_deterministic():
     assert np.array_equal(_np(Ua), _np(Ub)) and np.array_equal(_np(Pa), _np(Pb))
 
 
+MIXED_TOL = {"f32": (1e-6, 1e-5), "f16s": (1e-5, 1e-4), "bf16s": (1e-4, 1e-3)}
+
+
 @pytest.mark.parametrize("ce_format", ["f32", "f16s", "bf16s"])
 def test_mixed_precision_cavity(ce_format):
     """Mixed form (FP32 V-cycle in FP64 flexible CG, reduced-precision Ce):
-    after 5 steps at tol 1e-8 the fields stay within 1e-6 relative L2 of the
-    FP64 form (velocity) and 1e-5 (pressure)."""
+    after 5 steps at tol 1e-8 the fields stay within MIXED_TOL relative L2 of
+    the FP64 form -- (velocity, pressure) = (1e-6, 1e-5) for FP32 Ce,
+    (1e-5, 1e-4) for FP16+scale (11-bit entries), (1e-4, 1e-3) for
+    BF16+scale (8-bit entries)."""
     n = 64
     c64, t64, U_d, U0, P0, nn = _cavity(n, 1e-8)
     cm, tm, *_ = _cavity(n, 1e-8, precision="mixed", ce_format=ce_format)
@@ -298,4 +303,5 @@ def test_mixed_precision_cavity(ce_format):
         Um, Pm, ui, pi = tm(Um, Pm, 5e-4, 1.0, 1e-3, U_d, 0, 0)
     eu, ep = _rel(_np(Um), _np(U)), _rel(_np(Pm), _np(P))
     print(f"mixed {ce_format}: relL2 U {eu:.2e} P {ep:.2e} iters {ui},{pi}")
-    assert eu < 1e-6 and ep < 1e-5
+    tu, tp = MIXED_TOL[ce_format]
+    assert eu < tu and ep < tp
