@@ -260,9 +260,8 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     c.c_x = dalloc<double>(c, nn);
     c.c_rhs = dalloc<double>(c, nn);
     c.ustar = dalloc<double>(c, 2 * nn);
-    c.partials = dalloc<double>(c, (size_t)MAX_RED * MAX_BLOCKS);
-    c.counter = dalloc<unsigned>(c, 1);
-    FCFD_HIP(hipMemset(c.counter, 0, sizeof(unsigned)));
+    c.pstride = blocks_for(2 * nn) + 1;
+    c.partials = dalloc<double>(c, (size_t)MAX_RED * c.pstride);
     c.st = dalloc<DevState>(c, 1);
     FCFD_HIP(hipMemset(c.st, 0, sizeof(DevState)));
     FCFD_HIP(hipHostMalloc((void**)&c.h_pinned, 64, hipHostMallocDefault));

@@ -11,7 +11,6 @@
 namespace fcfd {
 
 constexpr int BLOCK = 256;         // threads per node/element kernel block (4 waves)
-constexpr int MAX_BLOCKS = 2048;   // cap on grid size of reducing kernels
 constexpr int NXCD = 8;            // MI355X: 8 XCDs, blocks dispatched round-robin
 
 // ---------------------------------------------------------------------------------------------
@@ -177,87 +176,51 @@ __device__ __forceinline__ void block_sum(double (&v)[NR]) {
     __syncthreads();
 }
 
-// Publish this block's partials; returns true (uniformly) in the last block, which then sees
-// every block's partials.
-template <int NR>
-__device__ __forceinline__ bool publish_partials(const double (&v)[NR], double* partials,
-                                                 unsigned* counter, int lb, int nb) {
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r)
-            __hip_atomic_store(&partials[r * MAX_BLOCKS + lb], v[r], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == (unsigned)(nb - 1));
-        if (last) atomicExch(counter, 0u);
-    }
-    __syncthreads();
-    bool l = last;
-    if (l) __threadfence();
-    return l;
-}
+// One item (node / element) per thread; logical block lb owns items [lb*BLOCK, lb*BLOCK+BLOCK).
+inline int blocks_for(int n) { return n > 0 ? (n + BLOCK - 1) / BLOCK : 1; }
 
-// Called by every thread of the last block: fixed-order totals of all partials.
-template <int NR>
-__device__ __forceinline__ void gather_partials(double (&tot)[NR], const double* partials, int nb) {
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        double s = 0.0;
-        for (int k = threadIdx.x; k < nb; k += BLOCK)
-            s += __hip_atomic_load(&partials[r * MAX_BLOCKS + k], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        tot[r] = s;
-    }
-    block_sum<NR>(tot);
-}
-
-// Work split of a node/element range over a capped grid: logical block lb owns the contiguous
-// range [lo, hi).
-struct Range {
-    int lo, hi;
-};
-__device__ __forceinline__ Range block_range(int n, int lb, int nb) {
-    int chunk = (n + nb - 1) / nb;
-    chunk = (chunk + BLOCK - 1) / BLOCK * BLOCK;
-    int lo = lb * chunk;
-    int hi = lo + chunk;
-    if (hi > n) hi = n;
-    if (lo > n) lo = n;
-    return {lo, hi};
-}
-
-inline int grid_for(int n) {
-    int b = (n + BLOCK - 1) / BLOCK;
-    if (b > MAX_BLOCKS) b = MAX_BLOCKS;
-    if (b < 1) b = 1;
-    return b;
-}
-
-// Generic node kernel with NR fused reductions.  Body: void operator()(int n, double (&acc)[NR])
-// Fin: void operator()(const double (&tot)[NR]) run by thread 0 of the last block.
-// `active` (device flag, may be null) lets a whole chunk of solver iterations be enqueued
-// ahead: once the solver has converged every later kernel returns at entry.
-template <int NR, class Body, class Fin>
+// Generic node kernel with NR fused reductions.  Body: void operator()(int n, double (&acc)[NR]).
+// Each block reduces its per-thread sums in a fixed tree and stores one partial per reduction
+// (plain stores); finalize_kernel (one block, launched right after on the same stream) sums the
+// partials in fixed order and runs the solver's scalar update.  No atomics and no grid-scope
+// fences: visibility comes from the kernel boundary, and results are bitwise reproducible.
+// `active` (device flag, may be null) lets a whole chunk of solver iterations be enqueued ahead:
+// once the solver has converged every later kernel returns at entry.
+template <int NR, class Body>
 __global__ __launch_bounds__(BLOCK) void node_kernel(int n, const int* __restrict__ active, Body body,
-                                                    Fin fin, double* partials, unsigned* counter) {
+                                                    double* __restrict__ partials, int pstride) {
     if (active && *active == 0) return;
-    const int nb = gridDim.x;
-    const int lb = xcd_block(blockIdx.x, nb);
-    Range rg = block_range(n, lb, nb);
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int k = lb * BLOCK + threadIdx.x;
     double acc[NR > 0 ? NR : 1];
 #pragma unroll
     for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
-    for (int k = rg.lo + threadIdx.x; k < rg.hi; k += BLOCK) body(k, acc);
+    if (k < n) body(k, acc);
     if constexpr (NR > 0) {
         block_sum<NR>(acc);
-        if (publish_partials<NR>(acc, partials, counter, lb, nb)) {
-            double tot[NR];
-            gather_partials<NR>(tot, partials, nb);
-            if (threadIdx.x == 0) fin(tot);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) partials[r * pstride + lb] = acc[r];
         }
     }
+}
+
+constexpr int FIN_BLOCK = 1024;
+
+// Fin: void operator()(const double (&tot)[NR]) run by thread 0 with the grid totals.
+template <int NR, class Fin>
+__global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(int nb, const int* __restrict__ active, Fin fin,
+                                                            const double* __restrict__ partials, int pstride) {
+    if (active && *active == 0) return;
+    double tot[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        double s = 0.0;
+        for (int k = threadIdx.x; k < nb; k += FIN_BLOCK) s += partials[r * pstride + k];
+        tot[r] = s;
+    }
+    block_sum<NR>(tot);
+    if (threadIdx.x == 0) fin(tot);
 }
 
 struct NoFin {

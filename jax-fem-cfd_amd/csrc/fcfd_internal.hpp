@@ -90,7 +90,7 @@ struct Ctx {
     float *cc_r32 = nullptr, *cc_p32 = nullptr, *cc_Ap32 = nullptr;
     // reductions
     double* partials = nullptr;
-    unsigned* counter = nullptr;
+    int pstride = 0;            // partials per reduction slot (max blocks of a reducing kernel)
     DevState* st = nullptr;
     int* h_pinned = nullptr;   // pinned host mirror for flags / counts
     // iteration predictions for chunked launching
@@ -122,6 +122,16 @@ T* dalloc(Ctx& c, size_t n) {
     FCFD_HIP(hipMalloc(&p, n * sizeof(T)));
     c.allocs.push_back(p);
     return (T*)p;
+}
+
+// Launch a node kernel over n items and, when it reduces (NR > 0), its finalize kernel.
+template <int NR, class Body, class Fin>
+inline void launch_node(Ctx& c, int n, const int* active, const Body& body, const Fin& fin) {
+    const int nb = blocks_for(n);
+    if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+    node_kernel<NR><<<nb, BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
+    if constexpr (NR > 0)
+        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, fin, c.partials, c.pstride);
 }
 
 // host element constants --------------------------------------------------------------------

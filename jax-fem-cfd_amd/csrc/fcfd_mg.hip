@@ -15,6 +15,7 @@
 namespace fcfd {
 
 constexpr double OMEGA = 2.0 / 3.0;
+constexpr int COARSE_BLOCK = 1024;
 
 template <class T> struct LvBufs;
 template <> struct LvBufs<double> {
@@ -169,37 +170,113 @@ struct Jacobi {
 };
 
 // ------------------------------------------------------------------------------ coarse level
-// One workgroup.  Pinned: CG (M = identity) to tol with maxiter 1000 on L with pinned rows
-// zeroed (multigrid.py:278-287).  Neumann: one Jacobi from zero (:289-301).
-constexpr int COARSE_BLOCK = 1024;
+// One workgroup.  Pinned form: CG (M = identity) to tol with maxiter 1000 on L with the pinned
+// rows zeroed (multigrid.py:278-287).  Neumann form: one Jacobi from zero (:289-301).
 
+// Neumann: x = omega (rN - Le_off 0) / L_diag + (1 - omega) 0, then its mean
 template <class T, class TR>
-__global__ __launch_bounds__(COARSE_BLOCK) void k_coarse(Grid2D g, const ElemConst* __restrict__ ec, const T* Ld,
-                                                         const uint8_t* pin, const TR* r, T* x, T* cr, T* cp,
-                                                         T* cAp, DevState* st, int l, int neumann, double tol,
-                                                         int maxit, const int* active) {
+__global__ __launch_bounds__(1024) void k_coarse_jacobi(int nn, const T* Ld, const TR* r, T* x, DevState* st, int l,
+                                                        const int* active) {
+    if (active && *active == 0) return;
+    const T mr = (T)st->mg[l][MG_R];
+    double acc[1] = {0.0};
+    for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+        const T v = (T(OMEGA) * (((T)r[n] - mr) - T(0))) / Ld[n] + T(1.0 - OMEGA) * T(0);
+        x[n] = v;
+        acc[0] += (double)v;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) st->mg[l][MG_Y] = acc[0] / nn;
+}
+
+// Pinned, coarse grid of at most NPT * blockDim nodes: search direction in LDS, x / r / Ap in
+// registers (node n = tid + i * blockDim, i < NPT).
+template <class T, class TR, int NPT>
+__global__ __launch_bounds__(1024) void k_coarse_cg_lds(Grid2D g, const ElemConst* __restrict__ ec,
+                                                        const uint8_t* pin, const TR* r, T* x_out, DevState* st,
+                                                        double tol, int maxit, const int* active) {
+    if (active && *active == 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* p = reinterpret_cast<T*>(smem_raw);
+    const int nn = g.nn, bs = blockDim.x;
+    T xr[NPT], rv[NPT], ap[NPT];
+    double g2[1] = {0.0};
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int n = threadIdx.x + i * bs;
+        xr[i] = T(0); rv[i] = T(0); ap[i] = T(0);
+        if (n < nn) {
+            const T b = (pin && pin[n]) ? T(0) : (T)r[n];
+            rv[i] = b;
+            p[n] = b;
+            g2[0] += (double)b * (double)b;
+        }
+    }
+    block_sum<1>(g2);  // barrier: p complete
+    double gamma = g2[0];
+    const double atol2 = fmax(tol * tol * gamma, 0.0);
+    int k = 0;
+    while (gamma > atol2 && k < maxit) {
+        double pap[1] = {0.0};
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int n = threadIdx.x + i * bs;
+            if (n < nn) {
+                NodeNbr s;
+                node_nbr(g, n, s);
+                T X[3][3];
+                load3x3(s, [&](int m) { return p[m]; }, X);
+                const T a = (pin && pin[n]) ? T(0) : ebe_const(s, ec->Le, X);
+                ap[i] = a;
+                pap[0] += (double)X[1][1] * (double)a;
+            }
+        }
+        block_sum<1>(pap);  // barrier: every read of p by the matvec is done
+        const double alpha = gamma / pap[0];
+        double rr[1] = {0.0};
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int n = threadIdx.x + i * bs;
+            if (n < nn) {
+                xr[i] = xr[i] + (T)alpha * p[n];
+                rv[i] = rv[i] - (T)alpha * ap[i];
+                rr[0] += (double)rv[i] * (double)rv[i];
+            }
+        }
+        block_sum<1>(rr);
+        const double beta = rr[0] / gamma;
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int n = threadIdx.x + i * bs;
+            if (n < nn) p[n] = rv[i] + (T)beta * p[n];
+        }
+        __syncthreads();
+        gamma = rr[0];
+        ++k;
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int n = threadIdx.x + i * bs;
+        if (n < nn) x_out[n] = (pin && pin[n]) ? T(0) : xr[i];
+    }
+    if (threadIdx.x == 0) st->coarse_k = k;
+}
+
+// Pinned, larger coarse grids: same recurrence on global (L2-resident) vectors.
+template <class T, class TR>
+__global__ __launch_bounds__(1024) void k_coarse_cg_global(Grid2D g, const ElemConst* __restrict__ ec,
+                                                           const uint8_t* pin, const TR* r, T* x, T* cr, T* cp,
+                                                           T* cAp, DevState* st, double tol, int maxit,
+                                                           const int* active) {
     if (active && *active == 0) return;
     const int nn = g.nn;
-    if (neumann) {
-        const T mr = (T)st->mg[l][MG_R];
-        double acc[1] = {0.0};
-        for (int n = threadIdx.x; n < nn; n += blockDim.x) {
-            // single_jacobi_iter(0, rN): omega (rN - Le_off 0) / L_diag + (1 - omega) 0
-            const T v = (T(OMEGA) * (((T)r[n] - mr) - T(0))) / Ld[n] + T(1.0 - OMEGA) * T(0);
-            x[n] = v;
-            acc[0] += (double)v;
-        }
-        block_sum<1>(acc);
-        if (threadIdx.x == 0) st->mg[l][MG_Y] = acc[0] / nn;
-        return;
-    }
     double g2[1] = {0.0};
     for (int n = threadIdx.x; n < nn; n += blockDim.x) {
         const T b = (pin && pin[n]) ? T(0) : (T)r[n];
         cr[n] = b; cp[n] = b; x[n] = T(0);
         g2[0] += (double)b * (double)b;
     }
-    block_sum<1>(g2);  // includes a barrier: cr / cp visible to the block
+    block_sum<1>(g2);
     double gamma = g2[0];
     const double atol2 = fmax(tol * tol * gamma, 0.0);
     int k = 0;
@@ -235,73 +312,86 @@ __global__ __launch_bounds__(COARSE_BLOCK) void k_coarse(Grid2D g, const ElemCon
     if (threadIdx.x == 0) st->coarse_k = k;
 }
 
+template <class T, class TR>
+static void coarse_solve(Ctx& c, const TR* r, T* y, T* cr, T* cp, T* cAp, double tol, const int* active) {
+    const int l = c.nlev - 1;
+    Level& C = c.lv[l];
+    const int nn = C.g.nn;
+    if (!c.set_zeroP) {
+        const T* Ld = std::is_same<T, float>::value ? (const T*)C.Ld32 : (const T*)C.Ld;
+        k_coarse_jacobi<T, TR><<<1, 1024, 0, c.stream>>>(nn, Ld, r, y, c.st, l, active);
+    } else if (nn <= 4096) {
+        const int npt = nn <= 1024 ? 1 : (nn <= 2048 ? 2 : 4);
+        int bs = (nn + npt - 1) / npt;
+        bs = (bs + 63) / 64 * 64;
+        const size_t shm = (size_t)nn * sizeof(T);
+        if (npt == 1)
+            k_coarse_cg_lds<T, TR, 1><<<1, bs, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
+        else if (npt == 2)
+            k_coarse_cg_lds<T, TR, 2><<<1, bs, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
+        else
+            k_coarse_cg_lds<T, TR, 4><<<1, bs, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
+    } else {
+        k_coarse_cg_global<T, TR><<<1, 1024, 0, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, cr, cp, cAp, c.st, tol,
+                                                            1000, active);
+    }
+    FCFD_CHECK_LAUNCH();
+}
+
 // ------------------------------------------------------------------------------ V-cycle driver
 template <class T>
 static void vcycle_t(Ctx& c, const double* r0, const int* active) {
     using B = LvBufs<T>;
     const int N = c.nlev;
     const int neu = !c.set_zeroP;
+    // pinned form: no mean projections, so its kernels carry no reduction
+    auto run = [&](int n, const auto& body, const MeanFin& fin) {
+        if (neu) launch_node<1>(c, n, active, body, fin);
+        else launch_node<0>(c, n, active, body, fin);
+    };
     for (int l = 0; l < N - 1; ++l) {
         Level& L = c.lv[l];
         Level& Lc = c.lv[l + 1];
-        const int nb = grid_for(L.g.nn);
+        const int n = L.g.nn;
         if (l == 0) {
             if (neu)
-                node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
-                    L.g.nn, active, DownX<T, double>{r0, B::Ld(L), B::x(L), c.st, l},
-                    MeanFin{&c.st->mg[l][MG_X], nullptr, L.g.nn}, c.partials, c.counter);
-            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
-                L.g.nn, active,
-                DownRes<T, double>{L.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::f(L), c.st, l, neu},
-                MeanFin{&c.st->mg[l][MG_F], nullptr, L.g.nn}, c.partials, c.counter);
+                launch_node<1>(c, n, active, DownX<T, double>{r0, B::Ld(L), B::x(L), c.st, l},
+                               MeanFin{&c.st->mg[l][MG_X], nullptr, n});
+            run(n, DownRes<T, double>{L.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::f(L), c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_F], nullptr, n});
         } else {
             if (neu)
-                node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
-                    L.g.nn, active, DownX<T, T>{B::r(L), B::Ld(L), B::x(L), c.st, l},
-                    MeanFin{&c.st->mg[l][MG_X], nullptr, L.g.nn}, c.partials, c.counter);
-            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
-                L.g.nn, active,
-                DownRes<T, T>{L.g, L.ec_dev->Le, B::r(L), B::Ld(L), L.pin, B::x(L), B::f(L), c.st, l, neu},
-                MeanFin{&c.st->mg[l][MG_F], nullptr, L.g.nn}, c.partials, c.counter);
+                launch_node<1>(c, n, active, DownX<T, T>{B::r(L), B::Ld(L), B::x(L), c.st, l},
+                               MeanFin{&c.st->mg[l][MG_X], nullptr, n});
+            run(n, DownRes<T, T>{L.g, L.ec_dev->Le, B::r(L), B::Ld(L), L.pin, B::x(L), B::f(L), c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_F], nullptr, n});
         }
-        node_kernel<1><<<grid_for(Lc.g.nn), BLOCK, 0, c.stream>>>(
-            Lc.g.nn, active, Restrict<T>{L.g, Lc.g, B::f(L), B::r(Lc), c.st, l, neu},
-            MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g.nn}, c.partials, c.counter);
+        run(Lc.g.nn, Restrict<T>{L.g, Lc.g, B::f(L), B::r(Lc), c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g.nn});
     }
     {
         Level& C = c.lv[N - 1];
         const double ctol = std::is_same<T, float>::value ? (c.tol > 1e-5 ? c.tol : 1e-5) : c.tol;
         if (N == 1)
-            k_coarse<T, double><<<1, COARSE_BLOCK, 0, c.stream>>>(C.g, C.ec_dev, B::Ld(C), C.pin, r0, B::y(C),
-                                                                 B::cr(c), B::cp(c), B::cAp(c), c.st, N - 1, neu,
-                                                                 ctol, 1000, active);
+            coarse_solve<T, double>(c, r0, B::y(C), B::cr(c), B::cp(c), B::cAp(c), ctol, active);
         else
-            k_coarse<T, T><<<1, COARSE_BLOCK, 0, c.stream>>>(C.g, C.ec_dev, B::Ld(C), C.pin, B::r(C), B::y(C),
-                                                            B::cr(c), B::cp(c), B::cAp(c), c.st, N - 1, neu, ctol,
-                                                            1000, active);
-        FCFD_CHECK_LAUNCH();
+            coarse_solve<T, T>(c, B::r(C), B::y(C), B::cr(c), B::cp(c), B::cAp(c), ctol, active);
     }
     for (int l = N - 2; l >= 0; --l) {
         Level& L = c.lv[l];
         Level& Lc = c.lv[l + 1];
-        const int nb = grid_for(L.g.nn);
+        const int n = L.g.nn;
         if (neu)
-            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(L.g.nn, active,
-                                                       Prolong<T>{L.g, Lc.g, B::y(Lc), B::f(L), c.st, l},
-                                                       MeanFin{&c.st->mg[l][MG_P], nullptr, L.g.nn}, c.partials,
-                                                       c.counter);
+            launch_node<1>(c, n, active, Prolong<T>{L.g, Lc.g, B::y(Lc), B::f(L), c.st, l},
+                           MeanFin{&c.st->mg[l][MG_P], nullptr, n});
         if (l == 0)
-            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
-                L.g.nn, active,
-                Jacobi<T, double>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, r0, B::x(L), B::f(L), B::y(Lc),
-                                  B::y(L), c.st, l, neu},
-                MeanFin{&c.st->mg[l][MG_Y], nullptr, L.g.nn}, c.partials, c.counter);
+            run(n, Jacobi<T, double>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, r0, B::x(L), B::f(L), B::y(Lc),
+                                     B::y(L), c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
         else
-            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
-                L.g.nn, active,
-                Jacobi<T, T>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, B::r(L), B::x(L), B::f(L), B::y(Lc),
-                             B::y(L), c.st, l, neu},
-                MeanFin{&c.st->mg[l][MG_Y], nullptr, L.g.nn}, c.partials, c.counter);
+            run(n, Jacobi<T, T>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, B::r(L), B::x(L), B::f(L), B::y(Lc),
+                                B::y(L), c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
     }
     FCFD_CHECK_LAUNCH();
 }
@@ -323,19 +413,18 @@ void vcycle(Ctx& c, const double* r, double* z) {
         const double* r;
         __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += r[k]; }
     };
-    node_kernel<1><<<grid_for(nn), BLOCK, 0, c.stream>>>(nn, nullptr, SumBody{r},
-                                                         MeanFin{&c.st->mg[0][MG_R], nullptr, nn}, c.partials,
-                                                         c.counter);
+    launch_node<1>(c, nn, nullptr, SumBody{r},
+                                                         MeanFin{&c.st->mg[0][MG_R], nullptr, nn});
     if (c.precision == FCFD_PREC_MIXED) {
         vcycle_t<float>(c, r, nullptr);
-        node_kernel<0><<<grid_for(nn), BLOCK, 0, c.stream>>>(nn, nullptr,
+        launch_node<0>(c, nn, nullptr,
                                                              ZOut<float>{c.lv[0].y32, z, c.st, !c.set_zeroP},
-                                                             NoFin{}, c.partials, c.counter);
+                                                             NoFin{});
     } else {
         vcycle_t<double>(c, r, nullptr);
-        node_kernel<0><<<grid_for(nn), BLOCK, 0, c.stream>>>(nn, nullptr,
+        launch_node<0>(c, nn, nullptr,
                                                              ZOut<double>{c.lv[0].y, z, c.st, !c.set_zeroP},
-                                                             NoFin{}, c.partials, c.counter);
+                                                             NoFin{});
     }
     FCFD_CHECK_LAUNCH();
 }
@@ -468,8 +557,8 @@ static const void* precondition(Ctx& c, const int* active, bool& is_f32) {
         case FCFD_PC_NONE:
             return c.c_r;
         case FCFD_PC_JACOBI:
-            node_kernel<0><<<grid_for(nn), BLOCK, 0, c.stream>>>(nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y},
-                                                                 NoFin{}, c.partials, c.counter);
+            launch_node<0>(c, nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y},
+                                                                 NoFin{});
             return c.lv[0].y;
         case FCFD_PC_MULTIGRID:
             if (c.precision == FCFD_PREC_MIXED) {
@@ -487,24 +576,22 @@ static const void* precondition(Ctx& c, const int* active, bool& is_f32) {
 static int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
                bool* converged) {
     const Grid2D& g = c.g;
-    const int nn = g.nn, nb = grid_for(nn);
+    const int nn = g.nn;
     const int neu = (c.precond == FCFD_PC_MULTIGRID) && !c.set_zeroP;
     const int flex = c.precision == FCFD_PREC_MIXED && c.precond == FCFD_PC_MULTIGRID;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* Le = c.lv[0].ec_dev->Le;
-    node_kernel<3><<<nb, BLOCK, 0, c.stream>>>(nn, nullptr, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
-                                               CgInitFin{c.st, tol, maxit, nn}, c.partials, c.counter);
+    launch_node<3>(c, nn, nullptr, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
+                                               CgInitFin{c.st, tol, maxit, nn});
     const int* act = &c.st->c_active;
     bool f32;
     const void* y0 = precondition(c, act, f32);
     if (f32)
-        node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(nn, act,
+        launch_node<1>(c, nn, act,
                                                    CgGamma0<float>{c.c_r, (const float*)y0, c.c_z[0], c.c_p[0], c.st, neu},
-                                                   CgGamma0Fin{c.st}, c.partials, c.counter);
+                                                   CgGamma0Fin{c.st});
     else
-        node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(
-            nn, act, CgGamma0<double>{c.c_r, (const double*)y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st},
-            c.partials, c.counter);
+        launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, (const double*)y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st});
     FCFD_CHECK_LAUNCH();
     int launched = 0;
     int chunk = c.pred_p > 0 ? c.pred_p : 1;
@@ -513,21 +600,19 @@ static int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in,
     while (true) {
         for (int t = 0; t < chunk && launched < maxit; ++t, ++launched) {
             const int o = launched & 1, nw = o ^ 1;
-            node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(nn, act,
+            launch_node<1>(c, nn, act,
                                                        CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                                                       CgK1Fin{c.st}, c.partials, c.counter);
-            node_kernel<2><<<nb, BLOCK, 0, c.stream>>>(nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st},
-                                                       CgK2Fin{c.st, nn}, c.partials, c.counter);
+                                                       CgK1Fin{c.st});
+            launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st},
+                                                       CgK2Fin{c.st, nn});
             bool f;
             const void* y = precondition(c, act, f);
             if (f)
-                node_kernel<2><<<nb, BLOCK, 0, c.stream>>>(
-                    nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
-                    CgK3Fin{c.st, flex}, c.partials, c.counter);
+                launch_node<2>(c, nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                    CgK3Fin{c.st, flex});
             else
-                node_kernel<2><<<nb, BLOCK, 0, c.stream>>>(
-                    nn, act, CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
-                    CgK3Fin{c.st, flex}, c.partials, c.counter);
+                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                    CgK3Fin{c.st, flex});
         }
         FCFD_CHECK_LAUNCH();
         read_state_ints(c, &c.st->c_k, 2, h);  // c_k, c_active
@@ -579,19 +664,18 @@ struct SumXBody {
 
 int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
                    double* P_out) {
-    const int nn = c.g.nn, nb = grid_for(nn);
+    const int nn = c.g.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
-    node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(nn, nullptr,
+    launch_node<1>(c, nn, nullptr,
                                                PrhsBody{c.g, c.ec_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
-                                               MeanFin{&c.st->mean_rhs, nullptr, nn}, c.partials, c.counter);
+                                               MeanFin{&c.st->mean_rhs, nullptr, nn});
     const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
     bool conv;
     int it = pcg(c, c.c_rhs, c.set_zeroP ? nullptr : &c.st->mean_rhs, P_n, c.set_zeroP, c.tol, maxit, &conv);
     if (!c.set_zeroP)
-        node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(nn, nullptr, SumXBody{c.c_x},
-                                                   MeanFin{&c.st->mean_p, nullptr, nn}, c.partials, c.counter);
-    node_kernel<0><<<nb, BLOCK, 0, c.stream>>>(nn, nullptr, PoutBody{c.c_x, pin, P_out, c.st, !c.set_zeroP}, NoFin{},
-                                               c.partials, c.counter);
+        launch_node<1>(c, nn, nullptr, SumXBody{c.c_x},
+                                                   MeanFin{&c.st->mean_p, nullptr, nn});
+    launch_node<0>(c, nn, nullptr, PoutBody{c.c_x, pin, P_out, c.st, !c.set_zeroP}, NoFin{});
     FCFD_CHECK_LAUNCH();
     return it;
 }

@@ -1,6 +1,7 @@
 // Element constants, convection-matrix build, element-by-element matvecs, boundary integral,
 // mesh fill and BLAS-1 helpers.
 #include <cmath>
+#include <cstring>
 #include "fcfd_internal.hpp"
 #include "fcfd_ops.hpp"
 
@@ -72,10 +73,8 @@ void launch_fill32(Ctx& c, float* x, float v, int n) {
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_build_ce(Grid2D g, const ElemConst* __restrict__ ec,
                                                    const double* __restrict__ U, double beta, S store) {
-    const int nb = gridDim.x;
-    const int lb = xcd_block(blockIdx.x, nb);
-    Range rg = block_range(g.ne, lb, nb);
-    for (int e = rg.lo + threadIdx.x; e < rg.hi; e += BLOCK) {
+    const int e = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (e < g.ne) {
         int cn[4];
         elem_nodes(g, e, cn);
         double ul[4], vl[4];
@@ -88,7 +87,7 @@ __global__ __launch_bounds__(BLOCK) void k_build_ce(Grid2D g, const ElemConst* _
 }
 
 void build_ce(Ctx& c, const double* U, double beta, int fmt, void* store, float* scale) {
-    int nb = grid_for(c.g.ne);
+    int nb = blocks_for(c.g.ne);
     switch (fmt) {
         case FCFD_CE_F64:
             k_build_ce<<<nb, BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeF64{(double*)store, c.g.ne});
@@ -117,7 +116,7 @@ struct CeAPIStore {
     }
 };
 void build_ce_api(Ctx& c, const double* U, double beta, double* Ce) {
-    k_build_ce<<<grid_for(c.g.ne), BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeAPIStore{Ce});
+    k_build_ce<<<blocks_for(c.g.ne), BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeAPIStore{Ce});
     FCFD_CHECK_LAUNCH();
 }
 
@@ -148,10 +147,8 @@ struct ConstMat {
 
 __global__ __launch_bounds__(BLOCK) void k_const_apply(Grid2D g, ConstMat m, const double* __restrict__ x,
                                                       double* __restrict__ y, int acc) {
-    const int nb = gridDim.x;
-    const int lb = xcd_block(blockIdx.x, nb);
-    Range rg = block_range(g.nn, lb, nb);
-    for (int n = rg.lo + threadIdx.x; n < rg.hi; n += BLOCK) {
+    const int n = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (n < g.nn) {
         NodeNbr s;
         node_nbr(g, n, s);
         double X[3][3];
@@ -166,7 +163,7 @@ void elem_mult_const(Ctx& c, int level, const double* A16, const double* x, doub
     ConstMat m;
     for (int k = 0; k < 16; ++k) m.A[k] = A16[k];
     const Grid2D& g = c.lv[level].g;
-    k_const_apply<<<grid_for(g.nn), BLOCK, 0, c.stream>>>(g, m, x, y, acc);
+    k_const_apply<<<blocks_for(g.nn), BLOCK, 0, c.stream>>>(g, m, x, y, acc);
     FCFD_CHECK_LAUNCH();
 }
 
@@ -181,10 +178,8 @@ __global__ __launch_bounds__(BLOCK) void k_stored_general(Grid2D g, ConstMat K0,
                                                          double c1, const double* __restrict__ A2, double c2,
                                                          int ncomp, const double* __restrict__ x,
                                                          double* __restrict__ y) {
-    const int nb = gridDim.x;
-    const int lb = xcd_block(blockIdx.x, nb);
-    Range rg = block_range(g.nn, lb, nb);
-    for (int n = rg.lo + threadIdx.x; n < rg.hi; n += BLOCK) {
+    const int n = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (n < g.nn) {
         NodeNbr s;
         node_nbr(g, n, s);
         for (int comp = 0; comp < ncomp; ++comp) {
@@ -216,7 +211,7 @@ void elem_mult_general(Ctx& c, const double* K0_16, const double* A1, double c1,
                        int ncomp, const double* x, double* y) {
     ConstMat m;
     for (int k = 0; k < 16; ++k) m.A[k] = K0_16 ? K0_16[k] : 0.0;
-    k_stored_general<<<grid_for(c.g.nn), BLOCK, 0, c.stream>>>(c.g, m, A1, c1, A2, c2, ncomp, x, y);
+    k_stored_general<<<blocks_for(c.g.nn), BLOCK, 0, c.stream>>>(c.g, m, A1, c1, A2, c2, ncomp, x, y);
     FCFD_CHECK_LAUNCH();
 }
 
@@ -318,8 +313,16 @@ void read_state_ints(Ctx& c, const int* dev, int n, int* host) {
 }
 
 double dot(Ctx& c, const double* x, const double* y, int n) {
-    node_kernel<1><<<grid_for(n), BLOCK, 0, c.stream>>>(n, nullptr, DotBody{x, y}, DotFin{c.st}, c.partials,
-                                                        c.counter);
+    const int nb = blocks_for(n);
+    if (nb > c.pstride) {  // generic-vector use on a small grid context: grow the partials buffer
+        FCFD_HIP(hipStreamSynchronize(c.stream));
+        for (auto& p : c.allocs)
+            if (p == (void*)c.partials) p = nullptr;
+        FCFD_HIP(hipFree(c.partials));
+        c.pstride = nb + 1;
+        c.partials = dalloc<double>(c, (size_t)MAX_RED * c.pstride);
+    }
+    launch_node<1>(c, n, nullptr, DotBody{x, y}, DotFin{c.st});
     FCFD_CHECK_LAUNCH();
     double out;
     FCFD_HIP(hipMemcpyAsync(c.h_pinned, &c.st->dot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
