@@ -41,14 +41,18 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * per + (x < rem ? x : rem) + local;
 }
 
-// Stencil context of one node: the four incident elements (element to the
-// lower-left, lower-right, upper-left, upper-right of the node) with the node's
-// local index in each (2, 3, 1, 0), and the 3x3 neighbour node ids.
+// Stencil context of one node: the four incident elements (element to the lower-left,
+// lower-right, upper-left, upper-right of the node) with the node's local index in each
+// (2, 3, 1, 0), and the 3x3 neighbour node ids.  Ids of missing neighbours / elements (edges of a
+// non-periodic grid) are clamped to valid ones (the node itself, element 0) and flagged, so every
+// load can be issued unconditionally and masked afterwards with a select -- no per-lane branches
+// around memory accesses.
 struct NodeNbr {
     int i, j;
-    int e[4];       // element ids (valid only where has[k])
+    int e[4];       // element ids (clamped to 0 where !has)
     bool has[4];
-    int nb[3][3];   // nb[dy+1][dx+1] node id, -1 when outside a non-periodic grid
+    int nb[3][3];   // nb[dy+1][dx+1] node id (clamped to the node itself where !ok)
+    bool ok[3][3];
 };
 
 __device__ __forceinline__ void node_nbr(const Grid2D& g, int n, NodeNbr& s) {
@@ -58,25 +62,59 @@ __device__ __forceinline__ void node_nbr(const Grid2D& g, int n, NodeNbr& s) {
         int im = i == 0 ? g.nnx - 1 : i - 1, ip = i == g.nnx - 1 ? 0 : i + 1;
         int jm = j == 0 ? g.nny - 1 : j - 1, jp = j == g.nny - 1 ? 0 : j + 1;
         int xs[3] = {im, i, ip}, ys[3] = {jm, j, jp};
+#pragma unroll
         for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) s.nb[a][b] = ys[a] * g.nnx + xs[b];
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                s.nb[a][b] = ys[a] * g.nnx + xs[b];
+                s.ok[a][b] = true;
+            }
         s.e[0] = jm * g.nex + im; s.e[1] = jm * g.nex + i;
         s.e[2] = j * g.nex + im;  s.e[3] = j * g.nex + i;
         s.has[0] = s.has[1] = s.has[2] = s.has[3] = true;
     } else {
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
-            int jj = j + a - 1;
+            const int jj = j + a - 1;
+#pragma unroll
             for (int b = 0; b < 3; ++b) {
-                int ii = i + b - 1;
-                bool ok = (ii >= 0) && (ii < g.nnx) && (jj >= 0) && (jj < g.nny);
-                s.nb[a][b] = ok ? jj * g.nnx + ii : -1;
+                const int ii = i + b - 1;
+                const bool ok = (ii >= 0) && (ii < g.nnx) && (jj >= 0) && (jj < g.nny);
+                s.ok[a][b] = ok;
+                s.nb[a][b] = ok ? jj * g.nnx + ii : n;
             }
         }
-        bool l = i >= 1, r = i < g.nex, d = j >= 1, u = j < g.ney;
+        const bool l = i >= 1, r = i < g.nex, d = j >= 1, u = j < g.ney;
         s.has[0] = l && d; s.has[1] = r && d; s.has[2] = l && u; s.has[3] = r && u;
-        s.e[0] = (j - 1) * g.nex + (i - 1); s.e[1] = (j - 1) * g.nex + i;
-        s.e[2] = j * g.nex + (i - 1);       s.e[3] = j * g.nex + i;
+        s.e[0] = s.has[0] ? (j - 1) * g.nex + (i - 1) : 0;
+        s.e[1] = s.has[1] ? (j - 1) * g.nex + i : 0;
+        s.e[2] = s.has[2] ? j * g.nex + (i - 1) : 0;
+        s.e[3] = s.has[3] ? j * g.nex + i : 0;
     }
+}
+
+// Full-wavefront sum with DPP lane moves (quad_perm, row_shr, row_bcast; gfx9 family): the
+// total lands in lane 63 and is broadcast with readlane.  FP64 moves go as two 32-bit halves.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_move(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+    const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false);
+    const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false);
+    return __longlong_as_double(((long long)h2 << 32) | (unsigned int)l2);
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_move<0xb1>(v);        // quad_perm [1,0,3,2]
+    v += dpp_move<0x4e>(v);        // quad_perm [2,3,0,1]
+    v += dpp_move<0x114>(v);       // row_shr:4
+    v += dpp_move<0x118>(v);       // row_shr:8
+    v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+    v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Local row of the node in incident element k (0..3 = element to the lower-left,
@@ -151,10 +189,7 @@ struct CeBF16S {
 // the partials in fixed order and runs the kernel's finalize step (device scalar updates).
 // No float atomics anywhere: results are bitwise reproducible run to run.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_dpp(v); }
 
 // All threads return the block totals (any block size that is a multiple of 64, up to 1024).
 template <int NR>

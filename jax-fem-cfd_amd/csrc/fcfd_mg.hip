@@ -15,7 +15,6 @@
 namespace fcfd {
 
 constexpr double OMEGA = 2.0 / 3.0;
-constexpr int COARSE_BLOCK = 1024;
 
 template <class T> struct LvBufs;
 template <> struct LvBufs<double> {
@@ -75,7 +74,10 @@ struct DownRes {
             load3x3(s, [&](int k) { return x[k] - mx; }, X);
             fv = ((T)r[n] - mr) - ebe_const(s, Le, X);
         } else {
-            load3x3(s, [&](int k) { return (pin && pin[k]) ? T(0) : (T(2.0 / 3.0) * (T)r[k]) / Ld[k]; }, X);
+            if (pin)
+                load3x3(s, [&](int k) { const T v = (T(2.0 / 3.0) * (T)r[k]) / Ld[k]; return pin[k] ? T(0) : v; }, X);
+            else
+                load3x3(s, [&](int k) { return (T(2.0 / 3.0) * (T)r[k]) / Ld[k]; }, X);
             x[n] = X[1][1];
             const bool pn = pin && pin[n];
             const T rh = pn ? T(0) : (T)r[n];
@@ -116,20 +118,20 @@ struct Restrict {
 // (multigrid.py:90-140; periodic coarse grids wrap)
 template <class T>
 __device__ __forceinline__ T interp_at(const Grid2D& gf, const Grid2D& gc, const T* yc, T my, int n) {
+    // Branch-free (lanes of a wave alternate parity): the two coarse columns / rows coincide for
+    // even fine indices, where 0.5 a + 0.5 a = a exactly.
     const int j = n / gf.nnx, i = n - j * gf.nnx;
-    auto H = [&](int I, int J) -> T {
-        if (gc.periodic) {
-            if (I >= gc.nny) I -= gc.nny;
-            if (J >= gc.nnx) J -= gc.nnx;
-        }
-        return yc[I * gc.nnx + J] - my;
-    };
-    const bool io = i & 1, jo = j & 1;
-    if (!io && !jo) return H(j >> 1, i >> 1);
-    if (io && !jo) return T(0.5) * H(j >> 1, i >> 1) + T(0.5) * H(j >> 1, (i >> 1) + 1);
-    if (!io && jo) return T(0.5) * H(j >> 1, i >> 1) + T(0.5) * H((j >> 1) + 1, i >> 1);
-    const int jb = j >> 1, il = i >> 1;
-    return ((T(0.25) * H(jb, il) + T(0.25) * H(jb, il + 1)) + T(0.25) * H(jb + 1, il)) + T(0.25) * H(jb + 1, il + 1);
+    const int i0 = i >> 1, j0 = j >> 1;
+    int i1 = (i + 1) >> 1, j1 = (j + 1) >> 1;
+    if (gc.periodic) {
+        i1 = i1 >= gc.nnx ? i1 - gc.nnx : i1;
+        j1 = j1 >= gc.nny ? j1 - gc.nny : j1;
+    }
+    const T a = yc[j0 * gc.nnx + i0] - my, b = yc[j0 * gc.nnx + i1] - my;
+    const T c = yc[j1 * gc.nnx + i0] - my, d = yc[j1 * gc.nnx + i1] - my;
+    const T r0 = T(0.5) * a + T(0.5) * b;
+    const T r1 = T(0.5) * c + T(0.5) * d;
+    return T(0.5) * r0 + T(0.5) * r1;
 }
 
 template <class T>
@@ -158,7 +160,10 @@ struct Jacobi {
             const T oup = l == 0 ? T(0) : (T)st->mg[l][MG_R];
             rh = (T)r[n] - oup;
         } else {
-            load3x3(s, [&](int k) { return (pin && pin[k]) ? T(0) : x[k] + interp_at(gf, gc, yc, T(0), k); }, X);
+            if (pin)
+                load3x3(s, [&](int k) { const T v = x[k] + interp_at(gf, gc, yc, T(0), k); return pin[k] ? T(0) : v; }, X);
+            else
+                load3x3(s, [&](int k) { return x[k] + interp_at(gf, gc, yc, T(0), k); }, X);
             rh = (T)r[n];
         }
         const T off = ebe_const(s, Leoff, X);
@@ -262,6 +267,82 @@ __global__ __launch_bounds__(1024) void k_coarse_cg_lds(Grid2D g, const ElemCons
     if (threadIdx.x == 0) st->coarse_k = k;
 }
 
+// Pinned, coarse grid of at most 64 * NPT nodes: one wavefront, no block barriers.  The search
+// direction lives in LDS, x / r / Ap and each owned node's stencil ids in registers; the dot
+// products are DPP wavefront sums.
+template <class T, class TR, int NPT>
+__global__ __launch_bounds__(64) void k_coarse_cg_wave(Grid2D g, const ElemConst* __restrict__ ec,
+                                                       const uint8_t* pin, const TR* r, T* x_out, DevState* st,
+                                                       double tol, int maxit, const int* active) {
+    if (active && *active == 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T* p = reinterpret_cast<T*>(smem_raw);
+    const int nn = g.nn, lane = threadIdx.x;
+    double Le[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) Le[k] = ec->Le[k];
+    T xr[NPT], rv[NPT], ap[NPT];
+    NodeNbr s[NPT];
+    bool own[NPT], pn[NPT];
+    double g2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int n = lane + 64 * i;
+        own[i] = n < nn;
+        node_nbr(g, own[i] ? n : 0, s[i]);
+        pn[i] = own[i] && pin && pin[own[i] ? n : 0];
+        xr[i] = T(0); ap[i] = T(0);
+        const T b = own[i] ? (pn[i] ? T(0) : (T)r[n]) : T(0);
+        rv[i] = b;
+        if (own[i]) p[n] = b;
+        g2 += (double)b * (double)b;
+    }
+    __syncthreads();
+    double gamma = wave_sum_dpp(g2);
+    const double atol2 = fmax(tol * tol * gamma, 0.0);
+    int k = 0;
+    while (gamma > atol2 && k < maxit) {
+        double pap = 0.0;
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            T X[3][3];
+            load3x3(s[i], [&](int m) { return p[m]; }, X);
+            T a = ebe_const(s[i], Le, X);
+            a = pn[i] ? T(0) : a;
+            ap[i] = a;
+            pap += own[i] ? (double)X[1][1] * (double)a : 0.0;
+        }
+        pap = wave_sum_dpp(pap);
+        const double alpha = gamma / pap;
+        double rr = 0.0;
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int n = lane + 64 * i;
+            const T pv = own[i] ? p[n] : T(0);
+            xr[i] = xr[i] + (T)alpha * pv;
+            rv[i] = rv[i] - (T)alpha * ap[i];
+            rr += own[i] ? (double)rv[i] * (double)rv[i] : 0.0;
+        }
+        rr = wave_sum_dpp(rr);
+        const double beta = rr / gamma;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NPT; ++i) {
+            const int n = lane + 64 * i;
+            if (own[i]) p[n] = rv[i] + (T)beta * p[n];
+        }
+        __syncthreads();
+        gamma = rr;
+        ++k;
+    }
+#pragma unroll
+    for (int i = 0; i < NPT; ++i) {
+        const int n = lane + 64 * i;
+        if (own[i]) x_out[n] = pn[i] ? T(0) : xr[i];
+    }
+    if (lane == 0) st->coarse_k = k;
+}
+
 // Pinned, larger coarse grids: same recurrence on global (L2-resident) vectors.
 template <class T, class TR>
 __global__ __launch_bounds__(1024) void k_coarse_cg_global(Grid2D g, const ElemConst* __restrict__ ec,
@@ -320,6 +401,21 @@ static void coarse_solve(Ctx& c, const TR* r, T* y, T* cr, T* cp, T* cAp, double
     if (!c.set_zeroP) {
         const T* Ld = std::is_same<T, float>::value ? (const T*)C.Ld32 : (const T*)C.Ld;
         k_coarse_jacobi<T, TR><<<1, 1024, 0, c.stream>>>(nn, Ld, r, y, c.st, l, active);
+    } else if (nn <= 64 * 16) {
+        const size_t shm = (size_t)nn * sizeof(T);
+        const int w = (nn + 63) / 64;
+        if (w <= 1)
+            k_coarse_cg_wave<T, TR, 1><<<1, 64, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
+        else if (w <= 2)
+            k_coarse_cg_wave<T, TR, 2><<<1, 64, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
+        else if (w <= 4)
+            k_coarse_cg_wave<T, TR, 4><<<1, 64, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
+        else if (w <= 6)
+            k_coarse_cg_wave<T, TR, 6><<<1, 64, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
+        else if (w <= 8)
+            k_coarse_cg_wave<T, TR, 8><<<1, 64, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
+        else
+            k_coarse_cg_wave<T, TR, 16><<<1, 64, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
     } else if (nn <= 4096) {
         const int npt = nn <= 1024 ? 1 : (nn <= 2048 ? 2 : 4);
         int bs = (nn + npt - 1) / npt;
@@ -437,7 +533,10 @@ struct CgInit {
         NodeNbr s;
         node_nbr(g, n, s);
         double X[3][3];
-        load3x3(s, [&](int k) { return (zero_pins && pin && pin[k]) ? 0.0 : x0in[k]; }, X);
+        if (zero_pins && pin)
+            load3x3(s, [&](int k) { const double v = x0in[k]; return pin[k] ? 0.0 : v; }, X);
+        else
+            load3x3(s, [&](int k) { return x0in[k]; }, X);
         const bool pn = pin && pin[n];
         const double ax = pn ? 0.0 : ebe_const(s, Le, X);
         const double bn = b[n] - (bmean ? *bmean : 0.0);

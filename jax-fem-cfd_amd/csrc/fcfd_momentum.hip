@@ -39,8 +39,13 @@ struct MomInit {
         if (mu_) bu = ud[n];
         if (mv_) bv = ud[n + nn];
         double X0u[3][3], X0v[3][3];
-        load3x3(s, [&](int k) { return (mask && mask[k]) ? ud[k] : Un[k]; }, X0u);
-        load3x3(s, [&](int k) { return (mask && mask[k + nn]) ? ud[k + nn] : Un[k + nn]; }, X0v);
+        if (mask) {
+            load3x3(s, [&](int k) { const double a = ud[k], b = Un[k]; return mask[k] ? a : b; }, X0u);
+            load3x3(s, [&](int k) { const double a = ud[k + nn], b = Un[k + nn]; return mask[k + nn] ? a : b; }, X0v);
+        } else {
+            load3x3(s, [&](int k) { return Un[k]; }, X0u);
+            load3x3(s, [&](int k) { return Un[k + nn]; }, X0v);
+        }
         double au, av;
         lhs.apply(s, X0u, X0v, au, av);
         if (mu_) au = X0u[1][1];

@@ -189,7 +189,6 @@ __global__ __launch_bounds__(BLOCK) void k_stored_general(Grid2D g, ConstMat K0,
             double acc = 0.0;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                if (!s.has[k]) continue;
                 const int a = kRow(k);
                 const size_t eo = (size_t)s.e[k] * 16 + a * 4;
                 double t = 0.0;
@@ -200,7 +199,7 @@ __global__ __launch_bounds__(BLOCK) void k_stored_general(Grid2D g, ConstMat K0,
                     if (A2) coef += c2 * A2[eo + b];
                     t += coef * X[kCy(k, b)][kCx(k, b)];
                 }
-                acc += t;
+                acc += s.has[k] ? t : 0.0;
             }
             y[(size_t)comp * g.nn + n] = acc;
         }

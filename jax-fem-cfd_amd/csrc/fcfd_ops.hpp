@@ -11,12 +11,17 @@
 
 namespace fcfd {
 
+// 3x3 gather: f is evaluated at every (clamped, valid) neighbour id, so all loads issue
+// back to back; missing neighbours are zeroed with a select afterwards.
 template <class T, class F>
 __device__ __forceinline__ void load3x3(const NodeNbr& s, F f, T (&X)[3][3]) {
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) X[a][b] = s.nb[a][b] >= 0 ? f(s.nb[a][b]) : T(0);
+        for (int b = 0; b < 3; ++b) {
+            const T v = f(s.nb[a][b]);
+            X[a][b] = s.ok[a][b] ? v : T(0);
+        }
 }
 
 // y(node) = sum_k sum_b A[row_k][b] X[corner_k(b)], one shared element matrix
@@ -25,12 +30,11 @@ __device__ __forceinline__ T ebe_const(const NodeNbr& s, const double* __restric
     T acc = T(0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (!s.has[k]) continue;
         const int a = kRow(k);
         T t = T(0);
 #pragma unroll
         for (int b = 0; b < 4; ++b) t += T(A[a * 4 + b]) * X[kCy(k, b)][kCx(k, b)];
-        acc += t;
+        acc += s.has[k] ? t : T(0);
     }
     return acc;
 }
@@ -41,12 +45,11 @@ __device__ __forceinline__ T ebe_const_T(const NodeNbr& s, const double* __restr
     T acc = T(0);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (!s.has[k]) continue;
         const int a = kRow(k);
         T t = T(0);
 #pragma unroll
         for (int b = 0; b < 4; ++b) t += T(A[b * 4 + a]) * X[kCy(k, b)][kCx(k, b)];
-        acc += t;
+        acc += s.has[k] ? t : T(0);
     }
     return acc;
 }
@@ -67,7 +70,6 @@ struct MomOp {
         yu = 0.0; yv = 0.0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (!s.has[k]) continue;
             const int a = kRow(k);
             const int e = s.e[k];
             double tu = 0.0, tv = 0.0;
@@ -77,7 +79,8 @@ struct MomOp {
                 tu += coef * Xu[kCy(k, b)][kCx(k, b)];
                 tv += coef * Xv[kCy(k, b)][kCx(k, b)];
             }
-            yu += tu; yv += tv;
+            yu += s.has[k] ? tu : 0.0;
+            yv += s.has[k] ? tv : 0.0;
         }
     }
 };
