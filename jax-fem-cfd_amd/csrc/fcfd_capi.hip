@@ -2,9 +2,25 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include "fcfd_internal.hpp"
 
 using namespace fcfd;
+
+namespace fcfd {
+void enter(Ctx& c) {
+    FCFD_HIP(hipEventRecord(c.ev_in, c.user_stream));
+    FCFD_HIP(hipStreamWaitEvent(c.stream, c.ev_in, 0));
+}
+void leave(Ctx& c) {
+    FCFD_HIP(hipEventRecord(c.ev_out, c.stream));
+    FCFD_HIP(hipStreamWaitEvent(c.user_stream, c.ev_out, 0));
+}
+void graph_reset(IterGraph& g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+}
+}  // namespace fcfd
 
 struct fcfd_ctx {
     Ctx c;
@@ -16,6 +32,24 @@ template <class F>
 static int guard(F&& f) {
     try {
         f();
+        return FCFD_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return FCFD_ERR_ARG;
+    }
+}
+
+// guard + stream join: the call's work runs on the context's private stream, ordered after
+// everything already queued on the caller's stream, and the caller's stream waits for it.
+template <class F>
+static int run(fcfd_ctx* ctx, F&& f) {
+    try {
+        enter(ctx->c);
+        f();
+        leave(ctx->c);
         return FCFD_OK;
     } catch (const Error& e) {
         g_last_error = e.what();
@@ -55,7 +89,14 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     FCFD_REQUIRE(m && s, "null descriptor");
     FCFD_REQUIRE(m->ndim == 2, "only 2D meshes are supported by this build");
     FCFD_REQUIRE(m->nex >= 1 && m->ney >= 1, "bad element counts");
-    c.stream = (hipStream_t)stream;
+    c.user_stream = (hipStream_t)stream;
+    FCFD_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    FCFD_HIP(hipEventCreateWithFlags(&c.ev_in, hipEventDisableTiming));
+    FCFD_HIP(hipEventCreateWithFlags(&c.ev_out, hipEventDisableTiming));
+    {
+        const char* e = std::getenv("FCFD_NO_GRAPHS");
+        c.use_graphs = !(e && e[0] == '1');
+    }
     c.g = make_grid(m->nex, m->ney, m->periodic);
     c.lx = m->lx; c.ly = m->ly;
     c.tol = s->tol;
@@ -270,7 +311,14 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
 
 void destroy_ctx(Ctx& c) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
-    else (void)hipDeviceSynchronize();
+    (void)hipDeviceSynchronize();
+    graph_reset(c.g_mom);
+    graph_reset(c.g_cg);
+    if (c.ev_in) (void)hipEventDestroy(c.ev_in);
+    if (c.ev_out) (void)hipEventDestroy(c.ev_out);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.ev_in = c.ev_out = nullptr;
+    c.stream = nullptr;
     for (void* p : c.allocs) (void)hipFree(p);
     c.allocs.clear();
     if (c.h_pinned) (void)hipHostFree(c.h_pinned);
@@ -314,7 +362,7 @@ int fcfd_ctx_destroy(fcfd_ctx* ctx) {
 
 int fcfd_set_stream(fcfd_ctx* ctx, void* stream) {
     if (!ctx) return FCFD_ERR_ARG;
-    ctx->c.stream = (hipStream_t)stream;
+    ctx->c.user_stream = (hipStream_t)stream;
     return FCFD_OK;
 }
 
@@ -337,7 +385,7 @@ int fcfd_level_nodes(const fcfd_ctx* ctx, int32_t level, int64_t* nn, int32_t* n
 
 int fcfd_mesh_fill(fcfd_ctx* ctx, double* coords, int64_t* nconn) {
     if (!ctx) return FCFD_ERR_ARG;
-    return guard([&] { mesh_fill(ctx->c, coords, nconn); });
+    return run(ctx, [&] { mesh_fill(ctx->c, coords, nconn); });
 }
 
 int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16) {
@@ -355,7 +403,7 @@ int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16) {
 
 int fcfd_lumped_mass(fcfd_ctx* ctx, double* M) {
     if (!ctx || !M) return FCFD_ERR_ARG;
-    return guard([&] {
+    return run(ctx, [&] {
         FCFD_HIP(hipMemcpyAsync(M, ctx->c.Mlump, sizeof(double) * 2 * ctx->c.g.nn, hipMemcpyDeviceToDevice,
                                 ctx->c.stream));
     });
@@ -363,7 +411,7 @@ int fcfd_lumped_mass(fcfd_ctx* ctx, double* M) {
 
 int fcfd_laplacian_diag(fcfd_ctx* ctx, int32_t level, double* out) {
     if (!ctx || !out || level < 0 || level >= ctx->c.nlev) return FCFD_ERR_ARG;
-    return guard([&] {
+    return run(ctx, [&] {
         const Level& L = ctx->c.lv[level];
         FCFD_HIP(hipMemcpyAsync(out, L.Ld, sizeof(double) * L.g.nn, hipMemcpyDeviceToDevice, ctx->c.stream));
     });
@@ -371,37 +419,37 @@ int fcfd_laplacian_diag(fcfd_ctx* ctx, int32_t level, double* out) {
 
 int fcfd_conv_elem_calc(fcfd_ctx* ctx, const double* U, double beta, double* Ce) {
     if (!ctx || !U || !Ce) return FCFD_ERR_ARG;
-    return guard([&] {
+    return run(ctx, [&] {
         build_ce_api(ctx->c, U, beta, Ce);
     });
 }
 
 int fcfd_boundary_integral(fcfd_ctx* ctx, const double* U_d, double* F_p) {
     if (!ctx || !F_p) return FCFD_ERR_ARG;
-    return guard([&] { boundary_integral(ctx->c, U_d, F_p); });
+    return run(ctx, [&] { boundary_integral(ctx->c, U_d, F_p); });
 }
 
 int fcfd_elem_mult_const(fcfd_ctx* ctx, int32_t level, const double* A16, const double* x, double* y,
                          int32_t accumulate) {
     if (!ctx || !A16 || !x || !y) return FCFD_ERR_ARG;
-    return guard([&] { elem_mult_const(ctx->c, level, A16, x, y, accumulate); });
+    return run(ctx, [&] { elem_mult_const(ctx->c, level, A16, x, y, accumulate); });
 }
 
 int fcfd_elem_mult_stored(fcfd_ctx* ctx, const double* K0_16, const double* A1, double c1, const double* A2,
                           double c2, int32_t ncomp, const double* x, double* y) {
     if (!ctx || !x || !y || ncomp < 1 || ncomp > 3) return FCFD_ERR_ARG;
-    return guard([&] { elem_mult_general(ctx->c, K0_16, A1, c1, A2, c2, ncomp, x, y); });
+    return run(ctx, [&] { elem_mult_general(ctx->c, K0_16, A1, c1, A2, c2, ncomp, x, y); });
 }
 
 int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z) {
     if (!ctx || !r || !z) return FCFD_ERR_ARG;
-    return guard([&] { vcycle(ctx->c, r, z); });
+    return run(ctx, [&] { vcycle(ctx->c, r, z); });
 }
 
 int fcfd_pressure_cg(fcfd_ctx* ctx, const double* b, double* x, double tol, int32_t maxiter, int32_t* iters,
                      int32_t* converged) {
     if (!ctx || !b || !x) return FCFD_ERR_ARG;
-    return guard([&] {
+    return run(ctx, [&] {
         bool conv = false;
         int it = pressure_cg(ctx->c, b, x, tol, maxiter > 0 ? maxiter : 1000, &conv);
         FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
@@ -413,7 +461,7 @@ int fcfd_pressure_cg(fcfd_ctx* ctx, const double* b, double* x, double tol, int3
 int fcfd_solve_pressure(fcfd_ctx* ctx, const double* U_star, const double* P_n, const double* F_p, double rho,
                         double dt, double* P, int32_t* iters) {
     if (!ctx || !U_star || !P_n || !P) return FCFD_ERR_ARG;
-    return guard([&] {
+    return run(ctx, [&] {
         int it = solve_pressure(ctx->c, U_star, P_n, F_p, rho, dt, P);
         FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
         if (iters) *iters = it;
@@ -423,7 +471,7 @@ int fcfd_solve_pressure(fcfd_ctx* ctx, const double* U_star, const double* P_n, 
 int fcfd_solve_momentum(fcfd_ctx* ctx, const double* U_n, const double* P_n, const double* Ce, double rho, double mu,
                         double dt, const double* U_d, const double* F, double* U_hat, int32_t* iters) {
     if (!ctx || !U_n || !P_n || !U_hat) return FCFD_ERR_ARG;
-    return guard([&] {
+    return run(ctx, [&] {
         Ctx& c = ctx->c;
         scatter_ud(c, U_d);
         int it;
@@ -444,7 +492,7 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
                             const double* U_d, const double* F_p, const double* F, double* U, double* P,
                             int32_t* iters) {
     if (!ctx || !U_n || !P_n || !U || !P) return FCFD_ERR_ARG;
-    return guard([&] {
+    return run(ctx, [&] {
         Ctx& c = ctx->c;
         scatter_ud(c, U_d);
         build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
@@ -461,17 +509,17 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
 
 int fcfd_dot(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out) {
     if (!ctx || !x || !y || !out || n < 0) return FCFD_ERR_ARG;
-    return guard([&] { *out = dot(ctx->c, x, y, (int)n); });
+    return run(ctx, [&] { *out = dot(ctx->c, x, y, (int)n); });
 }
 
 int fcfd_axpby(fcfd_ctx* ctx, double a, const double* x, double b, double* y, int64_t n) {
     if (!ctx || !x || !y || n < 0) return FCFD_ERR_ARG;
-    return guard([&] { axpby(ctx->c, a, x, b, y, (int)n); });
+    return run(ctx, [&] { axpby(ctx->c, a, x, b, y, (int)n); });
 }
 
 int fcfd_bench_momentum_apply(fcfd_ctx* ctx, int32_t reps, float* ms_per_apply) {
     if (!ctx || reps <= 0 || !ms_per_apply) return FCFD_ERR_ARG;
-    return guard([&] { *ms_per_apply = bench_momentum_apply(ctx->c, reps); });
+    return run(ctx, [&] { *ms_per_apply = bench_momentum_apply(ctx->c, reps); });
 }
 
 }  // extern "C"

@@ -267,34 +267,49 @@ __global__ __launch_bounds__(1024) void k_coarse_cg_lds(Grid2D g, const ElemCons
     if (threadIdx.x == 0) st->coarse_k = k;
 }
 
-// Pinned, coarse grid of at most 64 * NPT nodes: one wavefront, no block barriers.  The search
-// direction lives in LDS, x / r / Ap and each owned node's stencil ids in registers; the dot
-// products are DPP wavefront sums.
+// Pinned, coarse grid of at most 64 * NPT nodes: one wavefront, no block barriers and no
+// branches in the iteration.  Each owned node keeps its assembled 9-point row of L (boundary,
+// missing-element and pinned-row masks folded into the weights) and its clamped neighbour ids in
+// registers; the search direction lives in LDS (idle lanes store to a scratch slot past the end);
+// the dot products are DPP wavefront sums.
 template <class T, class TR, int NPT>
 __global__ __launch_bounds__(64) void k_coarse_cg_wave(Grid2D g, const ElemConst* __restrict__ ec,
                                                        const uint8_t* pin, const TR* r, T* x_out, DevState* st,
                                                        double tol, int maxit, const int* active) {
     if (active && *active == 0) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    T* p = reinterpret_cast<T*>(smem_raw);
+    T* p = reinterpret_cast<T*>(smem_raw);  // nn + 64 entries
     const int nn = g.nn, lane = threadIdx.x;
-    double Le[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) Le[k] = ec->Le[k];
-    T xr[NPT], rv[NPT], ap[NPT];
-    NodeNbr s[NPT];
-    bool own[NPT], pn[NPT];
+    T w[NPT][9], xr[NPT], rv[NPT], m[NPT];
+    int id[NPT][9], slot[NPT];
     double g2 = 0.0;
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
         const int n = lane + 64 * i;
-        own[i] = n < nn;
-        node_nbr(g, own[i] ? n : 0, s[i]);
-        pn[i] = own[i] && pin && pin[own[i] ? n : 0];
-        xr[i] = T(0); ap[i] = T(0);
-        const T b = own[i] ? (pn[i] ? T(0) : (T)r[n]) : T(0);
+        const bool own = n < nn;
+        const int nc = own ? n : 0;
+        NodeNbr s;
+        node_nbr(g, nc, s);
+        const bool pinned = pin && pin[nc];
+        double wd[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) wd[q] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (s.has[k]) wd[kCy(k, b) * 3 + kCx(k, b)] += ec->Le[kRow(k) * 4 + b];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            w[i][q] = (own && !pinned) ? (T)wd[q] : T(0);
+            id[i][q] = s.nb[q / 3][q % 3];
+        }
+        slot[i] = own ? n : nn + lane;
+        m[i] = own ? T(1) : T(0);
+        xr[i] = T(0);
+        const T b = (own && !pinned) ? (T)r[nc] : T(0);
         rv[i] = b;
-        if (own[i]) p[n] = b;
+        p[slot[i]] = b;
         g2 += (double)b * (double)b;
     }
     __syncthreads();
@@ -302,35 +317,31 @@ __global__ __launch_bounds__(64) void k_coarse_cg_wave(Grid2D g, const ElemConst
     const double atol2 = fmax(tol * tol * gamma, 0.0);
     int k = 0;
     while (gamma > atol2 && k < maxit) {
+        T ap[NPT], pv[NPT];
         double pap = 0.0;
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
-            T X[3][3];
-            load3x3(s[i], [&](int m) { return p[m]; }, X);
-            T a = ebe_const(s[i], Le, X);
-            a = pn[i] ? T(0) : a;
+            T a = T(0);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) a += w[i][q] * p[id[i][q]];
             ap[i] = a;
-            pap += own[i] ? (double)X[1][1] * (double)a : 0.0;
+            pv[i] = p[slot[i]];
+            pap += (double)m[i] * (double)pv[i] * (double)a;
         }
         pap = wave_sum_dpp(pap);
         const double alpha = gamma / pap;
         double rr = 0.0;
 #pragma unroll
         for (int i = 0; i < NPT; ++i) {
-            const int n = lane + 64 * i;
-            const T pv = own[i] ? p[n] : T(0);
-            xr[i] = xr[i] + (T)alpha * pv;
+            xr[i] = xr[i] + (T)alpha * pv[i];
             rv[i] = rv[i] - (T)alpha * ap[i];
-            rr += own[i] ? (double)rv[i] * (double)rv[i] : 0.0;
+            rr += (double)m[i] * (double)rv[i] * (double)rv[i];
         }
         rr = wave_sum_dpp(rr);
         const double beta = rr / gamma;
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < NPT; ++i) {
-            const int n = lane + 64 * i;
-            if (own[i]) p[n] = rv[i] + (T)beta * p[n];
-        }
+        for (int i = 0; i < NPT; ++i) p[slot[i]] = rv[i] + (T)beta * pv[i];
         __syncthreads();
         gamma = rr;
         ++k;
@@ -338,7 +349,7 @@ __global__ __launch_bounds__(64) void k_coarse_cg_wave(Grid2D g, const ElemConst
 #pragma unroll
     for (int i = 0; i < NPT; ++i) {
         const int n = lane + 64 * i;
-        if (own[i]) x_out[n] = pn[i] ? T(0) : xr[i];
+        if (n < nn) x_out[n] = (pin && pin[n]) ? T(0) : xr[i];
     }
     if (lane == 0) st->coarse_k = k;
 }
@@ -402,7 +413,7 @@ static void coarse_solve(Ctx& c, const TR* r, T* y, T* cr, T* cp, T* cAp, double
         const T* Ld = std::is_same<T, float>::value ? (const T*)C.Ld32 : (const T*)C.Ld;
         k_coarse_jacobi<T, TR><<<1, 1024, 0, c.stream>>>(nn, Ld, r, y, c.st, l, active);
     } else if (nn <= 64 * 16) {
-        const size_t shm = (size_t)nn * sizeof(T);
+        const size_t shm = (size_t)(nn + 64) * sizeof(T);
         const int w = (nn + 63) / 64;
         if (w <= 1)
             k_coarse_cg_wave<T, TR, 1><<<1, 64, shm, c.stream>>>(C.g, C.ec_dev, C.pin, r, y, c.st, tol, 1000, active);
@@ -692,31 +703,34 @@ static int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in,
     else
         launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, (const double*)y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st});
     FCFD_CHECK_LAUNCH();
-    int launched = 0;
-    int chunk = c.pred_p > 0 ? c.pred_p : 1;
-    int h[2] = {0, 0};
-    // an already converged start needs no iteration
-    while (true) {
-        for (int t = 0; t < chunk && launched < maxit; ++t, ++launched) {
-            const int o = launched & 1, nw = o ^ 1;
-            launch_node<1>(c, nn, act,
-                                                       CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                                                       CgK1Fin{c.st});
-            launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st},
-                                                       CgK2Fin{c.st, nn});
+    // one chunk = a pair of PCG iterations (parity of the p / z double buffers returns to 0)
+    auto pair = [&]() {
+        for (int o = 0; o < 2; ++o) {
+            const int nw = o ^ 1;
+            launch_node<1>(c, nn, act, CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st}, CgK1Fin{c.st});
+            launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
             bool f;
             const void* y = precondition(c, act, f);
             if (f)
                 launch_node<2>(c, nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
-                    CgK3Fin{c.st, flex});
+                               CgK3Fin{c.st, flex});
             else
-                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
-                    CgK3Fin{c.st, flex});
+                launch_node<2>(c, nn, act,
+                               CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                               CgK3Fin{c.st, flex});
         }
         FCFD_CHECK_LAUNCH();
+    };
+    const double key[4] = {0.0, 0.0, 0.0, 0.0};
+    int launched = 0;
+    int pairs = c.pred_p > 0 ? (c.pred_p + 1) / 2 : 1;
+    int h[2] = {0, 0};
+    while (true) {
+        replay(c, c.g_cg, key, nullptr, pairs, pair);
+        launched += 2 * pairs;
         read_state_ints(c, &c.st->c_k, 2, h);  // c_k, c_active
         if (!h[1] || launched >= maxit) break;
-        chunk = 2;
+        pairs = 1;
     }
     if (converged) *converged = !h[1] && h[0] <= maxit;
     if (h[0] > 0) c.pred_p = h[0];

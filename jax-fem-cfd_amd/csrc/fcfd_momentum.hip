@@ -188,26 +188,37 @@ static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, 
     MomOp<S> rhs{g, c.ec_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
     const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 2 * g.nn;
     launch_node<2>(c, g.nn, nullptr,
-        MomInit<S>{lhs, rhs, U_n, P_n, F, c.ud_full, c.dmask, U_hat, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
-        MomInitFin{c.st, c.tol, maxit});
+                   MomInit<S>{lhs, rhs, U_n, P_n, F, c.ud_full, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                   MomInitFin{c.st, c.tol, maxit});
     FCFD_CHECK_LAUNCH();
     const int* act = &c.st->b_active;
-    int launched = 0;
-    int chunk = c.pred_mom > 0 ? c.pred_mom : 1;
-    int h[2] = {0, 0};
-    while (true) {
-        for (int t = 0; t < chunk && launched < maxit; ++t, ++launched) {
-            const int o = launched & 1, nw = o ^ 1;
-            launch_node<1>(c, g.nn, act, MomK1<S>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
-                MomK1Fin{c.st});
-            launch_node<3>(c, g.nn, act, MomK2<S>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st}, MomK2Fin{c.st});
-            launch_node<2>(c, g.nn, act, MomK3{g.nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, U_hat, c.m_r, c.st}, MomK3Fin{c.st});
+    // one chunk = a pair of iterations, so the double-buffer parity returns to 0
+    auto pair = [&]() {
+        for (int o = 0; o < 2; ++o) {
+            const int nw = o ^ 1;
+            launch_node<1>(c, g.nn, act,
+                           MomK1<S>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
+                           MomK1Fin{c.st});
+            launch_node<3>(c, g.nn, act, MomK2<S>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                           MomK2Fin{c.st});
+            launch_node<2>(c, g.nn, act, MomK3{g.nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
+                           MomK3Fin{c.st});
         }
         FCFD_CHECK_LAUNCH();
+    };
+    const double key[4] = {rho, mu, dt, 0.0};
+    int launched = 0;
+    int pairs = c.pred_mom > 0 ? (c.pred_mom + 1) / 2 : 1;
+    int h[2] = {0, 0};
+    while (true) {
+        replay(c, c.g_mom, key, ce.c, pairs, pair);
+        launched += 2 * pairs;
         read_state_ints(c, &c.st->b_k, 2, h);  // b_k, b_active
         if (!h[1] || launched >= maxit) break;
-        chunk = 2;
+        pairs = 1;
     }
+    if (U_hat != c.m_x)
+        FCFD_HIP(hipMemcpyAsync(U_hat, c.m_x, sizeof(double) * 2 * g.nn, hipMemcpyDeviceToDevice, c.stream));
     if (h[0] > 0) c.pred_mom = h[0];
     return h[0];
 }
