@@ -78,6 +78,8 @@ typedef struct fcfd_solver_desc {
     const int64_t* bconn;       /* (n_bedge, 2) Dirichlet boundary edges         */
     const int64_t* nvec;        /* (n_bedge, 2) outward normals                  */
     int64_t n_bedge;
+    const int64_t* gamma_d_phi; /* passive-scalar Dirichlet nodes or NULL         */
+    int64_t n_gamma_d_phi;
 } fcfd_solver_desc;
 
 /* ---- context --------------------------------------------------------------------------- */
@@ -174,6 +176,27 @@ int fcfd_solve_momentum(fcfd_ctx* ctx, const double* U_n, const double* P_n, con
 int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, double dt,
                             double rho, double mu, const double* U_d, const double* F_p,
                             const double* F, double* U, double* P, int32_t* iters);
+
+/* ---- passive scalar transport (jax_fem_cfd/solvers/passive_transport.py) ------------ */
+
+/* update_phi (:39-61) or, supg = 1, update_phi_SUPG (:14-37): Crank-Nicolson
+ * advection-diffusion of phi from phi_n with Ce(U_n), Ce(U) (beta = 1), diffusion D,
+ * Dirichlet values phi_d on the context's gamma_d_phi (device, may be NULL), by BiCGSTAB. */
+int fcfd_solve_scalar(fcfd_ctx* ctx, const double* U_n, const double* U, const double* phi_n,
+                      double dt, double D, const double* phi_d, int32_t supg, double* phi,
+                      int32_t* iters);
+
+/* passive_transport.timestep (:64-97): four_step_timestep then the scalar update.
+ * iters: host int32[3] = (U_iters, P_iters, phi_iters). */
+int fcfd_passive_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, const double* phi_n,
+                          double dt, double rho, double mu, double D, const double* U_d,
+                          const double* phi_d, const double* F_p, const double* F, int32_t supg,
+                          double* U, double* P, double* phi, int32_t* iters);
+
+/* Ctau_tilde_elem_calc (element_calc.py:188-200) of U_n into Ct (ne,4,4) and, when Ce
+ * (ne,4,4) is given, calculate_tau (:202-229) into tau (ne).  Either output may be NULL. */
+int fcfd_supg_elem_calc(fcfd_ctx* ctx, const double* U_n, const double* Ce, double D, double dt,
+                        double r, double* Ct, double* tau);
 
 /* ---- BLAS-1 used by the generic cg / bicgstab wrappers ----------------------------- */
 

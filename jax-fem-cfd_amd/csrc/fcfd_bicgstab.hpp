@@ -1,0 +1,285 @@
+// Generic device-resident BiCGSTAB (jax_fem_cfd/utils/jax_iterative_solvers.py:123-167) over an
+// element-by-element operator with NC components sharing one element matrix per element, and
+// Dirichlet rows replaced by the identity (standard.py:20-28, passive_transport.py:17-24).
+//
+// One iteration = 3 kernels (+ their 1-block finalizes):
+//   K1  p = r + beta (p - omega q) (formed on the fly at each stencil node, double-buffered),
+//       q = A p,  alpha = rho' / (rhat . q)
+//   K2  s = r - alpha q (on the fly), t = A s,  exit_early, omega = (t.s)/(t.t)
+//   K3  x, r updates,  breakdown codes -10 / -11, next rho', beta, convergence flag
+// Operators provide  double coef(int a, int b, int e)  (row a, column b of element e's matrix).
+#pragma once
+#include "fcfd_internal.hpp"
+#include "fcfd_ops.hpp"
+
+namespace fcfd {
+
+// (cM Me + cL Le + cC Ce_e): the momentum operator and the momentum rhs operator
+template <class S>
+struct MomCoef {
+    Grid2D g;
+    const ElemConst* __restrict__ ec;
+    S ce;
+    double cM, cL, cC;
+    __device__ __forceinline__ double coef(int a, int b, int e) const {
+        return cM * ec->Me[a * 4 + b] + cL * ec->Le[a * 4 + b] + cC * ce.ld(a * 4 + b, e);
+    }
+};
+
+// per-element stored matrices (entry-major): the scalar-transport operators
+template <class S>
+struct StoredCoef {
+    Grid2D g;
+    S m;
+    __device__ __forceinline__ double coef(int a, int b, int e) const { return m.ld(a * 4 + b, e); }
+};
+
+template <int NC, class Op>
+__device__ __forceinline__ void op_apply(const Op& op, const NodeNbr& s, const double (&X)[NC][3][3],
+                                         double (&y)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) y[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int a = kRow(k);
+        const int e = s.e[k];
+        double t[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) t[c] = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const double cf = op.coef(a, b, e);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t[c] += cf * X[c][kCy(k, b)][kCx(k, b)];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) y[c] += s.has[k] ? t[c] : 0.0;
+    }
+}
+
+// No extra rhs term (scalar transport)
+struct NoExtra {
+    __device__ __forceinline__ void operator()(const NodeNbr&, int, double (&)[1]) const {}
+    __device__ __forceinline__ void operator()(const NodeNbr&, int, double (&)[2]) const {}
+};
+
+// ------------------------------------------------------------------------------ init
+// rhs = R x_n + extra, Dirichlet rows = ud; x0 = x_n with Dirichlet values; r0 = rhs - A x0;
+// rhat = p = q = r0.  Reductions: b.b, r0.r0
+template <int NC, class LOp, class ROp, class Extra>
+struct BInit {
+    LOp lhs;
+    ROp rhs;
+    Extra extra;
+    const double *xn, *ud;
+    const uint8_t* mask;
+    double *x, *r, *rhat, *p, *q;
+    __device__ void operator()(int n, double (&acc)[2]) const {
+        const int nn = lhs.g.nn;
+        NodeNbr s;
+        node_nbr(lhs.g, n, s);
+        double X[NC][3][3], X0[NC][3][3];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const double* xc = xn + c * nn;
+            load3x3(s, [&](int k) { return xc[k]; }, X[c]);
+            if (mask) {
+                const double* uc = ud + c * nn;
+                const uint8_t* mc = mask + c * nn;
+                load3x3(s, [&](int k) { const double a = uc[k], b = xc[k]; return mc[k] ? a : b; }, X0[c]);
+            } else {
+                load3x3(s, [&](int k) { return xc[k]; }, X0[c]);
+            }
+        }
+        double b[NC], ax[NC];
+        op_apply<NC>(rhs, s, X, b);
+        extra(s, n, b);
+        op_apply<NC>(lhs, s, X0, ax);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = n + c * nn;
+            const bool m = mask && mask[i];
+            const double bc = m ? ud[i] : b[c];
+            const double a = m ? X0[c][1][1] : ax[c];
+            const double rc = bc - a;
+            x[i] = X0[c][1][1];
+            r[i] = rc; rhat[i] = rc; p[i] = rc; q[i] = rc;
+            acc[0] += bc * bc;
+            acc[1] += rc * rc;
+        }
+    }
+};
+struct BInitFin {
+    DevState* st;
+    double tol;
+    int maxit;
+    __device__ void operator()(const double (&t)[2]) const {
+        const double a2 = tol * tol * t[0];
+        st->b_atol2 = a2 > 0.0 ? a2 : 0.0;
+        st->b_rho = 1.0; st->b_alpha = 1.0; st->b_omega = 1.0;
+        st->b_rho_new = t[1];
+        st->b_beta = t[1] / 1.0 * 1.0 / 1.0;
+        st->b_k = 0; st->b_ee = 0; st->b_maxit = maxit;
+        st->b_active = (t[1] > st->b_atol2) && (0 < maxit);
+    }
+};
+
+// ------------------------------------------------------------------------------ K1
+template <int NC, class LOp>
+struct BK1 {
+    LOp op;
+    const uint8_t* mask;
+    const double *r, *pold, *qold, *rhat;
+    double *pnew, *qnew;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const int nn = op.g.nn;
+        const double beta = st->b_beta, omega = st->b_omega;
+        NodeNbr s;
+        node_nbr(op.g, n, s);
+        double X[NC][3][3];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int o = c * nn;
+            load3x3(s, [&](int k) { return r[k + o] + beta * (pold[k + o] - omega * qold[k + o]); }, X[c]);
+        }
+        double y[NC];
+        op_apply<NC>(op, s, X, y);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = n + c * nn;
+            const double yc = (mask && mask[i]) ? X[c][1][1] : y[c];
+            pnew[i] = X[c][1][1];
+            qnew[i] = yc;
+            acc[0] += rhat[i] * yc;
+        }
+    }
+};
+struct BK1Fin {
+    DevState* st;
+    __device__ void operator()(const double (&t)[1]) const { st->b_alpha = st->b_rho_new / t[0]; }
+};
+
+// ------------------------------------------------------------------------------ K2
+// TAG only distinguishes otherwise identical instantiations (the standalone timing of the
+// operator apply runs under its own kernel symbol).
+template <int NC, class LOp, int TAG = 0>
+struct BK2 {
+    LOp op;
+    const uint8_t* mask;
+    const double *r, *q;
+    double *sv, *tv;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[3]) const {
+        const int nn = op.g.nn;
+        const double alpha = st->b_alpha;
+        NodeNbr s;
+        node_nbr(op.g, n, s);
+        double X[NC][3][3];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int o = c * nn;
+            load3x3(s, [&](int k) { return r[k + o] - alpha * q[k + o]; }, X[c]);
+        }
+        double y[NC];
+        op_apply<NC>(op, s, X, y);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = n + c * nn;
+            const double yc = (mask && mask[i]) ? X[c][1][1] : y[c];
+            const double sc = X[c][1][1];
+            sv[i] = sc;
+            tv[i] = yc;
+            acc[0] += sc * sc;
+            acc[1] += yc * sc;
+            acc[2] += yc * yc;
+        }
+    }
+};
+struct BK2Fin {
+    DevState* st;
+    __device__ void operator()(const double (&t)[3]) const {
+        st->b_ee = t[0] < st->b_atol2;
+        st->b_omega = t[1] / t[2];
+    }
+};
+
+// ------------------------------------------------------------------------------ K3
+template <int NC>
+struct BK3 {
+    int nn;
+    const double *p, *sv, *tv, *rhat;
+    double *x, *r;
+    const DevState* st;
+    __device__ void operator()(int k, double (&acc)[2]) const {
+        const double alpha = st->b_alpha, omega = st->b_omega;
+        const bool ee = st->b_ee;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = k + c * nn;
+            const double pi = p[i], si = sv[i], ti = tv[i];
+            double xi = x[i];
+            double ri;
+            if (ee) { xi = xi + alpha * pi; ri = si; }
+            else { xi = xi + (alpha * pi + omega * si); ri = si - omega * ti; }
+            x[i] = xi; r[i] = ri;
+            acc[0] += ri * ri;
+            acc[1] += rhat[i] * ri;
+        }
+    }
+};
+struct BK3Fin {
+    DevState* st;
+    __device__ void operator()(const double (&t)[2]) const {
+        const double alpha = st->b_alpha, omega = st->b_omega, rho_cur = st->b_rho_new;
+        int k = (omega == 0.0 || alpha == 0.0) ? -11 : st->b_k + 1;
+        if (rho_cur == 0.0) k = -10;
+        st->b_k = k;
+        st->b_rho = rho_cur;
+        st->b_rho_new = t[1];
+        st->b_beta = t[1] / rho_cur * alpha / omega;
+        st->b_active = (t[0] > st->b_atol2) && (k < st->b_maxit) && (k >= 0);
+    }
+};
+
+// ------------------------------------------------------------------------------ driver
+// Solves A x = R x_n + extra (Dirichlet rows: x = ud) from x0 = x_n with Dirichlet values, into
+// `x` (NC * nn, a context buffer: the captured iteration graph keeps its address).  Returns the
+// reference's iteration count (or -10 / -11).
+template <int NC, class LOp, class ROp, class Extra>
+int bicgstab_solve(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
+                   const ROp& rhs, const Extra& extra, const double* xn, const double* ud, const uint8_t* mask,
+                   int maxit, double* x, int& pred) {
+    const int nn = c.g.nn;
+    launch_node<2>(c, nn, nullptr,
+                   BInit<NC, LOp, ROp, Extra>{lhs, rhs, extra, xn, ud, mask, x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                   BInitFin{c.st, c.tol, maxit});
+    FCFD_CHECK_LAUNCH();
+    const int* act = &c.st->b_active;
+    // one chunk = a pair of iterations, so the double-buffer parity returns to 0
+    auto pair = [&]() {
+        for (int o = 0; o < 2; ++o) {
+            const int nw = o ^ 1;
+            launch_node<1>(c, nn, act,
+                           BK1<NC, LOp>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
+                           BK1Fin{c.st});
+            launch_node<3>(c, nn, act, BK2<NC, LOp>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st}, BK2Fin{c.st});
+            launch_node<2>(c, nn, act, BK3<NC>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, x, c.m_r, c.st}, BK3Fin{c.st});
+        }
+        FCFD_CHECK_LAUNCH();
+    };
+    int launched = 0;
+    int pairs = pred > 0 ? (pred + 1) / 2 : 1;
+    int h[2] = {0, 0};
+    while (true) {
+        replay(c, graph, key, kptr, pairs, pair);
+        launched += 2 * pairs;
+        read_state_ints(c, &c.st->b_k, 2, h);  // b_k, b_active
+        if (!h[1] || launched >= maxit) break;
+        pairs = 1;
+    }
+    if (h[0] > 0) pred = h[0];
+    return h[0];
+}
+
+}  // namespace fcfd

@@ -228,6 +228,23 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
         FCFD_HIP(hipMemset(c.ud_full, 0, sizeof(double) * 2 * nn));
     }
 
+    // passive-scalar Dirichlet nodes
+    if (s->gamma_d_phi && s->n_gamma_d_phi > 0) {
+        std::vector<int64_t> gp(s->gamma_d_phi, s->gamma_d_phi + s->n_gamma_d_phi);
+        std::vector<uint8_t> mask(nn, 0);
+        for (auto v : gp) {
+            FCFD_REQUIRE(v >= 0 && v < nn, "gamma_d_phi out of range");
+            mask[v] = 1;
+        }
+        c.n_gamma_d_phi = (int)gp.size();
+        c.gamma_d_phi_dev = dalloc<int64_t>(c, gp.size());
+        upload(gp, c.gamma_d_phi_dev);
+        c.phi_mask = dalloc<uint8_t>(c, nn);
+        upload(mask, c.phi_mask);
+        c.phi_ud = dalloc<double>(c, nn);
+        FCFD_HIP(hipMemset(c.phi_ud, 0, sizeof(double) * nn));
+    }
+
     // boundary edges -> per-node gather lists in bconn order (segment_sum order)
     if (s->bconn && s->nvec && s->n_bedge > 0 && !gd.empty()) {
         const int ne_b = (int)s->n_bedge;
@@ -323,6 +340,17 @@ void destroy_ctx(Ctx& c) {
     c.allocs.clear();
     if (c.h_pinned) (void)hipHostFree(c.h_pinned);
     c.h_pinned = nullptr;
+}
+
+// four_step_timestep (standard.py:148-165)
+void four_step(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
+               const double* F_p, const double* F, double* U, double* P, int* it) {
+    scatter_ud(c, U_d);
+    build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
+    it[0] = solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x);
+    projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
+    it[1] = solve_pressure(c, c.ustar, P_n, F_p, rho, dt, P);
+    projection_final(c, c.ustar, P, rho, dt, U);
 }
 
 double* api_ce_buffer(Ctx& c) {
@@ -493,18 +521,43 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
                             int32_t* iters) {
     if (!ctx || !U_n || !P_n || !U || !P) return FCFD_ERR_ARG;
     return run(ctx, [&] {
-        Ctx& c = ctx->c;
-        scatter_ud(c, U_d);
-        build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
-        int uit = solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x);
-        projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
-        int pit = solve_pressure(c, c.ustar, P_n, F_p, rho, dt, P);
-        projection_final(c, c.ustar, P, rho, dt, U);
+        int it[2];
+        four_step(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
         if (iters) {
-            iters[0] = uit;
-            iters[1] = pit;
+            iters[0] = it[0];
+            iters[1] = it[1];
         }
     });
+}
+
+int fcfd_solve_scalar(fcfd_ctx* ctx, const double* U_n, const double* U, const double* phi_n, double dt, double D,
+                      const double* phi_d, int32_t supg, double* phi, int32_t* iters) {
+    if (!ctx || !U_n || !U || !phi_n || !phi) return FCFD_ERR_ARG;
+    return run(ctx, [&] {
+        int it = solve_scalar(ctx->c, U_n, U, phi_n, dt, D, phi_d, supg, phi);
+        if (iters) *iters = it;
+    });
+}
+
+int fcfd_passive_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, const double* phi_n, double dt,
+                          double rho, double mu, double D, const double* U_d, const double* phi_d,
+                          const double* F_p, const double* F, int32_t supg, double* U, double* P, double* phi,
+                          int32_t* iters) {
+    if (!ctx || !U_n || !P_n || !phi_n || !U || !P || !phi) return FCFD_ERR_ARG;
+    return run(ctx, [&] {
+        Ctx& c = ctx->c;
+        int it[3];
+        four_step(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+        it[2] = solve_scalar(c, U_n, U, phi_n, dt, D, phi_d, supg, phi);
+        if (iters)
+            for (int k = 0; k < 3; ++k) iters[k] = it[k];
+    });
+}
+
+int fcfd_supg_elem_calc(fcfd_ctx* ctx, const double* U_n, const double* Ce, double D, double dt, double r, double* Ct,
+                        double* tau) {
+    if (!ctx || !U_n) return FCFD_ERR_ARG;
+    return run(ctx, [&] { supg_api(ctx->c, U_n, Ce, D, dt, r, Ct, tau); });
 }
 
 int fcfd_dot(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out) {

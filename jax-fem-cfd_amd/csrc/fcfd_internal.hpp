@@ -62,7 +62,7 @@ struct Ctx {
     hipStream_t stream = nullptr;       // private non-blocking stream every kernel runs on
     hipStream_t user_stream = nullptr;  // the caller's stream (joined with events per call)
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
-    IterGraph g_mom, g_cg;
+    IterGraph g_mom, g_cg, g_phi;
     bool use_graphs = true;
     Grid2D g;
     double lx, ly;
@@ -106,7 +106,13 @@ struct Ctx {
     DevState* st = nullptr;
     int* h_pinned = nullptr;   // pinned host mirror for flags / counts
     // iteration predictions for chunked launching
-    int pred_mom = 4, pred_p = 8;
+    int pred_mom = 4, pred_p = 8, pred_phi = 4;
+    // passive scalar: Dirichlet data and the per-element lhs / rhs matrices (lazy)
+    int n_gamma_d_phi = 0;
+    int64_t* gamma_d_phi_dev = nullptr;
+    uint8_t* phi_mask = nullptr;
+    double* phi_ud = nullptr;
+    double *ade_A = nullptr, *ade_B = nullptr, *s_x = nullptr;
     std::vector<void*> allocs;
 };
 
@@ -208,6 +214,11 @@ float bench_momentum_apply(Ctx& c, int reps);
 // momentum (fcfd_momentum.hip)
 int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n, const double* P_n, double rho,
                    double mu, double dt, const double* F, double* U_hat);
+
+// passive scalar (fcfd_scalar.hip)
+int solve_scalar(Ctx& c, const double* U_n, const double* U, const double* phi_n, double dt, double D,
+                 const double* phi_d, int supg, double* phi);
+void supg_api(Ctx& c, const double* U_n, const double* Ce_api, double D, double dt, double r, double* Ct, double* tau);
 
 // pressure / multigrid (fcfd_mg.hip)
 void vcycle(Ctx& c, const double* r, double* z);

@@ -3,224 +3,41 @@
 // (jax_fem_cfd/solvers/standard.py:14-43), and the projection node kernels of
 // four_step_timestep (standard.py:156, :160-163).
 //
-// One BiCGSTAB iteration = 3 kernels:
-//   K1  p = r + beta (p - omega q) (formed on the fly at each stencil node, double-buffered),
-//       q = A p,  finalize alpha = rho' / (rhat . q)
-//   K2  s = r - alpha q (on the fly), t = A s,  finalize exit_early, omega = (t.s)/(t.t)
-//   K3  x, r updates,  finalize breakdown codes, next rho', beta, convergence flag
-// Scalars live on the device; the host enqueues chunks of iterations and reads the 4-byte
-// active flag once per chunk.
-#include "fcfd_internal.hpp"
-#include "fcfd_ops.hpp"
+// The BiCGSTAB kernels are the generic ones of fcfd_bicgstab.hpp with 2 components and the
+// fused (cM Me + cL Le + cC Ce_e) element operator.
+#include "fcfd_bicgstab.hpp"
 
 namespace fcfd {
 
-// ------------------------------------------------------------------------------ init
-template <class S>
-struct MomInit {
-    MomOp<S> lhs, rhs;
-    const double *Un, *Pn, *F, *ud;
-    const uint8_t* mask;
-    double *x, *r, *rhat, *p, *q;
-    __device__ void operator()(int n, double (&acc)[2]) const {
-        const int nn = lhs.g.nn;
-        NodeNbr s;
-        node_nbr(lhs.g, n, s);
-        double Xu[3][3], Xv[3][3], XP[3][3];
-        load3x3(s, [&](int k) { return Un[k]; }, Xu);
-        load3x3(s, [&](int k) { return Un[k + nn]; }, Xv);
+// -(G_c P_n) + F_c on the momentum rhs (standard.py:31-33)
+struct MomExtra {
+    Grid2D g;
+    const ElemConst* __restrict__ ec;
+    const double *Pn, *F;
+    __device__ __forceinline__ void operator()(const NodeNbr& s, int n, double (&b)[2]) const {
+        double XP[3][3];
         load3x3(s, [&](int k) { return Pn[k]; }, XP);
-        double bu, bv;
-        rhs.apply(s, Xu, Xv, bu, bv);
-        bu -= ebe_const(s, lhs.ec->Gx, XP);
-        bv -= ebe_const(s, lhs.ec->Gy, XP);
-        if (F) { bu += F[n]; bv += F[n + nn]; }
-        const bool mu_ = mask && mask[n], mv_ = mask && mask[n + nn];
-        if (mu_) bu = ud[n];
-        if (mv_) bv = ud[n + nn];
-        double X0u[3][3], X0v[3][3];
-        if (mask) {
-            load3x3(s, [&](int k) { const double a = ud[k], b = Un[k]; return mask[k] ? a : b; }, X0u);
-            load3x3(s, [&](int k) { const double a = ud[k + nn], b = Un[k + nn]; return mask[k + nn] ? a : b; }, X0v);
-        } else {
-            load3x3(s, [&](int k) { return Un[k]; }, X0u);
-            load3x3(s, [&](int k) { return Un[k + nn]; }, X0v);
-        }
-        double au, av;
-        lhs.apply(s, X0u, X0v, au, av);
-        if (mu_) au = X0u[1][1];
-        if (mv_) av = X0v[1][1];
-        const double ru = bu - au, rv = bv - av;
-        x[n] = X0u[1][1]; x[n + nn] = X0v[1][1];
-        r[n] = ru; r[n + nn] = rv;
-        rhat[n] = ru; rhat[n + nn] = rv;
-        p[n] = ru; p[n + nn] = rv;
-        q[n] = ru; q[n + nn] = rv;
-        acc[0] += bu * bu + bv * bv;
-        acc[1] += ru * ru + rv * rv;
+        b[0] -= ebe_const(s, ec->Gx, XP);
+        b[1] -= ebe_const(s, ec->Gy, XP);
+        if (F) { b[0] += F[n]; b[1] += F[n + g.nn]; }
     }
-};
-struct MomInitFin {
-    DevState* st;
-    double tol;
-    int maxit;
-    __device__ void operator()(const double (&t)[2]) const {
-        double a2 = tol * tol * t[0];
-        st->b_atol2 = a2 > 0.0 ? a2 : 0.0;
-        st->b_rho = 1.0; st->b_alpha = 1.0; st->b_omega = 1.0;
-        st->b_rho_new = t[1];
-        st->b_beta = t[1] / 1.0 * 1.0 / 1.0;
-        st->b_k = 0; st->b_ee = 0; st->b_maxit = maxit;
-        st->b_active = (t[1] > st->b_atol2) && (0 < maxit);
-    }
+    __device__ __forceinline__ void operator()(const NodeNbr&, int, double (&)[1]) const {}
 };
 
-// ------------------------------------------------------------------------------ K1
-template <class S>
-struct MomK1 {
-    MomOp<S> op;
-    const uint8_t* mask;
-    const double *r, *pold, *qold, *rhat;
-    double *pnew, *qnew;
-    const DevState* st;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        const int nn = op.g.nn;
-        const double beta = st->b_beta, omega = st->b_omega;
-        NodeNbr s;
-        node_nbr(op.g, n, s);
-        double Xu[3][3], Xv[3][3];
-        load3x3(s, [&](int k) { return r[k] + beta * (pold[k] - omega * qold[k]); }, Xu);
-        load3x3(s, [&](int k) { return r[k + nn] + beta * (pold[k + nn] - omega * qold[k + nn]); }, Xv);
-        double yu, yv;
-        op.apply(s, Xu, Xv, yu, yv);
-        if (mask && mask[n]) yu = Xu[1][1];
-        if (mask && mask[n + nn]) yv = Xv[1][1];
-        pnew[n] = Xu[1][1]; pnew[n + nn] = Xv[1][1];
-        qnew[n] = yu; qnew[n + nn] = yv;
-        acc[0] += rhat[n] * yu + rhat[n + nn] * yv;
-    }
-};
-struct MomK1Fin {
-    DevState* st;
-    __device__ void operator()(const double (&t)[1]) const { st->b_alpha = st->b_rho_new / t[0]; }
-};
-
-// ------------------------------------------------------------------------------ K2
-template <class S>
-struct MomK2 {
-    MomOp<S> op;
-    const uint8_t* mask;
-    const double *r, *q;
-    double *sv, *tv;
-    const DevState* st;
-    __device__ void operator()(int n, double (&acc)[3]) const {
-        const int nn = op.g.nn;
-        const double alpha = st->b_alpha;
-        NodeNbr s;
-        node_nbr(op.g, n, s);
-        double Xu[3][3], Xv[3][3];
-        load3x3(s, [&](int k) { return r[k] - alpha * q[k]; }, Xu);
-        load3x3(s, [&](int k) { return r[k + nn] - alpha * q[k + nn]; }, Xv);
-        double yu, yv;
-        op.apply(s, Xu, Xv, yu, yv);
-        if (mask && mask[n]) yu = Xu[1][1];
-        if (mask && mask[n + nn]) yv = Xv[1][1];
-        const double su = Xu[1][1], sv_ = Xv[1][1];
-        sv[n] = su; sv[n + nn] = sv_;
-        tv[n] = yu; tv[n + nn] = yv;
-        acc[0] += su * su + sv_ * sv_;
-        acc[1] += yu * su + yv * sv_;
-        acc[2] += yu * yu + yv * yv;
-    }
-};
-struct MomK2Fin {
-    DevState* st;
-    __device__ void operator()(const double (&t)[3]) const {
-        st->b_ee = t[0] < st->b_atol2;
-        st->b_omega = t[1] / t[2];
-    }
-};
-
-// ------------------------------------------------------------------------------ K3
-struct MomK3 {
-    int nn;
-    const double *p, *sv, *tv, *rhat;
-    double *x, *r;
-    const DevState* st;
-    __device__ void operator()(int k, double (&acc)[2]) const {
-        const double alpha = st->b_alpha, omega = st->b_omega;
-        const bool ee = st->b_ee;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int i = k + c * nn;
-            const double pi = p[i], si = sv[i], ti = tv[i];
-            double xi = x[i];
-            double ri;
-            if (ee) { xi = xi + alpha * pi; ri = si; }
-            else { xi = xi + (alpha * pi + omega * si); ri = si - omega * ti; }
-            x[i] = xi; r[i] = ri;
-            acc[0] += ri * ri;
-            acc[1] += rhat[i] * ri;
-        }
-    }
-};
-struct MomK3Fin {
-    DevState* st;
-    __device__ void operator()(const double (&t)[2]) const {
-        const double alpha = st->b_alpha, omega = st->b_omega, rho_cur = st->b_rho_new;
-        int k = (omega == 0.0 || alpha == 0.0) ? -11 : st->b_k + 1;
-        if (rho_cur == 0.0) k = -10;
-        st->b_k = k;
-        st->b_rho = rho_cur;
-        st->b_rho_new = t[1];
-        st->b_beta = t[1] / rho_cur * alpha / omega;
-        st->b_active = (t[0] > st->b_atol2) && (k < st->b_maxit) && (k >= 0);
-    }
-};
-
-// ------------------------------------------------------------------------------ driver
 template <class S>
 static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, double rho, double mu, double dt,
                             const double* F, double* U_hat) {
     const double theta = 0.5;
     const Grid2D& g = c.g;
-    MomOp<S> lhs{g, c.ec_dev, ce, rho / dt, theta * mu, theta * rho};
-    MomOp<S> rhs{g, c.ec_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
+    MomCoef<S> lhs{g, c.ec_dev, ce, rho / dt, theta * mu, theta * rho};
+    MomCoef<S> rhs{g, c.ec_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
     const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 2 * g.nn;
-    launch_node<2>(c, g.nn, nullptr,
-                   MomInit<S>{lhs, rhs, U_n, P_n, F, c.ud_full, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
-                   MomInitFin{c.st, c.tol, maxit});
-    FCFD_CHECK_LAUNCH();
-    const int* act = &c.st->b_active;
-    // one chunk = a pair of iterations, so the double-buffer parity returns to 0
-    auto pair = [&]() {
-        for (int o = 0; o < 2; ++o) {
-            const int nw = o ^ 1;
-            launch_node<1>(c, g.nn, act,
-                           MomK1<S>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
-                           MomK1Fin{c.st});
-            launch_node<3>(c, g.nn, act, MomK2<S>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
-                           MomK2Fin{c.st});
-            launch_node<2>(c, g.nn, act, MomK3{g.nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
-                           MomK3Fin{c.st});
-        }
-        FCFD_CHECK_LAUNCH();
-    };
     const double key[4] = {rho, mu, dt, 0.0};
-    int launched = 0;
-    int pairs = c.pred_mom > 0 ? (c.pred_mom + 1) / 2 : 1;
-    int h[2] = {0, 0};
-    while (true) {
-        replay(c, c.g_mom, key, ce.c, pairs, pair);
-        launched += 2 * pairs;
-        read_state_ints(c, &c.st->b_k, 2, h);  // b_k, b_active
-        if (!h[1] || launched >= maxit) break;
-        pairs = 1;
-    }
+    const int it = bicgstab_solve<2>(c, c.g_mom, key, ce.c, lhs, rhs, MomExtra{g, c.ec_dev, P_n, F}, U_n, c.ud_full,
+                                     c.dmask, maxit, c.m_x, c.pred_mom);
     if (U_hat != c.m_x)
         FCFD_HIP(hipMemcpyAsync(U_hat, c.m_x, sizeof(double) * 2 * g.nn, hipMemcpyDeviceToDevice, c.stream));
-    if (h[0] > 0) c.pred_mom = h[0];
-    return h[0];
+    return it;
 }
 
 int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n, const double* P_n, double rho,
@@ -240,18 +57,20 @@ int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n
 }
 
 // ------------------------------------------------------------------------------ benchmark
+// The momentum operator apply of BiCGSTAB (K2: s = r - alpha q, t = A s, 3 dot partials), run
+// back to back `reps` times under its own kernel symbol (TAG = 1) and timed with HIP events on
+// the context stream.  The node kernel alone: its 1-block finalize is not part of the timing.
 template <class S>
 static float bench_t(Ctx& c, S ce, int reps) {
-    MomOp<S> lhs{c.g, c.ec_dev, ce, 2000.0, 0.5e-3, 0.5};
+    MomCoef<S> lhs{c.g, c.ec_dev, ce, 2000.0, 0.5e-3, 0.5};
     const int nb = blocks_for(c.g.nn);
     hipEvent_t e0, e1;
     FCFD_HIP(hipEventCreate(&e0));
     FCFD_HIP(hipEventCreate(&e1));
-    // the node kernel alone (its 1-block finalize is not part of the timed work)
     auto launch = [&]() {
-        node_kernel<3><<<nb, BLOCK, 0, c.stream>>>(c.g.nn, nullptr,
-                                                   MomK2<S>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st},
-                                                   c.partials, c.pstride);
+        node_kernel<3><<<nb, BLOCK, 0, c.stream>>>(
+            c.g.nn, nullptr, BK2<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st}, c.partials,
+            c.pstride);
     };
     launch();
     FCFD_HIP(hipEventRecord(e0, c.stream));

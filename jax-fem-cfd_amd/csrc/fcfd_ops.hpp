@@ -54,37 +54,6 @@ __device__ __forceinline__ T ebe_const_T(const NodeNbr& s, const double* __restr
     return acc;
 }
 
-// Fused momentum operator on both velocity components:
-//   y_c = sum_e (cM Me + cL Le + cC Ce_e) u_c(e)
-// (the reference evaluates (rho/dt) consistent_mass_mult + theta visc_conv_mult,
-// jax_fem_cfd/solvers/standard.py:20-28).  Ce_e row entries are loaded once and used for both
-// components.
-template <class S>
-struct MomOp {
-    Grid2D g;
-    const ElemConst* __restrict__ ec;
-    S ce;
-    double cM, cL, cC;
-    __device__ __forceinline__ void apply(const NodeNbr& s, const double (&Xu)[3][3], const double (&Xv)[3][3],
-                                          double& yu, double& yv) const {
-        yu = 0.0; yv = 0.0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int a = kRow(k);
-            const int e = s.e[k];
-            double tu = 0.0, tv = 0.0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                double coef = cM * ec->Me[a * 4 + b] + cL * ec->Le[a * 4 + b] + cC * ce.ld(a * 4 + b, e);
-                tu += coef * Xu[kCy(k, b)][kCx(k, b)];
-                tv += coef * Xv[kCy(k, b)][kCx(k, b)];
-            }
-            yu += s.has[k] ? tu : 0.0;
-            yv += s.has[k] ? tv : 0.0;
-        }
-    }
-};
-
 // Element corner node ids
 __device__ __forceinline__ void elem_nodes(const Grid2D& g, int e, int (&c)[4]) {
     int je = e / g.nex, ie = e - je * g.nex;

@@ -19,7 +19,7 @@ class Context:
 
     def __init__(self, nex, ney, lx, ly, periodic, *, tol=1e-8, set_zeroP=False, precond=None, mg_levels=1,
                  precision="fp64", ce_format="f64", gamma_d=None, gamma_p=None, mg_gamma_p=None, bconn=None,
-                 nvec=None, maxiter_momentum=0, maxiter_pressure=0):
+                 nvec=None, maxiter_momentum=0, maxiter_pressure=0, gamma_d_phi=None):
         lib = nat.load_library()
         t = nat.require_gpu()
         self.nex, self.ney, self.lx, self.ly, self.periodic = int(nex), int(ney), float(lx), float(ly), bool(periodic)
@@ -57,6 +57,8 @@ class Context:
             sd.mg_gamma_p, sd.mg_gamma_p_off = arr(flat), arr(off)
         if bconn is not None and len(bconn):
             sd.bconn, sd.nvec, sd.n_bedge = arr(bconn), arr(nvec), len(bconn)
+        if gamma_d_phi is not None and len(gamma_d_phi):
+            sd.gamma_d_phi, sd.n_gamma_d_phi = arr(gamma_d_phi), len(gamma_d_phi)
         h = ctypes.c_void_p()
         nat.check(lib.fcfd_ctx_create(ctypes.byref(md), ctypes.byref(sd), nat.stream_handle(), ctypes.byref(h)),
                   "fcfd_ctx_create")
@@ -210,6 +212,42 @@ class Context:
                                                     float(mu), nat.ptr(Ud), nat.ptr(Fp), nat.ptr(Fv), nat.ptr(U),
                                                     nat.ptr(P), iters), "fcfd_four_step_timestep")
         return U, P, int(iters[0]), int(iters[1])
+
+    def solve_scalar(self, U_n, U, phi_n, dt, D, phi_d=None, supg=False):
+        U_n, U, phi_n = nat.as_dev(U_n), nat.as_dev(U), nat.as_dev(phi_n)
+        pd = None if phi_d is None else nat.as_dev(phi_d)
+        phi = self._empty(self.nn)
+        it = ctypes.c_int32()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_solve_scalar(self._h, nat.ptr(U_n), nat.ptr(U), nat.ptr(phi_n), float(dt), float(D),
+                                              nat.ptr(pd), int(bool(supg)), nat.ptr(phi), ctypes.byref(it)),
+                  "fcfd_solve_scalar")
+        return phi, it.value
+
+    def passive_timestep(self, U_n, P_n, phi_n, dt, rho, mu, D, U_d=None, phi_d=None, F_p=None, F=None,
+                         supg=False):
+        U_n, P_n, phi_n = nat.as_dev(U_n), nat.as_dev(P_n), nat.as_dev(phi_n)
+        Ud = None if U_d is None else nat.as_dev(U_d)
+        pd = None if phi_d is None else nat.as_dev(phi_d)
+        Fp, Fv = _opt_vec(F_p, self.nn), _opt_vec(F, 2 * self.nn)
+        U, P, phi = self._empty(2 * self.nn), self._empty(self.nn), self._empty(self.nn)
+        iters = (ctypes.c_int32 * 3)()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_passive_timestep(self._h, nat.ptr(U_n), nat.ptr(P_n), nat.ptr(phi_n), float(dt),
+                                                  float(rho), float(mu), float(D), nat.ptr(Ud), nat.ptr(pd),
+                                                  nat.ptr(Fp), nat.ptr(Fv), int(bool(supg)), nat.ptr(U), nat.ptr(P),
+                                                  nat.ptr(phi), iters), "fcfd_passive_timestep")
+        return U, P, phi, int(iters[0]), int(iters[1]), int(iters[2])
+
+    def supg_elem_calc(self, U_n, Ce=None, D=0.0, dt=1.0, r=2.0):
+        U_n = nat.as_dev(U_n)
+        Ce = None if Ce is None else nat.as_dev(Ce)
+        Ct = self._empty(16 * self.ne)
+        tau = self._empty(self.ne) if Ce is not None else None
+        self._sync_stream()
+        nat.check(self._lib.fcfd_supg_elem_calc(self._h, nat.ptr(U_n), nat.ptr(Ce), float(D), float(dt), float(r),
+                                                nat.ptr(Ct), nat.ptr(tau)), "fcfd_supg_elem_calc")
+        return Ct.view(self.ne, 4, 4), tau
 
     def dot(self, x, y):
         out = ctypes.c_double()

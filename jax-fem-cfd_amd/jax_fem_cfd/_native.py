@@ -38,7 +38,8 @@ class SolverDesc(ctypes.Structure):
                 ("gamma_d", _I64P), ("n_gamma_d", ctypes.c_int64),
                 ("gamma_p", _I64P), ("n_gamma_p", ctypes.c_int64),
                 ("mg_gamma_p", _I64P), ("mg_gamma_p_off", _I64P),
-                ("bconn", _I64P), ("nvec", _I64P), ("n_bedge", ctypes.c_int64)]
+                ("bconn", _I64P), ("nvec", _I64P), ("n_bedge", ctypes.c_int64),
+                ("gamma_d_phi", _I64P), ("n_gamma_d_phi", ctypes.c_int64)]
 
 
 _vp = ctypes.c_void_p
@@ -68,6 +69,10 @@ _SIGS = {
     "fcfd_solve_momentum": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "fcfd_four_step_timestep": (ctypes.c_int, [_vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp, _vp, _vp,
                                                ctypes.POINTER(_i32)]),
+    "fcfd_solve_scalar": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _vp, _i32, _vp, ctypes.POINTER(_i32)]),
+    "fcfd_passive_timestep": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
+                                             _vp, ctypes.POINTER(_i32)]),
+    "fcfd_supg_elem_calc": (ctypes.c_int, [_vp, _vp, _vp, _d, _d, _d, _vp, _vp]),
     "fcfd_dot": (ctypes.c_int, [_vp, _vp, _vp, _i64, ctypes.POINTER(_d)]),
     "fcfd_axpby": (ctypes.c_int, [_vp, _d, _vp, _d, _vp, _i64]),
     "fcfd_bench_momentum_apply": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(ctypes.c_float)]),

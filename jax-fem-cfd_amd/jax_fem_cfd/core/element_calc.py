@@ -77,10 +77,17 @@ def u_dot_n_boundary_integral_1d(bconn, nvec, gamma_d, U_d, nn, node_coords, sha
 
 
 def Ctau_tilde_elem_calc(nconn, U, wq, N, Nderiv, nn, ndim):
-    """element_calc.py:188-200 (SUPG).  Not built in this version."""
-    raise NotImplementedError("SUPG element matrices (passive scalar transport) are not built yet")
+    """element_calc.py:188-200: Ct[a,b] = sum_q w (U_q . grad N_a)(U_q . grad N_b), (ne, 4, 4)."""
+    if ndim != 2:
+        raise NotImplementedError("SUPG element matrices are built for 2D Q1 only")
+    hx, hy = _h_from_shape(wq, Nderiv)
+    Ct, _ = grid_context(nconn, int(nn), hx, hy).supg_elem_calc(U)
+    return Ct
 
 
 def calculate_tau(nconn, U, Ce_all, Ctau_tilde_all, N, nn, ndim, D, dt, r):
-    """element_calc.py:202-229 (SUPG).  Not built in this version."""
-    raise NotImplementedError("SUPG tau (passive scalar transport) is not built yet")
+    """element_calc.py:202-229: SUPG tau per element from |Ce|, |Ct| (the latter recomputed
+    from U on the device) and the quadrature velocities of U."""
+    c = grid_context(nconn, int(nn))
+    _, tau = c.supg_elem_calc(U, Ce=Ce_all, D=D, dt=dt, r=r)
+    return tau

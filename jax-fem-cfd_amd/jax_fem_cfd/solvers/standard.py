@@ -41,7 +41,8 @@ def _mg_pins(num_elem, domain_size, outlet_faces, partial_dirichlet, mesh_func, 
     return pins
 
 
-def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: Config, ns_config=None):
+def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: Config, ns_config=None,
+                  gamma_d_phi=None):
     mc = config.mesh_config
     sc = ns_config if ns_config is not None else config.solver_config
     if len(mc.num_elem) != 2 or mc.shape_func_ord != 1:
@@ -57,7 +58,7 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
     return Context(nex, ney, mc.domain_size[0], mc.domain_size[1], mc.periodic, tol=sc.solver_tol,
                    set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=nlev, precision=sc.precision,
                    ce_format=ce_format, gamma_d=gamma_d, gamma_p=np.atleast_1d(gamma_p) if sc.set_zeroP else None,
-                   mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec)
+                   mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, gamma_d_phi=gamma_d_phi)
 
 
 def setup_solver(node_coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: Config):

@@ -305,3 +305,55 @@ def test_mixed_precision_cavity(ce_format):
     print(f"mixed {ce_format}: relL2 U {eu:.2e} P {ep:.2e} iters {ui},{pi}")
     tu, tp = MIXED_TOL[ce_format]
     assert eu < tu and ep < tp
+
+
+# ----------------------------------------------------------------------------------------- passive scalar
+def _rect_phi(coords, x0=0.3, x1=0.7, y0=0.2, y1=0.6):
+    x, y = coords[:, 0], coords[:, 1]
+    return ((x >= x0) & (x <= x1) & (y >= y0) & (y <= y1)).astype(float)
+
+
+@pytest.mark.parametrize("supg", [False, True])
+def test_passive_transport_matches_oracle(supg):
+    """Passive scalar (passive_transport.py:64-97): cavity 32x32 with a swirl, scalar
+    rectangle, 4 steps, plain and SUPG -- U, P, phi vs the oracle with equal iteration counts."""
+    import jax_fem_cfd.setup.simulation as sim
+    n, dt, rho, mu, D = 32, 2e-3, 1.0, 1e-2, 1e-4
+    cfg = sim.Config(mesh="simple", solver="passive_transport",
+                     mesh_config=sim.SimpleMeshConfig([n, n], [1.0, 1.0], 1, False),
+                     solver_config=sim.PassiveTransportConfig(
+                         sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, 1e-10, True, "multigrid", [4, 4]),
+                         sim.ADEConfig([], supg)))
+    compute, timestep, coords, surf, h, nn, nnx, nny, gamma_d, U_d, gamma_d_phi = sim.setup(cfg)
+    print(f"\n[case] passive transport supg={supg}", flush=True)
+    x, y = coords[:, 0], coords[:, 1]
+    U0 = np.concatenate([np.sin(np.pi * x) * np.cos(np.pi * y), -np.cos(np.pi * x) * np.sin(np.pi * y)])
+    U0[gamma_d] = U_d
+    phi0 = _rect_phi(coords)
+    s, osurf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-10, True, "multigrid", [4, 4])
+    U, P, phi = U0, np.zeros(nn), phi0
+    Uo, Po, phio = U0.copy(), np.zeros(nn), phi0.copy()
+    for _ in range(4):
+        F_p, F = compute(U_d, None, 0, 0)
+        U, P, phi, ui, pi, fi = timestep(U, P, phi, dt, rho, mu, D, U_d, None, F_p, F, None)
+        Uo, Po, phio, uio, pio, fio = ns.passive_timestep(s, Uo, Po, phio, dt, rho, mu, D, oUd, None, 0, 0, None,
+                                                          supg)
+        assert (ui, pi, fi) == (uio, pio, fio)
+    assert _rel(_np(U), Uo) < 1e-10 and _rel(_np(P), Po) < 1e-10
+    assert _rel(_np(phi), phio) < 1e-10
+
+
+def test_supg_element_calc_api():
+    from jax_fem_cfd.core import element_calc as ec
+    from jax_fem_cfd.core import shape_functions as sf
+    coords, nconn, *_ = oms.create_mesh_2d(8, 6, 1.0, 0.75)
+    nn = coords.shape[0]
+    wq, N, Nd = sf.quad4_shape_functions(coords[nconn[0]])
+    U = RNG.standard_normal(2 * nn)
+    Ct = _np(ec.Ctau_tilde_elem_calc(nconn, U, wq, N, Nd, nn, 2))
+    Cto = oel.Ctau_tilde_elem_calc(nconn, U, wq, N, Nd, nn, 2)
+    assert np.abs(Ct - Cto).max() <= 1e-13 * np.abs(Cto).max()
+    Ce = oel.conservative_convec_elem_calc(nconn, U, wq, N, Nd, nn, 2, 1.0)
+    tau = _np(ec.calculate_tau(nconn, U, Ce, Cto, N, nn, 2, 1e-4, 1e-3, 2))
+    tauo = oel.calculate_tau(nconn, U, Ce, Cto, N, nn, 2, 1e-4, 1e-3, 2)
+    assert _rel(tau, tauo) < 1e-13
