@@ -198,6 +198,36 @@ int fcfd_passive_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, c
 int fcfd_supg_elem_calc(fcfd_ctx* ctx, const double* U_n, const double* Ce, double D, double dt,
                         double r, double* Ct, double* tau);
 
+/* ---- reverse-mode products (adjoint twins of the steps) --------------------------- */
+
+/* Vector-Jacobian product of four_step_timestep (standard.py:148-165) as JAX's reverse-mode
+ * AD of the reference defines it: every solve differentiated implicitly through
+ * lax.custom_linear_solve (jax_iterative_solvers.py:201-211) -- the momentum solve by BiCGSTAB
+ * on the transposed masked operator, the pressure solve by the same MG-PCG (marked symmetric),
+ * both from the forward solve's x0 and tol -- and the convection matrix Ce(U_n) differentiated
+ * through its (linear) dependence on U_n.  Re-runs the forward step from (U_n, P_n) into U, P
+ * (required; must not alias the inputs), then writes gU_n (2nn) and gP_n (nn) for the output
+ * cotangents gU (2nn) and gP (nn) (NULL = zero).  The outputs may alias the cotangents.
+ * iters (4): forward momentum, forward pressure, adjoint pressure, adjoint momentum. */
+int fcfd_four_step_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_n, double dt,
+                                double rho, double mu, const double* U_d, const double* F_p,
+                                const double* F, const double* gU, const double* gP, double* U,
+                                double* P, double* gU_n, double* gP_n, int32_t* iters);
+
+/* Vector-Jacobian product of the passive-scalar timestep (passive_transport.py:64-97, plain
+ * scalar update; supg must be 0): re-runs the step into U, P, phi, then writes gU_n, gP_n,
+ * gphi_n for the output cotangents gU, gP, gphi (NULL = zero).  The scalar solve is
+ * differentiated like the momentum solve (transposed BiCGSTAB from its forward x0).
+ * iters (6): forward momentum, pressure, scalar; adjoint scalar, pressure, momentum.
+ * The reverse sweep of a rollout (timestep_loop_scan, simulation.py:362-374) calls this once
+ * per step, last step first, with the states recomputed from checkpoints. */
+int fcfd_passive_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_n,
+                              const double* phi_n, double dt, double rho, double mu, double D,
+                              const double* U_d, const double* phi_d, const double* F_p,
+                              const double* F, int32_t supg, const double* gU, const double* gP,
+                              const double* gphi, double* U, double* P, double* phi, double* gU_n,
+                              double* gP_n, double* gphi_n, int32_t* iters);
+
 /* ---- BLAS-1 used by the generic cg / bicgstab wrappers ----------------------------- */
 
 /* *out (host) = x . y, FP64, deterministic two-stage reduction */

@@ -8,6 +8,11 @@
 //   K2  s = r - alpha q (on the fly), t = A s,  exit_early, omega = (t.s)/(t.t)
 //   K3  x, r updates,  breakdown codes -10 / -11, next rho', beta, convergence flag
 // Operators provide  double coef(int a, int b, int e)  (row a, column b of element e's matrix).
+//
+// Masking.  The forward operators replace the Dirichlet rows:  A y = mask ? y : Ahat y.  Their
+// transposes (the adjoint solves of implicit differentiation, jax_iterative_solvers.py:209-211)
+// mask the input instead:  A^T y = Ahat^T (mask ? 0 : y) + (mask ? y : 0)   (MT = true, with
+// TransCoef swapping the element matrix indices).
 #pragma once
 #include "fcfd_internal.hpp"
 #include "fcfd_ops.hpp"
@@ -54,6 +59,50 @@ __device__ __forceinline__ void op_apply(const Op& op, const NodeNbr& s, const d
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c) y[c] += s.has[k] ? t[c] : 0.0;
+    }
+}
+
+// Element matrices transposed (adjoint operators)
+template <class Op>
+struct TransCoef {
+    Op op;
+    Grid2D g;
+    __device__ __forceinline__ double coef(int a, int b, int e) const { return op.coef(b, a, e); }
+};
+template <class Op>
+__host__ __device__ __forceinline__ TransCoef<Op> transpose_op(const Op& op) { return TransCoef<Op>{op, op.g}; }
+
+// y = masked operator applied to the stencil X (see the header comment for MT)
+template <int NC, bool MT, class Op>
+__device__ __forceinline__ void masked_apply(const Op& op, const NodeNbr& s, int n, int nn, const uint8_t* mask,
+                                             const double (&X)[NC][3][3], double (&y)[NC]) {
+    if constexpr (!MT) {
+        op_apply<NC>(op, s, X, y);
+        if (mask) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (mask[n + c * nn]) y[c] = X[c][1][1];
+        }
+    } else {
+        if (mask) {
+            double Xm[NC][3][3];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const uint8_t* mc = mask + c * nn;
+                uint8_t M[3][3];
+                load3x3(s, [&](int k) { return mc[k]; }, M);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) Xm[c][a][b] = M[a][b] ? 0.0 : X[c][a][b];
+            }
+            op_apply<NC>(op, s, Xm, y);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (mask[n + c * nn]) y[c] += X[c][1][1];
+        } else {
+            op_apply<NC>(op, s, X, y);
+        }
     }
 }
 
@@ -125,7 +174,7 @@ struct BInitFin {
 };
 
 // ------------------------------------------------------------------------------ K1
-template <int NC, class LOp>
+template <int NC, class LOp, bool MT = false>
 struct BK1 {
     LOp op;
     const uint8_t* mask;
@@ -144,11 +193,11 @@ struct BK1 {
             load3x3(s, [&](int k) { return r[k + o] + beta * (pold[k + o] - omega * qold[k + o]); }, X[c]);
         }
         double y[NC];
-        op_apply<NC>(op, s, X, y);
+        masked_apply<NC, MT>(op, s, n, nn, mask, X, y);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int i = n + c * nn;
-            const double yc = (mask && mask[i]) ? X[c][1][1] : y[c];
+            const double yc = y[c];
             pnew[i] = X[c][1][1];
             qnew[i] = yc;
             acc[0] += rhat[i] * yc;
@@ -163,7 +212,7 @@ struct BK1Fin {
 // ------------------------------------------------------------------------------ K2
 // TAG only distinguishes otherwise identical instantiations (the standalone timing of the
 // operator apply runs under its own kernel symbol).
-template <int NC, class LOp, int TAG = 0>
+template <int NC, class LOp, int TAG = 0, bool MT = false>
 struct BK2 {
     LOp op;
     const uint8_t* mask;
@@ -182,11 +231,11 @@ struct BK2 {
             load3x3(s, [&](int k) { return r[k + o] - alpha * q[k + o]; }, X[c]);
         }
         double y[NC];
-        op_apply<NC>(op, s, X, y);
+        masked_apply<NC, MT>(op, s, n, nn, mask, X, y);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int i = n + c * nn;
-            const double yc = (mask && mask[i]) ? X[c][1][1] : y[c];
+            const double yc = y[c];
             const double sc = X[c][1][1];
             sv[i] = sc;
             tv[i] = yc;
@@ -242,28 +291,84 @@ struct BK3Fin {
     }
 };
 
+// ------------------------------------------------------------------------------ init from a vector
+// Adjoint solves: rhs = b (a given vector), x0 = x_n with the Dirichlet values (the forward
+// solve's x0, which the reference's transposed solve reuses); r0 = b - A x0.  A rhs that is
+// exactly zero has the solution 0: the solve is marked done with x = 0 (b_zero) instead of
+// iterating towards a zero residual norm.
+template <int NC, bool MT, class LOp>
+struct BInitVec {
+    LOp lhs;
+    const double *bvec, *xn, *ud;
+    const uint8_t* mask;
+    double *x, *r, *rhat, *p, *q;
+    __device__ void operator()(int n, double (&acc)[2]) const {
+        const int nn = lhs.g.nn;
+        NodeNbr s;
+        node_nbr(lhs.g, n, s);
+        double X0[NC][3][3];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const double* xc = xn + c * nn;
+            if (mask) {
+                const double* uc = ud + c * nn;
+                const uint8_t* mc = mask + c * nn;
+                load3x3(s, [&](int k) { const double a = uc[k], b = xc[k]; return mc[k] ? a : b; }, X0[c]);
+            } else {
+                load3x3(s, [&](int k) { return xc[k]; }, X0[c]);
+            }
+        }
+        double ax[NC];
+        masked_apply<NC, MT>(lhs, s, n, nn, mask, X0, ax);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = n + c * nn;
+            const double bc = bvec[i];
+            const double rc = bc - ax[c];
+            x[i] = X0[c][1][1];
+            r[i] = rc; rhat[i] = rc; p[i] = rc; q[i] = rc;
+            acc[0] += bc * bc;
+            acc[1] += rc * rc;
+        }
+    }
+};
+struct BInitVecFin {
+    DevState* st;
+    double tol;
+    int maxit;
+    __device__ void operator()(const double (&t)[2]) const {
+        BInitFin{st, tol, maxit}(t);
+        st->b_zero = t[0] == 0.0;
+        if (t[0] == 0.0) st->b_active = 0;
+    }
+};
+struct ZeroIfFlag {
+    const int* flag;
+    double* x;
+    __device__ void operator()(int k, double (&)[1]) const {
+        if (*flag) x[k] = 0.0;
+    }
+};
+
 // ------------------------------------------------------------------------------ driver
-// Solves A x = R x_n + extra (Dirichlet rows: x = ud) from x0 = x_n with Dirichlet values, into
-// `x` (NC * nn, a context buffer: the captured iteration graph keeps its address).  Returns the
-// reference's iteration count (or -10 / -11).
-template <int NC, class LOp, class ROp, class Extra>
-int bicgstab_solve(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
-                   const ROp& rhs, const Extra& extra, const double* xn, const double* ud, const uint8_t* mask,
-                   int maxit, double* x, int& pred) {
+// BiCGSTAB iterations after an init kernel has set x, r, rhat, p, q and the state: chunks of
+// iteration pairs (graph-replayed) until the device flag says done.  Returns the reference's
+// iteration count (or -10 / -11).
+template <int NC, bool MT, class LOp>
+int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
+                     const uint8_t* mask, int maxit, double* x, int& pred) {
     const int nn = c.g.nn;
-    launch_node<2>(c, nn, nullptr,
-                   BInit<NC, LOp, ROp, Extra>{lhs, rhs, extra, xn, ud, mask, x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
-                   BInitFin{c.st, c.tol, maxit});
-    FCFD_CHECK_LAUNCH();
     const int* act = &c.st->b_active;
     // one chunk = a pair of iterations, so the double-buffer parity returns to 0
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
             launch_node<1>(c, nn, act,
-                           BK1<NC, LOp>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
+                           BK1<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw],
+                                            c.st},
                            BK1Fin{c.st});
-            launch_node<3>(c, nn, act, BK2<NC, LOp>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st}, BK2Fin{c.st});
+            launch_node<3>(c, nn, act, BK2<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                           BK2Fin{c.st});
             launch_node<2>(c, nn, act, BK3<NC>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, x, c.m_r, c.st}, BK3Fin{c.st});
         }
         FCFD_CHECK_LAUNCH();
@@ -280,6 +385,34 @@ int bicgstab_solve(Ctx& c, IterGraph& graph, const double (&key)[4], const void*
     }
     if (h[0] > 0) pred = h[0];
     return h[0];
+}
+
+// Solves A x = R x_n + extra (Dirichlet rows: x = ud) from x0 = x_n with Dirichlet values, into
+// `x` (NC * nn, a context buffer: the captured iteration graph keeps its address).  Returns the
+// reference's iteration count (or -10 / -11).
+template <int NC, class LOp, class ROp, class Extra>
+int bicgstab_solve(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
+                   const ROp& rhs, const Extra& extra, const double* xn, const double* ud, const uint8_t* mask,
+                   int maxit, double* x, int& pred) {
+    launch_node<2>(c, c.g.nn, nullptr,
+                   BInit<NC, LOp, ROp, Extra>{lhs, rhs, extra, xn, ud, mask, x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                   BInitFin{c.st, c.tol, maxit});
+    FCFD_CHECK_LAUNCH();
+    return bicgstab_iterate<NC, false>(c, graph, key, kptr, lhs, mask, maxit, x, pred);
+}
+
+// Transposed solve A^T x = b (adjoint of a masked forward solve), x0 = x_n with Dirichlet values.
+template <int NC, class LOp>
+int bicgstab_solve_T(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhsT,
+                     const double* b, const double* xn, const double* ud, const uint8_t* mask, int maxit, double* x,
+                     int& pred) {
+    const int nn = c.g.nn;
+    launch_node<2>(c, nn, nullptr,
+                   BInitVec<NC, true, LOp>{lhsT, b, xn, ud, mask, x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                   BInitVecFin{c.st, c.tol, maxit});
+    launch_node<0>(c, NC * nn, nullptr, ZeroIfFlag{&c.st->b_zero, x}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+    return bicgstab_iterate<NC, true>(c, graph, key, kptr, lhsT, mask, maxit, x, pred);
 }
 
 }  // namespace fcfd

@@ -329,8 +329,7 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
 void destroy_ctx(Ctx& c) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     (void)hipDeviceSynchronize();
-    graph_reset(c.g_mom);
-    graph_reset(c.g_cg);
+    for (IterGraph* g : {&c.g_mom, &c.g_cg, &c.g_phi, &c.g_amom, &c.g_aphi}) graph_reset(*g);
     if (c.ev_in) (void)hipEventDestroy(c.ev_in);
     if (c.ev_out) (void)hipEventDestroy(c.ev_out);
     if (c.stream) (void)hipStreamDestroy(c.stream);
@@ -551,6 +550,43 @@ int fcfd_passive_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, c
         it[2] = solve_scalar(c, U_n, U, phi_n, dt, D, phi_d, supg, phi);
         if (iters)
             for (int k = 0; k < 3; ++k) iters[k] = it[k];
+    });
+}
+
+int fcfd_four_step_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_n, double dt, double rho,
+                                double mu, const double* U_d, const double* F_p, const double* F, const double* gU,
+                                const double* gP, double* U, double* P, double* gU_n, double* gP_n, int32_t* iters) {
+    if (!ctx || !U_n || !P_n || !U || !P || !gU_n || !gP_n) return FCFD_ERR_ARG;
+    if (U == U_n || P == P_n || gU_n == U_n || gP_n == P_n) return FCFD_ERR_ARG;
+    return run(ctx, [&] {
+        Ctx& c = ctx->c;
+        int it[4];
+        four_step(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+        ns_vjp(c, U_n, P_n, dt, rho, mu, gU, gP, nullptr, gU_n, gP_n, it + 2);
+        if (iters)
+            for (int k = 0; k < 4; ++k) iters[k] = it[k];
+    });
+}
+
+int fcfd_passive_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_n, const double* phi_n, double dt,
+                              double rho, double mu, double D, const double* U_d, const double* phi_d,
+                              const double* F_p, const double* F, int32_t supg, const double* gU, const double* gP,
+                              const double* gphi, double* U, double* P, double* phi, double* gU_n, double* gP_n,
+                              double* gphi_n, int32_t* iters) {
+    if (!ctx || !U_n || !P_n || !phi_n || !U || !P || !phi || !gU_n || !gP_n || !gphi_n) return FCFD_ERR_ARG;
+    if (U == U_n || P == P_n || phi == phi_n || gU_n == U_n || gP_n == P_n || gphi_n == phi_n) return FCFD_ERR_ARG;
+    if (supg) {
+        g_last_error = "the SUPG scalar step has no reverse-mode product";
+        return FCFD_ERR_UNSUPPORTED;
+    }
+    return run(ctx, [&] {
+        Ctx& c = ctx->c;
+        int it[6];
+        four_step(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+        it[2] = solve_scalar(c, U_n, U, phi_n, dt, D, phi_d, 0, phi);
+        passive_vjp(c, U_n, P_n, phi_n, phi, dt, rho, mu, D, gU, gP, gphi, gU_n, gP_n, gphi_n, it + 3);
+        if (iters)
+            for (int k = 0; k < 6; ++k) iters[k] = it[k];
     });
 }
 

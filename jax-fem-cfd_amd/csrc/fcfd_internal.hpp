@@ -21,6 +21,7 @@ struct DevState {
     // BiCGSTAB (jax_iterative_solvers.py:123-167)
     double b_atol2, b_rho, b_rho_new, b_alpha, b_omega, b_beta;
     int b_k, b_active, b_ee, b_maxit;
+    int b_zero, c_zero;  // adjoint solves: the rhs was exactly zero (solution 0)
     // PCG (jax_iterative_solvers.py:87-120)
     double c_atol2, c_gamma, c_alpha, c_beta, c_rr, c_mean_r;
     int c_k, c_active, c_maxit, c_flex;
@@ -62,7 +63,7 @@ struct Ctx {
     hipStream_t stream = nullptr;       // private non-blocking stream every kernel runs on
     hipStream_t user_stream = nullptr;  // the caller's stream (joined with events per call)
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
-    IterGraph g_mom, g_cg, g_phi;
+    IterGraph g_mom, g_cg, g_phi, g_amom, g_aphi;
     bool use_graphs = true;
     Grid2D g;
     double lx, ly;
@@ -113,6 +114,9 @@ struct Ctx {
     uint8_t* phi_mask = nullptr;
     double* phi_ud = nullptr;
     double *ade_A = nullptr, *ade_B = nullptr, *s_x = nullptr;
+    // adjoint (VJP) work: scalar adjoint solution (nn); U cotangent with the scalar terms, the
+    // scalar part of the U_n cotangent, the U_hat cotangent and the momentum adjoint solution (2nn)
+    double *a_sx = nullptr, *a_gut = nullptr, *a_guns = nullptr, *a_gus = nullptr, *a_mx = nullptr;
     std::vector<void*> allocs;
 };
 
@@ -223,11 +227,22 @@ void supg_api(Ctx& c, const double* U_n, const double* Ce_api, double D, double 
 // pressure / multigrid (fcfd_mg.hip)
 void vcycle(Ctx& c, const double* r, double* z);
 int pressure_cg(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged);
+// zero_exit: a right-hand side that is exactly zero ends the solve at once with c_zero set
+// (the solution is 0; adjoint solves only)
+int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
+        bool* converged, bool zero_exit = false);
 int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho,
                    double dt, double* P_out);
 void projection_ustar(Ctx& c, const double* U_hat, const double* P_n, double rho, double dt, double* U_star);
 void projection_final(Ctx& c, const double* U_star, const double* P, double rho, double dt, double* U);
 
+
+// reverse-mode products of the steps (fcfd_adjoint.hip)
+void ns_vjp(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,
+            const double* gP, const double* gUn_extra, double* gU_n, double* gP_n, int* it);
+void passive_vjp(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* phi, double dt,
+                 double rho, double mu, double D, const double* gU, const double* gP, const double* gphi,
+                 double* gU_n, double* gP_n, double* gphi_n, int* it);
 
 // host sync helper: read device ints into the pinned mirror
 void read_state_ints(Ctx& c, const int* dev, int n, int* host);

@@ -560,14 +560,15 @@ struct CgInit {
     }
 };
 struct CgInitFin {
-    DevState* st; double tol; int maxit; int nn;
+    DevState* st; double tol; int maxit; int nn; int zero_exit;
     __device__ void operator()(const double (&t)[3]) const {
         st->c_atol2 = fmax(tol * tol * t[0], 0.0);
+        st->c_zero = zero_exit && t[0] == 0.0;
         st->c_rr = t[1];
         st->mg[0][MG_R] = t[2] / nn;
         st->c_k = 0;
         st->c_maxit = maxit;
-        st->c_active = (t[1] > st->c_atol2) && (0 < maxit);
+        st->c_active = (t[1] > st->c_atol2) && (0 < maxit) && !st->c_zero;
     }
 };
 
@@ -683,8 +684,8 @@ static const void* precondition(Ctx& c, const int* active, bool& is_f32) {
 }
 
 // PCG: returns iterations; x0 in x0in (pins zeroed if zero_pins), solution in c.c_x
-static int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
-               bool* converged) {
+int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
+        bool* converged, bool zero_exit) {
     const Grid2D& g = c.g;
     const int nn = g.nn;
     const int neu = (c.precond == FCFD_PC_MULTIGRID) && !c.set_zeroP;
@@ -692,7 +693,7 @@ static int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in,
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* Le = c.lv[0].ec_dev->Le;
     launch_node<3>(c, nn, nullptr, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
-                                               CgInitFin{c.st, tol, maxit, nn});
+                                               CgInitFin{c.st, tol, maxit, nn, zero_exit});
     const int* act = &c.st->c_active;
     bool f32;
     const void* y0 = precondition(c, act, f32);

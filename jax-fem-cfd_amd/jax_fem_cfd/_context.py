@@ -239,6 +239,49 @@ class Context:
                                                   nat.ptr(phi), iters), "fcfd_passive_timestep")
         return U, P, phi, int(iters[0]), int(iters[1]), int(iters[2])
 
+    def four_step_timestep_vjp(self, U_n, P_n, dt, rho, mu, gU=None, gP=None, U_d=None, F_p=None, F=None):
+        """Reverse-mode product of four_step_timestep: ((gU_n, gP_n), (U, P), iters[4])."""
+        U_n, P_n = nat.as_dev(U_n), nat.as_dev(P_n)
+        Ud = None if U_d is None else nat.as_dev(U_d)
+        gU = None if gU is None else nat.as_dev(gU)
+        gP = None if gP is None else nat.as_dev(gP)
+        Fp, Fv = _opt_vec(F_p, self.nn), _opt_vec(F, 2 * self.nn)
+        U, P = self._empty(2 * self.nn), self._empty(self.nn)
+        gUn, gPn = self._empty(2 * self.nn), self._empty(self.nn)
+        iters = (ctypes.c_int32 * 4)()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_four_step_timestep_vjp(
+            self._h, nat.ptr(U_n), nat.ptr(P_n), float(dt), float(rho), float(mu), nat.ptr(Ud), nat.ptr(Fp),
+            nat.ptr(Fv), nat.ptr(gU), nat.ptr(gP), nat.ptr(U), nat.ptr(P), nat.ptr(gUn), nat.ptr(gPn), iters),
+            "fcfd_four_step_timestep_vjp")
+        return (gUn, gPn), (U, P), tuple(int(i) for i in iters)
+
+    def passive_timestep_vjp(self, U_n, P_n, phi_n, dt, rho, mu, D, gU=None, gP=None, gphi=None, U_d=None,
+                             phi_d=None, F_p=None, F=None, supg=False, out=None):
+        """Reverse-mode product of passive_timestep:
+        ((gU_n, gP_n, gphi_n), (U, P, phi), iters[6]).  `out` = optional 6 preallocated
+        tensors (U, P, phi, gU_n, gP_n, gphi_n); the cotangent outputs may alias gU, gP, gphi."""
+        U_n, P_n, phi_n = nat.as_dev(U_n), nat.as_dev(P_n), nat.as_dev(phi_n)
+        Ud = None if U_d is None else nat.as_dev(U_d)
+        pd = None if phi_d is None else nat.as_dev(phi_d)
+        gU = None if gU is None else nat.as_dev(gU)
+        gP = None if gP is None else nat.as_dev(gP)
+        gphi = None if gphi is None else nat.as_dev(gphi)
+        Fp, Fv = _opt_vec(F_p, self.nn), _opt_vec(F, 2 * self.nn)
+        if out is None:
+            nn = self.nn
+            out = (self._empty(2 * nn), self._empty(nn), self._empty(nn), self._empty(2 * nn), self._empty(nn),
+                   self._empty(nn))
+        U, P, phi, gUn, gPn, gphin = out
+        iters = (ctypes.c_int32 * 6)()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_passive_timestep_vjp(
+            self._h, nat.ptr(U_n), nat.ptr(P_n), nat.ptr(phi_n), float(dt), float(rho), float(mu), float(D),
+            nat.ptr(Ud), nat.ptr(pd), nat.ptr(Fp), nat.ptr(Fv), int(bool(supg)), nat.ptr(gU), nat.ptr(gP),
+            nat.ptr(gphi), nat.ptr(U), nat.ptr(P), nat.ptr(phi), nat.ptr(gUn), nat.ptr(gPn), nat.ptr(gphin), iters),
+            "fcfd_passive_timestep_vjp")
+        return (gUn, gPn, gphin), (U, P, phi), tuple(int(i) for i in iters)
+
     def supg_elem_calc(self, U_n, Ce=None, D=0.0, dt=1.0, r=2.0):
         U_n = nat.as_dev(U_n)
         Ce = None if Ce is None else nat.as_dev(Ce)

@@ -263,7 +263,25 @@ def timestep_loop(t_final, dt, save_interval, print_interval, compute_func, time
     return U, P, U_iter_list, P_iter_list, step_list, U_list
 
 
-def timestep_loop_scan(dt, num_steps, compute_func, timestep_func, rho, mu, D, U_d, phi_d, U0, P0, phi0, S_func):
-    """simulation.py:362-374 (differentiable rollout of the passive-scalar
-    solver).  Not built in this version."""
-    raise NotImplementedError("the differentiable passive-scalar rollout is not built yet")
+def timestep_loop_scan(dt, num_steps, compute_func, timestep_func, rho, mu, D, U_d, phi_d, U0, P0, phi0, S_func,
+                       checkpoint=20):
+    """simulation.py:362-374: F_p, F = compute(U_d, S_func, 0, 0) once, then num_steps
+    passive-scalar steps from (U0, P0, phi0); returns the final (U, P, phi).
+
+    Differentiable w.r.t. U0, P0, phi0 (float64 CUDA tensors with requires_grad) through
+    torch autograd; the backward pass is the hand-written adjoint of each step
+    (fcfd_passive_timestep_vjp) swept last step first, with the states recomputed from
+    checkpoints kept every `checkpoint` steps.
+
+    The reference's scan body hands F_p and phi_d to the step in each other's positions
+    (simulation.py:368); here each goes to its own parameter.  Both agree whenever the
+    reference's call runs at all (numeric phi_d, no scalar Dirichlet nodes, F_p = 0)."""
+    from ..solvers.adjoint import rollout
+    F_p, F = compute_func(U_d, S_func, 0, 0)
+    info = getattr(timestep_func, "rollout_info", None)
+    if info is None:
+        raise TypeError("timestep_func must be the passive-scalar timestep returned by setup()")
+    ctx = timestep_func.context
+    return rollout(ctx, int(num_steps), U0, P0, phi0, dt, rho, mu, D,
+                   U_d if info["has_dirichlet"] else None, phi_d if info["has_phi_dirichlet"] else None,
+                   F_p, F, info["supg"], checkpoint)

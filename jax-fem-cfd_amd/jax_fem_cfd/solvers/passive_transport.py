@@ -46,6 +46,8 @@ def setup_solver(node_coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, gamma_d_
                                     None if gamma_d_phi is None else phi_d, F_p, F, supg)
 
     timestep_loaded.context = ctx
+    timestep_loaded.rollout_info = dict(supg=supg, has_dirichlet=has_dirichlet,
+                                        has_phi_dirichlet=gamma_d_phi is not None)
     compute_loaded.context = ctx
     return compute_loaded, timestep_loaded
 
