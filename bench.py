@@ -121,6 +121,37 @@ print(json.dumps(dict(sec=sec, steps={steps}, its=its)))
     return r
 
 
+def roofline_only(reps):
+    """The dominant kernel alone (for counter runs): one mixed-form timestep to build the FP32
+    Ce, `reps` momentum operator applies, then a plain device copy of known size (the
+    calibration of the HBM byte counters)."""
+    import torch
+    mix = run_form("mixed", "f32", 1, 0)
+    ms = mix["ctx"].bench_momentum_apply(reps)
+    n_copy = 1 << 25  # 32 Mi doubles = 256 MiB read + 256 MiB written
+    x = torch.ones(n_copy, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    ne, nn = N_EL * N_EL, mix["nn"]
+    print(json.dumps({"kernel_us": ms * 1e3, "reps": reps, "bytes_per_launch": 16 * ne * 4 + 8 * nn * 8,
+                      "copy_bytes_each_way": n_copy * 8}))
+
+
+def _traffic_from_profile():
+    """HBM bytes per launch of the roofline kernel from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "momentum_apply_1024_counters.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +159,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="run only the dominant kernel and a calibration copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the timestep is a single-GPU path; "
@@ -138,6 +171,9 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
 
+    if args.roofline_only:
+        roofline_only(20)
+        return
     f64 = run_form("fp64", "f64", args.steps, args.warmup)
     mix = run_form("mixed", "f32", args.steps, args.warmup)
     nn = mix["nn"]
@@ -163,7 +199,7 @@ def main():
 
     steps_s_m = args.steps / mix["sec"]
     steps_s_d = args.steps / f64["sec"]
-    traffic = None
+    traffic = _traffic_from_profile()
     line = {
         "metric": "timesteps/s 1024² Q1: mixed-prec vs FP64 MG-PCG at 1e-8 residual",
         "value": steps_s_m,

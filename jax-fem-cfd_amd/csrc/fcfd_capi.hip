@@ -85,6 +85,57 @@ void upload(const std::vector<T>& h, T* d) {
     FCFD_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
+// Dense inverse of a level's Laplacian with its pinned rows and columns replaced by the identity
+// (the operator the pinned coarse CG inverts: x = 0 at the pins), by Cholesky in FP64.  Returns
+// false if the matrix is not positive definite (no pin on a pure-Neumann level).
+constexpr int COARSE_DENSE_MAX = 1089;  // 33 x 33 nodes: 4.7 MB of FP32
+bool coarse_inverse(const Level& L, std::vector<double>& inv) {
+    const Grid2D& g = L.g;
+    const int n = g.nn;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int e = 0; e < g.ne; ++e) {
+        const int je = e / g.nex, ie = e - je * g.nex;
+        const int ip = g.periodic && ie + 1 == g.nnx ? 0 : ie + 1;
+        const int jp = g.periodic && je + 1 == g.nny ? 0 : je + 1;
+        const int cn[4] = {je * g.nnx + ie, je * g.nnx + ip, jp * g.nnx + ip, jp * g.nnx + ie};
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) A[(size_t)cn[a] * n + cn[b]] += L.ec.Le[a * 4 + b];
+    }
+    for (int64_t p : L.pins) {
+        for (int k = 0; k < n; ++k) A[(size_t)p * n + k] = A[(size_t)k * n + p] = 0.0;
+        A[(size_t)p * n + p] = 1.0;
+    }
+    // A = R^T R (upper R in place of the upper triangle)
+    for (int j = 0; j < n; ++j) {
+        double d = A[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) d -= A[(size_t)k * n + j] * A[(size_t)k * n + j];
+        if (!(d > 0.0)) return false;
+        d = std::sqrt(d);
+        A[(size_t)j * n + j] = d;
+        for (int i = j + 1; i < n; ++i) {
+            double v = A[(size_t)j * n + i];
+            for (int k = 0; k < j; ++k) v -= A[(size_t)k * n + j] * A[(size_t)k * n + i];
+            A[(size_t)j * n + i] = v / d;
+        }
+    }
+    inv.assign((size_t)n * n, 0.0);
+    std::vector<double> z(n);
+    for (int c = 0; c < n; ++c) {
+        // R^T z = e_c, then R x = z
+        for (int i = 0; i < n; ++i) {
+            double v = i == c ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) v -= A[(size_t)k * n + i] * z[k];
+            z[i] = v / A[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double v = z[i];
+            for (int k = i + 1; k < n; ++k) v -= A[(size_t)i * n + k] * inv[(size_t)c * n + k];
+            inv[(size_t)c * n + i] = v / A[(size_t)i * n + i];
+        }
+    }
+    return true;
+}
+
 void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void* stream) {
     FCFD_REQUIRE(m && s, "null descriptor");
     FCFD_REQUIRE(m->ndim == 2, "only 2D meshes are supported by this build");
@@ -191,6 +242,16 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
             c.cc_r32 = dalloc<float>(c, ncoarse);
             c.cc_p32 = dalloc<float>(c, ncoarse);
             c.cc_Ap32 = dalloc<float>(c, ncoarse);
+        }
+        // mixed form, pinned pressure: the coarse solve is the exact inverse of the pinned
+        // coarse operator (the FP64 form keeps the reference's CG to tol)
+        if (mixed && c.set_zeroP && nlev > 1 && ncoarse <= COARSE_DENSE_MAX) {
+            std::vector<double> inv;
+            if (coarse_inverse(c.lv[nlev - 1], inv)) {
+                std::vector<float> inv32(inv.begin(), inv.end());
+                c.coarse_inv32 = dalloc<float>(c, inv32.size());
+                upload(inv32, c.coarse_inv32);
+            }
         }
     }
 

@@ -101,6 +101,7 @@ struct Ctx {
     // coarse CG scratch (coarsest level)
     double *cc_r = nullptr, *cc_p = nullptr, *cc_Ap = nullptr;
     float *cc_r32 = nullptr, *cc_p32 = nullptr, *cc_Ap32 = nullptr;
+    float* coarse_inv32 = nullptr;  // mixed form, pinned: dense inverse of the coarse operator
     // reductions
     double* partials = nullptr;
     int pstride = 0;            // partials per reduction slot (max blocks of a reducing kernel)

@@ -134,6 +134,53 @@ __device__ __forceinline__ T interp_at(const Grid2D& gf, const Grid2D& gc, const
     return T(0.5) * r0 + T(0.5) * r1;
 }
 
+// interp_at at all 9 stencil points of a node from one 3x3 patch of coarse values: fine index f
+// interpolates coarse (f >> 1, (f + 1) >> 1), so fine f - 1 .. f + 1 touch coarse
+// floor((f - 1) / 2) + {0, 1, 2}.  Which patch entries pair up depends only on the parity of the
+// node's (i, j): selected per lane, with the same operands and operation order as interp_at, so
+// every value is bitwise the one interp_at returns.  9 coarse loads instead of 36.
+template <class T>
+__device__ __forceinline__ void interp3x3(const Grid2D& gc, const T* yc, T my, const NodeNbr& s, T (&Y)[3][3]) {
+    const int bx = (s.i - 1) >> 1, by = (s.j - 1) >> 1;  // arithmetic shift: floor
+    int cx[3], cy[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        int x = bx + t, y = by + t;
+        if (gc.periodic) {
+            x = x < 0 ? x + gc.nnx : (x >= gc.nnx ? x - gc.nnx : x);
+            y = y < 0 ? y + gc.nny : (y >= gc.nny ? y - gc.nny : y);
+        } else {
+            x = x < 0 ? 0 : (x >= gc.nnx ? gc.nnx - 1 : x);
+            y = y < 0 ? 0 : (y >= gc.nny ? gc.nny - 1 : y);
+        }
+        cx[t] = x;
+        cy[t] = y;
+    }
+    T P[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) P[a][b] = yc[cy[a] * gc.nnx + cx[b]] - my;
+    // pair (first, second) of patch columns for fine offsets -1, 0, +1:
+    //   even i: (0,1) (1,1) (1,2)      odd i: (0,0) (0,1) (1,1)
+    const bool ox = s.i & 1, oy = s.j & 1;
+    T H[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T p0 = P[a][0], p1 = P[a][1], p2 = P[a][2];
+        H[a][0] = T(0.5) * p0 + T(0.5) * (ox ? p0 : p1);
+        H[a][1] = T(0.5) * (ox ? p0 : p1) + T(0.5) * p1;
+        H[a][2] = T(0.5) * p1 + T(0.5) * (ox ? p1 : p2);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const T h0 = H[0][b], h1 = H[1][b], h2 = H[2][b];
+        Y[0][b] = T(0.5) * h0 + T(0.5) * (oy ? h0 : h1);
+        Y[1][b] = T(0.5) * (oy ? h0 : h1) + T(0.5) * h1;
+        Y[2][b] = T(0.5) * h1 + T(0.5) * (oy ? h1 : h2);
+    }
+}
+
 template <class T>
 struct Prolong {
     Grid2D gf, gc; const T* yc; T* p; const DevState* st; int l;
@@ -160,10 +207,16 @@ struct Jacobi {
             const T oup = l == 0 ? T(0) : (T)st->mg[l][MG_R];
             rh = (T)r[n] - oup;
         } else {
-            if (pin)
-                load3x3(s, [&](int k) { const T v = x[k] + interp_at(gf, gc, yc, T(0), k); return pin[k] ? T(0) : v; }, X);
-            else
-                load3x3(s, [&](int k) { return x[k] + interp_at(gf, gc, yc, T(0), k); }, X);
+            T Y[3][3];
+            interp3x3(gc, yc, T(0), s, Y);
+            load3x3(s, [&](int k) { return x[k]; }, X);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const bool keep = s.ok[a][b] && !(pin && pin[s.nb[a][b]]);
+                    X[a][b] = keep ? X[a][b] + Y[a][b] : T(0);
+                }
             rh = (T)r[n];
         }
         const T off = ebe_const(s, Leoff, X);
@@ -445,18 +498,54 @@ static void coarse_solve(Ctx& c, const TR* r, T* y, T* cr, T* cp, T* cAp, double
     FCFD_CHECK_LAUNCH();
 }
 
+// ------------------------------------------------------------------------------ dense coarse solve
+// Mixed form, pinned pressure: y = Ainv b with b = r zeroed at the pins, Ainv the exact inverse
+// of the pinned coarse operator (FP32 entries, FP64 accumulation).  Each wave computes
+// DENSE_ROWS rows: its lanes stride the columns (independent coalesced loads), then a DPP wave
+// sum per row.  ceil(nn / (4 DENSE_ROWS)) blocks of 4 waves.
+constexpr int DENSE_ROWS = 2;
+template <class T>
+__global__ __launch_bounds__(256) void k_coarse_dense(int nn, const float* __restrict__ Ainv,
+                                                      const uint8_t* __restrict__ pin, const T* __restrict__ r,
+                                                      T* __restrict__ y, const int* active) {
+    if (active && *active == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * DENSE_ROWS;
+    double acc[DENSE_ROWS];
+#pragma unroll
+    for (int m = 0; m < DENSE_ROWS; ++m) acc[m] = 0.0;
+#pragma unroll 6
+    for (int j = lane; j < nn; j += 64) {
+        const double bj = (pin && pin[j]) ? 0.0 : (double)r[j];
+#pragma unroll
+        for (int m = 0; m < DENSE_ROWS; ++m) {
+            const int i = row0 + m < nn ? row0 + m : nn - 1;
+            acc[m] += (double)Ainv[(size_t)i * nn + j] * bj;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < DENSE_ROWS; ++m) {
+        const double v = wave_sum_dpp(acc[m]);
+        const int i = row0 + m;
+        if (lane == 0 && i < nn) y[i] = (pin && pin[i]) ? T(0) : (T)v;
+    }
+}
+
 // ------------------------------------------------------------------------------ V-cycle driver
 template <class T>
 static void vcycle_t(Ctx& c, const double* r0, const int* active) {
     using B = LvBufs<T>;
     const int N = c.nlev;
     const int neu = !c.set_zeroP;
+    const bool f32 = std::is_same<T, float>::value;
+    const double ctol = f32 ? (c.tol > 1e-5 ? c.tol : 1e-5) : c.tol;
+    const int ld = N - 1;  // levels [0, ld) above the coarse level
     // pinned form: no mean projections, so its kernels carry no reduction
     auto run = [&](int n, const auto& body, const MeanFin& fin) {
         if (neu) launch_node<1>(c, n, active, body, fin);
         else launch_node<0>(c, n, active, body, fin);
     };
-    for (int l = 0; l < N - 1; ++l) {
+    for (int l = 0; l < ld; ++l) {
         Level& L = c.lv[l];
         Level& Lc = c.lv[l + 1];
         const int n = L.g.nn;
@@ -478,13 +567,17 @@ static void vcycle_t(Ctx& c, const double* r0, const int* active) {
     }
     {
         Level& C = c.lv[N - 1];
-        const double ctol = std::is_same<T, float>::value ? (c.tol > 1e-5 ? c.tol : 1e-5) : c.tol;
-        if (N == 1)
+        if (f32 && c.coarse_inv32 && N > 1) {
+            const int nb = (C.g.nn + 4 * DENSE_ROWS - 1) / (4 * DENSE_ROWS);
+            k_coarse_dense<T><<<nb, 256, 0, c.stream>>>(C.g.nn, c.coarse_inv32, C.pin, B::r(C), B::y(C), active);
+            FCFD_CHECK_LAUNCH();
+        } else if (N == 1) {
             coarse_solve<T, double>(c, r0, B::y(C), B::cr(c), B::cp(c), B::cAp(c), ctol, active);
-        else
+        } else {
             coarse_solve<T, T>(c, B::r(C), B::y(C), B::cr(c), B::cp(c), B::cAp(c), ctol, active);
+        }
     }
-    for (int l = N - 2; l >= 0; --l) {
+    for (int l = ld - 1; l >= 0; --l) {
         Level& L = c.lv[l];
         Level& Lc = c.lv[l + 1];
         const int n = L.g.nn;

@@ -149,6 +149,26 @@ def test_vcycle(periodic, set_zeroP):
     assert _rel(got, ref) < 1e-12
 
 
+@pytest.mark.parametrize("set_zeroP,nf", [(True, 4), (True, 16), (False, 8)])
+def test_vcycle_mixed_form(set_zeroP, nf):
+    """FP32 V-cycle of the mixed form against the oracle's FP64 V-cycle.  Pinned: the coarse
+    solve is the exact (dense, FP32) inverse of the pinned coarse operator, the oracle's is CG to
+    1e-12 -- both solve the coarse system, so only FP32 rounding separates them (2e-5)."""
+    n = 64
+    s, *_ = ns.build([n, n], [1.0, 1.0], False, [], [], 1e-12, set_zeroP, "multigrid", [nf, nf])
+    c = _ctx(n, n, 1.0, 1.0, False, tol=1e-12, set_zeroP=set_zeroP, precond="multigrid", mg_levels=len(s.levels),
+             precision="mixed", gamma_p=[0] if set_zeroP else None,
+             mg_gamma_p=[L["gamma_p"] for L in s.levels])
+    r = RNG.standard_normal(s.nn)
+    if set_zeroP:
+        r[0] = 0
+    else:
+        r -= r.mean()
+    ref = s.M_op(r)
+    got = _np(c.vcycle(r))
+    assert _rel(got, ref) < 2e-5, _rel(got, ref)
+
+
 def test_vcycle_api_tuple():
     from jax_fem_cfd.core import multigrid as mg
     from jax_fem_cfd.core import shape_functions as sf
