@@ -92,6 +92,86 @@ def poisson_alone(precision, solves=3):
     return dict(solves_per_s=1.0 / sec, ms_per_solve=sec * 1e3, iters=k, converged=conv, true_rel_residual=res)
 
 
+def fourier_velocity(coords, seed, kmax=8):
+    """Divergence-free velocity whose modes |k| <= kmax have amplitudes N(0, 1) / |k|^2 (seeded):
+    stream function psi = sum_{m, n >= 1, m^2 + n^2 <= kmax^2} a_mn sin(m pi x) sin(n pi y) / (pi |k|^3),
+    a_mn ~ N(0, 1), |k| = sqrt(m^2 + n^2); U = (d psi / dy, -d psi / dx), so each mode's velocity
+    amplitude is a_mn (n or m) / |k|^3 <= |a_mn| / |k|^2."""
+    x, y = coords[:, 0], coords[:, 1]
+    rng = np.random.default_rng(seed)
+    u = np.zeros_like(x)
+    v = np.zeros_like(x)
+    for m in range(1, kmax + 1):
+        for n in range(1, kmax + 1):
+            k2 = m * m + n * n
+            if k2 > kmax * kmax:
+                continue
+            a = rng.standard_normal() / (np.pi * k2 ** 1.5)
+            u += a * n * np.pi * np.sin(m * np.pi * x) * np.cos(n * np.pi * y)
+            v -= a * m * np.pi * np.cos(m * np.pi * x) * np.sin(n * np.pi * y)
+    return np.concatenate([u, v])
+
+
+def gradient_256(steps=200, checkpoint=20):
+    """Config 5: 256^2 walled unit box, rho = 1, mu = 1e-3, kappa = 1e-4, dt = 1e-3, plain scalar
+    step, MG 256^2 -> 16^2, solves to 1e-10.  phi0 = 1 in a seeded rectangle (seed 2), 0 elsewhere;
+    the target is the solver's phi after `steps` steps from fourier_velocity(seed 5); the
+    gradient is taken at U0 = fourier_velocity(seed 6) (walls set to 0) of
+    L = 1/2 |phi_steps - phi_target|^2 (max |U0| ~ 0.5: CFL ~ 0.1 at dt = 1e-3, h = 1/256), by the
+    hand-written adjoint with checkpoints every
+    `checkpoint` steps.  Reported: wall time of one value-and-gradient, and the central finite
+    difference of L along one random direction (seed 7) against <grad, d>."""
+    import torch
+    import jax_fem_cfd.setup.simulation as sim
+    n, dt, rho, mu, D = 256, 1e-3, 1.0, 1e-3, 1e-4
+    cfg = sim.Config(mesh="simple", solver="passive_transport",
+                     mesh_config=sim.SimpleMeshConfig([n, n], [1.0, 1.0], 1, False),
+                     solver_config=sim.PassiveTransportConfig(
+                         sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, 1e-10, True, "multigrid", [16, 16]),
+                         sim.ADEConfig([], False)))
+    compute, timestep, coords, surf, h, nn, nnx, nny, gamma_d, U_d, gdp = sim.setup(cfg)
+    rng = np.random.default_rng(2)
+    x0, y0 = rng.uniform(0.1, 0.5, 2)
+    w, hgt = rng.uniform(0.2, 0.4, 2)
+    x, y = coords[:, 0], coords[:, 1]
+    phi0 = ((x >= x0) & (x <= x0 + w) & (y >= y0) & (y <= y0 + hgt)).astype(float)
+    dev = torch.device("cuda", 0)
+    P0 = torch.zeros(nn, dtype=torch.float64, device=dev)
+    phi0t = torch.tensor(phi0, dtype=torch.float64, device=dev)
+
+    def vel(seed):
+        U = fourier_velocity(coords, seed)
+        U[gamma_d] = U_d
+        return U
+
+    def run(U):
+        return sim.timestep_loop_scan(dt, steps, compute, timestep, rho, mu, D, U_d, None, U, P0, phi0t, None,
+                                      checkpoint=checkpoint)[2]
+
+    with torch.no_grad():
+        target = run(torch.tensor(vel(5), dtype=torch.float64, device=dev))
+    U0 = torch.tensor(vel(6), dtype=torch.float64, device=dev, requires_grad=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = 0.5 * torch.sum((run(U0) - target) ** 2)
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = U0.grad.detach()
+    d = torch.tensor(np.random.default_rng(7).standard_normal(2 * nn), dtype=torch.float64, device=dev)
+    eps = 1e-4
+    with torch.no_grad():
+        lp = 0.5 * torch.sum((run(U0.detach() + eps * d) - target) ** 2).item()
+        lm = 0.5 * torch.sum((run(U0.detach() - eps * d) - target) ** 2).item()
+    fd = (lp - lm) / (2 * eps)
+    an = float(torch.dot(g, d))
+    return {"grid": "256x256 Q1", "steps": steps, "checkpoint": checkpoint, "loss": loss.item(),
+            "seconds_value_and_grad": t2 - t0, "seconds_forward": t1 - t0, "seconds_backward": t2 - t1,
+            "fd_directional": fd, "adjoint_directional": an, "fd_rel_err": abs(fd - an) / max(abs(an), 1e-300),
+            "params": 2 * nn}
+
+
 def cpu_baseline(steps=2):
     """Oracle (NumPy FP64 restatement) on this host, one thread, same workload,
     bounded sample of `steps` timesteps after set-up."""
@@ -159,6 +239,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-grad", action="store_true", help="skip the 256^2 adjoint-gradient measurement")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant kernel and a calibration copy (for counter runs)")
     args = ap.parse_args()
@@ -195,6 +276,7 @@ def main():
 
     pois64 = poisson_alone("fp64")
     poism = poisson_alone("mixed")
+    grad = None if args.no_grad else gradient_256()
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps)
 
     steps_s_m = args.steps / mix["sec"]
@@ -224,6 +306,7 @@ def main():
         "speedup_mixed_vs_fp64": steps_s_m / steps_s_d,
         "mixed_vs_fp64_rel_l2": {"u": rel(Um[:nn], U64[:nn]), "v": rel(Um[nn:], U64[nn:]), "p": rel(Pm, P64)},
         "poisson_1024_neumann": {"fp64": pois64, "mixed": poism},
+        "gradient_256": grad,
         "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_m / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "node_kernel<3, BK2<2, MomCoef<CeF32>, 1, false>> (BiCGSTAB s/t: momentum operator apply + 3 dots)",

@@ -377,3 +377,52 @@ def test_supg_element_calc_api():
     tau = _np(ec.calculate_tau(nconn, U, Ce, Cto, N, nn, 2, 1e-4, 1e-3, 2))
     tauo = oel.calculate_tau(nconn, U, Ce, Cto, N, nn, 2, 1e-4, 1e-3, 2)
     assert _rel(tau, tauo) < 1e-13
+
+
+# ----------------------------------------------------------------------------------------- host loop
+def test_timestep_loop_matches_oracle_loop():
+    """timestep_loop (simulation.py:229-360): step count rule, compute -> timestep per step,
+    iteration counts and solutions recorded on save steps and the last step."""
+    import jax_fem_cfd.setup.simulation as sim
+    n, dt = 16, 5e-3
+    cfg = sim.Config(mesh="simple", solver="standard", mesh_config=sim.SimpleMeshConfig([n, n], [1.0, 1.0], 1, False),
+                     solver_config=sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, 1e-10, True, "multigrid",
+                                                            [4, 4]))
+    compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
+    U_d = sim.set_face_velocity(U_d, surf[2][1:-1], gamma_d, 1.0, nn, 1)
+    U0 = np.random.default_rng(3).normal(0, 1e-2, 2 * nn)
+    U0[gamma_d] = U_d
+    t_final = 12 * dt
+    U, P, ui, pi, steps, U_list = sim.timestep_loop(t_final, dt, 5, 100, compute, timestep, [1.0, 1e-2], [U_d],
+                                                    [U0, np.zeros(nn)])
+    s, osurf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-10, True, "multigrid", [4, 4])
+    oUd = ns.cavity_lid(s, osurf, oUd)
+    oU, oP, oui, opi, osteps, oU_list = ns.timestep_loop(s, t_final, dt, 5, 1.0, 1e-2, oUd, U0, np.zeros(nn))
+    assert np.array_equal(steps, osteps) and np.array_equal(ui, oui) and np.array_equal(pi, opi)
+    assert U_list.shape == oU_list.shape
+    assert _rel(U_list, oU_list) < 1e-10 and _rel(_np(P), oP) < 1e-10
+
+
+def test_timestep_loop_passive_returns_reference_tuple():
+    import jax_fem_cfd.setup.simulation as sim
+    n, dt, rho, mu, D = 16, 5e-3, 1.0, 1e-2, 1e-3
+    cfg = sim.Config(mesh="simple", solver="passive_transport",
+                     mesh_config=sim.SimpleMeshConfig([n, n], [1.0, 1.0], 1, False),
+                     solver_config=sim.PassiveTransportConfig(
+                         sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, 1e-10, True, "multigrid", [4, 4]),
+                         sim.ADEConfig([], False)))
+    compute, timestep, coords, surf, h, nn, nnx, nny, gamma_d, U_d, gamma_d_phi = sim.setup(cfg)
+    U_d = sim.set_face_velocity(U_d, surf[2][1:-1], gamma_d, 1.0, nn, 1)
+    U0 = np.zeros(2 * nn)
+    U0[gamma_d] = U_d
+    phi0 = _rect_phi(coords)
+    out = sim.timestep_loop(7 * dt, dt, 3, 100, compute, timestep, [rho, mu, D], [U_d, None],
+                            [U0, np.zeros(nn), phi0])
+    U, P, phi, ui, pi, fi, steps, U_list, phi_list = out
+    assert list(steps) == [3, 6, 7] and len(fi) == 3 and phi_list.shape == (4, nn) and U_list.shape == (4, 2 * nn)
+    s, osurf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-10, True, "multigrid", [4, 4])
+    oUd = ns.cavity_lid(s, osurf, oUd)
+    oU, oP, ophi = U0.copy(), np.zeros(nn), phi0.copy()
+    for _ in range(7):
+        oU, oP, ophi, *_ = ns.passive_timestep(s, oU, oP, ophi, dt, rho, mu, D, oUd, None, 0, 0, None, False)
+    assert _rel(_np(phi), ophi) < 1e-10 and _rel(phi_list[-1], ophi) < 1e-10
