@@ -157,8 +157,9 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     c.ce_format = s->ce_format;
     c.maxit_mom = s->maxiter_momentum;
     c.maxit_p = s->maxiter_pressure;
-    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_MULTIGRID,
-                 "pressure preconditioner not supported (approx_inv is not built)");
+    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV ||
+                     c.precond == FCFD_PC_MULTIGRID,
+                 "unknown pressure preconditioner");
     const Grid2D& g = c.g;
     const int nn = g.nn;
     quad4_constants(c.lx / g.nex, c.ly / g.ney, c.ec);
@@ -255,8 +256,9 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
         }
     }
 
-    // fine-level pressure pins
-    if (c.set_zeroP) {
+    // fine-level pressure pins (the approximate-inverse preconditioner zeroes the gamma_p rows of
+    // its two inner products in both forms, standard.py:225-234)
+    if (c.set_zeroP || c.precond == FCFD_PC_APPROX_INV) {
         std::vector<uint8_t> pin(nn, 0);
         if (s->gamma_p && s->n_gamma_p > 0) {
             for (int64_t k = 0; k < s->n_gamma_p; ++k) {

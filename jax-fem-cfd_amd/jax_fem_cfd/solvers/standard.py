@@ -47,8 +47,6 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
     sc = ns_config if ns_config is not None else config.solver_config
     if len(mc.num_elem) != 2 or mc.shape_func_ord != 1:
         raise NotImplementedError("the GPU timestep is built for 2D Q1 meshes")
-    if sc.pressure_precond == "approx_inv":
-        raise NotImplementedError("the approx_inv pressure preconditioner is not built")
     nex, ney = mc.num_elem
     nlev = _mg_levels(mc.num_elem, sc.final_multigrid_mesh) if sc.pressure_precond == "multigrid" else 1
     mg_pins = None
@@ -57,7 +55,8 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
     ce_format = sc.ce_format or ("f64" if sc.precision == "fp64" else "f32")
     return Context(nex, ney, mc.domain_size[0], mc.domain_size[1], mc.periodic, tol=sc.solver_tol,
                    set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=nlev, precision=sc.precision,
-                   ce_format=ce_format, gamma_d=gamma_d, gamma_p=np.atleast_1d(gamma_p) if sc.set_zeroP else None,
+                   ce_format=ce_format, gamma_d=gamma_d,
+                   gamma_p=np.atleast_1d(gamma_p) if (sc.set_zeroP or sc.pressure_precond == "approx_inv") else None,
                    mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, gamma_d_phi=gamma_d_phi)
 
 

@@ -207,6 +207,55 @@ def test_pressure_cg_multigrid(set_zeroP):
     assert _rel(_np(x), xr) < 1e-10
 
 
+@pytest.mark.parametrize("set_zeroP", [True, False])
+def test_pressure_cg_approx_inv(set_zeroP):
+    """PCG with the two-term approximate inverse (standard.py:225-234); its gamma_p rows (node 0
+    without outlets) are zeroed in both the pinned and the pure-Neumann form, as the reference's."""
+    n = 48
+    s, *_ = ns.build([n, n], [1.0, 1.0], False, [], [], 1e-10, set_zeroP, "approx_inv", None)
+    c = _ctx(n, n, 1.0, 1.0, False, tol=1e-10, set_zeroP=set_zeroP, precond="approx_inv", gamma_p=[0])
+    b = RNG.standard_normal(s.nn)
+    if set_zeroP:
+        b[0] = 0
+    else:
+        b -= b.mean()
+
+    def A(x):
+        y = oem.laplacian_element_mult(x, s.Le, s.nconn, s.nn)
+        if set_zeroP:
+            y[0] = 0
+        return y
+
+    from oracle.solvers import cg
+    xr, kr = cg(A, b, tol=1e-10, maxiter=1000, M=s.M_op)
+    x, k, conv = c.pressure_cg(b, tol=1e-10, maxiter=1000)
+    assert k == kr and conv == (kr < 1000)
+    assert _rel(_np(x), xr) < 1e-10
+
+
+def test_cavity_approx_inv_steps_match_oracle():
+    """The full step with the approx_inv pressure preconditioner (cavity 32x32, 3 steps)."""
+    import jax_fem_cfd.setup.simulation as sim
+    n = 32
+    cfg = sim.Config(mesh="simple", solver="standard", mesh_config=sim.SimpleMeshConfig([n, n], [1.0, 1.0], 1, False),
+                     solver_config=sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, 1e-10, True, "approx_inv",
+                                                            [4, 4]))
+    compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
+    U_d = sim.set_face_velocity(U_d, surf[2][1:-1], gamma_d, 1.0, nn, 1)
+    s, osurf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-10, True, "approx_inv", None)
+    oUd = ns.cavity_lid(s, osurf, oUd)
+    U = np.zeros(2 * nn)
+    U[gamma_d] = U_d
+    P = np.zeros(nn)
+    Uo, Po = U.copy(), P.copy()
+    for _ in range(3):
+        F_p, F = compute(U_d, None, 0, 0)
+        U, P, ui, pi = timestep(U, P, 2e-3, 1.0, 1e-2, U_d, F_p, F)
+        Uo, Po, uio, pio = ns.four_step_timestep(s, Uo, Po, 2e-3, 1.0, 1e-2, oUd, 0, 0)
+        assert (ui, pi) == (uio, pio)
+    assert _rel(_np(U), Uo) < 1e-10 and _rel(_np(P), Po) < 1e-10
+
+
 def test_generic_solver_wrappers():
     from jax_fem_cfd.utils import jax_iterative_solvers as jis
     from oracle import solvers as osv
