@@ -309,7 +309,7 @@ def main():
         "gradient_256": grad,
         "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_m / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "node_kernel<3, BK2<2, MomCoef<CeF32>, 1, false>> (BiCGSTAB s/t: momentum operator apply + 3 dots)",
+                     "kernel": "stencil_kernel<3, BK2S<2, MomCoef<CeF32>, 1, false>> (BiCGSTAB s/t: LDS-staged momentum operator apply + 3 dots)",
                      "bytes_per_launch": bytes_m, "us_per_launch": ms_m * 1e3,
                      "fp64_ce": {"achieved": ach_d, "bytes_per_launch": bytes_d, "us_per_launch": ms_d * 1e3}},
         "cpu_baseline": None if cpu is None else (

@@ -253,6 +253,104 @@ struct BK2Fin {
     }
 };
 
+// ------------------------------------------------------------------------------ K1 / K2, LDS-staged
+// Non-periodic grids: the same recurrences with the on-the-fly vector (p, s) staged per block.
+// MT: the staged values are the masked ones (the transposed operator's input); the raw centre
+// value comes from global memory.
+template <int NC, class LOp, bool MT = false>
+struct BK1S {
+    static constexpr int NV = NC;
+    LOp op;
+    const uint8_t* mask;
+    const double *r, *pold, *qold, *rhat;
+    double *pnew, *qnew;
+    const DevState* st;
+    __device__ __forceinline__ int nnx() const { return op.g.nnx; }
+    __device__ __forceinline__ double pval(int i) const {
+        return r[i] + st->b_beta * (pold[i] - st->b_omega * qold[i]);
+    }
+    __device__ __forceinline__ void stage(int id, double (&v)[NV]) const {
+        const int nn = op.g.nn;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = id + c * nn;
+            const double pv = pval(i);
+            v[c] = (MT && mask && mask[i]) ? 0.0 : pv;
+        }
+    }
+    __device__ void operator()(int n, const StageView<NV>& sv, double (&acc)[1]) const {
+        const int nn = op.g.nn;
+        NodeNbr s;
+        node_elems(op.g, n, s);
+        double X[NC][3][3];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) X[c][a][b] = sv.at(c, n, a - 1, b - 1);
+        double y[NC];
+        op_apply<NC>(op, s, X, y);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = n + c * nn;
+            const double pc = MT ? pval(i) : X[c][1][1];
+            const bool m = mask && mask[i];
+            const double yc = MT ? y[c] + (m ? pc : 0.0) : (m ? pc : y[c]);
+            pnew[i] = pc;
+            qnew[i] = yc;
+            acc[0] += rhat[i] * yc;
+        }
+    }
+};
+
+template <int NC, class LOp, int TAG = 0, bool MT = false>
+struct BK2S {
+    static constexpr int NV = NC;
+    LOp op;
+    const uint8_t* mask;
+    const double *r, *q;
+    double *sv, *tv;
+    const DevState* st;
+    __device__ __forceinline__ int nnx() const { return op.g.nnx; }
+    __device__ __forceinline__ double sval(int i) const { return r[i] - st->b_alpha * q[i]; }
+    __device__ __forceinline__ void stage(int id, double (&v)[NV]) const {
+        const int nn = op.g.nn;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = id + c * nn;
+            const double s = sval(i);
+            v[c] = (MT && mask && mask[i]) ? 0.0 : s;
+        }
+    }
+    __device__ void operator()(int n, const StageView<NV>& sw, double (&acc)[3]) const {
+        const int nn = op.g.nn;
+        NodeNbr s;
+        node_elems(op.g, n, s);
+        double X[NC][3][3];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) X[c][a][b] = sw.at(c, n, a - 1, b - 1);
+        double y[NC];
+        op_apply<NC>(op, s, X, y);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = n + c * nn;
+            const double sc = MT ? sval(i) : X[c][1][1];
+            const bool m = mask && mask[i];
+            const double yc = MT ? y[c] + (m ? sc : 0.0) : (m ? sc : y[c]);
+            sv[i] = sc;
+            tv[i] = yc;
+            acc[0] += sc * sc;
+            acc[1] += yc * sc;
+            acc[2] += yc * yc;
+        }
+    }
+};
+
 // ------------------------------------------------------------------------------ K3
 template <int NC>
 struct BK3 {
@@ -363,12 +461,22 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_node<1>(c, nn, act,
-                           BK1<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw],
-                                            c.st},
-                           BK1Fin{c.st});
-            launch_node<3>(c, nn, act, BK2<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
-                           BK2Fin{c.st});
+            if (c.g.periodic) {
+                launch_node<1>(c, nn, act,
+                               BK1<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
+                                                c.m_q[nw], c.st},
+                               BK1Fin{c.st});
+                launch_node<3>(c, nn, act, BK2<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                               BK2Fin{c.st});
+            } else {
+                launch_stencil<1>(c, nn, act,
+                                  BK1S<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
+                                                    c.m_q[nw], c.st},
+                                  BK1Fin{c.st});
+                launch_stencil<3>(c, nn, act,
+                                  BK2S<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                                  BK2Fin{c.st});
+            }
             launch_node<2>(c, nn, act, BK3<NC>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, x, c.m_r, c.st}, BK3Fin{c.st});
         }
         FCFD_CHECK_LAUNCH();

@@ -157,6 +157,16 @@ inline void launch_node(Ctx& c, int n, const int* active, const Body& body, cons
         finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, fin, c.partials, c.pstride);
 }
 
+// Same for an LDS-staged stencil kernel (non-periodic grids).
+template <int NR, class Body, class Fin>
+inline void launch_stencil(Ctx& c, int n, const int* active, const Body& body, const Fin& fin) {
+    const int nb = blocks_for(n);
+    if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+    stencil_kernel<NR><<<nb, BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
+    if constexpr (NR > 0)
+        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, fin, c.partials, c.pstride);
+}
+
 // join the private stream to the caller's stream on entry / exit of an API call
 void enter(Ctx& c);
 void leave(Ctx& c);

@@ -68,9 +68,14 @@ static float bench_t(Ctx& c, S ce, int reps) {
     FCFD_HIP(hipEventCreate(&e0));
     FCFD_HIP(hipEventCreate(&e1));
     auto launch = [&]() {
-        node_kernel<3><<<nb, BLOCK, 0, c.stream>>>(
-            c.g.nn, nullptr, BK2<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st}, c.partials,
-            c.pstride);
+        if (c.g.periodic)
+            node_kernel<3><<<nb, BLOCK, 0, c.stream>>>(
+                c.g.nn, nullptr, BK2<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st},
+                c.partials, c.pstride);
+        else
+            stencil_kernel<3><<<nb, BLOCK, 0, c.stream>>>(
+                c.g.nn, nullptr, BK2S<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st},
+                c.partials, c.pstride);
     };
     launch();
     FCFD_HIP(hipEventRecord(e0, c.stream));
