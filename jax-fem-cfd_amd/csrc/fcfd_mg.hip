@@ -89,6 +89,45 @@ struct DownRes {
     }
 };
 
+// LDS-staged form (non-periodic grids): the smoothed x (pinned: 2/3 r / L_diag, 0 at the pins;
+// Neumann: x - mean x) is staged once per node of the block window instead of being formed at
+// each of the 9 stencil points; same values bit for bit.
+template <class T, class TR>
+struct DownResS {
+    static constexpr int NV = 1;
+    Grid2D g; const double* Le; const TR* r; const T* Ld; const uint8_t* pin; T* x; T* f;
+    const DevState* st; int l; int neumann;
+    __device__ __forceinline__ int nnx() const { return g.nnx; }
+    __device__ __forceinline__ T xval(int k) const {
+        if (neumann) return x[k] - (T)st->mg[l][MG_X];
+        const T v = (T(2.0 / 3.0) * (T)r[k]) / Ld[k];
+        return (pin && pin[k]) ? T(0) : v;
+    }
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = (double)xval(id); }
+    __device__ void operator()(int n, const StageView<1>& sv, double (&acc)[1]) const {
+        NodeNbr s;
+        node_elems(g, n, s);
+        T X[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) X[a][b] = (T)sv.at(0, n, a - 1, b - 1);
+        T fv;
+        if (neumann) {
+            const T mr = (T)st->mg[l][MG_R];
+            fv = ((T)r[n] - mr) - ebe_const(s, Le, X);
+        } else {
+            x[n] = X[1][1];
+            const bool pn = pin && pin[n];
+            const T rh = pn ? T(0) : (T)r[n];
+            const T lx = pn ? T(0) : ebe_const(s, Le, X);
+            fv = rh - lx;
+        }
+        f[n] = fv;
+        acc[0] += (double)fv;
+    }
+};
+
 // coarse[I] = R (f - mean f): R = I^T without the 1/4 factor (multigrid.py:142-212);
 // missing neighbours (domain edges) contribute zero.  Periodic grids restrict the padded fine
 // grid with the same formulas and drop the wrap row/column, so only negative fine indices can
@@ -553,8 +592,19 @@ static void vcycle_t(Ctx& c, const double* r0, const int* active) {
             if (neu)
                 launch_node<1>(c, n, active, DownX<T, double>{r0, B::Ld(L), B::x(L), c.st, l},
                                MeanFin{&c.st->mg[l][MG_X], nullptr, n});
-            run(n, DownRes<T, double>{L.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::f(L), c.st, l, neu},
-                MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+            if (L.g.periodic)
+                run(n, DownRes<T, double>{L.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::f(L), c.st, l, neu},
+                    MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+            else if (neu)
+                launch_stencil<1>(c, n, active,
+                                  DownResS<T, double>{L.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::f(L), c.st,
+                                                      l, neu},
+                                  MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+            else
+                launch_stencil<0>(c, n, active,
+                                  DownResS<T, double>{L.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::f(L), c.st,
+                                                      l, neu},
+                                  MeanFin{&c.st->mg[l][MG_F], nullptr, n});
         } else {
             if (neu)
                 launch_node<1>(c, n, active, DownX<T, T>{B::r(L), B::Ld(L), B::x(L), c.st, l},
@@ -700,6 +750,27 @@ struct CgK1 {
         acc[0] += X[1][1] * a;
     }
 };
+// LDS-staged form (non-periodic grids): p staged once per block window
+struct CgK1S {
+    static constexpr int NV = 1;
+    Grid2D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
+    const DevState* st;
+    __device__ __forceinline__ int nnx() const { return g.nnx; }
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = z[id] + st->c_beta * pold[id]; }
+    __device__ void operator()(int n, const StageView<1>& sv, double (&acc)[1]) const {
+        NodeNbr s;
+        node_elems(g, n, s);
+        double X[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) X[a][b] = sv.at(0, n, a - 1, b - 1);
+        const double a = (pin && pin[n]) ? 0.0 : ebe_const(s, Le, X);
+        pnew[n] = X[1][1];
+        Ap[n] = a;
+        acc[0] += X[1][1] * a;
+    }
+};
 struct CgK1Fin {
     DevState* st;
     __device__ void operator()(const double (&t)[1]) const { st->c_alpha = st->c_gamma / t[0]; }
@@ -835,7 +906,12 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_node<1>(c, nn, act, CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st}, CgK1Fin{c.st});
+            if (g.periodic)
+                launch_node<1>(c, nn, act, CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                               CgK1Fin{c.st});
+            else
+                launch_stencil<1>(c, nn, act, CgK1S{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                                  CgK1Fin{c.st});
             launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
             bool f;
             const void* y = precondition(c, act, f);
