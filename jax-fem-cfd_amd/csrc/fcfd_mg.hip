@@ -152,6 +152,75 @@ struct Restrict {
     }
 };
 
+// ------------------------------------------------------------------------------ fused down leg
+// Pinned form, non-periodic grids: DownRes and Restrict of one level in one kernel.  A block owns
+// DF_TC coarse nodes of one coarse row J; it stages x = 2/3 r / L_diag (0 at the pins and outside
+// the grid) on the fine window rows 2J-2 .. 2J+2, writes the x of the fine nodes it owns (rows
+// 2J, 2J+1), forms f = r - L x on rows 2J-1 .. 2J+1 in LDS (recomputing the halo of its
+// neighbours) and restricts f into its coarse nodes.  f never goes to memory: in the pinned form
+// only the restriction reads it.  Values and operation order are those of DownRes + Restrict.
+constexpr int DF_TC = 64;
+template <class T, class TR>
+__global__ __launch_bounds__(256) void k_down_fused(Grid2D gf, Grid2D gc, const double* __restrict__ Le,
+                                                    const TR* __restrict__ r, const T* __restrict__ Ld,
+                                                    const uint8_t* __restrict__ pin, T* __restrict__ x,
+                                                    T* __restrict__ rc, const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    constexpr int XW = 2 * DF_TC + 3, FW = 2 * DF_TC + 1;
+    __shared__ T xs[5][XW];
+    __shared__ T fs[3][FW];
+    const int J = blockIdx.y, I0 = blockIdx.x * DF_TC;
+    const int fi0 = 2 * I0 - 2, fj0 = 2 * J - 2;
+    for (int t = threadIdx.x; t < 5 * XW; t += blockDim.x) {
+        const int a = t / XW, b = t - a * XW;
+        const int fi = fi0 + b, fj = fj0 + a;
+        T v = T(0);
+        if (fi >= 0 && fi < gf.nnx && fj >= 0 && fj < gf.nny) {
+            const int k = fj * gf.nnx + fi;
+            const T xv = (T(2.0 / 3.0) * (T)r[k]) / Ld[k];
+            v = (pin && pin[k]) ? T(0) : xv;
+        }
+        xs[a][b] = v;
+    }
+    __syncthreads();
+    {
+        const int a = threadIdx.x / (2 * DF_TC), b = threadIdx.x - a * (2 * DF_TC);
+        const int fi = 2 * I0 + b, fj = 2 * J + a;
+        if (fi < gf.nnx && fj < gf.nny) x[fj * gf.nnx + fi] = xs[a + 2][b + 2];
+    }
+    for (int t = threadIdx.x; t < 3 * FW; t += blockDim.x) {
+        const int a = t / FW, b = t - a * FW;
+        const int fi = 2 * I0 - 1 + b, fj = 2 * J - 1 + a;
+        T v = T(0);
+        if (fi >= 0 && fi < gf.nnx && fj >= 0 && fj < gf.nny) {
+            const int n = fj * gf.nnx + fi;
+            NodeNbr s;
+            node_elems(gf, n, s);
+            T X[3][3];
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) X[dy][dx] = xs[a + dy][b + dx];
+            const bool pn = pin && pin[n];
+            const T rh = pn ? T(0) : (T)r[n];
+            const T lx = pn ? T(0) : ebe_const(s, Le, X);
+            v = rh - lx;
+        }
+        fs[a][b] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < DF_TC) {
+        const int I = I0 + threadIdx.x;
+        if (I < gc.nnx) {
+            const int cc = 2 * threadIdx.x + 1;
+            auto val = [&](int dy, int dx) -> T { return fs[dy + 1][cc + dx]; };
+            const T E = ((val(1, 0) + val(-1, 0)) + val(0, 1)) + val(0, -1);
+            const T D = ((val(1, 1) + val(1, -1)) + val(-1, 1)) + val(-1, -1);
+            rc[J * gc.nnx + I] = (val(0, 0) + T(0.5) * E) + T(0.25) * D;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------ up leg
 // bilinear prolongation of the coarse vector (minus its mean) at fine node n
 // (multigrid.py:90-140; periodic coarse grids wrap)
@@ -588,6 +657,17 @@ static void vcycle_t(Ctx& c, const double* r0, const int* active) {
         Level& L = c.lv[l];
         Level& Lc = c.lv[l + 1];
         const int n = L.g.nn;
+        if (!neu && !L.g.periodic) {
+            const dim3 grid((Lc.g.nnx + DF_TC - 1) / DF_TC, Lc.g.nny);
+            if (l == 0)
+                k_down_fused<T, double><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Le, r0, B::Ld(L), L.pin,
+                                                                     B::x(L), B::r(Lc), active);
+            else
+                k_down_fused<T, T><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Le, B::r(L), B::Ld(L), L.pin,
+                                                                B::x(L), B::r(Lc), active);
+            FCFD_CHECK_LAUNCH();
+            continue;
+        }
         if (l == 0) {
             if (neu)
                 launch_node<1>(c, n, active, DownX<T, double>{r0, B::Ld(L), B::x(L), c.st, l},
