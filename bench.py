@@ -218,6 +218,24 @@ def roofline_only(reps):
                       "copy_bytes_each_way": n_copy * 8}))
 
 
+def copy_bandwidth(n_bytes=1 << 30, reps=10):
+    """Device-to-device copy of n_bytes (read + write counted), timed with events on torch's
+    stream: the practical streaming rate the roofline kernel is also compared against."""
+    import torch
+    x = torch.ones(n_bytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3 / reps
+    del x, y
+    return 2 * n_bytes / sec / 1e9
+
+
 def _traffic_from_profile():
     """HBM bytes per launch of the roofline kernel from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
@@ -274,6 +292,7 @@ def main():
     ach_m = bytes_m / (ms_m * 1e-3) / 1e9
     ach_d = bytes_d / (ms_d * 1e-3) / 1e9
 
+    copy_gbs = copy_bandwidth()
     pois64 = poisson_alone("fp64")
     poism = poisson_alone("mixed")
     grad = None if args.no_grad else gradient_256()
@@ -311,6 +330,7 @@ def main():
                      "frac": ach_m / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "stencil_kernel<3, BK2S<2, MomCoef<CeF32>, 1, false>> (BiCGSTAB s/t: LDS-staged momentum operator apply + 3 dots)",
                      "bytes_per_launch": bytes_m, "us_per_launch": ms_m * 1e3,
+                     "device_copy_gbs": copy_gbs, "frac_of_device_copy": ach_m / copy_gbs,
                      "fp64_ce": {"achieved": ach_d, "bytes_per_launch": bytes_d, "us_per_launch": ms_d * 1e3}},
         "cpu_baseline": None if cpu is None else (
             {"error": cpu["error"]} if "error" in cpu else

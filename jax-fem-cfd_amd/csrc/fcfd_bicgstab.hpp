@@ -29,14 +29,22 @@ struct MomCoef {
     __device__ __forceinline__ double coef(int a, int b, int e) const {
         return cM * ec->Me[a * 4 + b] + cL * ec->Le[a * 4 + b] + cC * ce.ld(a * 4 + b, e);
     }
+    // row a of element e (one vector load), entries as coef() forms them
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[4]) const {
+        double v[4];
+        ce.ld_row(a, e, v);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) cf[b] = cM * ec->Me[a * 4 + b] + cL * ec->Le[a * 4 + b] + cC * v[b];
+    }
 };
 
-// per-element stored matrices (entry-major): the scalar-transport operators
+// per-element stored matrices (row-blocked): the scalar-transport operators
 template <class S>
 struct StoredCoef {
     Grid2D g;
     S m;
     __device__ __forceinline__ double coef(int a, int b, int e) const { return m.ld(a * 4 + b, e); }
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[4]) const { m.ld_row(a, e, cf); }
 };
 
 template <int NC, class Op>
@@ -51,11 +59,12 @@ __device__ __forceinline__ void op_apply(const Op& op, const NodeNbr& s, const d
         double t[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) t[c] = 0.0;
+        double cf[4];
+        op.row(a, e, cf);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const double cf = op.coef(a, b, e);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) t[c] += cf * X[c][kCy(k, b)][kCx(k, b)];
+            for (int c = 0; c < NC; ++c) t[c] += cf[b] * X[c][kCy(k, b)][kCx(k, b)];
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c) y[c] += s.has[k] ? t[c] : 0.0;
@@ -68,6 +77,11 @@ struct TransCoef {
     Op op;
     Grid2D g;
     __device__ __forceinline__ double coef(int a, int b, int e) const { return op.coef(b, a, e); }
+    // row a of the transposed matrix = column a of the stored one (4 scalar loads)
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[4]) const {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) cf[b] = op.coef(b, a, e);
+    }
 };
 template <class Op>
 __host__ __device__ __forceinline__ TransCoef<Op> transpose_op(const Op& op) { return TransCoef<Op>{op, op.g}; }
@@ -260,6 +274,7 @@ struct BK2Fin {
 template <int NC, class LOp, bool MT = false>
 struct BK1S {
     static constexpr int NV = NC;
+    static constexpr int NPT = 1;
     LOp op;
     const uint8_t* mask;
     const double *r, *pold, *qold, *rhat;
@@ -278,7 +293,7 @@ struct BK1S {
             v[c] = (MT && mask && mask[i]) ? 0.0 : pv;
         }
     }
-    __device__ void operator()(int n, const StageView<NV>& sv, double (&acc)[1]) const {
+    __device__ void operator()(int n, const StageView<NV, NPT>& sv, double (&acc)[1]) const {
         const int nn = op.g.nn;
         NodeNbr s;
         node_elems(op.g, n, s);
@@ -307,6 +322,7 @@ struct BK1S {
 template <int NC, class LOp, int TAG = 0, bool MT = false>
 struct BK2S {
     static constexpr int NV = NC;
+    static constexpr int NPT = 1;
     LOp op;
     const uint8_t* mask;
     const double *r, *q;
@@ -323,7 +339,7 @@ struct BK2S {
             v[c] = (MT && mask && mask[i]) ? 0.0 : s;
         }
     }
-    __device__ void operator()(int n, const StageView<NV>& sw, double (&acc)[3]) const {
+    __device__ void operator()(int n, const StageView<NV, NPT>& sw, double (&acc)[3]) const {
         const int nn = op.g.nn;
         NodeNbr s;
         node_elems(op.g, n, s);

@@ -134,22 +134,38 @@ struct ElemConst {
 };
 
 // ---------------------------------------------------------------------------------------------
-// Per-element matrix storage, entry-major: entry k = a*4 + b of element e at k*ne + e, so a
-// wavefront reading one entry of 64 consecutive elements is one coalesced 512 B (FP64) read.
+// Per-element matrix storage, row-blocked: entry (a, b) of element e at ((a * ne + e) * 4 + b),
+// i.e. the 4 entries of row a of an element are contiguous and row a of consecutive elements is
+// consecutive.  A node reads the row of each incident element that belongs to it with one vector
+// load (16 B FP32, 32 B FP64, 8 B FP16 / BF16), and a wavefront reading row a of 64 consecutive
+// elements reads one contiguous span (coalesced).
 // ---------------------------------------------------------------------------------------------
+__host__ __device__ __forceinline__ size_t ce_idx(int k, int e, int ne) {
+    return ((size_t)(k >> 2) * ne + e) * 4 + (k & 3);
+}
+
 struct CeF64 {
     double* c; int ne;
-    __device__ __forceinline__ double ld(int k, int e) const { return c[(size_t)k * ne + e]; }
+    __device__ __forceinline__ double ld(int k, int e) const { return c[ce_idx(k, e, ne)]; }
+    __device__ __forceinline__ void ld_row(int a, int e, double (&v)[4]) const {
+        const double2* p = reinterpret_cast<const double2*>(c + ce_idx(a * 4, e, ne));
+        const double2 x = p[0], y = p[1];
+        v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+    }
     __device__ __forceinline__ void st_elem(int e, const double* v) const {
-        for (int k = 0; k < 16; ++k) c[(size_t)k * ne + e] = v[k];
+        for (int k = 0; k < 16; ++k) c[ce_idx(k, e, ne)] = v[k];
     }
     static constexpr size_t bytes_per_elem = 16 * 8;
 };
 struct CeF32 {
     float* c; int ne;
-    __device__ __forceinline__ double ld(int k, int e) const { return (double)c[(size_t)k * ne + e]; }
+    __device__ __forceinline__ double ld(int k, int e) const { return (double)c[ce_idx(k, e, ne)]; }
+    __device__ __forceinline__ void ld_row(int a, int e, double (&v)[4]) const {
+        const float4 x = *reinterpret_cast<const float4*>(c + ce_idx(a * 4, e, ne));
+        v[0] = (double)x.x; v[1] = (double)x.y; v[2] = (double)x.z; v[3] = (double)x.w;
+    }
     __device__ __forceinline__ void st_elem(int e, const double* v) const {
-        for (int k = 0; k < 16; ++k) c[(size_t)k * ne + e] = (float)v[k];
+        for (int k = 0; k < 16; ++k) c[ce_idx(k, e, ne)] = (float)v[k];
     }
     static constexpr size_t bytes_per_elem = 16 * 4;
 };
@@ -157,28 +173,42 @@ struct CeF32 {
 struct CeF16S {
     __half* c; float* s; int ne;
     __device__ __forceinline__ double ld(int k, int e) const {
-        return (double)(__half2float(c[(size_t)k * ne + e]) * s[e]);
+        return (double)(__half2float(c[ce_idx(k, e, ne)]) * s[e]);
+    }
+    __device__ __forceinline__ void ld_row(int a, int e, double (&v)[4]) const {
+        const uint2 w = *reinterpret_cast<const uint2*>(c + ce_idx(a * 4, e, ne));
+        const float sc = s[e];
+        const __half2 h0 = *reinterpret_cast<const __half2*>(&w.x), h1 = *reinterpret_cast<const __half2*>(&w.y);
+        v[0] = (double)(__low2float(h0) * sc); v[1] = (double)(__high2float(h0) * sc);
+        v[2] = (double)(__low2float(h1) * sc); v[3] = (double)(__high2float(h1) * sc);
     }
     __device__ __forceinline__ void st_elem(int e, const double* v) const {
         double m = 0.0;
         for (int k = 0; k < 16; ++k) m = fmax(m, fabs(v[k]));
         float sc = m > 0.0 ? (float)m : 1.0f;
         s[e] = sc;
-        for (int k = 0; k < 16; ++k) c[(size_t)k * ne + e] = __float2half((float)(v[k] / (double)sc));
+        for (int k = 0; k < 16; ++k) c[ce_idx(k, e, ne)] = __float2half((float)(v[k] / (double)sc));
     }
     static constexpr size_t bytes_per_elem = 16 * 2 + 4;
 };
 struct CeBF16S {
     __hip_bfloat16* c; float* s; int ne;
     __device__ __forceinline__ double ld(int k, int e) const {
-        return (double)(__bfloat162float(c[(size_t)k * ne + e]) * s[e]);
+        return (double)(__bfloat162float(c[ce_idx(k, e, ne)]) * s[e]);
+    }
+    __device__ __forceinline__ void ld_row(int a, int e, double (&v)[4]) const {
+        const uint2 w = *reinterpret_cast<const uint2*>(c + ce_idx(a * 4, e, ne));
+        const float sc = s[e];
+        // bf16 -> f32: the 16 bits become the high half of the float
+        v[0] = (double)(__uint_as_float(w.x << 16) * sc); v[1] = (double)(__uint_as_float(w.x & 0xffff0000u) * sc);
+        v[2] = (double)(__uint_as_float(w.y << 16) * sc); v[3] = (double)(__uint_as_float(w.y & 0xffff0000u) * sc);
     }
     __device__ __forceinline__ void st_elem(int e, const double* v) const {
         double m = 0.0;
         for (int k = 0; k < 16; ++k) m = fmax(m, fabs(v[k]));
         float sc = m > 0.0 ? (float)m : 1.0f;
         s[e] = sc;
-        for (int k = 0; k < 16; ++k) c[(size_t)k * ne + e] = __float2bfloat16((float)(v[k] / (double)sc));
+        for (int k = 0; k < 16; ++k) c[ce_idx(k, e, ne)] = __float2bfloat16((float)(v[k] / (double)sc));
     }
     static constexpr size_t bytes_per_elem = 16 * 2 + 4;
 };
@@ -249,11 +279,15 @@ __global__ __launch_bounds__(BLOCK) void node_kernel(int n, const int* __restric
 // across a row end belong only to missing elements (masked by has[k]); ids outside [0, nn) are
 // staged as 0.  Per node: ~3 global loads per staged vector instead of 9.
 // ---------------------------------------------------------------------------------------------
-constexpr int SWIN = BLOCK + 2;
+// Window width for NPT nodes per thread (a block owns BLOCK * NPT consecutive nodes).
+template <int NPT>
+struct SWin {
+    static constexpr int W = BLOCK * NPT + 2;
+};
 
-template <int NV>
+template <int NV, int NPT = 1>
 struct StageView {
-    const double (*s)[3][SWIN];
+    const double (*s)[3][SWin<NPT>::W];
     int n0, nnx;
     __device__ __forceinline__ double at(int v, int n, int dy, int dx) const { return s[v][dy + 1][n - n0 + dx + 1]; }
 };
@@ -270,19 +304,21 @@ __device__ __forceinline__ void node_elems(const Grid2D& g, int n, NodeNbr& s) {
     s.e[3] = s.has[3] ? j * g.nex + i : 0;
 }
 
-// Body: static constexpr int NV; int nnx() const; void stage(int id, double (&v)[NV]) const;
-//       void operator()(int n, const StageView<NV>&, double (&acc)[NR]) const.
+// Body: static constexpr int NV; static constexpr int NPT; int nnx() const;
+//       void stage(int id, double (&v)[NV]) const;
+//       void operator()(int n, const StageView<NV, NPT>&, double (&acc)[NR]) const.
+// A block owns BLOCK * NPT consecutive nodes; thread t computes nodes n0 + t + m * BLOCK.
 template <int NR, class Body>
 __global__ __launch_bounds__(BLOCK) void stencil_kernel(int n, const int* __restrict__ active, Body body,
                                                        double* __restrict__ partials, int pstride) {
     if (active && *active == 0) return;
-    constexpr int NV = Body::NV;
-    __shared__ double sm[NV][3][SWIN];
+    constexpr int NV = Body::NV, NPT = Body::NPT, W = SWin<NPT>::W;
+    __shared__ double sm[NV][3][W];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int n0 = lb * BLOCK;
+    const int n0 = lb * BLOCK * NPT;
     const int nnx = body.nnx();
-    for (int idx = threadIdx.x; idx < 3 * SWIN; idx += BLOCK) {
-        const int seg = idx / SWIN, off = idx - seg * SWIN;
+    for (int idx = threadIdx.x; idx < 3 * W; idx += BLOCK) {
+        const int seg = idx / W, off = idx - seg * W;
         const int id = n0 + (seg - 1) * nnx - 1 + off;
         const bool in = id >= 0 && id < n;
         double v[NV];
@@ -291,11 +327,14 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(int n, const int* __rest
         for (int q = 0; q < NV; ++q) sm[q][seg][off] = in ? v[q] : 0.0;
     }
     __syncthreads();
-    const int k = n0 + threadIdx.x;
     double acc[NR > 0 ? NR : 1];
 #pragma unroll
     for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
-    if (k < n) body(k, StageView<NV>{sm, n0, nnx}, acc);
+#pragma unroll
+    for (int m = 0; m < NPT; ++m) {
+        const int k = n0 + threadIdx.x + m * BLOCK;
+        if (k < n) body(k, StageView<NV, NPT>{sm, n0, nnx}, acc);
+    }
     if constexpr (NR > 0) {
         block_sum<NR>(acc);
         if (threadIdx.x == 0) {

@@ -90,7 +90,7 @@ struct Ctx {
     // element matrices (timestep storage)
     void* ce = nullptr;
     float* ce_scale = nullptr;
-    double* ce64_api = nullptr;  // FP64 entry-major buffer for API calls taking (ne,4,4)
+    double* ce64_api = nullptr;  // FP64 row-blocked buffer for API calls taking (ne,4,4)
     // momentum work (2nn)
     double *m_r = nullptr, *m_rhat = nullptr, *m_p[2] = {nullptr, nullptr}, *m_q[2] = {nullptr, nullptr},
            *m_s = nullptr, *m_t = nullptr, *m_x = nullptr;
@@ -160,7 +160,8 @@ inline void launch_node(Ctx& c, int n, const int* active, const Body& body, cons
 // Same for an LDS-staged stencil kernel (non-periodic grids).
 template <int NR, class Body, class Fin>
 inline void launch_stencil(Ctx& c, int n, const int* active, const Body& body, const Fin& fin) {
-    const int nb = blocks_for(n);
+    const int per = BLOCK * Body::NPT;
+    const int nb = n > 0 ? (n + per - 1) / per : 1;
     if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
     stencil_kernel<NR><<<nb, BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
     if constexpr (NR > 0)

@@ -95,6 +95,7 @@ struct DownRes {
 template <class T, class TR>
 struct DownResS {
     static constexpr int NV = 1;
+    static constexpr int NPT = 1;
     Grid2D g; const double* Le; const TR* r; const T* Ld; const uint8_t* pin; T* x; T* f;
     const DevState* st; int l; int neumann;
     __device__ __forceinline__ int nnx() const { return g.nnx; }
@@ -104,7 +105,7 @@ struct DownResS {
         return (pin && pin[k]) ? T(0) : v;
     }
     __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = (double)xval(id); }
-    __device__ void operator()(int n, const StageView<1>& sv, double (&acc)[1]) const {
+    __device__ void operator()(int n, const StageView<1, NPT>& sv, double (&acc)[1]) const {
         NodeNbr s;
         node_elems(g, n, s);
         T X[3][3];
@@ -833,11 +834,12 @@ struct CgK1 {
 // LDS-staged form (non-periodic grids): p staged once per block window
 struct CgK1S {
     static constexpr int NV = 1;
+    static constexpr int NPT = 1;
     Grid2D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
     const DevState* st;
     __device__ __forceinline__ int nnx() const { return g.nnx; }
     __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = z[id] + st->c_beta * pold[id]; }
-    __device__ void operator()(int n, const StageView<1>& sv, double (&acc)[1]) const {
+    __device__ void operator()(int n, const StageView<1, NPT>& sv, double (&acc)[1]) const {
         NodeNbr s;
         node_elems(g, n, s);
         double X[3][3];

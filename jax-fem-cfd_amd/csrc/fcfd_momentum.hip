@@ -64,6 +64,8 @@ template <class S>
 static float bench_t(Ctx& c, S ce, int reps) {
     MomCoef<S> lhs{c.g, c.ec_dev, ce, 2000.0, 0.5e-3, 0.5};
     const int nb = blocks_for(c.g.nn);
+    using K = BK2S<2, MomCoef<S>, 1>;
+    const int nbs = (c.g.nn + BLOCK * K::NPT - 1) / (BLOCK * K::NPT);
     hipEvent_t e0, e1;
     FCFD_HIP(hipEventCreate(&e0));
     FCFD_HIP(hipEventCreate(&e1));
@@ -73,9 +75,8 @@ static float bench_t(Ctx& c, S ce, int reps) {
                 c.g.nn, nullptr, BK2<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st},
                 c.partials, c.pstride);
         else
-            stencil_kernel<3><<<nb, BLOCK, 0, c.stream>>>(
-                c.g.nn, nullptr, BK2S<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st},
-                c.partials, c.pstride);
+            stencil_kernel<3><<<nbs, BLOCK, 0, c.stream>>>(
+                c.g.nn, nullptr, K{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st}, c.partials, c.pstride);
     };
     launch();
     FCFD_HIP(hipEventRecord(e0, c.stream));

@@ -68,7 +68,7 @@ void launch_fill32(Ctx& c, float* x, float v, int n) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Ce build: one thread per element, entry-major store.
+// Ce build: one thread per element, row-blocked store (ce_idx).
 // ---------------------------------------------------------------------------------------------
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_build_ce(Grid2D g, const ElemConst* __restrict__ ec,
@@ -120,13 +120,13 @@ void build_ce_api(Ctx& c, const double* U, double beta, double* Ce) {
     FCFD_CHECK_LAUNCH();
 }
 
-// (ne,4,4) row-major  <->  entry-major [16][ne]
+// (ne,4,4) row-major  <->  row-blocked (ce_idx)
 __global__ void k_ce_layout(const double* __restrict__ in, double* __restrict__ out, int ne, int to_api) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     for (int k = 0; k < 16; ++k) {
-        if (to_api) out[(size_t)e * 16 + k] = in[(size_t)k * ne + e];
-        else out[(size_t)k * ne + e] = in[(size_t)e * 16 + k];
+        if (to_api) out[(size_t)e * 16 + k] = in[ce_idx(k, e, ne)];
+        else out[ce_idx(k, e, ne)] = in[(size_t)e * 16 + k];
     }
 }
 void ce_to_api(Ctx& c, const double* em, double* api) {

@@ -99,8 +99,8 @@ __global__ __launch_bounds__(BLOCK) void k_ade_elem(Grid2D g, const ElemConst* _
             const int k = a * 4 + b;
             const double Mt = ec->Me[k] + (p.supg ? tau * cen[b * 4 + a] : 0.0);
             const double K = p.D * ec->Le[k] + (p.supg ? tau * ct[k] : 0.0);
-            A[k * ne + e] = (1.0 / p.dt) * Mt + 0.5 * (ce[k] + K);
-            B[k * ne + e] = (1.0 / p.dt) * Mt - 0.5 * (cen[k] + K);
+            A[ce_idx(k, e, (int)ne)] = (1.0 / p.dt) * Mt + 0.5 * (ce[k] + K);
+            B[ce_idx(k, e, (int)ne)] = (1.0 / p.dt) * Mt - 0.5 * (cen[k] + K);
         }
 }
 
