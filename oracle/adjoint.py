@@ -21,8 +21,14 @@ Ce(U, beta) (element_calc.py:165-185) is linear in U, so
 g_Id = sum_e sum_ab w_a T_d[I, a, b] x_b and T_d[I] = Ce(unit velocity at local
 node I, component d).  T is built here from the oracle's own Ce routine.
 
-The step restated here is the plain (non-SUPG) passive step and the NS step of
-standard.py:148-165 / passive_transport.py:64-97.
+The steps restated here are the NS step (standard.py:148-165) and the passive step
+(passive_transport.py:64-97), plain and SUPG.  With SUPG the scalar matrices
+A_e = (Me + tau Ce(U_n)^T)/dt + (Ce(U) + D Le + tau Ct(U_n))/2 and
+B_e = (Me + tau Ce(U_n)^T)/dt - (Ce(U_n) + D Le + tau Ct(U_n))/2, tau = tau(U_n, Ce(U))
+(element_calc.py:188-229) depend nonlinearly on U_n and U; the cotangents of
+sum_e w_e^T (B_e phi_n - A_e phi) are taken here by complex-step differentiation of the
+oracle's own (pinned) element routines, with a 3x3 node colouring so that no element
+holds two perturbed DOFs -- exact to rounding, and independent of any hand derivation.
 Pinned by: adjoint identities <A x, y> = <x, A^T y> for every transposed operator
 and central finite differences of the forward map (tests/test_adjoint_oracle.py).
 """
@@ -204,19 +210,76 @@ def four_step_vjp(s, U_n, P_n, dt, rho, mu, U_d, F_p, F, gU, gP, theta=0.5, fw=N
     return gUn, gPn, fw, (m_it, p_it)
 
 
+# ------------------------------------------------------------------------------ SUPG element matrices
+def supg_matrices(s, U_n, U, D, dt):
+    """Per-element SUPG scalar matrices (A_e, B_e) of passive_transport.py:83-91, from the
+    oracle's element routines (complex-safe: einsum, sqrt of sums of squares, real powers)."""
+    nconn, nn = s.nconn, s.nn
+    Ce_n = el.conservative_convec_elem_calc(nconn, U_n, s.wq, s.N, s.Nderiv, nn, 2, 1)
+    Ce = el.conservative_convec_elem_calc(nconn, U, s.wq, s.N, s.Nderiv, nn, 2, 1)
+    Ct = el.Ctau_tilde_elem_calc(nconn, U_n, s.wq, s.N, s.Nderiv, nn, 2)
+    tau = el.calculate_tau(nconn, U_n, Ce, Ct, s.N, nn, 2, D, dt, 2)
+    Mt = s.Me[None] + tau[:, None, None] * Ce_n.transpose(0, 2, 1)
+    K = D * s.Le[None] + tau[:, None, None] * Ct
+    return Mt / dt + 0.5 * (Ce + K), Mt / dt - 0.5 * (Ce_n + K)
+
+
+def supg_param_grad(s, U_n, U, w, phi, phi_n, D, dt, h=1e-30):
+    """(d/dU_n, d/dU) of sum_e w_e^T (B_e phi_n,e - A_e phi_e), by complex step.  Nodes of one
+    colour (i mod 3, j mod 3) share no element, so one complex evaluation per colour and
+    component gives the derivative of every element term w.r.t. its single perturbed DOF."""
+    nn, nconn = s.nn, s.nconn
+    nnx = int(np.unique(s.coords[:, 0]).size)
+    ii, jj = np.arange(nn) % nnx, np.arange(nn) // nnx
+    we, pe, pne = w[nconn], phi[nconn], phi_n[nconn]
+
+    def ell(Un, Uc):
+        A, B = supg_matrices(s, Un, Uc, D, dt)
+        return np.einsum("ea,eab,eb->e", we, B, pne) - np.einsum("ea,eab,eb->e", we, A, pe)
+
+    out = []
+    for which in (0, 1):
+        g = np.zeros(2 * nn)
+        for ci in range(3):
+            for cj in range(3):
+                sel = (ii % 3 == ci) & (jj % 3 == cj)
+                for d in range(2):
+                    P = np.zeros(2 * nn, dtype=complex)
+                    P[d * nn:(d + 1) * nn][sel] = 1j * h
+                    Un = U_n + (P if which == 0 else 0)
+                    Uc = U + (P if which == 1 else 0)
+                    de = ell(Un, Uc).imag / h                       # (ne,)
+                    hit = sel[nconn]                                # (ne, 4): the element's perturbed node
+                    node = np.where(hit.any(axis=1), nconn[np.arange(nconn.shape[0]), hit.argmax(axis=1)], -1)
+                    ok = node >= 0
+                    g[d * nn:(d + 1) * nn] += np.bincount(node[ok], de[ok], minlength=nn)
+        out.append(g)
+    return out[0], out[1]
+
+
 # ------------------------------------------------------------------------------ passive step VJP
-def passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, gU, gP, gphi):
-    """Cotangents (gU_n, gP_n, gphi_n) of the plain passive step
-    (passive_transport.py:64-97 with phi_SUPG False).  Returns
-    (gU_n, gP_n, gphi_n, outputs (U, P, phi), adjoint iteration counts)."""
+def passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, gU, gP, gphi, supg=False):
+    """Cotangents (gU_n, gP_n, gphi_n) of the passive step (passive_transport.py:64-97, plain
+    or SUPG).  Returns (gU_n, gP_n, gphi_n, outputs (U, P, phi), adjoint iteration counts)."""
+    from .ns import passive_timestep
     nn, ndim, nconn = s.nn, s.ndim, s.nconn
     fw = ns_forward(s, U_n, P_n, dt, rho, mu, U_d, F_p, F)
     U = fw["U"]
-    Ce_n = el.conservative_convec_elem_calc(nconn, U_n, s.wq, s.N, s.Nderiv, nn, ndim, 1)
-    Ce = el.conservative_convec_elem_calc(nconn, U, s.wq, s.N, s.Nderiv, nn, ndim, 1)
-    phi, phi_it = update_phi(phi_n, nconn, s.Me, Ce, Ce_n, s.Le, D, dt, nn, gamma_d_phi, phi_d, s.tol)
+    _, _, phi, _, _, _ = passive_timestep(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi,
+                                          supg)
     mphi = _mask(nn, gamma_d_phi)
-    _, lhs_sT = scalar_ops(s, Ce, D, dt, gamma_d_phi)
+    if supg:
+        A_all, B_all = supg_matrices(s, U_n, U, D, dt)
+    else:
+        Ce_n = el.conservative_convec_elem_calc(nconn, U_n, s.wq, s.N, s.Nderiv, nn, ndim, 1)
+        Ce = el.conservative_convec_elem_calc(nconn, U, s.wq, s.N, s.Nderiv, nn, ndim, 1)
+        M_all = np.broadcast_to(s.Me, Ce.shape)
+        K = D * s.Le[None]
+        A_all, B_all = M_all / dt + 0.5 * (Ce + K), M_all / dt - 0.5 * (Ce_n + K)
+
+    def lhs_sT(y):
+        return _elem_mult_T(np.where(mphi, 0.0, y), A_all, nconn, nn) + np.where(mphi, y, 0.0)
+
     gU_tot = np.zeros(ndim * nn) if gU is None else gU
     gphin, gUn_s, s_it = np.zeros(nn), 0.0, 0
     if gphi is not None:
@@ -225,34 +288,37 @@ def passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_
             x0s[gamma_d_phi] = phi_d
         mu_s, s_it = _solve_or_zero(bicgstab, lhs_sT, gphi, x0=x0s, tol=s.tol)
         w = np.where(mphi, 0.0, mu_s)
-        gphin = ((1 / dt) * emult.consistent_mass_mult(w, s.Me, nconn, nn, 1)
-                 - 0.5 * (_elem_mult_T(w, Ce_n, nconn, nn) + D * emult.laplacian_element_mult(w, s.Le, nconn, nn)))
-        T1 = conv_tensor(s, 1.0)
-        gU_tot = gU_tot - 0.5 * conv_grad(s, T1, w, phi, 1)
-        gUn_s = -0.5 * conv_grad(s, T1, w, phi_n, 1)
+        gphin = _elem_mult_T(w, B_all, nconn, nn)
+        if supg:
+            gUn_s, gUs = supg_param_grad(s, U_n, U, w, phi, phi_n, D, dt)
+            gU_tot = gU_tot + gUs
+        else:
+            T1 = conv_tensor(s, 1.0)
+            gU_tot = gU_tot - 0.5 * conv_grad(s, T1, w, phi, 1)
+            gUn_s = -0.5 * conv_grad(s, T1, w, phi_n, 1)
     gUn, gPn, _, (m_it, p_it) = four_step_vjp(s, U_n, P_n, dt, rho, mu, U_d, F_p, F, gU_tot, gP, fw=fw)
     return gUn + gUn_s, gPn, gphin, (U, fw["P"], phi), (s_it, p_it, m_it)
 
 
 # ------------------------------------------------------------------------------ rollout gradient
-def rollout(s, nsteps, U0, P0, phi0, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi):
-    """timestep_loop_scan (simulation.py:362-374) on the plain passive step: final (U, P, phi)."""
+def rollout(s, nsteps, U0, P0, phi0, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, supg=False):
+    """timestep_loop_scan (simulation.py:362-374) on the passive step: final (U, P, phi)."""
     from .ns import passive_timestep
     U, P, phi = U0, P0, phi0
     for _ in range(nsteps):
-        U, P, phi, _, _, _ = passive_timestep(s, U, P, phi, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, False)
+        U, P, phi, _, _, _ = passive_timestep(s, U, P, phi, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, supg)
     return U, P, phi
 
 
 def rollout_vjp(s, nsteps, U0, P0, phi0, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, gU, gP, gphi,
-                checkpoint=20):
+                checkpoint=20, supg=False):
     """Reverse sweep over the rollout with states stored every `checkpoint` steps and each
     segment recomputed forward before it is swept backward.  Returns (gU0, gP0, gphi0)."""
     from .ns import passive_timestep
     ckpt = {0: (U0, P0, phi0)}
     U, P, phi = U0, P0, phi0
     for k in range(nsteps):
-        U, P, phi, _, _, _ = passive_timestep(s, U, P, phi, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, False)
+        U, P, phi, _, _, _ = passive_timestep(s, U, P, phi, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, supg)
         if (k + 1) % checkpoint == 0 and k + 1 < nsteps:
             ckpt[k + 1] = (U, P, phi)
     nn = s.nn
@@ -265,9 +331,9 @@ def rollout_vjp(s, nsteps, U0, P0, phi0, dt, rho, mu, D, U_d, phi_d, F_p, F, gam
         for _ in range(seg0, seg1 - 1):
             Ua, Pa, pa = states[-1]
             Ub, Pb, pb, _, _, _ = passive_timestep(s, Ua, Pa, pa, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi,
-                                                   False)
+                                                   supg)
             states.append((Ub, Pb, pb))
         for Ua, Pa, pa in reversed(states):
             gU, gP, gphi, _, _ = passive_vjp(s, Ua, Pa, pa, dt, rho, mu, D, U_d, phi_d, F_p, F, gamma_d_phi, gU, gP,
-                                             gphi)
+                                             gphi, supg)
     return gU, gP, gphi

@@ -172,3 +172,54 @@ def test_zero_cotangent_gives_zero_without_iterating():
     gUn, gPn, gfn, _, its = adj.passive_vjp(s, U_n, P_n, phi_n, 5e-3, 1.0, 1e-2, 1e-3, U_d, None, 0, 0, None, z2,
                                             z1, z1)
     assert not gUn.any() and not gPn.any() and not gfn.any() and its == (0, 0, 0)
+
+
+def test_supg_matrices_match_the_forward_step_operators():
+    """The SUPG element matrices used by the adjoint are the operators of update_phi_SUPG
+    (passive_transport.py:14-37): A phi = lhs(phi), B phi_n = rhs."""
+    s, _, U_d = _case(8)
+    U_n, P_n, phi_n = _state(s, U_d)
+    U = U_n + 0.01 * RNG.standard_normal(U_n.size)
+    D, dt = 1e-3, 5e-3
+    A, B = adj.supg_matrices(s, U_n, U, D, dt)
+    Ce_n = el.conservative_convec_elem_calc(s.nconn, U_n, s.wq, s.N, s.Nderiv, s.nn, 2, 1)
+    Ce = el.conservative_convec_elem_calc(s.nconn, U, s.wq, s.N, s.Nderiv, s.nn, 2, 1)
+    Ct = el.Ctau_tilde_elem_calc(s.nconn, U_n, s.wq, s.N, s.Nderiv, s.nn, 2)
+    tau = el.calculate_tau(s.nconn, U_n, Ce, Ct, s.N, s.nn, 2, D, dt, 2)
+    Mtau, Ctau = tau[:, None, None] * Ce_n.transpose(0, 2, 1), tau[:, None, None] * Ct
+    x = RNG.standard_normal(s.nn)
+    lhs = (1 / dt) * emult.scalar_consistent_mass_SUPG_mult(x, Mtau, s.Me, s.nconn, s.nn) + \
+        0.5 * emult.scalar_stabilized_visc_conv_mult(x, Ce, Ctau, s.Le, D, s.nconn, s.nn)
+    rhs = (1 / dt) * emult.scalar_consistent_mass_SUPG_mult(x, Mtau, s.Me, s.nconn, s.nn) - \
+        0.5 * emult.scalar_stabilized_visc_conv_mult(x, Ce_n, Ctau, s.Le, D, s.nconn, s.nn)
+    Ax = np.bincount(s.nconn.ravel(), np.einsum("eab,eb->ea", A, x[s.nconn]).ravel(), minlength=s.nn)
+    Bx = np.bincount(s.nconn.ravel(), np.einsum("eab,eb->ea", B, x[s.nconn]).ravel(), minlength=s.nn)
+    assert np.abs(Ax - lhs).max() <= 1e-12 * np.abs(lhs).max()
+    assert np.abs(Bx - rhs).max() <= 1e-12 * np.abs(rhs).max()
+
+
+def test_supg_passive_step_vjp_matches_finite_differences():
+    s, _, U_d = _case(8)
+    U_n, P_n, phi_n = _state(s, U_d)
+    nn = s.nn
+    dt, rho, mu, D = 5e-3, 1.0, 1e-2, 1e-3
+    a, b, c = RNG.standard_normal(2 * nn), RNG.standard_normal(nn), RNG.standard_normal(nn)
+
+    def loss(U0, P0, f0):
+        U, P, phi, _, _, _ = ns.passive_timestep(s, U0, P0, f0, dt, rho, mu, D, U_d, None, 0, 0, None, True)
+        return a @ U + b @ P + c @ phi
+
+    gUn, gPn, gfn, _, _ = adj.passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, None, 0, 0, None, a, b, c,
+                                          supg=True)
+    for _ in range(3):
+        ds = [RNG.standard_normal(2 * nn), RNG.standard_normal(nn), RNG.standard_normal(nn)]
+        fd = _fd(loss, [U_n, P_n, phi_n], ds, 1e-4)
+        an = gUn @ ds[0] + gPn @ ds[1] + gfn @ ds[2]
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+    # the U_n and U (through tau, Ct, Ce) cotangents alone: phi cotangent only
+    gUn, _, _, _, _ = adj.passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, None, 0, 0, None, None, None, c,
+                                      supg=True)
+    dU = RNG.standard_normal(2 * nn)
+    fd = _fd(lambda U0: c @ ns.passive_timestep(s, U0, P_n, phi_n, dt, rho, mu, D, U_d, None, 0, 0, None, True)[2],
+             [U_n], [dU], 1e-4)
+    assert abs(fd - gUn @ dU) <= 1e-6 * abs(fd), (fd, gUn @ dU)
