@@ -215,9 +215,11 @@ int fcfd_four_step_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* 
                                 double* P, double* gU_n, double* gP_n, int32_t* iters);
 
 /* Vector-Jacobian product of the passive-scalar timestep (passive_transport.py:64-97, plain
- * scalar update; supg must be 0): re-runs the step into U, P, phi, then writes gU_n, gP_n,
- * gphi_n for the output cotangents gU, gP, gphi (NULL = zero).  The scalar solve is
- * differentiated like the momentum solve (transposed BiCGSTAB from its forward x0).
+ * scalar update or, supg = 1, update_phi_SUPG :14-37): re-runs the step into U, P, phi, then
+ * writes gU_n, gP_n, gphi_n for the output cotangents gU, gP, gphi (NULL = zero).  The scalar
+ * solve is differentiated like the momentum solve (transposed BiCGSTAB from its forward x0);
+ * with SUPG the element matrices are also differentiated through tau, Ctau_tilde(U_n) and the
+ * transposed Ce(U_n) mass term (element_calc.py:188-229).
  * iters (6): forward momentum, pressure, scalar; adjoint scalar, pressure, momentum.
  * The reverse sweep of a rollout (timestep_loop_scan, simulation.py:362-374) calls this once
  * per step, last step first, with the states recomputed from checkpoints. */

@@ -74,6 +74,7 @@ struct AdjScalarBody {
     const double *mu, *phi, *phin, *gU;
     const uint8_t* mphi;
     double *gphin, *gUt, *gUns;
+    const double* gel;  // SUPG element cotangents (k_supg_adj_elem layout) or null
     __device__ void operator()(int n, double (&)[1]) const {
         NodeNbr s;
         node_nbr(g, n, s);
@@ -89,9 +90,20 @@ struct AdjScalarBody {
         conv_grad_node(ec, s, W[0], Xn, 1.0, g1n);
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
+            double en = 0.0, eu = 0.0;
+            if (gel) {
+                const size_t ne = g.ne;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const size_t r = (size_t)kRow(k) * 2 + d;
+                    const double a = gel[r * ne + s.e[k]], b = gel[(8 + r) * ne + s.e[k]];
+                    en += s.has[k] ? a : 0.0;
+                    eu += s.has[k] ? b : 0.0;
+                }
+            }
             const int i = n + d * g.nn;
-            gUt[i] = (gU ? gU[i] : 0.0) - 0.5 * g1[d];
-            gUns[i] = -0.5 * g1n[d];
+            gUt[i] = (gU ? gU[i] : 0.0) - 0.5 * g1[d] + eu;
+            gUns[i] = -0.5 * g1n[d] + en;
         }
     }
 };
@@ -276,13 +288,12 @@ void ns_vjp(Ctx& c, const double* U_n, const double* P_n, double dt, double rho,
     });
 }
 
-// Passive step VJP (plain scalar update).  Requires the forward step's intermediates in the
-// context (as ns_vjp, plus ade_A / ade_B of this step); phi = the step's scalar output.
+// Passive step VJP (plain or SUPG scalar update).  Requires the forward step's intermediates in
+// the context (as ns_vjp, plus ade_A / ade_B of this step); U, phi = the step's outputs.
 // it[0] = scalar adjoint BiCGSTAB, it[1] = pressure adjoint CG, it[2] = momentum adjoint BiCGSTAB.
-void passive_vjp(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* phi, double dt,
-                 double rho, double mu, double D, const double* gU, const double* gP, const double* gphi,
-                 double* gU_n, double* gP_n, double* gphi_n, int* it) {
-    (void)D;
+void passive_vjp(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* U,
+                 const double* phi, double dt, double rho, double mu, double D, int supg, const double* gU,
+                 const double* gP, const double* gphi, double* gU_n, double* gP_n, double* gphi_n, int* it) {
     ensure_adj(c);
     const int nn = c.g.nn;
     const double* gUt = gU;
@@ -293,9 +304,13 @@ void passive_vjp(Ctx& c, const double* U_n, const double* P_n, const double* phi
         const double key[4] = {0.0, 0.0, 0.0, 0.0};
         it[0] = bicgstab_solve_T<1>(c, c.g_aphi, key, c.ade_A, transpose_op(A), gphi, phi_n, c.phi_ud, c.phi_mask,
                                     10 * nn, c.a_sx, c.pred_phi);
+        if (supg) {
+            if (!c.a_gel) c.a_gel = dalloc<double>(c, 16 * (size_t)c.g.ne);
+            supg_adjoint_elem(c, U_n, U, c.a_sx, phi, phi_n, D, dt, c.a_gel);
+        }
         launch_node<0>(c, nn, nullptr,
                        AdjScalarBody{c.g, c.ec_dev, StoredCoef<CeF64>{c.g, CeF64{c.ade_B, c.g.ne}}, c.a_sx, phi,
-                                     phi_n, gU, c.phi_mask, gphi_n, c.a_gut, c.a_guns},
+                                     phi_n, gU, c.phi_mask, gphi_n, c.a_gut, c.a_guns, supg ? c.a_gel : nullptr},
                        NoFin{});
         FCFD_CHECK_LAUNCH();
         gUt = c.a_gut;

@@ -118,6 +118,7 @@ struct Ctx {
     // adjoint (VJP) work: scalar adjoint solution (nn); U cotangent with the scalar terms, the
     // scalar part of the U_n cotangent, the U_hat cotangent and the momentum adjoint solution (2nn)
     double *a_sx = nullptr, *a_gut = nullptr, *a_guns = nullptr, *a_gus = nullptr, *a_mx = nullptr;
+    double* a_gel = nullptr;  // SUPG element cotangents, 16 per element
     std::vector<void*> allocs;
 };
 
@@ -252,9 +253,11 @@ void projection_final(Ctx& c, const double* U_star, const double* P, double rho,
 // reverse-mode products of the steps (fcfd_adjoint.hip)
 void ns_vjp(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,
             const double* gP, const double* gUn_extra, double* gU_n, double* gP_n, int* it);
-void passive_vjp(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* phi, double dt,
-                 double rho, double mu, double D, const double* gU, const double* gP, const double* gphi,
-                 double* gU_n, double* gP_n, double* gphi_n, int* it);
+void passive_vjp(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* U,
+                 const double* phi, double dt, double rho, double mu, double D, int supg, const double* gU,
+                 const double* gP, const double* gphi, double* gU_n, double* gP_n, double* gphi_n, int* it);
+void supg_adjoint_elem(Ctx& c, const double* U_n, const double* U, const double* mu, const double* phi,
+                       const double* phi_n, double D, double dt, double* gel);
 
 // host sync helper: read device ints into the pinned mirror
 void read_state_ints(Ctx& c, const int* dev, int n, int* host);

@@ -31,8 +31,6 @@ class _Rollout:
         self.ctx, self.dt, self.rho, self.mu, self.D = ctx, dt, rho, mu, D
         self.U_d, self.phi_d, self.F_p, self.F = U_d, phi_d, F_p, F
         self.supg, self.passive = bool(supg), passive
-        if self.passive and self.supg:
-            raise NotImplementedError("reverse mode is built for the plain (non-SUPG) scalar step")
 
     def step(self, U, P, phi):
         c = self.ctx

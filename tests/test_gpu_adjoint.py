@@ -85,10 +85,10 @@ def test_ns_step_vjp_matches_oracle(set_zeroP):
     assert its[:3] == (fw["U_it"], fw["P_it"], p_it) and abs(its[3] - m_it) <= 1
 
 
-@pytest.mark.parametrize("phi_faces", [(), (3,)])
-def test_passive_step_vjp_matches_oracle(phi_faces):
+@pytest.mark.parametrize("phi_faces,supg", [((), False), ((3,), False), ((), True), ((3,), True)])
+def test_passive_step_vjp_matches_oracle(phi_faces, supg):
     n, dt, rho, mu, D = 32, 2e-3, 1.0, 1e-2, 1e-3
-    print(f"\n[case] passive step VJP {n}x{n} phi_faces={phi_faces}", flush=True)
+    print(f"\n[case] passive step VJP {n}x{n} phi_faces={phi_faces} supg={supg}", flush=True)
     (compute, timestep, coords, surf, h, nn, nnx, nny, gamma_d, U_d, gamma_d_phi), s, osurf = _setup(
         n, "passive", phi_faces=phi_faces)
     U_d = _lid(s, osurf, U_d, gamma_d)
@@ -98,16 +98,20 @@ def test_passive_step_vjp_matches_oracle(phi_faces):
     ctx = timestep.context
     F_p, F = compute(U_d, None, 0, 0)
     (gUn, gPn, gfn), (U, P, phi), its = ctx.passive_timestep_vjp(U_n, P_n, phi_n, dt, rho, mu, D, gU, gP, gphi,
-                                                                 U_d, phi_d, F_p, F)
+                                                                 U_d, phi_d, F_p, F, supg)
     oF_p = _np(F_p)
     ogUn, ogPn, ogfn, (oU, oP, ophi), oits = oadj.passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, oF_p,
-                                                              0, gamma_d_phi, gU, gP, gphi)
+                                                              0, gamma_d_phi, gU, gP, gphi, supg=supg)
     print(f"  iters {its[3:]} oracle {oits}  rel " + " ".join(
         f"{_rel(_np(a), b):.3e}" for a, b in ((gUn, ogUn), (gPn, ogPn), (gfn, ogfn))), flush=True)
     assert _rel(_np(phi), ophi) < 1e-10 and _rel(_np(U), oU) < 1e-10
     for a, b in ((gUn, ogUn), (gPn, ogPn), (gfn, ogfn)):
         assert _rel(_np(a), b) < 1e-9, _rel(_np(a), b)
-    assert its[4] == oits[1] and abs(its[3] - oits[0]) <= 1 and abs(its[5] - oits[2]) <= 1
+    # The transposed BiCGSTAB solves start from the forward x0 against a random cotangent; the
+    # iteration at which their residual first drops below tol moves with the summation order
+    # (GPU column gather vs the oracle's scatter), so the counts may differ by a few; the
+    # symmetric pressure CG count is exact.
+    assert its[4] == oits[1] and abs(its[3] - oits[0]) <= 2 and abs(its[5] - oits[2]) <= 2
 
 
 def _loss_setup(n, nsteps, tol=1e-10):
@@ -119,25 +123,31 @@ def _loss_setup(n, nsteps, tol=1e-10):
     return compute, timestep, s, nn, U_d, U0, P0, phi0, target
 
 
-def test_rollout_gradient_matches_oracle():
+@pytest.mark.parametrize("supg", [False, True])
+def test_rollout_gradient_matches_oracle(supg):
     import torch
     import jax_fem_cfd.setup.simulation as sim
     n, nsteps, dt, rho, mu, D = 16, 6, 2e-3, 1.0, 1e-2, 1e-3
-    print(f"\n[case] rollout gradient {n}x{n} steps={nsteps} checkpoint=4", flush=True)
+    print(f"\n[case] rollout gradient {n}x{n} steps={nsteps} checkpoint=4 supg={supg}", flush=True)
     compute, timestep, s, nn, U_d, U0, P0, phi0, target = _loss_setup(n, nsteps)
     dev = torch.device("cuda", 0)
     U0t = torch.tensor(U0, dtype=torch.float64, device=dev, requires_grad=True)
     phi0t = torch.tensor(phi0, dtype=torch.float64, device=dev, requires_grad=True)
     P0t = torch.tensor(P0, dtype=torch.float64, device=dev)
     tt = torch.tensor(target, dtype=torch.float64, device=dev)
-    U, P, phi = sim.timestep_loop_scan(dt, nsteps, compute, timestep, rho, mu, D, U_d, None, U0t, P0t, phi0t, None,
-                                       checkpoint=4)
+    if supg:
+        from jax_fem_cfd.solvers import adjoint
+        U, P, phi = adjoint.rollout(timestep.context, nsteps, U0t, P0t, phi0t, dt, rho, mu, D, U_d, supg=True,
+                                   checkpoint=4)
+    else:
+        U, P, phi = sim.timestep_loop_scan(dt, nsteps, compute, timestep, rho, mu, D, U_d, None, U0t, P0t, phi0t,
+                                           None, checkpoint=4)
     loss = 0.5 * torch.sum((phi - tt) ** 2)
     loss.backward()
-    _, _, ophi = oadj.rollout(s, nsteps, U0, P0, phi0, dt, rho, mu, D, U_d, None, 0, 0, None)
+    _, _, ophi = oadj.rollout(s, nsteps, U0, P0, phi0, dt, rho, mu, D, U_d, None, 0, 0, None, supg=supg)
     assert _rel(_np(phi), ophi) < 1e-10
     ogU, _, ogphi = oadj.rollout_vjp(s, nsteps, U0, P0, phi0, dt, rho, mu, D, U_d, None, 0, 0, None, None, None,
-                                     ophi - target, checkpoint=4)
+                                     ophi - target, checkpoint=4, supg=supg)
     assert _rel(_np(U0t.grad), ogU) < 1e-9, _rel(_np(U0t.grad), ogU)
     assert _rel(_np(phi0t.grad), ogphi) < 1e-9, _rel(_np(phi0t.grad), ogphi)
 
@@ -176,8 +186,6 @@ def test_vjp_argument_errors():
     (compute, timestep, coords, surf, h, nn, nnx, nny, gamma_d, U_d, gamma_d_phi), s, osurf = _setup(8, "passive")
     ctx = timestep.context
     U_n, P_n, phi_n = _state(coords, nn, gamma_d, U_d)
-    with pytest.raises(NativeError):
-        ctx.passive_timestep_vjp(U_n, P_n, phi_n, 1e-3, 1.0, 1e-2, 1e-3, gphi=np.ones(nn), U_d=U_d, supg=True)
     import jax_fem_cfd._native as nat
     Ud, Pd, fd = nat.as_dev(U_n), nat.as_dev(P_n), nat.as_dev(phi_n)
     with pytest.raises(NativeError):  # forward outputs must not alias the inputs
