@@ -54,10 +54,12 @@ enum {
 enum { FCFD_CE_F64 = 0, FCFD_CE_F32 = 1, FCFD_CE_F16S = 2, FCFD_CE_BF16S = 3 };
 
 typedef struct fcfd_mesh_desc {
-    int32_t ndim;          /* 2 */
+    int32_t ndim;          /* 2, or 3 for hex8 boxes (create_mesh_3d, regular_mesh.py:45-105) */
     int32_t nex, ney;      /* elements per direction */
     double lx, ly;         /* domain size */
-    int32_t periodic;      /* 1: create_periodic_mesh_2d, 0: create_mesh_2d */
+    int32_t periodic;      /* 1: create_periodic_mesh_2d, 0: create_mesh_2d (3D: must be 0) */
+    int32_t nez;           /* ndim = 3: elements in z */
+    double lz;             /* ndim = 3: domain size in z */
 } fcfd_mesh_desc;
 
 typedef struct fcfd_solver_desc {
@@ -75,8 +77,9 @@ typedef struct fcfd_solver_desc {
     int64_t n_gamma_p;
     const int64_t* mg_gamma_p;      /* per-level pins, concatenated              */
     const int64_t* mg_gamma_p_off;  /* mg_levels + 1 offsets into mg_gamma_p     */
-    const int64_t* bconn;       /* (n_bedge, 2) Dirichlet boundary edges         */
-    const int64_t* nvec;        /* (n_bedge, 2) outward normals                  */
+    const int64_t* bconn;       /* (n_bedge, 2) Dirichlet boundary edges; 3D:    */
+                                /* (n_bedge, 4) Dirichlet boundary faces         */
+    const int64_t* nvec;        /* (n_bedge, ndim) outward normals               */
     int64_t n_bedge;
     const int64_t* gamma_d_phi; /* passive-scalar Dirichlet nodes or NULL         */
     int64_t n_gamma_d_phi;
@@ -109,6 +112,14 @@ int fcfd_mesh_fill(fcfd_ctx* ctx, double* coords, int64_t* nconn);
  * replaces lumped_mass_matrix / laplacian_element_calc / gradient_element_calc
  * (jax_fem_cfd/core/element_calc.py:9-40). */
 int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16);
+/* 3D contexts: the hex8 constants (64 doubles each, row-major): which = 0 Me, 1 Le, 2..4 Ge[0..2].
+ * replaces lumped_mass_matrix / laplacian_element_calc / gradient_element_calc on hex8
+ * (element_calc.py:9-40, shape_functions.py:47-86). */
+int fcfd_element_constant3(const fcfd_ctx* ctx, int32_t which, double* out64);
+/* y = M_c x for every velocity component (ndim * nn, device): the momentum source term of
+ * compute (standard.py:184-186).  replaces consistent_mass_mult (element_mult.py:11-36; its
+ * ndim = 3 branch fails in the reference, here it is the per-component product). */
+int fcfd_mass_matvec(fcfd_ctx* ctx, const double* x, double* y);
 int fcfd_lumped_mass(fcfd_ctx* ctx, double* M);
 /* L_diag of level l (device out).  replaces laplacian_diag (element_calc.py:99-109) */
 int fcfd_laplacian_diag(fcfd_ctx* ctx, int32_t level, double* out);

@@ -60,6 +60,16 @@ static int run(fcfd_ctx* ctx, F&& f) {
     }
 }
 
+// entry points built for 2D contexts only
+template <class F>
+static int run2d(fcfd_ctx* ctx, const char* what, F&& f) {
+    if (ctx->c.dim != 2) {
+        g_last_error = std::string(what) + " is not available for 3D contexts";
+        return FCFD_ERR_UNSUPPORTED;
+    }
+    return run(ctx, std::forward<F>(f));
+}
+
 namespace {
 
 // L_diag and lumped mass on a level: per node, the sum over its incident elements (in element
@@ -138,7 +148,7 @@ bool coarse_inverse(const Level& L, std::vector<double>& inv) {
 
 void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void* stream) {
     FCFD_REQUIRE(m && s, "null descriptor");
-    FCFD_REQUIRE(m->ndim == 2, "only 2D meshes are supported by this build");
+    FCFD_REQUIRE(m->ndim == 2 || m->ndim == 3, "ndim must be 2 or 3");
     FCFD_REQUIRE(m->nex >= 1 && m->ney >= 1, "bad element counts");
     c.user_stream = (hipStream_t)stream;
     FCFD_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
@@ -148,8 +158,6 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
         const char* e = std::getenv("FCFD_NO_GRAPHS");
         c.use_graphs = !(e && e[0] == '1');
     }
-    c.g = make_grid(m->nex, m->ney, m->periodic);
-    c.lx = m->lx; c.ly = m->ly;
     c.tol = s->tol;
     c.set_zeroP = s->set_zeroP;
     c.precond = s->precond;
@@ -160,6 +168,12 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV ||
                      c.precond == FCFD_PC_MULTIGRID,
                  "unknown pressure preconditioner");
+    if (m->ndim == 3) {
+        build_ctx3(c, m, s);
+        return;
+    }
+    c.g = make_grid(m->nex, m->ney, m->periodic);
+    c.lx = m->lx; c.ly = m->ly;
     const Grid2D& g = c.g;
     const int nn = g.nn;
     quad4_constants(c.lx / g.nex, c.ly / g.ney, c.ec);
@@ -475,11 +489,15 @@ int fcfd_level_nodes(const fcfd_ctx* ctx, int32_t level, int64_t* nn, int32_t* n
 
 int fcfd_mesh_fill(fcfd_ctx* ctx, double* coords, int64_t* nconn) {
     if (!ctx) return FCFD_ERR_ARG;
-    return run(ctx, [&] { mesh_fill(ctx->c, coords, nconn); });
+    return run2d(ctx, "fcfd_mesh_fill", [&] { mesh_fill(ctx->c, coords, nconn); });
 }
 
 int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16) {
     if (!ctx || !out16) return FCFD_ERR_ARG;
+    if (ctx->c.dim != 2) {
+        g_last_error = "fcfd_element_constant returns 4 x 4 (2D) constants; use fcfd_element_constant3";
+        return FCFD_ERR_UNSUPPORTED;
+    }
     const ElemConst& e = ctx->c.ec;
     const double* src = which == 0 ? e.Me : which == 1 ? e.Le : which == 2 ? e.Gx : which == 3 ? e.Gy
                       : which == 4 ? e.Leoff : nullptr;
@@ -491,11 +509,40 @@ int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16) {
     return FCFD_OK;
 }
 
+int fcfd_element_constant3(const fcfd_ctx* ctx, int32_t which, double* out64) {
+    if (!ctx || !out64) return FCFD_ERR_ARG;
+    if (ctx->c.dim != 3) {
+        g_last_error = "fcfd_element_constant3 needs a 3D context";
+        return FCFD_ERR_UNSUPPORTED;
+    }
+    const ElemConst3& e = ctx->c.ec3;
+    const double* src = which == 0 ? e.Me : which == 1 ? e.Le : (which >= 2 && which <= 4) ? e.G[which - 2] : nullptr;
+    if (!src) {
+        g_last_error = "unknown element constant";
+        return FCFD_ERR_ARG;
+    }
+    std::memcpy(out64, src, 64 * sizeof(double));
+    return FCFD_OK;
+}
+
+int fcfd_mass_matvec(fcfd_ctx* ctx, const double* x, double* y) {
+    if (!ctx || !x || !y || x == y) return FCFD_ERR_ARG;
+    return run(ctx, [&] {
+        Ctx& c = ctx->c;
+        if (c.dim == 3) {
+            mass_matvec3(c, x, y);
+        } else {
+            elem_mult_const(c, 0, c.ec.Me, x, y, 0);
+            elem_mult_const(c, 0, c.ec.Me, x + c.g.nn, y + c.g.nn, 0);
+        }
+    });
+}
+
 int fcfd_lumped_mass(fcfd_ctx* ctx, double* M) {
     if (!ctx || !M) return FCFD_ERR_ARG;
     return run(ctx, [&] {
-        FCFD_HIP(hipMemcpyAsync(M, ctx->c.Mlump, sizeof(double) * 2 * ctx->c.g.nn, hipMemcpyDeviceToDevice,
-                                ctx->c.stream));
+        FCFD_HIP(hipMemcpyAsync(M, ctx->c.Mlump, sizeof(double) * ctx->c.dim * ctx->c.g.nn,
+                                hipMemcpyDeviceToDevice, ctx->c.stream));
     });
 }
 
@@ -510,36 +557,41 @@ int fcfd_laplacian_diag(fcfd_ctx* ctx, int32_t level, double* out) {
 int fcfd_conv_elem_calc(fcfd_ctx* ctx, const double* U, double beta, double* Ce) {
     if (!ctx || !U || !Ce) return FCFD_ERR_ARG;
     return run(ctx, [&] {
-        build_ce_api(ctx->c, U, beta, Ce);
+        if (ctx->c.dim == 3) build_ce3_api(ctx->c, U, beta, Ce);
+        else build_ce_api(ctx->c, U, beta, Ce);
     });
 }
 
 int fcfd_boundary_integral(fcfd_ctx* ctx, const double* U_d, double* F_p) {
     if (!ctx || !F_p) return FCFD_ERR_ARG;
-    return run(ctx, [&] { boundary_integral(ctx->c, U_d, F_p); });
+    return run(ctx, [&] {
+        if (ctx->c.dim == 3) boundary_integral3(ctx->c, U_d, F_p);
+        else boundary_integral(ctx->c, U_d, F_p);
+    });
 }
 
 int fcfd_elem_mult_const(fcfd_ctx* ctx, int32_t level, const double* A16, const double* x, double* y,
                          int32_t accumulate) {
     if (!ctx || !A16 || !x || !y) return FCFD_ERR_ARG;
-    return run(ctx, [&] { elem_mult_const(ctx->c, level, A16, x, y, accumulate); });
+    return run2d(ctx, "fcfd_elem_mult_const", [&] { elem_mult_const(ctx->c, level, A16, x, y, accumulate); });
 }
 
 int fcfd_elem_mult_stored(fcfd_ctx* ctx, const double* K0_16, const double* A1, double c1, const double* A2,
                           double c2, int32_t ncomp, const double* x, double* y) {
     if (!ctx || !x || !y || ncomp < 1 || ncomp > 3) return FCFD_ERR_ARG;
-    return run(ctx, [&] { elem_mult_general(ctx->c, K0_16, A1, c1, A2, c2, ncomp, x, y); });
+    return run2d(ctx, "fcfd_elem_mult_stored",
+                 [&] { elem_mult_general(ctx->c, K0_16, A1, c1, A2, c2, ncomp, x, y); });
 }
 
 int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z) {
     if (!ctx || !r || !z) return FCFD_ERR_ARG;
-    return run(ctx, [&] { vcycle(ctx->c, r, z); });
+    return run2d(ctx, "fcfd_vcycle", [&] { vcycle(ctx->c, r, z); });
 }
 
 int fcfd_pressure_cg(fcfd_ctx* ctx, const double* b, double* x, double tol, int32_t maxiter, int32_t* iters,
                      int32_t* converged) {
     if (!ctx || !b || !x) return FCFD_ERR_ARG;
-    return run(ctx, [&] {
+    return run2d(ctx, "fcfd_pressure_cg", [&] {
         bool conv = false;
         int it = pressure_cg(ctx->c, b, x, tol, maxiter > 0 ? maxiter : 1000, &conv);
         FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
@@ -552,7 +604,8 @@ int fcfd_solve_pressure(fcfd_ctx* ctx, const double* U_star, const double* P_n, 
                         double dt, double* P, int32_t* iters) {
     if (!ctx || !U_star || !P_n || !P) return FCFD_ERR_ARG;
     return run(ctx, [&] {
-        int it = solve_pressure(ctx->c, U_star, P_n, F_p, rho, dt, P);
+        int it = ctx->c.dim == 3 ? solve_pressure3(ctx->c, U_star, P_n, F_p, rho, dt, P)
+                                 : solve_pressure(ctx->c, U_star, P_n, F_p, rho, dt, P);
         FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
         if (iters) *iters = it;
     });
@@ -565,7 +618,10 @@ int fcfd_solve_momentum(fcfd_ctx* ctx, const double* U_n, const double* P_n, con
         Ctx& c = ctx->c;
         scatter_ud(c, U_d);
         int it;
-        if (Ce) {
+        if (c.dim == 3) {
+            FCFD_REQUIRE(!Ce, "3D contexts build Ce(U_n) themselves (pass Ce = NULL)");
+            it = solve_momentum3_from(c, U_n, P_n, rho, mu, dt, F, U_hat);
+        } else if (Ce) {
             double* em = api_ce_buffer(c);
             ce_from_api(c, Ce, em);
             it = solve_momentum(c, FCFD_CE_F64, em, nullptr, U_n, P_n, rho, mu, dt, F, U_hat);
@@ -584,7 +640,8 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
     if (!ctx || !U_n || !P_n || !U || !P) return FCFD_ERR_ARG;
     return run(ctx, [&] {
         int it[2];
-        four_step(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+        if (ctx->c.dim == 3) four_step3(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+        else four_step(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
         if (iters) {
             iters[0] = it[0];
             iters[1] = it[1];
@@ -595,7 +652,7 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
 int fcfd_solve_scalar(fcfd_ctx* ctx, const double* U_n, const double* U, const double* phi_n, double dt, double D,
                       const double* phi_d, int32_t supg, double* phi, int32_t* iters) {
     if (!ctx || !U_n || !U || !phi_n || !phi) return FCFD_ERR_ARG;
-    return run(ctx, [&] {
+    return run2d(ctx, "fcfd_solve_scalar", [&] {
         int it = solve_scalar(ctx->c, U_n, U, phi_n, dt, D, phi_d, supg, phi);
         if (iters) *iters = it;
     });
@@ -606,7 +663,7 @@ int fcfd_passive_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, c
                           const double* F_p, const double* F, int32_t supg, double* U, double* P, double* phi,
                           int32_t* iters) {
     if (!ctx || !U_n || !P_n || !phi_n || !U || !P || !phi) return FCFD_ERR_ARG;
-    return run(ctx, [&] {
+    return run2d(ctx, "fcfd_passive_timestep", [&] {
         Ctx& c = ctx->c;
         int it[3];
         four_step(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
@@ -621,7 +678,7 @@ int fcfd_four_step_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* 
                                 const double* gP, double* U, double* P, double* gU_n, double* gP_n, int32_t* iters) {
     if (!ctx || !U_n || !P_n || !U || !P || !gU_n || !gP_n) return FCFD_ERR_ARG;
     if (U == U_n || P == P_n || gU_n == U_n || gP_n == P_n) return FCFD_ERR_ARG;
-    return run(ctx, [&] {
+    return run2d(ctx, "fcfd_four_step_timestep_vjp", [&] {
         Ctx& c = ctx->c;
         int it[4];
         four_step(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
@@ -638,7 +695,7 @@ int fcfd_passive_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_
                               double* gphi_n, int32_t* iters) {
     if (!ctx || !U_n || !P_n || !phi_n || !U || !P || !phi || !gU_n || !gP_n || !gphi_n) return FCFD_ERR_ARG;
     if (U == U_n || P == P_n || phi == phi_n || gU_n == U_n || gP_n == P_n || gphi_n == phi_n) return FCFD_ERR_ARG;
-    return run(ctx, [&] {
+    return run2d(ctx, "fcfd_passive_timestep_vjp", [&] {
         Ctx& c = ctx->c;
         int it[6];
         four_step(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
@@ -652,7 +709,7 @@ int fcfd_passive_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_
 int fcfd_supg_elem_calc(fcfd_ctx* ctx, const double* U_n, const double* Ce, double D, double dt, double r, double* Ct,
                         double* tau) {
     if (!ctx || !U_n) return FCFD_ERR_ARG;
-    return run(ctx, [&] { supg_api(ctx->c, U_n, Ce, D, dt, r, Ct, tau); });
+    return run2d(ctx, "fcfd_supg_elem_calc", [&] { supg_api(ctx->c, U_n, Ce, D, dt, r, Ct, tau); });
 }
 
 int fcfd_dot(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out) {
@@ -667,7 +724,7 @@ int fcfd_axpby(fcfd_ctx* ctx, double a, const double* x, double b, double* y, in
 
 int fcfd_bench_momentum_apply(fcfd_ctx* ctx, int32_t reps, float* ms_per_apply) {
     if (!ctx || reps <= 0 || !ms_per_apply) return FCFD_ERR_ARG;
-    return run(ctx, [&] { *ms_per_apply = bench_momentum_apply(ctx->c, reps); });
+    return run2d(ctx, "fcfd_bench_momentum_apply", [&] { *ms_per_apply = bench_momentum_apply(ctx->c, reps); });
 }
 
 }  // extern "C"

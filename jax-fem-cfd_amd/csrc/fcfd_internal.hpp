@@ -7,6 +7,7 @@
 #include <string>
 #include <stdexcept>
 #include "fcfd_common.hpp"
+#include "fcfd_hex.hpp"
 #include "../../include/fcfd.h"
 
 namespace fcfd {
@@ -69,6 +70,14 @@ struct Ctx {
     double lx, ly;
     ElemConst ec;
     ElemConst* ec_dev = nullptr;
+    // 3D (hex8) contexts: dim = 3, g3 / ec3 describe the grid; g.nn and g.ne hold its node and
+    // element counts so that the dimension-free helpers (reductions, fills, graphs) apply as is
+    int dim = 2;
+    Grid3D g3{};
+    double lz = 0.0;
+    ElemConst3 ec3{};
+    ElemConst3* ec3_dev = nullptr;
+    double* fb_w = nullptr;     // 3D: per Dirichlet face its 4 x 4 weights sum_q w N_a N_b
     double tol;
     int set_zeroP, precond, precision, ce_format, maxit_mom, maxit_p;
     int nlev = 0;
@@ -249,6 +258,21 @@ int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double
 void projection_ustar(Ctx& c, const double* U_hat, const double* P_n, double rho, double dt, double* U_star);
 void projection_final(Ctx& c, const double* U_star, const double* P, double rho, double dt, double* U);
 
+
+// 3D (hex8) path (fcfd_3d.hip)
+void hex8_constants(double dx, double dy, double dz, ElemConst3& ec);
+void build_ctx3(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s);
+void four_step3(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
+                const double* F_p, const double* F, double* U, double* P, int* it);
+int solve_momentum3(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt, const double* F,
+                    double* U_hat);
+int solve_pressure3(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                    double* P_out);
+void boundary_integral3(Ctx& c, const double* U_d, double* F_p);
+void mass_matvec3(Ctx& c, const double* x, double* y);
+void build_ce3_api(Ctx& c, const double* U, double beta, double* Ce);
+int solve_momentum3_from(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt,
+                         const double* F, double* U_hat);
 
 // reverse-mode products of the steps (fcfd_adjoint.hip)
 void ns_vjp(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,

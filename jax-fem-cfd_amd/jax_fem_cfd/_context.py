@@ -19,17 +19,25 @@ class Context:
 
     def __init__(self, nex, ney, lx, ly, periodic, *, tol=1e-8, set_zeroP=False, precond=None, mg_levels=1,
                  precision="fp64", ce_format="f64", gamma_d=None, gamma_p=None, mg_gamma_p=None, bconn=None,
-                 nvec=None, maxiter_momentum=0, maxiter_pressure=0, gamma_d_phi=None):
+                 nvec=None, maxiter_momentum=0, maxiter_pressure=0, gamma_d_phi=None, nez=None, lz=None):
         lib = nat.load_library()
         t = nat.require_gpu()
         self.nex, self.ney, self.lx, self.ly, self.periodic = int(nex), int(ney), float(lx), float(ly), bool(periodic)
+        self.ndim = 2 if nez is None else 3
         self.nnx = self.nex if periodic else self.nex + 1
         self.nny = self.ney if periodic else self.ney + 1
         self.nn = self.nnx * self.nny
         self.ne = self.nex * self.ney
+        if self.ndim == 3:
+            self.nez, self.lz = int(nez), float(lz)
+            self.nnz = self.nez + 1
+            self.nn *= self.nnz
+            self.ne *= self.nez
+        self.nen = 2 ** self.ndim  # nodes per element
         self.tol, self.set_zeroP, self.precision, self.ce_format = tol, bool(set_zeroP), precision, ce_format
         self.precond = precond
-        md = nat.MeshDesc(2, self.nex, self.ney, self.lx, self.ly, int(self.periodic))
+        md = nat.MeshDesc(self.ndim, self.nex, self.ney, self.lx, self.ly, int(self.periodic),
+                          getattr(self, "nez", 0), getattr(self, "lz", 0.0))
         keep = []
 
         def arr(a):
@@ -97,13 +105,18 @@ class Context:
         return coords.view(self.nn, 2), nconn.view(self.ne, 4)
 
     def element_constant(self, which):
+        if self.ndim == 3:
+            out = (ctypes.c_double * 64)()
+            nat.check(self._lib.fcfd_element_constant3(
+                self._h, {"Me": 0, "Le": 1, "Gx": 2, "Gy": 3, "Gz": 4}[which], out), "fcfd_element_constant3")
+            return np.array(out[:]).reshape(8, 8)
         out = (ctypes.c_double * 16)()
         nat.check(self._lib.fcfd_element_constant(self._h, {"Me": 0, "Le": 1, "Gx": 2, "Gy": 3, "Le_off": 4}[which],
                                                   out), "fcfd_element_constant")
         return np.array(out[:]).reshape(4, 4)
 
     def lumped_mass(self):
-        M = self._empty(2 * self.nn)
+        M = self._empty(self.ndim * self.nn)
         self._sync_stream()
         nat.check(self._lib.fcfd_lumped_mass(self._h, nat.ptr(M)), "fcfd_lumped_mass")
         return M
@@ -125,10 +138,19 @@ class Context:
     # ---------------------------------------------------------------------------------- element ops
     def conv_elem_calc(self, U, beta):
         U = nat.as_dev(U)
-        Ce = self._empty(16 * self.ne)
+        k = self.nen
+        Ce = self._empty(k * k * self.ne)
         self._sync_stream()
         nat.check(self._lib.fcfd_conv_elem_calc(self._h, nat.ptr(U), float(beta), nat.ptr(Ce)), "fcfd_conv_elem_calc")
-        return Ce.view(self.ne, 4, 4)
+        return Ce.view(self.ne, k, k)
+
+    def mass_matvec(self, x):
+        """M_c x per velocity component (consistent_mass_mult, element_mult.py:11-36)."""
+        x = nat.as_dev(x)
+        y = self._empty(self.ndim * self.nn)
+        self._sync_stream()
+        nat.check(self._lib.fcfd_mass_matvec(self._h, nat.ptr(x), nat.ptr(y)), "fcfd_mass_matvec")
+        return y
 
     def boundary_integral(self, U_d):
         Fp = self._empty(self.nn)
@@ -187,9 +209,9 @@ class Context:
     def solve_momentum(self, U_n, P_n, rho, mu, dt, U_d=None, F=None, Ce=None):
         U_n, P_n = nat.as_dev(U_n), nat.as_dev(P_n)
         Ud = None if U_d is None else nat.as_dev(U_d)
-        Fv = _opt_vec(F, 2 * self.nn)
+        Fv = _opt_vec(F, self.ndim * self.nn)
         Ce = None if Ce is None else nat.as_dev(Ce)
-        out = self._empty(2 * self.nn)
+        out = self._empty(self.ndim * self.nn)
         it = ctypes.c_int32()
         self._sync_stream()
         nat.check(self._lib.fcfd_solve_momentum(self._h, nat.ptr(U_n), nat.ptr(P_n), nat.ptr(Ce), float(rho), float(mu),
@@ -201,9 +223,9 @@ class Context:
         U_n, P_n = nat.as_dev(U_n), nat.as_dev(P_n)
         Ud = None if U_d is None else nat.as_dev(U_d)
         Fp = _opt_vec(F_p, self.nn)
-        Fv = _opt_vec(F, 2 * self.nn)
+        Fv = _opt_vec(F, self.ndim * self.nn)
         if out is None:
-            U, P = self._empty(2 * self.nn), self._empty(self.nn)
+            U, P = self._empty(self.ndim * self.nn), self._empty(self.nn)
         else:
             U, P = out
         iters = (ctypes.c_int32 * 2)()

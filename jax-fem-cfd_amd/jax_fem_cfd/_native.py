@@ -25,7 +25,8 @@ class NativeError(RuntimeError):
 
 class MeshDesc(ctypes.Structure):
     _fields_ = [("ndim", ctypes.c_int32), ("nex", ctypes.c_int32), ("ney", ctypes.c_int32),
-                ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("periodic", ctypes.c_int32)]
+                ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("periodic", ctypes.c_int32),
+                ("nez", ctypes.c_int32), ("lz", ctypes.c_double)]
 
 
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -58,6 +59,8 @@ _SIGS = {
     "fcfd_mesh_fill": (ctypes.c_int, [_vp, _vp, _vp]),
     "fcfd_element_constant": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_d)]),
     "fcfd_lumped_mass": (ctypes.c_int, [_vp, _vp]),
+    "fcfd_element_constant3": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_d)]),
+    "fcfd_mass_matvec": (ctypes.c_int, [_vp, _vp, _vp]),
     "fcfd_laplacian_diag": (ctypes.c_int, [_vp, _i32, _vp]),
     "fcfd_conv_elem_calc": (ctypes.c_int, [_vp, _vp, _d, _vp]),
     "fcfd_boundary_integral": (ctypes.c_int, [_vp, _vp, _vp]),

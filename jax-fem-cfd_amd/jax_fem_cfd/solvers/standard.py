@@ -45,14 +45,23 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
                   gamma_d_phi=None):
     mc = config.mesh_config
     sc = ns_config if ns_config is not None else config.solver_config
-    if len(mc.num_elem) != 2 or mc.shape_func_ord != 1:
-        raise NotImplementedError("the GPU timestep is built for 2D Q1 meshes")
+    if mc.shape_func_ord != 1:
+        raise NotImplementedError("the GPU timestep is built for Q1 (quad4 / hex8) meshes")
+    ce_format = sc.ce_format or ("f64" if sc.precision == "fp64" else "f32")
+    if len(mc.num_elem) == 3:
+        if gamma_d_phi is not None:
+            raise NotImplementedError("the 3D GPU path has no passive scalar")
+        nex, ney, nez = mc.num_elem
+        return Context(nex, ney, mc.domain_size[0], mc.domain_size[1], False, tol=sc.solver_tol,
+                       set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=1, precision=sc.precision,
+                       ce_format=ce_format, gamma_d=gamma_d,
+                       gamma_p=np.atleast_1d(gamma_p) if (sc.set_zeroP or sc.pressure_precond == "approx_inv")
+                       else None, bconn=bconn, nvec=nvec, nez=nez, lz=mc.domain_size[2])
     nex, ney = mc.num_elem
     nlev = _mg_levels(mc.num_elem, sc.final_multigrid_mesh) if sc.pressure_precond == "multigrid" else 1
     mg_pins = None
     if sc.pressure_precond == "multigrid" and sc.set_zeroP:
         mg_pins = _mg_pins(mc.num_elem, mc.domain_size, sc.outlet_faces, mc.partial_dirichlet, mesh_func, nlev)
-    ce_format = sc.ce_format or ("f64" if sc.precision == "fp64" else "f32")
     return Context(nex, ney, mc.domain_size[0], mc.domain_size[1], mc.periodic, tol=sc.solver_tol,
                    set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=nlev, precision=sc.precision,
                    ce_format=ce_format, gamma_d=gamma_d,
@@ -73,9 +82,12 @@ def setup_solver(node_coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_fun
             F_p = ctx.boundary_integral(U_d)
         if S_func is not None:
             S = (nat.as_dev(S_func(t_n)) + nat.as_dev(S_func(t))) / 2
-            F = ctx._empty(2 * nn)
-            for d in range(2):
-                ctx.elem_mult_const(Me, S[d * nn:(d + 1) * nn], out=F[d * nn:(d + 1) * nn])
+            if ctx.ndim == 3:
+                F = ctx.mass_matvec(S)
+            else:
+                F = ctx._empty(2 * nn)
+                for d in range(2):
+                    ctx.elem_mult_const(Me, S[d * nn:(d + 1) * nn], out=F[d * nn:(d + 1) * nn])
         return F_p, F
 
     def timestep_loaded(U_n, P_n, dt, rho, mu, U_d, F_p, F):

@@ -129,6 +129,32 @@ def u_dot_n_boundary_integral_1d(bconn, nvec, gamma_d, U_d, nn, coords, shapefun
     return np.bincount(bconn.ravel(), Fe.ravel(), minlength=nn)
 
 
+def u_dot_n_boundary_integral_2d(bconn, nvec, gamma_d, U_d, nn, coords):
+    """F_p = int_Gamma q (u.n) over Dirichlet faces of a hex mesh, quad4 2x2 Gauss per face,
+    then scatter.  element_calc.py:137-163.  As in the reference every face is mapped from the
+    (z, y) coordinates of its nodes (columns [2, 1]), so faces of constant y or z have det J = 0
+    and contribute nothing; only w = det J and N are used (J^-1 is not formed, the reference
+    discards it)."""
+    ui = np.searchsorted(gamma_d, bconn)
+    vi = np.searchsorted(gamma_d, bconn + nn)
+    wi = np.searchsorted(gamma_d, bconn + 2 * nn)
+    Ub = np.stack([U_d[ui], U_d[vi], U_d[wi]], axis=-1)              # (nf, I, 3)
+    xi = np.array([-1.0, 1.0, 1.0, -1.0]) * _S3
+    eta = np.array([-1.0, -1.0, 1.0, 1.0]) * _S3
+    N = np.stack([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta),
+                  (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)], axis=1) / 4
+    dNdxi = np.stack([-(1 - eta), 1 - eta, 1 + eta, -(1 + eta)], axis=1) / 4
+    dNdeta = np.stack([-(1 - xi), -(1 + xi), 1 + xi, 1 - xi], axis=1) / 4
+    P = coords[bconn][:, :, [2, 1]]                                  # (nf, I, 2)
+    Jx = np.einsum("qi,fic->fqc", dNdxi, P)
+    Je = np.einsum("qi,fic->fqc", dNdeta, P)
+    wq = Jx[..., 0] * Je[..., 1] - Jx[..., 1] * Je[..., 0]           # (nf, q) = det J
+    Uq = np.einsum("qi,fid->fqd", N, Ub)
+    un = np.einsum("fqd,fd->fq", Uq, nvec.astype(float))
+    Fe = np.einsum("qi,fq,fq->fi", N, wq, un)
+    return np.bincount(bconn.ravel(), Fe.ravel(), minlength=nn)
+
+
 def Ctau_tilde_elem_calc(nconn, U, wq, N, Nderiv, nn, ndim):
     """Ct[a,b] = sum_q w dN_qa^T (U_q U_q^T) dN_qb.  element_calc.py:188-200."""
     Ul = element_velocity(nconn, U, nn, ndim)

@@ -159,6 +159,8 @@ def compute(s, U_d, S_func, t_n, t):
     F, F_p = 0, 0
     if s.gamma_d is not None and s.ndim == 2:
         F_p = el.u_dot_n_boundary_integral_1d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords)
+    elif s.gamma_d is not None:
+        F_p = el.u_dot_n_boundary_integral_2d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords)
     if S_func is not None:
         S = (S_func(t_n) + S_func(t)) / 2
         F = emult.consistent_mass_mult(S, s.Me, s.nconn, s.nn, s.ndim)
@@ -219,6 +221,14 @@ def taylor_green(coords, t, rho, mu):
 def cavity_lid(s, surf, U_d, u_top=1.0):
     """Lid u = u_top on the top face without its corners.  cavity.py:115-117."""
     return ms.set_face_velocity(U_d, surf[2][1:-1], s.gamma_d, u_top, s.nn, 1)
+
+
+def channel_inflow(s, surf, U_d, ly, lz, u_max=1.0):
+    """3D channel: parabolic inflow u = 16 u_max (y/ly)(1-y/ly)(z/lz)(1-z/lz) on the left
+    face (surfnodes[3]); the profile vanishes on the face edges, matching the no-slip walls."""
+    nodes = surf[3]
+    y, z = s.coords[nodes, 1] / ly, s.coords[nodes, 2] / lz
+    return ms.set_face_velocity(U_d, nodes, s.gamma_d, 16 * u_max * y * (1 - y) * z * (1 - z), s.nn, 1)
 
 
 # --------------------------------------------------------------------------------------------
