@@ -382,18 +382,6 @@ __device__ __forceinline__ void op_apply3(const Op& op, const Nbr3& s, const dou
     }
 }
 
-// A y with Dirichlet rows replaced by the identity
-template <int NC, class Op>
-__device__ __forceinline__ void masked_apply3(const Op& op, const Nbr3& s, int n, int nn, const uint8_t* mask,
-                                              const double (&X)[NC][27], double (&y)[NC]) {
-    op_apply3<NC>(op, s, X, y);
-    if (mask) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-            if (mask[n + c * nn]) y[c] = X[c][S3C];
-    }
-}
-
 // rhs = R U_n - G P_n + F (Dirichlet rows U_d), x0 = U_n with Dirichlet values, r0 = rhs - A x0,
 // rhat = p = q = r0; reductions b.b, r0.r0 (the 2D BInit with 3 components)
 template <class Op>
@@ -447,63 +435,99 @@ struct BInit3 {
     }
 };
 
-// p = r + beta (p - omega q) on the fly, q = A p, partial rhat.q
+// LDS-staged forms: the on-the-fly vector (p or s, 3 components) staged per block; the element
+// rows are applied to the staged stencil directly, one element at a time (a rolled loop keeps the
+// live set to one row; the element id and validity are recomputed per element so that no private
+// array is indexed at run time)
 template <class Op>
-struct BK1_3 {
+__device__ __forceinline__ void op_apply3s(const Op& op, const StageView3<3>& sv, int n, double (&y)[3]) {
+    const Grid3D& g = op.g;
+    const int kz = n / g.nxy, rr = n - kz * g.nxy;
+    const int jy = rr / g.nnx, ix = rr - jy * g.nnx;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c] = 0.0;
+#pragma unroll 1
+    for (int k = 0; k < 8; ++k) {
+        const int bx = k & 1, by = (k >> 1) & 1, bz = k >> 2;
+        const int ie = ix - 1 + bx, je = jy - 1 + by, ke = kz - 1 + bz;
+        const bool has = ie >= 0 && ie < g.nex && je >= 0 && je < g.ney && ke >= 0 && ke < g.nez;
+        const int e = has ? (ke * g.ney + je) * g.nex + ie : 0;
+        const int a = hloc(1 - bx, 1 - by, 1 - bz);
+        double cf[8];
+        op.row(a, e, cf);
+        double t[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const int slot = ((hbz(b) + bz) * 3 + (hby(b) + by)) * 3 + (hbx(b) + bx);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) t[c] += cf[b] * sv.at(c, n, slot);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[c] += has ? t[c] : 0.0;
+    }
+}
+
+template <class Op>
+struct BK1S3 {
+    static constexpr int NV = 3;
     Op op;
     const uint8_t* mask;
     const double *r, *pold, *qold, *rhat;
     double *pnew, *qnew;
     const DevState* st;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        const int nn = op.g.nn;
+    __device__ __forceinline__ int nnx() const { return op.g.nnx; }
+    __device__ __forceinline__ int nxy() const { return op.g.nxy; }
+    __device__ __forceinline__ void stage(int id, double (&v)[3]) const {
         const double beta = st->b_beta, omega = st->b_omega;
-        Nbr3 s;
-        node_nbr3(op.g, n, s);
-        double X[3][27];
+        const int nn = op.g.nn;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const int o = c * nn;
-            load27(s, [&](int k) { return r[k + o] + beta * (pold[k + o] - omega * qold[k + o]); }, X[c]);
+            const int i = id + c * nn;
+            v[c] = r[i] + beta * (pold[i] - omega * qold[i]);
         }
+    }
+    __device__ void operator()(int n, const StageView3<3>& sv, double (&acc)[1]) const {
+        const int nn = op.g.nn;
         double y[3];
-        masked_apply3<3>(op, s, n, nn, mask, X, y);
+        op_apply3s(op, sv, n, y);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int i = n + c * nn;
-            pnew[i] = X[c][S3C];
-            qnew[i] = y[c];
-            acc[0] += rhat[i] * y[c];
+            const double pc = sv.at(c, n, S3C);
+            const double yc = (mask && mask[i]) ? pc : y[c];
+            pnew[i] = pc;
+            qnew[i] = yc;
+            acc[0] += rhat[i] * yc;
         }
     }
 };
 
-// s = r - alpha q on the fly, t = A s, partials s.s, t.s, t.t
 template <class Op>
-struct BK2_3 {
+struct BK2S3 {
+    static constexpr int NV = 3;
     Op op;
     const uint8_t* mask;
     const double *r, *q;
-    double *sv, *tv;
+    double *sv_, *tv;
     const DevState* st;
-    __device__ void operator()(int n, double (&acc)[3]) const {
-        const int nn = op.g.nn;
+    __device__ __forceinline__ int nnx() const { return op.g.nnx; }
+    __device__ __forceinline__ int nxy() const { return op.g.nxy; }
+    __device__ __forceinline__ void stage(int id, double (&v)[3]) const {
         const double alpha = st->b_alpha;
-        Nbr3 s;
-        node_nbr3(op.g, n, s);
-        double X[3][27];
+        const int nn = op.g.nn;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int o = c * nn;
-            load27(s, [&](int k) { return r[k + o] - alpha * q[k + o]; }, X[c]);
-        }
+        for (int c = 0; c < 3; ++c) v[c] = r[id + c * nn] - alpha * q[id + c * nn];
+    }
+    __device__ void operator()(int n, const StageView3<3>& sv, double (&acc)[3]) const {
+        const int nn = op.g.nn;
         double y[3];
-        masked_apply3<3>(op, s, n, nn, mask, X, y);
+        op_apply3s(op, sv, n, y);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int i = n + c * nn;
-            const double sc = X[c][S3C], yc = y[c];
-            sv[i] = sc;
+            const double sc = sv.at(c, n, S3C);
+            const double yc = (mask && mask[i]) ? sc : y[c];
+            sv_[i] = sc;
             tv[i] = yc;
             acc[0] += sc * sc;
             acc[1] += yc * sc;
@@ -511,6 +535,15 @@ struct BK2_3 {
         }
     }
 };
+
+template <int NR, class Body, class Fin>
+static void launch_stencil3(Ctx& c, int n, const int* active, const Body& body, const Fin& fin) {
+    const int nb = blocks_for(n);
+    if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+    stencil3_kernel<NR><<<nb, BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
+    if constexpr (NR > 0)
+        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, fin, c.partials, c.pstride);
+}
 
 template <class S>
 static int solve_momentum3_t(Ctx& c, S ce, const double* U_n, const double* P_n, double rho, double mu, double dt,
@@ -530,12 +563,12 @@ static int solve_momentum3_t(Ctx& c, S ce, const double* U_n, const double* P_n,
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_node<1>(c, nn, act,
-                           BK1_3<MomCoef3<S>>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw],
-                                              c.st},
-                           BK1Fin{c.st});
-            launch_node<3>(c, nn, act, BK2_3<MomCoef3<S>>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
-                           BK2Fin{c.st});
+            launch_stencil3<1>(c, nn, act,
+                               BK1S3<MomCoef3<S>>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
+                                                  c.m_q[nw], c.st},
+                               BK1Fin{c.st});
+            launch_stencil3<3>(c, nn, act, BK2S3<MomCoef3<S>>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                               BK2Fin{c.st});
             launch_node<2>(c, nn, act, BK3<3>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
                            BK3Fin{c.st});
         }
@@ -574,18 +607,22 @@ int solve_momentum3_from(Ctx& c, const double* U_n, const double* P_n, double rh
 // ------------------------------------------------------------------------------ projection
 // U* = U_hat + (dt/rho) G P_n / M   (standard.py:156); sgn = -1: U = U* - (dt/rho) G P / M with
 // U[gamma_d] = U_d   (standard.py:160-163)
-struct Proj3Body {
+struct Proj3S {
+    static constexpr int NV = 1;
     Grid3D g;
     const ElemConst3* ec;
     const double *Uin, *P, *M, *ud;
     const uint8_t* mask;
     double* Uout;
     double c;
-    __device__ void operator()(int n, double (&)[1]) const {
+    __device__ __forceinline__ int nnx() const { return g.nnx; }
+    __device__ __forceinline__ int nxy() const { return g.nxy; }
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = P[id]; }
+    __device__ void operator()(int n, const StageView3<1>& sv, double (&)[1]) const {
         Nbr3 s;
-        node_nbr3(g, n, s);
+        node_elems3(g, n, s);
         double X[27];
-        load27(s, [&](int k) { return P[k]; }, X);
+        view27(sv, 0, n, X);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const int i = n + d * g.nn;
@@ -642,23 +679,6 @@ struct CgInit3 {
     }
 };
 
-// p = z + beta p_old (on the fly), Ap = L p (pinned rows 0), partial p.Ap
-struct CgK1_3 {
-    Grid3D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
-    const DevState* st;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        const double beta = st->c_beta;
-        Nbr3 s;
-        node_nbr3(g, n, s);
-        double X[27];
-        load27(s, [&](int k) { return z[k] + beta * pold[k]; }, X);
-        const double a = (pin && pin[n]) ? 0.0 : ebe3(s, Le, X);
-        pnew[n] = X[S3C];
-        Ap[n] = a;
-        acc[0] += X[S3C] * a;
-    }
-};
-
 // two-term Neumann-series approximate inverse (standard.py:225-234)
 struct ApproxInv3a {
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
@@ -682,6 +702,91 @@ struct ApproxInv3b {
         if (pin && pin[n]) l2 = 0.0;
         const double d1x = r[n] / Ld[n];
         z[n] = (3.0 * d1x - 3.0 * X[S3C]) + l2 / Ld[n];
+    }
+};
+
+// LDS-staged forms of CgK1_3 / ApproxInv3a / ApproxInv3b (the latter fused with the CG update
+// z_new = M r, partial r.z_new that follows it)
+struct CgK1S3 {
+    static constexpr int NV = 1;
+    Grid3D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
+    const DevState* st;
+    __device__ __forceinline__ int nnx() const { return g.nnx; }
+    __device__ __forceinline__ int nxy() const { return g.nxy; }
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = z[id] + st->c_beta * pold[id]; }
+    __device__ void operator()(int n, const StageView3<1>& sv, double (&acc)[1]) const {
+        Nbr3 s;
+        node_elems3(g, n, s);
+        double X[27];
+        view27(sv, 0, n, X);
+        const double a = (pin && pin[n]) ? 0.0 : ebe3(s, Le, X);
+        pnew[n] = X[S3C];
+        Ap[n] = a;
+        acc[0] += X[S3C] * a;
+    }
+};
+struct ApproxInv3aS {
+    static constexpr int NV = 1;
+    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
+    __device__ __forceinline__ int nnx() const { return g.nnx; }
+    __device__ __forceinline__ int nxy() const { return g.nxy; }
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = r[id] / Ld[id]; }
+    __device__ void operator()(int n, const StageView3<1>& sv, double (&)[1]) const {
+        Nbr3 s;
+        node_elems3(g, n, s);
+        double X[27];
+        view27(sv, 0, n, X);
+        const double v = ebe3(s, Le, X) / Ld[n];
+        d1l[n] = (pin && pin[n]) ? 0.0 : v;
+    }
+};
+struct ApproxInv3bK3S {
+    static constexpr int NV = 1;
+    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l;
+    double* z;
+    __device__ __forceinline__ int nnx() const { return g.nnx; }
+    __device__ __forceinline__ int nxy() const { return g.nxy; }
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = d1l[id]; }
+    __device__ void operator()(int n, const StageView3<1>& sv, double (&acc)[2]) const {
+        Nbr3 s;
+        node_elems3(g, n, s);
+        double X[27];
+        view27(sv, 0, n, X);
+        double l2 = ebe3(s, Le, X);
+        if (pin && pin[n]) l2 = 0.0;
+        const double d1x = r[n] / Ld[n];
+        const double zk = (3.0 * d1x - 3.0 * X[S3C]) + l2 / Ld[n];
+        z[n] = zk;
+        acc[0] += r[n] * zk;
+    }
+};
+
+// x += alpha p ; r -= alpha Ap ; z_new = r / L_diag (jacobi) or r (none) ; partials r.r, r.z_new:
+// the CG update, the preconditioner and the new gamma in one pass (CgK2 + JacobiPrec + CgK3)
+struct CgK23 {
+    const double* p; const double* Ap; const double* Ld; double* x; double* r; double* znew; const DevState* st;
+    __device__ void operator()(int k, double (&acc)[2]) const {
+        const double a = st->c_alpha;
+        x[k] = x[k] + a * p[k];
+        const double rk = r[k] - a * Ap[k];
+        r[k] = rk;
+        const double zk = Ld ? rk / Ld[k] : rk;
+        znew[k] = zk;
+        acc[0] += rk * rk;
+        acc[1] += rk * zk;
+    }
+};
+// CgK2Fin then CgK3Fin (flex = 0)
+struct CgK23Fin {
+    DevState* st;
+    __device__ void operator()(const double (&t)[2]) const {
+        st->c_rr = t[0];
+        const double gnew = t[1];
+        st->c_beta = gnew / st->c_gamma;
+        st->c_gamma = gnew;
+        const int k = st->c_k + 1;
+        st->c_k = k;
+        st->c_active = (st->c_rr > st->c_atol2) && (k < st->c_maxit);
     }
 };
 
@@ -715,14 +820,23 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
     const double* y0 = precondition3(c, act);
     launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, y0, c.c_z[0], c.c_p[0], c.st, 0}, CgGamma0Fin{c.st});
     FCFD_CHECK_LAUNCH();
+    const double* Ld = c.lv[0].Ld;
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_node<1>(c, nn, act, CgK1_3{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                           CgK1Fin{c.st});
-            launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
-            const double* y = precondition3(c, act);
-            launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, 0, 0}, CgK3Fin{c.st, 0});
+            launch_stencil3<1>(c, nn, act, CgK1S3{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                               CgK1Fin{c.st});
+            if (c.precond == FCFD_PC_APPROX_INV) {
+                launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
+                launch_stencil3<0>(c, nn, act, ApproxInv3aS{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x}, NoFin{});
+                launch_stencil3<2>(c, nn, act, ApproxInv3bK3S{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x, c.c_z[nw]},
+                                   CgK3Fin{c.st, 0});
+            } else {
+                launch_node<2>(c, nn, act,
+                               CgK23{c.c_p[nw], c.c_Ap, c.precond == FCFD_PC_JACOBI ? Ld : nullptr, c.c_x, c.c_r,
+                                     c.c_z[nw], c.st},
+                               CgK23Fin{c.st});
+            }
         }
         FCFD_CHECK_LAUNCH();
     };
@@ -817,12 +931,12 @@ void four_step3(Ctx& c, const double* U_n, const double* P_n, double dt, double 
     scatter_ud(c, U_d);
     build_ce3(c, U_n, 0.5);
     it[0] = solve_momentum3(c, U_n, P_n, rho, mu, dt, F, c.m_x);
-    launch_node<0>(c, c.g3.nn, nullptr, Proj3Body{c.g3, c.ec3_dev, c.m_x, P_n, c.Mlump, nullptr, nullptr, c.ustar, dt / rho},
-                   NoFin{});
+    launch_stencil3<0>(c, c.g3.nn, nullptr,
+                       Proj3S{c.g3, c.ec3_dev, c.m_x, P_n, c.Mlump, nullptr, nullptr, c.ustar, dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
     it[1] = solve_pressure3(c, c.ustar, P_n, F_p, rho, dt, P);
-    launch_node<0>(c, c.g3.nn, nullptr,
-                   Proj3Body{c.g3, c.ec3_dev, c.ustar, P, c.Mlump, c.ud_full, c.dmask, U, -dt / rho}, NoFin{});
+    launch_stencil3<0>(c, c.g3.nn, nullptr,
+                       Proj3S{c.g3, c.ec3_dev, c.ustar, P, c.Mlump, c.ud_full, c.dmask, U, -dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
 }
 

@@ -244,6 +244,25 @@ __device__ __forceinline__ void block_sum(double (&v)[NR]) {
 // One item (node / element) per thread; logical block lb owns items [lb*BLOCK, lb*BLOCK+BLOCK).
 inline int blocks_for(int n) { return n > 0 ? (n + BLOCK - 1) / BLOCK : 1; }
 
+constexpr int FIN_BLOCK = 1024;  // threads of the one-block finalize kernel
+
+struct NoFin {
+    template <class T>
+    __device__ void operator()(const T&) const {}
+};
+
+// Epilogue of a reducing kernel: the block totals go to partials[r * pstride + lb]; the
+// finalize step runs as its own one-block kernel (finalize_kernel) right after.
+template <int NR>
+__device__ __forceinline__ void reduce_epilogue(double (&acc)[NR], double* __restrict__ partials, int pstride,
+                                                int lb) {
+    block_sum<NR>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) partials[r * pstride + lb] = acc[r];
+    }
+}
+
 // Generic node kernel with NR fused reductions.  Body: void operator()(int n, double (&acc)[NR]).
 // Each block reduces its per-thread sums in a fixed tree and stores one partial per reduction
 // (plain stores); finalize_kernel (one block, launched right after on the same stream) sums the
@@ -261,13 +280,7 @@ __global__ __launch_bounds__(BLOCK) void node_kernel(int n, const int* __restric
 #pragma unroll
     for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
     if (k < n) body(k, acc);
-    if constexpr (NR > 0) {
-        block_sum<NR>(acc);
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int r = 0; r < NR; ++r) partials[r * pstride + lb] = acc[r];
-        }
-    }
+    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -335,16 +348,8 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(int n, const int* __rest
         const int k = n0 + threadIdx.x + m * BLOCK;
         if (k < n) body(k, StageView<NV, NPT>{sm, n0, nnx}, acc);
     }
-    if constexpr (NR > 0) {
-        block_sum<NR>(acc);
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int r = 0; r < NR; ++r) partials[r * pstride + lb] = acc[r];
-        }
-    }
+    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
 }
-
-constexpr int FIN_BLOCK = 1024;
 
 // Fin: void operator()(const double (&tot)[NR]) run by thread 0 with the grid totals.
 template <int NR, class Fin>
@@ -361,10 +366,5 @@ __global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(int nb, const int* 
     block_sum<NR>(tot);
     if (threadIdx.x == 0) fin(tot);
 }
-
-struct NoFin {
-    template <class T>
-    __device__ void operator()(const T&) const {}
-};
 
 }  // namespace fcfd

@@ -109,6 +109,87 @@ __device__ __forceinline__ double ebe3(const Nbr3& s, const double* __restrict__
     return acc;
 }
 
+// ---------------------------------------------------------------------------------------------
+// LDS-staged 27-point kernels.  The neighbours of the nodes [n0, n0 + BLOCK) of a block are the
+// linear ids n + dz*nxy + dy*nnx + dx, which lie in nine contiguous windows
+// [n0 + dz*nxy + dy*nnx - 1, ... + BLOCK + 1).  The block stages those windows once (coalesced
+// loads; the body forms the staged value, e.g. z + beta p, once per entry) and every node reads
+// its stencil from LDS.  Ids that wrap across a row or layer end belong only to missing
+// elements (masked by has[k]); ids outside [0, nn) are staged as 0.
+// ---------------------------------------------------------------------------------------------
+constexpr int W3 = BLOCK + 2;
+
+template <int NV>
+struct StageView3 {
+    const double (*s)[9][W3];
+    int n0;
+    // stencil slot t = ((dz+1)*3 + (dy+1))*3 + (dx+1) of node n
+    __device__ __forceinline__ double at(int v, int n, int t) const { return s[v][t / 3][n - n0 + (t % 3)]; }
+};
+
+// Body: static constexpr int NV; int nnx() const; int nxy() const;
+//       void stage(int id, double (&v)[NV]) const;
+//       void operator()(int n, const StageView3<NV>&, double (&acc)[NR]) const.
+template <int NR, class Body>
+__global__ __launch_bounds__(BLOCK) void stencil3_kernel(int n, const int* __restrict__ active, Body body,
+                                                        double* __restrict__ partials, int pstride) {
+    if (active && *active == 0) return;
+    constexpr int NV = Body::NV;
+    __shared__ double sm[NV][9][W3];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int n0 = lb * BLOCK;
+    const int nnx = body.nnx(), nxy = body.nxy();
+    // all of a thread's staging loads are issued before the first LDS store (a fixed trip count,
+    // unrolled), so their latencies overlap instead of adding up
+    constexpr int NIT = (9 * W3 + BLOCK - 1) / BLOCK;
+    double v[NIT][NV];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int idx = threadIdx.x + it * BLOCK;
+        const int w = idx / W3, off = idx - w * W3;
+        const int id = n0 + (w / 3 - 1) * nxy + (w % 3 - 1) * nnx - 1 + off;
+        const bool in = idx < 9 * W3 && id >= 0 && id < n;
+        body.stage(in ? id : 0, v[it]);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[it][q] = in ? v[it][q] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int idx = threadIdx.x + it * BLOCK;
+        if (idx < 9 * W3) {
+            const int w = idx / W3, off = idx - w * W3;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sm[q][w][off] = v[it][q];
+        }
+    }
+    __syncthreads();
+    double acc[NR > 0 ? NR : 1];
+#pragma unroll
+    for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
+    const int k = n0 + threadIdx.x;
+    if (k < n) body(k, StageView3<NV>{sm, n0}, acc);
+    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
+}
+
+// incident elements of node n (ids clamped to 0 where missing)
+__device__ __forceinline__ void node_elems3(const Grid3D& g, int n, Nbr3& s) {
+    const int k = n / g.nxy, r = n - k * g.nxy;
+    const int j = r / g.nnx, i = r - j * g.nnx;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int ie = i - 1 + (q & 1), je = j - 1 + ((q >> 1) & 1), ke = k - 1 + (q >> 2);
+        const bool h = ie >= 0 && ie < g.nex && je >= 0 && je < g.ney && ke >= 0 && ke < g.nez;
+        s.has[q] = h;
+        s.e[q] = h ? (ke * g.ney + je) * g.nex + ie : 0;
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void view27(const StageView3<NV>& sv, int v, int n, double (&X)[27]) {
+#pragma unroll
+    for (int t = 0; t < 27; ++t) X[t] = sv.at(v, n, t);
+}
+
 // Row-blocked 8 x 8 storage: entry (a, b) of element e at ((a * ne + e) * 8 + b); a node reads
 // its row of an incident element with contiguous vector loads (32 B FP32, 64 B FP64).
 __host__ __device__ __forceinline__ size_t ce3_idx(int k, int e, int ne) {
