@@ -5,7 +5,7 @@
 //   BiCGSTAB on (rho/dt) M_c + 1/2 (rho Ce + mu Le) for the 3 velocity components with Dirichlet
 //   rows -> identity (standard.py:14-43), U* = U_hat + (dt/rho) G P_n / M, PCG on the pressure
 //   Laplacian (standard.py:85-125) preconditioned by jacobi / approx_inv / none -- the reference
-//   has no 3D multigrid (simulation.py:80-81) -- and U = U* - (dt/rho) G P / M, U[gamma_d] = U_d.
+//   has no 3D multigrid (simulation.py:75-76) -- and U = U* - (dt/rho) G P / M, U[gamma_d] = U_d.
 // The reference's consistent_mass_mult fails for ndim = 3 (element_mult.py:26 reads an undefined
 // y_result); the mass operator here is the per-component product of its 2D form.
 //

@@ -4,7 +4,7 @@
 // Mesh of create_mesh_3d (jax_fem_cfd/setup/regular_mesh.py:45-105): node (i, j, k) ->
 // k*nnx*nny + j*nnx + i at (i dx, j dy, k dz); element (ie, je, ke) -> (ke*ney + je)*nex + ie
 // with corners [quad at ke] ++ [quad at ke+1], quad = (ie,je), (ie+1,je), (ie+1,je+1), (ie,je+1).
-// The reference has no periodic 3D mesh (simulation.py:73-76 rejects it), so neither has this.
+// The reference has no periodic 3D mesh (simulation.py:77-78 rejects it), so neither has this.
 #pragma once
 #include "fcfd_common.hpp"
 
