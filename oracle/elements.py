@@ -57,7 +57,45 @@ def hex8_shape_functions(coords):
     return wq, N, Nderiv
 
 
-def shape_functions_for(ndim):
+_S35 = np.sqrt(3.0 / 5.0)
+
+
+def lin3_shape_functions(coords):
+    """3-node line, 3-point Gauss (-sqrt(3/5), 0, sqrt(3/5)), weights (5, 8, 5)/9 scaled by
+    |dx/dxi|.  shape_functions.py:89-106."""
+    xi = np.array([-_S35, 0.0, _S35])
+    w = np.array([5.0, 8.0, 5.0]) / 9
+    N = np.stack([xi * (xi - 1) / 2, 1 - xi ** 2, xi * (xi + 1) / 2], axis=1)
+    dN = np.stack([(2 * xi - 1) / 2, -2 * xi, (2 * xi + 1) / 2], axis=1)
+    J = np.sqrt((dN @ coords[:, 0]) ** 2 + (dN @ coords[:, 1]) ** 2)
+    return (w * J)[:, None], N, (dN / J[:, None])[:, None, :]
+
+
+def quad9_shape_functions(coords):
+    """Biquadratic 9-node quad, 3x3 Gauss; q = (eta index) * 3 + (xi index), node order of
+    create_mesh_2d_quad9.  shape_functions.py:108-159."""
+    x1 = np.array([-_S35, 0.0, _S35])
+    w1 = np.array([5.0, 8.0, 5.0]) / 9
+    xi, eta = np.meshgrid(x1, x1)
+    xi, eta = xi.ravel(), eta.ravel()
+    wq = np.outer(w1, w1).ravel()
+    # 1D quadratic Lagrange factors at local positions 0, 1, 2 (xi = -1, 0, 1) and derivatives
+    L = lambda t: [t * (t - 1) / 2, 1 - t ** 2, t * (t + 1) / 2]
+    dL = lambda t: [(2 * t - 1) / 2, -2 * t, (2 * t + 1) / 2]
+    pos = [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (1, 2), (0, 2), (0, 1), (1, 1)]
+    Lx, Ly, dLx, dLy = L(xi), L(eta), dL(xi), dL(eta)
+    N = np.stack([Lx[a] * Ly[b] for a, b in pos], axis=1)
+    dNdxi = np.stack([dLx[a] * Ly[b] for a, b in pos], axis=1)
+    dNdeta = np.stack([Lx[a] * dLy[b] for a, b in pos], axis=1)
+    J = np.stack([dNdxi @ coords, dNdeta @ coords], axis=1)
+    w = wq * np.linalg.det(J)
+    Nderiv = np.linalg.inv(J) @ np.stack([dNdxi, dNdeta], axis=1)
+    return w[:, None], N, Nderiv
+
+
+def shape_functions_for(ndim, shape_func_ord=1):
+    if ndim == 2 and shape_func_ord == 2:
+        return quad9_shape_functions, lin3_shape_functions
     if ndim == 2:
         return quad4_shape_functions, lin2_shape_functions
     return hex8_shape_functions, quad4_shape_functions

@@ -1,4 +1,4 @@
-"""Structured Q1 meshes and boundary-condition index sets (oracle, test-only).
+"""Structured Q1 / Q2 meshes and boundary-condition index sets (oracle, test-only).
 
 Follows jax_fem_cfd/setup/regular_mesh.py and jax_fem_cfd/setup/simulation.py.
 """
@@ -88,14 +88,58 @@ def create_mesh_3d(nex, ney, nez, lx, ly, lz):
     return coords, nconn, surf, bconns, nvecs
 
 
-def mesh_func_for(ndim, periodic):
+def mesh_func_for(ndim, periodic, shape_func_ord=1):
+    if ndim == 2 and shape_func_ord == 2:
+        return create_periodic_mesh_2d_quad9 if periodic else create_mesh_2d_quad9
     if ndim == 2:
         return create_periodic_mesh_2d if periodic else create_mesh_2d
     return create_mesh_3d
 
 
+def create_mesh_2d_quad9(nex, ney, lx, ly):
+    """Rectangular quad9 mesh.  regular_mesh.py:107-145.
+
+    Node grid (2 nex + 1) x (2 ney + 1) at half spacing; element (ie, je) -> je*nex + ie with
+    local nodes [corner (0,0), mid (1,0), corner (2,0), mid (2,1), corner (2,2), mid (1,2),
+    corner (0,2), mid (0,1), centre (1,1)] in node-grid offsets from (2 ie, 2 je).  Faces
+    [down, right, top, left]; bconns are consecutive node triples along each face.
+    """
+    nnx, nny = 2 * nex + 1, 2 * ney + 1
+    dx, dy = lx / nex, ly / ney
+    i = np.arange(nnx)
+    j = np.arange(nny)
+    coords = np.stack(np.broadcast_arrays(i[None, :] * dx / 2, j[:, None] * dy / 2), axis=-1).reshape(-1, 2)
+    e = np.arange(nex * ney)
+    ie, je = e % nex, e // nex
+    off = [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (1, 2), (0, 2), (0, 1), (1, 1)]
+    nconn = np.stack([(2 * ie + a) + (2 * je + b) * nnx for a, b in off], axis=1).astype(np.int64)
+    surf = [np.arange(nnx, dtype=np.int64),
+            np.arange(nnx - 1, nnx * nny, nnx, dtype=np.int64),
+            np.arange(nnx * (nny - 1), nnx * nny, dtype=np.int64),
+            np.arange(0, nnx * nny, nnx, dtype=np.int64)]
+    bconns = [np.stack([s_[:-2:2], s_[1:-1:2], s_[2::2]], axis=1) for s_ in surf]
+    normals = np.array([[0, -1], [1, 0], [0, 1], [-1, 0]], dtype=np.int64)
+    nvecs = [np.tile(normals[f], (bconns[f].shape[0], 1)) for f in range(4)]
+    return coords, nconn, surf, bconns, nvecs
+
+
+def create_periodic_mesh_2d_quad9(nex, ney, lx, ly):
+    """Doubly periodic quad9 mesh, node grid 2 nex x 2 ney (no duplicated nodes).
+    regular_mesh.py:182-218."""
+    nnx, nny = 2 * nex, 2 * ney
+    dx, dy = lx / nex, ly / ney
+    i = np.arange(nnx)
+    j = np.arange(nny)
+    coords = np.stack(np.broadcast_arrays(i[None, :] * dx / 2, j[:, None] * dy / 2), axis=-1).reshape(-1, 2)
+    e = np.arange(nex * ney)
+    ie, je = e % nex, e // nex
+    off = [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (1, 2), (0, 2), (0, 1), (1, 1)]
+    nconn = np.stack([(2 * ie + a) % nnx + ((2 * je + b) % nny) * nnx for a, b in off], axis=1).astype(np.int64)
+    return coords, nconn, None, None, None
+
+
 def simple_mesh(num_elem, domain_size, periodic, dirichlet_faces, outlet_faces,
-                partial_dirichlet=None):
+                partial_dirichlet=None, shape_func_ord=1):
     """Mesh + BC index sets.  simulation.py:96-183.
 
     gamma_d: sorted unique Dirichlet nodes tiled per component (+nn*i);
@@ -103,7 +147,7 @@ def simple_mesh(num_elem, domain_size, periodic, dirichlet_faces, outlet_faces,
     outlet nodes, else the scalar 0.
     """
     ndim = len(num_elem)
-    mesh_func = mesh_func_for(ndim, periodic)
+    mesh_func = mesh_func_for(ndim, periodic, shape_func_ord)
     coords, nconn, surf, bconns, nvecs = mesh_func(*num_elem, *domain_size)
     nn = coords.shape[0]
     h = domain_size[0] / num_elem[0]
@@ -125,6 +169,8 @@ def simple_mesh(num_elem, domain_size, periodic, dirichlet_faces, outlet_faces,
             e_idx = round(e_frac * nfe)
             bconn_list.append(bconns[f][s_idx:e_idx])
             nvec_list.append(nvecs[f][s_idx:e_idx])
+            if shape_func_ord == 2:
+                s_idx, e_idx = 2 * s_idx, 2 * e_idx
             surf[f] = nodes[s_idx:e_idx + 1]
             dirichlet_nodes.append(nodes[s_idx:e_idx + 1])
             if rem == "outlet":

@@ -12,7 +12,7 @@ from .solvers import cg
 
 
 def multigrid_setup(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, shapefunc,
-                    partial_dirichlet=None):
+                    partial_dirichlet=None, shape_func_ord=1):
     """Per level: mesh, Le, L_diag, Le_off = Le - diag(Le), gamma_p (outlet
     nodes or [0]).  multigrid.py:9-88.  Level count uses the x ratio only
     (:12) and both directions halve in lockstep (:80)."""
@@ -30,12 +30,15 @@ def multigrid_setup(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, s
             for f, (s_frac, e_frac, rem) in partial_dirichlet.items():
                 nfe = bconns[f].shape[0]
                 s_idx, e_idx = round(s_frac * nfe), round(e_frac * nfe)
+                if shape_func_ord == 2:
+                    s_idx, e_idx = 2 * s_idx, 2 * e_idx
                 if rem == "outlet":
                     outlet.append(surf[f][:s_idx])
                     outlet.append(surf[f][e_idx + 1:])
         gamma_p = np.unique(np.concatenate(outlet)).astype(np.int64) if outlet else np.array([0])
         Le = el.laplacian_element_calc(wq, Nderiv)
-        levels.append(dict(nex=nex, ney=ney, nnx=nex + 1, nny=ney + 1, nn=nn, nconn=nconn,
+        k = shape_func_ord
+        levels.append(dict(nex=nex, ney=ney, nnx=k * nex + 1, nny=k * ney + 1, nn=nn, nconn=nconn,
                            Le=Le, Le_off=Le - np.diag(np.diag(Le)),
                            L_diag=el.laplacian_diag(Le, nconn, nn), gamma_p=gamma_p))
         nex, ney = nex // 2, ney // 2

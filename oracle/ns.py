@@ -96,22 +96,24 @@ class Solver:
     bconn: object = None
     nvec: object = None
     L_diag: np.ndarray = None
+    bshape: object = None
 
 
 def setup_solver(coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, num_elem, domain_size,
                  periodic, outlet_faces, solver_tol, set_zeroP, pressure_precond,
-                 final_multigrid_mesh, partial_dirichlet=None):
+                 final_multigrid_mesh, partial_dirichlet=None, shape_func_ord=1):
     """standard.py:191-247: element constants from element 0 and the pressure
     preconditioner closure (jacobi | approx_inv | multigrid | None)."""
     ndim = len(num_elem)
-    sf, _ = el.shape_functions_for(ndim)
+    sf, bsf = el.shape_functions_for(ndim, shape_func_ord)
     wq, N, Nderiv = sf(coords[nconn[0]])
     M, Me = el.lumped_mass_matrix(nconn, wq, N, nn, ndim)
     Le = el.laplacian_element_calc(wq, Nderiv)
     Ge = el.gradient_element_calc(wq, N, Nderiv, ndim)
     s = Solver(nn=nn, ndim=ndim, nconn=nconn, M=M, Me=Me, Le=Le, Ge=Ge, wq=wq, N=N, Nderiv=Nderiv,
                gamma_d=gamma_d, gamma_p=gamma_p, set_zeroP=set_zeroP, tol=solver_tol,
-               periodic=periodic, precond=pressure_precond, coords=coords, bconn=bconn, nvec=nvec)
+               periodic=periodic, precond=pressure_precond, coords=coords, bconn=bconn, nvec=nvec,
+               bshape=bsf)
     if pressure_precond == "jacobi":
         Ld = el.laplacian_diag(Le, nconn, nn)
         s.L_diag = Ld
@@ -131,7 +133,7 @@ def setup_solver(coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, nu
         s.M_op = approx_inv
     elif pressure_precond == "multigrid":
         levels = mg.multigrid_setup(num_elem, final_multigrid_mesh, domain_size, outlet_faces,
-                                    mesh_func, sf, partial_dirichlet)
+                                    mesh_func, sf, partial_dirichlet, shape_func_ord)
         s.levels = levels
         s.M_op = lambda r: mg.v_cycle(r, levels, solver_tol, periodic, set_zeroP)
     return s
@@ -158,7 +160,8 @@ def compute(s, U_d, S_func, t_n, t):
     F = M_c (S(t_n) + S(t)) / 2.  standard.py:167-188."""
     F, F_p = 0, 0
     if s.gamma_d is not None and s.ndim == 2:
-        F_p = el.u_dot_n_boundary_integral_1d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords)
+        F_p = el.u_dot_n_boundary_integral_1d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords,
+                                              s.bshape or el.lin2_shape_functions)
     elif s.gamma_d is not None:
         F_p = el.u_dot_n_boundary_integral_2d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords)
     if S_func is not None:
@@ -199,13 +202,13 @@ def timestep_loop(s, t_final, dt, save_interval, rho, mu, U_d, U0, P0, S_func=No
 # --------------------------------------------------------------------------------------------
 
 def build(num_elem, domain_size, periodic, dirichlet_faces, outlet_faces, solver_tol, set_zeroP,
-          pressure_precond, final_multigrid_mesh, partial_dirichlet=None):
+          pressure_precond, final_multigrid_mesh, partial_dirichlet=None, shape_func_ord=1):
     """simulation.setup for the 'standard' solver (simulation.py:185-200)."""
     coords, nconn, surf, nn, h, bconn, nvec, gamma_d, U_d, gamma_p, mesh_func = ms.simple_mesh(
-        num_elem, domain_size, periodic, dirichlet_faces, outlet_faces, partial_dirichlet)
+        num_elem, domain_size, periodic, dirichlet_faces, outlet_faces, partial_dirichlet, shape_func_ord)
     s = setup_solver(coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, num_elem,
                      domain_size, periodic, outlet_faces, solver_tol, set_zeroP, pressure_precond,
-                     final_multigrid_mesh, partial_dirichlet)
+                     final_multigrid_mesh, partial_dirichlet, shape_func_ord)
     return s, surf, h, U_d
 
 
