@@ -63,8 +63,8 @@ static int run(fcfd_ctx* ctx, F&& f) {
 // entry points built for 2D contexts only
 template <class F>
 static int run2d(fcfd_ctx* ctx, const char* what, F&& f) {
-    if (ctx->c.dim != 2) {
-        g_last_error = std::string(what) + " is not available for 3D contexts";
+    if (ctx->c.dim != 2 || ctx->c.order != 1) {
+        g_last_error = std::string(what) + " is available for 2D Q1 contexts only";
         return FCFD_ERR_UNSUPPORTED;
     }
     return run(ctx, std::forward<F>(f));
@@ -170,6 +170,11 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
                  "unknown pressure preconditioner");
     if (m->ndim == 3) {
         build_ctx3(c, m, s);
+        return;
+    }
+    FCFD_REQUIRE(m->order == 0 || m->order == 1 || m->order == 2, "order must be 1 (quad4) or 2 (quad9)");
+    if (m->order == 2) {
+        build_ctx_q2(c, m, s);
         return;
     }
     c.g = make_grid(m->nex, m->ney, m->periodic);
@@ -494,8 +499,8 @@ int fcfd_mesh_fill(fcfd_ctx* ctx, double* coords, int64_t* nconn) {
 
 int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16) {
     if (!ctx || !out16) return FCFD_ERR_ARG;
-    if (ctx->c.dim != 2) {
-        g_last_error = "fcfd_element_constant returns 4 x 4 (2D) constants; use fcfd_element_constant3";
+    if (ctx->c.dim != 2 || ctx->c.order != 1) {
+        g_last_error = "fcfd_element_constant returns 4 x 4 (quad4) constants; use fcfd_element_matrix";
         return FCFD_ERR_UNSUPPORTED;
     }
     const ElemConst& e = ctx->c.ec;
@@ -525,12 +530,40 @@ int fcfd_element_constant3(const fcfd_ctx* ctx, int32_t which, double* out64) {
     return FCFD_OK;
 }
 
+int fcfd_element_matrix(const fcfd_ctx* ctx, int32_t which, double* out, int32_t n) {
+    if (!ctx || !out) return FCFD_ERR_ARG;
+    const Ctx& c = ctx->c;
+    const int nen = c.dim == 3 ? 8 : (c.order == 2 ? 9 : 4);
+    if (n != nen * nen) {
+        g_last_error = "element matrix size mismatch: this context has " + std::to_string(nen * nen) + " entries";
+        return FCFD_ERR_ARG;
+    }
+    const double* src = nullptr;
+    if (c.dim == 3) {
+        src = which == 0 ? c.ec3.Me : which == 1 ? c.ec3.Le : (which >= 2 && which <= 4) ? c.ec3.G[which - 2] : nullptr;
+    } else if (c.order == 2) {
+        src = which == 0 ? c.eq2.Me : which == 1 ? c.eq2.Le : which == 2 ? c.eq2.Gx : which == 3 ? c.eq2.Gy
+            : which == 5 ? c.eq2.Leoff : nullptr;
+    } else {
+        src = which == 0 ? c.ec.Me : which == 1 ? c.ec.Le : which == 2 ? c.ec.Gx : which == 3 ? c.ec.Gy
+            : which == 5 ? c.ec.Leoff : nullptr;
+    }
+    if (!src) {
+        g_last_error = "unknown element matrix";
+        return FCFD_ERR_ARG;
+    }
+    std::memcpy(out, src, (size_t)n * sizeof(double));
+    return FCFD_OK;
+}
+
 int fcfd_mass_matvec(fcfd_ctx* ctx, const double* x, double* y) {
     if (!ctx || !x || !y || x == y) return FCFD_ERR_ARG;
     return run(ctx, [&] {
         Ctx& c = ctx->c;
         if (c.dim == 3) {
             mass_matvec3(c, x, y);
+        } else if (c.order == 2) {
+            mass_matvec_q2(c, x, y);
         } else {
             elem_mult_const(c, 0, c.ec.Me, x, y, 0);
             elem_mult_const(c, 0, c.ec.Me, x + c.g.nn, y + c.g.nn, 0);
@@ -558,6 +591,7 @@ int fcfd_conv_elem_calc(fcfd_ctx* ctx, const double* U, double beta, double* Ce)
     if (!ctx || !U || !Ce) return FCFD_ERR_ARG;
     return run(ctx, [&] {
         if (ctx->c.dim == 3) build_ce3_api(ctx->c, U, beta, Ce);
+        else if (ctx->c.order == 2) build_ce_q2_api(ctx->c, U, beta, Ce);
         else build_ce_api(ctx->c, U, beta, Ce);
     });
 }
@@ -566,6 +600,7 @@ int fcfd_boundary_integral(fcfd_ctx* ctx, const double* U_d, double* F_p) {
     if (!ctx || !F_p) return FCFD_ERR_ARG;
     return run(ctx, [&] {
         if (ctx->c.dim == 3) boundary_integral3(ctx->c, U_d, F_p);
+        else if (ctx->c.order == 2) boundary_integral_q2(ctx->c, U_d, F_p);
         else boundary_integral(ctx->c, U_d, F_p);
     });
 }
@@ -604,8 +639,9 @@ int fcfd_solve_pressure(fcfd_ctx* ctx, const double* U_star, const double* P_n, 
                         double dt, double* P, int32_t* iters) {
     if (!ctx || !U_star || !P_n || !P) return FCFD_ERR_ARG;
     return run(ctx, [&] {
-        int it = ctx->c.dim == 3 ? solve_pressure3(ctx->c, U_star, P_n, F_p, rho, dt, P)
-                                 : solve_pressure(ctx->c, U_star, P_n, F_p, rho, dt, P);
+        int it = ctx->c.dim == 3     ? solve_pressure3(ctx->c, U_star, P_n, F_p, rho, dt, P)
+                 : ctx->c.order == 2 ? solve_pressure_q2(ctx->c, U_star, P_n, F_p, rho, dt, P)
+                                     : solve_pressure(ctx->c, U_star, P_n, F_p, rho, dt, P);
         FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
         if (iters) *iters = it;
     });
@@ -621,6 +657,9 @@ int fcfd_solve_momentum(fcfd_ctx* ctx, const double* U_n, const double* P_n, con
         if (c.dim == 3) {
             FCFD_REQUIRE(!Ce, "3D contexts build Ce(U_n) themselves (pass Ce = NULL)");
             it = solve_momentum3_from(c, U_n, P_n, rho, mu, dt, F, U_hat);
+        } else if (c.order == 2) {
+            FCFD_REQUIRE(!Ce, "quad9 contexts build Ce(U_n) themselves (pass Ce = NULL)");
+            it = solve_momentum_q2(c, U_n, P_n, rho, mu, dt, F, U_hat);
         } else if (Ce) {
             double* em = api_ce_buffer(c);
             ce_from_api(c, Ce, em);
@@ -641,6 +680,7 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
     return run(ctx, [&] {
         int it[2];
         if (ctx->c.dim == 3) four_step3(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+        else if (ctx->c.order == 2) four_step_q2(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
         else four_step(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
         if (iters) {
             iters[0] = it[0];

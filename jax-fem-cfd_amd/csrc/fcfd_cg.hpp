@@ -118,4 +118,33 @@ struct SumXBody {
     __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += x[k]; }
 };
 
+// x += alpha p ; r -= alpha Ap ; z_new = r / L_diag (jacobi) or r (none) ; partials r.r, r.z_new:
+// the CG update, the preconditioner and the new gamma in one pass (CgK2 + JacobiPrec + CgK3)
+struct CgK23 {
+    const double* p; const double* Ap; const double* Ld; double* x; double* r; double* znew; const DevState* st;
+    __device__ void operator()(int k, double (&acc)[2]) const {
+        const double a = st->c_alpha;
+        x[k] = x[k] + a * p[k];
+        const double rk = r[k] - a * Ap[k];
+        r[k] = rk;
+        const double zk = Ld ? rk / Ld[k] : rk;
+        znew[k] = zk;
+        acc[0] += rk * rk;
+        acc[1] += rk * zk;
+    }
+};
+// CgK2Fin then CgK3Fin (flex = 0)
+struct CgK23Fin {
+    DevState* st;
+    __device__ void operator()(const double (&t)[2]) const {
+        st->c_rr = t[0];
+        const double gnew = t[1];
+        st->c_beta = gnew / st->c_gamma;
+        st->c_gamma = gnew;
+        const int k = st->c_k + 1;
+        st->c_k = k;
+        st->c_active = (st->c_rr > st->c_atol2) && (k < st->c_maxit);
+    }
+};
+
 }  // namespace fcfd

@@ -133,6 +133,12 @@ struct ElemConst {
     double N[16], dNx[16], dNy[16], wq[4];  // quad4 2x2 Gauss data: [q][I]
 };
 
+// quad9 (Q2) element constants: 3x3 Gauss data [q][I] and the 9 x 9 matrices
+struct ElemConstQ2 {
+    double N[81], dNx[81], dNy[81], wq[9];
+    double Me[81], Le[81], Leoff[81], Gx[81], Gy[81];
+};
+
 // ---------------------------------------------------------------------------------------------
 // Per-element matrix storage, row-blocked: entry (a, b) of element e at ((a * ne + e) * 4 + b),
 // i.e. the 4 entries of row a of an element are contiguous and row a of consecutive elements is

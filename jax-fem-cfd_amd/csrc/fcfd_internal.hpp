@@ -77,7 +77,11 @@ struct Ctx {
     double lz = 0.0;
     ElemConst3 ec3{};
     ElemConst3* ec3_dev = nullptr;
-    double* fb_w = nullptr;     // 3D: per Dirichlet face its 4 x 4 weights sum_q w N_a N_b
+    double* fb_w = nullptr;     // 3D / quad9: per Dirichlet face (edge) its weights sum_q w N_a N_b
+    // quad9 (Q2) contexts: order = 2, g is the NODE grid (nnx = 2 nex + 1, or 2 nex periodic)
+    int order = 1;
+    ElemConstQ2 eq2{};
+    ElemConstQ2* eq2_dev = nullptr;
     double tol;
     int set_zeroP, precond, precision, ce_format, maxit_mom, maxit_p;
     int nlev = 0;
@@ -273,6 +277,19 @@ void mass_matvec3(Ctx& c, const double* x, double* y);
 void build_ce3_api(Ctx& c, const double* U, double beta, double* Ce);
 int solve_momentum3_from(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt,
                          const double* F, double* U_hat);
+
+// quad9 (Q2) path (fcfd_q2.hip)
+void quad9_constants(double hx, double hy, ElemConstQ2& ec);
+void build_ctx_q2(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s);
+void four_step_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
+                  const double* F_p, const double* F, double* U, double* P, int* it);
+int solve_momentum_q2(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt, const double* F,
+                      double* U_hat);
+int solve_pressure_q2(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                      double* P_out);
+void boundary_integral_q2(Ctx& c, const double* U_d, double* F_p);
+void mass_matvec_q2(Ctx& c, const double* x, double* y);
+void build_ce_q2_api(Ctx& c, const double* U, double beta, double* Ce);
 
 // reverse-mode products of the steps (fcfd_adjoint.hip)
 void ns_vjp(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,

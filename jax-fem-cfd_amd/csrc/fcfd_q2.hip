@@ -1,0 +1,839 @@
+// Q2 (quad9) four-step timestep on structured 2D grids.
+//
+// The chain of jax_fem_cfd/solvers/standard.py:148-165 on the meshes of create_mesh_2d_quad9 /
+// create_periodic_mesh_2d_quad9 (regular_mesh.py:107-145, :182-218) with the quad9 shape
+// functions (shape_functions.py:108-159, 3x3 Gauss) and lin3 boundary edges
+// (shape_functions.py:89-106).  The grid of a Q2 context is its NODE grid: Grid2D with
+// nnx = 2 nex + 1 (2 nex periodic), nny likewise; element (ie, je) owns the nodes
+// (2 ie + a, 2 je + b), a, b in {0, 1, 2}, in the local order
+//   0 (0,0)  1 (1,0)  2 (2,0)  3 (2,1)  4 (2,2)  5 (1,2)  6 (0,2)  7 (0,1)  8 (1,1).
+//
+// Kernels are owner-computes node gathers: node (i, j) has 1, 2 or 4 incident elements (by the
+// parity of i and j); it sums, in ascending element order, its row of each, applied to that
+// element's 9 nodal values (read directly, so no private array is indexed at run time).
+#include <algorithm>
+#include <cmath>
+#include "fcfd_bicgstab.hpp"
+#include "fcfd_cg.hpp"
+
+namespace fcfd {
+
+// ------------------------------------------------------------------------------ element data
+__host__ __device__ constexpr int q9x(int b) { return b == 0 || b == 6 || b == 7 ? 0 : (b == 1 || b == 5 || b == 8 ? 1 : 2); }
+__host__ __device__ constexpr int q9y(int b) { return b <= 2 ? 0 : (b == 3 || b == 7 || b == 8 ? 1 : 2); }
+__host__ __device__ constexpr int q9loc(int x, int y) {
+    return y == 0 ? x : (y == 2 ? 6 - x : (x == 0 ? 7 : (x == 2 ? 3 : 8)));
+}
+
+__host__ __device__ __forceinline__ size_t q9_idx(int k, int e, int ne) { return ((size_t)(k / 9) * ne + e) * 9 + k % 9; }
+
+struct Ce9F64 {
+    double* c; int ne;
+    __device__ __forceinline__ void ld_row(int a, int e, double (&v)[9]) const {
+        const double* p = c + q9_idx(a * 9, e, ne);
+#pragma unroll
+        for (int b = 0; b < 9; ++b) v[b] = p[b];
+    }
+    __device__ __forceinline__ void st(int k, int e, double v) const { c[q9_idx(k, e, ne)] = v; }
+};
+struct Ce9F32 {
+    float* c; int ne;
+    __device__ __forceinline__ void ld_row(int a, int e, double (&v)[9]) const {
+        const float* p = c + q9_idx(a * 9, e, ne);
+#pragma unroll
+        for (int b = 0; b < 9; ++b) v[b] = (double)p[b];
+    }
+    __device__ __forceinline__ void st(int k, int e, double v) const { c[q9_idx(k, e, ne)] = (float)v; }
+};
+struct Ce9Api {
+    double* c;
+    __device__ __forceinline__ void st(int k, int e, double v) const { c[(size_t)e * 81 + k] = v; }
+};
+
+// quad9 3x3 Gauss data of element 0 and its constants (shape_functions.py:108-159,
+// element_calc.py:9-40); q = (eta index) * 3 + (xi index)
+void quad9_constants(double hx, double hy, ElemConstQ2& ec) {
+    const double g = std::sqrt(3.0 / 5.0);
+    const double x1[3] = {-g, 0.0, g}, w1[3] = {5.0 / 9, 8.0 / 9, 5.0 / 9};
+    auto L = [](double t, int a) { return a == 0 ? t * (t - 1) / 2 : (a == 1 ? 1 - t * t : t * (t + 1) / 2); };
+    auto dL = [](double t, int a) { return a == 0 ? (2 * t - 1) / 2 : (a == 1 ? -2 * t : (2 * t + 1) / 2); };
+    for (int qe = 0; qe < 3; ++qe)
+        for (int qx = 0; qx < 3; ++qx) {
+            const int q = qe * 3 + qx;
+            const double xi = x1[qx], eta = x1[qe];
+            double dxi[9], deta[9];
+            double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+            for (int b = 0; b < 9; ++b) {
+                const int a = q9x(b), c = q9y(b);
+                ec.N[q * 9 + b] = L(xi, a) * L(eta, c);
+                dxi[b] = dL(xi, a) * L(eta, c);
+                deta[b] = L(xi, a) * dL(eta, c);
+                const double X = a * hx / 2, Y = c * hy / 2;
+                J00 += dxi[b] * X; J01 += dxi[b] * Y;
+                J10 += deta[b] * X; J11 += deta[b] * Y;
+            }
+            const double det = J00 * J11 - J01 * J10;
+            ec.wq[q] = w1[qx] * w1[qe] * det;
+            for (int b = 0; b < 9; ++b) {
+                ec.dNx[q * 9 + b] = (J11 * dxi[b] - J01 * deta[b]) / det;
+                ec.dNy[q * 9 + b] = (-J10 * dxi[b] + J00 * deta[b]) / det;
+            }
+        }
+    for (int a = 0; a < 9; ++a)
+        for (int b = 0; b < 9; ++b) {
+            double m = 0, l = 0, gx = 0, gy = 0;
+            for (int q = 0; q < 9; ++q) {
+                const double w = ec.wq[q], Na = ec.N[q * 9 + a], Nb = ec.N[q * 9 + b];
+                m += w * Na * Nb;
+                l += w * (ec.dNx[q * 9 + a] * ec.dNx[q * 9 + b] + ec.dNy[q * 9 + a] * ec.dNy[q * 9 + b]);
+                gx += w * Na * ec.dNx[q * 9 + b];
+                gy += w * Na * ec.dNy[q * 9 + b];
+            }
+            ec.Me[a * 9 + b] = m;
+            ec.Le[a * 9 + b] = l;
+            ec.Gx[a * 9 + b] = gx;
+            ec.Gy[a * 9 + b] = gy;
+        }
+    for (int k = 0; k < 81; ++k) ec.Leoff[k] = (k % 10 == 0) ? 0.0 : ec.Le[k];
+}
+
+// ------------------------------------------------------------------------------ incidence
+// Incident elements of node (i, j): x candidates (i even) element i/2 - 1 at local column 2 and
+// element i/2 at column 0, (i odd) element (i-1)/2 at column 1 only; likewise in y.  k = kx + 2 ky
+// gives ascending element ids.  Missing elements are clamped to element 0 and flagged.
+struct IncQ2 {
+    int e[4];
+    bool has[4];
+    int row[4];
+    int ex[4], ey[4];
+};
+
+__device__ __forceinline__ void node_inc_q2(const Grid2D& g, int n, IncQ2& s) {
+    const int j = n / g.nnx, i = n - j * g.nnx;
+    int exs[2], cxs[2], eys[2], cys[2];
+    bool vx[2], vy[2];
+    const bool ox = i & 1, oy = j & 1;
+    exs[0] = ox ? (i - 1) >> 1 : (i >> 1) - 1; cxs[0] = ox ? 1 : 2;
+    exs[1] = i >> 1; cxs[1] = 0;
+    eys[0] = oy ? (j - 1) >> 1 : (j >> 1) - 1; cys[0] = oy ? 1 : 2;
+    eys[1] = j >> 1; cys[1] = 0;
+    if (g.periodic) {
+        exs[0] = exs[0] < 0 ? exs[0] + g.nex : exs[0];
+        eys[0] = eys[0] < 0 ? eys[0] + g.ney : eys[0];
+        vx[0] = true; vx[1] = !ox;
+        vy[0] = true; vy[1] = !oy;
+    } else {
+        vx[0] = exs[0] >= 0; vx[1] = !ox && exs[1] < g.nex;
+        vy[0] = eys[0] >= 0; vy[1] = !oy && eys[1] < g.ney;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int kx = k & 1, ky = k >> 1;
+        const bool h = vx[kx] && vy[ky];
+        s.has[k] = h;
+        s.ex[k] = h ? exs[kx] : 0;
+        s.ey[k] = h ? eys[ky] : 0;
+        s.e[k] = s.ey[k] * g.nex + s.ex[k];
+        s.row[k] = q9loc(cxs[kx], cys[ky]);
+    }
+}
+
+// node id of local node b of element (ex, ey)
+__device__ __forceinline__ int q9_node(const Grid2D& g, int ex, int ey, int b) {
+    int x = 2 * ex + q9x(b), y = 2 * ey + q9y(b);
+    if (g.periodic) {
+        x = x >= g.nnx ? x - g.nnx : x;
+        y = y >= g.nny ? y - g.nny : y;
+    }
+    return y * g.nnx + x;
+}
+
+// sum_k has_k sum_b A(row_k, b, e_k) X(node_b of e_k) for NC components; A from rowf(a, e, cf)
+template <int NC, class RowF, class ValF>
+__device__ __forceinline__ void q2_apply(const Grid2D& g, const IncQ2& s, RowF rowf, ValF val, double (&y)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) y[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool h = s.has[k];
+        const int ex = s.ex[k], ey = s.ey[k], e = s.e[k], a = s.row[k];
+        double cf[9];
+        rowf(a, e, cf);
+        double t[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) t[c] = 0.0;
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const int id = q9_node(g, ex, ey, b);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t[c] += cf[b] * val(c, id);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) y[c] += h ? t[c] : 0.0;
+    }
+}
+
+// one shared 9 x 9 matrix (TR: its transpose)
+template <bool TR = false, class ValF>
+__device__ __forceinline__ double q2_const(const Grid2D& g, const IncQ2& s, const double* __restrict__ A, ValF val) {
+    double y[1];
+    q2_apply<1>(g, s,
+                [&](int a, int, double (&cf)[9]) {
+#pragma unroll
+                    for (int b = 0; b < 9; ++b) cf[b] = TR ? A[b * 9 + a] : A[a * 9 + b];
+                },
+                [&](int, int id) { return val(id); }, y);
+    return y[0];
+}
+
+// ------------------------------------------------------------------------------ Ce
+// Ce[a][b] = sum_q w N_qa (U_q . grad N_qb + beta div U_q N_qb)   (element_calc.py:165-185)
+template <class S>
+__global__ __launch_bounds__(BLOCK) void k_build_ce_q2(Grid2D g, const ElemConstQ2* __restrict__ ec,
+                                                      const double* __restrict__ U, double beta, S store) {
+    const int e = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (e >= g.ne) return;
+    const int je = e / g.nex, ie = e - je * g.nex;
+    double ul[9], vl[9];
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        const int id = q9_node(g, ie, je, b);
+        ul[b] = U[id];
+        vl[b] = U[id + g.nn];
+    }
+    double A[81];
+#pragma unroll
+    for (int k = 0; k < 81; ++k) A[k] = 0.0;
+    for (int q = 0; q < 9; ++q) {
+        double uq = 0.0, vq = 0.0, div = 0.0;
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            uq += ec->N[q * 9 + b] * ul[b];
+            vq += ec->N[q * 9 + b] * vl[b];
+            div += ec->dNx[q * 9 + b] * ul[b] + ec->dNy[q * 9 + b] * vl[b];
+        }
+        const double w = ec->wq[q];
+        double adv[9];
+#pragma unroll
+        for (int b = 0; b < 9; ++b) adv[b] = uq * ec->dNx[q * 9 + b] + vq * ec->dNy[q * 9 + b] + beta * div * ec->N[q * 9 + b];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) {
+            const double wNa = w * ec->N[q * 9 + a];
+#pragma unroll
+            for (int b = 0; b < 9; ++b) A[a * 9 + b] += wNa * adv[b];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 81; ++k) store.st(k, e, A[k]);
+}
+
+static void build_ce_q2(Ctx& c, const double* U, double beta) {
+    const int nb = blocks_for(c.g.ne);
+    if (c.ce_format == FCFD_CE_F64)
+        k_build_ce_q2<<<nb, BLOCK, 0, c.stream>>>(c.g, c.eq2_dev, U, beta, Ce9F64{(double*)c.ce, c.g.ne});
+    else
+        k_build_ce_q2<<<nb, BLOCK, 0, c.stream>>>(c.g, c.eq2_dev, U, beta, Ce9F32{(float*)c.ce, c.g.ne});
+    FCFD_CHECK_LAUNCH();
+}
+
+void build_ce_q2_api(Ctx& c, const double* U, double beta, double* Ce) {
+    k_build_ce_q2<<<blocks_for(c.g.ne), BLOCK, 0, c.stream>>>(c.g, c.eq2_dev, U, beta, Ce9Api{Ce});
+    FCFD_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------------------ momentum
+template <class S>
+struct MomCoefQ2 {
+    Grid2D g;
+    const ElemConstQ2* __restrict__ ec;
+    S ce;
+    double cM, cL, cC;
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[9]) const {
+        double v[9];
+        ce.ld_row(a, e, v);
+#pragma unroll
+        for (int b = 0; b < 9; ++b) cf[b] = cM * ec->Me[a * 9 + b] + cL * ec->Le[a * 9 + b] + cC * v[b];
+    }
+};
+
+// rhs = R U_n - G P_n + F (Dirichlet rows U_d), x0 = U_n with Dirichlet values, r0 = rhs - A x0
+template <class Op>
+struct BInitQ2 {
+    Op lhs, rhs;
+    const double *xn, *ud, *Pn, *F;
+    const uint8_t* mask;
+    double *x, *r, *rhat, *p, *q;
+    __device__ void operator()(int n, double (&acc)[2]) const {
+        const Grid2D& g = lhs.g;
+        const int nn = g.nn;
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        double b[2], ax[2];
+        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { rhs.row(a, e, cf); },
+                    [&](int c, int id) { return xn[id + c * nn]; }, b);
+        b[0] -= q2_const(g, s, lhs.ec->Gx, [&](int id) { return Pn[id]; });
+        b[1] -= q2_const(g, s, lhs.ec->Gy, [&](int id) { return Pn[id]; });
+        if (F) { b[0] += F[n]; b[1] += F[n + nn]; }
+        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { lhs.row(a, e, cf); },
+                    [&](int c, int id) {
+                        const int i = id + c * nn;
+                        const double xv = xn[i];
+                        return (mask && mask[i]) ? ud[i] : xv;
+                    },
+                    ax);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int i = n + c * nn;
+            const bool m = mask && mask[i];
+            const double x0 = m ? ud[i] : xn[i];
+            const double bc = m ? ud[i] : b[c];
+            const double a = m ? x0 : ax[c];
+            const double rc = bc - a;
+            x[i] = x0;
+            r[i] = rc; rhat[i] = rc; p[i] = rc; q[i] = rc;
+            acc[0] += bc * bc;
+            acc[1] += rc * rc;
+        }
+    }
+};
+
+template <class Op>
+struct BK1Q2 {
+    Op op;
+    const uint8_t* mask;
+    const double *r, *pold, *qold, *rhat;
+    double *pnew, *qnew;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const Grid2D& g = op.g;
+        const int nn = g.nn;
+        const double beta = st->b_beta, omega = st->b_omega;
+        auto pv = [&](int c, int id) {
+            const int i = id + c * nn;
+            return r[i] + beta * (pold[i] - omega * qold[i]);
+        };
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        double y[2];
+        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); }, pv, y);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int i = n + c * nn;
+            const double pc = pv(c, n);
+            const double yc = (mask && mask[i]) ? pc : y[c];
+            pnew[i] = pc;
+            qnew[i] = yc;
+            acc[0] += rhat[i] * yc;
+        }
+    }
+};
+
+template <class Op>
+struct BK2Q2 {
+    Op op;
+    const uint8_t* mask;
+    const double *r, *q;
+    double *sv, *tv;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[3]) const {
+        const Grid2D& g = op.g;
+        const int nn = g.nn;
+        const double alpha = st->b_alpha;
+        auto svv = [&](int c, int id) { return r[id + c * nn] - alpha * q[id + c * nn]; };
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        double y[2];
+        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); }, svv, y);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int i = n + c * nn;
+            const double sc = svv(c, n);
+            const double yc = (mask && mask[i]) ? sc : y[c];
+            sv[i] = sc;
+            tv[i] = yc;
+            acc[0] += sc * sc;
+            acc[1] += yc * sc;
+            acc[2] += yc * yc;
+        }
+    }
+};
+
+template <class S>
+static int solve_momentum_q2_t(Ctx& c, S ce, const double* U_n, const double* P_n, double rho, double mu, double dt,
+                               const double* F, double* U_hat) {
+    const double theta = 0.5;
+    const Grid2D& g = c.g;
+    const int nn = g.nn;
+    MomCoefQ2<S> lhs{g, c.eq2_dev, ce, rho / dt, theta * mu, theta * rho};
+    MomCoefQ2<S> rhs{g, c.eq2_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
+    const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 2 * nn;
+    launch_node<2>(c, nn, nullptr,
+                   BInitQ2<MomCoefQ2<S>>{lhs, rhs, U_n, c.ud_full, P_n, F, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0],
+                                         c.m_q[0]},
+                   BInitFin{c.st, c.tol, maxit});
+    FCFD_CHECK_LAUNCH();
+    const int* act = &c.st->b_active;
+    auto pair = [&]() {
+        for (int o = 0; o < 2; ++o) {
+            const int nw = o ^ 1;
+            launch_node<1>(c, nn, act,
+                           BK1Q2<MomCoefQ2<S>>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
+                                               c.m_q[nw], c.st},
+                           BK1Fin{c.st});
+            launch_node<3>(c, nn, act, BK2Q2<MomCoefQ2<S>>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                           BK2Fin{c.st});
+            launch_node<2>(c, nn, act, BK3<2>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
+                           BK3Fin{c.st});
+        }
+        FCFD_CHECK_LAUNCH();
+    };
+    const double key[4] = {rho, mu, dt, 0.0};
+    int launched = 0;
+    int pairs = c.pred_mom > 0 ? (c.pred_mom + 1) / 2 : 1;
+    int h[2] = {0, 0};
+    while (true) {
+        replay(c, c.g_mom, key, ce.c, pairs, pair);
+        launched += 2 * pairs;
+        read_state_ints(c, &c.st->b_k, 2, h);  // b_k, b_active
+        if (!h[1] || launched >= maxit) break;
+        pairs = 1;
+    }
+    if (h[0] > 0) c.pred_mom = h[0];
+    if (U_hat != c.m_x)
+        FCFD_HIP(hipMemcpyAsync(U_hat, c.m_x, sizeof(double) * 2 * nn, hipMemcpyDeviceToDevice, c.stream));
+    return h[0];
+}
+
+int solve_momentum_q2(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt, const double* F,
+                      double* U_hat) {
+    build_ce_q2(c, U_n, 0.5);
+    if (c.ce_format == FCFD_CE_F64)
+        return solve_momentum_q2_t(c, Ce9F64{(double*)c.ce, c.g.ne}, U_n, P_n, rho, mu, dt, F, U_hat);
+    return solve_momentum_q2_t(c, Ce9F32{(float*)c.ce, c.g.ne}, U_n, P_n, rho, mu, dt, F, U_hat);
+}
+
+// ------------------------------------------------------------------------------ projection
+// Uout = Uin + c G P / M (c = dt/rho: U*; c = -dt/rho with Dirichlet values: the final U)
+struct ProjQ2Body {
+    Grid2D g;
+    const ElemConstQ2* ec;
+    const double *Uin, *P, *M, *ud;
+    const uint8_t* mask;
+    double* Uout;
+    double c;
+    __device__ void operator()(int n, double (&)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        const double gx = q2_const(g, s, ec->Gx, [&](int id) { return P[id]; });
+        const double gy = q2_const(g, s, ec->Gy, [&](int id) { return P[id]; });
+        const double gd[2] = {gx, gy};
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const int i = n + d * g.nn;
+            double u = Uin[i] + (c * gd[d]) / M[i];
+            if (mask && mask[i]) u = ud[i];
+            Uout[i] = u;
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------ pressure
+struct PrhsQ2Body {
+    Grid2D g; const ElemConstQ2* ec; const double* Us; const double* Fp; const uint8_t* pin; double* rhs; double c;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        const double div = q2_const<true>(g, s, ec->Gx, [&](int id) { return Us[id]; }) +
+                           q2_const<true>(g, s, ec->Gy, [&](int id) { return Us[id + g.nn]; });
+        double v = c * (div - (Fp ? Fp[n] : 0.0));
+        if (pin && pin[n]) v = 0.0;
+        rhs[n] = v;
+        acc[0] += v;
+    }
+};
+
+struct CgInitQ2 {
+    Grid2D g; const double* Le; const uint8_t* pin; const double* b; const double* bmean; const double* x0in;
+    int zero_pins; double* x; double* r;
+    __device__ void operator()(int n, double (&acc)[3]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        auto xv = [&](int id) {
+            const double v = x0in[id];
+            return (zero_pins && pin && pin[id]) ? 0.0 : v;
+        };
+        const bool pn = pin && pin[n];
+        const double ax = pn ? 0.0 : q2_const(g, s, Le, xv);
+        const double bn = b[n] - (bmean ? *bmean : 0.0);
+        const double rn = bn - ax;
+        x[n] = xv(n);
+        r[n] = rn;
+        acc[0] += bn * bn;
+        acc[1] += rn * rn;
+        acc[2] += rn;
+    }
+};
+
+struct CgK1Q2 {
+    Grid2D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const double beta = st->c_beta;
+        auto pv = [&](int id) { return z[id] + beta * pold[id]; };
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        const double a = (pin && pin[n]) ? 0.0 : q2_const(g, s, Le, pv);
+        const double pn = pv(n);
+        pnew[n] = pn;
+        Ap[n] = a;
+        acc[0] += pn * a;
+    }
+};
+
+struct ApproxInvQ2a {
+    Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
+    __device__ void operator()(int n, double (&)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        const double v = q2_const(g, s, Le, [&](int id) { return r[id] / Ld[id]; }) / Ld[n];
+        d1l[n] = (pin && pin[n]) ? 0.0 : v;
+    }
+};
+struct ApproxInvQ2b {
+    Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l; double* z;
+    __device__ void operator()(int n, double (&)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        double l2 = q2_const(g, s, Le, [&](int id) { return d1l[id]; });
+        if (pin && pin[n]) l2 = 0.0;
+        const double d1x = r[n] / Ld[n];
+        z[n] = (3.0 * d1x - 3.0 * d1l[n]) + l2 / Ld[n];
+    }
+};
+
+static const double* precondition_q2(Ctx& c, const int* active) {
+    const int nn = c.g.nn;
+    switch (c.precond) {
+        case FCFD_PC_NONE: return c.c_r;
+        case FCFD_PC_JACOBI:
+            launch_node<0>(c, nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y}, NoFin{});
+            return c.lv[0].y;
+        case FCFD_PC_APPROX_INV:
+            launch_node<0>(c, nn, active, ApproxInvQ2a{c.g, c.eq2_dev->Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x},
+                           NoFin{});
+            launch_node<0>(c, nn, active,
+                           ApproxInvQ2b{c.g, c.eq2_dev->Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y},
+                           NoFin{});
+            return c.lv[0].y;
+    }
+    throw Error(FCFD_ERR_UNSUPPORTED, "pressure preconditioner not supported for quad9");
+}
+
+// PCG (jax_iterative_solvers.py:87-120); solution in c.c_x
+static int pcg_q2(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol,
+                  int maxit, bool* converged) {
+    const Grid2D& g = c.g;
+    const int nn = g.nn;
+    const int neu = c.precond == FCFD_PC_MULTIGRID && !c.set_zeroP;
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    const double* Le = c.eq2_dev->Le;
+    launch_node<3>(c, nn, nullptr, CgInitQ2{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
+                   CgInitFin{c.st, tol, maxit, nn, 0});
+    const int* act = &c.st->c_active;
+    const double* y0 = precondition_q2(c, act);
+    launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st});
+    FCFD_CHECK_LAUNCH();
+    const double* Ld = c.lv[0].Ld;
+    auto pair = [&]() {
+        for (int o = 0; o < 2; ++o) {
+            const int nw = o ^ 1;
+            launch_node<1>(c, nn, act, CgK1Q2{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                           CgK1Fin{c.st});
+            if (c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_NONE) {
+                launch_node<2>(c, nn, act,
+                               CgK23{c.c_p[nw], c.c_Ap, c.precond == FCFD_PC_JACOBI ? Ld : nullptr, c.c_x, c.c_r,
+                                     c.c_z[nw], c.st},
+                               CgK23Fin{c.st});
+            } else {
+                launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
+                const double* y = precondition_q2(c, act);
+                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, 0},
+                               CgK3Fin{c.st, 0});
+            }
+        }
+        FCFD_CHECK_LAUNCH();
+    };
+    const double key[4] = {0.0, 0.0, 0.0, 0.0};
+    int launched = 0;
+    int pairs = c.pred_p > 0 ? (c.pred_p + 1) / 2 : 1;
+    int h[2] = {0, 0};
+    while (true) {
+        replay(c, c.g_cg, key, nullptr, pairs, pair);
+        launched += 2 * pairs;
+        read_state_ints(c, &c.st->c_k, 2, h);  // c_k, c_active
+        if (!h[1] || launched >= maxit) break;
+        pairs = std::max(1, std::min(16, pairs * 2));
+    }
+    if (converged) *converged = !h[1] && h[0] <= maxit;
+    if (h[0] > 0) c.pred_p = h[0];
+    return h[0];
+}
+
+int solve_pressure_q2(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                      double* P_out) {
+    const int nn = c.g.nn;
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    launch_node<1>(c, nn, nullptr, PrhsQ2Body{c.g, c.eq2_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
+                   MeanFin{&c.st->mean_rhs, nullptr, nn});
+    const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
+    bool conv;
+    const int it = pcg_q2(c, c.c_rhs, c.set_zeroP ? nullptr : &c.st->mean_rhs, P_n, c.set_zeroP, c.tol, maxit, &conv);
+    if (!c.set_zeroP) launch_node<1>(c, nn, nullptr, SumXBody{c.c_x}, MeanFin{&c.st->mean_p, nullptr, nn});
+    launch_node<0>(c, nn, nullptr, PoutBody{c.c_x, pin, P_out, c.st, !c.set_zeroP}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+    return it;
+}
+
+// ------------------------------------------------------------------------------ mass matvec
+struct MassQ2Body {
+    Grid2D g; const ElemConstQ2* ec; const double* x; double* y;
+    __device__ void operator()(int n, double (&)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const double* xc = x + (size_t)d * g.nn;
+            y[n + (size_t)d * g.nn] = q2_const(g, s, ec->Me, [&](int id) { return xc[id]; });
+        }
+    }
+};
+void mass_matvec_q2(Ctx& c, const double* x, double* y) {
+    launch_node<0>(c, c.g.nn, nullptr, MassQ2Body{c.g, c.eq2_dev, x, y}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------------------ boundary integral
+// F_p[n] = sum over the node's Dirichlet-edge slots (bconn order) of sum_b W_f[a][b] (n_f . U_d(b)),
+// W_f[a][b] = sum_q w_q N_qa N_qb of lin3 on edge f (element_calc.py:111-135)
+__global__ void k_boundary_integral_q2(int nn, const int* __restrict__ off, const int* __restrict__ slot,
+                                       const double* __restrict__ W, const double* __restrict__ nv,
+                                       const int* __restrict__ ui, const double* __restrict__ U_d, double* F_p) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    double acc = 0.0;
+    for (int k = off[n]; k < off[n + 1]; ++k) {
+        const int f = slot[k] / 3, a = slot[k] - 3 * f;
+        double t = 0.0;
+        for (int b = 0; b < 3; ++b) {
+            const int* u = ui + 6 * (size_t)f + 2 * b;
+            t += W[9 * (size_t)f + a * 3 + b] * (nv[2 * f] * U_d[u[0]] + nv[2 * f + 1] * U_d[u[1]]);
+        }
+        acc += t;
+    }
+    F_p[n] = acc;
+}
+
+void boundary_integral_q2(Ctx& c, const double* U_d, double* F_p) {
+    const int nn = c.g.nn;
+    if (!c.fb_off) {
+        launch_fill(c, F_p, 0.0, nn);
+        return;
+    }
+    k_boundary_integral_q2<<<(nn + 255) / 256, 256, 0, c.stream>>>(nn, c.fb_off, c.fb_edge, c.fb_w, c.fb_geom,
+                                                                    c.fb_ui, U_d, F_p);
+    FCFD_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------------------ host set-up
+template <class T>
+static void upload_q2(const std::vector<T>& h, T* d) {
+    FCFD_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// out[n] = sum over the incident elements of per_local[the node's local index], element order
+static void node_sums_q2(const Grid2D& g, const double (&per_local)[9], std::vector<double>& out) {
+    out.assign(g.nn, 0.0);
+    for (int je = 0; je < g.ney; ++je)
+        for (int ie = 0; ie < g.nex; ++ie)
+            for (int b = 0; b < 9; ++b) {
+                int x = 2 * ie + q9x(b), y = 2 * je + q9y(b);
+                if (g.periodic) { x %= g.nnx; y %= g.nny; }
+                out[y * g.nnx + x] += per_local[b];
+            }
+}
+
+void build_ctx_q2(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
+    FCFD_REQUIRE(m->nex >= 1 && m->ney >= 1, "bad element counts");
+    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV,
+                 "quad9 pressure preconditioner: none, jacobi or approx_inv");
+    FCFD_REQUIRE(c.ce_format == FCFD_CE_F64 || c.ce_format == FCFD_CE_F32, "quad9 Ce formats: FP64 or FP32");
+    c.order = 2;
+    Grid2D g;
+    g.nex = m->nex; g.ney = m->ney; g.periodic = m->periodic;
+    g.nnx = m->periodic ? 2 * m->nex : 2 * m->nex + 1;
+    g.nny = m->periodic ? 2 * m->ney : 2 * m->ney + 1;
+    g.nn = g.nnx * g.nny;
+    g.ne = m->nex * m->ney;
+    FCFD_REQUIRE((int64_t)g.nn * 2 < ((int64_t)1 << 31), "grid too large");
+    c.g = g;
+    c.lx = m->lx; c.ly = m->ly;
+    const int nn = g.nn;
+    quad9_constants(c.lx / g.nex, c.ly / g.ney, c.eq2);
+    c.eq2_dev = dalloc<ElemConstQ2>(c, 1);
+    FCFD_HIP(hipMemcpy(c.eq2_dev, &c.eq2, sizeof(ElemConstQ2), hipMemcpyHostToDevice));
+    {
+        double rs[9];
+        for (int a = 0; a < 9; ++a) {
+            rs[a] = 0.0;
+            for (int b = 0; b < 9; ++b) rs[a] += c.eq2.Me[a * 9 + b];
+        }
+        std::vector<double> M;
+        node_sums_q2(g, rs, M);
+        M.resize(2 * (size_t)nn);
+        std::copy(M.begin(), M.begin() + nn, M.begin() + nn);
+        c.Mlump = dalloc<double>(c, 2 * (size_t)nn);
+        upload_q2(M, c.Mlump);
+    }
+    c.nlev = 1;
+    c.lv.resize(1);
+    {
+        Level& L = c.lv[0];
+        L.g = g;
+        double dg[9];
+        for (int a = 0; a < 9; ++a) dg[a] = c.eq2.Le[a * 9 + a];
+        std::vector<double> Ld;
+        node_sums_q2(g, dg, Ld);
+        L.Ld = dalloc<double>(c, nn);
+        upload_q2(Ld, L.Ld);
+        L.x = dalloc<double>(c, nn);
+        L.y = dalloc<double>(c, nn);
+    }
+    if (c.set_zeroP || c.precond == FCFD_PC_APPROX_INV) {
+        std::vector<uint8_t> pin(nn, 0);
+        if (s->gamma_p && s->n_gamma_p > 0) {
+            for (int64_t k = 0; k < s->n_gamma_p; ++k) {
+                FCFD_REQUIRE(s->gamma_p[k] >= 0 && s->gamma_p[k] < nn, "gamma_p out of range");
+                pin[s->gamma_p[k]] = 1;
+            }
+        } else {
+            pin[0] = 1;
+        }
+        c.ppin = dalloc<uint8_t>(c, nn);
+        upload_q2(pin, c.ppin);
+    }
+    std::vector<int64_t> gd;
+    if (s->gamma_d && s->n_gamma_d > 0) {
+        gd.assign(s->gamma_d, s->gamma_d + s->n_gamma_d);
+        for (size_t k = 0; k < gd.size(); ++k) {
+            FCFD_REQUIRE(gd[k] >= 0 && gd[k] < 2 * (int64_t)nn, "gamma_d out of range");
+            FCFD_REQUIRE(k == 0 || gd[k] > gd[k - 1], "gamma_d must be sorted and unique");
+        }
+        c.n_gamma_d = (int)gd.size();
+        std::vector<uint8_t> mask(2 * (size_t)nn, 0);
+        for (auto v : gd) mask[v] = 1;
+        c.dmask = dalloc<uint8_t>(c, 2 * (size_t)nn);
+        upload_q2(mask, c.dmask);
+        c.gamma_d_dev = dalloc<int64_t>(c, gd.size());
+        upload_q2(gd, c.gamma_d_dev);
+        c.ud_full = dalloc<double>(c, 2 * (size_t)nn);
+        FCFD_HIP(hipMemset(c.ud_full, 0, sizeof(double) * 2 * nn));
+    }
+    // Dirichlet edges (3 nodes each) -> per-node gather lists in bconn order; lin3 weights from
+    // the edge's node coordinates (shape_functions.py:89-106)
+    if (s->bconn && s->nvec && s->n_bedge > 0 && !gd.empty()) {
+        const int nf = (int)s->n_bedge;
+        c.n_bedge = nf;
+        std::vector<std::vector<int>> lists(nn);
+        std::vector<double> W(9 * (size_t)nf), geom(2 * (size_t)nf);
+        std::vector<int> ui(6 * (size_t)nf);
+        const double hx = c.lx / g.nex / 2, hy = c.ly / g.ney / 2;
+        const double gq = std::sqrt(3.0 / 5.0);
+        const double xq[3] = {-gq, 0.0, gq}, wq[3] = {5.0 / 9, 8.0 / 9, 5.0 / 9};
+        auto sorted_index = [&](int64_t v) { return (int)(std::lower_bound(gd.begin(), gd.end(), v) - gd.begin()); };
+        for (int f = 0; f < nf; ++f) {
+            double X[3], Y[3];
+            for (int a = 0; a < 3; ++a) {
+                const int64_t v = s->bconn[3 * (size_t)f + a];
+                FCFD_REQUIRE(v >= 0 && v < nn, "bconn out of range");
+                X[a] = (v % g.nnx) * hx;
+                Y[a] = (v / g.nnx) * hy;
+                for (int d = 0; d < 2; ++d) {
+                    const int idx = sorted_index(v + (int64_t)d * nn);
+                    FCFD_REQUIRE(idx < (int)gd.size() && gd[idx] == v + (int64_t)d * nn,
+                                 "boundary node is not a Dirichlet DOF");
+                    ui[6 * (size_t)f + 2 * a + d] = idx;
+                }
+                lists[v].push_back(f * 3 + a);
+            }
+            for (int k = 0; k < 9; ++k) W[9 * (size_t)f + k] = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                const double t = xq[q];
+                const double N[3] = {t * (t - 1) / 2, 1 - t * t, t * (t + 1) / 2};
+                const double dN[3] = {(2 * t - 1) / 2, -2 * t, (2 * t + 1) / 2};
+                const double dx = dN[0] * X[0] + dN[1] * X[1] + dN[2] * X[2];
+                const double dy = dN[0] * Y[0] + dN[1] * Y[1] + dN[2] * Y[2];
+                const double w = wq[q] * std::sqrt(dx * dx + dy * dy);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) W[9 * (size_t)f + a * 3 + b] += w * N[a] * N[b];
+            }
+            for (int d = 0; d < 2; ++d) geom[2 * (size_t)f + d] = (double)s->nvec[2 * (size_t)f + d];
+        }
+        std::vector<int> off(nn + 1, 0), slots;
+        for (int n = 0; n < nn; ++n) {
+            off[n + 1] = off[n] + (int)lists[n].size();
+            slots.insert(slots.end(), lists[n].begin(), lists[n].end());
+        }
+        c.fb_off = dalloc<int>(c, nn + 1);
+        upload_q2(off, c.fb_off);
+        c.fb_edge = dalloc<int>(c, slots.size());
+        upload_q2(slots, c.fb_edge);
+        c.fb_w = dalloc<double>(c, W.size());
+        upload_q2(W, c.fb_w);
+        c.fb_geom = dalloc<double>(c, geom.size());
+        upload_q2(geom, c.fb_geom);
+        c.fb_ui = dalloc<int>(c, ui.size());
+        upload_q2(ui, c.fb_ui);
+    }
+    const size_t ne = g.ne;
+    if (c.ce_format == FCFD_CE_F64) c.ce = dalloc<double>(c, 81 * ne);
+    else c.ce = dalloc<float>(c, 81 * ne);
+    const size_t n2 = 2 * (size_t)nn;
+    c.m_r = dalloc<double>(c, n2);
+    c.m_rhat = dalloc<double>(c, n2);
+    for (int k = 0; k < 2; ++k) {
+        c.m_p[k] = dalloc<double>(c, n2);
+        c.m_q[k] = dalloc<double>(c, n2);
+        c.c_p[k] = dalloc<double>(c, nn);
+        c.c_z[k] = dalloc<double>(c, nn);
+    }
+    c.m_s = dalloc<double>(c, n2);
+    c.m_t = dalloc<double>(c, n2);
+    c.m_x = dalloc<double>(c, n2);
+    c.c_r = dalloc<double>(c, nn);
+    c.c_Ap = dalloc<double>(c, nn);
+    c.c_x = dalloc<double>(c, nn);
+    c.c_rhs = dalloc<double>(c, nn);
+    c.ustar = dalloc<double>(c, n2);
+    c.pstride = blocks_for((int)n2) + 1;
+    c.partials = dalloc<double>(c, (size_t)MAX_RED * c.pstride);
+    c.st = dalloc<DevState>(c, 1);
+    FCFD_HIP(hipMemset(c.st, 0, sizeof(DevState)));
+    FCFD_HIP(hipHostMalloc((void**)&c.h_pinned, 64, hipHostMallocDefault));
+    FCFD_HIP(hipDeviceSynchronize());
+}
+
+// ------------------------------------------------------------------------------ the step
+void four_step_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
+                  const double* F_p, const double* F, double* U, double* P, int* it) {
+    scatter_ud(c, U_d);
+    it[0] = solve_momentum_q2(c, U_n, P_n, rho, mu, dt, F, c.m_x);
+    launch_node<0>(c, c.g.nn, nullptr,
+                   ProjQ2Body{c.g, c.eq2_dev, c.m_x, P_n, c.Mlump, nullptr, nullptr, c.ustar, dt / rho}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+    it[1] = solve_pressure_q2(c, c.ustar, P_n, F_p, rho, dt, P);
+    launch_node<0>(c, c.g.nn, nullptr,
+                   ProjQ2Body{c.g, c.eq2_dev, c.ustar, P, c.Mlump, c.ud_full, c.dmask, U, -dt / rho}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
+}  // namespace fcfd

@@ -19,13 +19,15 @@ class Context:
 
     def __init__(self, nex, ney, lx, ly, periodic, *, tol=1e-8, set_zeroP=False, precond=None, mg_levels=1,
                  precision="fp64", ce_format="f64", gamma_d=None, gamma_p=None, mg_gamma_p=None, bconn=None,
-                 nvec=None, maxiter_momentum=0, maxiter_pressure=0, gamma_d_phi=None, nez=None, lz=None):
+                 nvec=None, maxiter_momentum=0, maxiter_pressure=0, gamma_d_phi=None, nez=None, lz=None, order=1):
         lib = nat.load_library()
         t = nat.require_gpu()
         self.nex, self.ney, self.lx, self.ly, self.periodic = int(nex), int(ney), float(lx), float(ly), bool(periodic)
         self.ndim = 2 if nez is None else 3
-        self.nnx = self.nex if periodic else self.nex + 1
-        self.nny = self.ney if periodic else self.ney + 1
+        self.order = int(order)
+        k = self.order  # quad9: node grid of 2 nex (+ 1) per direction
+        self.nnx = k * self.nex if periodic else k * self.nex + 1
+        self.nny = k * self.ney if periodic else k * self.ney + 1
         self.nn = self.nnx * self.nny
         self.ne = self.nex * self.ney
         if self.ndim == 3:
@@ -33,11 +35,11 @@ class Context:
             self.nnz = self.nez + 1
             self.nn *= self.nnz
             self.ne *= self.nez
-        self.nen = 2 ** self.ndim  # nodes per element
+        self.nen = 9 if self.order == 2 else 2 ** self.ndim  # nodes per element
         self.tol, self.set_zeroP, self.precision, self.ce_format = tol, bool(set_zeroP), precision, ce_format
         self.precond = precond
         md = nat.MeshDesc(self.ndim, self.nex, self.ney, self.lx, self.ly, int(self.periodic),
-                          getattr(self, "nez", 0), getattr(self, "lz", 0.0))
+                          getattr(self, "nez", 0), getattr(self, "lz", 0.0), self.order)
         keep = []
 
         def arr(a):
@@ -105,6 +107,11 @@ class Context:
         return coords.view(self.nn, 2), nconn.view(self.ne, 4)
 
     def element_constant(self, which):
+        if self.order == 2:
+            out = (ctypes.c_double * 81)()
+            nat.check(self._lib.fcfd_element_matrix(
+                self._h, {"Me": 0, "Le": 1, "Gx": 2, "Gy": 3, "Le_off": 5}[which], out, 81), "fcfd_element_matrix")
+            return np.array(out[:]).reshape(9, 9)
         if self.ndim == 3:
             out = (ctypes.c_double * 64)()
             nat.check(self._lib.fcfd_element_constant3(

@@ -26,7 +26,7 @@ class NativeError(RuntimeError):
 class MeshDesc(ctypes.Structure):
     _fields_ = [("ndim", ctypes.c_int32), ("nex", ctypes.c_int32), ("ney", ctypes.c_int32),
                 ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("periodic", ctypes.c_int32),
-                ("nez", ctypes.c_int32), ("lz", ctypes.c_double)]
+                ("nez", ctypes.c_int32), ("lz", ctypes.c_double), ("order", ctypes.c_int32)]
 
 
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -61,6 +61,7 @@ _SIGS = {
     "fcfd_lumped_mass": (ctypes.c_int, [_vp, _vp]),
     "fcfd_element_constant3": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_d)]),
     "fcfd_mass_matvec": (ctypes.c_int, [_vp, _vp, _vp]),
+    "fcfd_element_matrix": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_d), _i32]),
     "fcfd_laplacian_diag": (ctypes.c_int, [_vp, _i32, _vp]),
     "fcfd_conv_elem_calc": (ctypes.c_int, [_vp, _vp, _d, _vp]),
     "fcfd_boundary_integral": (ctypes.c_int, [_vp, _vp, _vp]),

@@ -45,9 +45,18 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
                   gamma_d_phi=None):
     mc = config.mesh_config
     sc = ns_config if ns_config is not None else config.solver_config
-    if mc.shape_func_ord != 1:
-        raise NotImplementedError("the GPU timestep is built for Q1 (quad4 / hex8) meshes")
     ce_format = sc.ce_format or ("f64" if sc.precision == "fp64" else "f32")
+    if mc.shape_func_ord == 2:
+        if gamma_d_phi is not None:
+            raise NotImplementedError("the quad9 GPU path has no passive scalar")
+        nex, ney = mc.num_elem
+        return Context(nex, ney, mc.domain_size[0], mc.domain_size[1], mc.periodic, tol=sc.solver_tol,
+                       set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=1, precision=sc.precision,
+                       ce_format=ce_format, gamma_d=gamma_d,
+                       gamma_p=np.atleast_1d(gamma_p) if (sc.set_zeroP or sc.pressure_precond == "approx_inv")
+                       else None, bconn=bconn, nvec=nvec, order=2)
+    if mc.shape_func_ord != 1:
+        raise NotImplementedError("the GPU timestep is built for quad4, quad9 and hex8 meshes")
     if len(mc.num_elem) == 3:
         if gamma_d_phi is not None:
             raise NotImplementedError("the 3D GPU path has no passive scalar")
@@ -82,7 +91,7 @@ def setup_solver(node_coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_fun
             F_p = ctx.boundary_integral(U_d)
         if S_func is not None:
             S = (nat.as_dev(S_func(t_n)) + nat.as_dev(S_func(t))) / 2
-            if ctx.ndim == 3:
+            if ctx.ndim == 3 or ctx.order == 2:
                 F = ctx.mass_matvec(S)
             else:
                 F = ctx._empty(2 * nn)

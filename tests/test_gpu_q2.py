@@ -1,0 +1,130 @@
+"""GPU parity of the quad9 (Q2) path against the oracle: element constants, convection matrices,
+the lin3 boundary integral, the mass matvec and whole four-step timesteps (lid-driven cavity with
+each pressure preconditioner, periodic Taylor-Green, the mixed form)."""
+import numpy as np
+import pytest
+
+from oracle import elements as el
+from oracle import emult
+from oracle import mesh as ms
+from oracle import ns
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(99)
+
+
+def _np(x):
+    return x.detach().cpu().numpy()
+
+
+def _rel(a, b):
+    return ns.rel_l2(a, b)
+
+
+def _setup(n=(8, 8), L=(1.0, 1.0), periodic=False, faces=(0, 1, 2, 3), outlets=(), precond="jacobi", tol=1e-10,
+           set_zeroP=True, precision="fp64", final=None):
+    import jax_fem_cfd.setup.simulation as sim
+    cfg = sim.Config(mesh="simple", solver="standard",
+                     mesh_config=sim.SimpleMeshConfig(list(n), list(L), 2, periodic),
+                     solver_config=sim.StandardSolverConfig(list(faces), [], list(outlets), False, tol, set_zeroP,
+                                                            precond, final, precision))
+    out = sim.setup(cfg)
+    s, surf, _, U_d = ns.build(list(n), list(L), periodic, list(faces), list(outlets), tol, set_zeroP, precond, final,
+                               shape_func_ord=2)
+    compute, timestep, coords, osurf, h, nn, gamma_d, _ = out
+    assert nn == s.nn and np.allclose(coords, s.coords)
+    if gamma_d is not None:
+        assert np.array_equal(np.asarray(gamma_d), s.gamma_d)
+    return compute, timestep, s, surf, U_d
+
+
+def test_quad9_constants_match_oracle():
+    print("\n[case] quad9 constants", flush=True)
+    _, timestep, s, _, _ = _setup((4, 3), (2.0, 1.5))
+    ctx = timestep.context
+    assert ctx.order == 2 and ctx.nn == s.nn
+    assert np.allclose(ctx.element_constant("Me"), s.Me, rtol=1e-13, atol=1e-17)
+    assert np.allclose(ctx.element_constant("Le"), s.Le, rtol=1e-12, atol=1e-14)
+    assert np.allclose(ctx.element_constant("Gx"), s.Ge[0], rtol=1e-12, atol=1e-16)
+    assert np.allclose(ctx.element_constant("Gy"), s.Ge[1], rtol=1e-12, atol=1e-16)
+    assert np.allclose(_np(ctx.lumped_mass()), s.M, rtol=1e-13)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_quad9_convection_matrices(periodic):
+    print(f"\n[case] quad9 Ce periodic={periodic}", flush=True)
+    faces = () if periodic else (0, 1, 2, 3)
+    _, timestep, s, _, _ = _setup((5, 3), (2.0, 1.5), periodic, faces, set_zeroP=not periodic)
+    U = RNG.standard_normal(2 * s.nn)
+    for beta in (0.5, 1.0):
+        Ce = _np(timestep.context.conv_elem_calc(U, beta))
+        ref = el.conservative_convec_elem_calc(s.nconn, U, s.wq, s.N, s.Nderiv, s.nn, 2, beta)
+        assert np.abs(Ce - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_quad9_boundary_integral_and_mass():
+    print("\n[case] quad9 F_p and M_c", flush=True)
+    compute, timestep, s, surf, U_d = _setup((6, 4), (1.5, 1.0), faces=(0, 2, 3), outlets=(1,))
+    U_d = ms.set_face_velocity(U_d, surf[3], s.gamma_d, RNG.standard_normal(len(surf[3])), s.nn, 1)
+    U_d = ms.set_face_velocity(U_d, surf[0], s.gamma_d, RNG.standard_normal(len(surf[0])), s.nn, 2)
+    F_p = _np(timestep.context.boundary_integral(U_d))
+    ref = el.u_dot_n_boundary_integral_1d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords, el.lin3_shape_functions)
+    assert np.abs(F_p - ref).max() <= 1e-14 * np.abs(ref).max()
+    x = RNG.standard_normal(2 * s.nn)
+    assert _rel(_np(timestep.context.mass_matvec(x)), emult.consistent_mass_mult(x, s.Me, s.nconn, s.nn, 2)) < 1e-14
+
+
+@pytest.mark.parametrize("precond", ["jacobi", "approx_inv", None])
+def test_quad9_cavity_steps_match_oracle(precond):
+    n, dt, rho, mu = (8, 8), 1e-3, 1.0, 1e-2
+    print(f"\n[case] quad9 cavity {n} precond={precond}", flush=True)
+    compute, timestep, s, surf, U_d = _setup(n, precond=precond)
+    U_d = ns.cavity_lid(s, surf, U_d)
+    U = np.zeros(2 * s.nn)
+    U[s.gamma_d] = U_d
+    P = np.zeros(s.nn)
+    Ug, Pg = U.copy(), P.copy()
+    for k in range(4):
+        F_p, F = compute(U_d, None, 0, 0)
+        Ug, Pg, ui, pi = timestep(Ug, Pg, dt, rho, mu, U_d, F_p, F)
+        oF_p, oF = ns.compute(s, U_d, None, 0, 0)
+        U, P, oui, opi = ns.four_step_timestep(s, U, P, dt, rho, mu, U_d, oF_p, oF)
+        eu, ep = _rel(_np(Ug), U), _rel(_np(Pg), P)
+        print(f"  step {k}: iters {ui},{pi} oracle {oui},{opi}  rel U {eu:.2e} P {ep:.2e}", flush=True)
+        assert (ui, pi) == (oui, opi)
+        assert eu < 1e-10 and ep < 1e-10
+
+
+def test_quad9_taylor_green_periodic_matches_oracle():
+    n, dt, rho, mu = (8, 8), 1e-2, 1.0, 0.01
+    print(f"\n[case] quad9 Taylor-Green {n}", flush=True)
+    compute, timestep, s, surf, _ = _setup(n, (2 * np.pi, 2 * np.pi), True, (), precond="jacobi", set_zeroP=False)
+    u, v, p = ns.taylor_green(s.coords, 0.0, rho, mu)
+    U, P = np.concatenate([u, v]), p
+    Ug, Pg = U.copy(), P.copy()
+    for _ in range(3):
+        Ug, Pg, ui, pi = timestep(Ug, Pg, dt, rho, mu, None, 0, 0)
+        U, P, oui, opi = ns.four_step_timestep(s, U, P, dt, rho, mu, None, 0, 0)
+        assert (ui, pi) == (oui, opi)
+    assert _rel(_np(Ug), U) < 1e-10 and _rel(_np(Pg), P) < 1e-10
+
+
+def test_quad9_mixed_form_error():
+    """FP32-stored Ce: per-field error against the FP64 oracle after 4 cavity steps < 1e-6."""
+    n, dt, rho, mu = (8, 8), 1e-3, 1.0, 1e-2
+    print(f"\n[case] quad9 cavity {n} mixed", flush=True)
+    compute, timestep, s, surf, U_d = _setup(n, precision="mixed")
+    U_d = ns.cavity_lid(s, surf, U_d)
+    U = np.zeros(2 * s.nn)
+    U[s.gamma_d] = U_d
+    P = np.zeros(s.nn)
+    Ug, Pg = U.copy(), P.copy()
+    for _ in range(4):
+        F_p, F = compute(U_d, None, 0, 0)
+        Ug, Pg, _, _ = timestep(Ug, Pg, dt, rho, mu, U_d, F_p, F)
+        oF_p, oF = ns.compute(s, U_d, None, 0, 0)
+        U, P, _, _ = ns.four_step_timestep(s, U, P, dt, rho, mu, U_d, oF_p, oF)
+    Ug = _np(Ug)
+    eu, ev, ep = _rel(Ug[:s.nn], U[:s.nn]), _rel(Ug[s.nn:], U[s.nn:]), _rel(_np(Pg), P)
+    print(f"  rel u {eu:.2e} v {ev:.2e} p {ep:.2e}", flush=True)
+    assert eu < 1e-6 and ev < 1e-6 and ep < 1e-6

@@ -24,3 +24,20 @@ def test_iteration_history_layout_and_round_trip(tmp_path):
     assert np.array_equal(np.load(path), np.stack([m, pr, st]))
     a, b, c = running.load_iter_hist(str(path))
     assert np.array_equal(a, m) and np.array_equal(b, pr) and np.array_equal(c, st)
+
+
+def test_host_quad9_meshes_match_oracle():
+    """Host quad9 mesh generators equal the oracle's (regular_mesh.py:107-145, :182-218)."""
+    import numpy as np
+    from jax_fem_cfd.setup import regular_mesh as rm
+    from oracle import mesh as ms
+    for f_host, f_or in ((rm.create_mesh_2d_quad9, ms.create_mesh_2d_quad9),
+                         (rm.create_periodic_mesh_2d_quad9, ms.create_periodic_mesh_2d_quad9)):
+        a, b = f_host(5, 3, 2.0, 1.5), f_or(5, 3, 2.0, 1.5)
+        assert np.allclose(np.asarray(a[0]), b[0], rtol=0, atol=1e-15)
+        assert np.array_equal(np.asarray(a[1]), b[1])
+        if b[2] is not None:
+            for x, y in zip(a[2], b[2]):
+                assert np.array_equal(np.asarray(x), y)
+            for x, y in zip(a[3], b[3]):
+                assert np.array_equal(np.asarray(x), y)
