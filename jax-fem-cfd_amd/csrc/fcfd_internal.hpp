@@ -42,6 +42,7 @@ struct Level {
     Grid2D g;
     ElemConst ec;
     ElemConst* ec_dev = nullptr;
+    ElemConstQ2* eq2_dev = nullptr;  // quad9 contexts: this level's 9 x 9 constants
     double* Ld = nullptr;       // L_diag
     float* Ld32 = nullptr;
     uint8_t* pin = nullptr;     // pinned form: 1 at gamma_p of this level
@@ -290,6 +291,7 @@ int solve_pressure_q2(Ctx& c, const double* U_star, const double* P_n, const dou
 void boundary_integral_q2(Ctx& c, const double* U_d, double* F_p);
 void mass_matvec_q2(Ctx& c, const double* x, double* y);
 void build_ce_q2_api(Ctx& c, const double* U, double beta, double* Ce);
+void vcycle_q2_api(Ctx& c, const double* r, double* z);
 
 // reverse-mode products of the steps (fcfd_adjoint.hip)
 void ns_vjp(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,

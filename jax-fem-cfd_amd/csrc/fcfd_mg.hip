@@ -12,10 +12,10 @@
 #include "fcfd_internal.hpp"
 #include "fcfd_ops.hpp"
 #include "fcfd_cg.hpp"
+#include "fcfd_mgnodes.hpp"
 
 namespace fcfd {
 
-constexpr double OMEGA = 2.0 / 3.0;
 
 template <class T> struct LvBufs;
 template <> struct LvBufs<double> {
@@ -40,18 +40,6 @@ template <> struct LvBufs<float> {
 };
 
 // ------------------------------------------------------------------------------ down leg
-// xh = 2/3 (r - mean r) / L_diag   (multigrid.py:257-261), sum -> mean(xh)
-template <class T, class TR>
-struct DownX {
-    const TR* r; const T* Ld; T* x; const DevState* st; int l;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        const T mr = (T)st->mg[l][MG_R];
-        const T v = (T(2.0 / 3.0) * ((T)r[n] - mr)) / Ld[n];
-        x[n] = v;
-        acc[0] += (double)v;
-    }
-};
-
 // rH = rh - L xh   (Neumann: multigrid.py:263-264; pinned: :246-251)
 template <class T, class TR>
 struct DownRes {
@@ -119,30 +107,6 @@ struct DownResS {
         }
         f[n] = fv;
         acc[0] += (double)fv;
-    }
-};
-
-// coarse[I] = R (f - mean f): R = I^T without the 1/4 factor (multigrid.py:142-212);
-// missing neighbours (domain edges) contribute zero.  Periodic grids restrict the padded fine
-// grid with the same formulas and drop the wrap row/column, so only negative fine indices can
-// be missing there.
-template <class T>
-struct Restrict {
-    Grid2D gf, gc; const T* f; T* rc; const DevState* st; int l; int neumann;
-    __device__ void operator()(int nc, double (&acc)[1]) const {
-        const int I = nc / gc.nnx, J = nc - I * gc.nnx;
-        const int fi = 2 * I, fj = 2 * J;
-        const T mf = neumann ? (T)st->mg[l][MG_F] : T(0);
-        auto val = [&](int dy, int dx) -> T {
-            int a = fi + dy, b = fj + dx;
-            bool ok = a >= 0 && b >= 0 && (gf.periodic || (a < gf.nny && b < gf.nnx));
-            return ok ? f[a * gf.nnx + b] - mf : T(0);
-        };
-        const T E = ((val(1, 0) + val(-1, 0)) + val(0, 1)) + val(0, -1);
-        const T D = ((val(1, 1) + val(1, -1)) + val(-1, 1)) + val(-1, -1);
-        const T v = (val(0, 0) + T(0.5) * E) + T(0.25) * D;
-        rc[nc] = v;
-        acc[0] += (double)v;
     }
 };
 
@@ -216,26 +180,6 @@ __global__ __launch_bounds__(256) void k_down_fused(Grid2D gf, Grid2D gc, const 
 }
 
 // ------------------------------------------------------------------------------ up leg
-// bilinear prolongation of the coarse vector (minus its mean) at fine node n
-// (multigrid.py:90-140; periodic coarse grids wrap)
-template <class T>
-__device__ __forceinline__ T interp_at(const Grid2D& gf, const Grid2D& gc, const T* yc, T my, int n) {
-    // Branch-free (lanes of a wave alternate parity): the two coarse columns / rows coincide for
-    // even fine indices, where 0.5 a + 0.5 a = a exactly.
-    const int j = n / gf.nnx, i = n - j * gf.nnx;
-    const int i0 = i >> 1, j0 = j >> 1;
-    int i1 = (i + 1) >> 1, j1 = (j + 1) >> 1;
-    if (gc.periodic) {
-        i1 = i1 >= gc.nnx ? i1 - gc.nnx : i1;
-        j1 = j1 >= gc.nny ? j1 - gc.nny : j1;
-    }
-    const T a = yc[j0 * gc.nnx + i0] - my, b = yc[j0 * gc.nnx + i1] - my;
-    const T c = yc[j1 * gc.nnx + i0] - my, d = yc[j1 * gc.nnx + i1] - my;
-    const T r0 = T(0.5) * a + T(0.5) * b;
-    const T r1 = T(0.5) * c + T(0.5) * d;
-    return T(0.5) * r0 + T(0.5) * r1;
-}
-
 // interp_at at all 9 stencil points of a node from one 3x3 patch of coarse values: fine index f
 // interpolates coarse (f >> 1, (f + 1) >> 1), so fine f - 1 .. f + 1 touch coarse
 // floor((f - 1) / 2) + {0, 1, 2}.  Which patch entries pair up depends only on the parity of the
@@ -283,16 +227,6 @@ __device__ __forceinline__ void interp3x3(const Grid2D& gc, const T* yc, T my, c
     }
 }
 
-template <class T>
-struct Prolong {
-    Grid2D gf, gc; const T* yc; T* p; const DevState* st; int l;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        const T v = interp_at(gf, gc, yc, (T)st->mg[l + 1][MG_Y], n);
-        p[n] = v;
-        acc[0] += (double)v;
-    }
-};
-
 // xh <- xh + I xH; one damped Jacobi (multigrid.py:214-228, :306-326)
 template <class T, class TR>
 struct Jacobi {
@@ -332,22 +266,6 @@ struct Jacobi {
 // ------------------------------------------------------------------------------ coarse level
 // One workgroup.  Pinned form: CG (M = identity) to tol with maxiter 1000 on L with the pinned
 // rows zeroed (multigrid.py:278-287).  Neumann form: one Jacobi from zero (:289-301).
-
-// Neumann: x = omega (rN - Le_off 0) / L_diag + (1 - omega) 0, then its mean
-template <class T, class TR>
-__global__ __launch_bounds__(1024) void k_coarse_jacobi(int nn, const T* Ld, const TR* r, T* x, DevState* st, int l,
-                                                        const int* active) {
-    if (active && *active == 0) return;
-    const T mr = (T)st->mg[l][MG_R];
-    double acc[1] = {0.0};
-    for (int n = threadIdx.x; n < nn; n += blockDim.x) {
-        const T v = (T(OMEGA) * (((T)r[n] - mr) - T(0))) / Ld[n] + T(1.0 - OMEGA) * T(0);
-        x[n] = v;
-        acc[0] += (double)v;
-    }
-    block_sum<1>(acc);
-    if (threadIdx.x == 0) st->mg[l][MG_Y] = acc[0] / nn;
-}
 
 // Pinned, coarse grid of at most NPT * blockDim nodes: search direction in LDS, x / r / Ap in
 // registers (node n = tid + i * blockDim, i < NPT).

@@ -15,6 +15,7 @@
 #include <cmath>
 #include "fcfd_bicgstab.hpp"
 #include "fcfd_cg.hpp"
+#include "fcfd_mgnodes.hpp"
 
 namespace fcfd {
 
@@ -511,6 +512,181 @@ struct ApproxInvQ2b {
     }
 };
 
+// ------------------------------------------------------------------------------ multigrid
+// The V-cycle of multigrid.py:231-330 on quad9 levels (FP64): the level operators are the 9 x 9
+// quad9 Laplacians, the transfers are the node-grid R = I^T and bilinear prolongation shared
+// with quad4 (the quad9 node grid of nex elements is the quad4 node grid of 2 nex).  Neumann
+// means are applied lazily through the DevState slots, as in the quad4 V-cycle.
+
+// f = r - L x: pinned (x = 2/3 r / L_diag, 0 at the pins, pinned rows 0; multigrid.py:243-254)
+// or Neumann (x, r and f mean-projected; :256-268)
+struct DownResQ2 {
+    Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* x; double* f;
+    const DevState* st; int l; int neumann;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        double fv;
+        if (neumann) {
+            const double mx = st->mg[l][MG_X], mr = st->mg[l][MG_R];
+            fv = (r[n] - mr) - q2_const(g, s, Le, [&](int id) { return x[id] - mx; });
+        } else {
+            auto xv = [&](int id) {
+                const double v = (2.0 / 3.0 * r[id]) / Ld[id];
+                return (pin && pin[id]) ? 0.0 : v;
+            };
+            x[n] = xv(n);
+            const bool pn = pin && pin[n];
+            const double rh = pn ? 0.0 : r[n];
+            const double lx = pn ? 0.0 : q2_const(g, s, Le, xv);
+            fv = rh - lx;
+        }
+        f[n] = fv;
+        acc[0] += fv;
+    }
+};
+
+// x_h <- x_h + I x_H, then one damped Jacobi (multigrid.py:214-228, :306-326)
+struct JacobiQ2 {
+    Grid2D gf, gc; const double* Leoff; const double* Ld; const uint8_t* pin; const double* r; const double* x;
+    const double* p; const double* yc; double* y; const DevState* st; int l; int neumann;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        IncQ2 s;
+        node_inc_q2(gf, n, s);
+        double rh, off, xc;
+        if (neumann) {
+            const double mx = st->mg[l][MG_X], mp = st->mg[l][MG_P];
+            auto xv = [&](int id) { return (x[id] - mx) + (p[id] - mp); };
+            rh = r[n] - (l == 0 ? 0.0 : st->mg[l][MG_R]);
+            off = q2_const(gf, s, Leoff, xv);
+            xc = xv(n);
+        } else {
+            auto xv = [&](int id) {
+                const double v = x[id] + interp_at(gf, gc, yc, 0.0, id);
+                return (pin && pin[id]) ? 0.0 : v;
+            };
+            rh = r[n];
+            off = q2_const(gf, s, Leoff, xv);
+            xc = xv(n);
+        }
+        double v = (OMEGA * (rh - off)) / Ld[n] + (1.0 - OMEGA) * xc;
+        if (!neumann && pin && pin[n]) v = 0.0;
+        y[n] = v;
+        acc[0] += v;
+    }
+};
+
+// Pinned coarse level: CG (M = identity) from 0 to tol, maxiter 1000, on L with the pinned rows
+// zeroed and the rhs zeroed at the pins (multigrid.py:278-287); one workgroup, vectors in global
+// memory (L2-resident), the iteration bounded by maxit.
+__global__ __launch_bounds__(1024) void k_coarse_cg_q2(Grid2D g, const double* __restrict__ Le,
+                                                      const uint8_t* pin, const double* r, double* x, double* cr,
+                                                      double* cp, double* cAp, DevState* st, double tol, int maxit,
+                                                      const int* active) {
+    if (active && *active == 0) return;
+    const int nn = g.nn;
+    double g2[1] = {0.0};
+    for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+        const double b = (pin && pin[n]) ? 0.0 : r[n];
+        cr[n] = b; cp[n] = b; x[n] = 0.0;
+        g2[0] += b * b;
+    }
+    block_sum<1>(g2);
+    double gamma = g2[0];
+    const double atol2 = fmax(tol * tol * gamma, 0.0);
+    int k = 0;
+    while (gamma > atol2 && k < maxit) {
+        double pap[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+            IncQ2 s;
+            node_inc_q2(g, n, s);
+            const double a = (pin && pin[n]) ? 0.0 : q2_const(g, s, Le, [&](int id) { return cp[id]; });
+            cAp[n] = a;
+            pap[0] += cp[n] * a;
+        }
+        block_sum<1>(pap);
+        const double alpha = gamma / pap[0];
+        double rr[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+            x[n] = x[n] + alpha * cp[n];
+            const double rn = cr[n] - alpha * cAp[n];
+            cr[n] = rn;
+            rr[0] += rn * rn;
+        }
+        block_sum<1>(rr);
+        const double beta = rr[0] / gamma;
+        for (int n = threadIdx.x; n < nn; n += blockDim.x) cp[n] = cr[n] + beta * cp[n];
+        __syncthreads();
+        gamma = rr[0];
+        ++k;
+    }
+    for (int n = threadIdx.x; n < nn; n += blockDim.x)
+        if (pin && pin[n]) x[n] = 0.0;
+    if (threadIdx.x == 0) st->coarse_k = k;
+}
+
+static void vcycle_q2(Ctx& c, const double* r0, const int* active) {
+    const int N = c.nlev, ld = N - 1;
+    const int neu = !c.set_zeroP;
+    auto run = [&](int n, const auto& body, const MeanFin& fin) {
+        if (neu) launch_node<1>(c, n, active, body, fin);
+        else launch_node<0>(c, n, active, body, fin);
+    };
+    for (int l = 0; l < ld; ++l) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        const int n = L.g.nn;
+        const double* rl = l == 0 ? r0 : L.r;
+        if (neu)
+            launch_node<1>(c, n, active, DownX<double, double>{rl, L.Ld, L.x, c.st, l},
+                           MeanFin{&c.st->mg[l][MG_X], nullptr, n});
+        run(n, DownResQ2{L.g, L.eq2_dev->Le, rl, L.Ld, L.pin, L.x, L.f, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+        run(Lc.g.nn, Restrict<double>{L.g, Lc.g, L.f, Lc.r, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g.nn});
+    }
+    {
+        Level& C = c.lv[N - 1];
+        const double* rc = N == 1 ? r0 : C.r;
+        if (neu)
+            k_coarse_jacobi<double, double><<<1, 1024, 0, c.stream>>>(C.g.nn, C.Ld, rc, C.y, c.st, N - 1, active);
+        else
+            k_coarse_cg_q2<<<1, 1024, 0, c.stream>>>(C.g, C.eq2_dev->Le, C.pin, rc, C.y, c.cc_r, c.cc_p, c.cc_Ap, c.st,
+                                                     c.tol, 1000, active);
+        FCFD_CHECK_LAUNCH();
+    }
+    for (int l = ld - 1; l >= 0; --l) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        const int n = L.g.nn;
+        const double* rl = l == 0 ? r0 : L.r;
+        if (neu)
+            launch_node<1>(c, n, active, Prolong<double>{L.g, Lc.g, Lc.y, L.f, c.st, l},
+                           MeanFin{&c.st->mg[l][MG_P], nullptr, n});
+        run(n, JacobiQ2{L.g, Lc.g, L.eq2_dev->Leoff, L.Ld, L.pin, rl, L.x, L.f, Lc.y, L.y, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
+    }
+    FCFD_CHECK_LAUNCH();
+}
+
+// z = y0 - mean (Neumann) into an API vector
+struct ZOutQ2 {
+    const double* y; double* z; const DevState* st; int neumann;
+    __device__ void operator()(int k, double (&)[1]) const { z[k] = y[k] - (neumann ? st->mg[0][MG_Y] : 0.0); }
+};
+void vcycle_q2_api(Ctx& c, const double* r, double* z) {
+    FCFD_REQUIRE(c.precond == FCFD_PC_MULTIGRID, "context was not created with the multigrid preconditioner");
+    const int nn = c.g.nn;
+    struct SumBody {
+        const double* r;
+        __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += r[k]; }
+    };
+    launch_node<1>(c, nn, nullptr, SumBody{r}, MeanFin{&c.st->mg[0][MG_R], nullptr, nn});
+    vcycle_q2(c, r, nullptr);
+    launch_node<0>(c, nn, nullptr, ZOutQ2{c.lv[0].y, z, c.st, !c.set_zeroP}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
 static const double* precondition_q2(Ctx& c, const int* active) {
     const int nn = c.g.nn;
     switch (c.precond) {
@@ -524,6 +700,9 @@ static const double* precondition_q2(Ctx& c, const int* active) {
             launch_node<0>(c, nn, active,
                            ApproxInvQ2b{c.g, c.eq2_dev->Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y},
                            NoFin{});
+            return c.lv[0].y;
+        case FCFD_PC_MULTIGRID:
+            vcycle_q2(c, c.c_r, active);
             return c.lv[0].y;
     }
     throw Error(FCFD_ERR_UNSUPPORTED, "pressure preconditioner not supported for quad9");
@@ -664,8 +843,7 @@ static void node_sums_q2(const Grid2D& g, const double (&per_local)[9], std::vec
 
 void build_ctx_q2(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
     FCFD_REQUIRE(m->nex >= 1 && m->ney >= 1, "bad element counts");
-    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV,
-                 "quad9 pressure preconditioner: none, jacobi or approx_inv");
+
     FCFD_REQUIRE(c.ce_format == FCFD_CE_F64 || c.ce_format == FCFD_CE_F32, "quad9 Ce formats: FP64 or FP32");
     c.order = 2;
     Grid2D g;
@@ -694,19 +872,60 @@ void build_ctx_q2(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
         c.Mlump = dalloc<double>(c, 2 * (size_t)nn);
         upload_q2(M, c.Mlump);
     }
-    c.nlev = 1;
-    c.lv.resize(1);
-    {
-        Level& L = c.lv[0];
-        L.g = g;
+    // levels: quad9 meshes of nex >> l elements (multigrid.py:9-88); level 0 always exists
+    const int nlev = c.precond == FCFD_PC_MULTIGRID ? s->mg_levels : 1;
+    FCFD_REQUIRE(nlev >= 1 && nlev <= MAX_LEVELS, "bad number of multigrid levels");
+    if (nlev > 1)
+        FCFD_REQUIRE(g.nex % (1 << (nlev - 1)) == 0 && g.ney % (1 << (nlev - 1)) == 0,
+                     "element counts must be divisible by 2^(levels-1)");
+    c.nlev = nlev;
+    c.lv.resize(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        Level& L = c.lv[l];
+        Grid2D gl;
+        gl.nex = g.nex >> l; gl.ney = g.ney >> l; gl.periodic = g.periodic;
+        gl.nnx = g.periodic ? 2 * gl.nex : 2 * gl.nex + 1;
+        gl.nny = g.periodic ? 2 * gl.ney : 2 * gl.ney + 1;
+        gl.nn = gl.nnx * gl.nny;
+        gl.ne = gl.nex * gl.ney;
+        L.g = gl;
+        ElemConstQ2 eq;
+        quad9_constants(c.lx / gl.nex, c.ly / gl.ney, eq);
+        if (l == 0) eq = c.eq2;
+        L.eq2_dev = dalloc<ElemConstQ2>(c, 1);
+        FCFD_HIP(hipMemcpy(L.eq2_dev, &eq, sizeof(ElemConstQ2), hipMemcpyHostToDevice));
         double dg[9];
-        for (int a = 0; a < 9; ++a) dg[a] = c.eq2.Le[a * 9 + a];
+        for (int a = 0; a < 9; ++a) dg[a] = eq.Le[a * 9 + a];
         std::vector<double> Ld;
-        node_sums_q2(g, dg, Ld);
-        L.Ld = dalloc<double>(c, nn);
+        node_sums_q2(gl, dg, Ld);
+        L.Ld = dalloc<double>(c, gl.nn);
         upload_q2(Ld, L.Ld);
-        L.x = dalloc<double>(c, nn);
-        L.y = dalloc<double>(c, nn);
+        if (l > 0) L.r = dalloc<double>(c, gl.nn);
+        L.x = dalloc<double>(c, gl.nn);
+        L.y = dalloc<double>(c, gl.nn);
+        L.f = dalloc<double>(c, gl.nn);
+        if (c.set_zeroP && c.precond == FCFD_PC_MULTIGRID) {
+            std::vector<uint8_t> pin(gl.nn, 0);
+            if (s->mg_gamma_p && s->mg_gamma_p_off) {
+                for (int64_t k = s->mg_gamma_p_off[l]; k < s->mg_gamma_p_off[l + 1]; ++k) {
+                    const int64_t v = s->mg_gamma_p[k];
+                    FCFD_REQUIRE(v >= 0 && v < gl.nn, "multigrid pin out of range");
+                    pin[v] = 1;
+                    L.pins.push_back(v);
+                }
+            } else {
+                pin[0] = 1;
+                L.pins.push_back(0);
+            }
+            L.pin = dalloc<uint8_t>(c, gl.nn);
+            upload_q2(pin, L.pin);
+        }
+    }
+    {
+        const int ncoarse = c.lv[nlev - 1].g.nn;
+        c.cc_r = dalloc<double>(c, ncoarse);
+        c.cc_p = dalloc<double>(c, ncoarse);
+        c.cc_Ap = dalloc<double>(c, ncoarse);
     }
     if (c.set_zeroP || c.precond == FCFD_PC_APPROX_INV) {
         std::vector<uint8_t> pin(nn, 0);

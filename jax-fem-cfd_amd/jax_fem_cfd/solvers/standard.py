@@ -20,7 +20,7 @@ def _mg_levels(num_elem, final):
     return int(np.log2(num_elem[0] // final[0])) + 1
 
 
-def _mg_pins(num_elem, domain_size, outlet_faces, partial_dirichlet, mesh_func, nlev):
+def _mg_pins(num_elem, domain_size, outlet_faces, partial_dirichlet, mesh_func, nlev, shape_func_ord=1):
     """per-level gamma_p as multigrid_setup builds them (multigrid.py:35-62)."""
     pins = []
     nex, ney = num_elem
@@ -33,6 +33,8 @@ def _mg_pins(num_elem, domain_size, outlet_faces, partial_dirichlet, mesh_func, 
             for f, (s_frac, e_frac, rem) in partial_dirichlet.items():
                 nfe = bconns[f].shape[0]
                 s_idx, e_idx = round(s_frac * nfe), round(e_frac * nfe)
+                if shape_func_ord == 2:
+                    s_idx, e_idx = 2 * s_idx, 2 * e_idx
                 if rem == "outlet":
                     outlet.append(surf[f][:s_idx])
                     outlet.append(surf[f][e_idx + 1:])
@@ -50,11 +52,15 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
         if gamma_d_phi is not None:
             raise NotImplementedError("the quad9 GPU path has no passive scalar")
         nex, ney = mc.num_elem
+        nlev = _mg_levels(mc.num_elem, sc.final_multigrid_mesh) if sc.pressure_precond == "multigrid" else 1
+        mg_pins = None
+        if sc.pressure_precond == "multigrid" and sc.set_zeroP:
+            mg_pins = _mg_pins(mc.num_elem, mc.domain_size, sc.outlet_faces, mc.partial_dirichlet, mesh_func, nlev, 2)
         return Context(nex, ney, mc.domain_size[0], mc.domain_size[1], mc.periodic, tol=sc.solver_tol,
-                       set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=1, precision=sc.precision,
+                       set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=nlev, precision=sc.precision,
                        ce_format=ce_format, gamma_d=gamma_d,
                        gamma_p=np.atleast_1d(gamma_p) if (sc.set_zeroP or sc.pressure_precond == "approx_inv")
-                       else None, bconn=bconn, nvec=nvec, order=2)
+                       else None, mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, order=2)
     if mc.shape_func_ord != 1:
         raise NotImplementedError("the GPU timestep is built for quad4, quad9 and hex8 meshes")
     if len(mc.num_elem) == 3:

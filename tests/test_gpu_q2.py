@@ -128,3 +128,55 @@ def test_quad9_mixed_form_error():
     eu, ev, ep = _rel(Ug[:s.nn], U[:s.nn]), _rel(Ug[s.nn:], U[s.nn:]), _rel(_np(Pg), P)
     print(f"  rel u {eu:.2e} v {ev:.2e} p {ep:.2e}", flush=True)
     assert eu < 1e-6 and ev < 1e-6 and ep < 1e-6
+
+
+@pytest.mark.parametrize("set_zeroP", [True, False])
+def test_quad9_vcycle_matches_oracle(set_zeroP):
+    """One quad9 V-cycle 16x16 -> 2x2 (4 levels), pinned and pure Neumann, against the oracle."""
+    from oracle import multigrid as mg
+    print(f"\n[case] quad9 V-cycle set_zeroP={set_zeroP}", flush=True)
+    compute, timestep, s, surf, U_d = _setup((16, 16), precond="multigrid", set_zeroP=set_zeroP, final=[2, 2])
+    r = RNG.standard_normal(s.nn)
+    if not set_zeroP:
+        r -= r.mean()
+    z = _np(timestep.context.vcycle(r))
+    ref = mg.v_cycle(r, s.levels, s.tol, False, set_zeroP)
+    print(f"  rel {_rel(z, ref):.3e}", flush=True)
+    # the pinned cycle ends in a coarse CG to tol whose rounding follows the summation order
+    assert _rel(z, ref) < 1e-11
+
+
+@pytest.mark.parametrize("set_zeroP", [True, False])
+def test_quad9_cavity_multigrid_steps_match_oracle(set_zeroP):
+    n, dt, rho, mu = (16, 16), 1e-3, 1.0, 1e-2
+    print(f"\n[case] quad9 cavity {n} multigrid set_zeroP={set_zeroP}", flush=True)
+    compute, timestep, s, surf, U_d = _setup(n, precond="multigrid", set_zeroP=set_zeroP, final=[2, 2])
+    U_d = ns.cavity_lid(s, surf, U_d)
+    U = np.zeros(2 * s.nn)
+    U[s.gamma_d] = U_d
+    P = np.zeros(s.nn)
+    Ug, Pg = U.copy(), P.copy()
+    for k in range(3):
+        F_p, F = compute(U_d, None, 0, 0)
+        Ug, Pg, ui, pi = timestep(Ug, Pg, dt, rho, mu, U_d, F_p, F)
+        oF_p, oF = ns.compute(s, U_d, None, 0, 0)
+        U, P, oui, opi = ns.four_step_timestep(s, U, P, dt, rho, mu, U_d, oF_p, oF)
+        eu, ep = _rel(_np(Ug), U), _rel(_np(Pg), P)
+        print(f"  step {k}: iters {ui},{pi} oracle {oui},{opi}  rel U {eu:.2e} P {ep:.2e}", flush=True)
+        assert (ui, pi) == (oui, opi)
+        assert eu < 1e-10 and ep < 1e-10
+
+
+def test_quad9_taylor_green_multigrid_matches_oracle():
+    n, dt, rho, mu = (16, 16), 1e-2, 1.0, 0.01
+    print(f"\n[case] quad9 Taylor-Green {n} multigrid", flush=True)
+    compute, timestep, s, surf, _ = _setup(n, (2 * np.pi, 2 * np.pi), True, (), precond="multigrid",
+                                           set_zeroP=False, final=[2, 2])
+    u, v, p = ns.taylor_green(s.coords, 0.0, rho, mu)
+    U, P = np.concatenate([u, v]), p
+    Ug, Pg = U.copy(), P.copy()
+    for _ in range(3):
+        Ug, Pg, ui, pi = timestep(Ug, Pg, dt, rho, mu, None, 0, 0)
+        U, P, oui, opi = ns.four_step_timestep(s, U, P, dt, rho, mu, None, 0, 0)
+        assert (ui, pi) == (oui, opi)
+    assert _rel(_np(Ug), U) < 1e-10 and _rel(_np(Pg), P) < 1e-10
