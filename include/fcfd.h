@@ -86,6 +86,9 @@ typedef struct fcfd_solver_desc {
     int64_t n_bedge;
     const int64_t* gamma_d_phi; /* passive-scalar Dirichlet nodes or NULL         */
     int64_t n_gamma_d_phi;
+    int32_t approx_inv_variant; /* approx_inv closure: 0 that of standard.py:225-234 (gamma_p rows of
+                                   both inner products 0), 1 that of passive_transport.py:154-162
+                                   (those rows keep D1x and D1L) -- quad4 contexts */
 } fcfd_solver_desc;
 
 /* ---- context --------------------------------------------------------------------------- */

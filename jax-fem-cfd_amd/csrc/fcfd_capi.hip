@@ -165,6 +165,7 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     c.ce_format = s->ce_format;
     c.maxit_mom = s->maxiter_momentum;
     c.maxit_p = s->maxiter_pressure;
+    c.approx_keep = s->approx_inv_variant == 1;
     FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV ||
                      c.precond == FCFD_PC_MULTIGRID,
                  "unknown pressure preconditioner");

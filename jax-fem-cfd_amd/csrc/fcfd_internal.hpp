@@ -85,6 +85,7 @@ struct Ctx {
     ElemConstQ2* eq2_dev = nullptr;
     double tol;
     int set_zeroP, precond, precision, ce_format, maxit_mom, maxit_p;
+    int approx_keep = 0;  // approx_inv gamma_p rows: 0 zeroed (standard.py), 1 kept (passive_transport.py)
     int nlev = 0;
     std::vector<Level> lv;
     // Dirichlet data

@@ -741,25 +741,26 @@ struct CgK1S {
 // Approximate inverse, two-term Neumann series (standard.py:225-234):
 //   D1x = r / L_diag;  D1L = L D1x / L_diag, 0 at gamma_p;  z = 3 D1x - 3 D1L + (L D1L, 0 at gamma_p) / L_diag
 struct ApproxInv1 {
-    Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
+    Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l; int keep;
     __device__ void operator()(int n, double (&)[1]) const {
         NodeNbr s;
         node_nbr(g, n, s);
         double X[3][3];
         load3x3(s, [&](int k) { return r[k] / Ld[k]; }, X);
         const double v = ebe_const(s, Le, X) / Ld[n];
-        d1l[n] = (pin && pin[n]) ? 0.0 : v;
+        d1l[n] = (pin && pin[n]) ? (keep ? X[1][1] : 0.0) : v;
     }
 };
 struct ApproxInv2 {
     Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l; double* z;
+    int keep;
     __device__ void operator()(int n, double (&)[1]) const {
         NodeNbr s;
         node_nbr(g, n, s);
         double X[3][3];
         load3x3(s, [&](int k) { return d1l[k]; }, X);
         double l2 = ebe_const(s, Le, X);
-        if (pin && pin[n]) l2 = 0.0;
+        if (pin && pin[n]) l2 = keep ? X[1][1] : 0.0;
         const double d1x = r[n] / Ld[n];
         z[n] = (3.0 * d1x - 3.0 * X[1][1]) + l2 / Ld[n];
     }
@@ -778,8 +779,10 @@ static const void* precondition(Ctx& c, const int* active, bool& is_f32) {
             return c.lv[0].y;
         case FCFD_PC_APPROX_INV: {
             const double* Le = c.lv[0].ec_dev->Le;
-            launch_node<0>(c, nn, active, ApproxInv1{c.g, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x}, NoFin{});
-            launch_node<0>(c, nn, active, ApproxInv2{c.g, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y},
+            launch_node<0>(c, nn, active, ApproxInv1{c.g, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.approx_keep},
+                           NoFin{});
+            launch_node<0>(c, nn, active,
+                           ApproxInv2{c.g, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y, c.approx_keep},
                            NoFin{});
             return c.lv[0].y;
         }

@@ -19,7 +19,8 @@ class Context:
 
     def __init__(self, nex, ney, lx, ly, periodic, *, tol=1e-8, set_zeroP=False, precond=None, mg_levels=1,
                  precision="fp64", ce_format="f64", gamma_d=None, gamma_p=None, mg_gamma_p=None, bconn=None,
-                 nvec=None, maxiter_momentum=0, maxiter_pressure=0, gamma_d_phi=None, nez=None, lz=None, order=1):
+                 nvec=None, maxiter_momentum=0, maxiter_pressure=0, gamma_d_phi=None, nez=None, lz=None, order=1,
+                 approx_inv_variant=0):
         lib = nat.load_library()
         t = nat.require_gpu()
         self.nex, self.ney, self.lx, self.ly, self.periodic = int(nex), int(ney), float(lx), float(ly), bool(periodic)
@@ -56,6 +57,7 @@ class Context:
         sd.ce_format = nat.CE_FMT[ce_format]
         sd.maxiter_momentum = int(maxiter_momentum)
         sd.maxiter_pressure = int(maxiter_pressure)
+        sd.approx_inv_variant = int(approx_inv_variant)
         if gamma_d is not None and len(gamma_d):
             sd.gamma_d, sd.n_gamma_d = arr(gamma_d), len(gamma_d)
         if gamma_p is not None and np.size(gamma_p):

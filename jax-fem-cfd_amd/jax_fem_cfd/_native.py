@@ -40,7 +40,8 @@ class SolverDesc(ctypes.Structure):
                 ("gamma_p", _I64P), ("n_gamma_p", ctypes.c_int64),
                 ("mg_gamma_p", _I64P), ("mg_gamma_p_off", _I64P),
                 ("bconn", _I64P), ("nvec", _I64P), ("n_bedge", ctypes.c_int64),
-                ("gamma_d_phi", _I64P), ("n_gamma_d_phi", ctypes.c_int64)]
+                ("gamma_d_phi", _I64P), ("n_gamma_d_phi", ctypes.c_int64),
+                ("approx_inv_variant", ctypes.c_int32)]
 
 
 _vp = ctypes.c_void_p

@@ -25,7 +25,7 @@ def setup_solver(node_coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, gamma_d_
     if config.mesh_config.periodic:
         raise NotImplementedError("the passive-scalar solver uses non-periodic meshes (as the reference)")
     ctx = build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, rm.create_mesh_2d, config, ns_config=ns,
-                        gamma_d_phi=gamma_d_phi)
+                        gamma_d_phi=gamma_d_phi, passive=True)
     has_dirichlet = gamma_d is not None
     Me = ctx.element_constant("Me")
     supg = bool(ade.phi_SUPG)

@@ -44,7 +44,10 @@ def _mg_pins(num_elem, domain_size, outlet_faces, partial_dirichlet, mesh_func, 
 
 
 def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: Config, ns_config=None,
-                  gamma_d_phi=None):
+                  gamma_d_phi=None, passive=False):
+    """One native context for the configuration.  passive: the pressure preconditioner of the
+    passive-transport solver, whose approx_inv closure keeps D1x and D1L at gamma_p
+    (passive_transport.py:154-162) where standard.py:225-234 zeroes them."""
     mc = config.mesh_config
     sc = ns_config if ns_config is not None else config.solver_config
     ce_format = sc.ce_format or ("f64" if sc.precision == "fp64" else "f32")
@@ -81,7 +84,8 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
                    set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=nlev, precision=sc.precision,
                    ce_format=ce_format, gamma_d=gamma_d,
                    gamma_p=np.atleast_1d(gamma_p) if (sc.set_zeroP or sc.pressure_precond == "approx_inv") else None,
-                   mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, gamma_d_phi=gamma_d_phi)
+                   mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, gamma_d_phi=gamma_d_phi,
+                   approx_inv_variant=1 if passive else 0)
 
 
 def setup_solver(node_coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: Config):

@@ -101,7 +101,7 @@ class Solver:
 
 def setup_solver(coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, num_elem, domain_size,
                  periodic, outlet_faces, solver_tol, set_zeroP, pressure_precond,
-                 final_multigrid_mesh, partial_dirichlet=None, shape_func_ord=1):
+                 final_multigrid_mesh, partial_dirichlet=None, shape_func_ord=1, approx_keep=False):
     """standard.py:191-247: element constants from element 0 and the pressure
     preconditioner closure (jacobi | approx_inv | multigrid | None)."""
     ndim = len(num_elem)
@@ -123,12 +123,14 @@ def setup_solver(coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, nu
         s.L_diag = Ld
 
         def approx_inv(r):
-            # standard.py:228-234 (pinned rows of the two inner products set to 0)
+            # standard.py:228-234: the gamma_p rows of the two inner products are set to 0.
+            # The passive solver's own closure (passive_transport.py:154-162, approx_keep) sets
+            # them to D1x and D1L instead.
             D1x = r / Ld
             D1L = emult.laplacian_element_mult(D1x, Le, nconn, nn) / Ld
-            D1L[gamma_p] = 0
+            D1L[gamma_p] = D1x[gamma_p] if approx_keep else 0
             L2 = emult.laplacian_element_mult(D1L, Le, nconn, nn)
-            L2[gamma_p] = 0
+            L2[gamma_p] = D1L[gamma_p] if approx_keep else 0
             return 3 * D1x - 3 * D1L + L2 / Ld
         s.M_op = approx_inv
     elif pressure_precond == "multigrid":
@@ -202,13 +204,14 @@ def timestep_loop(s, t_final, dt, save_interval, rho, mu, U_d, U0, P0, S_func=No
 # --------------------------------------------------------------------------------------------
 
 def build(num_elem, domain_size, periodic, dirichlet_faces, outlet_faces, solver_tol, set_zeroP,
-          pressure_precond, final_multigrid_mesh, partial_dirichlet=None, shape_func_ord=1):
-    """simulation.setup for the 'standard' solver (simulation.py:185-200)."""
+          pressure_precond, final_multigrid_mesh, partial_dirichlet=None, shape_func_ord=1, passive=False):
+    """simulation.setup for the 'standard' solver (simulation.py:185-200); passive = True builds
+    the pressure preconditioner of the passive-transport solver (passive_transport.py:133-172)."""
     coords, nconn, surf, nn, h, bconn, nvec, gamma_d, U_d, gamma_p, mesh_func = ms.simple_mesh(
         num_elem, domain_size, periodic, dirichlet_faces, outlet_faces, partial_dirichlet, shape_func_ord)
     s = setup_solver(coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, num_elem,
                      domain_size, periodic, outlet_faces, solver_tol, set_zeroP, pressure_precond,
-                     final_multigrid_mesh, partial_dirichlet, shape_func_ord)
+                     final_multigrid_mesh, partial_dirichlet, shape_func_ord, approx_keep=passive)
     return s, surf, h, U_d
 
 

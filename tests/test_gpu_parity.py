@@ -412,6 +412,35 @@ def test_passive_transport_matches_oracle(supg):
     assert _rel(_np(phi), phio) < 1e-10
 
 
+def test_passive_transport_approx_inv_matches_oracle():
+    """The passive solver with approx_inv: its own closure (passive_transport.py:154-162) keeps
+    D1x / D1L on the gamma_p rows, unlike the standard solver's (standard.py:225-234)."""
+    import jax_fem_cfd.setup.simulation as sim
+    n, dt, rho, mu, D = 32, 2e-3, 1.0, 1e-2, 1e-4
+    cfg = sim.Config(mesh="simple", solver="passive_transport",
+                     mesh_config=sim.SimpleMeshConfig([n, n], [1.0, 1.0], 1, False),
+                     solver_config=sim.PassiveTransportConfig(
+                         sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, 1e-10, True, "approx_inv"),
+                         sim.ADEConfig([], False)))
+    compute, timestep, coords, surf, h, nn, nnx, nny, gamma_d, U_d, gamma_d_phi = sim.setup(cfg)
+    print("\n[case] passive transport approx_inv", flush=True)
+    x, y = coords[:, 0], coords[:, 1]
+    U0 = np.concatenate([np.sin(np.pi * x) * np.cos(np.pi * y), -np.cos(np.pi * x) * np.sin(np.pi * y)])
+    U0[gamma_d] = U_d
+    phi0 = _rect_phi(coords)
+    s, osurf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-10, True, "approx_inv", None,
+                                passive=True)
+    U, P, phi = U0, np.zeros(nn), phi0
+    Uo, Po, phio = U0.copy(), np.zeros(nn), phi0.copy()
+    for _ in range(3):
+        F_p, F = compute(U_d, None, 0, 0)
+        U, P, phi, ui, pi, fi = timestep(U, P, phi, dt, rho, mu, D, U_d, None, F_p, F, None)
+        Uo, Po, phio, uio, pio, fio = ns.passive_timestep(s, Uo, Po, phio, dt, rho, mu, D, oUd, None, 0, 0, None,
+                                                          False)
+        assert (ui, pi, fi) == (uio, pio, fio)
+    assert _rel(_np(U), Uo) < 1e-10 and _rel(_np(P), Po) < 1e-10 and _rel(_np(phi), phio) < 1e-10
+
+
 def test_supg_element_calc_api():
     from jax_fem_cfd.core import element_calc as ec
     from jax_fem_cfd.core import shape_functions as sf

@@ -308,3 +308,23 @@ def test_boundary_integral_inflow():
     left = F_p[surf[3]]
     assert np.allclose(left, [-h / 2, -h, -h, -h, -h / 2])
     assert np.isclose(F_p.sum(), -1.0)
+
+
+def test_passive_approx_inv_variant():
+    """The passive solver's approx_inv closure (passive_transport.py:154-162) keeps D1x and D1L on
+    the gamma_p rows: there z = r / L_diag^2 exactly, and away from the pins (where neither L D1x
+    nor L D1L reads a pinned entry) it equals the standard closure (standard.py:225-234), whose
+    pinned rows give z = 3 r / L_diag."""
+    n = 8
+    s0, *_ = ns.build([n, n], [1.0, 1.0], False, [], [], 1e-10, True, "approx_inv", None)
+    s1, *_ = ns.build([n, n], [1.0, 1.0], False, [], [], 1e-10, True, "approx_inv", None, passive=True)
+    r = RNG.standard_normal(s0.nn)
+    z0, z1 = s0.M_op(r), s1.M_op(r)
+    gp = np.atleast_1d(s0.gamma_p)
+    Ld = s0.L_diag
+    assert np.allclose(z1[gp], r[gp] / Ld[gp] ** 2, rtol=1e-14)
+    assert np.allclose(z0[gp], 3 * r[gp] / Ld[gp], rtol=1e-14)
+    nnx = n + 1
+    ii, jj = np.arange(s0.nn) % nnx, np.arange(s0.nn) // nnx
+    far = (ii > 2) | (jj > 2)  # pin is node 0; L D1L reaches two nodes away
+    assert np.array_equal(z0[far], z1[far]) and not np.allclose(z0, z1)
