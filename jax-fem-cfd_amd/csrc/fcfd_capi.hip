@@ -694,6 +694,11 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
 int fcfd_solve_scalar(fcfd_ctx* ctx, const double* U_n, const double* U, const double* phi_n, double dt, double D,
                       const double* phi_d, int32_t supg, double* phi, int32_t* iters) {
     if (!ctx || !U_n || !U || !phi_n || !phi) return FCFD_ERR_ARG;
+    if (ctx->c.dim == 2 && ctx->c.order == 2)
+        return run(ctx, [&] {
+            int it = solve_scalar_q2(ctx->c, U_n, U, phi_n, dt, D, phi_d, supg, phi);
+            if (iters) *iters = it;
+        });
     return run2d(ctx, "fcfd_solve_scalar", [&] {
         int it = solve_scalar(ctx->c, U_n, U, phi_n, dt, D, phi_d, supg, phi);
         if (iters) *iters = it;
@@ -705,6 +710,15 @@ int fcfd_passive_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, c
                           const double* F_p, const double* F, int32_t supg, double* U, double* P, double* phi,
                           int32_t* iters) {
     if (!ctx || !U_n || !P_n || !phi_n || !U || !P || !phi) return FCFD_ERR_ARG;
+    if (ctx->c.dim == 2 && ctx->c.order == 2)
+        return run(ctx, [&] {
+            Ctx& c = ctx->c;
+            int it[3];
+            four_step_q2(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+            it[2] = solve_scalar_q2(c, U_n, U, phi_n, dt, D, phi_d, supg, phi);
+            if (iters)
+                for (int k = 0; k < 3; ++k) iters[k] = it[k];
+        });
     return run2d(ctx, "fcfd_passive_timestep", [&] {
         Ctx& c = ctx->c;
         int it[3];

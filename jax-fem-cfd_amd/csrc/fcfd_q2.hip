@@ -257,11 +257,36 @@ struct MomCoefQ2 {
     }
 };
 
-// rhs = R U_n - G P_n + F (Dirichlet rows U_d), x0 = U_n with Dirichlet values, r0 = rhs - A x0
-template <class Op>
+// -G_c P_n + F_c on the momentum rhs (standard.py:31-33)
+struct MomExtraQ2 {
+    const ElemConstQ2* ec;
+    const double *Pn, *F;
+    __device__ __forceinline__ void operator()(const Grid2D& g, const IncQ2& s, int n, double (&b)[2]) const {
+        b[0] -= q2_const(g, s, ec->Gx, [&](int id) { return Pn[id]; });
+        b[1] -= q2_const(g, s, ec->Gy, [&](int id) { return Pn[id]; });
+        if (F) { b[0] += F[n]; b[1] += F[n + g.nn]; }
+    }
+};
+struct NoExtraQ2 {
+    template <int NC>
+    __device__ __forceinline__ void operator()(const Grid2D&, const IncQ2&, int, double (&)[NC]) const {}
+};
+
+// stored per-element 9 x 9 matrices (row-blocked FP64): the scalar-transport operators
+struct StoredQ2 {
+    Grid2D g;
+    Ce9F64 m;
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[9]) const { m.ld_row(a, e, cf); }
+};
+
+// rhs = R x_n + extra (Dirichlet rows ud), x0 = x_n with Dirichlet values, r0 = rhs - A x0,
+// rhat = p = q = r0; reductions b.b, r0.r0 (the quad4 BInit on quad9 gathers)
+template <int NC, class LOp, class ROp, class Extra>
 struct BInitQ2 {
-    Op lhs, rhs;
-    const double *xn, *ud, *Pn, *F;
+    LOp lhs;
+    ROp rhs;
+    Extra extra;
+    const double *xn, *ud;
     const uint8_t* mask;
     double *x, *r, *rhat, *p, *q;
     __device__ void operator()(int n, double (&acc)[2]) const {
@@ -269,21 +294,19 @@ struct BInitQ2 {
         const int nn = g.nn;
         IncQ2 s;
         node_inc_q2(g, n, s);
-        double b[2], ax[2];
-        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { rhs.row(a, e, cf); },
-                    [&](int c, int id) { return xn[id + c * nn]; }, b);
-        b[0] -= q2_const(g, s, lhs.ec->Gx, [&](int id) { return Pn[id]; });
-        b[1] -= q2_const(g, s, lhs.ec->Gy, [&](int id) { return Pn[id]; });
-        if (F) { b[0] += F[n]; b[1] += F[n + nn]; }
-        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { lhs.row(a, e, cf); },
-                    [&](int c, int id) {
-                        const int i = id + c * nn;
-                        const double xv = xn[i];
-                        return (mask && mask[i]) ? ud[i] : xv;
-                    },
-                    ax);
+        double b[NC], ax[NC];
+        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { rhs.row(a, e, cf); },
+                     [&](int c, int id) { return xn[id + c * nn]; }, b);
+        extra(g, s, n, b);
+        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { lhs.row(a, e, cf); },
+                     [&](int c, int id) {
+                         const int i = id + c * nn;
+                         const double xv = xn[i];
+                         return (mask && mask[i]) ? ud[i] : xv;
+                     },
+                     ax);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < NC; ++c) {
             const int i = n + c * nn;
             const bool m = mask && mask[i];
             const double x0 = m ? ud[i] : xn[i];
@@ -298,7 +321,7 @@ struct BInitQ2 {
     }
 };
 
-template <class Op>
+template <int NC, class Op>
 struct BK1Q2 {
     Op op;
     const uint8_t* mask;
@@ -315,10 +338,10 @@ struct BK1Q2 {
         };
         IncQ2 s;
         node_inc_q2(g, n, s);
-        double y[2];
-        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); }, pv, y);
+        double y[NC];
+        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); }, pv, y);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < NC; ++c) {
             const int i = n + c * nn;
             const double pc = pv(c, n);
             const double yc = (mask && mask[i]) ? pc : y[c];
@@ -329,7 +352,7 @@ struct BK1Q2 {
     }
 };
 
-template <class Op>
+template <int NC, class Op>
 struct BK2Q2 {
     Op op;
     const uint8_t* mask;
@@ -343,10 +366,10 @@ struct BK2Q2 {
         auto svv = [&](int c, int id) { return r[id + c * nn] - alpha * q[id + c * nn]; };
         IncQ2 s;
         node_inc_q2(g, n, s);
-        double y[2];
-        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); }, svv, y);
+        double y[NC];
+        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); }, svv, y);
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < NC; ++c) {
             const int i = n + c * nn;
             const double sc = svv(c, n);
             const double yc = (mask && mask[i]) ? sc : y[c];
@@ -359,6 +382,39 @@ struct BK2Q2 {
     }
 };
 
+// BiCGSTAB iterations after the init kernel (graph-replayed pairs until the device flag says
+// done); returns the reference's iteration count (or -10 / -11)
+template <int NC, class LOp>
+static int bicgstab_q2_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
+                               const uint8_t* mask, int maxit, double* x, int& pred) {
+    const int nn = c.g.nn;
+    const int* act = &c.st->b_active;
+    auto pair = [&]() {
+        for (int o = 0; o < 2; ++o) {
+            const int nw = o ^ 1;
+            launch_node<1>(c, nn, act,
+                           BK1Q2<NC, LOp>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
+                           BK1Fin{c.st});
+            launch_node<3>(c, nn, act, BK2Q2<NC, LOp>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                           BK2Fin{c.st});
+            launch_node<2>(c, nn, act, BK3<NC>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, x, c.m_r, c.st}, BK3Fin{c.st});
+        }
+        FCFD_CHECK_LAUNCH();
+    };
+    int launched = 0;
+    int pairs = pred > 0 ? (pred + 1) / 2 : 1;
+    int h[2] = {0, 0};
+    while (true) {
+        replay(c, graph, key, kptr, pairs, pair);
+        launched += 2 * pairs;
+        read_state_ints(c, &c.st->b_k, 2, h);  // b_k, b_active
+        if (!h[1] || launched >= maxit) break;
+        pairs = 1;
+    }
+    if (h[0] > 0) pred = h[0];
+    return h[0];
+}
+
 template <class S>
 static int solve_momentum_q2_t(Ctx& c, S ce, const double* U_n, const double* P_n, double rho, double mu, double dt,
                                const double* F, double* U_hat) {
@@ -369,40 +425,16 @@ static int solve_momentum_q2_t(Ctx& c, S ce, const double* U_n, const double* P_
     MomCoefQ2<S> rhs{g, c.eq2_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
     const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 2 * nn;
     launch_node<2>(c, nn, nullptr,
-                   BInitQ2<MomCoefQ2<S>>{lhs, rhs, U_n, c.ud_full, P_n, F, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0],
-                                         c.m_q[0]},
+                   BInitQ2<2, MomCoefQ2<S>, MomCoefQ2<S>, MomExtraQ2>{lhs, rhs, MomExtraQ2{c.eq2_dev, P_n, F}, U_n,
+                                                                     c.ud_full, c.dmask, c.m_x, c.m_r, c.m_rhat,
+                                                                     c.m_p[0], c.m_q[0]},
                    BInitFin{c.st, c.tol, maxit});
     FCFD_CHECK_LAUNCH();
-    const int* act = &c.st->b_active;
-    auto pair = [&]() {
-        for (int o = 0; o < 2; ++o) {
-            const int nw = o ^ 1;
-            launch_node<1>(c, nn, act,
-                           BK1Q2<MomCoefQ2<S>>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
-                                               c.m_q[nw], c.st},
-                           BK1Fin{c.st});
-            launch_node<3>(c, nn, act, BK2Q2<MomCoefQ2<S>>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
-                           BK2Fin{c.st});
-            launch_node<2>(c, nn, act, BK3<2>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
-                           BK3Fin{c.st});
-        }
-        FCFD_CHECK_LAUNCH();
-    };
     const double key[4] = {rho, mu, dt, 0.0};
-    int launched = 0;
-    int pairs = c.pred_mom > 0 ? (c.pred_mom + 1) / 2 : 1;
-    int h[2] = {0, 0};
-    while (true) {
-        replay(c, c.g_mom, key, ce.c, pairs, pair);
-        launched += 2 * pairs;
-        read_state_ints(c, &c.st->b_k, 2, h);  // b_k, b_active
-        if (!h[1] || launched >= maxit) break;
-        pairs = 1;
-    }
-    if (h[0] > 0) c.pred_mom = h[0];
+    const int it = bicgstab_q2_iterate<2>(c, c.g_mom, key, ce.c, lhs, c.dmask, maxit, c.m_x, c.pred_mom);
     if (U_hat != c.m_x)
         FCFD_HIP(hipMemcpyAsync(U_hat, c.m_x, sizeof(double) * 2 * nn, hipMemcpyDeviceToDevice, c.stream));
-    return h[0];
+    return it;
 }
 
 int solve_momentum_q2(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt, const double* F,
@@ -491,22 +523,25 @@ struct CgK1Q2 {
     }
 };
 
+// keep: the passive solver's closure, which keeps D1x and D1L at the pins instead of zeroing
+// them (passive_transport.py:154-162 against standard.py:225-234)
 struct ApproxInvQ2a {
-    Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
+    Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l; int keep;
     __device__ void operator()(int n, double (&)[1]) const {
         IncQ2 s;
         node_inc_q2(g, n, s);
         const double v = q2_const(g, s, Le, [&](int id) { return r[id] / Ld[id]; }) / Ld[n];
-        d1l[n] = (pin && pin[n]) ? 0.0 : v;
+        d1l[n] = (pin && pin[n]) ? (keep ? r[n] / Ld[n] : 0.0) : v;
     }
 };
 struct ApproxInvQ2b {
     Grid2D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l; double* z;
+    int keep;
     __device__ void operator()(int n, double (&)[1]) const {
         IncQ2 s;
         node_inc_q2(g, n, s);
         double l2 = q2_const(g, s, Le, [&](int id) { return d1l[id]; });
-        if (pin && pin[n]) l2 = 0.0;
+        if (pin && pin[n]) l2 = keep ? d1l[n] : 0.0;
         const double d1x = r[n] / Ld[n];
         z[n] = (3.0 * d1x - 3.0 * d1l[n]) + l2 / Ld[n];
     }
@@ -695,10 +730,11 @@ static const double* precondition_q2(Ctx& c, const int* active) {
             launch_node<0>(c, nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y}, NoFin{});
             return c.lv[0].y;
         case FCFD_PC_APPROX_INV:
-            launch_node<0>(c, nn, active, ApproxInvQ2a{c.g, c.eq2_dev->Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x},
+            launch_node<0>(c, nn, active,
+                           ApproxInvQ2a{c.g, c.eq2_dev->Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.approx_keep},
                            NoFin{});
             launch_node<0>(c, nn, active,
-                           ApproxInvQ2b{c.g, c.eq2_dev->Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y},
+                           ApproxInvQ2b{c.g, c.eq2_dev->Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y, c.approx_keep},
                            NoFin{});
             return c.lv[0].y;
         case FCFD_PC_MULTIGRID:
@@ -957,6 +993,22 @@ void build_ctx_q2(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
         c.ud_full = dalloc<double>(c, 2 * (size_t)nn);
         FCFD_HIP(hipMemset(c.ud_full, 0, sizeof(double) * 2 * nn));
     }
+    // passive-scalar Dirichlet nodes
+    if (s->gamma_d_phi && s->n_gamma_d_phi > 0) {
+        std::vector<int64_t> gp(s->gamma_d_phi, s->gamma_d_phi + s->n_gamma_d_phi);
+        std::vector<uint8_t> mask(nn, 0);
+        for (auto v : gp) {
+            FCFD_REQUIRE(v >= 0 && v < nn, "gamma_d_phi out of range");
+            mask[v] = 1;
+        }
+        c.n_gamma_d_phi = (int)gp.size();
+        c.gamma_d_phi_dev = dalloc<int64_t>(c, gp.size());
+        upload_q2(gp, c.gamma_d_phi_dev);
+        c.phi_mask = dalloc<uint8_t>(c, nn);
+        upload_q2(mask, c.phi_mask);
+        c.phi_ud = dalloc<double>(c, nn);
+        FCFD_HIP(hipMemset(c.phi_ud, 0, sizeof(double) * nn));
+    }
     // Dirichlet edges (3 nodes each) -> per-node gather lists in bconn order; lin3 weights from
     // the edge's node coordinates (shape_functions.py:89-106)
     if (s->bconn && s->nvec && s->n_bedge > 0 && !gd.empty()) {
@@ -1039,6 +1091,127 @@ void build_ctx_q2(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
     FCFD_HIP(hipMemset(c.st, 0, sizeof(DevState)));
     FCFD_HIP(hipHostMalloc((void**)&c.h_pinned, 64, hipHostMallocDefault));
     FCFD_HIP(hipDeviceSynchronize());
+}
+
+// ------------------------------------------------------------------------------ passive scalar
+// Per element, from U_n and U (passive_transport.py:64-97, element_calc.py:188-229):
+//   Mt = Me + tau Ce(U_n)^T, K = D Le + tau Ct(U_n),
+//   A_e = Mt / dt + (Ce(U) + K) / 2,  B_e = Mt / dt - (Ce(U_n) + K) / 2   (Ce with beta = 1)
+// tau from calculate_tau with r = 2 (the Frobenius norms of Ce(U) and Ct(U_n), mean |U_n,q|^2 over
+// the 9 Gauss points, D + 1e-6); tau = 0 without SUPG.  Row by row, so that no 9 x 9 matrix is
+// held in registers: pass 1 forms the norms, pass 2 the rows of A_e and B_e.
+__global__ __launch_bounds__(BLOCK) void k_ade_elem_q2(Grid2D g, const ElemConstQ2* __restrict__ ec,
+                                                      const double* __restrict__ Un, const double* __restrict__ U,
+                                                      double dt, double D, int supg, double* __restrict__ A,
+                                                      double* __restrict__ B) {
+    const int e = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (e >= g.ne) return;
+    const int je = e / g.nex, ie = e - je * g.nex;
+    double un[9], vn[9], u[9], v[9];
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        const int id = q9_node(g, ie, je, b);
+        un[b] = Un[id]; vn[b] = Un[id + g.nn];
+        u[b] = U[id]; v[b] = U[id + g.nn];
+    }
+    // quadrature values: velocities and divergences of U_n and U
+    double uq[9], vq[9], dq[9], unq[9], vnq[9], dnq[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const double Nb = ec->N[q * 9 + b], dx = ec->dNx[q * 9 + b], dy = ec->dNy[q * 9 + b];
+            a0 += Nb * u[b]; a1 += Nb * v[b]; a2 += dx * u[b] + dy * v[b];
+            b0 += Nb * un[b]; b1 += Nb * vn[b]; b2 += dx * un[b] + dy * vn[b];
+        }
+        uq[q] = a0; vq[q] = a1; dq[q] = a2;
+        unq[q] = b0; vnq[q] = b1; dnq[q] = b2;
+    }
+    // Ce(U)[a][b], Ce(U_n)[a][b] and Ct(U_n)[a][b] (beta = 1)
+    auto ce_ab = [&](const double* wu, const double* wv, const double* wd, int a, int b) {
+        double t = 0.0;
+        for (int q = 0; q < 9; ++q) {
+            const double adv = wu[q] * ec->dNx[q * 9 + b] + wv[q] * ec->dNy[q * 9 + b] + wd[q] * ec->N[q * 9 + b];
+            t += ec->wq[q] * ec->N[q * 9 + a] * adv;
+        }
+        return t;
+    };
+    auto ct_ab = [&](int a, int b) {
+        double t = 0.0;
+        for (int q = 0; q < 9; ++q) {
+            const double ga = unq[q] * ec->dNx[q * 9 + a] + vnq[q] * ec->dNy[q * 9 + a];
+            const double gb = unq[q] * ec->dNx[q * 9 + b] + vnq[q] * ec->dNy[q * 9 + b];
+            t += ec->wq[q] * ga * gb;
+        }
+        return t;
+    };
+    double tau = 0.0;
+    if (supg) {
+        double nce = 0.0, nct = 0.0, u2 = 0.0;
+        for (int a = 0; a < 9; ++a)
+            for (int b = 0; b < 9; ++b) {
+                const double x = ce_ab(uq, vq, dq, a, b), y = ct_ab(a, b);
+                nce += x * x;
+                nct += y * y;
+            }
+        for (int q = 0; q < 9; ++q) u2 += unq[q] * unq[q] + vnq[q] * vnq[q];
+        u2 /= 9.0;
+        nce = sqrt(nce);
+        nct = sqrt(nct);
+        const double Dp = D + 1e-6;
+        const double Re = (u2 * nce) / (Dp * nct);
+        const double t1 = nce / nct, t2 = dt / 2, t3 = t1 * Re;
+        tau = pow(1.0 / (t1 * t1) + 1.0 / (t2 * t2) + 1.0 / (t3 * t3), -0.5);
+    }
+    const size_t ne = g.ne;
+    for (int a = 0; a < 9; ++a)
+        for (int b = 0; b < 9; ++b) {
+            const int k = a * 9 + b;
+            const double cu = ce_ab(uq, vq, dq, a, b), cn = ce_ab(unq, vnq, dnq, a, b);
+            double Mt = ec->Me[k], K = D * ec->Le[k];
+            if (supg) {
+                Mt = Mt + tau * ce_ab(unq, vnq, dnq, b, a);
+                K = K + tau * ct_ab(a, b);
+            }
+            A[q9_idx(k, e, (int)ne)] = (1.0 / dt) * Mt + 0.5 * (cu + K);
+            B[q9_idx(k, e, (int)ne)] = (1.0 / dt) * Mt - 0.5 * (cn + K);
+        }
+}
+
+__global__ void k_scatter_phi_q2(int n, const int64_t* __restrict__ gd, const double* __restrict__ v, double* full) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) full[gd[k]] = v ? v[k] : 0.0;
+}
+
+int solve_scalar_q2(Ctx& c, const double* U_n, const double* U, const double* phi_n, double dt, double D,
+                    const double* phi_d, int supg, double* phi) {
+    const size_t ne = c.g.ne;
+    const int nn = c.g.nn;
+    if (!c.ade_A) {
+        c.ade_A = dalloc<double>(c, 81 * ne);
+        c.ade_B = dalloc<double>(c, 81 * ne);
+        c.s_x = dalloc<double>(c, nn);
+    }
+    k_ade_elem_q2<<<blocks_for(c.g.ne), BLOCK, 0, c.stream>>>(c.g, c.eq2_dev, U_n, U, dt, D, supg, c.ade_A, c.ade_B);
+    FCFD_CHECK_LAUNCH();
+    if (c.n_gamma_d_phi > 0) {
+        k_scatter_phi_q2<<<(c.n_gamma_d_phi + 255) / 256, 256, 0, c.stream>>>(c.n_gamma_d_phi, c.gamma_d_phi_dev,
+                                                                               phi_d, c.phi_ud);
+        FCFD_CHECK_LAUNCH();
+    }
+    StoredQ2 lhs{c.g, Ce9F64{c.ade_A, c.g.ne}}, rhs{c.g, Ce9F64{c.ade_B, c.g.ne}};
+    const int maxit = 10 * nn;
+    launch_node<2>(c, nn, nullptr,
+                   BInitQ2<1, StoredQ2, StoredQ2, NoExtraQ2>{lhs, rhs, NoExtraQ2{}, phi_n, c.phi_ud, c.phi_mask, c.s_x,
+                                                            c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                   BInitFin{c.st, c.tol, maxit});
+    FCFD_CHECK_LAUNCH();
+    const double key[4] = {0.0, 0.0, 0.0, 0.0};
+    const int it = bicgstab_q2_iterate<1>(c, c.g_phi, key, c.ade_A, lhs, c.phi_mask, maxit, c.s_x, c.pred_phi);
+    if (phi != c.s_x)
+        FCFD_HIP(hipMemcpyAsync(phi, c.s_x, sizeof(double) * nn, hipMemcpyDeviceToDevice, c.stream));
+    return it;
 }
 
 // ------------------------------------------------------------------------------ the step

@@ -24,7 +24,9 @@ def setup_solver(node_coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, gamma_d_
     ade = config.solver_config.ADE_config
     if config.mesh_config.periodic:
         raise NotImplementedError("the passive-scalar solver uses non-periodic meshes (as the reference)")
-    ctx = build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, rm.create_mesh_2d, config, ns_config=ns,
+    # the mesh function follows the element order (passive_transport.py:107-116)
+    mesh_func = rm.create_mesh_2d_quad9 if config.mesh_config.shape_func_ord == 2 else rm.create_mesh_2d
+    ctx = build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config, ns_config=ns,
                         gamma_d_phi=gamma_d_phi, passive=True)
     has_dirichlet = gamma_d is not None
     Me = ctx.element_constant("Me")
@@ -36,9 +38,12 @@ def setup_solver(node_coords, nconn, bconn, nvec, gamma_d, gamma_p, nn, gamma_d_
             F_p = ctx.boundary_integral(U_d)
         if S_func is not None:
             S = (nat.as_dev(S_func(t_n)) + nat.as_dev(S_func(t))) / 2
-            F = ctx._empty(2 * nn)
-            for d in range(2):
-                ctx.elem_mult_const(Me, S[d * nn:(d + 1) * nn], out=F[d * nn:(d + 1) * nn])
+            if ctx.order == 2:
+                F = ctx.mass_matvec(S)
+            else:
+                F = ctx._empty(2 * nn)
+                for d in range(2):
+                    ctx.elem_mult_const(Me, S[d * nn:(d + 1) * nn], out=F[d * nn:(d + 1) * nn])
         return F_p, F
 
     def timestep_loaded(U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, F_p, F, phi_S_func):

@@ -52,8 +52,6 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
     sc = ns_config if ns_config is not None else config.solver_config
     ce_format = sc.ce_format or ("f64" if sc.precision == "fp64" else "f32")
     if mc.shape_func_ord == 2:
-        if gamma_d_phi is not None:
-            raise NotImplementedError("the quad9 GPU path has no passive scalar")
         nex, ney = mc.num_elem
         nlev = _mg_levels(mc.num_elem, sc.final_multigrid_mesh) if sc.pressure_precond == "multigrid" else 1
         mg_pins = None
@@ -63,7 +61,8 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
                        set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=nlev, precision=sc.precision,
                        ce_format=ce_format, gamma_d=gamma_d,
                        gamma_p=np.atleast_1d(gamma_p) if (sc.set_zeroP or sc.pressure_precond == "approx_inv")
-                       else None, mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, order=2)
+                       else None, mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, order=2, gamma_d_phi=gamma_d_phi,
+                       approx_inv_variant=1 if passive else 0)
     if mc.shape_func_ord != 1:
         raise NotImplementedError("the GPU timestep is built for quad4, quad9 and hex8 meshes")
     if len(mc.num_elem) == 3:

@@ -180,3 +180,49 @@ def test_quad9_taylor_green_multigrid_matches_oracle():
         U, P, oui, opi = ns.four_step_timestep(s, U, P, dt, rho, mu, None, 0, 0)
         assert (ui, pi) == (oui, opi)
     assert _rel(_np(Ug), U) < 1e-10 and _rel(_np(Pg), P) < 1e-10
+
+
+# ----------------------------------------------------------------------------------------- passive scalar
+def _passive_setup(n, precond, supg, phi_faces=(), final=None):
+    import jax_fem_cfd.setup.simulation as sim
+    cfg = sim.Config(mesh="simple", solver="passive_transport",
+                     mesh_config=sim.SimpleMeshConfig(list(n), [1.0, 1.0], 2, False),
+                     solver_config=sim.PassiveTransportConfig(
+                         sim.StandardSolverConfig([0, 1, 2, 3], [], [], False, 1e-10, True, precond, final),
+                         sim.ADEConfig(list(phi_faces), supg)))
+    compute, timestep, coords, surf, h, nn, nnx, nny, gamma_d, U_d, gamma_d_phi = sim.setup(cfg)
+    s, osurf, _, oUd = ns.build(list(n), [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-10, True, precond, final,
+                                shape_func_ord=2, passive=True)
+    assert nn == s.nn and np.allclose(coords, s.coords)
+    return compute, timestep, coords, gamma_d, U_d, gamma_d_phi, s, oUd
+
+
+@pytest.mark.parametrize("precond,supg,phi_faces", [("jacobi", False, ()), ("jacobi", True, ()),
+                                                    ("multigrid", True, (3,)), ("approx_inv", False, (3,))])
+def test_quad9_passive_transport_matches_oracle(precond, supg, phi_faces):
+    """Passive scalar on quad9 (passive_transport.py:64-97, 114-116): swirl in a cavity, scalar
+    rectangle, optional phi = 1 on the left wall; U, P, phi against the oracle with equal
+    iteration counts.  approx_inv uses the passive solver's closure (:154-162)."""
+    n, dt, rho, mu, D = (8, 8), 2e-3, 1.0, 1e-2, 1e-4
+    final = [2, 2] if precond == "multigrid" else None
+    print(f"\n[case] quad9 passive {n} precond={precond} supg={supg} phi_faces={phi_faces}", flush=True)
+    compute, timestep, coords, gamma_d, U_d, gamma_d_phi, s, oUd = _passive_setup(n, precond, supg, phi_faces,
+                                                                                  final)
+    x, y = coords[:, 0], coords[:, 1]
+    U0 = np.concatenate([np.sin(np.pi * x) * np.cos(np.pi * y), -np.cos(np.pi * x) * np.sin(np.pi * y)])
+    U0[gamma_d] = U_d
+    phi0 = ((x >= 0.3) & (x <= 0.7) & (y >= 0.2) & (y <= 0.6)).astype(float)
+    phi_d = None if gamma_d_phi is None else np.ones(len(gamma_d_phi))
+    U, P, phi = U0, np.zeros(s.nn), phi0
+    Uo, Po, phio = U0.copy(), np.zeros(s.nn), phi0.copy()
+    for k in range(3):
+        F_p, F = compute(U_d, None, 0, 0)
+        U, P, phi, ui, pi, fi = timestep(U, P, phi, dt, rho, mu, D, U_d, phi_d, F_p, F, None)
+        Uo, Po, phio, uio, pio, fio = ns.passive_timestep(s, Uo, Po, phio, dt, rho, mu, D, oUd, phi_d, 0, 0,
+                                                          gamma_d_phi, supg)
+        ephi = _rel(_np(phi), phio)
+        print(f"  step {k}: iters {ui},{pi},{fi} oracle {uio},{pio},{fio}  rel phi {ephi:.2e}", flush=True)
+        assert (ui, pi, fi) == (uio, pio, fio)
+    assert _rel(_np(U), Uo) < 1e-10 and _rel(_np(P), Po) < 1e-10 and _rel(_np(phi), phio) < 1e-10
+    if gamma_d_phi is not None:
+        assert np.array_equal(_np(phi)[gamma_d_phi], phi_d)
