@@ -362,12 +362,26 @@ template <int NR, class Fin>
 __global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(int nb, const int* __restrict__ active, Fin fin,
                                                             const double* __restrict__ partials, int pstride) {
     if (active && *active == 0) return;
+    // The partials were written by blocks on every XCD, so each read is a far-memory round trip:
+    // a thread issues FIN_U loads per reduction before it adds any of them (fixed order, so the
+    // sums stay bitwise reproducible), and 4096 partials take one round trip instead of four.
+    constexpr int FIN_U = 8;
     double tot[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        double s = 0.0;
-        for (int k = threadIdx.x; k < nb; k += FIN_BLOCK) s += partials[r * pstride + k];
-        tot[r] = s;
+    for (int r = 0; r < NR; ++r) tot[r] = 0.0;
+    for (int k0 = threadIdx.x; k0 < nb; k0 += FIN_BLOCK * FIN_U) {
+        double v[NR][FIN_U];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int u = 0; u < FIN_U; ++u) {
+                const int k = k0 + u * FIN_BLOCK;
+                v[r][u] = k < nb ? partials[r * pstride + k] : 0.0;
+            }
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int u = 0; u < FIN_U; ++u) tot[r] += v[r][u];
     }
     block_sum<NR>(tot);
     if (threadIdx.x == 0) fin(tot);
