@@ -4,8 +4,9 @@
 //   Ce(U_n, beta = 1/2) per element (element_calc.py:165-185, 8 x 8),
 //   BiCGSTAB on (rho/dt) M_c + 1/2 (rho Ce + mu Le) for the 3 velocity components with Dirichlet
 //   rows -> identity (standard.py:14-43), U* = U_hat + (dt/rho) G P_n / M, PCG on the pressure
-//   Laplacian (standard.py:85-125) preconditioned by jacobi / approx_inv / none -- the reference
-//   has no 3D multigrid (simulation.py:75-76) -- and U = U* - (dt/rho) G P / M, U[gamma_d] = U_d.
+//   Laplacian (standard.py:85-125) preconditioned by jacobi / approx_inv / none or the box-grid
+//   V-cycle of fcfd_mg3.hip (the reference has no 3D multigrid, simulation.py:75-76), and
+//   U = U* - (dt/rho) G P / M, U[gamma_d] = U_d.
 // The reference's consistent_mass_mult fails for ndim = 3 (element_mult.py:26 reads an undefined
 // y_result); the mass operator here is the per-component product of its 2D form.
 //
@@ -89,11 +90,12 @@ void hex8_constants(double dx, double dy, double dz, ElemConst3& ec) {
             ec.G[0][a * 8 + b] = g0;
             ec.G[1][a * 8 + b] = g1;
             ec.G[2][a * 8 + b] = g2;
+            ec.Leoff[a * 8 + b] = a == b ? 0.0 : l;
         }
 }
 
 // out[n] = sum over the incident elements of per_corner[the node's corner], in element order
-static void node_sums3(const Grid3D& g, const double (&per_corner)[8], std::vector<double>& out) {
+void node_sums3(const Grid3D& g, const double (&per_corner)[8], std::vector<double>& out) {
     out.assign(g.nn, 0.0);
     for (int ke = 0; ke < g.nez; ++ke)
         for (int je = 0; je < g.ney; ++je)
@@ -130,8 +132,9 @@ static void face_weights(const double (&P)[4][2], double (&W)[16]) {
 void build_ctx3(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
     FCFD_REQUIRE(m->nex >= 1 && m->ney >= 1 && m->nez >= 1, "bad element counts");
     FCFD_REQUIRE(!m->periodic, "3D periodic meshes are not supported (the reference rejects them)");
-    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV,
-                 "3D pressure preconditioner must be none, jacobi or approx_inv (the reference has no 3D multigrid)");
+    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV ||
+                     c.precond == FCFD_PC_MULTIGRID,
+                 "3D pressure preconditioner must be none, jacobi, approx_inv or multigrid");
     FCFD_REQUIRE(c.ce_format == FCFD_CE_F64 || c.ce_format == FCFD_CE_F32, "3D Ce formats: FP64 or FP32");
     FCFD_REQUIRE((int64_t)(m->nex + 1) * (m->ney + 1) * (m->nez + 1) * 3 < (int64_t)1 << 31, "grid too large");
     c.dim = 3;
@@ -174,6 +177,7 @@ void build_ctx3(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
         L.x = dalloc<double>(c, nn);
         L.y = dalloc<double>(c, nn);
     }
+    if (c.precond == FCFD_PC_MULTIGRID) build_mg3(c, s);
     // pressure pins (outlets, else node 0)
     if (c.set_zeroP || c.precond == FCFD_PC_APPROX_INV) {
         std::vector<uint8_t> pin(nn, 0);
@@ -769,6 +773,7 @@ static const double* precondition3(Ctx& c, const int* active) {
         case FCFD_PC_JACOBI:
             launch_node<0>(c, nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y}, NoFin{});
             return c.lv[0].y;
+        case FCFD_PC_MULTIGRID: return vcycle3(c, c.c_r, active);
         case FCFD_PC_APPROX_INV:
             launch_node<0>(c, nn, active, ApproxInv3a{c.g3, c.ec3_dev->Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x},
                            NoFin{});
@@ -789,8 +794,9 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
     launch_node<3>(c, nn, nullptr, CgInit3{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
                    CgInitFin{c.st, tol, maxit, nn, 0});
     const int* act = &c.st->c_active;
+    const int neu = c.precond == FCFD_PC_MULTIGRID && !c.set_zeroP;
     const double* y0 = precondition3(c, act);
-    launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, y0, c.c_z[0], c.c_p[0], c.st, 0}, CgGamma0Fin{c.st});
+    launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st});
     FCFD_CHECK_LAUNCH();
     const double* Ld = c.lv[0].Ld;
     auto pair = [&]() {
@@ -798,7 +804,12 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
             const int nw = o ^ 1;
             launch_stencil3<1>(c, nn, act, CgK1S3{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
                                CgK1Fin{c.st});
-            if (c.precond == FCFD_PC_APPROX_INV) {
+            if (c.precond == FCFD_PC_MULTIGRID) {
+                launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
+                const double* y = precondition3(c, act);
+                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, 0},
+                               CgK3Fin{c.st, 0});
+            } else if (c.precond == FCFD_PC_APPROX_INV) {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
                 launch_stencil3<0>(c, nn, act, ApproxInv3aS{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x}, NoFin{});
                 launch_stencil3<2>(c, nn, act, ApproxInv3bK3S{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x, c.c_z[nw]},
@@ -826,6 +837,12 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
     if (converged) *converged = !h[1] && h[0] <= maxit;
     if (h[0] > 0) c.pred_p = h[0];
     return h[0];
+}
+
+int pressure_cg3(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged) {
+    const int it = pcg3(c, b, nullptr, x, 0, tol, maxit, converged);
+    FCFD_HIP(hipMemcpyAsync(x, c.c_x, sizeof(double) * c.g3.nn, hipMemcpyDeviceToDevice, c.stream));
+    return it;
 }
 
 int solve_pressure3(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,

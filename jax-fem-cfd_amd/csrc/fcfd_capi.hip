@@ -622,12 +622,21 @@ int fcfd_elem_mult_stored(fcfd_ctx* ctx, const double* K0_16, const double* A1, 
 int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z) {
     if (!ctx || !r || !z) return FCFD_ERR_ARG;
     if (ctx->c.dim == 2 && ctx->c.order == 2) return run(ctx, [&] { vcycle_q2_api(ctx->c, r, z); });
+    if (ctx->c.dim == 3) return run(ctx, [&] { vcycle3_api(ctx->c, r, z); });
     return run2d(ctx, "fcfd_vcycle", [&] { vcycle(ctx->c, r, z); });
 }
 
 int fcfd_pressure_cg(fcfd_ctx* ctx, const double* b, double* x, double tol, int32_t maxiter, int32_t* iters,
                      int32_t* converged) {
     if (!ctx || !b || !x) return FCFD_ERR_ARG;
+    if (ctx->c.dim == 3)
+        return run(ctx, [&] {
+            bool conv = false;
+            int it = pressure_cg3(ctx->c, b, x, tol, maxiter > 0 ? maxiter : 1000, &conv);
+            FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
+            if (iters) *iters = it;
+            if (converged) *converged = conv;
+        });
     return run2d(ctx, "fcfd_pressure_cg", [&] {
         bool conv = false;
         int it = pressure_cg(ctx->c, b, x, tol, maxiter > 0 ? maxiter : 1000, &conv);

@@ -30,6 +30,7 @@ inline Grid3D make_grid3(int nex, int ney, int nez) {
 struct ElemConst3 {
     double N[64], dN[3][64], wq[8];
     double Me[64], Le[64], G[3][64];
+    double Leoff[64];  // Le with its diagonal zeroed (the Jacobi smoother's off-diagonal part)
 };
 
 // Local corner numbering: corner b sits at (hbx, hby, hbz) in {0,1}^3 of its element.

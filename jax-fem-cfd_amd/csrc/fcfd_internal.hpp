@@ -40,6 +40,8 @@ enum { MG_R = 0, MG_X = 1, MG_F = 2, MG_C = 3, MG_P = 4, MG_Y = 5 };
 
 struct Level {
     Grid2D g;
+    Grid3D g3;                          // 3D contexts: this level's box grid
+    ElemConst3* ec3_dev = nullptr;      // 3D contexts: this level's hex8 constants
     ElemConst ec;
     ElemConst* ec_dev = nullptr;
     ElemConstQ2* eq2_dev = nullptr;  // quad9 contexts: this level's 9 x 9 constants
@@ -268,6 +270,12 @@ void projection_final(Ctx& c, const double* U_star, const double* P, double rho,
 // 3D (hex8) path (fcfd_3d.hip)
 void hex8_constants(double dx, double dy, double dz, ElemConst3& ec);
 void build_ctx3(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s);
+void node_sums3(const Grid3D& g, const double (&per_corner)[8], std::vector<double>& out);
+// 3D multigrid (fcfd_mg3.hip)
+void build_mg3(Ctx& c, const fcfd_solver_desc* s);
+const double* vcycle3(Ctx& c, const double* r0, const int* active);
+void vcycle3_api(Ctx& c, const double* r, double* z);
+int pressure_cg3(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged);
 void four_step3(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
                 const double* F_p, const double* F, double* U, double* P, int* it);
 int solve_momentum3(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt, const double* F,

@@ -1,0 +1,365 @@
+// Geometric multigrid V-cycle on Q1 hex8 box grids (FP64), the 3D pressure preconditioner.
+//
+// The reference stops at 2D (simulation.py:75-76: "3D multigrid not yet implemented").  This is
+// the V-cycle of multigrid.py:231-330 carried over to box grids unchanged in structure:
+//   - the x ratio sets the level count and all three directions halve in lockstep (:12, :80);
+//   - each level rediscretises Le on its own element (in 3D Le scales with h);
+//   - down: x = 2/3 r / L_diag (one Jacobi from zero), f = r - L x, restriction R = P^T;
+//   - coarse: CG to tol with the pinned rows zeroed (pinned form) or one Jacobi from zero
+//     (Neumann form);
+//   - up: x += P y, one damped Jacobi (omega = 2/3) against the level's residual;
+//   - the Neumann form mean-projects as the 2D V-cycle does, with the means applied lazily
+//     through the same device-state slots.
+// P is trilinear interpolation applied along x, then y, then z; R = P^T is applied in the same
+// order (c[i] = f[2i] + 0.5 (f[2i-1] + f[2i+1]) per direction), so both are bitwise the
+// separable transfers of the oracle.
+#include "fcfd_internal.hpp"
+#include "fcfd_hex.hpp"
+#include "fcfd_cg.hpp"
+#include "fcfd_mgnodes.hpp"
+
+namespace fcfd {
+
+// ------------------------------------------------------------------------------ down leg
+// f = r - L x: pinned (x = 2/3 r / L_diag with the pins 0, pinned rows of L x 0) or Neumann (x
+// and r mean-projected)
+struct DownRes3 {
+    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* x; double* f;
+    const DevState* st; int l; int neumann;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        Nbr3 s;
+        node_nbr3(g, n, s);
+        double X[27];
+        double fv;
+        if (neumann) {
+            const double mx = st->mg[l][MG_X], mr = st->mg[l][MG_R];
+            load27(s, [&](int k) { return x[k] - mx; }, X);
+            fv = (r[n] - mr) - ebe3(s, Le, X);
+        } else {
+            load27(s, [&](int k) {
+                const double v = (2.0 / 3.0 * r[k]) / Ld[k];
+                return (pin && pin[k]) ? 0.0 : v;
+            }, X);
+            x[n] = X[S3C];
+            const bool pn = pin && pin[n];
+            const double rh = pn ? 0.0 : r[n];
+            const double lx = pn ? 0.0 : ebe3(s, Le, X);
+            fv = rh - lx;
+        }
+        f[n] = fv;
+        acc[0] += fv;
+    }
+};
+
+// coarse[I,J,K] = (R (f - mean f))[I,J,K]: along x, then y, then z; missing fine nodes are 0
+struct Restrict3 {
+    Grid3D gf, gc; const double* f; double* rc; const DevState* st; int l; int neumann;
+    __device__ void operator()(int nc, double (&acc)[1]) const {
+        const int K = nc / gc.nxy, rem = nc - K * gc.nxy;
+        const int J = rem / gc.nnx, I = rem - J * gc.nnx;
+        const double mf = neumann ? st->mg[l][MG_F] : 0.0;
+        auto val = [&](int di, int dj, int dk) -> double {
+            const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+            const bool ok = i >= 0 && j >= 0 && k >= 0 && i < gf.nnx && j < gf.nny && k < gf.nnz;
+            const int id = ok ? k * gf.nxy + j * gf.nnx + i : 0;
+            const double v = f[id] - mf;
+            return ok ? v : 0.0;
+        };
+        double Y[3];
+#pragma unroll
+        for (int dk = -1; dk <= 1; ++dk) {
+            double Xr[3];
+#pragma unroll
+            for (int dj = -1; dj <= 1; ++dj) Xr[dj + 1] = val(0, dj, dk) + 0.5 * (val(-1, dj, dk) + val(1, dj, dk));
+            Y[dk + 1] = Xr[1] + 0.5 * (Xr[0] + Xr[2]);
+        }
+        const double v = Y[1] + 0.5 * (Y[0] + Y[2]);
+        rc[nc] = v;
+        acc[0] += v;
+    }
+};
+
+// ------------------------------------------------------------------------------ up leg
+// trilinear interpolation of (yc - my) at fine node (i, j, k): x, then y, then z; at even fine
+// indices both coarse neighbours coincide and 0.5 a + 0.5 a = a exactly
+__device__ __forceinline__ double interp3_at(const Grid3D& gf, const Grid3D& gc, const double* yc, double my,
+                                             int n) {
+    const int k = n / gf.nxy, rem = n - k * gf.nxy;
+    const int j = rem / gf.nnx, i = rem - j * gf.nnx;
+    const int i0 = i >> 1, i1 = (i + 1) >> 1, j0 = j >> 1, j1 = (j + 1) >> 1, k0 = k >> 1, k1 = (k + 1) >> 1;
+    auto at = [&](int a, int b, int c) { return yc[c * gc.nxy + b * gc.nnx + a] - my; };
+    auto row = [&](int b, int c) { return 0.5 * at(i0, b, c) + 0.5 * at(i1, b, c); };
+    auto lay = [&](int c) { return 0.5 * row(j0, c) + 0.5 * row(j1, c); };
+    return 0.5 * lay(k0) + 0.5 * lay(k1);
+}
+
+// the interpolated values at all 27 stencil points of a node from one 3x3x3 patch of coarse
+// values (fine f - 1 .. f + 1 touch coarse floor((f - 1) / 2) + {0, 1, 2}); the pairing depends
+// on the parity of (i, j, k) only, with the operands and order of interp3_at
+__device__ __forceinline__ void interp27(const Grid3D& gc, const double* yc, int i, int j, int k,
+                                         double (&Y)[27]) {
+    const int bx = (i - 1) >> 1, by = (j - 1) >> 1, bz = (k - 1) >> 1;
+    int cx[3], cy[3], cz[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int x = bx + t, y = by + t, z = bz + t;
+        cx[t] = x < 0 ? 0 : (x >= gc.nnx ? gc.nnx - 1 : x);
+        cy[t] = y < 0 ? 0 : (y >= gc.nny ? gc.nny - 1 : y);
+        cz[t] = z < 0 ? 0 : (z >= gc.nnz ? gc.nnz - 1 : z);
+    }
+    const bool ox = i & 1, oy = j & 1, oz = k & 1;
+    // pairs (first, second) of patch indices for fine offsets -1, 0, +1:
+    //   even index: (0,1) (1,1) (1,2)      odd index: (0,0) (0,1) (1,1)
+    double Hy[3][3][3];  // [z patch][y offset][x offset]
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double Hx[3][3];  // [y patch][x offset]
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const double* rowp = yc + (size_t)cz[c] * gc.nxy + (size_t)cy[b] * gc.nnx;
+            const double p0 = rowp[cx[0]], p1 = rowp[cx[1]], p2 = rowp[cx[2]];
+            Hx[b][0] = 0.5 * p0 + 0.5 * (ox ? p0 : p1);
+            Hx[b][1] = 0.5 * (ox ? p0 : p1) + 0.5 * p1;
+            Hx[b][2] = 0.5 * p1 + 0.5 * (ox ? p1 : p2);
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double h0 = Hx[0][a], h1 = Hx[1][a], h2 = Hx[2][a];
+            Hy[c][0][a] = 0.5 * h0 + 0.5 * (oy ? h0 : h1);
+            Hy[c][1][a] = 0.5 * (oy ? h0 : h1) + 0.5 * h1;
+            Hy[c][2][a] = 0.5 * h1 + 0.5 * (oy ? h1 : h2);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double h0 = Hy[0][b][a], h1 = Hy[1][b][a], h2 = Hy[2][b][a];
+            Y[(0 * 3 + b) * 3 + a] = 0.5 * h0 + 0.5 * (oz ? h0 : h1);
+            Y[(1 * 3 + b) * 3 + a] = 0.5 * (oz ? h0 : h1) + 0.5 * h1;
+            Y[(2 * 3 + b) * 3 + a] = 0.5 * h1 + 0.5 * (oz ? h1 : h2);
+        }
+}
+
+// Neumann form: p = P (yc - mean yc), sum -> mean p
+struct Prolong3 {
+    Grid3D gf, gc; const double* yc; double* p; const DevState* st; int l;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const double v = interp3_at(gf, gc, yc, st->mg[l + 1][MG_Y], n);
+        p[n] = v;
+        acc[0] += v;
+    }
+};
+
+// x <- x + P yc; one damped Jacobi (multigrid.py:214-228, :306-326)
+struct Jacobi3 {
+    Grid3D gf, gc; const double* Leoff; const double* Ld; const uint8_t* pin; const double* r; const double* x;
+    const double* p; const double* yc; double* y; const DevState* st; int l; int neumann;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        Nbr3 s;
+        node_nbr3(gf, n, s);
+        double X[27];
+        double rh;
+        if (neumann) {
+            const double mx = st->mg[l][MG_X], mp = st->mg[l][MG_P];
+            load27(s, [&](int k) { return (x[k] - mx) + (p[k] - mp); }, X);
+            const double oup = l == 0 ? 0.0 : st->mg[l][MG_R];
+            rh = r[n] - oup;
+        } else {
+            const int k = n / gf.nxy, rem = n - k * gf.nxy;
+            const int j = rem / gf.nnx, i = rem - j * gf.nnx;
+            double Y[27];
+            interp27(gc, yc, i, j, k, Y);
+            load27(s, [&](int m) { return x[m]; }, X);
+#pragma unroll
+            for (int t = 0; t < 27; ++t) {
+                const int m = s.nb[t];
+                const bool ok = m != n || t == S3C;  // clamped ids of missing neighbours point at n
+                const bool keep = ok && !(pin && pin[m]);
+                X[t] = keep ? X[t] + Y[t] : 0.0;
+            }
+            rh = r[n];
+        }
+        const double off = ebe3(s, Leoff, X);
+        double v = (OMEGA * (rh - off)) / Ld[n] + (1.0 - OMEGA) * X[S3C];
+        if (!neumann && pin && pin[n]) v = 0.0;
+        y[n] = v;
+        acc[0] += v;
+    }
+};
+
+// ------------------------------------------------------------------------------ coarse level
+// Pinned: CG (M = identity) to tol, maxiter 1000, on L with the pinned rows zeroed
+// (multigrid.py:278-287), in one workgroup; the search direction lives in LDS, x / r / Ap in
+// memory (L2-resident).
+__global__ __launch_bounds__(1024) void k_coarse_cg3(Grid3D g, const ElemConst3* __restrict__ ec,
+                                                     const uint8_t* pin, const double* r, double* x, double* cr,
+                                                     double* cAp, DevState* st, double tol, int maxit,
+                                                     const int* active) {
+    if (active && *active == 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* p = reinterpret_cast<double*>(smem_raw);
+    const int nn = g.nn, bs = blockDim.x;
+    double g2[1] = {0.0};
+    for (int n = threadIdx.x; n < nn; n += bs) {
+        const double b = (pin && pin[n]) ? 0.0 : r[n];
+        cr[n] = b; p[n] = b; x[n] = 0.0;
+        g2[0] += b * b;
+    }
+    block_sum<1>(g2);  // barrier: p complete
+    double gamma = g2[0];
+    const double atol2 = fmax(tol * tol * gamma, 0.0);
+    int k = 0;
+    while (gamma > atol2 && k < maxit) {
+        double pap[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += bs) {
+            Nbr3 s;
+            node_nbr3(g, n, s);
+            double X[27];
+            load27(s, [&](int m) { return p[m]; }, X);
+            const double a = (pin && pin[n]) ? 0.0 : ebe3(s, ec->Le, X);
+            cAp[n] = a;
+            pap[0] += X[S3C] * a;
+        }
+        block_sum<1>(pap);  // barrier: every read of p is done
+        const double alpha = gamma / pap[0];
+        double rr[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += bs) {
+            x[n] = x[n] + alpha * p[n];
+            const double rn = cr[n] - alpha * cAp[n];
+            cr[n] = rn;
+            rr[0] += rn * rn;
+        }
+        block_sum<1>(rr);
+        const double beta = rr[0] / gamma;
+        for (int n = threadIdx.x; n < nn; n += bs) p[n] = cr[n] + beta * p[n];
+        __syncthreads();
+        gamma = rr[0];
+        ++k;
+    }
+    for (int n = threadIdx.x; n < nn; n += bs)
+        if (pin && pin[n]) x[n] = 0.0;
+    if (threadIdx.x == 0) st->coarse_k = k;
+}
+
+// ------------------------------------------------------------------------------ set-up
+void build_mg3(Ctx& c, const fcfd_solver_desc* s) {
+    const int nlev = s->mg_levels;
+    FCFD_REQUIRE(nlev >= 1 && nlev <= MAX_LEVELS, "bad multigrid level count");
+    const Grid3D& g0 = c.g3;
+    FCFD_REQUIRE((g0.nex >> (nlev - 1)) << (nlev - 1) == g0.nex && (g0.ney >> (nlev - 1)) << (nlev - 1) == g0.ney &&
+                     (g0.nez >> (nlev - 1)) << (nlev - 1) == g0.nez && (g0.nex >> (nlev - 1)) >= 1 &&
+                     (g0.ney >> (nlev - 1)) >= 1 && (g0.nez >> (nlev - 1)) >= 1,
+                 "3D multigrid: every element count must halve exactly at each level");
+    c.nlev = nlev;
+    c.lv.resize(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        Level& L = c.lv[l];
+        L.g3 = make_grid3(g0.nex >> l, g0.ney >> l, g0.nez >> l);
+        L.g = Grid2D{0, 0, 0, 0, L.g3.nn, L.g3.ne, 0};
+        ElemConst3 ec;
+        hex8_constants(c.lx / L.g3.nex, c.ly / L.g3.ney, c.lz / L.g3.nez, ec);
+        L.ec3_dev = dalloc<ElemConst3>(c, 1);
+        FCFD_HIP(hipMemcpy(L.ec3_dev, &ec, sizeof(ElemConst3), hipMemcpyHostToDevice));
+        double dg[8];
+        for (int a = 0; a < 8; ++a) dg[a] = ec.Le[a * 8 + a];
+        std::vector<double> Ld;
+        node_sums3(L.g3, dg, Ld);
+        if (!L.Ld) L.Ld = dalloc<double>(c, L.g3.nn);
+        FCFD_HIP(hipMemcpy(L.Ld, Ld.data(), sizeof(double) * Ld.size(), hipMemcpyHostToDevice));
+        const int n = L.g3.nn;
+        if (l > 0) L.r = dalloc<double>(c, n);
+        if (!L.x) L.x = dalloc<double>(c, n);
+        if (!L.y) L.y = dalloc<double>(c, n);
+        L.f = dalloc<double>(c, n);
+        if (c.set_zeroP) {
+            std::vector<uint8_t> pin(n, 0);
+            if (s->mg_gamma_p && s->mg_gamma_p_off) {
+                for (int64_t k = s->mg_gamma_p_off[l]; k < s->mg_gamma_p_off[l + 1]; ++k) {
+                    const int64_t v = s->mg_gamma_p[k];
+                    FCFD_REQUIRE(v >= 0 && v < n, "multigrid pin out of range");
+                    pin[v] = 1;
+                }
+            } else {
+                pin[0] = 1;
+            }
+            L.pin = dalloc<uint8_t>(c, n);
+            FCFD_HIP(hipMemcpy(L.pin, pin.data(), n, hipMemcpyHostToDevice));
+        }
+    }
+    const int ncoarse = c.lv[nlev - 1].g3.nn;
+    FCFD_REQUIRE(!c.set_zeroP || (size_t)ncoarse * sizeof(double) <= 64 * 1024,
+                 "3D multigrid: the pinned coarse grid must have at most 8192 nodes");
+    c.cc_r = dalloc<double>(c, ncoarse);
+    c.cc_Ap = dalloc<double>(c, ncoarse);
+}
+
+// ------------------------------------------------------------------------------ V-cycle driver
+// r0: the fine residual (its mean in mg[0][MG_R] for the Neumann form); result in lv[0].y (its
+// mean in mg[0][MG_Y] for the Neumann form, applied by the consumer)
+const double* vcycle3(Ctx& c, const double* r0, const int* active) {
+    const int N = c.nlev;
+    const int neu = !c.set_zeroP;
+    const int ld = N - 1;
+    auto run = [&](int n, const auto& body, const MeanFin& fin) {
+        if (neu) launch_node<1>(c, n, active, body, fin);
+        else launch_node<0>(c, n, active, body, fin);
+    };
+    auto rl = [&](int l) -> const double* { return l == 0 ? r0 : c.lv[l].r; };
+    for (int l = 0; l < ld; ++l) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        const int n = L.g3.nn;
+        if (neu)
+            launch_node<1>(c, n, active, DownX<double, double>{rl(l), L.Ld, L.x, c.st, l},
+                           MeanFin{&c.st->mg[l][MG_X], nullptr, n});
+        run(n, DownRes3{L.g3, L.ec3_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+        run(Lc.g3.nn, Restrict3{L.g3, Lc.g3, L.f, Lc.r, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g3.nn});
+    }
+    {
+        Level& C = c.lv[ld];
+        if (neu) {
+            k_coarse_jacobi<double, double><<<1, 1024, 0, c.stream>>>(C.g3.nn, C.Ld, rl(ld), C.y, c.st, ld, active);
+        } else {
+            const size_t shm = sizeof(double) * (size_t)C.g3.nn;
+            k_coarse_cg3<<<1, 1024, shm, c.stream>>>(C.g3, C.ec3_dev, C.pin, rl(ld), C.y, c.cc_r, c.cc_Ap, c.st,
+                                                     c.tol, 1000, active);
+        }
+        FCFD_CHECK_LAUNCH();
+    }
+    for (int l = ld - 1; l >= 0; --l) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        const int n = L.g3.nn;
+        if (neu)
+            launch_node<1>(c, n, active, Prolong3{L.g3, Lc.g3, Lc.y, L.f, c.st, l},
+                           MeanFin{&c.st->mg[l][MG_P], nullptr, n});
+        run(n, Jacobi3{L.g3, Lc.g3, L.ec3_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, Lc.y, L.y, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
+    }
+    FCFD_CHECK_LAUNCH();
+    return c.lv[0].y;
+}
+
+// z = y0 - mean (Neumann)
+struct ZOut3 {
+    const double* y; double* z; const DevState* st; int neumann;
+    __device__ void operator()(int k, double (&)[1]) const { z[k] = y[k] - (neumann ? st->mg[0][MG_Y] : 0.0); }
+};
+
+void vcycle3_api(Ctx& c, const double* r, double* z) {
+    FCFD_REQUIRE(c.precond == FCFD_PC_MULTIGRID, "context was not created with the multigrid preconditioner");
+    const int nn = c.g3.nn;
+    struct SumBody {
+        const double* r;
+        __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += r[k]; }
+    };
+    launch_node<1>(c, nn, nullptr, SumBody{r}, MeanFin{&c.st->mg[0][MG_R], nullptr, nn});
+    const double* y = vcycle3(c, r, nullptr);
+    launch_node<0>(c, nn, nullptr, ZOut3{y, z, c.st, !c.set_zeroP}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
+}  // namespace fcfd

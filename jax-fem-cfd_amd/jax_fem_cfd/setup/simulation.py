@@ -82,8 +82,8 @@ class Config:
             if len(self.mesh_config.num_elem) == 3:
                 if self.mesh_config.shape_func_ord == 2:
                     raise ValueError("3D quadratic shape functions not yet implemented")
-                if sc.pressure_precond == "multigrid":
-                    raise ValueError("3D multigrid not yet implemented")
+                # 3D multigrid is built here (the box-grid V-cycle); the reference raises
+                # "3D multigrid not yet implemented" (simulation.py:75-76)
                 if self.mesh_config.periodic:
                     raise ValueError("3D periodic boundary conditions not yet implemented")
             if sc.pressure_precond == "multigrid":

@@ -30,7 +30,10 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
   solvers     scipy.sparse.linalg.spsolve solution of the assembled
               system, true-residual bound
   multigrid   R = I^T on non-periodic grids, I 1 = 1, R 1 row weights,
-              V-cycle linearity (Neumann form), dense two-level restatement
+              V-cycle linearity (Neumann form), dense two-level restatement;
+              3D: P3 against kron products of 1D prolongations, R3 = P3^T,
+              trilinear reproduction, per-level Le closed forms, dense
+              two-level restatement, PCG iteration counts
   ns          Taylor-Green analytic decay, discrete divergence reduction,
               Ghia cavity profile (coarse)
 """

@@ -15,7 +15,10 @@ def multigrid_setup(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, s
                     partial_dirichlet=None, shape_func_ord=1):
     """Per level: mesh, Le, L_diag, Le_off = Le - diag(Le), gamma_p (outlet
     nodes or [0]).  multigrid.py:9-88.  Level count uses the x ratio only
-    (:12) and both directions halve in lockstep (:80)."""
+    (:12) and both directions halve in lockstep (:80).  Three element counts
+    give the hex8 hierarchy (see multigrid_setup_3d)."""
+    if len(num_elem) == 3:
+        return multigrid_setup_3d(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, shapefunc)
     nex, ney = num_elem
     nlev = int(np.log2(nex // num_elemf[0])) + 1
     levels = []
@@ -105,6 +108,85 @@ def single_jacobi_iter(x, rhs, Le_off, L_diag, gamma_p, nconn, nn, set_zeroP):
     return x
 
 
+# --------------------------------------------------------------------------------------------
+# 3D (hex8) hierarchy.  The reference stops at 2D (simulation.py:75-76 rejects 3D multigrid); this
+# is the same V-cycle on box grids: the x ratio sets the level count, all three directions halve
+# in lockstep, Le is recomputed from each level's element (it scales with h in 3D), the transfers
+# are trilinear prolongation and its exact transpose (separable: x, then y, then z).
+# --------------------------------------------------------------------------------------------
+
+def multigrid_setup_3d(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, shapefunc):
+    nex, ney, nez = num_elem
+    nlev = int(np.log2(nex // num_elemf[0])) + 1
+    levels = []
+    for _ in range(nlev):
+        coords, nconn, surf, _, _ = mesh_func(nex, ney, nez, *domain_size)
+        wq, _, Nderiv = shapefunc(coords[nconn[0]])
+        nn = coords.shape[0]
+        outlet = [surf[f] for f in (outlet_faces or [])]
+        gamma_p = np.unique(np.concatenate(outlet)).astype(np.int64) if outlet else np.array([0])
+        Le = el.laplacian_element_calc(wq, Nderiv)
+        levels.append(dict(ndim=3, nex=nex, ney=ney, nez=nez, nnx=nex + 1, nny=ney + 1, nnz=nez + 1, nn=nn,
+                           nconn=nconn, Le=Le, Le_off=Le - np.diag(np.diag(Le)),
+                           L_diag=el.laplacian_diag(Le, nconn, nn), gamma_p=gamma_p))
+        nex, ney, nez = nex // 2, ney // 2, nez // 2
+    return levels
+
+
+def _prolong_axis(H, axis):
+    """Linear interpolation along one axis of a node array: 2n - 1 fine nodes from n coarse ones
+    (injection at even fine nodes, 0.5 a + 0.5 b at odd ones)."""
+    n = H.shape[axis]
+    shape = list(H.shape)
+    shape[axis] = 2 * n - 1
+    h = np.zeros(shape)
+    idx = lambda sl: tuple(sl if a == axis else slice(None) for a in range(H.ndim))
+    h[idx(slice(0, None, 2))] = H
+    h[idx(slice(1, None, 2))] = 0.5 * H[idx(slice(0, -1))] + 0.5 * H[idx(slice(1, None))]
+    return h
+
+
+def _restrict_axis(h, axis):
+    """Transpose of _prolong_axis: c[i] = f[2i] + 0.5 (f[2i-1] + f[2i+1]), missing ends 0."""
+    n = (h.shape[axis] + 1) // 2
+    idx = lambda sl: tuple(sl if a == axis else slice(None) for a in range(h.ndim))
+    pad = [(0, 0)] * h.ndim
+    pad[axis] = (1, 1)
+    hp = np.pad(h, pad)
+    c = hp[idx(slice(1, 2 * n, 2))]
+    left = hp[idx(slice(0, 2 * n - 1, 2))]
+    right = hp[idx(slice(2, 2 * n + 1, 2))]
+    return c + 0.5 * (left + right)
+
+
+def interpolation_operator_3d(xH, Lc, L):
+    """Trilinear prolongation: along x, then y, then z."""
+    H = xH.reshape(Lc["nnz"], Lc["nny"], Lc["nnx"])
+    h = _prolong_axis(_prolong_axis(_prolong_axis(H, 2), 1), 0)
+    assert h.shape == (L["nnz"], L["nny"], L["nnx"])
+    return h.ravel()
+
+
+def restriction_operator_3d(xh, L, Lc):
+    """R = P^T of the trilinear prolongation (unscaled, as the 2D R = I^T): along x, then y, then z."""
+    h = xh.reshape(L["nnz"], L["nny"], L["nnx"])
+    H = _restrict_axis(_restrict_axis(_restrict_axis(h, 2), 1), 0)
+    assert H.shape == (Lc["nnz"], Lc["nny"], Lc["nnx"])
+    return H.ravel()
+
+
+def _restrict(x, L, Lc, periodic):
+    if L.get("ndim") == 3:
+        return restriction_operator_3d(x, L, Lc)
+    return restriction_operator(x, L["nnx"], L["nny"], Lc["nnx"], Lc["nny"], periodic)
+
+
+def _prolong(x, Lc, L, periodic):
+    if L.get("ndim") == 3:
+        return interpolation_operator_3d(x, Lc, L)
+    return interpolation_operator(x, Lc["nnx"], Lc["nny"], L["nnx"], L["nny"], periodic)
+
+
 def v_cycle(rhs, levels, tol, periodic, set_zeroP):
     """One V-cycle from x0 = 0.  multigrid.py:231-330.
 
@@ -128,14 +210,14 @@ def v_cycle(rhs, levels, tol, periodic, set_zeroP):
             Lx = emult.laplacian_element_mult(xh, L["Le"], L["nconn"], L["nn"])
             Lx[g] = 0
             rH = rh - Lx
-            rH = restriction_operator(rH, L["nnx"], L["nny"], Lc["nnx"], Lc["nny"], periodic)
+            rH = _restrict(rH, L, Lc, periodic)
         else:
             rh = r_lv[n] - r_lv[n].mean()
             xh = 2.0 / 3.0 * rh / L["L_diag"]
             xh = xh - xh.mean()
             rH = rh - emult.laplacian_element_mult(xh, L["Le"], L["nconn"], L["nn"])
             rH = rH - rH.mean()
-            rH = restriction_operator(rH, L["nnx"], L["nny"], Lc["nnx"], Lc["nny"], periodic)
+            rH = _restrict(rH, L, Lc, periodic)
             rH = rH - rH.mean()
         x_lv[n] = xh
         r_lv[n + 1] = rH
@@ -159,7 +241,7 @@ def v_cycle(rhs, levels, tol, periodic, set_zeroP):
     x_lv[N - 1] = xN
     for n in range(N - 2, -1, -1):
         L, Lc = levels[n], levels[n + 1]
-        xH = interpolation_operator(x_lv[n + 1], Lc["nnx"], Lc["nny"], L["nnx"], L["nny"], periodic)
+        xH = _prolong(x_lv[n + 1], Lc, L, periodic)
         if not set_zeroP:
             xH = xH - xH.mean()
         xh = x_lv[n] + xH
