@@ -221,18 +221,19 @@ def roofline_only(reps):
 def channel_3d(steps=200, warmup=3):
     """Config 4 (Q1 hex8): 128x64x64 box [0,2]x[0,1]x[0,1], parabolic inflow
     u = 16 y(1-y) z(1-z) (u_max = 1) on the left face, outlet (p = 0) on the right face, no-slip on
-    the other four, rho = 1, mu = 1e-3, dt = 2e-3, BiCGSTAB + PCG with the approximate-inverse
-    preconditioner (the reference has no 3D multigrid), tol 1e-8, from rest.  FP64 form beside
-    the mixed form (FP32-stored Ce); per-field relative L2 of mixed vs FP64 after `steps` steps."""
+    the other four, rho = 1, mu = 1e-3, dt = 2e-3, BiCGSTAB + PCG, tol 1e-8, from rest.  Pressure
+    preconditioners: the approximate inverse (the reference's 3D choice: it has no 3D multigrid)
+    and the box-grid V-cycle (128x64x64 -> 8x4x4, 5 levels, outlet pins on every level).  FP64
+    form beside the mixed form (FP32-stored Ce); per-field relative L2 of mixed vs FP64 after
+    `steps` steps."""
     import torch
     import jax_fem_cfd.setup.simulation as sim
     n, L = [128, 64, 64], [2.0, 1.0, 1.0]
-    out = {}
-    fields = {}
-    for prec in ("fp64", "mixed"):
+
+    def run(prec, precond, final):
         cfg = sim.Config(mesh="simple", solver="standard", mesh_config=sim.SimpleMeshConfig(n, L, 1, False),
                          solver_config=sim.StandardSolverConfig([0, 2, 3, 4, 5], [], [1], False, 1e-8, True,
-                                                                "approx_inv", None, prec))
+                                                                precond, final, prec))
         compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
         nodes = surf[3]
         y, z = coords[nodes, 1], coords[nodes, 2]
@@ -252,20 +253,36 @@ def channel_3d(steps=200, warmup=3):
             its.append((ui, pi))
         torch.cuda.synchronize()
         sec = time.perf_counter() - t0
-        out[prec] = {"timesteps_per_s": steps / sec, "ms_per_step": 1e3 * sec / steps,
-                     "mean_iters": [float(np.mean([i[0] for i in its])), float(np.mean([i[1] for i in its]))]}
-        fields[prec] = (U.cpu().numpy(), P.cpu().numpy(), nn)
-        del timestep, compute
-    (U64, P64, nn), (Um, Pm, _) = fields["fp64"], fields["mixed"]
+        res = {"timesteps_per_s": steps / sec, "ms_per_step": 1e3 * sec / steps,
+               "mean_iters": [float(np.mean([i[0] for i in its])), float(np.mean([i[1] for i in its]))]}
+        return res, (U.cpu().numpy(), P.cpu().numpy(), nn)
 
     def rel(a, b):
         return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
-    out["mixed_vs_fp64_rel_l2"] = {c: rel(Um[i * nn:(i + 1) * nn], U64[i * nn:(i + 1) * nn])
-                                   for i, c in enumerate("uvw")}
-    out["mixed_vs_fp64_rel_l2"]["p"] = rel(Pm, P64)
+    def compare(f64, fm):
+        (U64, P64, nn), (Um, Pm, _) = f64, fm
+        d = {c: rel(Um[i * nn:(i + 1) * nn], U64[i * nn:(i + 1) * nn]) for i, c in enumerate("uvw")}
+        d["p"] = rel(Pm, P64)
+        return d
+
+    out = {}
+    fields = {}
+    for prec in ("fp64", "mixed"):
+        out[prec], fields[prec] = run(prec, "approx_inv", None)
+    nn = fields["fp64"][2]
+    out["mixed_vs_fp64_rel_l2"] = compare(fields["fp64"], fields["mixed"])
     out["speedup_mixed_vs_fp64"] = out["mixed"]["timesteps_per_s"] / out["fp64"]["timesteps_per_s"]
-    out["config"] = {"workload": "3D channel 128x64x64 Q1 hex8, approx_inv PCG, dt=2e-3, 200 steps from rest",
+    mgf = {}
+    mg = {"final_multigrid_mesh": [8, 4, 4], "mg_levels": 5}
+    for prec in ("fp64", "mixed"):
+        mg[prec], mgf[prec] = run(prec, "multigrid", [8, 4, 4])
+    mg["mixed_vs_fp64_rel_l2"] = compare(mgf["fp64"], mgf["mixed"])
+    # the two preconditioners solve the same systems to the same tolerance
+    mg["multigrid_vs_approx_inv_fp64_rel_l2"] = compare(fields["fp64"], mgf["fp64"])
+    out["multigrid"] = mg
+    out["config"] = {"workload": "3D channel 128x64x64 Q1 hex8, approx_inv PCG (and box-grid multigrid PCG), "
+                                 "dt=2e-3, 200 steps from rest",
                      "nodes": int(nn), "elements": 128 * 64 * 64, "steps": steps}
     return out
 
