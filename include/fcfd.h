@@ -167,7 +167,8 @@ int fcfd_elem_mult_stored(fcfd_ctx* ctx, const double* K0_16, const double* A1, 
 
 /* ---- multigrid and solvers --------------------------------------------------------- */
 
-/* z = V(r), one V-cycle from zero.  replaces v_cycle (jax_fem_cfd/core/multigrid.py:231-330) */
+/* z = V(r), one V-cycle from zero.  replaces v_cycle (jax_fem_cfd/core/multigrid.py:231-330);
+ * quad4, quad9 and (new: the reference has none, simulation.py:75-76) hex8 box contexts */
 int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z);
 
 /* PCG on the (pinned or Neumann) pressure Laplacian with the context's
