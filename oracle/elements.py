@@ -93,11 +93,45 @@ def quad9_shape_functions(coords):
     return w[:, None], N, Nderiv
 
 
+def hex27_shape_functions(coords):
+    """Triquadratic 27-node hex, 3x3x3 Gauss; q = (zeta index * 3 + eta index) * 3 + xi index,
+    node b = (c*3 + b_y)*3 + a at local position (a, b_y, c), as create_mesh_3d_hex27.  The
+    reference has no 3D quadratic element (simulation.py:73-74); this is the tensor product of
+    the 1D quadratic Lagrange factors that quad9_shape_functions (shape_functions.py:108-159)
+    uses in 2D."""
+    x1 = np.array([-_S35, 0.0, _S35])
+    w1 = np.array([5.0, 8.0, 5.0]) / 9
+    zeta, eta, xi = np.meshgrid(x1, x1, x1, indexing="ij")
+    xi, eta, zeta = xi.ravel(), eta.ravel(), zeta.ravel()
+    wz, wy, wx = np.meshgrid(w1, w1, w1, indexing="ij")
+    w = (wx * wy * wz).ravel()
+    L = lambda t: [t * (t - 1) / 2, 1 - t ** 2, t * (t + 1) / 2]
+    dL = lambda t: [(2 * t - 1) / 2, -2 * t, (2 * t + 1) / 2]
+    Lx, Ly, Lz, dLx, dLy, dLz = L(xi), L(eta), L(zeta), dL(xi), dL(eta), dL(zeta)
+    pos = [(a, b, c) for c in range(3) for b in range(3) for a in range(3)]
+    N = np.stack([Lx[a] * Ly[b] * Lz[c] for a, b, c in pos], axis=1)
+    dNdxi = np.stack([dLx[a] * Ly[b] * Lz[c] for a, b, c in pos], axis=1)
+    dNdeta = np.stack([Lx[a] * dLy[b] * Lz[c] for a, b, c in pos], axis=1)
+    dNdzeta = np.stack([Lx[a] * Ly[b] * dLz[c] for a, b, c in pos], axis=1)
+    J = np.stack([dNdxi @ coords, dNdeta @ coords, dNdzeta @ coords], axis=1)
+    wq = (w * np.linalg.det(J))[:, None]
+    Nderiv = np.linalg.inv(J) @ np.stack([dNdxi, dNdeta, dNdzeta], axis=1)
+    return wq, N, Nderiv
+
+
+def quad9_face_shape_functions(coords):
+    """Marker for the quad9-faced boundary integral of hex27 meshes (see
+    u_dot_n_boundary_integral_quad9_faces); evaluating it is the quad9 shape function."""
+    return quad9_shape_functions(coords)
+
+
 def shape_functions_for(ndim, shape_func_ord=1):
     if ndim == 2 and shape_func_ord == 2:
         return quad9_shape_functions, lin3_shape_functions
     if ndim == 2:
         return quad4_shape_functions, lin2_shape_functions
+    if shape_func_ord == 2:
+        return hex27_shape_functions, quad9_face_shape_functions
     return hex8_shape_functions, quad4_shape_functions
 
 
@@ -187,6 +221,36 @@ def u_dot_n_boundary_integral_2d(bconn, nvec, gamma_d, U_d, nn, coords):
     Jx = np.einsum("qi,fic->fqc", dNdxi, P)
     Je = np.einsum("qi,fic->fqc", dNdeta, P)
     wq = Jx[..., 0] * Je[..., 1] - Jx[..., 1] * Je[..., 0]           # (nf, q) = det J
+    Uq = np.einsum("qi,fid->fqd", N, Ub)
+    un = np.einsum("fqd,fd->fq", Uq, nvec.astype(float))
+    Fe = np.einsum("qi,fq,fq->fi", N, wq, un)
+    return np.bincount(bconn.ravel(), Fe.ravel(), minlength=nn)
+
+
+def u_dot_n_boundary_integral_quad9_faces(bconn, nvec, gamma_d, U_d, nn, coords):
+    """u_dot_n_boundary_integral_2d (element_calc.py:137-163) on the quad9 faces of a hex27
+    mesh, 3x3 Gauss per face.  As the reference's 3D face integral does, every face is mapped from
+    the (z, y) coordinates of its nodes, so only faces of constant x contribute."""
+    ui = np.searchsorted(gamma_d, bconn)
+    vi = np.searchsorted(gamma_d, bconn + nn)
+    wi = np.searchsorted(gamma_d, bconn + 2 * nn)
+    Ub = np.stack([U_d[ui], U_d[vi], U_d[wi]], axis=-1)              # (nf, 9, 3)
+    x1 = np.array([-_S35, 0.0, _S35])
+    w1 = np.array([5.0, 8.0, 5.0]) / 9
+    xi, eta = np.meshgrid(x1, x1)
+    xi, eta = xi.ravel(), eta.ravel()
+    w = np.outer(w1, w1).ravel()
+    L = lambda t: [t * (t - 1) / 2, 1 - t ** 2, t * (t + 1) / 2]
+    dL = lambda t: [(2 * t - 1) / 2, -2 * t, (2 * t + 1) / 2]
+    pos = [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (1, 2), (0, 2), (0, 1), (1, 1)]
+    Lx, Ly, dLx, dLy = L(xi), L(eta), dL(xi), dL(eta)
+    N = np.stack([Lx[a] * Ly[b] for a, b in pos], axis=1)
+    dNdxi = np.stack([dLx[a] * Ly[b] for a, b in pos], axis=1)
+    dNdeta = np.stack([Lx[a] * dLy[b] for a, b in pos], axis=1)
+    P = coords[bconn][:, :, [2, 1]]
+    Jx = np.einsum("qi,fic->fqc", dNdxi, P)
+    Je = np.einsum("qi,fic->fqc", dNdeta, P)
+    wq = w[None, :] * (Jx[..., 0] * Je[..., 1] - Jx[..., 1] * Je[..., 0])
     Uq = np.einsum("qi,fid->fqd", N, Ub)
     un = np.einsum("fqd,fd->fq", Uq, nvec.astype(float))
     Fe = np.einsum("qi,fq,fq->fi", N, wq, un)

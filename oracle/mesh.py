@@ -88,11 +88,69 @@ def create_mesh_3d(nex, ney, nez, lx, ly, lz):
     return coords, nconn, surf, bconns, nvecs
 
 
+# quad9 local node order over face / element coordinates (s, t), as create_mesh_2d_quad9
+Q9_POS = [(0, 0), (1, 0), (2, 0), (2, 1), (2, 2), (1, 2), (0, 2), (0, 1), (1, 1)]
+
+
+def create_mesh_3d_hex27(nex, ney, nez, lx, ly, lz):
+    """Triquadratic 27-node box mesh.  The reference has no 3D quadratic mesh
+    (simulation.py:73-74 rejects shape_func_ord = 2 in 3D); this one extends create_mesh_3d
+    (regular_mesh.py:45-105) the way create_mesh_2d_quad9 extends create_mesh_2d:
+    node grid (2 nex + 1) x (2 ney + 1) x (2 nez + 1) at half spacing, node k*nnx*nny + j*nnx + i
+    at (i dx/2, j dy/2, k dz/2); element (ie, je, ke) -> (ke*ney + je)*nex + ie owns the nodes
+    (2 ie + a, 2 je + b, 2 ke + c) with local index (c*3 + b)*3 + a.  Faces [D, R, T, L, F, B]
+    (y = 0, x = lx, y = ly, x = 0, z = 0, z = lz) list their nodes in create_mesh_3d's order; a
+    face element is a quad9 patch in quad9 local order over (s, t) = (z, y) on x-faces (counter-
+    clockwise in the (z, y) coordinates the face integral maps from), (x, z) on y-faces and (x, y)
+    on z-faces, ordered with the first in-face element index fastest."""
+    nnx, nny, nnz = 2 * nex + 1, 2 * ney + 1, 2 * nez + 1
+    hx, hy, hz = lx / nex / 2, ly / ney / 2, lz / nez / 2
+    k, j, i = np.meshgrid(np.arange(nnz), np.arange(nny), np.arange(nnx), indexing="ij")
+    coords = np.stack([i * hx, j * hy, k * hz], axis=-1).reshape(-1, 3)
+    lay = nnx * nny
+    ke, je, ie = np.meshgrid(np.arange(nez), np.arange(ney), np.arange(nex), indexing="ij")
+    base = (2 * ke * lay + 2 * je * nnx + 2 * ie).reshape(-1, 1)
+    c, b, a = np.meshgrid(np.arange(3), np.arange(3), np.arange(3), indexing="ij")
+    nconn = (base + (c * lay + b * nnx + a).ravel()[None, :]).astype(np.int64)
+    node = lambda x, y, z: z * lay + y * nnx + x
+
+    def faces(n1, n2, f):
+        rows = []
+        for t2 in range(n2):
+            for t1 in range(n1):
+                rows.append([f(t1, t2, ps, pt) for ps, pt in Q9_POS])
+        return np.array(rows, dtype=np.int64)
+    # x-faces: s = z, t = y (elements ordered je fastest, then ke)
+    bR = faces(ney, nez, lambda je_, ke_, ps, pt: node(nnx - 1, 2 * je_ + pt, 2 * ke_ + ps))
+    bL = faces(ney, nez, lambda je_, ke_, ps, pt: node(0, 2 * je_ + pt, 2 * ke_ + ps))
+    # y-faces: s = x, t = z
+    bD = faces(nex, nez, lambda ie_, ke_, ps, pt: node(2 * ie_ + ps, 0, 2 * ke_ + pt))
+    bT = faces(nex, nez, lambda ie_, ke_, ps, pt: node(2 * ie_ + ps, nny - 1, 2 * ke_ + pt))
+    # z-faces: s = x, t = y
+    bF = faces(nex, ney, lambda ie_, je_, ps, pt: node(2 * ie_ + ps, 2 * je_ + pt, 0))
+    bB = faces(nex, ney, lambda ie_, je_, ps, pt: node(2 * ie_ + ps, 2 * je_ + pt, nnz - 1))
+    bconns = [bD, bR, bT, bL, bF, bB]
+    zoff = np.repeat(np.arange(nnz) * lay, nnx)
+    zoffy = np.repeat(np.arange(nnz) * lay, nny)
+    surf = [np.tile(np.arange(nnx), nnz) + zoff,
+            np.tile(nnx * np.arange(1, nny + 1) - 1, nnz) + zoffy,
+            np.tile(nnx * (nny - 1) + np.arange(nnx), nnz) + zoff,
+            np.tile(nnx * np.arange(nny), nnz) + zoffy,
+            np.arange(lay),
+            lay * (nnz - 1) + np.arange(lay)]
+    surf = [x.astype(np.int64) for x in surf]
+    normals = np.array([[0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, -1], [0, 0, 1]])
+    nvecs = [np.tile(normals[f], (bconns[f].shape[0], 1)) for f in range(6)]
+    return coords, nconn, surf, bconns, nvecs
+
+
 def mesh_func_for(ndim, periodic, shape_func_ord=1):
     if ndim == 2 and shape_func_ord == 2:
         return create_periodic_mesh_2d_quad9 if periodic else create_mesh_2d_quad9
     if ndim == 2:
         return create_periodic_mesh_2d if periodic else create_mesh_2d
+    if shape_func_ord == 2:
+        return create_mesh_3d_hex27
     return create_mesh_3d
 
 

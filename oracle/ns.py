@@ -164,6 +164,8 @@ def compute(s, U_d, S_func, t_n, t):
     if s.gamma_d is not None and s.ndim == 2:
         F_p = el.u_dot_n_boundary_integral_1d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords,
                                               s.bshape or el.lin2_shape_functions)
+    elif s.gamma_d is not None and s.nconn.shape[1] == 27:
+        F_p = el.u_dot_n_boundary_integral_quad9_faces(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords)
     elif s.gamma_d is not None:
         F_p = el.u_dot_n_boundary_integral_2d(s.bconn, s.nvec, s.gamma_d, U_d, s.nn, s.coords)
     if S_func is not None:
