@@ -170,7 +170,9 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
                      c.precond == FCFD_PC_MULTIGRID,
                  "unknown pressure preconditioner");
     if (m->ndim == 3) {
-        build_ctx3(c, m, s);
+        FCFD_REQUIRE(m->order == 0 || m->order == 1 || m->order == 2, "order must be 1 (hex8) or 2 (hex27)");
+        if (m->order == 2) build_ctx_h27(c, m, s);
+        else build_ctx3(c, m, s);
         return;
     }
     FCFD_REQUIRE(m->order == 0 || m->order == 1 || m->order == 2, "order must be 1 (quad4) or 2 (quad9)");
@@ -517,8 +519,8 @@ int fcfd_element_constant(const fcfd_ctx* ctx, int32_t which, double* out16) {
 
 int fcfd_element_constant3(const fcfd_ctx* ctx, int32_t which, double* out64) {
     if (!ctx || !out64) return FCFD_ERR_ARG;
-    if (ctx->c.dim != 3) {
-        g_last_error = "fcfd_element_constant3 needs a 3D context";
+    if (ctx->c.dim != 3 || ctx->c.order == 2) {
+        g_last_error = "fcfd_element_constant3 needs a hex8 context (hex27: fcfd_element_matrix)";
         return FCFD_ERR_UNSUPPORTED;
     }
     const ElemConst3& e = ctx->c.ec3;
@@ -534,13 +536,17 @@ int fcfd_element_constant3(const fcfd_ctx* ctx, int32_t which, double* out64) {
 int fcfd_element_matrix(const fcfd_ctx* ctx, int32_t which, double* out, int32_t n) {
     if (!ctx || !out) return FCFD_ERR_ARG;
     const Ctx& c = ctx->c;
-    const int nen = c.dim == 3 ? 8 : (c.order == 2 ? 9 : 4);
+    const int nen = c.dim == 3 ? (c.order == 2 ? 27 : 8) : (c.order == 2 ? 9 : 4);
     if (n != nen * nen) {
         g_last_error = "element matrix size mismatch: this context has " + std::to_string(nen * nen) + " entries";
         return FCFD_ERR_ARG;
     }
     const double* src = nullptr;
-    if (c.dim == 3) {
+    if (c.dim == 3 && c.order == 2) {
+        const ElemConstH27& e = c.eh27[0];
+        src = which == 0 ? e.Me : which == 1 ? e.Le : (which >= 2 && which <= 4) ? e.G[which - 2]
+            : which == 5 ? e.Leoff : nullptr;
+    } else if (c.dim == 3) {
         src = which == 0 ? c.ec3.Me : which == 1 ? c.ec3.Le : (which >= 2 && which <= 4) ? c.ec3.G[which - 2] : nullptr;
     } else if (c.order == 2) {
         src = which == 0 ? c.eq2.Me : which == 1 ? c.eq2.Le : which == 2 ? c.eq2.Gx : which == 3 ? c.eq2.Gy
@@ -561,7 +567,9 @@ int fcfd_mass_matvec(fcfd_ctx* ctx, const double* x, double* y) {
     if (!ctx || !x || !y || x == y) return FCFD_ERR_ARG;
     return run(ctx, [&] {
         Ctx& c = ctx->c;
-        if (c.dim == 3) {
+        if (c.dim == 3 && c.order == 2) {
+            mass_matvec_h27(c, x, y);
+        } else if (c.dim == 3) {
             mass_matvec3(c, x, y);
         } else if (c.order == 2) {
             mass_matvec_q2(c, x, y);
@@ -591,7 +599,8 @@ int fcfd_laplacian_diag(fcfd_ctx* ctx, int32_t level, double* out) {
 int fcfd_conv_elem_calc(fcfd_ctx* ctx, const double* U, double beta, double* Ce) {
     if (!ctx || !U || !Ce) return FCFD_ERR_ARG;
     return run(ctx, [&] {
-        if (ctx->c.dim == 3) build_ce3_api(ctx->c, U, beta, Ce);
+        if (ctx->c.dim == 3 && ctx->c.order == 2) build_ce_h27_api(ctx->c, U, beta, Ce);
+        else if (ctx->c.dim == 3) build_ce3_api(ctx->c, U, beta, Ce);
         else if (ctx->c.order == 2) build_ce_q2_api(ctx->c, U, beta, Ce);
         else build_ce_api(ctx->c, U, beta, Ce);
     });
@@ -600,7 +609,8 @@ int fcfd_conv_elem_calc(fcfd_ctx* ctx, const double* U, double beta, double* Ce)
 int fcfd_boundary_integral(fcfd_ctx* ctx, const double* U_d, double* F_p) {
     if (!ctx || !F_p) return FCFD_ERR_ARG;
     return run(ctx, [&] {
-        if (ctx->c.dim == 3) boundary_integral3(ctx->c, U_d, F_p);
+        if (ctx->c.dim == 3 && ctx->c.order == 2) boundary_integral_h27(ctx->c, U_d, F_p);
+        else if (ctx->c.dim == 3) boundary_integral3(ctx->c, U_d, F_p);
         else if (ctx->c.order == 2) boundary_integral_q2(ctx->c, U_d, F_p);
         else boundary_integral(ctx->c, U_d, F_p);
     });
@@ -632,7 +642,8 @@ int fcfd_pressure_cg(fcfd_ctx* ctx, const double* b, double* x, double tol, int3
     if (ctx->c.dim == 3)
         return run(ctx, [&] {
             bool conv = false;
-            int it = pressure_cg3(ctx->c, b, x, tol, maxiter > 0 ? maxiter : 1000, &conv);
+            int it = ctx->c.order == 2 ? pressure_cg_h27(ctx->c, b, x, tol, maxiter > 0 ? maxiter : 1000, &conv)
+                                       : pressure_cg3(ctx->c, b, x, tol, maxiter > 0 ? maxiter : 1000, &conv);
             FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
             if (iters) *iters = it;
             if (converged) *converged = conv;
@@ -650,7 +661,8 @@ int fcfd_solve_pressure(fcfd_ctx* ctx, const double* U_star, const double* P_n, 
                         double dt, double* P, int32_t* iters) {
     if (!ctx || !U_star || !P_n || !P) return FCFD_ERR_ARG;
     return run(ctx, [&] {
-        int it = ctx->c.dim == 3     ? solve_pressure3(ctx->c, U_star, P_n, F_p, rho, dt, P)
+        int it = ctx->c.dim == 3 && ctx->c.order == 2 ? solve_pressure_h27(ctx->c, U_star, P_n, F_p, rho, dt, P)
+               : ctx->c.dim == 3     ? solve_pressure3(ctx->c, U_star, P_n, F_p, rho, dt, P)
                  : ctx->c.order == 2 ? solve_pressure_q2(ctx->c, U_star, P_n, F_p, rho, dt, P)
                                      : solve_pressure(ctx->c, U_star, P_n, F_p, rho, dt, P);
         FCFD_HIP(hipStreamSynchronize(ctx->c.stream));
@@ -665,7 +677,10 @@ int fcfd_solve_momentum(fcfd_ctx* ctx, const double* U_n, const double* P_n, con
         Ctx& c = ctx->c;
         scatter_ud(c, U_d);
         int it;
-        if (c.dim == 3) {
+        if (c.dim == 3 && c.order == 2) {
+            FCFD_REQUIRE(!Ce, "3D contexts build Ce(U_n) themselves (pass Ce = NULL)");
+            it = solve_momentum_h27(c, U_n, P_n, rho, mu, dt, F, U_hat);
+        } else if (c.dim == 3) {
             FCFD_REQUIRE(!Ce, "3D contexts build Ce(U_n) themselves (pass Ce = NULL)");
             it = solve_momentum3_from(c, U_n, P_n, rho, mu, dt, F, U_hat);
         } else if (c.order == 2) {
@@ -690,7 +705,8 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
     if (!ctx || !U_n || !P_n || !U || !P) return FCFD_ERR_ARG;
     return run(ctx, [&] {
         int it[2];
-        if (ctx->c.dim == 3) four_step3(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+        if (ctx->c.dim == 3 && ctx->c.order == 2) four_step_h27(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+        else if (ctx->c.dim == 3) four_step3(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
         else if (ctx->c.order == 2) four_step_q2(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
         else four_step(ctx->c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
         if (iters) {

@@ -33,6 +33,14 @@ struct ElemConst3 {
     double Leoff[64];  // Le with its diagonal zeroed (the Jacobi smoother's off-diagonal part)
 };
 
+// Triquadratic hex27 constants of a box element (3x3x3 Gauss, q = (qz*3 + qy)*3 + qx; local node
+// b = (c*3 + b_y)*3 + a at (a, b_y, c) in {0,1,2}^3), [q][b] / [a][b] layouts as ElemConst3.
+struct ElemConstH27 {
+    double N[729], dN[3][729], wq[27];
+    double Me[729], Le[729], Leoff[729], G[3][729];
+    double WN[729];  // [a][q] = w_q N_qa: the test-function weights of the Ce build
+};
+
 // Local corner numbering: corner b sits at (hbx, hby, hbz) in {0,1}^3 of its element.
 __host__ __device__ constexpr int hbx(int b) { return ((b & 3) == 1 || (b & 3) == 2) ? 1 : 0; }
 __host__ __device__ constexpr int hby(int b) { return (b & 3) >= 2 ? 1 : 0; }

@@ -1,0 +1,878 @@
+// Triquadratic (hex27) four-step timestep on structured box grids.
+//
+// The reference has no 3D quadratic element (simulation.py:73-74: "3D quadratic shape functions
+// not yet implemented").  This path runs the chain of jax_fem_cfd/solvers/standard.py:148-165 with
+// ndim = 3 on hex27 box meshes, built the way the quad9 path extends quad4 in 2D:
+//   - the context's grid is its NODE grid: Grid3D with nnx = 2 nex + 1 (likewise y, z) and the
+//     element counts in nex, ney, nez; element (ie, je, ke) -> (ke*ney + je)*nex + ie owns the
+//     nodes (2 ie + a, 2 je + b, 2 ke + c), local index (c*3 + b)*3 + a;
+//   - 3x3x3 Gauss, 27 x 27 element matrices (Ce row-blocked, FP64 or FP32);
+//   - BiCGSTAB momentum (3 components), PCG pressure with none / jacobi / approx_inv;
+//   - the boundary integral on quad9 faces, mapped from the (z, y) coordinates of their nodes as
+//     the reference's 3D face integral does (element_calc.py:137-163), so x-faces carry it;
+//   - the mass operator is the per-component product (the reference's 3D consistent_mass_mult
+//     cannot run: element_mult.py:26).
+// Kernels are owner-computes node gathers: node (i, j, k) has 1, 2, 4 or 8 incident elements (by
+// the parity of i, j, k); it sums, in ascending element order, its row of each, applied to that
+// element's 27 nodal values read directly.
+#include <algorithm>
+#include <cmath>
+#include "fcfd_bicgstab.hpp"
+#include "fcfd_cg.hpp"
+
+namespace fcfd {
+
+// ------------------------------------------------------------------------------ element data
+__host__ __device__ constexpr int h27x(int b) { return b % 3; }
+__host__ __device__ constexpr int h27y(int b) { return (b / 3) % 3; }
+__host__ __device__ constexpr int h27z(int b) { return b / 9; }
+
+// Row-blocked 27 x 27 storage: entry (a, b) of element e at ((a * ne + e) * 27 + b)
+__host__ __device__ __forceinline__ size_t h27_idx(int k, int e, int ne) {
+    return ((size_t)(k / 27) * ne + e) * 27 + k % 27;
+}
+struct Ce27F64 {
+    double* c; int ne;
+    __device__ __forceinline__ void ld_row(int a, int e, double (&v)[27]) const {
+        const double* p = c + h27_idx(a * 27, e, ne);
+#pragma unroll
+        for (int b = 0; b < 27; ++b) v[b] = p[b];
+    }
+    __device__ __forceinline__ void st(int k, int e, double v) const { c[h27_idx(k, e, ne)] = v; }
+};
+struct Ce27F32 {
+    float* c; int ne;
+    __device__ __forceinline__ void ld_row(int a, int e, double (&v)[27]) const {
+        const float* p = c + h27_idx(a * 27, e, ne);
+#pragma unroll
+        for (int b = 0; b < 27; ++b) v[b] = (double)p[b];
+    }
+    __device__ __forceinline__ void st(int k, int e, double v) const { c[h27_idx(k, e, ne)] = (float)v; }
+};
+struct Ce27Api {
+    double* c;
+    __device__ __forceinline__ void st(int k, int e, double v) const { c[(size_t)e * 729 + k] = v; }
+};
+
+static Grid3D make_grid27(int nex, int ney, int nez) {
+    Grid3D g;
+    g.nex = nex; g.ney = ney; g.nez = nez;
+    g.nnx = 2 * nex + 1; g.nny = 2 * ney + 1; g.nnz = 2 * nez + 1;
+    g.nxy = g.nnx * g.nny;
+    g.nn = g.nxy * g.nnz;
+    g.ne = nex * ney * nez;
+    return g;
+}
+
+// hex27 3x3x3 Gauss data of a box element of size hx x hy x hz and its constants
+// (element_calc.py:9-40 on the triquadratic tensor-product basis)
+static void hex27_constants(double hx, double hy, double hz, ElemConstH27& ec) {
+    const double g = std::sqrt(3.0 / 5.0);
+    const double x1[3] = {-g, 0.0, g}, w1[3] = {5.0 / 9, 8.0 / 9, 5.0 / 9};
+    auto L = [](double t, int a) { return a == 0 ? t * (t - 1) / 2 : (a == 1 ? 1 - t * t : t * (t + 1) / 2); };
+    auto dL = [](double t, int a) { return a == 0 ? (2 * t - 1) / 2 : (a == 1 ? -2 * t : (2 * t + 1) / 2); };
+    for (int q = 0; q < 27; ++q) {
+        const double xi = x1[h27x(q)], eta = x1[h27y(q)], zeta = x1[h27z(q)];
+        double dxi[3][27];
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int b = 0; b < 27; ++b) {
+            const int a = h27x(b), by = h27y(b), c = h27z(b);
+            ec.N[q * 27 + b] = L(xi, a) * L(eta, by) * L(zeta, c);
+            dxi[0][b] = dL(xi, a) * L(eta, by) * L(zeta, c);
+            dxi[1][b] = L(xi, a) * dL(eta, by) * L(zeta, c);
+            dxi[2][b] = L(xi, a) * L(eta, by) * dL(zeta, c);
+            const double X[3] = {a * hx / 2, by * hy / 2, c * hz / 2};
+            for (int r = 0; r < 3; ++r)
+                for (int col = 0; col < 3; ++col) J[r][col] += dxi[r][b] * X[col];
+        }
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        double Ji[3][3];
+        Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+        ec.wq[q] = w1[h27x(q)] * w1[h27y(q)] * w1[h27z(q)] * det;
+        for (int d = 0; d < 3; ++d)
+            for (int b = 0; b < 27; ++b)
+                ec.dN[d][q * 27 + b] = Ji[d][0] * dxi[0][b] + Ji[d][1] * dxi[1][b] + Ji[d][2] * dxi[2][b];
+    }
+    for (int a = 0; a < 27; ++a) {
+        for (int b = 0; b < 27; ++b) {
+            double m = 0.0, l = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+            for (int q = 0; q < 27; ++q) {
+                const double w = ec.wq[q], Na = ec.N[q * 27 + a], Nb = ec.N[q * 27 + b];
+                m += w * Na * Nb;
+                l += w * (ec.dN[0][q * 27 + a] * ec.dN[0][q * 27 + b] + ec.dN[1][q * 27 + a] * ec.dN[1][q * 27 + b] +
+                          ec.dN[2][q * 27 + a] * ec.dN[2][q * 27 + b]);
+                g0 += w * Na * ec.dN[0][q * 27 + b];
+                g1 += w * Na * ec.dN[1][q * 27 + b];
+                g2 += w * Na * ec.dN[2][q * 27 + b];
+            }
+            ec.Me[a * 27 + b] = m;
+            ec.Le[a * 27 + b] = l;
+            ec.Leoff[a * 27 + b] = a == b ? 0.0 : l;
+            ec.G[0][a * 27 + b] = g0;
+            ec.G[1][a * 27 + b] = g1;
+            ec.G[2][a * 27 + b] = g2;
+        }
+        for (int q = 0; q < 27; ++q) ec.WN[a * 27 + q] = ec.wq[q] * ec.N[q * 27 + a];
+    }
+}
+
+// ------------------------------------------------------------------------------ incidence
+// Per axis (i on nnx): odd i -> element (i-1)/2 at local 1 only; even i -> element i/2 - 1 at
+// local 2 and element i/2 at local 0.  Candidate q = qx + 2 qy + 4 qz: ascending element ids.
+// The candidates are formed arithmetically inside a rolled loop (no private array is indexed at
+// run time and no more than one element's row and values are live), and missing ones skipped.
+struct Inc27 {
+    int i, j, k;
+};
+
+__device__ __forceinline__ void node_inc_h27(const Grid3D& g, int n, Inc27& s) {
+    s.k = n / g.nxy;
+    const int r = n - s.k * g.nxy;
+    s.j = r / g.nnx;
+    s.i = r - s.j * g.nnx;
+}
+
+__device__ __forceinline__ void cand_axis(int i, int ne, int q, int& el, int& loc, bool& v) {
+    const bool o = i & 1;
+    el = q ? (i >> 1) : (o ? (i - 1) >> 1 : (i >> 1) - 1);
+    loc = q ? 0 : (o ? 1 : 2);
+    v = q ? (!o && (i >> 1) < ne) : (el >= 0);
+}
+
+// sum_q has_q sum_b A(row_q, b, e_q) X(node_b of e_q) for NC components; A from rowf(a, e, cf)
+template <int NC, class RowF, class ValF>
+__device__ __forceinline__ void h27_apply(const Grid3D& g, const Inc27& s, RowF rowf, ValF val, double (&y)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) y[c] = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < 8; ++q) {
+        int ex, cx, ey, cy, ez, cz;
+        bool vx, vy, vz;
+        cand_axis(s.i, g.nex, q & 1, ex, cx, vx);
+        cand_axis(s.j, g.ney, (q >> 1) & 1, ey, cy, vy);
+        cand_axis(s.k, g.nez, q >> 2, ez, cz, vz);
+        if (vx && vy && vz) {
+            const int e = (ez * g.ney + ey) * g.nex + ex;
+            const int base = 2 * ez * g.nxy + 2 * ey * g.nnx + 2 * ex;
+            double cf[27];
+            rowf((cz * 3 + cy) * 3 + cx, e, cf);
+            double t[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) t[c] = 0.0;
+#pragma unroll
+            for (int b = 0; b < 27; ++b) {
+                const int id = base + h27z(b) * g.nxy + h27y(b) * g.nnx + h27x(b);
+#pragma unroll
+                for (int c = 0; c < NC; ++c) t[c] += cf[b] * val(c, id);
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) y[c] += t[c];
+        }
+    }
+}
+
+// one shared 27 x 27 matrix (TR: its transpose)
+template <bool TR = false, class ValF>
+__device__ __forceinline__ double h27_const(const Grid3D& g, const Inc27& s, const double* __restrict__ A, ValF val) {
+    double y[1];
+    h27_apply<1>(g, s,
+                 [&](int a, int, double (&cf)[27]) {
+#pragma unroll
+                     for (int b = 0; b < 27; ++b) cf[b] = TR ? A[b * 27 + a] : A[a * 27 + b];
+                 },
+                 [&](int, int id) { return val(id); }, y);
+    return y[0];
+}
+
+// ------------------------------------------------------------------------------ Ce
+// Ce[a][b] = sum_q w_q N_qa (U_q . grad N_qb + beta div U_q N_qb)   (element_calc.py:165-185)
+// One element per thread: the quadrature values of U and div U first (loops unrolled), then per
+// column b the advection term at the 27 points and the 27 entries of the column.
+template <class S>
+__global__ __launch_bounds__(64) void k_build_ce_h27(Grid3D g, const ElemConstH27* __restrict__ ec,
+                                                    const double* __restrict__ U, double beta, S store) {
+    const int e = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
+    if (e >= g.ne) return;
+    const int ke = e / (g.nex * g.ney), r = e - ke * g.nex * g.ney;
+    const int je = r / g.nex, ie = r - je * g.nex;
+    const int base = 2 * ke * g.nxy + 2 * je * g.nnx + 2 * ie;
+    double uq[27], vq[27], wq[27], dq[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) uq[q] = vq[q] = wq[q] = dq[q] = 0.0;
+#pragma unroll
+    for (int b = 0; b < 27; ++b) {
+        const int id = base + h27z(b) * g.nxy + h27y(b) * g.nnx + h27x(b);
+        const double u = U[id], v = U[id + g.nn], w = U[id + 2 * (size_t)g.nn];
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const double Nb = ec->N[q * 27 + b];
+            uq[q] += Nb * u;
+            vq[q] += Nb * v;
+            wq[q] += Nb * w;
+            dq[q] += ec->dN[0][q * 27 + b] * u + ec->dN[1][q * 27 + b] * v + ec->dN[2][q * 27 + b] * w;
+        }
+    }
+    for (int b = 0; b < 27; ++b) {
+        double adv[27];
+#pragma unroll
+        for (int q = 0; q < 27; ++q)
+            adv[q] = uq[q] * ec->dN[0][q * 27 + b] + vq[q] * ec->dN[1][q * 27 + b] + wq[q] * ec->dN[2][q * 27 + b] +
+                     beta * dq[q] * ec->N[q * 27 + b];
+        for (int a = 0; a < 27; ++a) {
+            double t = 0.0;
+#pragma unroll
+            for (int q = 0; q < 27; ++q) t += ec->WN[a * 27 + q] * adv[q];
+            store.st(a * 27 + b, e, t);
+        }
+    }
+}
+
+static void build_ce_h27(Ctx& c, const double* U, double beta) {
+    const int nb = (c.g3.ne + 63) / 64;
+    if (c.ce_format == FCFD_CE_F64)
+        k_build_ce_h27<<<nb, 64, 0, c.stream>>>(c.g3, c.eh27_dev, U, beta, Ce27F64{(double*)c.ce, c.g3.ne});
+    else
+        k_build_ce_h27<<<nb, 64, 0, c.stream>>>(c.g3, c.eh27_dev, U, beta, Ce27F32{(float*)c.ce, c.g3.ne});
+    FCFD_CHECK_LAUNCH();
+}
+
+void build_ce_h27_api(Ctx& c, const double* U, double beta, double* Ce) {
+    k_build_ce_h27<<<(c.g3.ne + 63) / 64, 64, 0, c.stream>>>(c.g3, c.eh27_dev, U, beta, Ce27Api{Ce});
+    FCFD_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------------------ momentum
+template <class S>
+struct MomCoef27 {
+    Grid3D g;
+    const ElemConstH27* __restrict__ ec;
+    S ce;
+    double cM, cL, cC;
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[27]) const {
+        double v[27];
+        ce.ld_row(a, e, v);
+#pragma unroll
+        for (int b = 0; b < 27; ++b) cf[b] = cM * ec->Me[a * 27 + b] + cL * ec->Le[a * 27 + b] + cC * v[b];
+    }
+};
+
+// rhs = R U_n - G P_n + F (Dirichlet rows U_d), x0 = U_n with Dirichlet values, r0 = rhs - A x0,
+// rhat = p = q = r0; reductions b.b, r0.r0 (the quad4 BInit on hex27 gathers)
+template <class S>
+struct BInit27 {
+    MomCoef27<S> lhs, rhs;
+    const double *xn, *ud, *Pn, *F;
+    const uint8_t* mask;
+    double *x, *r, *rhat, *p, *q;
+    __device__ void operator()(int n, double (&acc)[2]) const {
+        const Grid3D& g = lhs.g;
+        const int nn = g.nn;
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        double b[3], ax[3];
+        h27_apply<3>(g, s, [&](int a, int e, double (&cf)[27]) { rhs.row(a, e, cf); },
+                     [&](int c, int id) { return xn[id + c * nn]; }, b);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            b[c] -= h27_const(g, s, lhs.ec->G[c], [&](int id) { return Pn[id]; });
+            if (F) b[c] += F[n + c * nn];
+        }
+        h27_apply<3>(g, s, [&](int a, int e, double (&cf)[27]) { lhs.row(a, e, cf); },
+                     [&](int c, int id) {
+                         const int i = id + c * nn;
+                         const double xv = xn[i];
+                         return (mask && mask[i]) ? ud[i] : xv;
+                     },
+                     ax);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int i = n + c * nn;
+            const bool m = mask && mask[i];
+            const double x0 = m ? ud[i] : xn[i];
+            const double bc = m ? ud[i] : b[c];
+            const double a = m ? x0 : ax[c];
+            const double rc = bc - a;
+            x[i] = x0;
+            r[i] = rc; rhat[i] = rc; p[i] = rc; q[i] = rc;
+            acc[0] += bc * bc;
+            acc[1] += rc * rc;
+        }
+    }
+};
+
+template <class Op>
+struct BK1_27 {
+    Op op;
+    const uint8_t* mask;
+    const double *r, *pold, *qold, *rhat;
+    double *pnew, *qnew;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const Grid3D& g = op.g;
+        const int nn = g.nn;
+        const double beta = st->b_beta, omega = st->b_omega;
+        auto pv = [&](int c, int id) {
+            const int i = id + c * nn;
+            return r[i] + beta * (pold[i] - omega * qold[i]);
+        };
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        double y[3];
+        h27_apply<3>(g, s, [&](int a, int e, double (&cf)[27]) { op.row(a, e, cf); }, pv, y);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int i = n + c * nn;
+            const double pc = pv(c, n);
+            const double yc = (mask && mask[i]) ? pc : y[c];
+            pnew[i] = pc;
+            qnew[i] = yc;
+            acc[0] += rhat[i] * yc;
+        }
+    }
+};
+
+template <class Op>
+struct BK2_27 {
+    Op op;
+    const uint8_t* mask;
+    const double *r, *q;
+    double *sv, *tv;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[3]) const {
+        const Grid3D& g = op.g;
+        const int nn = g.nn;
+        const double alpha = st->b_alpha;
+        auto svv = [&](int c, int id) { return r[id + c * nn] - alpha * q[id + c * nn]; };
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        double y[3];
+        h27_apply<3>(g, s, [&](int a, int e, double (&cf)[27]) { op.row(a, e, cf); }, svv, y);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int i = n + c * nn;
+            const double sc = svv(c, n);
+            const double yc = (mask && mask[i]) ? sc : y[c];
+            sv[i] = sc;
+            tv[i] = yc;
+            acc[0] += sc * sc;
+            acc[1] += yc * sc;
+            acc[2] += yc * yc;
+        }
+    }
+};
+
+template <class S>
+static int solve_momentum_h27_t(Ctx& c, S ce, const double* U_n, const double* P_n, double rho, double mu, double dt,
+                                const double* F, double* U_hat) {
+    const double theta = 0.5;
+    const Grid3D& g = c.g3;
+    const int nn = g.nn;
+    using Op = MomCoef27<S>;
+    Op lhs{g, c.eh27_dev, ce, rho / dt, theta * mu, theta * rho};
+    Op rhs{g, c.eh27_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
+    const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 3 * nn;
+    launch_node<2>(c, nn, nullptr,
+                   BInit27<S>{lhs, rhs, U_n, c.ud_full, P_n, F, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                   BInitFin{c.st, c.tol, maxit});
+    FCFD_CHECK_LAUNCH();
+    const int* act = &c.st->b_active;
+    auto pair = [&]() {
+        for (int o = 0; o < 2; ++o) {
+            const int nw = o ^ 1;
+            launch_node<1>(c, nn, act,
+                           BK1_27<Op>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
+                           BK1Fin{c.st});
+            launch_node<3>(c, nn, act, BK2_27<Op>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st}, BK2Fin{c.st});
+            launch_node<2>(c, nn, act, BK3<3>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
+                           BK3Fin{c.st});
+        }
+        FCFD_CHECK_LAUNCH();
+    };
+    const double key[4] = {rho, mu, dt, 0.0};
+    int launched = 0;
+    int pairs = c.pred_mom > 0 ? (c.pred_mom + 1) / 2 : 1;
+    int h[2] = {0, 0};
+    while (true) {
+        replay(c, c.g_mom, key, ce.c, pairs, pair);
+        launched += 2 * pairs;
+        read_state_ints(c, &c.st->b_k, 2, h);  // b_k, b_active
+        if (!h[1] || launched >= maxit) break;
+        pairs = 1;
+    }
+    if (h[0] > 0) c.pred_mom = h[0];
+    if (U_hat != c.m_x)
+        FCFD_HIP(hipMemcpyAsync(U_hat, c.m_x, sizeof(double) * 3 * nn, hipMemcpyDeviceToDevice, c.stream));
+    return h[0];
+}
+
+int solve_momentum_h27(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt, const double* F,
+                       double* U_hat) {
+    build_ce_h27(c, U_n, 0.5);
+    if (c.ce_format == FCFD_CE_F64)
+        return solve_momentum_h27_t(c, Ce27F64{(double*)c.ce, c.g3.ne}, U_n, P_n, rho, mu, dt, F, U_hat);
+    return solve_momentum_h27_t(c, Ce27F32{(float*)c.ce, c.g3.ne}, U_n, P_n, rho, mu, dt, F, U_hat);
+}
+
+// ------------------------------------------------------------------------------ projection
+// Uout = Uin + c G P / M (c = dt/rho: U*; c = -dt/rho with Dirichlet values: the final U)
+struct Proj27Body {
+    Grid3D g;
+    const ElemConstH27* ec;
+    const double *Uin, *P, *M, *ud;
+    const uint8_t* mask;
+    double* Uout;
+    double c;
+    __device__ void operator()(int n, double (&)[1]) const {
+        Inc27 s;
+        node_inc_h27(g, n, s);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double gd = h27_const(g, s, ec->G[d], [&](int id) { return P[id]; });
+            const size_t i = n + (size_t)d * g.nn;
+            double u = Uin[i] + (c * gd) / M[i];
+            if (mask && mask[i]) u = ud[i];
+            Uout[i] = u;
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------ pressure
+struct Prhs27Body {
+    Grid3D g; const ElemConstH27* ec; const double* Us; const double* Fp; const uint8_t* pin; double* rhs; double c;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        double div = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double* u = Us + (size_t)d * g.nn;
+            div += h27_const<true>(g, s, ec->G[d], [&](int id) { return u[id]; });
+        }
+        double v = c * (div - (Fp ? Fp[n] : 0.0));
+        if (pin && pin[n]) v = 0.0;
+        rhs[n] = v;
+        acc[0] += v;
+    }
+};
+
+struct CgInit27 {
+    Grid3D g; const double* Le; const uint8_t* pin; const double* b; const double* bmean; const double* x0in;
+    int zero_pins; double* x; double* r;
+    __device__ void operator()(int n, double (&acc)[3]) const {
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        auto xv = [&](int id) {
+            const double v = x0in[id];
+            return (zero_pins && pin && pin[id]) ? 0.0 : v;
+        };
+        const bool pn = pin && pin[n];
+        const double ax = pn ? 0.0 : h27_const(g, s, Le, xv);
+        const double bn = b[n] - (bmean ? *bmean : 0.0);
+        const double rn = bn - ax;
+        x[n] = xv(n);
+        r[n] = rn;
+        acc[0] += bn * bn;
+        acc[1] += rn * rn;
+        acc[2] += rn;
+    }
+};
+
+struct CgK1_27 {
+    Grid3D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
+    const DevState* st;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        const double beta = st->c_beta;
+        auto pv = [&](int id) { return z[id] + beta * pold[id]; };
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        const double a = (pin && pin[n]) ? 0.0 : h27_const(g, s, Le, pv);
+        const double pn = pv(n);
+        pnew[n] = pn;
+        Ap[n] = a;
+        acc[0] += pn * a;
+    }
+};
+
+// two-term Neumann-series approximate inverse (standard.py:225-234)
+struct ApproxInv27a {
+    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
+    __device__ void operator()(int n, double (&)[1]) const {
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        const double v = h27_const(g, s, Le, [&](int id) { return r[id] / Ld[id]; }) / Ld[n];
+        d1l[n] = (pin && pin[n]) ? 0.0 : v;
+    }
+};
+struct ApproxInv27b {
+    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l; double* z;
+    __device__ void operator()(int n, double (&)[1]) const {
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        double l2 = h27_const(g, s, Le, [&](int id) { return d1l[id]; });
+        if (pin && pin[n]) l2 = 0.0;
+        const double d1x = r[n] / Ld[n];
+        z[n] = (3.0 * d1x - 3.0 * d1l[n]) + l2 / Ld[n];
+    }
+};
+
+static const double* precondition_h27(Ctx& c, const int* active) {
+    const int nn = c.g3.nn;
+    const double* Le = c.eh27_dev->Le;
+    switch (c.precond) {
+        case FCFD_PC_NONE: return c.c_r;
+        case FCFD_PC_JACOBI:
+            launch_node<0>(c, nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y}, NoFin{});
+            return c.lv[0].y;
+        case FCFD_PC_APPROX_INV:
+            launch_node<0>(c, nn, active, ApproxInv27a{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x}, NoFin{});
+            launch_node<0>(c, nn, active, ApproxInv27b{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y},
+                           NoFin{});
+            return c.lv[0].y;
+    }
+    throw Error(FCFD_ERR_UNSUPPORTED, "hex27 pressure preconditioner must be none, jacobi or approx_inv");
+}
+
+// PCG (jax_iterative_solvers.py:87-120); solution in c.c_x
+static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol,
+                   int maxit, bool* converged) {
+    const Grid3D& g = c.g3;
+    const int nn = g.nn;
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    const double* Le = c.eh27_dev->Le;
+    launch_node<3>(c, nn, nullptr, CgInit27{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
+                   CgInitFin{c.st, tol, maxit, nn, 0});
+    const int* act = &c.st->c_active;
+    const double* y0 = precondition_h27(c, act);
+    launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, y0, c.c_z[0], c.c_p[0], c.st, 0}, CgGamma0Fin{c.st});
+    FCFD_CHECK_LAUNCH();
+    const double* Ld = c.lv[0].Ld;
+    auto pair = [&]() {
+        for (int o = 0; o < 2; ++o) {
+            const int nw = o ^ 1;
+            launch_node<1>(c, nn, act, CgK1_27{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                           CgK1Fin{c.st});
+            if (c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_NONE) {
+                launch_node<2>(c, nn, act,
+                               CgK23{c.c_p[nw], c.c_Ap, c.precond == FCFD_PC_JACOBI ? Ld : nullptr, c.c_x, c.c_r,
+                                     c.c_z[nw], c.st},
+                               CgK23Fin{c.st});
+            } else {
+                launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
+                const double* y = precondition_h27(c, act);
+                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, 0, 0},
+                               CgK3Fin{c.st, 0});
+            }
+        }
+        FCFD_CHECK_LAUNCH();
+    };
+    const double key[4] = {0.0, 0.0, 0.0, 0.0};
+    int launched = 0;
+    int pairs = c.pred_p > 0 ? (c.pred_p + 1) / 2 : 1;
+    int h[2] = {0, 0};
+    while (true) {
+        replay(c, c.g_cg, key, nullptr, pairs, pair);
+        launched += 2 * pairs;
+        read_state_ints(c, &c.st->c_k, 2, h);  // c_k, c_active
+        if (!h[1] || launched >= maxit) break;
+        pairs = std::max(1, std::min(16, pairs * 2));
+    }
+    if (converged) *converged = !h[1] && h[0] <= maxit;
+    if (h[0] > 0) c.pred_p = h[0];
+    return h[0];
+}
+
+int pressure_cg_h27(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged) {
+    const int it = pcg_h27(c, b, nullptr, x, 0, tol, maxit, converged);
+    FCFD_HIP(hipMemcpyAsync(x, c.c_x, sizeof(double) * c.g3.nn, hipMemcpyDeviceToDevice, c.stream));
+    return it;
+}
+
+int solve_pressure_h27(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                       double* P_out) {
+    const int nn = c.g3.nn;
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    launch_node<1>(c, nn, nullptr, Prhs27Body{c.g3, c.eh27_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
+                   MeanFin{&c.st->mean_rhs, nullptr, nn});
+    const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
+    bool conv;
+    const int it = pcg_h27(c, c.c_rhs, c.set_zeroP ? nullptr : &c.st->mean_rhs, P_n, c.set_zeroP, c.tol, maxit, &conv);
+    if (!c.set_zeroP) launch_node<1>(c, nn, nullptr, SumXBody{c.c_x}, MeanFin{&c.st->mean_p, nullptr, nn});
+    launch_node<0>(c, nn, nullptr, PoutBody{c.c_x, pin, P_out, c.st, !c.set_zeroP}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+    return it;
+}
+
+// ------------------------------------------------------------------------------ mass matvec
+struct Mass27Body {
+    Grid3D g; const ElemConstH27* ec; const double* x; double* y;
+    __device__ void operator()(int n, double (&)[1]) const {
+        Inc27 s;
+        node_inc_h27(g, n, s);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double* xc = x + (size_t)d * g.nn;
+            y[n + (size_t)d * g.nn] = h27_const(g, s, ec->Me, [&](int id) { return xc[id]; });
+        }
+    }
+};
+void mass_matvec_h27(Ctx& c, const double* x, double* y) {
+    launch_node<0>(c, c.g3.nn, nullptr, Mass27Body{c.g3, c.eh27_dev, x, y}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------------------ boundary integral
+// F_p[n] = sum over the node's Dirichlet-face slots (bconn order) of sum_b W_f[a][b] (n_f . U_d(b)),
+// W_f[a][b] = sum_q w_q det J_q N_qa N_qb of the quad9 face mapped from its (z, y) coordinates
+__global__ void k_boundary_integral_h27(int nn, const int* __restrict__ off, const int* __restrict__ slot,
+                                        const double* __restrict__ W, const double* __restrict__ nv,
+                                        const int* __restrict__ ui, const double* __restrict__ U_d, double* F_p) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    double acc = 0.0;
+    for (int k = off[n]; k < off[n + 1]; ++k) {
+        const int f = slot[k] / 9, a = slot[k] - 9 * f;
+        double t = 0.0;
+        for (int b = 0; b < 9; ++b) {
+            const int* u = ui + 27 * (size_t)f + 3 * b;
+            t += W[81 * (size_t)f + a * 9 + b] *
+                 (nv[3 * f] * U_d[u[0]] + nv[3 * f + 1] * U_d[u[1]] + nv[3 * f + 2] * U_d[u[2]]);
+        }
+        acc += t;
+    }
+    F_p[n] = acc;
+}
+
+void boundary_integral_h27(Ctx& c, const double* U_d, double* F_p) {
+    const int nn = c.g3.nn;
+    if (!c.fb_off) {
+        launch_fill(c, F_p, 0.0, nn);
+        return;
+    }
+    k_boundary_integral_h27<<<(nn + 255) / 256, 256, 0, c.stream>>>(nn, c.fb_off, c.fb_edge, c.fb_w, c.fb_geom,
+                                                                     c.fb_ui, U_d, F_p);
+    FCFD_CHECK_LAUNCH();
+}
+
+// quad9 3x3 Gauss weights of a face from the (z, y) coordinates of its nodes (quad9 local order):
+// W[a][b] = sum_q w_q det J_q N_qa N_qb
+static void face_weights9(const double (&P)[9][2], double (&W)[81]) {
+    const double g = std::sqrt(3.0 / 5.0);
+    const double x1[3] = {-g, 0.0, g}, w1[3] = {5.0 / 9, 8.0 / 9, 5.0 / 9};
+    const int px[9] = {0, 1, 2, 2, 2, 1, 0, 0, 1}, py[9] = {0, 0, 0, 1, 2, 2, 2, 1, 1};
+    auto L = [](double t, int a) { return a == 0 ? t * (t - 1) / 2 : (a == 1 ? 1 - t * t : t * (t + 1) / 2); };
+    auto dL = [](double t, int a) { return a == 0 ? (2 * t - 1) / 2 : (a == 1 ? -2 * t : (2 * t + 1) / 2); };
+    for (int k = 0; k < 81; ++k) W[k] = 0.0;
+    for (int qe = 0; qe < 3; ++qe)
+        for (int qx = 0; qx < 3; ++qx) {
+            const double xi = x1[qx], eta = x1[qe];
+            double N[9], dxi[9], deta[9];
+            double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
+            for (int b = 0; b < 9; ++b) {
+                N[b] = L(xi, px[b]) * L(eta, py[b]);
+                dxi[b] = dL(xi, px[b]) * L(eta, py[b]);
+                deta[b] = L(xi, px[b]) * dL(eta, py[b]);
+                J00 += dxi[b] * P[b][0]; J01 += dxi[b] * P[b][1];
+                J10 += deta[b] * P[b][0]; J11 += deta[b] * P[b][1];
+            }
+            const double w = w1[qx] * w1[qe] * (J00 * J11 - J01 * J10);
+            for (int a = 0; a < 9; ++a)
+                for (int b = 0; b < 9; ++b) W[a * 9 + b] += w * N[a] * N[b];
+        }
+}
+
+// ------------------------------------------------------------------------------ host set-up
+template <class T>
+static void upload27(const std::vector<T>& h, T* d) {
+    FCFD_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// out[n] = sum over the incident elements of per_local[the node's local index], element order
+static void node_sums_h27(const Grid3D& g, const double (&per_local)[27], std::vector<double>& out) {
+    out.assign(g.nn, 0.0);
+    for (int ke = 0; ke < g.nez; ++ke)
+        for (int je = 0; je < g.ney; ++je)
+            for (int ie = 0; ie < g.nex; ++ie)
+                for (int b = 0; b < 27; ++b)
+                    out[(size_t)(2 * ke + h27z(b)) * g.nxy + (2 * je + h27y(b)) * g.nnx + 2 * ie + h27x(b)] +=
+                        per_local[b];
+}
+
+void build_ctx_h27(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
+    FCFD_REQUIRE(m->nex >= 1 && m->ney >= 1 && m->nez >= 1, "bad element counts");
+    FCFD_REQUIRE(!m->periodic, "3D periodic meshes are not supported (the reference rejects them)");
+    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV,
+                 "hex27 pressure preconditioner must be none, jacobi or approx_inv");
+    FCFD_REQUIRE(c.ce_format == FCFD_CE_F64 || c.ce_format == FCFD_CE_F32, "hex27 Ce formats: FP64 or FP32");
+    FCFD_REQUIRE((int64_t)(2 * m->nex + 1) * (2 * m->ney + 1) * (2 * m->nez + 1) * 3 < (int64_t)1 << 31,
+                 "grid too large");
+    c.dim = 3;
+    c.order = 2;
+    c.g3 = make_grid27(m->nex, m->ney, m->nez);
+    const Grid3D& g = c.g3;
+    c.g = Grid2D{0, 0, 0, 0, g.nn, g.ne, 0};
+    c.lx = m->lx; c.ly = m->ly; c.lz = m->lz;
+    const int nn = g.nn;
+    c.eh27.resize(1);
+    ElemConstH27& ec = c.eh27[0];
+    hex27_constants(c.lx / g.nex, c.ly / g.ney, c.lz / g.nez, ec);
+    c.eh27_dev = dalloc<ElemConstH27>(c, 1);
+    FCFD_HIP(hipMemcpy(c.eh27_dev, &ec, sizeof(ElemConstH27), hipMemcpyHostToDevice));
+    {
+        double rs[27];
+        for (int a = 0; a < 27; ++a) {
+            rs[a] = 0.0;
+            for (int b = 0; b < 27; ++b) rs[a] += ec.Me[a * 27 + b];
+        }
+        std::vector<double> M;
+        node_sums_h27(g, rs, M);
+        M.resize(3 * (size_t)nn);
+        std::copy(M.begin(), M.begin() + nn, M.begin() + nn);
+        std::copy(M.begin(), M.begin() + nn, M.begin() + 2 * (size_t)nn);
+        c.Mlump = dalloc<double>(c, 3 * (size_t)nn);
+        upload27(M, c.Mlump);
+    }
+    c.nlev = 1;
+    c.lv.resize(1);
+    {
+        Level& L = c.lv[0];
+        L.g = c.g;
+        L.g3 = g;
+        double dg[27];
+        for (int a = 0; a < 27; ++a) dg[a] = ec.Le[a * 27 + a];
+        std::vector<double> Ld;
+        node_sums_h27(g, dg, Ld);
+        L.Ld = dalloc<double>(c, nn);
+        upload27(Ld, L.Ld);
+        L.x = dalloc<double>(c, nn);
+        L.y = dalloc<double>(c, nn);
+    }
+    if (c.set_zeroP || c.precond == FCFD_PC_APPROX_INV) {
+        std::vector<uint8_t> pin(nn, 0);
+        if (s->gamma_p && s->n_gamma_p > 0) {
+            for (int64_t k = 0; k < s->n_gamma_p; ++k) {
+                FCFD_REQUIRE(s->gamma_p[k] >= 0 && s->gamma_p[k] < nn, "gamma_p out of range");
+                pin[s->gamma_p[k]] = 1;
+            }
+        } else {
+            pin[0] = 1;
+        }
+        c.ppin = dalloc<uint8_t>(c, nn);
+        upload27(pin, c.ppin);
+    }
+    std::vector<int64_t> gd;
+    if (s->gamma_d && s->n_gamma_d > 0) {
+        gd.assign(s->gamma_d, s->gamma_d + s->n_gamma_d);
+        for (size_t k = 0; k < gd.size(); ++k) {
+            FCFD_REQUIRE(gd[k] >= 0 && gd[k] < 3 * (int64_t)nn, "gamma_d out of range");
+            FCFD_REQUIRE(k == 0 || gd[k] > gd[k - 1], "gamma_d must be sorted and unique");
+        }
+        c.n_gamma_d = (int)gd.size();
+        std::vector<uint8_t> mask(3 * (size_t)nn, 0);
+        for (auto v : gd) mask[v] = 1;
+        c.dmask = dalloc<uint8_t>(c, 3 * (size_t)nn);
+        upload27(mask, c.dmask);
+        c.gamma_d_dev = dalloc<int64_t>(c, gd.size());
+        upload27(gd, c.gamma_d_dev);
+        c.ud_full = dalloc<double>(c, 3 * (size_t)nn);
+        FCFD_HIP(hipMemset(c.ud_full, 0, sizeof(double) * 3 * nn));
+    }
+    // Dirichlet faces (9 nodes each) -> per-node gather lists in bconn order
+    if (s->bconn && s->nvec && s->n_bedge > 0 && !gd.empty()) {
+        const int nf = (int)s->n_bedge;
+        c.n_bedge = nf;
+        std::vector<std::vector<int>> lists(nn);
+        std::vector<double> W(81 * (size_t)nf), geom(3 * (size_t)nf);
+        std::vector<int> ui(27 * (size_t)nf);
+        const double hy = c.ly / g.ney / 2, hz = c.lz / g.nez / 2;
+        auto sorted_index = [&](int64_t v) { return (int)(std::lower_bound(gd.begin(), gd.end(), v) - gd.begin()); };
+        for (int f = 0; f < nf; ++f) {
+            double P[9][2];
+            for (int a = 0; a < 9; ++a) {
+                const int64_t v = s->bconn[9 * (size_t)f + a];
+                FCFD_REQUIRE(v >= 0 && v < nn, "bconn out of range");
+                const int kk = (int)(v / g.nxy), r = (int)(v % g.nxy);
+                P[a][0] = kk * hz;            // z
+                P[a][1] = (r / g.nnx) * hy;   // y
+                for (int d = 0; d < 3; ++d) {
+                    const int idx = sorted_index(v + (int64_t)d * nn);
+                    FCFD_REQUIRE(idx < (int)gd.size() && gd[idx] == v + (int64_t)d * nn,
+                                 "boundary node is not a Dirichlet DOF");
+                    ui[27 * (size_t)f + 3 * a + d] = idx;
+                }
+                lists[v].push_back(f * 9 + a);
+            }
+            double w[81];
+            face_weights9(P, w);
+            for (int k = 0; k < 81; ++k) W[81 * (size_t)f + k] = w[k];
+            for (int d = 0; d < 3; ++d) geom[3 * (size_t)f + d] = (double)s->nvec[3 * (size_t)f + d];
+        }
+        std::vector<int> off(nn + 1, 0), slots;
+        for (int n = 0; n < nn; ++n) {
+            off[n + 1] = off[n] + (int)lists[n].size();
+            slots.insert(slots.end(), lists[n].begin(), lists[n].end());
+        }
+        c.fb_off = dalloc<int>(c, nn + 1);
+        upload27(off, c.fb_off);
+        c.fb_edge = dalloc<int>(c, slots.size());
+        upload27(slots, c.fb_edge);
+        c.fb_w = dalloc<double>(c, W.size());
+        upload27(W, c.fb_w);
+        c.fb_geom = dalloc<double>(c, geom.size());
+        upload27(geom, c.fb_geom);
+        c.fb_ui = dalloc<int>(c, ui.size());
+        upload27(ui, c.fb_ui);
+    }
+    const size_t ne = g.ne;
+    if (c.ce_format == FCFD_CE_F64) c.ce = dalloc<double>(c, 729 * ne);
+    else c.ce = dalloc<float>(c, 729 * ne);
+    const size_t n3 = 3 * (size_t)nn;
+    c.m_r = dalloc<double>(c, n3);
+    c.m_rhat = dalloc<double>(c, n3);
+    for (int k = 0; k < 2; ++k) {
+        c.m_p[k] = dalloc<double>(c, n3);
+        c.m_q[k] = dalloc<double>(c, n3);
+        c.c_p[k] = dalloc<double>(c, nn);
+        c.c_z[k] = dalloc<double>(c, nn);
+    }
+    c.m_s = dalloc<double>(c, n3);
+    c.m_t = dalloc<double>(c, n3);
+    c.m_x = dalloc<double>(c, n3);
+    c.c_r = dalloc<double>(c, nn);
+    c.c_Ap = dalloc<double>(c, nn);
+    c.c_x = dalloc<double>(c, nn);
+    c.c_rhs = dalloc<double>(c, nn);
+    c.ustar = dalloc<double>(c, n3);
+    c.pstride = blocks_for((int)n3) + 1;
+    c.partials = dalloc<double>(c, (size_t)MAX_RED * c.pstride);
+    c.st = dalloc<DevState>(c, 1);
+    FCFD_HIP(hipMemset(c.st, 0, sizeof(DevState)));
+    FCFD_HIP(hipHostMalloc((void**)&c.h_pinned, 64, hipHostMallocDefault));
+    FCFD_HIP(hipDeviceSynchronize());
+}
+
+// ------------------------------------------------------------------------------ the step
+void four_step_h27(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
+                   const double* F_p, const double* F, double* U, double* P, int* it) {
+    scatter_ud(c, U_d);
+    it[0] = solve_momentum_h27(c, U_n, P_n, rho, mu, dt, F, c.m_x);
+    const int nn = c.g3.nn;
+    launch_node<0>(c, nn, nullptr,
+                   Proj27Body{c.g3, c.eh27_dev, c.m_x, P_n, c.Mlump, nullptr, nullptr, c.ustar, dt / rho}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+    it[1] = solve_pressure_h27(c, c.ustar, P_n, F_p, rho, dt, P);
+    launch_node<0>(c, nn, nullptr,
+                   Proj27Body{c.g3, c.eh27_dev, c.ustar, P, c.Mlump, c.ud_full, c.dmask, U, -dt / rho}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
+}  // namespace fcfd

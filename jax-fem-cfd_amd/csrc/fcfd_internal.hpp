@@ -85,6 +85,8 @@ struct Ctx {
     int order = 1;
     ElemConstQ2 eq2{};
     ElemConstQ2* eq2_dev = nullptr;
+    std::vector<ElemConstH27> eh27;       // hex27 contexts: the 27 x 27 constants (host copy)
+    ElemConstH27* eh27_dev = nullptr;
     double tol;
     int set_zeroP, precond, precision, ce_format, maxit_mom, maxit_p;
     int approx_keep = 0;  // approx_inv gamma_p rows: 0 zeroed (standard.py), 1 kept (passive_transport.py)
@@ -271,6 +273,18 @@ void projection_final(Ctx& c, const double* U_star, const double* P, double rho,
 void hex8_constants(double dx, double dy, double dz, ElemConst3& ec);
 void build_ctx3(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s);
 void node_sums3(const Grid3D& g, const double (&per_corner)[8], std::vector<double>& out);
+// hex27 (triquadratic) path (fcfd_hex27.hip)
+void build_ctx_h27(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s);
+void four_step_h27(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
+                   const double* F_p, const double* F, double* U, double* P, int* it);
+int solve_momentum_h27(Ctx& c, const double* U_n, const double* P_n, double rho, double mu, double dt, const double* F,
+                       double* U_hat);
+int solve_pressure_h27(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                       double* P_out);
+int pressure_cg_h27(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged);
+void boundary_integral_h27(Ctx& c, const double* U_d, double* F_p);
+void build_ce_h27_api(Ctx& c, const double* U, double beta, double* Ce);
+void mass_matvec_h27(Ctx& c, const double* x, double* y);
 // 3D multigrid (fcfd_mg3.hip)
 void build_mg3(Ctx& c, const fcfd_solver_desc* s);
 const double* vcycle3(Ctx& c, const double* r0, const int* active);

@@ -286,7 +286,7 @@ __global__ void k_scatter_ud(int n, const int64_t* __restrict__ gd, const double
 void scatter_ud(Ctx& c, const double* U_d) {
     if (c.n_gamma_d == 0) return;
     if (U_d == nullptr) {
-        launch_fill(c, c.ud_full, 0.0, 2 * c.g.nn);
+        launch_fill(c, c.ud_full, 0.0, c.dim * c.g.nn);
         return;
     }
     k_scatter_ud<<<(c.n_gamma_d + 255) / 256, 256, 0, c.stream>>>(c.n_gamma_d, c.gamma_d_dev, U_d, c.ud_full);

@@ -33,10 +33,10 @@ class Context:
         self.ne = self.nex * self.ney
         if self.ndim == 3:
             self.nez, self.lz = int(nez), float(lz)
-            self.nnz = self.nez + 1
+            self.nnz = k * self.nez + 1  # hex27: node grid of 2 nez + 1
             self.nn *= self.nnz
             self.ne *= self.nez
-        self.nen = 9 if self.order == 2 else 2 ** self.ndim  # nodes per element
+        self.nen = (3 if self.order == 2 else 2) ** self.ndim  # nodes per element
         self.tol, self.set_zeroP, self.precision, self.ce_format = tol, bool(set_zeroP), precision, ce_format
         self.precond = precond
         md = nat.MeshDesc(self.ndim, self.nex, self.ney, self.lx, self.ly, int(self.periodic),
@@ -110,10 +110,12 @@ class Context:
 
     def element_constant(self, which):
         if self.order == 2:
-            out = (ctypes.c_double * 81)()
+            k = self.nen
+            out = (ctypes.c_double * (k * k))()
             nat.check(self._lib.fcfd_element_matrix(
-                self._h, {"Me": 0, "Le": 1, "Gx": 2, "Gy": 3, "Le_off": 5}[which], out, 81), "fcfd_element_matrix")
-            return np.array(out[:]).reshape(9, 9)
+                self._h, {"Me": 0, "Le": 1, "Gx": 2, "Gy": 3, "Gz": 4, "Le_off": 5}[which], out, k * k),
+                "fcfd_element_matrix")
+            return np.array(out[:]).reshape(k, k)
         if self.ndim == 3:
             out = (ctypes.c_double * 64)()
             nat.check(self._lib.fcfd_element_constant3(

@@ -45,8 +45,7 @@ def create_periodic_mesh_2d(nex, ney, Lx, Ly):
 
 
 def create_mesh_3d(ne_x, ne_y, ne_z, domain_size_x, domain_size_y, domain_size_z):
-    """regular_mesh.py:45-105 (node k*nnx*nny + j*nnx + i).  Index arrays only:
-    the 3D solver path is not built."""
+    """regular_mesh.py:45-105 (node k*nnx*nny + j*nnx + i)."""
     nnx, nny, nnz = ne_x + 1, ne_y + 1, ne_z + 1
     dx, dy, dz = domain_size_x / ne_x, domain_size_y / ne_y, domain_size_z / ne_z
     k, j, i = np.meshgrid(np.arange(nnz), np.arange(nny), np.arange(nnx), indexing="ij")
@@ -68,6 +67,47 @@ def create_mesh_3d(ne_x, ne_y, ne_z, domain_size_x, domain_size_y, domain_size_z
                  np.tile(nnx * (nny - 1) + np.arange(nnx), nnz) + zx, np.tile(nnx * np.arange(nny), nnz) + zy,
                  np.arange(lay), lay * (nnz - 1) + np.arange(lay)]
     surfnodes = [s.astype(np.int64) for s in surfnodes]
+    normals = np.array([[0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, -1], [0, 0, 1]])
+    nvecs = [np.tile(normals[f], (bconns[f].shape[0], 1)) for f in range(6)]
+    return node_coords, nconn, surfnodes, bconns, nvecs
+
+
+def create_mesh_3d_hex27(ne_x, ne_y, ne_z, domain_size_x, domain_size_y, domain_size_z):
+    """Triquadratic 27-node box mesh (the reference has none: simulation.py:73-74).  Node grid
+    (2 ne + 1) per direction at half spacing, node k*nnx*nny + j*nnx + i; element
+    (ie, je, ke) -> (ke*ne_y + je)*ne_x + ie with local node (c*3 + b)*3 + a at (2ie + a, 2je + b,
+    2ke + c).  Faces [D, R, T, L, F, B] list their nodes as create_mesh_3d does; each face element
+    is a quad9 patch (corners counterclockwise, mid-sides, centre) over (s, t) = (z, y) on x-faces,
+    (x, z) on y-faces, (x, y) on z-faces, first in-face element index fastest."""
+    nnx, nny, nnz = 2 * ne_x + 1, 2 * ne_y + 1, 2 * ne_z + 1
+    hx, hy, hz = domain_size_x / ne_x / 2, domain_size_y / ne_y / 2, domain_size_z / ne_z / 2
+    k, j, i = np.meshgrid(np.arange(nnz), np.arange(nny), np.arange(nnx), indexing="ij")
+    node_coords = np.stack([i * hx, j * hy, k * hz], axis=-1).reshape(-1, 3)
+    lay = nnx * nny
+    ke, je, ie = np.meshgrid(np.arange(ne_z), np.arange(ne_y), np.arange(ne_x), indexing="ij")
+    corner = (2 * ke * lay + 2 * je * nnx + 2 * ie).reshape(-1, 1)
+    loc = np.array([c * lay + b * nnx + a for c in range(3) for b in range(3) for a in range(3)])
+    nconn = (corner + loc[None, :]).astype(np.int64)
+    s9 = np.array([0, 1, 2, 2, 2, 1, 0, 0, 1])
+    t9 = np.array([0, 0, 0, 1, 2, 2, 2, 1, 1])
+
+    def patches(n1, n2, node):
+        e2, e1 = np.meshgrid(np.arange(n2), np.arange(n1), indexing="ij")
+        e1, e2 = e1.reshape(-1, 1), e2.reshape(-1, 1)
+        return node(e1, e2).astype(np.int64)
+    at = lambda x, y, z: z * lay + y * nnx + x
+    bconns = [patches(ne_x, ne_z, lambda a, c: at(2 * a + s9, 0, 2 * c + t9)),
+              patches(ne_y, ne_z, lambda b, c: at(nnx - 1, 2 * b + t9, 2 * c + s9)),
+              patches(ne_x, ne_z, lambda a, c: at(2 * a + s9, nny - 1, 2 * c + t9)),
+              patches(ne_y, ne_z, lambda b, c: at(0, 2 * b + t9, 2 * c + s9)),
+              patches(ne_x, ne_y, lambda a, b: at(2 * a + s9, 2 * b + t9, 0)),
+              patches(ne_x, ne_y, lambda a, b: at(2 * a + s9, 2 * b + t9, nnz - 1))]
+    zx = np.repeat(np.arange(nnz) * lay, nnx)
+    zy = np.repeat(np.arange(nnz) * lay, nny)
+    surfnodes = [np.tile(np.arange(nnx), nnz) + zx, np.tile(nnx * np.arange(1, nny + 1) - 1, nnz) + zy,
+                 np.tile(nnx * (nny - 1) + np.arange(nnx), nnz) + zx, np.tile(nnx * np.arange(nny), nnz) + zy,
+                 np.arange(lay), lay * (nnz - 1) + np.arange(lay)]
+    surfnodes = [x.astype(np.int64) for x in surfnodes]
     normals = np.array([[0, -1, 0], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [0, 0, -1], [0, 0, 1]])
     nvecs = [np.tile(normals[f], (bconns[f].shape[0], 1)) for f in range(6)]
     return node_coords, nconn, surfnodes, bconns, nvecs

@@ -80,8 +80,11 @@ class Config:
         if self.solver == "standard":
             sc = self.solver_config
             if len(self.mesh_config.num_elem) == 3:
-                if self.mesh_config.shape_func_ord == 2:
-                    raise ValueError("3D quadratic shape functions not yet implemented")
+                # hex27 is built here; the reference raises "3D quadratic shape functions not yet
+                # implemented" (simulation.py:73-74).  Its pressure preconditioners are none /
+                # jacobi / approx_inv.
+                if self.mesh_config.shape_func_ord == 2 and sc.pressure_precond == "multigrid":
+                    raise ValueError("multigrid is not built for hex27 meshes")
                 # 3D multigrid is built here (the box-grid V-cycle); the reference raises
                 # "3D multigrid not yet implemented" (simulation.py:75-76)
                 if self.mesh_config.periodic:
@@ -117,7 +120,7 @@ def simple_mesh(mesh_config: SimpleMeshConfig, NS_config: StandardSolverConfig):
         if ndim == 2:
             mesh_func = rm.create_periodic_mesh_2d_quad9 if mesh_config.periodic else rm.create_mesh_2d_quad9
         else:
-            raise ValueError("3D quadratic shape functions not yet implemented")
+            mesh_func = rm.create_mesh_3d_hex27
     node_coords, nconn, surfnodes, bconns, nvecs = mesh_func(*num_elem, *domain_size)
     nn = node_coords.shape[0]
     h = mesh_config.domain_size[0] / mesh_config.num_elem[0]

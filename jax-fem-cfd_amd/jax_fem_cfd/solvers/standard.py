@@ -61,6 +61,15 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
     mc = config.mesh_config
     sc = ns_config if ns_config is not None else config.solver_config
     ce_format = sc.ce_format or ("f64" if sc.precision == "fp64" else "f32")
+    if mc.shape_func_ord == 2 and len(mc.num_elem) == 3:
+        if gamma_d_phi is not None:
+            raise NotImplementedError("the 3D GPU path has no passive scalar")
+        nex, ney, nez = mc.num_elem
+        return Context(nex, ney, mc.domain_size[0], mc.domain_size[1], False, tol=sc.solver_tol,
+                       set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=1, precision=sc.precision,
+                       ce_format=ce_format, gamma_d=gamma_d,
+                       gamma_p=np.atleast_1d(gamma_p) if (sc.set_zeroP or sc.pressure_precond == "approx_inv")
+                       else None, bconn=bconn, nvec=nvec, nez=nez, lz=mc.domain_size[2], order=2)
     if mc.shape_func_ord == 2:
         nex, ney = mc.num_elem
         nlev = _mg_levels(mc.num_elem, sc.final_multigrid_mesh) if sc.pressure_precond == "multigrid" else 1
@@ -74,7 +83,7 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
                        else None, mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, order=2, gamma_d_phi=gamma_d_phi,
                        approx_inv_variant=1 if passive else 0)
     if mc.shape_func_ord != 1:
-        raise NotImplementedError("the GPU timestep is built for quad4, quad9 and hex8 meshes")
+        raise NotImplementedError("the GPU timestep is built for quad4, quad9, hex8 and hex27 meshes")
     if len(mc.num_elem) == 3:
         if gamma_d_phi is not None:
             raise NotImplementedError("the 3D GPU path has no passive scalar")

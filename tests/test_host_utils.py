@@ -41,3 +41,27 @@ def test_host_quad9_meshes_match_oracle():
                 assert np.array_equal(np.asarray(x), y)
             for x, y in zip(a[3], b[3]):
                 assert np.array_equal(np.asarray(x), y)
+
+
+def test_host_hex27_mesh_matches_oracle_bit_exact():
+    """The host hex27 mesh generator (written separately) equals the oracle's bit for bit:
+    coordinates, connectivity, face node lists, quad9 face patches and normals."""
+    from jax_fem_cfd.setup import regular_mesh as rm
+    from oracle import mesh as oms
+    for n in [(3, 2, 2), (1, 1, 1), (4, 3, 2)]:
+        a = oms.create_mesh_3d_hex27(*n, 1.5, 1.0, 0.5)
+        b = rm.create_mesh_3d_hex27(*n, 1.5, 1.0, 0.5)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        for f in range(6):
+            assert np.array_equal(a[2][f], b[2][f])
+            assert np.array_equal(a[3][f], b[3][f])
+            assert np.array_equal(a[4][f], b[4][f])
+
+
+def test_hex27_config_rejects_multigrid():
+    import pytest
+    import jax_fem_cfd.setup.simulation as sim
+    with pytest.raises(ValueError, match="hex27"):
+        sim.Config(mesh="simple", solver="standard", mesh_config=sim.SimpleMeshConfig([4, 2, 2], [2, 1, 1], 2, False),
+                   solver_config=sim.StandardSolverConfig([0, 2, 3, 4, 5], [], [1], False, 1e-8, True, "multigrid",
+                                                          [2, 1, 1]))
