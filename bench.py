@@ -287,6 +287,47 @@ def channel_3d(steps=200, warmup=3):
     return out
 
 
+def channel_3d_q2(steps=3, warmup=1):
+    """Config 4 on triquadratic (hex27) elements: the 128x64x64 channel of channel_3d with Q2
+    velocity and pressure (257x129x129 = 4,276,737 nodes, 27x27 Ce = 3.06 GB FP64 / 1.53 GB FP32),
+    approximate-inverse PCG (hex27 has no multigrid here), tol 1e-8, from rest: `warmup` steps,
+    then `steps` timed steps (each about 0.7 s, so the sample is short)."""
+    import torch
+    import jax_fem_cfd.setup.simulation as sim
+    n, L = [128, 64, 64], [2.0, 1.0, 1.0]
+    out = {}
+    for prec in ("fp64", "mixed"):
+        cfg = sim.Config(mesh="simple", solver="standard", mesh_config=sim.SimpleMeshConfig(n, L, 2, False),
+                         solver_config=sim.StandardSolverConfig([0, 2, 3, 4, 5], [], [1], False, 1e-8, True,
+                                                                "approx_inv", None, prec))
+        compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
+        nodes = surf[3]
+        y, z = coords[nodes, 1], coords[nodes, 2]
+        U_d = sim.set_face_velocity(U_d, nodes, gamma_d, 16 * y * (1 - y) * z * (1 - z), nn, 1)
+        U0 = np.zeros(3 * nn)
+        U0[np.asarray(gamma_d)] = U_d
+        U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+        P = torch.zeros(nn, dtype=torch.float64, device="cuda")
+        F_p, F = compute(U_d, None, 0, 0)
+        for _ in range(warmup):
+            U, P, _, _ = timestep(U, P, 2e-3, 1.0, 1e-3, U_d, F_p, F)
+        torch.cuda.synchronize()
+        its = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            U, P, ui, pi = timestep(U, P, 2e-3, 1.0, 1e-3, U_d, F_p, F)
+            its.append((ui, pi))
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        out[prec] = {"timesteps_per_s": steps / sec, "ms_per_step": 1e3 * sec / steps,
+                     "mean_iters": [float(np.mean([i[0] for i in its])), float(np.mean([i[1] for i in its]))]}
+        del timestep, compute
+    out["config"] = {"workload": f"3D channel 128x64x64 hex27 (Q2), approx_inv PCG, dt=2e-3, steps {warmup + 1}.."
+                                 f"{warmup + steps} from rest",
+                     "nodes": 257 * 129 * 129, "elements": 128 * 64 * 64, "steps": steps}
+    return out
+
+
 def copy_bandwidth(n_bytes=1 << 30, reps=10):
     """Device-to-device copy of n_bytes (read + write counted), timed with events on torch's
     stream: the practical streaming rate the roofline kernel is also compared against."""
@@ -367,6 +408,7 @@ def main():
     poism = poisson_alone("mixed")
     grad = None if args.no_grad else gradient_256()
     ch3d = None if args.no_3d else channel_3d()
+    ch3d_q2 = None if args.no_3d else channel_3d_q2()
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps)
 
     steps_s_m = args.steps / mix["sec"]
@@ -398,6 +440,7 @@ def main():
         "poisson_1024_neumann": {"fp64": pois64, "mixed": poism},
         "gradient_256": grad,
         "channel_3d": ch3d,
+        "channel_3d_q2": ch3d_q2,
         "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_m / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "stencil_kernel<3, BK2S<2, MomCoef<CeF32>, 1, false>> (BiCGSTAB s/t: LDS-staged momentum operator apply + 3 dots)",

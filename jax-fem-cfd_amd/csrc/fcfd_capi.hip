@@ -632,6 +632,7 @@ int fcfd_elem_mult_stored(fcfd_ctx* ctx, const double* K0_16, const double* A1, 
 int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z) {
     if (!ctx || !r || !z) return FCFD_ERR_ARG;
     if (ctx->c.dim == 2 && ctx->c.order == 2) return run(ctx, [&] { vcycle_q2_api(ctx->c, r, z); });
+    if (ctx->c.dim == 3 && ctx->c.order == 2) return run(ctx, [&] { vcycle_h27_api(ctx->c, r, z); });
     if (ctx->c.dim == 3) return run(ctx, [&] { vcycle3_api(ctx->c, r, z); });
     return run2d(ctx, "fcfd_vcycle", [&] { vcycle(ctx->c, r, z); });
 }

@@ -7,7 +7,8 @@
 //     element counts in nex, ney, nez; element (ie, je, ke) -> (ke*ney + je)*nex + ie owns the
 //     nodes (2 ie + a, 2 je + b, 2 ke + c), local index (c*3 + b)*3 + a;
 //   - 3x3x3 Gauss, 27 x 27 element matrices (Ce row-blocked, FP64 or FP32);
-//   - BiCGSTAB momentum (3 components), PCG pressure with none / jacobi / approx_inv;
+//   - BiCGSTAB momentum (3 components), PCG pressure with none / jacobi / approx_inv or the
+//     box-grid V-cycle on hex27 levels (the node-grid transfers of the hex8 hierarchy);
 //   - the boundary integral on quad9 faces, mapped from the (z, y) coordinates of their nodes as
 //     the reference's 3D face integral does (element_calc.py:137-163), so x-faces carry it;
 //   - the mass operator is the per-component product (the reference's 3D consistent_mass_mult
@@ -19,6 +20,8 @@
 #include <cmath>
 #include "fcfd_bicgstab.hpp"
 #include "fcfd_cg.hpp"
+#include "fcfd_mgnodes.hpp"
+#include "fcfd_mg3nodes.hpp"
 
 namespace fcfd {
 
@@ -525,6 +528,176 @@ struct ApproxInv27b {
     }
 };
 
+// ------------------------------------------------------------------------------ multigrid
+// The box-grid V-cycle (multigrid.py:231-330 carried to 3D) on hex27 levels of nex >> l elements:
+// the level operators are the 27 x 27 hex27 Laplacians, the transfers the node-grid trilinear P
+// and R = P^T shared with hex8 (a hex27 node grid of nex elements is a hex8 node grid of 2 nex).
+// The prolongation p = P y is formed by its own kernel (into the level's f, dead by then) and the
+// Jacobi reads it, since a hex27 row spans a 5x5x5 node neighbourhood.
+
+// f = r - L x: pinned (x = 2/3 r / L_diag with the pins 0, pinned rows of L x 0) or Neumann (x
+// and r mean-projected)
+struct DownRes27 {
+    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* x; double* f;
+    const DevState* st; int l; int neumann;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        double fv;
+        if (neumann) {
+            const double mx = st->mg[l][MG_X], mr = st->mg[l][MG_R];
+            fv = (r[n] - mr) - h27_const(g, s, Le, [&](int id) { return x[id] - mx; });
+        } else {
+            auto xv = [&](int id) {
+                const double v = (2.0 / 3.0 * r[id]) / Ld[id];
+                return (pin && pin[id]) ? 0.0 : v;
+            };
+            x[n] = xv(n);
+            const bool pn = pin && pin[n];
+            const double rh = pn ? 0.0 : r[n];
+            const double lx = pn ? 0.0 : h27_const(g, s, Le, xv);
+            fv = rh - lx;
+        }
+        f[n] = fv;
+        acc[0] += fv;
+    }
+};
+
+// x <- x + p (p = P y of the coarser level); one damped Jacobi (multigrid.py:214-228, :306-326)
+struct Jacobi27 {
+    Grid3D g; const double* Leoff; const double* Ld; const uint8_t* pin; const double* r; const double* x;
+    const double* p; double* y; const DevState* st; int l; int neumann;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        Inc27 s;
+        node_inc_h27(g, n, s);
+        const double mx = neumann ? st->mg[l][MG_X] : 0.0, mp = neumann ? st->mg[l][MG_P] : 0.0;
+        auto xh = [&](int id) {
+            if (neumann) return (x[id] - mx) + (p[id] - mp);
+            const double v = x[id] + p[id];
+            return (pin && pin[id]) ? 0.0 : v;
+        };
+        const double rh = neumann ? r[n] - (l == 0 ? 0.0 : st->mg[l][MG_R]) : r[n];
+        const double off = h27_const(g, s, Leoff, xh);
+        double v = (OMEGA * (rh - off)) / Ld[n] + (1.0 - OMEGA) * xh(n);
+        if (!neumann && pin && pin[n]) v = 0.0;
+        y[n] = v;
+        acc[0] += v;
+    }
+};
+
+// Pinned coarse level: CG (M = identity) to tol, maxiter 1000, pinned rows zeroed
+// (multigrid.py:278-287), one workgroup, the search direction in LDS
+__global__ __launch_bounds__(1024) void k_coarse_cg27(Grid3D g, const ElemConstH27* __restrict__ ec,
+                                                      const uint8_t* pin, const double* r, double* x, double* cr,
+                                                      double* cAp, DevState* st, double tol, int maxit,
+                                                      const int* active) {
+    if (active && *active == 0) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double* p = reinterpret_cast<double*>(smem_raw);
+    const int nn = g.nn, bs = blockDim.x;
+    double g2[1] = {0.0};
+    for (int n = threadIdx.x; n < nn; n += bs) {
+        const double b = (pin && pin[n]) ? 0.0 : r[n];
+        cr[n] = b; p[n] = b; x[n] = 0.0;
+        g2[0] += b * b;
+    }
+    block_sum<1>(g2);  // barrier: p complete
+    double gamma = g2[0];
+    const double atol2 = fmax(tol * tol * gamma, 0.0);
+    int k = 0;
+    while (gamma > atol2 && k < maxit) {
+        double pap[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += bs) {
+            Inc27 s;
+            node_inc_h27(g, n, s);
+            const double a = (pin && pin[n]) ? 0.0 : h27_const(g, s, ec->Le, [&](int m) { return p[m]; });
+            cAp[n] = a;
+            pap[0] += p[n] * a;
+        }
+        block_sum<1>(pap);  // barrier: every read of p is done
+        const double alpha = gamma / pap[0];
+        double rr[1] = {0.0};
+        for (int n = threadIdx.x; n < nn; n += bs) {
+            x[n] = x[n] + alpha * p[n];
+            const double rn = cr[n] - alpha * cAp[n];
+            cr[n] = rn;
+            rr[0] += rn * rn;
+        }
+        block_sum<1>(rr);
+        const double beta = rr[0] / gamma;
+        for (int n = threadIdx.x; n < nn; n += bs) p[n] = cr[n] + beta * p[n];
+        __syncthreads();
+        gamma = rr[0];
+        ++k;
+    }
+    for (int n = threadIdx.x; n < nn; n += bs)
+        if (pin && pin[n]) x[n] = 0.0;
+    if (threadIdx.x == 0) st->coarse_k = k;
+}
+
+// r0: the fine residual (its mean in mg[0][MG_R] for the Neumann form); result in lv[0].y
+static const double* vcycle_h27(Ctx& c, const double* r0, const int* active) {
+    const int N = c.nlev;
+    const int neu = !c.set_zeroP;
+    const int ld = N - 1;
+    auto run = [&](int n, const auto& body, const MeanFin& fin) {
+        if (neu) launch_node<1>(c, n, active, body, fin);
+        else launch_node<0>(c, n, active, body, fin);
+    };
+    auto rl = [&](int l) -> const double* { return l == 0 ? r0 : c.lv[l].r; };
+    for (int l = 0; l < ld; ++l) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        const int n = L.g3.nn;
+        if (neu)
+            launch_node<1>(c, n, active, DownX<double, double>{rl(l), L.Ld, L.x, c.st, l},
+                           MeanFin{&c.st->mg[l][MG_X], nullptr, n});
+        run(n, DownRes27{L.g3, L.eh27_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+        run(Lc.g3.nn, Restrict3{L.g3, Lc.g3, L.f, Lc.r, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g3.nn});
+    }
+    {
+        Level& C = c.lv[ld];
+        if (neu) {
+            k_coarse_jacobi<double, double><<<1, 1024, 0, c.stream>>>(C.g3.nn, C.Ld, rl(ld), C.y, c.st, ld, active);
+        } else {
+            const size_t shm = sizeof(double) * (size_t)C.g3.nn;
+            k_coarse_cg27<<<1, 1024, shm, c.stream>>>(C.g3, C.eh27_dev, C.pin, rl(ld), C.y, c.cc_r, c.cc_Ap, c.st,
+                                                      c.tol, 1000, active);
+        }
+        FCFD_CHECK_LAUNCH();
+    }
+    for (int l = ld - 1; l >= 0; --l) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        const int n = L.g3.nn;
+        run(n, Prolong3{L.g3, Lc.g3, Lc.y, L.f, c.st, l, neu}, MeanFin{&c.st->mg[l][MG_P], nullptr, n});
+        run(n, Jacobi27{L.g3, L.eh27_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, L.y, c.st, l, neu},
+            MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
+    }
+    FCFD_CHECK_LAUNCH();
+    return c.lv[0].y;
+}
+
+struct ZOut27 {
+    const double* y; double* z; const DevState* st; int neumann;
+    __device__ void operator()(int k, double (&)[1]) const { z[k] = y[k] - (neumann ? st->mg[0][MG_Y] : 0.0); }
+};
+
+void vcycle_h27_api(Ctx& c, const double* r, double* z) {
+    FCFD_REQUIRE(c.precond == FCFD_PC_MULTIGRID, "context was not created with the multigrid preconditioner");
+    const int nn = c.g3.nn;
+    struct SumBody {
+        const double* r;
+        __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += r[k]; }
+    };
+    launch_node<1>(c, nn, nullptr, SumBody{r}, MeanFin{&c.st->mg[0][MG_R], nullptr, nn});
+    const double* y = vcycle_h27(c, r, nullptr);
+    launch_node<0>(c, nn, nullptr, ZOut27{y, z, c.st, !c.set_zeroP}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
 static const double* precondition_h27(Ctx& c, const int* active) {
     const int nn = c.g3.nn;
     const double* Le = c.eh27_dev->Le;
@@ -538,8 +711,9 @@ static const double* precondition_h27(Ctx& c, const int* active) {
             launch_node<0>(c, nn, active, ApproxInv27b{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y},
                            NoFin{});
             return c.lv[0].y;
+        case FCFD_PC_MULTIGRID: return vcycle_h27(c, c.c_r, active);
     }
-    throw Error(FCFD_ERR_UNSUPPORTED, "hex27 pressure preconditioner must be none, jacobi or approx_inv");
+    throw Error(FCFD_ERR_UNSUPPORTED, "unknown hex27 pressure preconditioner");
 }
 
 // PCG (jax_iterative_solvers.py:87-120); solution in c.c_x
@@ -552,8 +726,9 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
     launch_node<3>(c, nn, nullptr, CgInit27{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
                    CgInitFin{c.st, tol, maxit, nn, 0});
     const int* act = &c.st->c_active;
+    const int neu = c.precond == FCFD_PC_MULTIGRID && !c.set_zeroP;
     const double* y0 = precondition_h27(c, act);
-    launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, y0, c.c_z[0], c.c_p[0], c.st, 0}, CgGamma0Fin{c.st});
+    launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st});
     FCFD_CHECK_LAUNCH();
     const double* Ld = c.lv[0].Ld;
     auto pair = [&]() {
@@ -569,7 +744,7 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
             } else {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
                 const double* y = precondition_h27(c, act);
-                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, 0, 0},
+                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, 0},
                                CgK3Fin{c.st, 0});
             }
         }
@@ -710,8 +885,6 @@ static void node_sums_h27(const Grid3D& g, const double (&per_local)[27], std::v
 void build_ctx_h27(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
     FCFD_REQUIRE(m->nex >= 1 && m->ney >= 1 && m->nez >= 1, "bad element counts");
     FCFD_REQUIRE(!m->periodic, "3D periodic meshes are not supported (the reference rejects them)");
-    FCFD_REQUIRE(c.precond == FCFD_PC_NONE || c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_APPROX_INV,
-                 "hex27 pressure preconditioner must be none, jacobi or approx_inv");
     FCFD_REQUIRE(c.ce_format == FCFD_CE_F64 || c.ce_format == FCFD_CE_F32, "hex27 Ce formats: FP64 or FP32");
     FCFD_REQUIRE((int64_t)(2 * m->nex + 1) * (2 * m->ney + 1) * (2 * m->nez + 1) * 3 < (int64_t)1 << 31,
                  "grid too large");
@@ -741,20 +914,56 @@ void build_ctx_h27(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
         c.Mlump = dalloc<double>(c, 3 * (size_t)nn);
         upload27(M, c.Mlump);
     }
-    c.nlev = 1;
-    c.lv.resize(1);
-    {
-        Level& L = c.lv[0];
-        L.g = c.g;
-        L.g3 = g;
+    // levels: hex27 meshes of nex >> l elements (level 0 always exists: jacobi / approx_inv use it)
+    const int nlev = c.precond == FCFD_PC_MULTIGRID ? s->mg_levels : 1;
+    FCFD_REQUIRE(nlev >= 1 && nlev <= MAX_LEVELS, "bad multigrid level count");
+    FCFD_REQUIRE((g.nex >> (nlev - 1)) << (nlev - 1) == g.nex && (g.ney >> (nlev - 1)) << (nlev - 1) == g.ney &&
+                     (g.nez >> (nlev - 1)) << (nlev - 1) == g.nez && (g.nex >> (nlev - 1)) >= 1 &&
+                     (g.ney >> (nlev - 1)) >= 1 && (g.nez >> (nlev - 1)) >= 1,
+                 "3D multigrid: every element count must halve exactly at each level");
+    c.nlev = nlev;
+    c.lv.resize(nlev);
+    for (int l = 0; l < nlev; ++l) {
+        Level& L = c.lv[l];
+        L.g3 = make_grid27(g.nex >> l, g.ney >> l, g.nez >> l);
+        L.g = Grid2D{0, 0, 0, 0, L.g3.nn, L.g3.ne, 0};
+        std::vector<ElemConstH27> el(1);
+        if (l == 0) el[0] = ec;
+        else hex27_constants(c.lx / L.g3.nex, c.ly / L.g3.ney, c.lz / L.g3.nez, el[0]);
+        L.eh27_dev = dalloc<ElemConstH27>(c, 1);
+        FCFD_HIP(hipMemcpy(L.eh27_dev, el.data(), sizeof(ElemConstH27), hipMemcpyHostToDevice));
         double dg[27];
-        for (int a = 0; a < 27; ++a) dg[a] = ec.Le[a * 27 + a];
+        for (int a = 0; a < 27; ++a) dg[a] = el[0].Le[a * 27 + a];
         std::vector<double> Ld;
-        node_sums_h27(g, dg, Ld);
-        L.Ld = dalloc<double>(c, nn);
+        node_sums_h27(L.g3, dg, Ld);
+        const int n = L.g3.nn;
+        L.Ld = dalloc<double>(c, n);
         upload27(Ld, L.Ld);
-        L.x = dalloc<double>(c, nn);
-        L.y = dalloc<double>(c, nn);
+        if (l > 0) L.r = dalloc<double>(c, n);
+        L.x = dalloc<double>(c, n);
+        L.y = dalloc<double>(c, n);
+        if (nlev > 1 || c.precond == FCFD_PC_MULTIGRID) L.f = dalloc<double>(c, n);
+        if (c.set_zeroP && c.precond == FCFD_PC_MULTIGRID) {
+            std::vector<uint8_t> pin(n, 0);
+            if (s->mg_gamma_p && s->mg_gamma_p_off) {
+                for (int64_t k = s->mg_gamma_p_off[l]; k < s->mg_gamma_p_off[l + 1]; ++k) {
+                    const int64_t v = s->mg_gamma_p[k];
+                    FCFD_REQUIRE(v >= 0 && v < n, "multigrid pin out of range");
+                    pin[v] = 1;
+                }
+            } else {
+                pin[0] = 1;
+            }
+            L.pin = dalloc<uint8_t>(c, n);
+            upload27(pin, L.pin);
+        }
+    }
+    if (c.precond == FCFD_PC_MULTIGRID) {
+        const int ncoarse = c.lv[nlev - 1].g3.nn;
+        FCFD_REQUIRE(!c.set_zeroP || (size_t)ncoarse * sizeof(double) <= 64 * 1024,
+                     "3D multigrid: the pinned coarse grid must have at most 8192 nodes");
+        c.cc_r = dalloc<double>(c, ncoarse);
+        c.cc_Ap = dalloc<double>(c, ncoarse);
     }
     if (c.set_zeroP || c.precond == FCFD_PC_APPROX_INV) {
         std::vector<uint8_t> pin(nn, 0);

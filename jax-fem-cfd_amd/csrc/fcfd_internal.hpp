@@ -42,6 +42,7 @@ struct Level {
     Grid2D g;
     Grid3D g3;                          // 3D contexts: this level's box grid
     ElemConst3* ec3_dev = nullptr;      // 3D contexts: this level's hex8 constants
+    ElemConstH27* eh27_dev = nullptr;   // hex27 contexts: this level's 27 x 27 constants
     ElemConst ec;
     ElemConst* ec_dev = nullptr;
     ElemConstQ2* eq2_dev = nullptr;  // quad9 contexts: this level's 9 x 9 constants
@@ -285,6 +286,7 @@ int pressure_cg_h27(Ctx& c, const double* b, double* x, double tol, int maxit, b
 void boundary_integral_h27(Ctx& c, const double* U_d, double* F_p);
 void build_ce_h27_api(Ctx& c, const double* U, double beta, double* Ce);
 void mass_matvec_h27(Ctx& c, const double* x, double* y);
+void vcycle_h27_api(Ctx& c, const double* r, double* z);
 // 3D multigrid (fcfd_mg3.hip)
 void build_mg3(Ctx& c, const fcfd_solver_desc* s);
 const double* vcycle3(Ctx& c, const double* r0, const int* active);

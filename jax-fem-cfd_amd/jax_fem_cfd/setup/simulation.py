@@ -81,10 +81,7 @@ class Config:
             sc = self.solver_config
             if len(self.mesh_config.num_elem) == 3:
                 # hex27 is built here; the reference raises "3D quadratic shape functions not yet
-                # implemented" (simulation.py:73-74).  Its pressure preconditioners are none /
-                # jacobi / approx_inv.
-                if self.mesh_config.shape_func_ord == 2 and sc.pressure_precond == "multigrid":
-                    raise ValueError("multigrid is not built for hex27 meshes")
+                # implemented" (simulation.py:73-74)
                 # 3D multigrid is built here (the box-grid V-cycle); the reference raises
                 # "3D multigrid not yet implemented" (simulation.py:75-76)
                 if self.mesh_config.periodic:

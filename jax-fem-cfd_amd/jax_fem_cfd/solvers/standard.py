@@ -65,11 +65,16 @@ def build_context(nconn, bconn, nvec, gamma_d, gamma_p, nn, mesh_func, config: C
         if gamma_d_phi is not None:
             raise NotImplementedError("the 3D GPU path has no passive scalar")
         nex, ney, nez = mc.num_elem
+        nlev = _mg_levels(mc.num_elem, sc.final_multigrid_mesh) if sc.pressure_precond == "multigrid" else 1
+        mg_pins = None
+        if sc.pressure_precond == "multigrid" and sc.set_zeroP:
+            mg_pins = _mg_pins(mc.num_elem, mc.domain_size, sc.outlet_faces, None, mesh_func, nlev)
         return Context(nex, ney, mc.domain_size[0], mc.domain_size[1], False, tol=sc.solver_tol,
-                       set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=1, precision=sc.precision,
+                       set_zeroP=sc.set_zeroP, precond=sc.pressure_precond, mg_levels=nlev, precision=sc.precision,
                        ce_format=ce_format, gamma_d=gamma_d,
                        gamma_p=np.atleast_1d(gamma_p) if (sc.set_zeroP or sc.pressure_precond == "approx_inv")
-                       else None, bconn=bconn, nvec=nvec, nez=nez, lz=mc.domain_size[2], order=2)
+                       else None, mg_gamma_p=mg_pins, bconn=bconn, nvec=nvec, nez=nez, lz=mc.domain_size[2],
+                       order=2)
     if mc.shape_func_ord == 2:
         nex, ney = mc.num_elem
         nlev = _mg_levels(mc.num_elem, sc.final_multigrid_mesh) if sc.pressure_precond == "multigrid" else 1

@@ -18,7 +18,8 @@ def multigrid_setup(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, s
     (:12) and both directions halve in lockstep (:80).  Three element counts
     give the hex8 hierarchy (see multigrid_setup_3d)."""
     if len(num_elem) == 3:
-        return multigrid_setup_3d(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, shapefunc)
+        return multigrid_setup_3d(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, shapefunc,
+                                  shape_func_ord)
     nex, ney = num_elem
     nlev = int(np.log2(nex // num_elemf[0])) + 1
     levels = []
@@ -115,7 +116,10 @@ def single_jacobi_iter(x, rhs, Le_off, L_diag, gamma_p, nconn, nn, set_zeroP):
 # are trilinear prolongation and its exact transpose (separable: x, then y, then z).
 # --------------------------------------------------------------------------------------------
 
-def multigrid_setup_3d(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, shapefunc):
+def multigrid_setup_3d(num_elem, num_elemf, domain_size, outlet_faces, mesh_func, shapefunc, shape_func_ord=1):
+    """Box-grid levels; hex27 levels (shape_func_ord 2) keep their node grids (2 ne + 1 per
+    direction), which the node-grid transfers below treat as hex8 grids of 2 ne elements."""
+    k = shape_func_ord
     nex, ney, nez = num_elem
     nlev = int(np.log2(nex // num_elemf[0])) + 1
     levels = []
@@ -126,7 +130,7 @@ def multigrid_setup_3d(num_elem, num_elemf, domain_size, outlet_faces, mesh_func
         outlet = [surf[f] for f in (outlet_faces or [])]
         gamma_p = np.unique(np.concatenate(outlet)).astype(np.int64) if outlet else np.array([0])
         Le = el.laplacian_element_calc(wq, Nderiv)
-        levels.append(dict(ndim=3, nex=nex, ney=ney, nez=nez, nnx=nex + 1, nny=ney + 1, nnz=nez + 1, nn=nn,
+        levels.append(dict(ndim=3, nex=nex, ney=ney, nez=nez, nnx=k * nex + 1, nny=k * ney + 1, nnz=k * nez + 1, nn=nn,
                            nconn=nconn, Le=Le, Le_off=Le - np.diag(np.diag(Le)),
                            L_diag=el.laplacian_diag(Le, nconn, nn), gamma_p=gamma_p))
         nex, ney, nez = nex // 2, ney // 2, nez // 2

@@ -23,14 +23,14 @@ def _rel(a, b):
     return ns.rel_l2(a, b)
 
 
-def _channel(n=(4, 2, 2), L=(2.0, 1.0, 1.0), precond="jacobi", tol=1e-10, precision="fp64"):
+def _channel(n=(4, 2, 2), L=(2.0, 1.0, 1.0), precond="jacobi", tol=1e-10, precision="fp64", final=None):
     import jax_fem_cfd.setup.simulation as sim
     cfg = sim.Config(mesh="simple", solver="standard",
                      mesh_config=sim.SimpleMeshConfig(list(n), list(L), 2, False),
                      solver_config=sim.StandardSolverConfig([0, 2, 3, 4, 5], [], [1], False, tol, True, precond,
-                                                            None, precision))
+                                                            final, precision))
     out = sim.setup(cfg)
-    s, surf, _, _ = ns.build(list(n), list(L), False, [0, 2, 3, 4, 5], [1], tol, True, precond, None,
+    s, surf, _, _ = ns.build(list(n), list(L), False, [0, 2, 3, 4, 5], [1], tol, True, precond, final,
                              shape_func_ord=2)
     compute, timestep, coords, osurf, h, nn, gamma_d, U_d = out
     assert nn == s.nn and np.allclose(coords, s.coords)
@@ -94,7 +94,7 @@ def test_hex27_pressure_cg_matches_oracle():
     assert _rel(_np(x), xo) < 1e-10
 
 
-@pytest.mark.parametrize("precond", ["jacobi", "approx_inv", None])
+@pytest.mark.parametrize("precond", ["jacobi", "approx_inv", None, "multigrid"])
 def test_hex27_channel_steps_match_oracle(precond):
     """4 channel steps from rest, FP64: equal BiCGSTAB and PCG iteration counts every step, U and
     P to 1e-9 relative L2.  The bar is looser than the Q1 paths' 1e-10 because this system
@@ -105,7 +105,8 @@ def test_hex27_channel_steps_match_oracle(precond):
     pressure count is compared within 1 there."""
     n, dt, rho, mu = (4, 2, 2), 2e-3, 1.0, 1e-3
     print(f"\n[case] hex27 channel {n} precond={precond}", flush=True)
-    compute, timestep, s, surf, U_d = _channel(n, precond=precond)
+    compute, timestep, s, surf, U_d = _channel(n, precond=precond, final=[2, 1, 1] if precond == "multigrid"
+                                               else None)
     U = np.zeros(3 * s.nn)
     U[s.gamma_d] = U_d
     P = np.zeros(s.nn)
@@ -160,3 +161,26 @@ def test_hex27_uniform_flow_is_kept():
     F_p, F = compute(U_d, None, 0, 0)
     U, P, ui, pi = timestep(U0, np.zeros(nn), 1e-2, 1.0, 1e-2, U_d, F_p, F)
     assert np.abs(_np(U) - U0).max() < 1e-11 and np.abs(_np(P)).max() < 1e-11
+
+
+@pytest.mark.parametrize("set_zeroP", [True, False])
+def test_vcycle_h27_matches_oracle(set_zeroP):
+    """One V-cycle on hex27 levels (8x4x4 -> 2x1x1 elements, node grids 17x9x9 -> 5x3x3): pinned
+    (outlet face, coarse CG to 1e-12) and pure Neumann, against the oracle's V-cycle."""
+    from jax_fem_cfd._context import Context
+    n, L, nf = [8, 4, 4], [2.0, 1.0, 1.0], [2, 1, 1]
+    outlets = [1] if set_zeroP else []
+    print(f"\n[case] hex27 V-cycle {n} set_zeroP={set_zeroP}", flush=True)
+    s, *_ = ns.build(n, L, False, [0, 2, 3, 4, 5], outlets, 1e-12, set_zeroP, "multigrid", nf, shape_func_ord=2)
+    c = Context(n[0], n[1], L[0], L[1], False, tol=1e-12, set_zeroP=set_zeroP, precond="multigrid",
+                mg_levels=len(s.levels), gamma_p=s.gamma_p if set_zeroP else None,
+                mg_gamma_p=[Lv["gamma_p"] for Lv in s.levels] if set_zeroP else None, nez=n[2], lz=L[2], order=2)
+    r = RNG.standard_normal(s.nn)
+    if set_zeroP:
+        r[s.gamma_p] = 0
+    else:
+        r -= r.mean()
+    ref = s.M_op(r)
+    got = _np(c.vcycle(r))
+    print(f"  rel {_rel(got, ref):.2e}", flush=True)
+    assert _rel(got, ref) < 1e-11

@@ -57,11 +57,3 @@ def test_host_hex27_mesh_matches_oracle_bit_exact():
             assert np.array_equal(a[3][f], b[3][f])
             assert np.array_equal(a[4][f], b[4][f])
 
-
-def test_hex27_config_rejects_multigrid():
-    import pytest
-    import jax_fem_cfd.setup.simulation as sim
-    with pytest.raises(ValueError, match="hex27"):
-        sim.Config(mesh="simple", solver="standard", mesh_config=sim.SimpleMeshConfig([4, 2, 2], [2, 1, 1], 2, False),
-                   solver_config=sim.StandardSolverConfig([0, 2, 3, 4, 5], [], [1], False, 1e-8, True, "multigrid",
-                                                          [2, 1, 1]))
