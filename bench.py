@@ -287,19 +287,20 @@ def channel_3d(steps=200, warmup=3):
     return out
 
 
-def channel_3d_q2(steps=3, warmup=1):
+def channel_3d_q2(warmup=1):
     """Config 4 on triquadratic (hex27) elements: the 128x64x64 channel of channel_3d with Q2
     velocity and pressure (257x129x129 = 4,276,737 nodes, 27x27 Ce = 3.06 GB FP64 / 1.53 GB FP32),
-    approximate-inverse PCG (hex27 has no multigrid here), tol 1e-8, from rest: `warmup` steps,
-    then `steps` timed steps (each about 0.7 s, so the sample is short)."""
+    tol 1e-8, from rest, FP64 and mixed forms.  Pressure preconditioners: approx_inv (3 timed
+    steps, about 0.7 s each) and the box-grid V-cycle on hex27 levels (128x64x64 -> 8x4x4, 5 timed
+    steps); `warmup` untimed steps first."""
     import torch
     import jax_fem_cfd.setup.simulation as sim
     n, L = [128, 64, 64], [2.0, 1.0, 1.0]
-    out = {}
-    for prec in ("fp64", "mixed"):
+
+    def run(prec, precond, final, steps):
         cfg = sim.Config(mesh="simple", solver="standard", mesh_config=sim.SimpleMeshConfig(n, L, 2, False),
                          solver_config=sim.StandardSolverConfig([0, 2, 3, 4, 5], [], [1], False, 1e-8, True,
-                                                                "approx_inv", None, prec))
+                                                                precond, final, prec))
         compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
         nodes = surf[3]
         y, z = coords[nodes, 1], coords[nodes, 2]
@@ -319,12 +320,15 @@ def channel_3d_q2(steps=3, warmup=1):
             its.append((ui, pi))
         torch.cuda.synchronize()
         sec = time.perf_counter() - t0
-        out[prec] = {"timesteps_per_s": steps / sec, "ms_per_step": 1e3 * sec / steps,
-                     "mean_iters": [float(np.mean([i[0] for i in its])), float(np.mean([i[1] for i in its]))]}
-        del timestep, compute
-    out["config"] = {"workload": f"3D channel 128x64x64 hex27 (Q2), approx_inv PCG, dt=2e-3, steps {warmup + 1}.."
-                                 f"{warmup + steps} from rest",
-                     "nodes": 257 * 129 * 129, "elements": 128 * 64 * 64, "steps": steps}
+        return {"timesteps_per_s": steps / sec, "ms_per_step": 1e3 * sec / steps, "steps": steps,
+                "mean_iters": [float(np.mean([i[0] for i in its])), float(np.mean([i[1] for i in its]))]}
+
+    out = {}
+    for precond, final, steps in (("approx_inv", None, 3), ("multigrid", [8, 4, 4], 5)):
+        out[precond] = {prec: run(prec, precond, final, steps) for prec in ("fp64", "mixed")}
+    out["config"] = {"workload": "3D channel 128x64x64 hex27 (Q2), approx_inv and multigrid PCG, dt=2e-3, "
+                                 f"timed from step {warmup + 1} after rest",
+                     "nodes": 257 * 129 * 129, "elements": 128 * 64 * 64}
     return out
 
 

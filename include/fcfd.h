@@ -62,7 +62,10 @@ typedef struct fcfd_mesh_desc {
     double lz;             /* ndim = 3: domain size in z */
     int32_t order;         /* ndim = 2: 1 (or 0) quad4, 2 quad9 (create_mesh_2d_quad9,
                               regular_mesh.py:107-145, :182-218); vectors then live on the
-                              (2 nex + 1) x (2 ney + 1) node grid (2 nex x 2 ney periodic) */
+                              (2 nex + 1) x (2 ney + 1) node grid (2 nex x 2 ney periodic).
+                              ndim = 3: 1 (or 0) hex8, 2 hex27 (no counterpart in the
+                              reference, simulation.py:73-74 rejects it): vectors on the
+                              (2 nex + 1) x (2 ney + 1) x (2 nez + 1) node grid */
 } fcfd_mesh_desc;
 
 typedef struct fcfd_solver_desc {
@@ -81,7 +84,8 @@ typedef struct fcfd_solver_desc {
     const int64_t* mg_gamma_p;      /* per-level pins, concatenated              */
     const int64_t* mg_gamma_p_off;  /* mg_levels + 1 offsets into mg_gamma_p     */
     const int64_t* bconn;       /* (n_bedge, 2) Dirichlet boundary edges; quad9: */
-                                /* (n_bedge, 3); 3D: (n_bedge, 4) faces          */
+                                /* (n_bedge, 3); hex8: (n_bedge, 4) faces;       */
+                                /* hex27: (n_bedge, 9) quad9 faces               */
     const int64_t* nvec;        /* (n_bedge, ndim) outward normals               */
     int64_t n_bedge;
     const int64_t* gamma_d_phi; /* passive-scalar Dirichlet nodes or NULL         */
@@ -126,8 +130,8 @@ int fcfd_element_constant3(const fcfd_ctx* ctx, int32_t which, double* out64);
  * compute (standard.py:184-186).  replaces consistent_mass_mult (element_mult.py:11-36; its
  * ndim = 3 branch fails in the reference, here it is the per-component product). */
 int fcfd_mass_matvec(fcfd_ctx* ctx, const double* x, double* y);
-/* Element matrix of any context, n = nen * nen doubles row-major (quad4 16, quad9 81, hex8 64):
- * which = 0 Me, 1 Le, 2..(1 + ndim) Ge[d], 5 Le_off (2D).  replaces lumped_mass_matrix /
+/* Element matrix of any context, n = nen * nen doubles row-major (quad4 16, quad9 81, hex8 64,
+ * hex27 729): which = 0 Me, 1 Le, 2..(1 + ndim) Ge[d], 5 Le_off (2D, hex27).  replaces lumped_mass_matrix /
  * laplacian_element_calc / gradient_element_calc (element_calc.py:9-40) on the context's
  * element (quad4 / quad9 shape_functions.py:27-45, :108-159; hex8 :47-86). */
 int fcfd_element_matrix(const fcfd_ctx* ctx, int32_t which, double* out, int32_t n);
