@@ -336,14 +336,28 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(int n, const int* __rest
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int n0 = lb * BLOCK * NPT;
     const int nnx = body.nnx();
-    for (int idx = threadIdx.x; idx < 3 * W; idx += BLOCK) {
+    // all of a thread's staging loads are issued before the first LDS store (a fixed trip count,
+    // unrolled), so their latencies overlap instead of adding up
+    constexpr int NIT = (3 * W + BLOCK - 1) / BLOCK;
+    double v[NIT][NV];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int idx = threadIdx.x + it * BLOCK;
         const int seg = idx / W, off = idx - seg * W;
         const int id = n0 + (seg - 1) * nnx - 1 + off;
-        const bool in = id >= 0 && id < n;
-        double v[NV];
-        body.stage(in ? id : 0, v);
+        const bool in = idx < 3 * W && id >= 0 && id < n;
+        body.stage(in ? id : 0, v[it]);
 #pragma unroll
-        for (int q = 0; q < NV; ++q) sm[q][seg][off] = in ? v[q] : 0.0;
+        for (int q = 0; q < NV; ++q) v[it][q] = in ? v[it][q] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int idx = threadIdx.x + it * BLOCK;
+        if (idx < 3 * W) {
+            const int seg = idx / W, off = idx - seg * W;
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sm[q][seg][off] = v[it][q];
+        }
     }
     __syncthreads();
     double acc[NR > 0 ? NR : 1];

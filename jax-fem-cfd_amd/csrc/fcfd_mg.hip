@@ -118,38 +118,55 @@ struct DownResS {
 // neighbours) and restricts f into its coarse nodes.  f never goes to memory: in the pinned form
 // only the restriction reads it.  Values and operation order are those of DownRes + Restrict.
 constexpr int DF_TC = 64;
-template <class T, class TR>
+// DR coarse rows per block: the block stages fine rows 2 J0 - 2 .. 2 J0 + 2 DR (2 DR + 3 rows for
+// 2 DR owned ones), so larger DR reads fewer halo rows twice.  Every staging load of a thread is
+// issued before the first LDS store.
+template <class T, class TR, int DR>
 __global__ __launch_bounds__(256) void k_down_fused(Grid2D gf, Grid2D gc, const double* __restrict__ Le,
                                                     const TR* __restrict__ r, const T* __restrict__ Ld,
                                                     const uint8_t* __restrict__ pin, T* __restrict__ x,
                                                     T* __restrict__ rc, const int* __restrict__ active) {
     if (active && *active == 0) return;
     constexpr int XW = 2 * DF_TC + 3, FW = 2 * DF_TC + 1;
-    __shared__ T xs[5][XW];
-    __shared__ T fs[3][FW];
-    const int J = blockIdx.y, I0 = blockIdx.x * DF_TC;
-    const int fi0 = 2 * I0 - 2, fj0 = 2 * J - 2;
-    for (int t = threadIdx.x; t < 5 * XW; t += blockDim.x) {
+    constexpr int XR = 2 * DR + 3, FR = 2 * DR + 1;
+    __shared__ T xs[XR][XW];
+    __shared__ T fs[FR][FW];
+    // XCD-aware: each XCD takes a contiguous range of coarse rows, so the fine halo rows that
+    // neighbouring blocks both stage are read through the same L2
+    const int nbx = gridDim.x;
+    const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    const int J0 = (lb / nbx) * DR, I0 = (lb - (lb / nbx) * nbx) * DF_TC;
+    const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;
+    constexpr int NIT = (XR * XW + 255) / 256;
+    T v[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int t = threadIdx.x + it * 256;
         const int a = t / XW, b = t - a * XW;
         const int fi = fi0 + b, fj = fj0 + a;
-        T v = T(0);
-        if (fi >= 0 && fi < gf.nnx && fj >= 0 && fj < gf.nny) {
-            const int k = fj * gf.nnx + fi;
-            const T xv = (T(2.0 / 3.0) * (T)r[k]) / Ld[k];
-            v = (pin && pin[k]) ? T(0) : xv;
+        const bool in = t < XR * XW && fi >= 0 && fi < gf.nnx && fj >= 0 && fj < gf.nny;
+        const int k = in ? fj * gf.nnx + fi : 0;
+        const T xv = (T(2.0 / 3.0) * (T)r[k]) / Ld[k];
+        v[it] = (!in || (pin && pin[k])) ? T(0) : xv;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int t = threadIdx.x + it * 256;
+        if (t < XR * XW) {
+            const int a = t / XW, b = t - a * XW;
+            xs[a][b] = v[it];
         }
-        xs[a][b] = v;
     }
     __syncthreads();
-    {
-        const int a = threadIdx.x / (2 * DF_TC), b = threadIdx.x - a * (2 * DF_TC);
-        const int fi = 2 * I0 + b, fj = 2 * J + a;
+    for (int t = threadIdx.x; t < 2 * DR * 2 * DF_TC; t += 256) {
+        const int a = t / (2 * DF_TC), b = t - a * (2 * DF_TC);
+        const int fi = 2 * I0 + b, fj = 2 * J0 + a;
         if (fi < gf.nnx && fj < gf.nny) x[fj * gf.nnx + fi] = xs[a + 2][b + 2];
     }
-    for (int t = threadIdx.x; t < 3 * FW; t += blockDim.x) {
+    for (int t = threadIdx.x; t < FR * FW; t += 256) {
         const int a = t / FW, b = t - a * FW;
-        const int fi = 2 * I0 - 1 + b, fj = 2 * J - 1 + a;
-        T v = T(0);
+        const int fi = 2 * I0 - 1 + b, fj = 2 * J0 - 1 + a;
+        T fv = T(0);
         if (fi >= 0 && fi < gf.nnx && fj >= 0 && fj < gf.nny) {
             const int n = fj * gf.nnx + fi;
             NodeNbr s;
@@ -162,20 +179,34 @@ __global__ __launch_bounds__(256) void k_down_fused(Grid2D gf, Grid2D gc, const 
             const bool pn = pin && pin[n];
             const T rh = pn ? T(0) : (T)r[n];
             const T lx = pn ? T(0) : ebe_const(s, Le, X);
-            v = rh - lx;
+            fv = rh - lx;
         }
-        fs[a][b] = v;
+        fs[a][b] = fv;
     }
     __syncthreads();
-    if (threadIdx.x < DF_TC) {
-        const int I = I0 + threadIdx.x;
-        if (I < gc.nnx) {
-            const int cc = 2 * threadIdx.x + 1;
-            auto val = [&](int dy, int dx) -> T { return fs[dy + 1][cc + dx]; };
+    for (int t = threadIdx.x; t < DR * DF_TC; t += 256) {
+        const int jr = t / DF_TC, ci = t - jr * DF_TC;
+        const int I = I0 + ci, J = J0 + jr;
+        if (I < gc.nnx && J < gc.nny) {
+            const int cc = 2 * ci + 1, rr = 2 * jr + 1;
+            auto val = [&](int dy, int dx) -> T { return fs[rr + dy][cc + dx]; };
             const T E = ((val(1, 0) + val(-1, 0)) + val(0, 1)) + val(0, -1);
             const T D = ((val(1, 1) + val(1, -1)) + val(-1, 1)) + val(-1, -1);
             rc[J * gc.nnx + I] = (val(0, 0) + T(0.5) * E) + T(0.25) * D;
         }
+    }
+}
+
+template <class T, class TR>
+static void launch_down_fused(Ctx& c, const Grid2D& gf, const Grid2D& gc, const double* Le, const TR* r, const T* Ld,
+                              const uint8_t* pin, T* x, T* rc, const int* active) {
+    const int nbx = (gc.nnx + DF_TC - 1) / DF_TC;
+    if (gc.nny >= 200) {
+        constexpr int DR = 4;
+        k_down_fused<T, TR, DR><<<dim3(nbx, (gc.nny + DR - 1) / DR), 256, 0, c.stream>>>(gf, gc, Le, r, Ld, pin, x,
+                                                                                       rc, active);
+    } else {
+        k_down_fused<T, TR, 1><<<dim3(nbx, gc.nny), 256, 0, c.stream>>>(gf, gc, Le, r, Ld, pin, x, rc, active);
     }
 }
 
@@ -258,6 +289,40 @@ struct Jacobi {
         const T off = ebe_const(s, Leoff, X);
         T v = (T(OMEGA) * (rh - off)) / Ld[n] + T(1.0 - OMEGA) * X[1][1];
         if (!neumann && pin && pin[n]) v = T(0);
+        y[n] = v;
+        acc[0] += (double)v;
+    }
+};
+
+// LDS-staged form of the pinned Jacobi (non-periodic grids): x and the pin flags are staged per
+// block window; the operands and operation order are those of Jacobi, so every value is bitwise
+// the same.
+template <class T, class TR>
+struct JacobiS {
+    static constexpr int NV = 2;
+    static constexpr int NPT = 1;
+    Grid2D gf, gc; const double* Leoff; const T* Ld; const uint8_t* pin; const TR* r; const T* x; const T* yc; T* y;
+    __device__ __forceinline__ int nnx() const { return gf.nnx; }
+    __device__ __forceinline__ void stage(int id, double (&v)[2]) const {
+        v[0] = (double)x[id];
+        v[1] = (pin && pin[id]) ? 1.0 : 0.0;
+    }
+    __device__ void operator()(int n, const StageView<2, 1>& sv, double (&acc)[1]) const {
+        NodeNbr s;
+        node_nbr(gf, n, s);
+        T Y[3][3];
+        interp3x3(gc, yc, T(0), s, Y);
+        T X[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const bool keep = s.ok[a][b] && sv.at(1, n, a - 1, b - 1) == 0.0;
+                X[a][b] = keep ? (T)sv.at(0, n, a - 1, b - 1) + Y[a][b] : T(0);
+            }
+        const T off = ebe_const(s, Leoff, X);
+        T v = (T(OMEGA) * ((T)r[n] - off)) / Ld[n] + T(1.0 - OMEGA) * X[1][1];
+        if (sv.at(1, n, 0, 0) != 0.0) v = T(0);
         y[n] = v;
         acc[0] += (double)v;
     }
@@ -570,13 +635,12 @@ static void vcycle_t(Ctx& c, const double* r0, const int* active) {
         Level& Lc = c.lv[l + 1];
         const int n = L.g.nn;
         if (!neu && !L.g.periodic) {
-            const dim3 grid((Lc.g.nnx + DF_TC - 1) / DF_TC, Lc.g.nny);
             if (l == 0)
-                k_down_fused<T, double><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Le, r0, B::Ld(L), L.pin,
-                                                                     B::x(L), B::r(Lc), active);
+                launch_down_fused<T, double>(c, L.g, Lc.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::r(Lc),
+                                             active);
             else
-                k_down_fused<T, T><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Le, B::r(L), B::Ld(L), L.pin,
-                                                                B::x(L), B::r(Lc), active);
+                launch_down_fused<T, T>(c, L.g, Lc.g, L.ec_dev->Le, B::r(L), B::Ld(L), L.pin, B::x(L), B::r(Lc),
+                                        active);
             FCFD_CHECK_LAUNCH();
             continue;
         }
@@ -626,6 +690,19 @@ static void vcycle_t(Ctx& c, const double* r0, const int* active) {
         if (neu)
             launch_node<1>(c, n, active, Prolong<T>{L.g, Lc.g, B::y(Lc), B::f(L), c.st, l},
                            MeanFin{&c.st->mg[l][MG_P], nullptr, n});
+        if (!neu && !L.g.periodic && n >= (1 << 17)) {  // staging pays on the large levels only
+            if (l == 0)
+                launch_stencil<0>(c, n, active,
+                                  JacobiS<T, double>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, r0, B::x(L),
+                                                     B::y(Lc), B::y(L)},
+                                  NoFin{});
+            else
+                launch_stencil<0>(c, n, active,
+                                  JacobiS<T, T>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, B::r(L), B::x(L),
+                                                B::y(Lc), B::y(L)},
+                                  NoFin{});
+            continue;
+        }
         if (l == 0)
             run(n, Jacobi<T, double>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, r0, B::x(L), B::f(L), B::y(Lc),
                                      B::y(L), c.st, l, neu},
