@@ -42,14 +42,16 @@ from .solvers import bicgstab, cg
 
 # ------------------------------------------------------------------------------ building blocks
 def conv_tensor(s, beta):
-    """T[d, I, a, b] = d Ce[a, b] / d U_{I, d} for one element of the uniform grid."""
-    nconn1 = np.arange(4, dtype=np.int64)[None, :]
-    T = np.zeros((s.ndim, 4, 4, 4))
+    """T[d, I, a, b] = d Ce[a, b] / d U_{I, d} for one element of the uniform grid (any element
+    type: nen = nodes per element)."""
+    nen = s.nconn.shape[1]
+    nconn1 = np.arange(nen, dtype=np.int64)[None, :]
+    T = np.zeros((s.ndim, nen, nen, nen))
     for d in range(s.ndim):
-        for i in range(4):
-            U = np.zeros(4 * s.ndim)
-            U[d * 4 + i] = 1.0
-            T[d, i] = el.conservative_convec_elem_calc(nconn1, U, s.wq, s.N, s.Nderiv, 4, s.ndim, beta)[0]
+        for i in range(nen):
+            U = np.zeros(nen * s.ndim)
+            U[d * nen + i] = 1.0
+            T[d, i] = el.conservative_convec_elem_calc(nconn1, U, s.wq, s.N, s.Nderiv, nen, s.ndim, beta)[0]
     return T
 
 
@@ -67,7 +69,7 @@ def conv_grad(s, T, w, x, ncomp):
 
 
 def _elem_mult_T(x, A_all, nconn, nn):
-    """sum_e A_e^T x_e (per-element matrices (ne, 4, 4))."""
+    """sum_e A_e^T x_e (per-element matrices (ne, nen, nen))."""
     return np.bincount(nconn.ravel(), np.einsum("eba,eb->ea", A_all, x[nconn]).ravel(), minlength=nn)
 
 

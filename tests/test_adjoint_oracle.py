@@ -223,3 +223,49 @@ def test_supg_passive_step_vjp_matches_finite_differences():
     fd = _fd(lambda U0: c @ ns.passive_timestep(s, U0, P_n, phi_n, dt, rho, mu, D, U_d, None, 0, 0, None, True)[2],
              [U_n], [dU], 1e-4)
     assert abs(fd - gUn @ dU) <= 1e-6 * abs(fd), (fd, gUn @ dU)
+
+
+# ----------------------------------------------------------------------------- quad9
+def _case_q2(n=4, set_zeroP=True, tol=1e-13):
+    s, surf, _, U_d = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], tol, set_zeroP, "multigrid", [2, 2],
+                               shape_func_ord=2)
+    return s, surf, ns.cavity_lid(s, surf, U_d)
+
+
+def test_quad9_conv_grad_and_transposed_operators():
+    """The same identities on quad9: <w, C(delta) x> = <g(w, x), delta> and <A x, y> = <x, A^T y>."""
+    s, _, _ = _case_q2(3)
+    nn = s.nn
+    T = adj.conv_tensor(s, 0.5)
+    w, x = RNG.standard_normal(2 * nn), RNG.standard_normal(2 * nn)
+    delta = RNG.standard_normal(2 * nn)
+    Ce = el.conservative_convec_elem_calc(s.nconn, delta, s.wq, s.N, s.Nderiv, nn, 2, 0.5)
+    lhs = sum(w[c * nn:(c + 1) * nn] @ emult.scalar_visc_conv_mult(x[c * nn:(c + 1) * nn], Ce, 0 * s.Le, 0.0,
+                                                                     s.nconn, nn) for c in range(2))
+    assert abs(lhs - adj.conv_grad(s, T, w, x, 2) @ delta) <= 1e-12 * (abs(lhs) + 1.0)
+    U = RNG.standard_normal(2 * nn)
+    Ce = el.conservative_convec_elem_calc(s.nconn, U, s.wq, s.N, s.Nderiv, nn, 2, 0.5)
+    A, AT = adj.momentum_ops(s, Ce, 1.0, 1e-2, 1e-2)
+    x, y = RNG.standard_normal(2 * nn), RNG.standard_normal(2 * nn)
+    assert abs(A(x) @ y - x @ AT(y)) <= 1e-12 * abs(A(x) @ y)
+
+
+@pytest.mark.parametrize("set_zeroP", [True, False])
+def test_quad9_ns_step_vjp_matches_finite_differences(set_zeroP):
+    s, _, U_d = _case_q2(4, set_zeroP=set_zeroP)
+    U_n, P_n, _ = _state(s, U_d)
+    nn = s.nn
+    dt, rho, mu = 5e-3, 1.0, 1e-2
+    F_p = el.u_dot_n_boundary_integral_1d(s.bconn, s.nvec, s.gamma_d, U_d, nn, s.coords, el.lin3_shape_functions)
+    a, b = RNG.standard_normal(2 * nn), RNG.standard_normal(nn)
+
+    def loss(U0, P0):
+        U, P, _, _ = ns.four_step_timestep(s, U0, P0, dt, rho, mu, U_d, F_p, 0)
+        return a @ U + b @ P
+
+    gUn, gPn, _, _ = adj.four_step_vjp(s, U_n, P_n, dt, rho, mu, U_d, F_p, 0, a, b)
+    for _ in range(2):
+        dU, dP = RNG.standard_normal(2 * nn), RNG.standard_normal(nn)
+        fd = _fd(loss, [U_n, P_n], [dU, dP], 1e-4)
+        an = gUn @ dU + gPn @ dP
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
