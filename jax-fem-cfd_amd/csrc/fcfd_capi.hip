@@ -760,6 +760,15 @@ int fcfd_four_step_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* 
                                 const double* gP, double* U, double* P, double* gU_n, double* gP_n, int32_t* iters) {
     if (!ctx || !U_n || !P_n || !U || !P || !gU_n || !gP_n) return FCFD_ERR_ARG;
     if (U == U_n || P == P_n || gU_n == U_n || gP_n == P_n) return FCFD_ERR_ARG;
+    if (ctx->c.dim == 2 && ctx->c.order == 2)
+        return run(ctx, [&] {
+            Ctx& c = ctx->c;
+            int it[4];
+            four_step_q2(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
+            ns_vjp_q2(c, U_n, P_n, dt, rho, mu, gU, gP, gU_n, gP_n, it + 2);
+            if (iters)
+                for (int k = 0; k < 4; ++k) iters[k] = it[k];
+        });
     return run2d(ctx, "fcfd_four_step_timestep_vjp", [&] {
         Ctx& c = ctx->c;
         int it[4];

@@ -317,6 +317,8 @@ void boundary_integral_q2(Ctx& c, const double* U_d, double* F_p);
 void mass_matvec_q2(Ctx& c, const double* x, double* y);
 void build_ce_q2_api(Ctx& c, const double* U, double beta, double* Ce);
 void vcycle_q2_api(Ctx& c, const double* r, double* z);
+void ns_vjp_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,
+               const double* gP, double* gU_n, double* gP_n, int* it);
 int solve_scalar_q2(Ctx& c, const double* U_n, const double* U, const double* phi_n, double dt, double D,
                     const double* phi_d, int supg, double* phi);
 

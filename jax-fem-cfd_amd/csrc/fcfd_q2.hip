@@ -30,6 +30,7 @@ __host__ __device__ __forceinline__ size_t q9_idx(int k, int e, int ne) { return
 
 struct Ce9F64 {
     double* c; int ne;
+    __device__ __forceinline__ double ld(int k, int e) const { return c[q9_idx(k, e, ne)]; }
     __device__ __forceinline__ void ld_row(int a, int e, double (&v)[9]) const {
         const double* p = c + q9_idx(a * 9, e, ne);
 #pragma unroll
@@ -39,6 +40,7 @@ struct Ce9F64 {
 };
 struct Ce9F32 {
     float* c; int ne;
+    __device__ __forceinline__ double ld(int k, int e) const { return (double)c[q9_idx(k, e, ne)]; }
     __device__ __forceinline__ void ld_row(int a, int e, double (&v)[9]) const {
         const float* p = c + q9_idx(a * 9, e, ne);
 #pragma unroll
@@ -321,7 +323,8 @@ struct BInitQ2 {
     }
 };
 
-template <int NC, class Op>
+// MT: the masked transpose  A^T y = Ahat^T (mask ? 0 : y) + (mask ? y : 0)  of the adjoint solves
+template <int NC, class Op, bool MT = false>
 struct BK1Q2 {
     Op op;
     const uint8_t* mask;
@@ -339,12 +342,18 @@ struct BK1Q2 {
         IncQ2 s;
         node_inc_q2(g, n, s);
         double y[NC];
-        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); }, pv, y);
+        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); },
+                     [&](int c, int id) {
+                         const double v = pv(c, id);
+                         return (MT && mask && mask[id + c * nn]) ? 0.0 : v;
+                     },
+                     y);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int i = n + c * nn;
             const double pc = pv(c, n);
-            const double yc = (mask && mask[i]) ? pc : y[c];
+            const bool m = mask && mask[i];
+            const double yc = MT ? y[c] + (m ? pc : 0.0) : (m ? pc : y[c]);
             pnew[i] = pc;
             qnew[i] = yc;
             acc[0] += rhat[i] * yc;
@@ -352,7 +361,7 @@ struct BK1Q2 {
     }
 };
 
-template <int NC, class Op>
+template <int NC, class Op, bool MT = false>
 struct BK2Q2 {
     Op op;
     const uint8_t* mask;
@@ -367,12 +376,18 @@ struct BK2Q2 {
         IncQ2 s;
         node_inc_q2(g, n, s);
         double y[NC];
-        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); }, svv, y);
+        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { op.row(a, e, cf); },
+                     [&](int c, int id) {
+                         const double v = svv(c, id);
+                         return (MT && mask && mask[id + c * nn]) ? 0.0 : v;
+                     },
+                     y);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int i = n + c * nn;
             const double sc = svv(c, n);
-            const double yc = (mask && mask[i]) ? sc : y[c];
+            const bool m = mask && mask[i];
+            const double yc = MT ? y[c] + (m ? sc : 0.0) : (m ? sc : y[c]);
             sv[i] = sc;
             tv[i] = yc;
             acc[0] += sc * sc;
@@ -384,7 +399,7 @@ struct BK2Q2 {
 
 // BiCGSTAB iterations after the init kernel (graph-replayed pairs until the device flag says
 // done); returns the reference's iteration count (or -10 / -11)
-template <int NC, class LOp>
+template <int NC, class LOp, bool MT = false>
 static int bicgstab_q2_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
                                const uint8_t* mask, int maxit, double* x, int& pred) {
     const int nn = c.g.nn;
@@ -393,9 +408,10 @@ static int bicgstab_q2_iterate(Ctx& c, IterGraph& graph, const double (&key)[4],
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
             launch_node<1>(c, nn, act,
-                           BK1Q2<NC, LOp>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
+                           BK1Q2<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw],
+                                              c.st},
                            BK1Fin{c.st});
-            launch_node<3>(c, nn, act, BK2Q2<NC, LOp>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+            launch_node<3>(c, nn, act, BK2Q2<NC, LOp, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
                            BK2Fin{c.st});
             launch_node<2>(c, nn, act, BK3<NC>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, x, c.m_r, c.st}, BK3Fin{c.st});
         }
@@ -746,14 +762,14 @@ static const double* precondition_q2(Ctx& c, const int* active) {
 
 // PCG (jax_iterative_solvers.py:87-120); solution in c.c_x
 static int pcg_q2(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol,
-                  int maxit, bool* converged) {
+                  int maxit, bool* converged, bool zero_exit = false) {
     const Grid2D& g = c.g;
     const int nn = g.nn;
     const int neu = c.precond == FCFD_PC_MULTIGRID && !c.set_zeroP;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* Le = c.eq2_dev->Le;
     launch_node<3>(c, nn, nullptr, CgInitQ2{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
-                   CgInitFin{c.st, tol, maxit, nn, 0});
+                   CgInitFin{c.st, tol, maxit, nn, zero_exit});
     const int* act = &c.st->c_active;
     const double* y0 = precondition_q2(c, act);
     launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st});
@@ -1212,6 +1228,282 @@ int solve_scalar_q2(Ctx& c, const double* U_n, const double* U, const double* ph
     if (phi != c.s_x)
         FCFD_HIP(hipMemcpyAsync(phi, c.s_x, sizeof(double) * nn, hipMemcpyDeviceToDevice, c.stream));
     return it;
+}
+
+// ------------------------------------------------------------------------------ reverse mode (NS step)
+// The NS-step VJP of the quad4 path (fcfd_adjoint.hip: every solve differentiated implicitly as
+// lax.custom_linear_solve does, utils/jax_iterative_solvers.py:201-211) on quad9 gathers:
+//   3  gP~ = gP - dt/rho D((mask ? 0 : gU) / M), pinned rows zeroed (or its mean recorded)
+//   4  mu_p = CG(L, gP~) from the forward x0 (CG is symmetric: the transposed solve is the forward)
+//   5  gU* = (mask ? 0 : gU) + rho/dt G mu_p
+//   6  mu_m = A_m^-T gU*  (BiCGSTAB on the masked transpose, from the forward x0)
+//   7  w = mask ? 0 : mu_m;  gU_n = R_m^T w - rho sum_c g_1/2(w_c, theta U_hat_c + (1-theta) U_n,c);
+//      gP_n = D(dt/rho gU* / M - w)
+// g_beta(w, x) is the cotangent of U through <w, Ce(U, beta) x> (Ce is linear in U), gathered per
+// node over its incident elements at the 9 Gauss points.
+
+// transposed rows: row a of the transpose = column a of (cM Me + cL Le + cC Ce_e)
+template <class S>
+struct MomCoefQ2T {
+    Grid2D g;
+    const ElemConstQ2* __restrict__ ec;
+    S ce;
+    double cM, cL, cC;
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[9]) const {
+#pragma unroll
+        for (int b = 0; b < 9; ++b)
+            cf[b] = cM * ec->Me[b * 9 + a] + cL * ec->Le[b * 9 + a] + cC * ce.ld(b * 9 + a, e);
+    }
+};
+
+// x0 = x_n with Dirichlet values, r0 = b - A^T x0 (masked transpose), rhat = p = q = r0;
+// reductions b.b, r0.r0
+template <int NC, class LOp>
+struct BInitVecQ2 {
+    LOp lhs;
+    const double *bvec, *xn, *ud;
+    const uint8_t* mask;
+    double *x, *r, *rhat, *p, *q;
+    __device__ void operator()(int n, double (&acc)[2]) const {
+        const Grid2D& g = lhs.g;
+        const int nn = g.nn;
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        auto x0 = [&](int c, int id) {
+            const int i = id + c * nn;
+            const double xv = xn[i];
+            return (mask && mask[i]) ? ud[i] : xv;
+        };
+        double ax[NC];
+        q2_apply<NC>(g, s, [&](int a, int e, double (&cf)[9]) { lhs.row(a, e, cf); },
+                     [&](int c, int id) {
+                         const double v = x0(c, id);
+                         return (mask && mask[id + c * nn]) ? 0.0 : v;
+                     },
+                     ax);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = n + c * nn;
+            const double xc = x0(c, n);
+            const double a = ax[c] + ((mask && mask[i]) ? xc : 0.0);
+            const double bc = bvec[i];
+            const double rc = bc - a;
+            x[i] = xc;
+            r[i] = rc; rhat[i] = rc; p[i] = rc; q[i] = rc;
+            acc[0] += bc * bc;
+            acc[1] += rc * rc;
+        }
+    }
+};
+
+// candidate k (kx = k & 1, ky = k >> 1) of node (i, j), formed arithmetically (node_inc_q2's rules)
+__device__ __forceinline__ void q2_cand(const Grid2D& g, int i, int j, int k, int& ex, int& ey, int& row, bool& has) {
+    const int kx = k & 1, ky = k >> 1;
+    const bool ox = i & 1, oy = j & 1;
+    ex = kx ? (i >> 1) : (ox ? (i - 1) >> 1 : (i >> 1) - 1);
+    ey = ky ? (j >> 1) : (oy ? (j - 1) >> 1 : (j >> 1) - 1);
+    const int cx = kx ? 0 : (ox ? 1 : 2), cy = ky ? 0 : (oy ? 1 : 2);
+    bool vx, vy;
+    if (g.periodic) {
+        ex = ex < 0 ? ex + g.nex : ex;
+        ey = ey < 0 ? ey + g.ney : ey;
+        vx = kx ? !ox : true;
+        vy = ky ? !oy : true;
+    } else {
+        vx = kx ? (!ox && ex < g.nex) : ex >= 0;
+        vy = ky ? (!oy && ey < g.ney) : ey >= 0;
+    }
+    has = vx && vy;
+    row = q9loc(cx, cy);
+}
+
+// g_beta(w, x) at node n for one component pair: adds into g[2].  The candidate loop is rolled
+// (candidates formed arithmetically): one element's 9 + 9 values are live at a time.
+template <class WF, class XF>
+__device__ __forceinline__ void conv_grad_q2(const Grid2D& g, const ElemConstQ2* __restrict__ ec, int n, WF w, XF x,
+                                             double beta, double (&gd)[2]) {
+    const int j = n / g.nnx, i = n - j * g.nnx;
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        int ex, ey, I;
+        bool has;
+        q2_cand(g, i, j, k, ex, ey, I, has);
+        if (!has) continue;
+        double wv[9], xv[9];
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const int id = q9_node(g, ex, ey, b);
+            wv[b] = w(id);
+            xv[b] = x(id);
+        }
+        double gx = 0.0, gy = 0.0;
+        for (int q = 0; q < 9; ++q) {
+            double wq = 0.0, xq = 0.0, dx = 0.0, dy = 0.0;
+#pragma unroll
+            for (int b = 0; b < 9; ++b) {
+                wq += ec->N[q * 9 + b] * wv[b];
+                xq += ec->N[q * 9 + b] * xv[b];
+                dx += ec->dNx[q * 9 + b] * xv[b];
+                dy += ec->dNy[q * 9 + b] * xv[b];
+            }
+            const double wwq = ec->wq[q] * wq;
+            const double NI = ec->N[q * 9 + I];
+            gx += wwq * (NI * dx + beta * ec->dNx[q * 9 + I] * xq);
+            gy += wwq * (NI * dy + beta * ec->dNy[q * 9 + I] * xq);
+        }
+        gd[0] += gx;
+        gd[1] += gy;
+    }
+}
+
+struct AdjPrhsQ2 {
+    Grid2D g;
+    const ElemConstQ2* __restrict__ ec;
+    const double *gUt, *M, *gP;
+    const uint8_t *dmask, *pin;
+    double* out;
+    double c;  // dt / rho
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        const int nn = g.nn;
+        auto v = [&](int d, int id) {
+            const int i = id + d * nn;
+            const double a = gUt[i] / M[i];
+            return (dmask && dmask[i]) ? 0.0 : a;
+        };
+        const double div = q2_const<true>(g, s, ec->Gx, [&](int id) { return v(0, id); }) +
+                           q2_const<true>(g, s, ec->Gy, [&](int id) { return v(1, id); });
+        double r = (gP ? gP[n] : 0.0) - c * div;
+        if (pin && pin[n]) r = 0.0;
+        out[n] = r;
+        acc[0] += r;
+    }
+};
+
+// (the mean removal of mu_p in the pure-Neumann form is dropped: G maps constants to zero)
+struct AdjUstarQ2 {
+    Grid2D g;
+    const ElemConstQ2* __restrict__ ec;
+    const double *gUt, *mup;
+    const uint8_t *dmask, *pin;
+    const DevState* st;
+    double* gUs;
+    double c;  // rho / dt
+    __device__ void operator()(int n, double (&)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        const bool zero = st->c_zero;
+        auto mv = [&](int id) {
+            const double a = mup[id];
+            return (zero || (pin && pin[id])) ? 0.0 : a;
+        };
+        const double gx = q2_const(g, s, ec->Gx, mv), gy = q2_const(g, s, ec->Gy, mv);
+        const int nn = g.nn;
+        const double u = (dmask && dmask[n]) ? 0.0 : gUt[n];
+        const double v = (dmask && dmask[n + nn]) ? 0.0 : gUt[n + nn];
+        gUs[n] = u + c * gx;
+        gUs[n + nn] = v + c * gy;
+    }
+};
+
+template <class S>
+struct AdjFinalQ2 {
+    Grid2D g;
+    const ElemConstQ2* __restrict__ ec;
+    MomCoefQ2T<S> RT;  // transposed rhs operator (rho/dt) M - (1-theta)(rho Ce + mu Le)
+    const double *mum, *Uh, *Un, *gUs, *M, *gUns;
+    const uint8_t* dmask;
+    double *gUn, *gPn;
+    double rho, theta, c;  // c = dt / rho
+    __device__ void operator()(int n, double (&)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        const int nn = g.nn;
+        auto W = [&](int comp, int id) {
+            const int i = id + comp * nn;
+            const double a = mum[i];
+            return (dmask && dmask[i]) ? 0.0 : a;
+        };
+        double y[2];
+        q2_apply<2>(g, s, [&](int a, int e, double (&cf)[9]) { RT.row(a, e, cf); }, W, y);
+        double gc[2] = {0.0, 0.0};
+#pragma unroll
+        for (int comp = 0; comp < 2; ++comp) {
+            const int o = comp * nn;
+            conv_grad_q2(g, ec, n, [&](int id) { return W(comp, id); },
+                         [&](int id) { return theta * Uh[id + o] + (1.0 - theta) * Un[id + o]; }, 0.5, gc);
+        }
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const int i = n + d * nn;
+            gUn[i] = y[d] - rho * gc[d] + (gUns ? gUns[i] : 0.0);
+        }
+        gPn[n] = q2_const<true>(g, s, ec->Gx, [&](int id) { return c * gUs[id] / M[id] - W(0, id); }) +
+                 q2_const<true>(g, s, ec->Gy, [&](int id) { return c * gUs[id + nn] / M[id + nn] - W(1, id); });
+    }
+};
+
+template <class S>
+static void ns_vjp_q2_t(Ctx& c, S ce, const double* U_n, double rho, double mu, double dt, double* gU_n,
+                        double* gP_n, int* it) {
+    const double theta = 0.5;
+    const Grid2D& g = c.g;
+    const int nn = g.nn;
+    // 6: mu_m = A_m^-T gU*, from the forward x0
+    MomCoefQ2T<S> lhsT{g, c.eq2_dev, ce, rho / dt, theta * mu, theta * rho};
+    const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 2 * nn;
+    launch_node<2>(c, nn, nullptr,
+                   BInitVecQ2<2, MomCoefQ2T<S>>{lhsT, c.a_gus, U_n, c.ud_full, c.dmask, c.a_mx, c.m_r, c.m_rhat,
+                                                c.m_p[0], c.m_q[0]},
+                   BInitVecFin{c.st, c.tol, maxit});
+    launch_node<0>(c, 2 * nn, nullptr, ZeroIfFlag{&c.st->b_zero, c.a_mx}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+    const double key[4] = {rho, mu, dt, 0.0};
+    it[1] = bicgstab_q2_iterate<2, MomCoefQ2T<S>, true>(c, c.g_amom, key, ce.c, lhsT, c.dmask, maxit, c.a_mx,
+                                                         c.pred_mom);
+    // 7: U_n and P_n cotangents
+    MomCoefQ2T<S> RT{g, c.eq2_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
+    launch_node<0>(c, nn, nullptr,
+                   AdjFinalQ2<S>{g, c.eq2_dev, RT, c.a_mx, c.m_x, U_n, c.a_gus, c.Mlump, nullptr, c.dmask, gU_n, gP_n,
+                                 rho, theta, dt / rho},
+                   NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
+// Requires the forward step's intermediates in the context (U_hat in m_x, Ce(U_n, 1/2) in ce).
+// it[0] = pressure adjoint CG iterations, it[1] = momentum adjoint BiCGSTAB iterations.
+void ns_vjp_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,
+               const double* gP, double* gU_n, double* gP_n, int* it) {
+    const int nn = c.g.nn;
+    if (!c.a_gut) {
+        c.a_gut = dalloc<double>(c, 2 * (size_t)nn);
+        c.a_gus = dalloc<double>(c, 2 * (size_t)nn);
+        c.a_mx = dalloc<double>(c, 2 * (size_t)nn);
+    }
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    const double* gUt = gU;
+    if (!gUt) {
+        launch_fill(c, c.a_gut, 0.0, 2 * nn);
+        gUt = c.a_gut;
+    }
+    // 3: pressure adjoint rhs
+    launch_node<1>(c, nn, nullptr, AdjPrhsQ2{c.g, c.eq2_dev, gUt, c.Mlump, gP, c.dmask, pin, c.c_rhs, dt / rho},
+                   MeanFin{&c.st->mean_rhs, nullptr, nn});
+    FCFD_CHECK_LAUNCH();
+    // 4: CG on the forward operator from the forward x0 (P_n, pins zeroed)
+    const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
+    bool conv;
+    it[0] = pcg_q2(c, c.c_rhs, c.set_zeroP ? nullptr : &c.st->mean_rhs, P_n, c.set_zeroP, c.tol, maxit, &conv, true);
+    // 5: cotangent of U_hat
+    launch_node<0>(c, nn, nullptr, AdjUstarQ2{c.g, c.eq2_dev, gUt, c.c_x, c.dmask, pin, c.st, c.a_gus, rho / dt},
+                   NoFin{});
+    FCFD_CHECK_LAUNCH();
+    // 6, 7
+    if (c.ce_format == FCFD_CE_F64)
+        ns_vjp_q2_t(c, Ce9F64{(double*)c.ce, c.g.ne}, U_n, rho, mu, dt, gU_n, gP_n, it);
+    else
+        ns_vjp_q2_t(c, Ce9F32{(float*)c.ce, c.g.ne}, U_n, rho, mu, dt, gU_n, gP_n, it);
 }
 
 // ------------------------------------------------------------------------------ the step

@@ -226,3 +226,75 @@ def test_quad9_passive_transport_matches_oracle(precond, supg, phi_faces):
     assert _rel(_np(U), Uo) < 1e-10 and _rel(_np(P), Po) < 1e-10 and _rel(_np(phi), phio) < 1e-10
     if gamma_d_phi is not None:
         assert np.array_equal(_np(phi)[gamma_d_phi], phi_d)
+
+
+# ----------------------------------------------------------------------------------------- reverse mode
+def _vjp_state(s, U_d, seed=5):
+    r = np.random.default_rng(seed)
+    x, y = s.coords[:, 0], s.coords[:, 1]
+    U = np.concatenate([np.sin(np.pi * x) * np.cos(np.pi * y), -np.cos(np.pi * x) * np.sin(np.pi * y)])
+    U = U + 0.05 * r.standard_normal(U.size)
+    U[s.gamma_d] = U_d
+    return U, 0.1 * r.standard_normal(s.nn)
+
+
+@pytest.mark.parametrize("set_zeroP", [True, False])
+def test_quad9_ns_step_vjp_matches_oracle(set_zeroP):
+    """Reverse-mode product of one quad9 NS step (lid-driven cavity, multigrid PCG; pinned and
+    pure-Neumann pressure) against the oracle's VJP, itself pinned by finite differences
+    (tests/test_adjoint_oracle.py): cotangents to 1e-9, forward and pressure-adjoint iteration
+    counts equal, the momentum-adjoint BiCGSTAB count within 2 (its start is a random cotangent,
+    where rounding decides the last iteration)."""
+    from oracle import adjoint as oadj
+    n, dt, rho, mu = (8, 8), 2e-3, 1.0, 1e-2
+    print(f"\n[case] quad9 NS step VJP {n} set_zeroP={set_zeroP}", flush=True)
+    compute, timestep, s, surf, U_d = _setup(n, precond="multigrid", set_zeroP=set_zeroP, final=[2, 2])
+    U_d = ns.cavity_lid(s, surf, U_d)
+    U_n, P_n = _vjp_state(s, U_d)
+    gU, gP = RNG.standard_normal(2 * s.nn), RNG.standard_normal(s.nn)
+    F_p, F = compute(U_d, None, 0, 0)
+    ctx = timestep.context
+    (gUn, gPn), (U, P), its = ctx.four_step_timestep_vjp(U_n, P_n, dt, rho, mu, gU, gP, U_d, F_p, F)
+    oF_p, _ = ns.compute(s, U_d, None, 0, 0)
+    ogUn, ogPn, fw, (m_it, p_it) = oadj.four_step_vjp(s, U_n, P_n, dt, rho, mu, U_d, oF_p, 0, gU, gP)
+    print(f"  iters {its} oracle {(fw['U_it'], fw['P_it'], p_it, m_it)}  rel gU_n {_rel(_np(gUn), ogUn):.3e}"
+          f"  gP_n {_rel(_np(gPn), ogPn):.3e}", flush=True)
+    assert _rel(_np(U), fw["U"]) < 1e-10 and _rel(_np(P), fw["P"]) < 1e-10
+    assert _rel(_np(gUn), ogUn) < 1e-9 and _rel(_np(gPn), ogPn) < 1e-9
+    assert its[:3] == (fw["U_it"], fw["P_it"], p_it) and abs(its[3] - m_it) <= 2
+
+
+def test_quad9_rollout_gradient_matches_finite_differences_on_gpu():
+    """Gradient of L = a . U_N + b . P_N after 6 quad9 steps (checkpoint 2) through the
+    differentiable rollout (torch autograd over the adjoint sweep) against central differences
+    of the GPU forward rollout, solves at 1e-12."""
+    import torch
+    from jax_fem_cfd.solvers import adjoint as hadj
+    n, nsteps, dt, rho, mu = (8, 8), 6, 2e-3, 1.0, 1e-2
+    print(f"\n[case] quad9 rollout gradient vs FD {n} steps={nsteps}", flush=True)
+    compute, timestep, s, surf, U_d = _setup(n, precond="multigrid", tol=1e-12, final=[2, 2])
+    U_d = ns.cavity_lid(s, surf, U_d)
+    U0, P0 = _vjp_state(s, U_d)
+    ctx = timestep.context
+    F_p, F = compute(U_d, None, 0, 0)
+    dev = torch.device("cuda", 0)
+    a = torch.tensor(RNG.standard_normal(2 * s.nn), dtype=torch.float64, device=dev)
+    b = torch.tensor(RNG.standard_normal(s.nn), dtype=torch.float64, device=dev)
+    P0t = torch.tensor(P0, dtype=torch.float64, device=dev)
+
+    def loss_of(U):
+        Uf, Pf, _ = hadj.rollout(ctx, nsteps, U, P0t, None, dt, rho, mu, U_d=U_d, F_p=F_p, F=F, checkpoint=2)
+        return torch.dot(a, Uf) + torch.dot(b, Pf)
+
+    U0t = torch.tensor(U0, dtype=torch.float64, device=dev, requires_grad=True)
+    loss_of(U0t).backward()
+    g = U0t.grad.detach()
+    with torch.no_grad():
+        for seed in range(2):
+            d = torch.tensor(np.random.default_rng(seed).standard_normal(2 * s.nn), dtype=torch.float64, device=dev)
+            d[torch.as_tensor(s.gamma_d, device=dev)] = 0.0
+            eps = 1e-4
+            fd = (loss_of(U0t.detach() + eps * d) - loss_of(U0t.detach() - eps * d)).item() / (2 * eps)
+            an = torch.dot(g, d).item()
+            print(f"  FD {fd:.12e}  adjoint {an:.12e}", flush=True)
+            assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
