@@ -235,7 +235,8 @@ int fcfd_supg_elem_calc(fcfd_ctx* ctx, const double* U_n, const double* Ce, doub
  * through its (linear) dependence on U_n.  Re-runs the forward step from (U_n, P_n) into U, P
  * (required; must not alias the inputs), then writes gU_n (2nn) and gP_n (nn) for the output
  * cotangents gU (2nn) and gP (nn) (NULL = zero).  The outputs may alias the cotangents.
- * iters (4): forward momentum, forward pressure, adjoint pressure, adjoint momentum. */
+ * iters (4): forward momentum, forward pressure, adjoint pressure, adjoint momentum.
+ * quad4 and quad9 contexts. */
 int fcfd_four_step_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_n, double dt,
                                 double rho, double mu, const double* U_d, const double* F_p,
                                 const double* F, const double* gU, const double* gP, double* U,
