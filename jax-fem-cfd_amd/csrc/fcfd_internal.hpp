@@ -318,7 +318,10 @@ void mass_matvec_q2(Ctx& c, const double* x, double* y);
 void build_ce_q2_api(Ctx& c, const double* U, double beta, double* Ce);
 void vcycle_q2_api(Ctx& c, const double* r, double* z);
 void ns_vjp_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,
-               const double* gP, double* gU_n, double* gP_n, int* it);
+               const double* gP, const double* gUn_extra, double* gU_n, double* gP_n, int* it);
+void passive_vjp_q2(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* phi, double dt,
+                    double rho, double mu, const double* gU, const double* gP, const double* gphi, double* gU_n,
+                    double* gP_n, double* gphi_n, int* it);
 int solve_scalar_q2(Ctx& c, const double* U_n, const double* U, const double* phi_n, double dt, double D,
                     const double* phi_d, int supg, double* phi);
 

@@ -1445,8 +1445,8 @@ struct AdjFinalQ2 {
 };
 
 template <class S>
-static void ns_vjp_q2_t(Ctx& c, S ce, const double* U_n, double rho, double mu, double dt, double* gU_n,
-                        double* gP_n, int* it) {
+static void ns_vjp_q2_t(Ctx& c, S ce, const double* U_n, double rho, double mu, double dt, const double* gUn_extra,
+                        double* gU_n, double* gP_n, int* it) {
     const double theta = 0.5;
     const Grid2D& g = c.g;
     const int nn = g.nn;
@@ -1465,22 +1465,29 @@ static void ns_vjp_q2_t(Ctx& c, S ce, const double* U_n, double rho, double mu, 
     // 7: U_n and P_n cotangents
     MomCoefQ2T<S> RT{g, c.eq2_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
     launch_node<0>(c, nn, nullptr,
-                   AdjFinalQ2<S>{g, c.eq2_dev, RT, c.a_mx, c.m_x, U_n, c.a_gus, c.Mlump, nullptr, c.dmask, gU_n, gP_n,
-                                 rho, theta, dt / rho},
+                   AdjFinalQ2<S>{g, c.eq2_dev, RT, c.a_mx, c.m_x, U_n, c.a_gus, c.Mlump, gUn_extra, c.dmask, gU_n,
+                                 gP_n, rho, theta, dt / rho},
                    NoFin{});
     FCFD_CHECK_LAUNCH();
 }
 
 // Requires the forward step's intermediates in the context (U_hat in m_x, Ce(U_n, 1/2) in ce).
 // it[0] = pressure adjoint CG iterations, it[1] = momentum adjoint BiCGSTAB iterations.
-void ns_vjp_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,
-               const double* gP, double* gU_n, double* gP_n, int* it) {
-    const int nn = c.g.nn;
+static void ensure_adj_q2(Ctx& c) {
     if (!c.a_gut) {
-        c.a_gut = dalloc<double>(c, 2 * (size_t)nn);
-        c.a_gus = dalloc<double>(c, 2 * (size_t)nn);
-        c.a_mx = dalloc<double>(c, 2 * (size_t)nn);
+        const size_t nn = c.g.nn;
+        c.a_sx = dalloc<double>(c, nn);
+        c.a_gut = dalloc<double>(c, 2 * nn);
+        c.a_guns = dalloc<double>(c, 2 * nn);
+        c.a_gus = dalloc<double>(c, 2 * nn);
+        c.a_mx = dalloc<double>(c, 2 * nn);
     }
+}
+
+void ns_vjp_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,
+               const double* gP, const double* gUn_extra, double* gU_n, double* gP_n, int* it) {
+    const int nn = c.g.nn;
+    ensure_adj_q2(c);
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* gUt = gU;
     if (!gUt) {
@@ -1501,9 +1508,92 @@ void ns_vjp_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double r
     FCFD_CHECK_LAUNCH();
     // 6, 7
     if (c.ce_format == FCFD_CE_F64)
-        ns_vjp_q2_t(c, Ce9F64{(double*)c.ce, c.g.ne}, U_n, rho, mu, dt, gU_n, gP_n, it);
+        ns_vjp_q2_t(c, Ce9F64{(double*)c.ce, c.g.ne}, U_n, rho, mu, dt, gUn_extra, gU_n, gP_n, it);
     else
-        ns_vjp_q2_t(c, Ce9F32{(float*)c.ce, c.g.ne}, U_n, rho, mu, dt, gU_n, gP_n, it);
+        ns_vjp_q2_t(c, Ce9F32{(float*)c.ce, c.g.ne}, U_n, rho, mu, dt, gUn_extra, gU_n, gP_n, it);
+}
+
+// ------------------------------------------------------------------------------ reverse mode (passive step)
+// Plain scalar update (passive_transport.py:39-61): with the forward step's stored element matrices
+// A_e, B_e (ade_A, ade_B) and outputs U, phi:
+//   1  mu_s = A_s^-T gphi  (BiCGSTAB on the masked transpose, from the forward x0)
+//   2  w = mask_phi ? 0 : mu_s;  gphi_n = B_s^T w;  gU~ = gU - 1/2 g_1(w, phi);
+//      gU_n,s = -1/2 g_1(w, phi_n)   (A_e, B_e hold +-Ce(U), Ce(U_n) with beta = 1 halved)
+//   then the NS-step VJP with gU~ and the extra U_n cotangent gU_n,s.
+struct StoredQ2T {
+    Grid2D g;
+    Ce9F64 m;
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[9]) const {
+#pragma unroll
+        for (int b = 0; b < 9; ++b) cf[b] = m.ld(b * 9 + a, e);
+    }
+};
+
+struct AdjScalarQ2 {
+    Grid2D g;
+    const ElemConstQ2* __restrict__ ec;
+    StoredQ2T BT;
+    const double *mu, *phi, *phin, *gU;
+    const uint8_t* mphi;
+    double *gphin, *gUt, *gUns;
+    __device__ void operator()(int n, double (&)[1]) const {
+        IncQ2 s;
+        node_inc_q2(g, n, s);
+        auto W = [&](int id) {
+            const double a = mu[id];
+            return (mphi && mphi[id]) ? 0.0 : a;
+        };
+        double y[1];
+        q2_apply<1>(g, s, [&](int a, int e, double (&cf)[9]) { BT.row(a, e, cf); }, [&](int, int id) { return W(id); },
+                    y);
+        gphin[n] = y[0];
+        double g1[2] = {0.0, 0.0}, g1n[2] = {0.0, 0.0};
+        conv_grad_q2(g, ec, n, W, [&](int id) { return phi[id]; }, 1.0, g1);
+        conv_grad_q2(g, ec, n, W, [&](int id) { return phin[id]; }, 1.0, g1n);
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            const int i = n + d * g.nn;
+            gUt[i] = (gU ? gU[i] : 0.0) - 0.5 * g1[d];
+            gUns[i] = -0.5 * g1n[d];
+        }
+    }
+};
+
+// it[0] = scalar adjoint BiCGSTAB, it[1] = pressure adjoint CG, it[2] = momentum adjoint BiCGSTAB
+void passive_vjp_q2(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* phi, double dt,
+                    double rho, double mu, const double* gU, const double* gP, const double* gphi, double* gU_n,
+                    double* gP_n, double* gphi_n, int* it) {
+    ensure_adj_q2(c);
+    const int nn = c.g.nn;
+    const double* gUt = gU;
+    const double* gUns = nullptr;
+    it[0] = 0;
+    if (gphi) {
+        StoredQ2T AT{c.g, Ce9F64{c.ade_A, c.g.ne}};
+        const int maxit = 10 * nn;
+        launch_node<2>(c, nn, nullptr,
+                       BInitVecQ2<1, StoredQ2T>{AT, gphi, phi_n, c.phi_ud, c.phi_mask, c.a_sx, c.m_r, c.m_rhat,
+                                                c.m_p[0], c.m_q[0]},
+                       BInitVecFin{c.st, c.tol, maxit});
+        launch_node<0>(c, nn, nullptr, ZeroIfFlag{&c.st->b_zero, c.a_sx}, NoFin{});
+        FCFD_CHECK_LAUNCH();
+        const double key[4] = {0.0, 0.0, 0.0, 0.0};
+        it[0] = bicgstab_q2_iterate<1, StoredQ2T, true>(c, c.g_aphi, key, c.ade_A, AT, c.phi_mask, maxit, c.a_sx,
+                                                         c.pred_phi);
+        launch_node<0>(c, nn, nullptr,
+                       AdjScalarQ2{c.g, c.eq2_dev, StoredQ2T{c.g, Ce9F64{c.ade_B, c.g.ne}}, c.a_sx, phi, phi_n, gU,
+                                   c.phi_mask, gphi_n, c.a_gut, c.a_guns},
+                       NoFin{});
+        FCFD_CHECK_LAUNCH();
+        gUt = c.a_gut;
+        gUns = c.a_guns;
+    } else {
+        launch_fill(c, gphi_n, 0.0, nn);
+    }
+    int itn[2];
+    ns_vjp_q2(c, U_n, P_n, dt, rho, mu, gUt, gP, gUns, gU_n, gP_n, itn);
+    it[1] = itn[0];
+    it[2] = itn[1];
 }
 
 // ------------------------------------------------------------------------------ the step

@@ -269,3 +269,25 @@ def test_quad9_ns_step_vjp_matches_finite_differences(set_zeroP):
         fd = _fd(loss, [U_n, P_n], [dU, dP], 1e-4)
         an = gUn @ dU + gPn @ dP
         assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+def test_quad9_passive_step_vjp_matches_finite_differences():
+    s, surf, U_d = _case_q2(4)
+    s_p, *_ = ns.build([4, 4], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-13, True, "multigrid", [2, 2],
+                       shape_func_ord=2, passive=True)
+    s = s_p
+    U_n, P_n, phi_n = _state(s, U_d)
+    nn = s.nn
+    dt, rho, mu, D = 5e-3, 1.0, 1e-2, 1e-3
+    a, b, c = RNG.standard_normal(2 * nn), RNG.standard_normal(nn), RNG.standard_normal(nn)
+
+    def loss(U0, P0, f0):
+        U, P, phi, _, _, _ = ns.passive_timestep(s, U0, P0, f0, dt, rho, mu, D, U_d, None, 0, 0, None, False)
+        return a @ U + b @ P + c @ phi
+
+    gUn, gPn, gfn, outs, _ = adj.passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, None, 0, 0, None, a, b, c)
+    for _ in range(2):
+        ds = [RNG.standard_normal(2 * nn), RNG.standard_normal(nn), RNG.standard_normal(nn)]
+        fd = _fd(loss, [U_n, P_n, phi_n], ds, 1e-4)
+        an = gUn @ ds[0] + gPn @ ds[1] + gfn @ ds[2]
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
