@@ -275,6 +275,7 @@ template <int NC, class LOp, bool MT = false>
 struct BK1S {
     static constexpr int NV = NC;
     static constexpr int NPT = 1;
+    static constexpr bool LAZY_STAGE = true;
     LOp op;
     const uint8_t* mask;
     const double *r, *pold, *qold, *rhat;
@@ -323,6 +324,7 @@ template <int NC, class LOp, int TAG = 0, bool MT = false>
 struct BK2S {
     static constexpr int NV = NC;
     static constexpr int NPT = 1;
+    static constexpr bool LAZY_STAGE = true;
     LOp op;
     const uint8_t* mask;
     const double *r, *q;

@@ -5,6 +5,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 #include <cstdint>
+#include <type_traits>
 #include <cstdio>
 #include <string>
 
@@ -326,6 +327,19 @@ __device__ __forceinline__ void node_elems(const Grid2D& g, int n, NodeNbr& s) {
 // Body: static constexpr int NV; static constexpr int NPT; int nnx() const;
 //       void stage(int id, double (&v)[NV]) const;
 //       void operator()(int n, const StageView<NV, NPT>&, double (&acc)[NR]) const.
+// Staging order.  By default a thread issues all of its staging loads before its first LDS store
+// (their latencies overlap: the short stencil kernels gain).  A body that streams much more than
+// it stages (the momentum operator apply, whose element matrices follow the staging) sets
+// LAZY_STAGE: one load-store pair at a time measured faster there.
+template <class B, class = void>
+struct LazyStage {
+    static constexpr bool value = false;
+};
+template <class B>
+struct LazyStage<B, std::void_t<decltype(B::LAZY_STAGE)>> {
+    static constexpr bool value = B::LAZY_STAGE;
+};
+
 // A block owns BLOCK * NPT consecutive nodes; thread t computes nodes n0 + t + m * BLOCK.
 template <int NR, class Body>
 __global__ __launch_bounds__(BLOCK) void stencil_kernel(int n, const int* __restrict__ active, Body body,
@@ -336,27 +350,38 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(int n, const int* __rest
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int n0 = lb * BLOCK * NPT;
     const int nnx = body.nnx();
-    // all of a thread's staging loads are issued before the first LDS store (a fixed trip count,
-    // unrolled), so their latencies overlap instead of adding up
-    constexpr int NIT = (3 * W + BLOCK - 1) / BLOCK;
-    double v[NIT][NV];
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int idx = threadIdx.x + it * BLOCK;
-        const int seg = idx / W, off = idx - seg * W;
-        const int id = n0 + (seg - 1) * nnx - 1 + off;
-        const bool in = idx < 3 * W && id >= 0 && id < n;
-        body.stage(in ? id : 0, v[it]);
-#pragma unroll
-        for (int q = 0; q < NV; ++q) v[it][q] = in ? v[it][q] : 0.0;
-    }
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int idx = threadIdx.x + it * BLOCK;
-        if (idx < 3 * W) {
+    if constexpr (LazyStage<Body>::value) {
+        for (int idx = threadIdx.x; idx < 3 * W; idx += BLOCK) {
             const int seg = idx / W, off = idx - seg * W;
+            const int id = n0 + (seg - 1) * nnx - 1 + off;
+            const bool in = id >= 0 && id < n;
+            double v[NV];
+            body.stage(in ? id : 0, v);
 #pragma unroll
-            for (int q = 0; q < NV; ++q) sm[q][seg][off] = v[it][q];
+            for (int q = 0; q < NV; ++q) sm[q][seg][off] = in ? v[q] : 0.0;
+        }
+    } else {
+        // a fixed trip count, unrolled: all loads in flight before the first LDS store
+        constexpr int NIT = (3 * W + BLOCK - 1) / BLOCK;
+        double v[NIT][NV];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = threadIdx.x + it * BLOCK;
+            const int seg = idx / W, off = idx - seg * W;
+            const int id = n0 + (seg - 1) * nnx - 1 + off;
+            const bool in = idx < 3 * W && id >= 0 && id < n;
+            body.stage(in ? id : 0, v[it]);
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[it][q] = in ? v[it][q] : 0.0;
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int idx = threadIdx.x + it * BLOCK;
+            if (idx < 3 * W) {
+                const int seg = idx / W, off = idx - seg * W;
+#pragma unroll
+                for (int q = 0; q < NV; ++q) sm[q][seg][off] = v[it][q];
+            }
         }
     }
     __syncthreads();
