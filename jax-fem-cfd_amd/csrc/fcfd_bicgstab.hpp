@@ -4,7 +4,9 @@
 //
 // One iteration = 3 kernels (+ their 1-block finalizes):
 //   K1  p = r + beta (p - omega q) (formed on the fly at each stencil node, double-buffered),
-//       q = A p,  alpha = rho' / (rhat . q)
+//       q = A p,  alpha = rho' / (rhat . q).  At the first iteration p0 = q0 = r0 and omega = 1
+//       make the bracket exactly 0, so p = r there: K1 reads r alone (b_k == 0) and the init
+//       kernel writes no p0, q0 copies of r0.
 //   K2  s = r - alpha q (on the fly), t = A s,  exit_early, omega = (t.s)/(t.t)
 //   K3  x, r updates,  breakdown codes -10 / -11, next rho', beta, convergence flag
 // Operators provide  double coef(int a, int b, int e)  (row a, column b of element e's matrix).
@@ -68,6 +70,59 @@ __device__ __forceinline__ void op_apply(const Op& op, const NodeNbr& s, const d
         }
 #pragma unroll
         for (int c = 0; c < NC; ++c) y[c] += s.has[k] ? t[c] : 0.0;
+    }
+}
+
+// The init kernel's two operators (rhs and lhs) over the same stencil position.  Generic form:
+// two passes.  Two momentum forms of one Ce stream (cM Me + cL Le + cC Ce with different
+// coefficients): each Ce row is loaded once and feeds both; every value is formed with the
+// operands and operation order of op_apply, so the results are bitwise those of two passes.
+template <int NC, class LOp, class ROp>
+__device__ __forceinline__ void op_apply_pair(const LOp& lo, const ROp& ro, const NodeNbr& s,
+                                              const double (&XL)[NC][3][3], const double (&XR)[NC][3][3],
+                                              double (&yl)[NC], double (&yr)[NC]) {
+    op_apply<NC>(ro, s, XR, yr);
+    op_apply<NC>(lo, s, XL, yl);
+}
+template <int NC, class S>
+__device__ __forceinline__ void op_apply_pair(const MomCoef<S>& lo, const MomCoef<S>& ro, const NodeNbr& s,
+                                              const double (&XL)[NC][3][3], const double (&XR)[NC][3][3],
+                                              double (&yl)[NC], double (&yr)[NC]) {
+    if (lo.ce.c != ro.ce.c) {
+        op_apply<NC>(ro, s, XR, yr);
+        op_apply<NC>(lo, s, XL, yl);
+        return;
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) { yl[c] = 0.0; yr[c] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int a = kRow(k);
+        const int e = s.e[k];
+        double v[4];
+        lo.ce.ld_row(a, e, v);
+        double cl[4], cr[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            cl[b] = lo.cM * lo.ec->Me[a * 4 + b] + lo.cL * lo.ec->Le[a * 4 + b] + lo.cC * v[b];
+            cr[b] = ro.cM * ro.ec->Me[a * 4 + b] + ro.cL * ro.ec->Le[a * 4 + b] + ro.cC * v[b];
+        }
+        double tl[NC], tr[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) { tl[c] = 0.0; tr[c] = 0.0; }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                tl[c] += cl[b] * XL[c][kCy(k, b)][kCx(k, b)];
+                tr[c] += cr[b] * XR[c][kCy(k, b)][kCx(k, b)];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            yl[c] += s.has[k] ? tl[c] : 0.0;
+            yr[c] += s.has[k] ? tr[c] : 0.0;
+        }
     }
 }
 
@@ -136,7 +191,7 @@ struct BInit {
     Extra extra;
     const double *xn, *ud;
     const uint8_t* mask;
-    double *x, *r, *rhat, *p, *q;
+    double *x, *r, *rhat;
     __device__ void operator()(int n, double (&acc)[2]) const {
         const int nn = lhs.g.nn;
         NodeNbr s;
@@ -155,9 +210,8 @@ struct BInit {
             }
         }
         double b[NC], ax[NC];
-        op_apply<NC>(rhs, s, X, b);
+        op_apply_pair<NC>(lhs, rhs, s, X0, X, ax, b);
         extra(s, n, b);
-        op_apply<NC>(lhs, s, X0, ax);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int i = n + c * nn;
@@ -166,7 +220,7 @@ struct BInit {
             const double a = m ? X0[c][1][1] : ax[c];
             const double rc = bc - a;
             x[i] = X0[c][1][1];
-            r[i] = rc; rhat[i] = rc; p[i] = rc; q[i] = rc;
+            r[i] = rc; rhat[i] = rc;
             acc[0] += bc * bc;
             acc[1] += rc * rc;
         }
@@ -201,10 +255,18 @@ struct BK1 {
         NodeNbr s;
         node_nbr(op.g, n, s);
         double X[NC][3][3];
+        if (st->b_k == 0) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const int o = c * nn;
-            load3x3(s, [&](int k) { return r[k + o] + beta * (pold[k + o] - omega * qold[k + o]); }, X[c]);
+            for (int c = 0; c < NC; ++c) {
+                const int o = c * nn;
+                load3x3(s, [&](int k) { return r[k + o]; }, X[c]);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const int o = c * nn;
+                load3x3(s, [&](int k) { return r[k + o] + beta * (pold[k + o] - omega * qold[k + o]); }, X[c]);
+            }
         }
         double y[NC];
         masked_apply<NC, MT>(op, s, n, nn, mask, X, y);
@@ -283,7 +345,7 @@ struct BK1S {
     const DevState* st;
     __device__ __forceinline__ int nnx() const { return op.g.nnx; }
     __device__ __forceinline__ double pval(int i) const {
-        return r[i] + st->b_beta * (pold[i] - st->b_omega * qold[i]);
+        return st->b_k == 0 ? r[i] : r[i] + st->b_beta * (pold[i] - st->b_omega * qold[i]);
     }
     __device__ __forceinline__ void stage(int id, double (&v)[NV]) const {
         const int nn = op.g.nn;
@@ -417,7 +479,7 @@ struct BInitVec {
     LOp lhs;
     const double *bvec, *xn, *ud;
     const uint8_t* mask;
-    double *x, *r, *rhat, *p, *q;
+    double *x, *r, *rhat;
     __device__ void operator()(int n, double (&acc)[2]) const {
         const int nn = lhs.g.nn;
         NodeNbr s;
@@ -442,7 +504,7 @@ struct BInitVec {
             const double bc = bvec[i];
             const double rc = bc - ax[c];
             x[i] = X0[c][1][1];
-            r[i] = rc; rhat[i] = rc; p[i] = rc; q[i] = rc;
+            r[i] = rc; rhat[i] = rc;
             acc[0] += bc * bc;
             acc[1] += rc * rc;
         }
@@ -521,7 +583,7 @@ int bicgstab_solve(Ctx& c, IterGraph& graph, const double (&key)[4], const void*
                    const ROp& rhs, const Extra& extra, const double* xn, const double* ud, const uint8_t* mask,
                    int maxit, double* x, int& pred) {
     launch_node<2>(c, c.g.nn, nullptr,
-                   BInit<NC, LOp, ROp, Extra>{lhs, rhs, extra, xn, ud, mask, x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                   BInit<NC, LOp, ROp, Extra>{lhs, rhs, extra, xn, ud, mask, x, c.m_r, c.m_rhat},
                    BInitFin{c.st, c.tol, maxit});
     FCFD_CHECK_LAUNCH();
     return bicgstab_iterate<NC, false>(c, graph, key, kptr, lhs, mask, maxit, x, pred);
@@ -534,7 +596,7 @@ int bicgstab_solve_T(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
                      int& pred) {
     const int nn = c.g.nn;
     launch_node<2>(c, nn, nullptr,
-                   BInitVec<NC, true, LOp>{lhsT, b, xn, ud, mask, x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                   BInitVec<NC, true, LOp>{lhsT, b, xn, ud, mask, x, c.m_r, c.m_rhat},
                    BInitVecFin{c.st, c.tol, maxit});
     launch_node<0>(c, NC * nn, nullptr, ZeroIfFlag{&c.st->b_zero, x}, NoFin{});
     FCFD_CHECK_LAUNCH();

@@ -277,12 +277,13 @@ __device__ __forceinline__ void reduce_epilogue(double (&acc)[NR], double* __res
 // fences: visibility comes from the kernel boundary, and results are bitwise reproducible.
 // `active` (device flag, may be null) lets a whole chunk of solver iterations be enqueued ahead:
 // once the solver has converged every later kernel returns at entry.
-template <int NR, class Body>
-__global__ __launch_bounds__(BLOCK) void node_kernel(int n, const int* __restrict__ active, Body body,
-                                                    double* __restrict__ partials, int pstride) {
+// BS: threads per block (a multiple of 64).
+template <int NR, class Body, int BS = BLOCK>
+__global__ __launch_bounds__(BS) void node_kernel(int n, const int* __restrict__ active, Body body,
+                                                 double* __restrict__ partials, int pstride) {
     if (active && *active == 0) return;
     const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int k = lb * BLOCK + threadIdx.x;
+    const int k = lb * BS + threadIdx.x;
     double acc[NR > 0 ? NR : 1];
 #pragma unroll
     for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;

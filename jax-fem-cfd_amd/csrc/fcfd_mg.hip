@@ -625,9 +625,11 @@ static void vcycle_t(Ctx& c, const double* r0, const int* active) {
     const bool f32 = std::is_same<T, float>::value;
     const double ctol = f32 ? (c.tol > 1e-5 ? c.tol : 1e-5) : c.tol;
     const int ld = N - 1;  // levels [0, ld) above the coarse level
-    // pinned form: no mean projections, so its kernels carry no reduction
+    // pinned form: no mean projections, so its kernels carry no reduction (and the small levels
+    // run in 64-thread blocks)
     auto run = [&](int n, const auto& body, const MeanFin& fin) {
         if (neu) launch_node<1>(c, n, active, body, fin);
+        else if (n < (1 << 17)) launch_node_nored<64>(c, n, active, body);
         else launch_node<0>(c, n, active, body, fin);
     };
     for (int l = 0; l < ld; ++l) {
