@@ -214,7 +214,7 @@ def roofline_only(reps):
     y.copy_(x)
     torch.cuda.synchronize()
     ne, nn = N_EL * N_EL, mix["nn"]
-    print(json.dumps({"kernel_us": ms * 1e3, "reps": reps, "bytes_per_launch": 16 * ne * 4 + 8 * nn * 8,
+    print(json.dumps({"kernel_us": ms * 1e3, "reps": reps, "bytes_per_launch": 16 * ne * 4 + 6 * nn * 8,
                       "copy_bytes_each_way": n_copy * 8}))
 
 
@@ -397,13 +397,13 @@ def main():
     def rel(a, b):
         return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
-    # dominant kernel: momentum operator apply (BiCGSTAB s = r - alpha q, t = A s, 3 dots)
+    # dominant kernel: momentum operator apply (BiCGSTAB s = r - alpha q on the fly, t = A s, 3 dots)
     ne = N_EL * N_EL
     reps = 50
     ms_m = mix["ctx"].bench_momentum_apply(reps)
     ms_d = f64["ctx"].bench_momentum_apply(reps)
-    bytes_m = 16 * ne * 4 + 8 * nn * 8      # FP32 Ce + r, q read, s, t written (2 components, FP64)
-    bytes_d = 16 * ne * 8 + 8 * nn * 8
+    bytes_m = 16 * ne * 4 + 6 * nn * 8      # FP32 Ce + r, q read, t written (2 components, FP64)
+    bytes_d = 16 * ne * 8 + 6 * nn * 8
     ach_m = bytes_m / (ms_m * 1e-3) / 1e9
     ach_d = bytes_d / (ms_d * 1e-3) / 1e9
 

@@ -9,6 +9,9 @@
 //       kernel writes no p0, q0 copies of r0.
 //   K2  s = r - alpha q (on the fly), t = A s,  exit_early, omega = (t.s)/(t.t)
 //   K3  x, r updates,  breakdown codes -10 / -11, next rho', beta, convergence flag
+// In the 2D solver s is never stored: K3 forms it again from r and q (bicg_s, one explicit fma,
+// so both kernels hold the same value).  The operator apply then writes one vector (t) per
+// component instead of two, and K3 reads r and q in place of s.
 // Operators provide  double coef(int a, int b, int e)  (row a, column b of element e's matrix).
 //
 // Masking.  The forward operators replace the Dirichlet rows:  A y = mask ? y : Ahat y.  Their
@@ -286,6 +289,9 @@ struct BK1Fin {
 };
 
 // ------------------------------------------------------------------------------ K2
+// s = r - alpha q of K2 and K3
+__device__ __forceinline__ double bicg_s(double r, double alpha, double q) { return __fma_rn(-alpha, q, r); }
+
 // TAG only distinguishes otherwise identical instantiations (the standalone timing of the
 // operator apply runs under its own kernel symbol).
 template <int NC, class LOp, int TAG = 0, bool MT = false>
@@ -293,7 +299,7 @@ struct BK2 {
     LOp op;
     const uint8_t* mask;
     const double *r, *q;
-    double *sv, *tv;
+    double* tv;
     const DevState* st;
     __device__ void operator()(int n, double (&acc)[3]) const {
         const int nn = op.g.nn;
@@ -304,7 +310,7 @@ struct BK2 {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int o = c * nn;
-            load3x3(s, [&](int k) { return r[k + o] - alpha * q[k + o]; }, X[c]);
+            load3x3(s, [&](int k) { return bicg_s(r[k + o], alpha, q[k + o]); }, X[c]);
         }
         double y[NC];
         masked_apply<NC, MT>(op, s, n, nn, mask, X, y);
@@ -313,7 +319,6 @@ struct BK2 {
             const int i = n + c * nn;
             const double yc = y[c];
             const double sc = X[c][1][1];
-            sv[i] = sc;
             tv[i] = yc;
             acc[0] += sc * sc;
             acc[1] += yc * sc;
@@ -390,10 +395,10 @@ struct BK2S {
     LOp op;
     const uint8_t* mask;
     const double *r, *q;
-    double *sv, *tv;
+    double* tv;
     const DevState* st;
     __device__ __forceinline__ int nnx() const { return op.g.nnx; }
-    __device__ __forceinline__ double sval(int i) const { return r[i] - st->b_alpha * q[i]; }
+    __device__ __forceinline__ double sval(int i) const { return bicg_s(r[i], st->b_alpha, q[i]); }
     __device__ __forceinline__ void stage(int id, double (&v)[NV]) const {
         const int nn = op.g.nn;
 #pragma unroll
@@ -422,7 +427,6 @@ struct BK2S {
             const double sc = MT ? sval(i) : X[c][1][1];
             const bool m = mask && mask[i];
             const double yc = MT ? y[c] + (m ? sc : 0.0) : (m ? sc : y[c]);
-            sv[i] = sc;
             tv[i] = yc;
             acc[0] += sc * sc;
             acc[1] += yc * sc;
@@ -466,6 +470,31 @@ struct BK3Fin {
         st->b_rho_new = t[1];
         st->b_beta = t[1] / rho_cur * alpha / omega;
         st->b_active = (t[0] > st->b_atol2) && (k < st->b_maxit) && (k >= 0);
+    }
+};
+
+// K3 of the 2D solver: s formed from r and q (the K2 that precedes it stores no s)
+template <int NC>
+struct BK3R {
+    int nn;
+    const double *p, *q, *tv, *rhat;
+    double *x, *r;
+    const DevState* st;
+    __device__ void operator()(int k, double (&acc)[2]) const {
+        const double alpha = st->b_alpha, omega = st->b_omega;
+        const bool ee = st->b_ee;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int i = k + c * nn;
+            const double pi = p[i], si = bicg_s(r[i], alpha, q[i]), ti = tv[i];
+            double xi = x[i];
+            double ri;
+            if (ee) { xi = xi + alpha * pi; ri = si; }
+            else { xi = xi + (alpha * pi + omega * si); ri = si - omega * ti; }
+            x[i] = xi; r[i] = ri;
+            acc[0] += ri * ri;
+            acc[1] += rhat[i] * ri;
+        }
     }
 };
 
@@ -546,7 +575,7 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
                                BK1<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
                                                 c.m_q[nw], c.st},
                                BK1Fin{c.st});
-                launch_node<3>(c, nn, act, BK2<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                launch_node<3>(c, nn, act, BK2<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st},
                                BK2Fin{c.st});
             } else {
                 launch_stencil<1>(c, nn, act,
@@ -554,10 +583,11 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
                                                     c.m_q[nw], c.st},
                                   BK1Fin{c.st});
                 launch_stencil<3>(c, nn, act,
-                                  BK2S<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                                  BK2S<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st},
                                   BK2Fin{c.st});
             }
-            launch_node<2>(c, nn, act, BK3<NC>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, x, c.m_r, c.st}, BK3Fin{c.st});
+            launch_node<2>(c, nn, act, BK3R<NC>{nn, c.m_p[nw], c.m_q[nw], c.m_t, c.m_rhat, x, c.m_r, c.st},
+                           BK3Fin{c.st});
         }
         FCFD_CHECK_LAUNCH();
     };

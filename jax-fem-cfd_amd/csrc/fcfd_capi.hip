@@ -97,11 +97,14 @@ void upload(const std::vector<T>& h, T* d) {
 
 // Dense inverse of a level's Laplacian with its pinned rows and columns replaced by the identity
 // (the operator the pinned coarse CG inverts: x = 0 at the pins), by Cholesky in FP64.  Returns
-// false if the matrix is not positive definite (no pin on a pure-Neumann level).
+// false if the matrix is not positive definite (no pin on a pure-Neumann level).  Node j * nnx + i
+// couples only to nodes within nnx + 1 of it on a non-periodic grid, so the factor R (A = R^T R)
+// lives in that band and every loop below skips the terms outside it (they are exact zeros).
 constexpr int COARSE_DENSE_MAX = 1089;  // 33 x 33 nodes: 4.7 MB of FP32
 bool coarse_inverse(const Level& L, std::vector<double>& inv) {
     const Grid2D& g = L.g;
     const int n = g.nn;
+    const int w = g.periodic ? n : g.nnx + 1;  // half-bandwidth
     std::vector<double> A((size_t)n * n, 0.0);
     for (int e = 0; e < g.ne; ++e) {
         const int je = e / g.nex, ie = e - je * g.nex;
@@ -115,32 +118,37 @@ bool coarse_inverse(const Level& L, std::vector<double>& inv) {
         for (int k = 0; k < n; ++k) A[(size_t)p * n + k] = A[(size_t)k * n + p] = 0.0;
         A[(size_t)p * n + p] = 1.0;
     }
-    // A = R^T R (upper R in place of the upper triangle)
+    auto lo = [&](int i) { return i - w > 0 ? i - w : 0; };
+    // A = R^T R (upper R in place of the upper triangle, within the band)
     for (int j = 0; j < n; ++j) {
         double d = A[(size_t)j * n + j];
-        for (int k = 0; k < j; ++k) d -= A[(size_t)k * n + j] * A[(size_t)k * n + j];
+        for (int k = lo(j); k < j; ++k) d -= A[(size_t)k * n + j] * A[(size_t)k * n + j];
         if (!(d > 0.0)) return false;
         d = std::sqrt(d);
         A[(size_t)j * n + j] = d;
-        for (int i = j + 1; i < n; ++i) {
+        const int ie = j + w < n ? j + w + 1 : n;
+        for (int i = j + 1; i < ie; ++i) {
             double v = A[(size_t)j * n + i];
-            for (int k = 0; k < j; ++k) v -= A[(size_t)k * n + j] * A[(size_t)k * n + i];
+            for (int k = lo(i); k < j; ++k) v -= A[(size_t)k * n + j] * A[(size_t)k * n + i];
             A[(size_t)j * n + i] = v / d;
         }
     }
     inv.assign((size_t)n * n, 0.0);
     std::vector<double> z(n);
     for (int c = 0; c < n; ++c) {
-        // R^T z = e_c, then R x = z
-        for (int i = 0; i < n; ++i) {
+        // R^T z = e_c (z = 0 above c), then R x = z
+        for (int i = 0; i < c; ++i) z[i] = 0.0;
+        for (int i = c; i < n; ++i) {
             double v = i == c ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) v -= A[(size_t)k * n + i] * z[k];
+            for (int k = lo(i) > c ? lo(i) : c; k < i; ++k) v -= A[(size_t)k * n + i] * z[k];
             z[i] = v / A[(size_t)i * n + i];
         }
+        double* x = &inv[(size_t)c * n];
         for (int i = n - 1; i >= 0; --i) {
             double v = z[i];
-            for (int k = i + 1; k < n; ++k) v -= A[(size_t)i * n + k] * inv[(size_t)c * n + k];
-            inv[(size_t)c * n + i] = v / A[(size_t)i * n + i];
+            const int ke = i + w < n ? i + w + 1 : n;
+            for (int k = i + 1; k < ke; ++k) v -= A[(size_t)i * n + k] * x[k];
+            x[i] = v / A[(size_t)i * n + i];
         }
     }
     return true;

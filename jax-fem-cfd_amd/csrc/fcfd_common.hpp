@@ -251,7 +251,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NR]) {
 // One item (node / element) per thread; logical block lb owns items [lb*BLOCK, lb*BLOCK+BLOCK).
 inline int blocks_for(int n) { return n > 0 ? (n + BLOCK - 1) / BLOCK : 1; }
 
-constexpr int FIN_BLOCK = 1024;  // threads of the one-block finalize kernel
+constexpr int FIN_BLOCK = 512;   // threads of the one-block finalize kernel (measured: 512 > 1024 > 256)
 
 struct NoFin {
     template <class T>

@@ -57,9 +57,10 @@ int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n
 }
 
 // ------------------------------------------------------------------------------ benchmark
-// The momentum operator apply of BiCGSTAB (K2: s = r - alpha q, t = A s, 3 dot partials), run
-// back to back `reps` times under its own kernel symbol (TAG = 1) and timed with HIP events on
-// the context stream.  The node kernel alone: its 1-block finalize is not part of the timing.
+// The momentum operator apply of BiCGSTAB (K2: s = r - alpha q formed on the fly, t = A s stored,
+// 3 dot partials), run back to back `reps` times under its own kernel symbol (TAG = 1) and timed
+// with HIP events on the context stream.  The node kernel alone: its 1-block finalize is not part
+// of the timing.
 template <class S>
 static float bench_t(Ctx& c, S ce, int reps) {
     MomCoef<S> lhs{c.g, c.ec_dev, ce, 2000.0, 0.5e-3, 0.5};
@@ -72,11 +73,11 @@ static float bench_t(Ctx& c, S ce, int reps) {
     auto launch = [&]() {
         if (c.g.periodic)
             node_kernel<3><<<nb, BLOCK, 0, c.stream>>>(
-                c.g.nn, nullptr, BK2<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st},
+                c.g.nn, nullptr, BK2<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_t, c.st},
                 c.partials, c.pstride);
         else
             stencil_kernel<3><<<nbs, BLOCK, 0, c.stream>>>(
-                c.g.nn, nullptr, K{lhs, c.dmask, c.m_r, c.m_q[0], c.m_s, c.m_t, c.st}, c.partials, c.pstride);
+                c.g.nn, nullptr, K{lhs, c.dmask, c.m_r, c.m_q[0], c.m_t, c.st}, c.partials, c.pstride);
     };
     launch();
     FCFD_HIP(hipEventRecord(e0, c.stream));
