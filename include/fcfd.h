@@ -250,7 +250,8 @@ int fcfd_four_step_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* 
  * transposed Ce(U_n) mass term (element_calc.py:188-229).
  * iters (6): forward momentum, pressure, scalar; adjoint scalar, pressure, momentum.
  * The reverse sweep of a rollout (timestep_loop_scan, simulation.py:362-374) calls this once
- * per step, last step first, with the states recomputed from checkpoints. */
+ * per step, last step first, with the states recomputed from checkpoints.  quad4 and quad9
+ * contexts, plain and SUPG. */
 int fcfd_passive_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_n,
                               const double* phi_n, double dt, double rho, double mu, double D,
                               const double* U_d, const double* phi_d, const double* F_p,

@@ -795,16 +795,12 @@ int fcfd_passive_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_
     if (!ctx || !U_n || !P_n || !phi_n || !U || !P || !phi || !gU_n || !gP_n || !gphi_n) return FCFD_ERR_ARG;
     if (U == U_n || P == P_n || phi == phi_n || gU_n == U_n || gP_n == P_n || gphi_n == phi_n) return FCFD_ERR_ARG;
     if (ctx->c.dim == 2 && ctx->c.order == 2) {
-        if (supg) {
-            g_last_error = "fcfd_passive_timestep_vjp: the SUPG scalar update has no quad9 reverse mode";
-            return FCFD_ERR_UNSUPPORTED;
-        }
         return run(ctx, [&] {
             Ctx& c = ctx->c;
             int it[6];
             four_step_q2(c, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, it);
-            it[2] = solve_scalar_q2(c, U_n, U, phi_n, dt, D, phi_d, 0, phi);
-            passive_vjp_q2(c, U_n, P_n, phi_n, phi, dt, rho, mu, gU, gP, gphi, gU_n, gP_n, gphi_n, it + 3);
+            it[2] = solve_scalar_q2(c, U_n, U, phi_n, dt, D, phi_d, supg, phi);
+            passive_vjp_q2(c, U_n, P_n, phi_n, U, phi, dt, rho, mu, D, supg, gU, gP, gphi, gU_n, gP_n, gphi_n, it + 3);
             if (iters)
                 for (int k = 0; k < 6; ++k) iters[k] = it[k];
         });

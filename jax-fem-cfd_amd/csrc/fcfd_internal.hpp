@@ -138,7 +138,7 @@ struct Ctx {
     // adjoint (VJP) work: scalar adjoint solution (nn); U cotangent with the scalar terms, the
     // scalar part of the U_n cotangent, the U_hat cotangent and the momentum adjoint solution (2nn)
     double *a_sx = nullptr, *a_gut = nullptr, *a_guns = nullptr, *a_gus = nullptr, *a_mx = nullptr;
-    double* a_gel = nullptr;  // SUPG element cotangents, 16 per element
+    double* a_gel = nullptr;  // SUPG element cotangents, 16 per element (quad4), 36 (quad9)
     std::vector<void*> allocs;
 };
 
@@ -328,9 +328,9 @@ void build_ce_q2_api(Ctx& c, const double* U, double beta, double* Ce);
 void vcycle_q2_api(Ctx& c, const double* r, double* z);
 void ns_vjp_q2(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* gU,
                const double* gP, const double* gUn_extra, double* gU_n, double* gP_n, int* it);
-void passive_vjp_q2(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* phi, double dt,
-                    double rho, double mu, const double* gU, const double* gP, const double* gphi, double* gU_n,
-                    double* gP_n, double* gphi_n, int* it);
+void passive_vjp_q2(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* U,
+                    const double* phi, double dt, double rho, double mu, double D, int supg, const double* gU,
+                    const double* gP, const double* gphi, double* gU_n, double* gP_n, double* gphi_n, int* it);
 int solve_scalar_q2(Ctx& c, const double* U_n, const double* U, const double* phi_n, double dt, double D,
                     const double* phi_d, int supg, double* phi);
 

@@ -1529,6 +1529,143 @@ struct StoredQ2T {
     }
 };
 
+// Reverse-mode product of the quad9 SUPG element terms: the quad4 kernel's formulas
+// (k_supg_adj_elem, fcfd_scalar.hip) with 9 nodes and 9 Gauss points.  With w the masked scalar
+// adjoint, d = phi_n - phi and s = phi_n + phi, per element
+//   tau-explicit:  d/dU_n [tau d^T Ce(U_n) w / dt - tau w^T Ct(U_n) s / 2]
+//   through tau:   dl/dtau = d^T Ce(U_n) w / dt - w^T Ct s / 2,  tau = S^(-1/2),
+//                  S = (|Ct|/|Ce|)^2 + (2/dt)^2 + ((D+1e-6)|Ct|^2 / (u2 |Ce|^2))^2,
+//                  |Ce| = |Ce(U)|, |Ct| = |Ct(U_n)|, u2 = mean over the 9 points of |U_n,q|^2
+//                  (so du2/dU_n,I = 2/9 sum_q N_I U_n,q).
+// The plain convection terms are gathered at the nodes (conv_grad_q2).  Ce(U) and Ct(U_n) are
+// formed once per element (81 entries each).  Output gel[((v * 9 + I) * 2 + d) * ne + e], v = 0
+// for U_n, 1 for U.
+__global__ __launch_bounds__(BLOCK) void k_supg_adj_elem_q2(Grid2D g, const ElemConstQ2* __restrict__ ec,
+                                                           const double* __restrict__ Un,
+                                                           const double* __restrict__ U,
+                                                           const double* __restrict__ mu,
+                                                           const uint8_t* __restrict__ mphi,
+                                                           const double* __restrict__ phi,
+                                                           const double* __restrict__ phin, double D, double dt,
+                                                           double* __restrict__ gel) {
+    const int e = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
+    if (e >= g.ne) return;
+    const int je = e / g.nex, ie = e - je * g.nex;
+    double un[9], vn[9], u[9], v[9], w[9], dl[9], sm[9];
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+        const int id = q9_node(g, ie, je, b);
+        un[b] = Un[id]; vn[b] = Un[id + g.nn];
+        u[b] = U[id]; v[b] = U[id + g.nn];
+        const double m = mu[id];
+        w[b] = (mphi && mphi[id]) ? 0.0 : m;
+        dl[b] = phin[id] - phi[id];
+        sm[b] = phin[id] + phi[id];
+    }
+    double uq[9], vq[9], dq[9], unq[9], vnq[9], dnq[9];
+#pragma unroll 1
+    for (int q = 0; q < 9; ++q) {
+        double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+        for (int b = 0; b < 9; ++b) {
+            const double Nb = ec->N[q * 9 + b], dx = ec->dNx[q * 9 + b], dy = ec->dNy[q * 9 + b];
+            a0 += Nb * u[b]; a1 += Nb * v[b]; a2 += dx * u[b] + dy * v[b];
+            b0 += Nb * un[b]; b1 += Nb * vn[b]; b2 += dx * un[b] + dy * vn[b];
+        }
+        uq[q] = a0; vq[q] = a1; dq[q] = a2;
+        unq[q] = b0; vnq[q] = b1; dnq[q] = b2;
+    }
+    double ce[81], ct[81];
+    double nce = 0.0, nct = 0.0, pt = 0.0, u2 = 0.0;
+#pragma unroll 1
+    for (int a = 0; a < 9; ++a)
+#pragma unroll 1
+        for (int b = 0; b < 9; ++b) {
+            double cu = 0.0, cn = 0.0, tt = 0.0;
+            for (int q = 0; q < 9; ++q) {
+                const double Na = ec->N[q * 9 + a], Nb = ec->N[q * 9 + b], wt = ec->wq[q];
+                const double bx = ec->dNx[q * 9 + b], by = ec->dNy[q * 9 + b];
+                cu += wt * Na * (uq[q] * bx + vq[q] * by + dq[q] * Nb);
+                cn += wt * Na * (unq[q] * bx + vnq[q] * by + dnq[q] * Nb);
+                tt += wt * (unq[q] * ec->dNx[q * 9 + a] + vnq[q] * ec->dNy[q * 9 + a]) * (unq[q] * bx + vnq[q] * by);
+            }
+            ce[a * 9 + b] = cu;
+            ct[a * 9 + b] = tt;
+            nce += cu * cu;
+            nct += tt * tt;
+            pt += dl[a] * cn * w[b] / dt - 0.5 * w[a] * tt * sm[b];
+        }
+    for (int q = 0; q < 9; ++q) u2 += unq[q] * unq[q] + vnq[q] * vnq[q];
+    u2 /= 9.0;
+    nce = sqrt(nce);
+    nct = sqrt(nct);
+    const double Dp = D + 1e-6;
+    const double A1 = nct / nce, A3 = Dp * nct * nct / (u2 * nce * nce);
+    const double S = A1 * A1 + (2.0 / dt) * (2.0 / dt) + A3 * A3;
+    const double tau = 1.0 / sqrt(S);
+    const double gS = -0.5 * tau * tau * tau * pt;
+    const double k13 = 2.0 * A1 * A1 + 4.0 * A3 * A3;
+    const double cG = gS * (-k13 / nce) / nce;  // d l / d Ce(U)_ab = cG Ce_ab
+    const double cH = gS * (k13 / nct) / nct;   // d l / d Ct_ab    = cH Ct_ab
+    const double gu2 = gS * (-2.0 * A3 * A3 / u2);
+    double gn[9][2], gu[9][2];
+#pragma unroll
+    for (int I = 0; I < 9; ++I) gn[I][0] = gn[I][1] = gu[I][0] = gu[I][1] = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < 9; ++q) {
+        const double* Nq = ec->N + q * 9;
+        const double* Xq = ec->dNx + q * 9;
+        const double* Yq = ec->dNy + q * 9;
+        const double wt = ec->wq[q];
+        const double au = unq[q], av = vnq[q];
+        double Dq = 0.0, Wq = 0.0, Wx = 0.0, Wy = 0.0, Sx = 0.0, Sy = 0.0;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) {
+            Dq += Nq[c] * dl[c]; Wq += Nq[c] * w[c];
+            Wx += Xq[c] * w[c]; Wy += Yq[c] * w[c];
+            Sx += Xq[c] * sm[c]; Sy += Yq[c] * sm[c];
+        }
+        const double aW = au * Wx + av * Wy, aS = au * Sx + av * Sy;
+        double gNx = 0.0, gNy = 0.0, gNN = 0.0, hx = 0.0, hy = 0.0;
+#pragma unroll 1
+        for (int b = 0; b < 9; ++b) {
+            double cNa = 0.0, tx = 0.0, ty = 0.0;
+#pragma unroll
+            for (int a = 0; a < 9; ++a) {
+                cNa += ce[a * 9 + b] * Nq[a];
+                tx += ct[a * 9 + b] * Xq[a];
+                ty += ct[a * 9 + b] * Yq[a];
+            }
+            gNx += cNa * Xq[b];
+            gNy += cNa * Yq[b];
+            gNN += cNa * Nq[b];
+            const double ab = au * Xq[b] + av * Yq[b];
+            hx += tx * ab;
+            hy += ty * ab;
+        }
+#pragma unroll
+        for (int I = 0; I < 9; ++I) {
+            const double NI = Nq[I];
+            gu[I][0] += cG * wt * (NI * gNx + Xq[I] * gNN);
+            gu[I][1] += cG * wt * (NI * gNy + Yq[I] * gNN);
+            const double cx = (tau / dt) * wt * Dq * (NI * Wx + Xq[I] * Wq);
+            const double cy = (tau / dt) * wt * Dq * (NI * Wy + Yq[I] * Wq);
+            const double tx = -0.5 * tau * wt * NI * (Wx * aS + Sx * aW);
+            const double ty = -0.5 * tau * wt * NI * (Wy * aS + Sy * aW);
+            gn[I][0] += cx + tx + 2.0 * cH * wt * NI * hx + (2.0 / 9.0) * gu2 * NI * au;
+            gn[I][1] += cy + ty + 2.0 * cH * wt * NI * hy + (2.0 / 9.0) * gu2 * NI * av;
+        }
+    }
+    const size_t ne = g.ne;
+#pragma unroll
+    for (int I = 0; I < 9; ++I)
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            gel[((size_t)(0 * 9 + I) * 2 + d) * ne + e] = gn[I][d];
+            gel[((size_t)(1 * 9 + I) * 2 + d) * ne + e] = gu[I][d];
+        }
+}
+
 struct AdjScalarQ2 {
     Grid2D g;
     const ElemConstQ2* __restrict__ ec;
@@ -1536,6 +1673,7 @@ struct AdjScalarQ2 {
     const double *mu, *phi, *phin, *gU;
     const uint8_t* mphi;
     double *gphin, *gUt, *gUns;
+    const double* gel;  // SUPG element cotangents (k_supg_adj_elem_q2 layout) or null
     __device__ void operator()(int n, double (&)[1]) const {
         IncQ2 s;
         node_inc_q2(g, n, s);
@@ -1550,19 +1688,37 @@ struct AdjScalarQ2 {
         double g1[2] = {0.0, 0.0}, g1n[2] = {0.0, 0.0};
         conv_grad_q2(g, ec, n, W, [&](int id) { return phi[id]; }, 1.0, g1);
         conv_grad_q2(g, ec, n, W, [&](int id) { return phin[id]; }, 1.0, g1n);
+        double en[2] = {0.0, 0.0}, eu[2] = {0.0, 0.0};
+        if (gel) {
+            const int j = n / g.nnx, i = n - j * g.nnx;
+            const size_t ne = g.ne;
+#pragma unroll 1
+            for (int k = 0; k < 4; ++k) {
+                int ex, ey, I;
+                bool has;
+                q2_cand(g, i, j, k, ex, ey, I, has);
+                if (!has) continue;
+                const size_t e = (size_t)ey * g.nex + ex;
+#pragma unroll
+                for (int d = 0; d < 2; ++d) {
+                    en[d] += gel[((size_t)(0 * 9 + I) * 2 + d) * ne + e];
+                    eu[d] += gel[((size_t)(1 * 9 + I) * 2 + d) * ne + e];
+                }
+            }
+        }
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
             const int i = n + d * g.nn;
-            gUt[i] = (gU ? gU[i] : 0.0) - 0.5 * g1[d];
-            gUns[i] = -0.5 * g1n[d];
+            gUt[i] = (gU ? gU[i] : 0.0) - 0.5 * g1[d] + eu[d];
+            gUns[i] = -0.5 * g1n[d] + en[d];
         }
     }
 };
 
 // it[0] = scalar adjoint BiCGSTAB, it[1] = pressure adjoint CG, it[2] = momentum adjoint BiCGSTAB
-void passive_vjp_q2(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* phi, double dt,
-                    double rho, double mu, const double* gU, const double* gP, const double* gphi, double* gU_n,
-                    double* gP_n, double* gphi_n, int* it) {
+void passive_vjp_q2(Ctx& c, const double* U_n, const double* P_n, const double* phi_n, const double* U,
+                    const double* phi, double dt, double rho, double mu, double D, int supg, const double* gU,
+                    const double* gP, const double* gphi, double* gU_n, double* gP_n, double* gphi_n, int* it) {
     ensure_adj_q2(c);
     const int nn = c.g.nn;
     const double* gUt = gU;
@@ -1580,9 +1736,15 @@ void passive_vjp_q2(Ctx& c, const double* U_n, const double* P_n, const double* 
         const double key[4] = {0.0, 0.0, 0.0, 0.0};
         it[0] = bicgstab_q2_iterate<1, StoredQ2T, true>(c, c.g_aphi, key, c.ade_A, AT, c.phi_mask, maxit, c.a_sx,
                                                          c.pred_phi);
+        if (supg) {
+            if (!c.a_gel) c.a_gel = dalloc<double>(c, 36 * (size_t)c.g.ne);
+            k_supg_adj_elem_q2<<<blocks_for(c.g.ne), BLOCK, 0, c.stream>>>(c.g, c.eq2_dev, U_n, U, c.a_sx,
+                                                                          c.phi_mask, phi, phi_n, D, dt, c.a_gel);
+            FCFD_CHECK_LAUNCH();
+        }
         launch_node<0>(c, nn, nullptr,
                        AdjScalarQ2{c.g, c.eq2_dev, StoredQ2T{c.g, Ce9F64{c.ade_B, c.g.ne}}, c.a_sx, phi, phi_n, gU,
-                                   c.phi_mask, gphi_n, c.a_gut, c.a_guns},
+                                   c.phi_mask, gphi_n, c.a_gut, c.a_guns, supg ? c.a_gel : nullptr},
                        NoFin{});
         FCFD_CHECK_LAUNCH();
         gUt = c.a_gut;

@@ -271,7 +271,12 @@ def test_quad9_ns_step_vjp_matches_finite_differences(set_zeroP):
         assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
 
 
-def test_quad9_passive_step_vjp_matches_finite_differences():
+@pytest.mark.parametrize("supg", [False, True])
+def test_quad9_passive_step_vjp_matches_finite_differences(supg):
+    """quad9 passive step, plain and SUPG: the oracle VJP's directional derivatives against central
+    finite differences of the oracle's own step.  With SUPG the element-parameter cotangents come
+    from complex steps over a node colouring (i mod 3, j mod 3): three consecutive node indices are
+    distinct mod 3, so a quad9 element holds at most one perturbed node of a colour."""
     s, surf, U_d = _case_q2(4)
     s_p, *_ = ns.build([4, 4], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-13, True, "multigrid", [2, 2],
                        shape_func_ord=2, passive=True)
@@ -282,10 +287,11 @@ def test_quad9_passive_step_vjp_matches_finite_differences():
     a, b, c = RNG.standard_normal(2 * nn), RNG.standard_normal(nn), RNG.standard_normal(nn)
 
     def loss(U0, P0, f0):
-        U, P, phi, _, _, _ = ns.passive_timestep(s, U0, P0, f0, dt, rho, mu, D, U_d, None, 0, 0, None, False)
+        U, P, phi, _, _, _ = ns.passive_timestep(s, U0, P0, f0, dt, rho, mu, D, U_d, None, 0, 0, None, supg)
         return a @ U + b @ P + c @ phi
 
-    gUn, gPn, gfn, outs, _ = adj.passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, None, 0, 0, None, a, b, c)
+    gUn, gPn, gfn, outs, _ = adj.passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, None, 0, 0, None, a, b, c,
+                                             supg=supg)
     for _ in range(2):
         ds = [RNG.standard_normal(2 * nn), RNG.standard_normal(nn), RNG.standard_normal(nn)]
         fd = _fd(loss, [U_n, P_n, phi_n], ds, 1e-4)

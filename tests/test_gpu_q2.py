@@ -300,15 +300,15 @@ def test_quad9_rollout_gradient_matches_finite_differences_on_gpu():
             assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
 
 
-@pytest.mark.parametrize("phi_faces", [(), (3,)])
-def test_quad9_passive_step_vjp_matches_oracle(phi_faces):
-    """Reverse-mode product of one plain passive quad9 step against the oracle's VJP (pinned by
-    finite differences in tests/test_adjoint_oracle.py): cotangents to 1e-9; the symmetric
-    pressure-adjoint CG count equal, the transposed BiCGSTAB counts within 2."""
+@pytest.mark.parametrize("phi_faces,supg", [((), False), ((3,), False), ((), True), ((3,), True)])
+def test_quad9_passive_step_vjp_matches_oracle(phi_faces, supg):
+    """Reverse-mode product of one passive quad9 step, plain and SUPG, against the oracle's VJP
+    (pinned by finite differences in tests/test_adjoint_oracle.py): cotangents to 1e-9; the
+    symmetric pressure-adjoint CG count equal, the transposed BiCGSTAB counts within 2."""
     from oracle import adjoint as oadj
     n, dt, rho, mu, D = (8, 8), 2e-3, 1.0, 1e-2, 1e-3
-    print(f"\n[case] quad9 passive step VJP {n} phi_faces={phi_faces}", flush=True)
-    compute, timestep, coords, gamma_d, U_d, gamma_d_phi, s, oUd = _passive_setup(n, "multigrid", False, phi_faces,
+    print(f"\n[case] quad9 passive step VJP {n} phi_faces={phi_faces} supg={supg}", flush=True)
+    compute, timestep, coords, gamma_d, U_d, gamma_d_phi, s, oUd = _passive_setup(n, "multigrid", supg, phi_faces,
                                                                                   [2, 2])
     U_d = ns.cavity_lid(s, [np.asarray(x) for x in ms.create_mesh_2d_quad9(8, 8, 1.0, 1.0)[2]], U_d)
     U_n, P_n = _vjp_state(s, U_d)
@@ -319,23 +319,13 @@ def test_quad9_passive_step_vjp_matches_oracle(phi_faces):
     ctx = timestep.context
     F_p, F = compute(U_d, None, 0, 0)
     (gUn, gPn, gfn), (U, P, phi), its = ctx.passive_timestep_vjp(U_n, P_n, phi_n, dt, rho, mu, D, gU, gP, gphi,
-                                                                 U_d, phi_d, F_p, F, False)
+                                                                 U_d, phi_d, F_p, F, supg)
     oF_p, _ = ns.compute(s, U_d, None, 0, 0)
     ogUn, ogPn, ogfn, (oU, oP, ophi), oits = oadj.passive_vjp(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, oF_p,
-                                                              0, gamma_d_phi, gU, gP, gphi, supg=False)
+                                                              0, gamma_d_phi, gU, gP, gphi, supg=supg)
     print(f"  iters {its[3:]} oracle {oits}  rel " + " ".join(
         f"{_rel(_np(a), b):.3e}" for a, b in ((gUn, ogUn), (gPn, ogPn), (gfn, ogfn))), flush=True)
     assert _rel(_np(phi), ophi) < 1e-10 and _rel(_np(U), oU) < 1e-10
     for a, b in ((gUn, ogUn), (gPn, ogPn), (gfn, ogfn)):
         assert _rel(_np(a), b) < 1e-9, _rel(_np(a), b)
     assert its[4] == oits[1] and abs(its[3] - oits[0]) <= 2 and abs(its[5] - oits[2]) <= 2
-
-
-def test_quad9_supg_vjp_is_refused():
-    from jax_fem_cfd._native import NativeError
-    print("\n[case] quad9 SUPG VJP refused", flush=True)
-    compute, timestep, coords, gamma_d, U_d, gamma_d_phi, s, oUd = _passive_setup((4, 4), "jacobi", True)
-    z = np.zeros(s.nn)
-    with pytest.raises(NativeError):
-        timestep.context.passive_timestep_vjp(np.zeros(2 * s.nn), z, z, 1e-3, 1.0, 1e-2, 1e-3, None, None, z, U_d,
-                                              None, None, None, True)
