@@ -207,6 +207,39 @@ def test_pressure_cg_multigrid(set_zeroP):
     assert _rel(_np(x), xr) < 1e-10
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_vcycle_and_pressure_cg_at_512(precision):
+    """512x512 elements (263,169 nodes), pinned: the sizes at which the V-cycle runs its staged
+    fine-level kernels (the fused down leg with 4 coarse rows per block from 200 coarse rows on,
+    the LDS-staged up-leg Jacobi from 2^17 nodes on) and the PCG its staged Laplacian.  FP64: the
+    V-cycle to 1e-12 and the PCG with equal iterations to 1e-10 against the oracle.  Mixed: the
+    FP32 V-cycle to 2e-5, and the flexible PCG converged to the same tolerance (solution 1e-7)."""
+    n, nf = 512, 16
+    s, *_ = ns.build([n, n], [1.0, 1.0], False, [], [], 1e-10, True, "multigrid", [nf, nf])
+    print(f"\n[case] V-cycle and PCG {n}x{n} {precision}", flush=True)
+    c = _ctx(n, n, 1.0, 1.0, False, tol=1e-10, set_zeroP=True, precond="multigrid", mg_levels=len(s.levels),
+             precision=precision, gamma_p=[0], mg_gamma_p=[L["gamma_p"] for L in s.levels])
+    r = RNG.standard_normal(s.nn)
+    r[0] = 0
+    got = _np(c.vcycle(r))
+    assert _rel(got, s.M_op(r)) < (1e-12 if precision == "fp64" else 2e-5)
+
+    def A(x):
+        y = oem.laplacian_element_mult(x, s.Le, s.nconn, s.nn)
+        y[0] = 0
+        return y
+
+    from oracle.solvers import cg
+    xr, kr = cg(A, r, tol=1e-10, maxiter=1000, M=s.M_op)
+    x, k, conv = c.pressure_cg(r, tol=1e-10, maxiter=1000)
+    print(f"  iterations {k} oracle {kr}  rel x {_rel(_np(x), xr):.2e}", flush=True)
+    assert conv
+    if precision == "fp64":
+        assert k == kr and _rel(_np(x), xr) < 1e-10
+    else:
+        assert _rel(_np(x), xr) < 1e-7
+
+
 @pytest.mark.parametrize("set_zeroP", [True, False])
 def test_pressure_cg_approx_inv(set_zeroP):
     """PCG with the two-term approximate inverse (standard.py:225-234); its gamma_p rows (node 0
