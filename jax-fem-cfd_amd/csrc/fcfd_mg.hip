@@ -307,7 +307,7 @@ struct JacobiS {
         v[0] = (double)x[id];
         v[1] = (pin && pin[id]) ? 1.0 : 0.0;
     }
-    __device__ void operator()(int n, const StageView<2, 1>& sv, double (&acc)[1]) const {
+    __device__ __forceinline__ T value(int n, const StageView<2, 1>& sv) const {
         NodeNbr s;
         node_nbr(gf, n, s);
         T Y[3][3];
@@ -323,9 +323,36 @@ struct JacobiS {
         const T off = ebe_const(s, Leoff, X);
         T v = (T(OMEGA) * ((T)r[n] - off)) / Ld[n] + T(1.0 - OMEGA) * X[1][1];
         if (sv.at(1, n, 0, 0) != 0.0) v = T(0);
-        y[n] = v;
-        acc[0] += (double)v;
+        return v;
     }
+    __device__ void operator()(int n, const StageView<2, 1>& sv, double (&)[1]) const { y[n] = value(n, sv); }
+};
+
+// The PCG's z-step fused into the last kernel of the pinned V-cycle (level 0, staged): the
+// preconditioned z = y is stored as FP64 into the CG's z buffer and the block partials of r.z
+// (and r.z_old for the flexible form) feed CgK3Fin, as CgK3 would after a separate pass: the
+// same values, summed per block in the same tree over the same node blocks, so gamma and beta
+// are bitwise those of CgK3.
+template <class T>
+struct JacobiSZ {
+    static constexpr int NV = 2;
+    static constexpr int NPT = 1;
+    JacobiS<T, double> j;
+    double* znew; const double* zold; int flex;
+    __device__ __forceinline__ int nnx() const { return j.nnx(); }
+    __device__ __forceinline__ void stage(int id, double (&v)[2]) const { j.stage(id, v); }
+    __device__ void operator()(int n, const StageView<2, 1>& sv, double (&acc)[2]) const {
+        const double zk = (double)j.value(n, sv);
+        const double rk = j.r[n];
+        znew[n] = zk;
+        acc[0] += rk * zk;
+        if (flex) acc[1] += rk * zold[n];
+    }
+};
+
+// CG tail handed to the V-cycle (see JacobiSZ)
+struct CgTail {
+    double* znew; const double* zold; int flex;
 };
 
 // ------------------------------------------------------------------------------ coarse level
@@ -617,8 +644,11 @@ __global__ __launch_bounds__(256) void k_coarse_dense(int nn, const float* __res
 }
 
 // ------------------------------------------------------------------------------ V-cycle driver
+// With `tail` (pinned form) the level-0 up leg, when staged, also runs the PCG's z-step and the
+// return value says so (nothing is then stored into the level-0 y).
 template <class T>
-static void vcycle_t(Ctx& c, const double* r0, const int* active) {
+static bool vcycle_t(Ctx& c, const double* r0, const int* active, const CgTail* tail = nullptr) {
+    bool fused = false;
     using B = LvBufs<T>;
     const int N = c.nlev;
     const int neu = !c.set_zeroP;
@@ -693,11 +723,13 @@ static void vcycle_t(Ctx& c, const double* r0, const int* active) {
             launch_node<1>(c, n, active, Prolong<T>{L.g, Lc.g, B::y(Lc), B::f(L), c.st, l},
                            MeanFin{&c.st->mg[l][MG_P], nullptr, n});
         if (!neu && !L.g.periodic && n >= (1 << 17)) {  // staging pays on the large levels only
-            if (l == 0)
-                launch_stencil<0>(c, n, active,
-                                  JacobiS<T, double>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, r0, B::x(L),
-                                                     B::y(Lc), B::y(L)},
-                                  NoFin{});
+            const JacobiS<T, double> j0{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, r0, B::x(L), B::y(Lc), B::y(L)};
+            if (l == 0 && tail) {
+                launch_stencil<2>(c, n, active, JacobiSZ<T>{j0, tail->znew, tail->zold, tail->flex},
+                                  CgK3Fin{c.st, tail->flex});
+                fused = true;
+            } else if (l == 0)
+                launch_stencil<0>(c, n, active, j0, NoFin{});
             else
                 launch_stencil<0>(c, n, active,
                                   JacobiS<T, T>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, B::r(L), B::x(L),
@@ -715,6 +747,7 @@ static void vcycle_t(Ctx& c, const double* r0, const int* active) {
                 MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
     }
     FCFD_CHECK_LAUNCH();
+    return fused;
 }
 
 // z = y0 - mean (Neumann) as FP64
@@ -845,8 +878,11 @@ struct ApproxInv2 {
     }
 };
 
-// applies the preconditioner to c.c_r; returns pointer + type of y0
-static const void* precondition(Ctx& c, const int* active, bool& is_f32) {
+// applies the preconditioner to c.c_r; returns pointer + type of y0.  With `tail`, a V-cycle
+// may also run the CG's z-step (*fused = true; y0 is then not written).
+static const void* precondition(Ctx& c, const int* active, bool& is_f32, const CgTail* tail = nullptr,
+                                bool* fused = nullptr) {
+    if (fused) *fused = false;
     const int nn = c.g.nn;
     is_f32 = false;
     switch (c.precond) {
@@ -867,11 +903,15 @@ static const void* precondition(Ctx& c, const int* active, bool& is_f32) {
         }
         case FCFD_PC_MULTIGRID:
             if (c.precision == FCFD_PREC_MIXED) {
-                vcycle_t<float>(c, c.c_r, active);
+                const bool f = vcycle_t<float>(c, c.c_r, active, tail);
+                if (fused) *fused = f;
                 is_f32 = true;
                 return c.lv[0].y32;
             }
-            vcycle_t<double>(c, c.c_r, active);
+            {
+                const bool f = vcycle_t<double>(c, c.c_r, active, tail);
+                if (fused) *fused = f;
+            }
             return c.lv[0].y;
     }
     throw Error(FCFD_ERR_UNSUPPORTED, "pressure preconditioner not supported");
@@ -909,12 +949,13 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
                 launch_stencil<1>(c, nn, act, CgK1S{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
                                   CgK1Fin{c.st});
             launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
-            bool f;
-            const void* y = precondition(c, act, f);
-            if (f)
+            bool f, fused;
+            const CgTail tail{c.c_z[nw], c.c_z[o], flex};
+            const void* y = precondition(c, act, f, neu ? nullptr : &tail, &fused);
+            if (!fused && f)
                 launch_node<2>(c, nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
                                CgK3Fin{c.st, flex});
-            else
+            else if (!fused)
                 launch_node<2>(c, nn, act,
                                CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
                                CgK3Fin{c.st, flex});
