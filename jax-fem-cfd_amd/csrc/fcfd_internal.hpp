@@ -269,10 +269,16 @@ void supg_api(Ctx& c, const double* U_n, const double* Ce_api, double D, double 
 // pressure / multigrid (fcfd_mg.hip)
 void vcycle(Ctx& c, const double* r, double* z);
 int pressure_cg(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged);
+// The pressure right-hand side c (D U* - F_p), 0 at the pins (standard.py:99-104), for a PCG
+// whose init kernel forms it per node instead of reading a stored b
+struct PrhsSpec {
+    const ElemConst* ec; const double* Us; const double* Fp; double c;
+};
 // zero_exit: a right-hand side that is exactly zero ends the solve at once with c_zero set
-// (the solution is 0; adjoint solves only)
+// (the solution is 0; adjoint solves only).  prhs (pinned form only): b is formed by the init
+// kernel from prhs (b and bmean are then unused).
 int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
-        bool* converged, bool zero_exit = false);
+        bool* converged, bool zero_exit = false, const PrhsSpec* prhs = nullptr);
 int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho,
                    double dt, double* P_out);
 void projection_ustar(Ctx& c, const double* U_hat, const double* P_n, double rho, double dt, double* U_star);

@@ -808,6 +808,35 @@ struct CgInit {
 };
 
 
+// CgInit with the pinned pressure rhs formed per node (PrhsBody's value, so bitwise the b that
+// CgInit would read): one pass instead of the rhs kernel, its finalize and the b store and load
+struct CgInitD {
+    Grid2D g; const double* Le; const uint8_t* pin; PrhsSpec ps; const double* x0in; int zero_pins;
+    double* x; double* r;
+    __device__ void operator()(int n, double (&acc)[3]) const {
+        NodeNbr s;
+        node_nbr(g, n, s);
+        double X[3][3], Xu[3][3], Xv[3][3];
+        if (zero_pins && pin)
+            load3x3(s, [&](int k) { const double v = x0in[k]; return pin[k] ? 0.0 : v; }, X);
+        else
+            load3x3(s, [&](int k) { return x0in[k]; }, X);
+        load3x3(s, [&](int k) { return ps.Us[k]; }, Xu);
+        load3x3(s, [&](int k) { return ps.Us[k + g.nn]; }, Xv);
+        const bool pn = pin && pin[n];
+        const double div = ebe_const_T(s, ps.ec->Gx, Xu) + ebe_const_T(s, ps.ec->Gy, Xv);
+        double bn = ps.c * (div - (ps.Fp ? ps.Fp[n] : 0.0));
+        if (pn) bn = 0.0;
+        const double ax = pn ? 0.0 : ebe_const(s, Le, X);
+        const double rn = bn - ax;
+        x[n] = X[1][1];
+        r[n] = rn;
+        acc[0] += bn * bn;
+        acc[1] += rn * rn;
+        acc[2] += rn;
+    }
+};
+
 // p = z + beta p_old (on the fly), Ap = L p (pinned rows 0), finalize alpha = gamma / p.Ap
 struct CgK1 {
     Grid2D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
@@ -919,15 +948,19 @@ static const void* precondition(Ctx& c, const int* active, bool& is_f32, const C
 
 // PCG: returns iterations; x0 in x0in (pins zeroed if zero_pins), solution in c.c_x
 int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
-        bool* converged, bool zero_exit) {
+        bool* converged, bool zero_exit, const PrhsSpec* prhs) {
     const Grid2D& g = c.g;
     const int nn = g.nn;
     const int neu = (c.precond == FCFD_PC_MULTIGRID) && !c.set_zeroP;
     const int flex = c.precision == FCFD_PREC_MIXED && c.precond == FCFD_PC_MULTIGRID;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* Le = c.lv[0].ec_dev->Le;
-    launch_node<3>(c, nn, nullptr, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
-                                               CgInitFin{c.st, tol, maxit, nn, zero_exit});
+    if (prhs)
+        launch_node<3>(c, nn, nullptr, CgInitD{g, Le, pin, *prhs, x0in, zero_pins, c.c_x, c.c_r},
+                       CgInitFin{c.st, tol, maxit, nn, zero_exit});
+    else
+        launch_node<3>(c, nn, nullptr, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
+                       CgInitFin{c.st, tol, maxit, nn, zero_exit});
     const int* act = &c.st->c_active;
     bool f32;
     const void* y0 = precondition(c, act, f32);
@@ -1007,12 +1040,18 @@ int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double
                    double* P_out) {
     const int nn = c.g.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
-    launch_node<1>(c, nn, nullptr,
-                                               PrhsBody{c.g, c.ec_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
-                                               MeanFin{&c.st->mean_rhs, nullptr, nn});
     const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
     bool conv;
-    int it = pcg(c, c.c_rhs, c.set_zeroP ? nullptr : &c.st->mean_rhs, P_n, c.set_zeroP, c.tol, maxit, &conv);
+    int it;
+    if (c.set_zeroP) {
+        // pinned: no mean to subtract, so the PCG's init kernel forms the rhs itself
+        const PrhsSpec ps{c.ec_dev, U_star, F_p, rho / dt};
+        it = pcg(c, nullptr, nullptr, P_n, 1, c.tol, maxit, &conv, false, &ps);
+    } else {
+        launch_node<1>(c, nn, nullptr, PrhsBody{c.g, c.ec_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
+                       MeanFin{&c.st->mean_rhs, nullptr, nn});
+        it = pcg(c, c.c_rhs, &c.st->mean_rhs, P_n, 0, c.tol, maxit, &conv);
+    }
     if (!c.set_zeroP)
         launch_node<1>(c, nn, nullptr, SumXBody{c.c_x},
                                                    MeanFin{&c.st->mean_p, nullptr, nn});
