@@ -330,29 +330,33 @@ struct JacobiS {
 
 // The PCG's z-step fused into the last kernel of the pinned V-cycle (level 0, staged): the
 // preconditioned z = y is stored as FP64 into the CG's z buffer and the block partials of r.z
-// (and r.z_old for the flexible form) feed CgK3Fin, as CgK3 would after a separate pass: the
-// same values, summed per block in the same tree over the same node blocks, so gamma and beta
-// are bitwise those of CgK3.
-template <class T>
+// feed the CG's finalize, as the separate pass would (CgK3: r.z and, flexible form, r.z_old;
+// CgGamma0 before the first iteration: r.z, with p_old cleared).  The same values, summed per
+// block in the same tree over the same node blocks, so gamma and beta are bitwise those of the
+// separate pass.
+template <class T, int NR>
 struct JacobiSZ {
     static constexpr int NV = 2;
     static constexpr int NPT = 1;
     JacobiS<T, double> j;
-    double* znew; const double* zold; int flex;
+    double* znew; const double* zold; int flex; double* pzero;
     __device__ __forceinline__ int nnx() const { return j.nnx(); }
     __device__ __forceinline__ void stage(int id, double (&v)[2]) const { j.stage(id, v); }
-    __device__ void operator()(int n, const StageView<2, 1>& sv, double (&acc)[2]) const {
+    __device__ void operator()(int n, const StageView<2, 1>& sv, double (&acc)[NR]) const {
         const double zk = (double)j.value(n, sv);
         const double rk = j.r[n];
         znew[n] = zk;
+        if (pzero) pzero[n] = 0.0;
         acc[0] += rk * zk;
-        if (flex) acc[1] += rk * zold[n];
+        if constexpr (NR == 2) {
+            if (flex) acc[1] += rk * zold[n];
+        }
     }
 };
 
-// CG tail handed to the V-cycle (see JacobiSZ)
+// CG tail handed to the V-cycle (see JacobiSZ); first: the pass before the first iteration
 struct CgTail {
-    double* znew; const double* zold; int flex;
+    double* znew; const double* zold; double* pzero; int flex; int first;
 };
 
 // ------------------------------------------------------------------------------ coarse level
@@ -724,8 +728,12 @@ static bool vcycle_t(Ctx& c, const double* r0, const int* active, const CgTail* 
                            MeanFin{&c.st->mg[l][MG_P], nullptr, n});
         if (!neu && !L.g.periodic && n >= (1 << 17)) {  // staging pays on the large levels only
             const JacobiS<T, double> j0{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, r0, B::x(L), B::y(Lc), B::y(L)};
-            if (l == 0 && tail) {
-                launch_stencil<2>(c, n, active, JacobiSZ<T>{j0, tail->znew, tail->zold, tail->flex},
+            if (l == 0 && tail && tail->first) {
+                launch_stencil<1>(c, n, active, JacobiSZ<T, 1>{j0, tail->znew, nullptr, 0, tail->pzero},
+                                  CgGamma0Fin{c.st});
+                fused = true;
+            } else if (l == 0 && tail) {
+                launch_stencil<2>(c, n, active, JacobiSZ<T, 2>{j0, tail->znew, tail->zold, tail->flex, nullptr},
                                   CgK3Fin{c.st, tail->flex});
                 fused = true;
             } else if (l == 0)
@@ -962,14 +970,15 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
         launch_node<3>(c, nn, nullptr, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
                        CgInitFin{c.st, tol, maxit, nn, zero_exit});
     const int* act = &c.st->c_active;
-    bool f32;
-    const void* y0 = precondition(c, act, f32);
-    if (f32)
-        launch_node<1>(c, nn, act,
-                                                   CgGamma0<float>{c.c_r, (const float*)y0, c.c_z[0], c.c_p[0], c.st, neu},
-                                                   CgGamma0Fin{c.st});
-    else
-        launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, (const double*)y0, c.c_z[0], c.c_p[0], c.st, neu}, CgGamma0Fin{c.st});
+    bool f32, fused0;
+    const CgTail tail0{c.c_z[0], nullptr, c.c_p[0], 0, 1};
+    const void* y0 = precondition(c, act, f32, neu ? nullptr : &tail0, &fused0);
+    if (!fused0 && f32)
+        launch_node<1>(c, nn, act, CgGamma0<float>{c.c_r, (const float*)y0, c.c_z[0], c.c_p[0], c.st, neu},
+                       CgGamma0Fin{c.st});
+    else if (!fused0)
+        launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, (const double*)y0, c.c_z[0], c.c_p[0], c.st, neu},
+                       CgGamma0Fin{c.st});
     FCFD_CHECK_LAUNCH();
     // one chunk = a pair of PCG iterations (parity of the p / z double buffers returns to 0)
     auto pair = [&]() {
@@ -983,7 +992,7 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
                                   CgK1Fin{c.st});
             launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
             bool f, fused;
-            const CgTail tail{c.c_z[nw], c.c_z[o], flex};
+            const CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
             const void* y = precondition(c, act, f, neu ? nullptr : &tail, &fused);
             if (!fused && f)
                 launch_node<2>(c, nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
