@@ -197,6 +197,149 @@ __global__ __launch_bounds__(256) void k_down_fused(Grid2D gf, Grid2D gc, const 
     }
 }
 
+// Two down legs in one kernel (small pinned levels l -> m = l + 1 -> c = l + 2, where each leg
+// alone is a few microseconds of launch and memory latency).  A block owns D2_TR x D2_TC nodes of
+// level c and recomputes what they depend on: the level-m residual f_m on the (2 D2_TR + 1) x
+// (2 D2_TC + 1) m-nodes around them, hence x_m and r_m on a one-wider m-window, r_m being the
+// restriction of f_l on the (2 (2 D2_TR + 3) + 1)-row l-window, hence x_l on a one-wider l-window.
+// It stores x_l, r_m and x_m of the nodes it owns (the ones the single-level kernel's block
+// would own) and r_c.  Every value is formed with the single-level kernel's operands and order,
+// so all four outputs are bitwise those of two k_down_fused launches.
+constexpr int D2_TR = 2, D2_TC = 16;
+template <class T, class TR>
+__global__ __launch_bounds__(256) void k_down2(Grid2D gf, Grid2D gm, Grid2D gc, const double* __restrict__ Lef,
+                                               const double* __restrict__ Lem, const TR* __restrict__ rf,
+                                               const T* __restrict__ Ldf, const uint8_t* __restrict__ pinf,
+                                               T* __restrict__ xf, const T* __restrict__ Ldm,
+                                               const uint8_t* __restrict__ pinm, T* __restrict__ rm,
+                                               T* __restrict__ xm, T* __restrict__ rc, const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    constexpr int XRm = 2 * D2_TR + 3, XWm = 2 * D2_TC + 3;  // m-window of x_m, r_m
+    constexpr int FRm = 2 * D2_TR + 1, FWm = 2 * D2_TC + 1;  // m-window of f_m
+    constexpr int FRl = 2 * XRm + 1, FWl = 2 * XWm + 1;      // l-window of f_l
+    constexpr int XRl = FRl + 2, XWl = FWl + 2;              // l-window of x_l
+    __shared__ T xl[XRl][XWl];
+    __shared__ T fl[FRl][FWl];
+    __shared__ T xms[XRm][XWm];
+    __shared__ T rms[XRm][XWm];
+    __shared__ uint8_t pms[XRm][XWm];
+    __shared__ T fm[FRm][FWm];
+    const int nbx = gridDim.x;
+    const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    const int J0 = (lb / nbx) * D2_TR, I0 = (lb - (lb / nbx) * nbx) * D2_TC;
+    const int mj0 = 2 * J0 - 2, mi0 = 2 * I0 - 2;    // origin of the x_m window
+    const int lj0 = 2 * mj0 - 2, li0 = 2 * mi0 - 2;  // origin of the x_l window (f_l: one in)
+    // x_l = (2/3 r_l) / L_diag, 0 at the pins and outside the grid
+    constexpr int NIT = (XRl * XWl + 255) / 256;
+    T v[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int t = threadIdx.x + it * 256;
+        const int a = t / XWl, b = t - a * XWl;
+        const int fi = li0 + b, fj = lj0 + a;
+        const bool in = t < XRl * XWl && fi >= 0 && fi < gf.nnx && fj >= 0 && fj < gf.nny;
+        const int k = in ? fj * gf.nnx + fi : 0;
+        const T xv = (T(2.0 / 3.0) * (T)rf[k]) / Ldf[k];
+        v[it] = (!in || (pinf && pinf[k])) ? T(0) : xv;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int t = threadIdx.x + it * 256;
+        if (t < XRl * XWl) {
+            const int a = t / XWl, b = t - a * XWl;
+            xl[a][b] = v[it];
+        }
+    }
+    __syncthreads();
+    // owned x_l: l-rows 4 J0 .. 4 J0 + 4 D2_TR - 1 (window rows 6 ..), same for columns
+    for (int t = threadIdx.x; t < 4 * D2_TR * 4 * D2_TC; t += 256) {
+        const int a = t / (4 * D2_TC), b = t - a * (4 * D2_TC);
+        const int fi = 4 * I0 + b, fj = 4 * J0 + a;
+        if (fi < gf.nnx && fj < gf.nny) xf[fj * gf.nnx + fi] = xl[a + 6][b + 6];
+    }
+    // f_l = r_l - L x_l (pinned rows 0), 0 outside the grid
+    for (int t = threadIdx.x; t < FRl * FWl; t += 256) {
+        const int a = t / FWl, b = t - a * FWl;
+        const int fi = li0 + 1 + b, fj = lj0 + 1 + a;
+        T fv = T(0);
+        if (fi >= 0 && fi < gf.nnx && fj >= 0 && fj < gf.nny) {
+            const int n = fj * gf.nnx + fi;
+            NodeNbr s;
+            node_elems(gf, n, s);
+            T X[3][3];
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) X[dy][dx] = xl[a + dy][b + dx];
+            const bool pn = pinf && pinf[n];
+            const T rh = pn ? T(0) : (T)rf[n];
+            const T lx = pn ? T(0) : ebe_const(s, Lef, X);
+            fv = rh - lx;
+        }
+        fl[a][b] = fv;
+    }
+    __syncthreads();
+    // r_m = R f_l on the x_m window, x_m from it; owned r_m, x_m stored
+    for (int t = threadIdx.x; t < XRm * XWm; t += 256) {
+        const int a = t / XWm, b = t - a * XWm;
+        const int mi = mi0 + b, mj = mj0 + a;
+        const bool in = mi >= 0 && mi < gm.nnx && mj >= 0 && mj < gm.nny;
+        T rv = T(0), xv = T(0);
+        bool pk = false;
+        if (in) {
+            const int rr = 2 * a + 1, cc = 2 * b + 1;
+            auto val = [&](int dy, int dx) -> T { return fl[rr + dy][cc + dx]; };
+            const T E = ((val(1, 0) + val(-1, 0)) + val(0, 1)) + val(0, -1);
+            const T D = ((val(1, 1) + val(1, -1)) + val(-1, 1)) + val(-1, -1);
+            rv = (val(0, 0) + T(0.5) * E) + T(0.25) * D;
+            const int k = mj * gm.nnx + mi;
+            pk = pinm && pinm[k];
+            const T x = (T(2.0 / 3.0) * rv) / Ldm[k];
+            xv = pk ? T(0) : x;
+            if (a >= 2 && a < 2 + 2 * D2_TR && b >= 2 && b < 2 + 2 * D2_TC) {
+                rm[k] = rv;
+                xm[k] = xv;
+            }
+        }
+        rms[a][b] = rv;
+        xms[a][b] = xv;
+        pms[a][b] = pk;
+    }
+    __syncthreads();
+    // f_m = r_m - L x_m (pinned rows 0), 0 outside the grid
+    for (int t = threadIdx.x; t < FRm * FWm; t += 256) {
+        const int a = t / FWm, b = t - a * FWm;
+        const int mi = mi0 + 1 + b, mj = mj0 + 1 + a;
+        T fv = T(0);
+        if (mi >= 0 && mi < gm.nnx && mj >= 0 && mj < gm.nny) {
+            NodeNbr s;
+            node_elems(gm, mj * gm.nnx + mi, s);
+            T X[3][3];
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) X[dy][dx] = xms[a + dy][b + dx];
+            const bool pn = pms[a + 1][b + 1];
+            const T rh = pn ? T(0) : rms[a + 1][b + 1];
+            const T lx = pn ? T(0) : ebe_const(s, Lem, X);
+            fv = rh - lx;
+        }
+        fm[a][b] = fv;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < D2_TR * D2_TC; t += 256) {
+        const int jr = t / D2_TC, ci = t - jr * D2_TC;
+        const int I = I0 + ci, J = J0 + jr;
+        if (I < gc.nnx && J < gc.nny) {
+            const int cc = 2 * ci + 1, rr = 2 * jr + 1;
+            auto val = [&](int dy, int dx) -> T { return fm[rr + dy][cc + dx]; };
+            const T E = ((val(1, 0) + val(-1, 0)) + val(0, 1)) + val(0, -1);
+            const T D = ((val(1, 1) + val(1, -1)) + val(-1, 1)) + val(-1, -1);
+            rc[J * gc.nnx + I] = (val(0, 0) + T(0.5) * E) + T(0.25) * D;
+        }
+    }
+}
+
 template <class T, class TR>
 static void launch_down_fused(Ctx& c, const Grid2D& gf, const Grid2D& gc, const double* Le, const TR* r, const T* Ld,
                               const uint8_t* pin, T* x, T* rc, const int* active) {
@@ -671,6 +814,22 @@ static bool vcycle_t(Ctx& c, const double* r0, const int* active, const CgTail* 
         Level& Lc = c.lv[l + 1];
         const int n = L.g.nn;
         if (!neu && !L.g.periodic) {
+            if (l + 1 < ld && Lc.g.nny < 200) {
+                // two small levels in one kernel
+                Level& L2 = c.lv[l + 2];
+                const dim3 grid((L2.g.nnx + D2_TC - 1) / D2_TC, (L2.g.nny + D2_TR - 1) / D2_TR);
+                if (l == 0)
+                    k_down2<T, double><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le, r0,
+                                                                   B::Ld(L), L.pin, B::x(L), B::Ld(Lc), Lc.pin,
+                                                                   B::r(Lc), B::x(Lc), B::r(L2), active);
+                else
+                    k_down2<T, T><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le, B::r(L),
+                                                              B::Ld(L), L.pin, B::x(L), B::Ld(Lc), Lc.pin, B::r(Lc),
+                                                              B::x(Lc), B::r(L2), active);
+                FCFD_CHECK_LAUNCH();
+                ++l;
+                continue;
+            }
             if (l == 0)
                 launch_down_fused<T, double>(c, L.g, Lc.g, L.ec_dev->Le, r0, B::Ld(L), L.pin, B::x(L), B::r(Lc),
                                              active);
