@@ -359,8 +359,9 @@ static void launch_down_fused(Ctx& c, const Grid2D& gf, const Grid2D& gc, const 
 // floor((f - 1) / 2) + {0, 1, 2}.  Which patch entries pair up depends only on the parity of the
 // node's (i, j): selected per lane, with the same operands and operation order as interp_at, so
 // every value is bitwise the one interp_at returns.  9 coarse loads instead of 36.
-template <class T>
-__device__ __forceinline__ void interp3x3(const Grid2D& gc, const T* yc, T my, const NodeNbr& s, T (&Y)[3][3]) {
+// yv(cy, cx): the coarse value at coarse row cy, column cx (global memory, or a block's LDS copy)
+template <class T, class YV>
+__device__ __forceinline__ void interp3x3_v(const Grid2D& gc, YV yv, T my, const NodeNbr& s, T (&Y)[3][3]) {
     const int bx = (s.i - 1) >> 1, by = (s.j - 1) >> 1;  // arithmetic shift: floor
     int cx[3], cy[3];
 #pragma unroll
@@ -380,7 +381,7 @@ __device__ __forceinline__ void interp3x3(const Grid2D& gc, const T* yc, T my, c
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) P[a][b] = yc[cy[a] * gc.nnx + cx[b]] - my;
+        for (int b = 0; b < 3; ++b) P[a][b] = yv(cy[a], cx[b]) - my;
     // pair (first, second) of patch columns for fine offsets -1, 0, +1:
     //   even i: (0,1) (1,1) (1,2)      odd i: (0,0) (0,1) (1,1)
     const bool ox = s.i & 1, oy = s.j & 1;
@@ -399,6 +400,37 @@ __device__ __forceinline__ void interp3x3(const Grid2D& gc, const T* yc, T my, c
         Y[1][b] = T(0.5) * (oy ? h0 : h1) + T(0.5) * h1;
         Y[2][b] = T(0.5) * h1 + T(0.5) * (oy ? h1 : h2);
     }
+}
+
+template <class T>
+__device__ __forceinline__ void interp3x3(const Grid2D& gc, const T* yc, T my, const NodeNbr& s, T (&Y)[3][3]) {
+    interp3x3_v(gc, [&](int cy, int cx) { return yc[cy * gc.nnx + cx]; }, my, s, Y);
+}
+
+// Pinned-form up-leg value at node n: x + I y_coarse on the stencil (0 at missing and pinned
+// nodes), then one damped Jacobi, 0 at the pins -- Jacobi's pinned path with its operands and
+// operation order.  yv reads the coarse y (see interp3x3_v).
+template <class T, class TR, class YV>
+__device__ __forceinline__ T jacobi_pinned(const Grid2D& gf, const Grid2D& gc, const double* Leoff, const T* Ld,
+                                           const uint8_t* pin, const TR* r, const T* x, int n, YV yv) {
+    NodeNbr s;
+    node_nbr(gf, n, s);
+    T Y[3][3];
+    interp3x3_v(gc, yv, T(0), s, Y);
+    T X[3][3];
+    load3x3(s, [&](int k) { return x[k]; }, X);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const bool keep = s.ok[a][b] && !(pin && pin[s.nb[a][b]]);
+            X[a][b] = keep ? X[a][b] + Y[a][b] : T(0);
+        }
+    const T rh = (T)r[n];
+    const T off = ebe_const(s, Leoff, X);
+    T v = (T(OMEGA) * (rh - off)) / Ld[n] + T(1.0 - OMEGA) * X[1][1];
+    if (pin && pin[n]) v = T(0);
+    return v;
 }
 
 // xh <- xh + I xH; one damped Jacobi (multigrid.py:214-228, :306-326)
@@ -436,6 +468,44 @@ struct Jacobi {
         acc[0] += (double)v;
     }
 };
+
+// Two up legs in one kernel (small pinned levels c = l + 2 -> m = l + 1 -> l): a block owns a
+// U2_LR x U2_LC tile of level-l nodes; it first forms the level-m Jacobi on the m-window that the
+// tile's prolongation patches read (rows (j0 - 1) >> 1 .. + U2_LR / 2 + 2, likewise columns)
+// into LDS, then the level-l Jacobi of its nodes with the prolongation read from that window.
+// Level m's y is not stored (only level l's prolongation reads it).  The values are those of
+// two Jacobi launches (jacobi_pinned, same operands and order).
+constexpr int U2_LR = 4, U2_LC = 64;
+template <class T, class TR>
+__global__ __launch_bounds__(256) void k_up2(Grid2D gf, Grid2D gm, Grid2D gc, const double* __restrict__ Leoff_f,
+                                             const double* __restrict__ Leoff_m, const T* __restrict__ Ldf,
+                                             const uint8_t* __restrict__ pinf, const TR* __restrict__ rf,
+                                             const T* __restrict__ xf, const T* __restrict__ Ldm,
+                                             const uint8_t* __restrict__ pinm, const T* __restrict__ rm,
+                                             const T* __restrict__ xm, const T* __restrict__ yc, T* __restrict__ yf,
+                                             const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    constexpr int MR = U2_LR / 2 + 3, MC = U2_LC / 2 + 3;
+    __shared__ T ym[MR][MC];
+    const int nbx = gridDim.x;
+    const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    const int j0 = (lb / nbx) * U2_LR, i0 = (lb - (lb / nbx) * nbx) * U2_LC;
+    const int mj0 = (j0 - 1) >> 1, mi0 = (i0 - 1) >> 1;
+    for (int t = threadIdx.x; t < MR * MC; t += 256) {
+        const int a = t / MC, b = t - a * MC;
+        const int mi = mi0 + b, mj = mj0 + a;
+        T v = T(0);
+        if (mi >= 0 && mi < gm.nnx && mj >= 0 && mj < gm.nny)
+            v = jacobi_pinned(gm, gc, Leoff_m, Ldm, pinm, rm, xm, mj * gm.nnx + mi,
+                              [&](int cy, int cx) { return yc[cy * gc.nnx + cx]; });
+        ym[a][b] = v;
+    }
+    __syncthreads();
+    const int j = j0 + threadIdx.x / U2_LC, i = i0 + threadIdx.x % U2_LC;
+    if (i < gf.nnx && j < gf.nny)
+        yf[j * gf.nnx + i] = jacobi_pinned(gf, gm, Leoff_f, Ldf, pinf, rf, xf, j * gf.nnx + i,
+                                           [&](int cy, int cx) { return ym[cy - mj0][cx - mi0]; });
+}
 
 // LDS-staged form of the pinned Jacobi (non-periodic grids): x and the pin flags are staged per
 // block window; the operands and operation order are those of Jacobi, so every value is bitwise
@@ -902,6 +972,22 @@ static bool vcycle_t(Ctx& c, const double* r0, const int* active, const CgTail* 
                                   JacobiS<T, T>{L.g, Lc.g, L.ec_dev->Leoff, B::Ld(L), L.pin, B::r(L), B::x(L),
                                                 B::y(Lc), B::y(L)},
                                   NoFin{});
+            continue;
+        }
+        if (!neu && !L.g.periodic && l >= 1 && c.lv[l - 1].g.nn < (1 << 17)) {
+            // this level and the next finer one in one kernel
+            Level& Lf = c.lv[l - 1];
+            const dim3 grid((Lf.g.nnx + U2_LC - 1) / U2_LC, (Lf.g.nny + U2_LR - 1) / U2_LR);
+            if (l - 1 == 0)
+                k_up2<T, double><<<grid, 256, 0, c.stream>>>(Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff,
+                                                             B::Ld(Lf), Lf.pin, r0, B::x(Lf), B::Ld(L), L.pin,
+                                                             B::r(L), B::x(L), B::y(Lc), B::y(Lf), active);
+            else
+                k_up2<T, T><<<grid, 256, 0, c.stream>>>(Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff,
+                                                        B::Ld(Lf), Lf.pin, B::r(Lf), B::x(Lf), B::Ld(L), L.pin,
+                                                        B::r(L), B::x(L), B::y(Lc), B::y(Lf), active);
+            FCFD_CHECK_LAUNCH();
+            --l;
             continue;
         }
         if (l == 0)
