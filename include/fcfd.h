@@ -204,6 +204,17 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
                             double rho, double mu, const double* U_d, const double* F_p,
                             const double* F, double* U, double* P, int32_t* iters);
 
+/* nsteps four_step_timestep calls in one native loop, for a rollout whose compute outputs
+ * (F_p, F) and Dirichlet values do not change: the compiled step loop of
+ * timestep_loop_scan (simulation.py:362-374, lax.scan over the step) and of timestep_loop
+ * (:229-360) when compute_func returns the same (F_p, F) every step.  (U0, P0) -> (U, P) after
+ * nsteps steps; the intermediate states alternate between (U, P) and a context buffer pair.
+ * iters: host int32[2 * nsteps] (U_iters, P_iters per step) or NULL.  U, P must not alias
+ * U0, P0. */
+int fcfd_four_step_rollout(fcfd_ctx* ctx, const double* U0, const double* P0, int32_t nsteps,
+                           double dt, double rho, double mu, const double* U_d, const double* F_p,
+                           const double* F, double* U, double* P, int32_t* iters);
+
 /* ---- passive scalar transport (jax_fem_cfd/solvers/passive_transport.py) ------------ */
 
 /* update_phi (:39-61) or, supg = 1, update_phi_SUPG (:14-37): Crank-Nicolson

@@ -725,6 +725,40 @@ int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n,
     });
 }
 
+int fcfd_four_step_rollout(fcfd_ctx* ctx, const double* U0, const double* P0, int32_t nsteps, double dt, double rho,
+                           double mu, const double* U_d, const double* F_p, const double* F, double* U, double* P,
+                           int32_t* iters) {
+    if (!ctx || !U0 || !P0 || !U || !P || nsteps < 1) return FCFD_ERR_ARG;
+    if (U == U0 || P == P0) return FCFD_ERR_ARG;
+    return run(ctx, [&] {
+        Ctx& c = ctx->c;
+        const size_t nn = (size_t)c.g.nn;
+        if (!c.roll_U) {
+            c.roll_U = dalloc<double>(c, (size_t)c.dim * nn);
+            c.roll_P = dalloc<double>(c, nn);
+        }
+        // the states alternate between (U, P) and the context's pair so that the last step
+        // writes (U, P)
+        const double *Us = U0, *Ps = P0;
+        for (int k = 0; k < nsteps; ++k) {
+            const bool to_out = (nsteps - 1 - k) % 2 == 0;
+            double* Ud = to_out ? U : c.roll_U;
+            double* Pd = to_out ? P : c.roll_P;
+            int it[2];
+            if (c.dim == 3 && c.order == 2) four_step_h27(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
+            else if (c.dim == 3) four_step3(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
+            else if (c.order == 2) four_step_q2(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
+            else four_step(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
+            if (iters) {
+                iters[2 * k] = it[0];
+                iters[2 * k + 1] = it[1];
+            }
+            Us = Ud;
+            Ps = Pd;
+        }
+    });
+}
+
 int fcfd_solve_scalar(fcfd_ctx* ctx, const double* U_n, const double* U, const double* phi_n, double dt, double D,
                       const double* phi_d, int32_t supg, double* phi, int32_t* iters) {
     if (!ctx || !U_n || !U || !phi_n || !phi) return FCFD_ERR_ARG;

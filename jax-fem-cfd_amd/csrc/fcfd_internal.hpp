@@ -138,6 +138,7 @@ struct Ctx {
     // adjoint (VJP) work: scalar adjoint solution (nn); U cotangent with the scalar terms, the
     // scalar part of the U_n cotangent, the U_hat cotangent and the momentum adjoint solution (2nn)
     double *a_sx = nullptr, *a_gut = nullptr, *a_guns = nullptr, *a_gus = nullptr, *a_mx = nullptr;
+    double *roll_U = nullptr, *roll_P = nullptr;  // fcfd_four_step_rollout's second state
     double* a_gel = nullptr;  // SUPG element cotangents, 16 per element (quad4), 36 (quad9)
     std::vector<void*> allocs;
 };

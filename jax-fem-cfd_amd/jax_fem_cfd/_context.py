@@ -246,6 +246,22 @@ class Context:
                                                     nat.ptr(P), iters), "fcfd_four_step_timestep")
         return U, P, int(iters[0]), int(iters[1])
 
+    def four_step_rollout(self, U0, P0, nsteps, dt, rho, mu, U_d=None, F_p=None, F=None):
+        """nsteps four_step_timestep calls in one native loop (fixed F_p, F, U_d): the compiled
+        step loop of timestep_loop_scan (simulation.py:362-374).  Returns (U, P, iters) with
+        iters an (nsteps, 2) int array of (U_iters, P_iters)."""
+        U0, P0 = nat.as_dev(U0), nat.as_dev(P0)
+        Ud = None if U_d is None else nat.as_dev(U_d)
+        Fp = _opt_vec(F_p, self.nn)
+        Fv = _opt_vec(F, self.ndim * self.nn)
+        U, P = self._empty(self.ndim * self.nn), self._empty(self.nn)
+        iters = (ctypes.c_int32 * (2 * int(nsteps)))()
+        self._sync_stream()
+        nat.check(self._lib.fcfd_four_step_rollout(self._h, nat.ptr(U0), nat.ptr(P0), int(nsteps), float(dt),
+                                                   float(rho), float(mu), nat.ptr(Ud), nat.ptr(Fp), nat.ptr(Fv),
+                                                   nat.ptr(U), nat.ptr(P), iters), "fcfd_four_step_rollout")
+        return U, P, np.frombuffer(iters, dtype=np.int32).reshape(int(nsteps), 2).copy()
+
     def solve_scalar(self, U_n, U, phi_n, dt, D, phi_d=None, supg=False):
         U_n, U, phi_n = nat.as_dev(U_n), nat.as_dev(U), nat.as_dev(phi_n)
         pd = None if phi_d is None else nat.as_dev(phi_d)

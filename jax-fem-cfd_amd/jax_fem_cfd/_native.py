@@ -74,6 +74,8 @@ _SIGS = {
     "fcfd_solve_momentum": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "fcfd_four_step_timestep": (ctypes.c_int, [_vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp, _vp, _vp,
                                                ctypes.POINTER(_i32)]),
+    "fcfd_four_step_rollout": (ctypes.c_int, [_vp, _vp, _vp, _i32, _d, _d, _d, _vp, _vp, _vp, _vp, _vp,
+                                              ctypes.POINTER(ctypes.c_int32)]),
     "fcfd_solve_scalar": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _vp, _i32, _vp, ctypes.POINTER(_i32)]),
     "fcfd_passive_timestep": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i32, _vp, _vp,
                                              _vp, ctypes.POINTER(_i32)]),

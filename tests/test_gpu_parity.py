@@ -389,6 +389,23 @@ def test_cavity_steps_match_oracle_and_are_deterministic():
 MIXED_TOL = {"f32": (1e-6, 1e-5), "f16s": (1e-5, 1e-4), "bf16s": (1e-4, 1e-3)}
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_four_step_rollout_equals_single_steps(precision):
+    """The native step loop (fcfd_four_step_rollout) runs the same kernels as one
+    fcfd_four_step_timestep call per step: after 5 steps the fields are bitwise equal and the
+    per-step iteration counts identical."""
+    n = 64
+    c, t, U_d, U0, P0, nn = _cavity(n, 1e-8, precision=precision)
+    print(f"\n[case] rollout {n} {precision}", flush=True)
+    U, P, its = U0, P0, []
+    for _ in range(5):
+        U, P, ui, pi = t(U, P, 5e-4, 1.0, 1e-3, U_d, 0, 0)
+        its.append((ui, pi))
+    Ur, Pr, itr = t.context.four_step_rollout(U0, P0, 5, 5e-4, 1.0, 1e-3, U_d, None, None)
+    assert np.array_equal(_np(Ur), _np(U)) and np.array_equal(_np(Pr), _np(P))
+    assert [tuple(map(int, r)) for r in itr] == its
+
+
 @pytest.mark.parametrize("ce_format", ["f32", "f16s", "bf16s"])
 def test_mixed_precision_cavity(ce_format):
     """Mixed form (FP32 V-cycle in FP64 flexible CG, reduced-precision Ce):
