@@ -561,9 +561,12 @@ struct ZeroIfFlag {
 // BiCGSTAB iterations after an init kernel has set x, r, rhat, p, q and the state: chunks of
 // iteration pairs (graph-replayed) until the device flag says done.  Returns the reference's
 // iteration count (or -10 / -11).
+// deferred (non-null): only the predicted chunk is enqueued, followed by an asynchronous copy of
+// (k, active) into h_pinned[8..9]; *deferred = iterations enqueued, and the caller settles the
+// solve after a later synchronisation of the stream (bicgstab_settle).
 template <int NC, bool MT, class LOp>
 int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
-                     const uint8_t* mask, int maxit, double* x, int& pred) {
+                     const uint8_t* mask, int maxit, double* x, int& pred, int* deferred = nullptr) {
     const int nn = c.g.nn;
     const int* act = &c.st->b_active;
     // one chunk = a pair of iterations, so the double-buffer parity returns to 0
@@ -594,6 +597,12 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
     int launched = 0;
     int pairs = pred > 0 ? (pred + 1) / 2 : 1;
     int h[2] = {0, 0};
+    if (deferred) {
+        replay(c, graph, key, kptr, pairs, pair);
+        FCFD_HIP(hipMemcpyAsync(c.h_pinned + 8, &c.st->b_k, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        *deferred = 2 * pairs;
+        return 0;
+    }
     while (true) {
         replay(c, graph, key, kptr, pairs, pair);
         launched += 2 * pairs;
@@ -605,18 +614,34 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
     return h[0];
 }
 
+// Settles a deferred solve once the stream has been synchronised past its state copy: returns
+// the iteration count; if the enqueued chunk had not converged, completes the solve here
+// (synchronously) and sets *resumed.
+template <int NC, bool MT, class LOp>
+int bicgstab_settle(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
+                    const uint8_t* mask, int maxit, double* x, int& pred, int launched, bool* resumed) {
+    const int k = c.h_pinned[8], active = c.h_pinned[9];
+    *resumed = false;
+    if (active && launched < maxit) {
+        *resumed = true;
+        return bicgstab_iterate<NC, MT>(c, graph, key, kptr, lhs, mask, maxit - launched, x, pred);
+    }
+    if (k > 0) pred = k;
+    return k;
+}
+
 // Solves A x = R x_n + extra (Dirichlet rows: x = ud) from x0 = x_n with Dirichlet values, into
 // `x` (NC * nn, a context buffer: the captured iteration graph keeps its address).  Returns the
 // reference's iteration count (or -10 / -11).
 template <int NC, class LOp, class ROp, class Extra>
 int bicgstab_solve(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
                    const ROp& rhs, const Extra& extra, const double* xn, const double* ud, const uint8_t* mask,
-                   int maxit, double* x, int& pred) {
+                   int maxit, double* x, int& pred, int* deferred = nullptr) {
     launch_node<2>(c, c.g.nn, nullptr,
                    BInit<NC, LOp, ROp, Extra>{lhs, rhs, extra, xn, ud, mask, x, c.m_r, c.m_rhat},
                    BInitFin{c.st, c.tol, maxit});
     FCFD_CHECK_LAUNCH();
-    return bicgstab_iterate<NC, false>(c, graph, key, kptr, lhs, mask, maxit, x, pred);
+    return bicgstab_iterate<NC, false>(c, graph, key, kptr, lhs, mask, maxit, x, pred, deferred);
 }
 
 // Transposed solve A^T x = b (adjoint of a masked forward solve), x0 = x_n with Dirichlet values.

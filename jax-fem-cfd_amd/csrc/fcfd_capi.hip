@@ -439,9 +439,19 @@ void four_step(Ctx& c, const double* U_n, const double* P_n, double dt, double r
                const double* F_p, const double* F, double* U, double* P, int* it) {
     scatter_ud(c, U_d);
     build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
-    it[0] = solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x);
+    // The momentum solve's predicted chunk is enqueued without waiting for its iteration count:
+    // the pressure solve's own synchronisation also delivers it.  If that chunk had not
+    // converged, the momentum solve is completed and U* and the pressure solve are redone.
+    int launched = 0;
+    solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x, &launched);
     projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
     it[1] = solve_pressure(c, c.ustar, P_n, F_p, rho, dt, P);
+    bool resumed = false;
+    it[0] = settle_momentum(c, c.ce_format, c.ce, c.ce_scale, rho, mu, dt, launched, &resumed);
+    if (resumed) {
+        projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
+        it[1] = solve_pressure(c, c.ustar, P_n, F_p, rho, dt, P);
+    }
     projection_final(c, c.ustar, P, rho, dt, U);
 }
 

@@ -259,8 +259,12 @@ void axpby(Ctx& c, double a, const double* x, double b, double* y, int n);
 float bench_momentum_apply(Ctx& c, int reps);
 
 // momentum (fcfd_momentum.hip)
+// deferred (non-null, U_hat = c.m_x): the predicted BiCGSTAB chunk only, settled later by
+// settle_momentum after a synchronisation of the stream (see bicgstab_iterate)
 int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n, const double* P_n, double rho,
-                   double mu, double dt, const double* F, double* U_hat);
+                   double mu, double dt, const double* F, double* U_hat, int* deferred = nullptr);
+int settle_momentum(Ctx& c, int fmt, void* store, float* scale, double rho, double mu, double dt, int launched,
+                    bool* resumed);
 
 // passive scalar (fcfd_scalar.hip)
 int solve_scalar(Ctx& c, const double* U_n, const double* U, const double* phi_n, double dt, double D,

@@ -25,35 +25,62 @@ struct MomExtra {
 };
 
 template <class S>
-static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, double rho, double mu, double dt,
-                            const double* F, double* U_hat) {
+struct MomOps {
+    MomCoef<S> lhs, rhs;
+    int maxit;
+    double key[4];
+};
+template <class S>
+static MomOps<S> mom_ops(Ctx& c, S ce, double rho, double mu, double dt) {
     const double theta = 0.5;
     const Grid2D& g = c.g;
-    MomCoef<S> lhs{g, c.ec_dev, ce, rho / dt, theta * mu, theta * rho};
-    MomCoef<S> rhs{g, c.ec_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
-    const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 2 * g.nn;
-    const double key[4] = {rho, mu, dt, 0.0};
-    const int it = bicgstab_solve<2>(c, c.g_mom, key, ce.c, lhs, rhs, MomExtra{g, c.ec_dev, P_n, F}, U_n, c.ud_full,
-                                     c.dmask, maxit, c.m_x, c.pred_mom);
+    return MomOps<S>{MomCoef<S>{g, c.ec_dev, ce, rho / dt, theta * mu, theta * rho},
+                     MomCoef<S>{g, c.ec_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho},
+                     c.maxit_mom > 0 ? c.maxit_mom : 10 * 2 * g.nn, {rho, mu, dt, 0.0}};
+}
+
+// deferred: see bicgstab_iterate (U_hat must then be c.m_x)
+template <class S>
+static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, double rho, double mu, double dt,
+                            const double* F, double* U_hat, int* deferred) {
+    const Grid2D& g = c.g;
+    const MomOps<S> o = mom_ops(c, ce, rho, mu, dt);
+    const int it = bicgstab_solve<2>(c, c.g_mom, o.key, ce.c, o.lhs, o.rhs, MomExtra{g, c.ec_dev, P_n, F}, U_n,
+                                     c.ud_full, c.dmask, o.maxit, c.m_x, c.pred_mom, deferred);
     if (U_hat != c.m_x)
         FCFD_HIP(hipMemcpyAsync(U_hat, c.m_x, sizeof(double) * 2 * g.nn, hipMemcpyDeviceToDevice, c.stream));
     return it;
 }
 
-int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n, const double* P_n, double rho,
-                   double mu, double dt, const double* F, double* U_hat) {
+template <class S>
+static int settle_momentum_t(Ctx& c, S ce, double rho, double mu, double dt, int launched, bool* resumed) {
+    const MomOps<S> o = mom_ops(c, ce, rho, mu, dt);
+    return bicgstab_settle<2, false>(c, c.g_mom, o.key, ce.c, o.lhs, c.dmask, o.maxit, c.m_x, c.pred_mom, launched,
+                                     resumed);
+}
+
+template <class F>
+static int with_ce(Ctx& c, int fmt, void* store, float* scale, F&& f) {
     switch (fmt) {
-        case FCFD_CE_F64:
-            return solve_momentum_t(c, CeF64{(double*)store, c.g.ne}, U_n, P_n, rho, mu, dt, F, U_hat);
-        case FCFD_CE_F32:
-            return solve_momentum_t(c, CeF32{(float*)store, c.g.ne}, U_n, P_n, rho, mu, dt, F, U_hat);
-        case FCFD_CE_F16S:
-            return solve_momentum_t(c, CeF16S{(__half*)store, scale, c.g.ne}, U_n, P_n, rho, mu, dt, F, U_hat);
-        case FCFD_CE_BF16S:
-            return solve_momentum_t(c, CeBF16S{(__hip_bfloat16*)store, scale, c.g.ne}, U_n, P_n, rho, mu, dt, F,
-                                    U_hat);
+        case FCFD_CE_F64: return f(CeF64{(double*)store, c.g.ne});
+        case FCFD_CE_F32: return f(CeF32{(float*)store, c.g.ne});
+        case FCFD_CE_F16S: return f(CeF16S{(__half*)store, scale, c.g.ne});
+        case FCFD_CE_BF16S: return f(CeBF16S{(__hip_bfloat16*)store, scale, c.g.ne});
     }
     throw Error(FCFD_ERR_ARG, "unknown Ce format");
+}
+
+int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n, const double* P_n, double rho,
+                   double mu, double dt, const double* F, double* U_hat, int* deferred) {
+    if (deferred && U_hat != c.m_x) throw Error(FCFD_ERR_STATE, "a deferred momentum solve works in place");
+    return with_ce(c, fmt, store, scale,
+                   [&](auto ce) { return solve_momentum_t(c, ce, U_n, P_n, rho, mu, dt, F, U_hat, deferred); });
+}
+
+int settle_momentum(Ctx& c, int fmt, void* store, float* scale, double rho, double mu, double dt, int launched,
+                    bool* resumed) {
+    return with_ce(c, fmt, store, scale,
+                   [&](auto ce) { return settle_momentum_t(c, ce, rho, mu, dt, launched, resumed); });
 }
 
 // ------------------------------------------------------------------------------ benchmark
