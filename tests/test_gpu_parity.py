@@ -386,6 +386,23 @@ def test_cavity_steps_match_oracle_and_are_deterministic():
     assert np.array_equal(_np(Ua), _np(Ub)) and np.array_equal(_np(Pa), _np(Pb))
 
 
+def test_step_whose_momentum_chunk_is_too_short():
+    """A fresh context predicts one BiCGSTAB pair (2 iterations) for its first momentum solve;
+    a first step from a strongly perturbed state at tol 1e-12 needs more, so the step settles
+    its momentum solve after the pressure solve, completes it and redoes U* and the pressure
+    solve.  Fields and iteration counts must be those of the oracle's step."""
+    n = 64
+    compute, timestep, U_d, U0, P0, nn = _cavity(n, 1e-12)
+    s, surf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-12, True, "multigrid", [4, 4])
+    oUd = ns.cavity_lid(s, surf, oUd)
+    U0 = U0 + np.random.default_rng(3).normal(0, 0.1, U0.shape)
+    Uo, Po, uio, pio = ns.four_step_timestep(s, U0.copy(), P0.copy(), 5e-4, 1.0, 1e-3, oUd, 0, 0)
+    U, P, ui, pi = timestep(U0, P0, 5e-4, 1.0, 1e-3, U_d, 0, 0)
+    print(f"\n[case] short momentum chunk: iters {ui},{pi} oracle {uio},{pio}", flush=True)
+    assert uio > 2 and (ui, pi) == (uio, pio)
+    assert _rel(_np(U), Uo) < 1e-10 and _rel(_np(P), Po) < 1e-10
+
+
 MIXED_TOL = {"f32": (1e-6, 1e-5), "f16s": (1e-5, 1e-4), "bf16s": (1e-4, 1e-3)}
 
 
