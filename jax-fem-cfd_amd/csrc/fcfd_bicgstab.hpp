@@ -566,49 +566,50 @@ struct ZeroIfFlag {
 // solve after a later synchronisation of the stream (bicgstab_settle).
 template <int NC, bool MT, class LOp>
 int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
-                     const uint8_t* mask, int maxit, double* x, int& pred, int* deferred = nullptr) {
+                     const uint8_t* mask, int maxit, double* x, int& pred, int* deferred = nullptr, int start = 0) {
     const int nn = c.g.nn;
     const int* act = &c.st->b_active;
-    // one chunk = a pair of iterations, so the double-buffer parity returns to 0
-    auto pair = [&]() {
-        for (int o = 0; o < 2; ++o) {
-            const int nw = o ^ 1;
-            if (c.g.periodic) {
-                launch_node<1>(c, nn, act,
-                               BK1<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
+    // one iteration on the double buffers of parity o
+    auto one = [&](int o) {
+        const int nw = o ^ 1;
+        if (c.g.periodic) {
+            launch_node<1>(c, nn, act,
+                           BK1<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw],
+                                            c.st},
+                           BK1Fin{c.st});
+            launch_node<3>(c, nn, act, BK2<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st}, BK2Fin{c.st});
+        } else {
+            launch_stencil<1>(c, nn, act,
+                              BK1S<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
                                                 c.m_q[nw], c.st},
-                               BK1Fin{c.st});
-                launch_node<3>(c, nn, act, BK2<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st},
-                               BK2Fin{c.st});
-            } else {
-                launch_stencil<1>(c, nn, act,
-                                  BK1S<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
-                                                    c.m_q[nw], c.st},
-                                  BK1Fin{c.st});
-                launch_stencil<3>(c, nn, act,
-                                  BK2S<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st},
-                                  BK2Fin{c.st});
-            }
-            launch_node<2>(c, nn, act, BK3R<NC>{nn, c.m_p[nw], c.m_q[nw], c.m_t, c.m_rhat, x, c.m_r, c.st},
-                           BK3Fin{c.st});
+                              BK1Fin{c.st});
+            launch_stencil<3>(c, nn, act, BK2S<NC, LOp, 0, MT>{lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st},
+                              BK2Fin{c.st});
         }
+        launch_node<2>(c, nn, act, BK3R<NC>{nn, c.m_p[nw], c.m_q[nw], c.m_t, c.m_rhat, x, c.m_r, c.st},
+                       BK3Fin{c.st});
         FCFD_CHECK_LAUNCH();
     };
-    int launched = 0;
-    int pairs = pred > 0 ? (pred + 1) / 2 : 1;
+    auto pair = [&]() {
+        one(0);
+        one(1);
+    };
+    // the first chunk is as long as the previous solve took; later chunks add two iterations
+    int launched = start;
+    int n = start > 0 ? 2 : (pred > 0 ? pred : 2);
     int h[2] = {0, 0};
     if (deferred) {
-        replay(c, graph, key, kptr, pairs, pair);
+        enqueue_iters(c, graph, key, kptr, launched & 1, n, one, pair);
         FCFD_HIP(hipMemcpyAsync(c.h_pinned + 8, &c.st->b_k, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        *deferred = 2 * pairs;
+        *deferred = n;
         return 0;
     }
     while (true) {
-        replay(c, graph, key, kptr, pairs, pair);
-        launched += 2 * pairs;
+        enqueue_iters(c, graph, key, kptr, launched & 1, n, one, pair);
+        launched += n;
         read_state_ints(c, &c.st->b_k, 2, h);  // b_k, b_active
         if (!h[1] || launched >= maxit) break;
-        pairs = 1;
+        n = 2;
     }
     if (h[0] > 0) pred = h[0];
     return h[0];
@@ -624,7 +625,7 @@ int bicgstab_settle(Ctx& c, IterGraph& graph, const double (&key)[4], const void
     *resumed = false;
     if (active && launched < maxit) {
         *resumed = true;
-        return bicgstab_iterate<NC, MT>(c, graph, key, kptr, lhs, mask, maxit - launched, x, pred);
+        return bicgstab_iterate<NC, MT>(c, graph, key, kptr, lhs, mask, maxit, x, pred, nullptr, launched);
     }
     if (k > 0) pred = k;
     return k;

@@ -19,6 +19,10 @@ void leave(Ctx& c) {
 void graph_reset(IterGraph& g) {
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     g.exec = nullptr;
+    for (int p = 0; p < 2; ++p) {
+        if (g.one[p]) (void)hipGraphExecDestroy(g.one[p]);
+        g.one[p] = nullptr;
+    }
 }
 }  // namespace fcfd
 

@@ -59,9 +59,12 @@ struct Level {
 // One captured chunk of solver iterations (a pair, so the double-buffer parity returns to 0),
 // replayed with hipGraphLaunch.  `key` holds the scalars baked into the kernel arguments.
 struct IterGraph {
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec = nullptr;  // a pair of iterations (parity 0 then 1)
     double key[4] = {0, 0, 0, 0};
     const void* kptr = nullptr;
+    hipGraphExec_t one[2] = {nullptr, nullptr};  // a single iteration of parity 0 / 1
+    double okey[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    const void* okptr[2] = {nullptr, nullptr};
 };
 
 struct Ctx {
@@ -215,7 +218,8 @@ inline void replay(Ctx& c, IterGraph& g, const double (&key)[4], const void* kpt
     }
     const bool ok = g.exec && g.kptr == kptr && std::memcmp(g.key, key, sizeof(key)) == 0;
     if (!ok) {
-        graph_reset(g);
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);  // the pair graph only (see enqueue_iters)
+        g.exec = nullptr;
         hipGraph_t graph = nullptr;
         FCFD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
         try {
@@ -236,6 +240,55 @@ inline void replay(Ctx& c, IterGraph& g, const double (&key)[4], const void* kpt
         g.kptr = kptr;
     }
     for (int i = 0; i < reps; ++i) FCFD_HIP(hipGraphLaunch(g.exec, c.stream));
+}
+
+// Captures `body` into exec (keyed by key, kptr) on first use, then launches it.
+template <class F>
+inline void replay_exec(Ctx& c, hipGraphExec_t& exec, double (&gkey)[4], const void*& gkptr, const double (&key)[4],
+                        const void* kptr, F&& body) {
+    if (!c.use_graphs) {
+        body();
+        return;
+    }
+    const bool ok = exec && gkptr == kptr && std::memcmp(gkey, key, sizeof(key)) == 0;
+    if (!ok) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        hipGraph_t graph = nullptr;
+        FCFD_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(c.stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        FCFD_HIP(hipStreamEndCapture(c.stream, &graph));
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            exec = nullptr;
+            throw Error(FCFD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        std::memcpy(gkey, key, sizeof(key));
+        gkptr = kptr;
+    }
+    FCFD_HIP(hipGraphLaunch(exec, c.stream));
+}
+
+// Enqueues n solver iterations, the first of parity p (iteration i uses the double buffers of
+// parity (p + i) % 2): a single iteration up to parity 0, then pairs, then a trailing single
+// one -- so a chunk is exactly as long as predicted, not rounded up to a pair.
+template <class ONE, class PAIR>
+inline void enqueue_iters(Ctx& c, IterGraph& g, const double (&key)[4], const void* kptr, int p, int n, ONE&& one,
+                          PAIR&& pair) {
+    if (n <= 0) return;
+    if (p == 1) {
+        replay_exec(c, g.one[1], g.okey[1], g.okptr[1], key, kptr, [&]() { one(1); });
+        --n;
+    }
+    if (n >= 2) replay(c, g, key, kptr, n / 2, pair);
+    if (n % 2) replay_exec(c, g.one[0], g.okey[0], g.okptr[0], key, kptr, [&]() { one(0); });
 }
 
 // host element constants --------------------------------------------------------------------

@@ -1226,39 +1226,42 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
                        CgGamma0Fin{c.st});
     FCFD_CHECK_LAUNCH();
     // one chunk = a pair of PCG iterations (parity of the p / z double buffers returns to 0)
-    auto pair = [&]() {
-        for (int o = 0; o < 2; ++o) {
-            const int nw = o ^ 1;
-            if (g.periodic)
-                launch_node<1>(c, nn, act, CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                               CgK1Fin{c.st});
-            else
-                launch_stencil<1>(c, nn, act, CgK1S{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                                  CgK1Fin{c.st});
-            launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
-            bool f, fused;
-            const CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
-            const void* y = precondition(c, act, f, neu ? nullptr : &tail, &fused);
-            if (!fused && f)
-                launch_node<2>(c, nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
-                               CgK3Fin{c.st, flex});
-            else if (!fused)
-                launch_node<2>(c, nn, act,
-                               CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
-                               CgK3Fin{c.st, flex});
-        }
+    // one PCG iteration on the p / z double buffers of parity o
+    auto one = [&](int o) {
+        const int nw = o ^ 1;
+        if (g.periodic)
+            launch_node<1>(c, nn, act, CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                           CgK1Fin{c.st});
+        else
+            launch_stencil<1>(c, nn, act, CgK1S{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                              CgK1Fin{c.st});
+        launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
+        bool f, fused;
+        const CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
+        const void* y = precondition(c, act, f, neu ? nullptr : &tail, &fused);
+        if (!fused && f)
+            launch_node<2>(c, nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                           CgK3Fin{c.st, flex});
+        else if (!fused)
+            launch_node<2>(c, nn, act, CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                           CgK3Fin{c.st, flex});
         FCFD_CHECK_LAUNCH();
     };
+    auto pair = [&]() {
+        one(0);
+        one(1);
+    };
     const double key[4] = {0.0, 0.0, 0.0, 0.0};
+    // the first chunk is as long as the previous solve took; later chunks add two iterations
     int launched = 0;
-    int pairs = c.pred_p > 0 ? (c.pred_p + 1) / 2 : 1;
+    int n = c.pred_p > 0 ? c.pred_p : 2;
     int h[2] = {0, 0};
     while (true) {
-        replay(c, c.g_cg, key, nullptr, pairs, pair);
-        launched += 2 * pairs;
+        enqueue_iters(c, c.g_cg, key, nullptr, launched & 1, n, one, pair);
+        launched += n;
+        n = 2;
         read_state_ints(c, &c.st->c_k, 2, h);  // c_k, c_active
         if (!h[1] || launched >= maxit) break;
-        pairs = 1;
     }
     if (converged) *converged = !h[1] && h[0] <= maxit;
     if (h[0] > 0) c.pred_p = h[0];
