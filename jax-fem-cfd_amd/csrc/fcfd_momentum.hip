@@ -24,6 +24,97 @@ struct MomExtra {
     __device__ __forceinline__ void operator()(const NodeNbr&, int, double (&)[1]) const {}
 };
 
+// LDS-staged momentum init (non-periodic grids): BInit<2, MomCoef<S>, MomCoef<S>, MomExtra> with
+// the stencils of x_n (both components), x0 (x_n with the Dirichlet values) and P_n staged per
+// block window (5 values per node) instead of gathered from global memory: the node-form init
+// held its 3x3 stencils of all of them in registers (~160 VGPRs, 3 waves per SIMD).  Operands and
+// operation order are BInit's (window entries of wrapped or missing neighbours only reach
+// elements that has[k] masks), so every value is bitwise the same.
+template <class S>
+struct BInitMomS {
+    static constexpr int NV = 5;
+    static constexpr int NPT = 1;
+    MomCoef<S> lhs, rhs;
+    const ElemConst* __restrict__ ec;
+    const double *Pn, *F, *xn, *ud;
+    const uint8_t* mask;
+    double *x, *r, *rhat;
+    __device__ __forceinline__ int nnx() const { return lhs.g.nnx; }
+    __device__ __forceinline__ void stage(int id, double (&v)[5]) const {
+        const int nn = lhs.g.nn;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const double xv = xn[id + c * nn];
+            v[c] = xv;
+            v[2 + c] = (mask && mask[id + c * nn]) ? ud[id + c * nn] : xv;
+        }
+        v[4] = Pn[id];
+    }
+    __device__ void operator()(int n, const StageView<5, 1>& sv, double (&acc)[2]) const {
+        const int nn = lhs.g.nn;
+        NodeNbr s;
+        node_elems(lhs.g, n, s);
+        // stencil values are read from the window where they are used (op_apply_pair's and
+        // ebe_const's operation order), not held as 3x3 register arrays
+        double b[2] = {0.0, 0.0}, ax[2] = {0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int a = kRow(k);
+            const int e = s.e[k];
+            double v[4];
+            lhs.ce.ld_row(a, e, v);
+            double cl[4], cr[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                cl[q] = lhs.cM * lhs.ec->Me[a * 4 + q] + lhs.cL * lhs.ec->Le[a * 4 + q] + lhs.cC * v[q];
+                cr[q] = rhs.cM * rhs.ec->Me[a * 4 + q] + rhs.cL * rhs.ec->Le[a * 4 + q] + rhs.cC * v[q];
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                double tl = 0.0, tr = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int dy = kCy(k, q) - 1, dx = kCx(k, q) - 1;
+                    tl += cl[q] * sv.at(2 + c, n, dy, dx);
+                    tr += cr[q] * sv.at(c, n, dy, dx);
+                }
+                ax[c] += s.has[k] ? tl : 0.0;
+                b[c] += s.has[k] ? tr : 0.0;
+            }
+        }
+        double gx = 0.0, gy = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int a = kRow(k);
+            double tx = 0.0, ty = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double p = sv.at(4, n, kCy(k, q) - 1, kCx(k, q) - 1);
+                tx += ec->Gx[a * 4 + q] * p;
+                ty += ec->Gy[a * 4 + q] * p;
+            }
+            gx += s.has[k] ? tx : 0.0;
+            gy += s.has[k] ? ty : 0.0;
+        }
+        b[0] -= gx;
+        b[1] -= gy;
+        if (F) { b[0] += F[n]; b[1] += F[n + nn]; }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int i = n + c * nn;
+            const bool m = mask && mask[i];
+            const double bc = m ? ud[i] : b[c];
+            const double x0 = sv.at(2 + c, n, 0, 0);
+            const double a = m ? x0 : ax[c];
+            const double rc = bc - a;
+            x[i] = x0;
+            r[i] = rc; rhat[i] = rc;
+            acc[0] += bc * bc;
+            acc[1] += rc * rc;
+        }
+    }
+};
+
 template <class S>
 struct MomOps {
     MomCoef<S> lhs, rhs;
@@ -45,8 +136,19 @@ static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, 
                             const double* F, double* U_hat, int* deferred) {
     const Grid2D& g = c.g;
     const MomOps<S> o = mom_ops(c, ce, rho, mu, dt);
-    const int it = bicgstab_solve<2>(c, c.g_mom, o.key, ce.c, o.lhs, o.rhs, MomExtra{g, c.ec_dev, P_n, F}, U_n,
-                                     c.ud_full, c.dmask, o.maxit, c.m_x, c.pred_mom, deferred);
+    int it;
+    if (!g.periodic) {
+        launch_stencil<2>(c, g.nn, nullptr,
+                          BInitMomS<S>{o.lhs, o.rhs, c.ec_dev, P_n, F, U_n, c.ud_full, c.dmask, c.m_x, c.m_r,
+                                       c.m_rhat},
+                          BInitFin{c.st, c.tol, o.maxit});
+        FCFD_CHECK_LAUNCH();
+        it = bicgstab_iterate<2, false>(c, c.g_mom, o.key, ce.c, o.lhs, c.dmask, o.maxit, c.m_x, c.pred_mom,
+                                        deferred);
+    } else {
+        it = bicgstab_solve<2>(c, c.g_mom, o.key, ce.c, o.lhs, o.rhs, MomExtra{g, c.ec_dev, P_n, F}, U_n, c.ud_full,
+                               c.dmask, o.maxit, c.m_x, c.pred_mom, deferred);
+    }
     if (U_hat != c.m_x)
         FCFD_HIP(hipMemcpyAsync(U_hat, c.m_x, sizeof(double) * 2 * g.nn, hipMemcpyDeviceToDevice, c.stream));
     return it;
