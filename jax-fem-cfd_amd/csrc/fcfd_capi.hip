@@ -439,9 +439,10 @@ void destroy_ctx(Ctx& c) {
 }
 
 // four_step_timestep (standard.py:148-165)
+// scatter = false: the Dirichlet values are already in place (a rollout scatters them once)
 void four_step(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
-               const double* F_p, const double* F, double* U, double* P, int* it) {
-    scatter_ud(c, U_d);
+               const double* F_p, const double* F, double* U, double* P, int* it, bool scatter = true) {
+    if (scatter) scatter_ud(c, U_d);
     build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
     // The momentum solve's predicted chunk is enqueued without waiting for its iteration count:
     // the pressure solve's own synchronisation also delivers it.  If that chunk had not
@@ -762,7 +763,7 @@ int fcfd_four_step_rollout(fcfd_ctx* ctx, const double* U0, const double* P0, in
             if (c.dim == 3 && c.order == 2) four_step_h27(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
             else if (c.dim == 3) four_step3(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
             else if (c.order == 2) four_step_q2(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
-            else four_step(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
+            else four_step(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it, k == 0);
             if (iters) {
                 iters[2 * k] = it[0];
                 iters[2 * k + 1] = it[1];
