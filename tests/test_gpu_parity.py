@@ -3,8 +3,8 @@ same seeded inputs.
 
 Tolerances (FP64 form): integer / index work bit-exact; single element and
 matvec kernels 1e-13 relative (different summation order than the oracle's
-BLAS); solves and timesteps 1e-10 relative L2, which is the accuracy the
-north-star sets for the FP64 form, with identical iteration counts.
+BLAS); solves and timesteps 1e-10 relative L2, the accuracy set for the FP64
+form, with identical iteration counts.
 """
 import numpy as np
 import pytest
