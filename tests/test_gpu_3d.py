@@ -100,8 +100,8 @@ def test_channel_steps_match_oracle(precond):
 
 
 def test_channel_mixed_form_error():
-    """FP32-stored Ce (mixed form): per-field error against the FP64 oracle after 5 steps
-    stays at the FP32 rounding level of the convection matrices (tolerance 1e-6)."""
+    """FP32-stored Ce (mixed form): per-field error (u, v, w, p) against the FP64 oracle after 5
+    steps stays at the FP32 rounding level of the convection matrices (tolerance 1e-6 each)."""
     n, dt, rho, mu = (8, 4, 4), 2e-3, 1.0, 1e-3
     print(f"\n[case] hex8 channel {n} mixed", flush=True)
     compute, timestep, s, surf, U_d = _channel(n, precision="mixed")
@@ -119,7 +119,7 @@ def test_channel_mixed_form_error():
     errs = [_rel(Ug[d * s.nn:(d + 1) * s.nn], U[d * s.nn:(d + 1) * s.nn]) for d in range(3)]
     ep = _rel(_np(Pg), P)
     print(f"  rel u {errs[0]:.2e} v {errs[1]:.2e} w {errs[2]:.2e} p {ep:.2e}", flush=True)
-    assert errs[0] < 1e-6 and ep < 1e-6
+    assert max(errs) < 1e-6 and ep < 1e-6
 
 
 def test_uniform_flow_is_kept():
