@@ -169,6 +169,8 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     {
         const char* e = std::getenv("FCFD_NO_GRAPHS");
         c.use_graphs = !(e && e[0] == '1');
+        const char* t = std::getenv("FCFD_MG_TILED");
+        c.mg_tiled = !(t && t[0] == '0');
     }
     c.tol = s->tol;
     c.set_zeroP = s->set_zeroP;
@@ -240,6 +242,23 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
         std::vector<float> Ld32(Ld.begin(), Ld.end());
         L.Ld32 = dalloc<float>(c, L.g.nn);
         upload(Ld32, L.Ld32);
+        // L_diag by node class (first / interior / last along each direction); the tiled V-cycle
+        // kernels read it from this table, so every node must carry its class's value
+        {
+            auto rep = [](int cls, int n) { return cls == 0 ? 0 : (cls == 2 ? n - 1 : (n > 2 ? 1 : 0)); };
+            for (int cj = 0; cj < 3; ++cj)
+                for (int ci = 0; ci < 3; ++ci)
+                    L.ldc[cj][ci] = Ld[(size_t)rep(cj, L.g.nny) * L.g.nnx + rep(ci, L.g.nnx)];
+            for (int j = 0; j < L.g.nny && c.mg_tiled; ++j)
+                for (int i = 0; i < L.g.nnx; ++i) {
+                    const int ci = i == 0 ? 0 : (i == L.g.nnx - 1 ? 2 : 1);
+                    const int cj = j == 0 ? 0 : (j == L.g.nny - 1 ? 2 : 1);
+                    if (Ld[(size_t)j * L.g.nnx + i] != L.ldc[cj][ci]) {
+                        c.mg_tiled = false;
+                        break;
+                    }
+                }
+        }
         const int n = L.g.nn;
         if (l > 0) L.r = dalloc<double>(c, n);
         L.x = dalloc<double>(c, n);
@@ -411,6 +430,7 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     c.m_t = dalloc<double>(c, 2 * nn);
     c.m_x = dalloc<double>(c, 2 * nn);
     c.c_r = dalloc<double>(c, nn);
+    if (c.precond == FCFD_PC_MULTIGRID && c.set_zeroP) c.c_r2 = dalloc<double>(c, nn);
     c.c_Ap = dalloc<double>(c, nn);
     c.c_x = dalloc<double>(c, nn);
     c.c_rhs = dalloc<double>(c, nn);

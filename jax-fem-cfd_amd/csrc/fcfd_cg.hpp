@@ -98,6 +98,17 @@ struct CgK3Fin {
     }
 };
 
+// CgK3Fin with the iteration's r . r arriving in the same finalize (t[2]: the x / r update fused
+// into the V-cycle's first down leg)
+struct CgK3RFin {
+    DevState* st; int flex;
+    __device__ void operator()(const double (&t)[3]) const {
+        st->c_rr = t[2];
+        const double t2[2] = {t[0], t[1]};
+        CgK3Fin{st, flex}(t2);
+    }
+};
+
 struct JacobiPrec {
     const double* r; const double* Ld; double* y;
     __device__ void operator()(int k, double (&)[1]) const { y[k] = r[k] / Ld[k]; }

@@ -427,4 +427,34 @@ __global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(int nb, const int* 
     if (threadIdx.x == 0) fin(tot);
 }
 
+// Two groups of partials with different block counts: slots [0, NR1) hold nb1 partials each,
+// slots [NR1, NR1 + NR2) hold nb2 each (two reducing kernels feeding one scalar update).  Each
+// slot is summed in the fixed order of finalize_kernel.
+template <int NR1, int NR2, class Fin>
+__global__ __launch_bounds__(FIN_BLOCK) void finalize2_kernel(int nb1, int nb2, const int* __restrict__ active, Fin fin,
+                                                             const double* __restrict__ partials, int pstride) {
+    if (active && *active == 0) return;
+    constexpr int NR = NR1 + NR2, FIN_U = 8;
+    double tot[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) tot[r] = 0.0;
+    const int nbm = nb1 > nb2 ? nb1 : nb2;
+    for (int k0 = threadIdx.x; k0 < nbm; k0 += FIN_BLOCK * FIN_U) {
+        double v[NR][FIN_U];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int u = 0; u < FIN_U; ++u) {
+                const int k = k0 + u * FIN_BLOCK;
+                v[r][u] = k < (r < NR1 ? nb1 : nb2) ? partials[r * pstride + k] : 0.0;
+            }
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int u = 0; u < FIN_U; ++u) tot[r] += v[r][u];
+    }
+    block_sum<NR>(tot);
+    if (threadIdx.x == 0) fin(tot);
+}
+
 }  // namespace fcfd

@@ -49,6 +49,7 @@ struct Level {
     double* Ld = nullptr;       // L_diag
     float* Ld32 = nullptr;
     uint8_t* pin = nullptr;     // pinned form: 1 at gamma_p of this level
+    double ldc[3][3] = {{0}};   // L_diag by node class [j class][i class] (first, interior, last)
     std::vector<int64_t> pins;
     // V-cycle vectors, FP64 form
     double *r = nullptr, *x = nullptr, *y = nullptr, *f = nullptr, *y2 = nullptr;
@@ -73,6 +74,7 @@ struct Ctx {
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     IterGraph g_mom, g_cg, g_phi, g_amom, g_aphi;
     bool use_graphs = true;
+    bool mg_tiled = true;   // pinned non-periodic quad4 V-cycle: the tiled kernels (L_diag by node class)
     Grid2D g;
     double lx, ly;
     ElemConst ec;
@@ -119,6 +121,7 @@ struct Ctx {
            *m_s = nullptr, *m_t = nullptr, *m_x = nullptr;
     // pressure work (nn)
     double *c_r = nullptr, *c_p[2] = {nullptr, nullptr}, *c_Ap = nullptr, *c_x = nullptr, *c_rhs = nullptr;
+    double* c_r2 = nullptr;  // second residual buffer of the PCG whose x / r update runs in the V-cycle
     double* c_z[2] = {nullptr, nullptr};
     double* ustar = nullptr;  // (2nn)
     // coarse CG scratch (coarsest level)

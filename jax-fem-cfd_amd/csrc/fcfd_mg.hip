@@ -13,6 +13,7 @@
 #include "fcfd_ops.hpp"
 #include "fcfd_cg.hpp"
 #include "fcfd_mgnodes.hpp"
+#include "fcfd_mgtile.hpp"
 
 namespace fcfd {
 
@@ -354,59 +355,6 @@ static void launch_down_fused(Ctx& c, const Grid2D& gf, const Grid2D& gc, const 
 }
 
 // ------------------------------------------------------------------------------ up leg
-// interp_at at all 9 stencil points of a node from one 3x3 patch of coarse values: fine index f
-// interpolates coarse (f >> 1, (f + 1) >> 1), so fine f - 1 .. f + 1 touch coarse
-// floor((f - 1) / 2) + {0, 1, 2}.  Which patch entries pair up depends only on the parity of the
-// node's (i, j): selected per lane, with the same operands and operation order as interp_at, so
-// every value is bitwise the one interp_at returns.  9 coarse loads instead of 36.
-// yv(cy, cx): the coarse value at coarse row cy, column cx (global memory, or a block's LDS copy)
-template <class T, class YV>
-__device__ __forceinline__ void interp3x3_v(const Grid2D& gc, YV yv, T my, const NodeNbr& s, T (&Y)[3][3]) {
-    const int bx = (s.i - 1) >> 1, by = (s.j - 1) >> 1;  // arithmetic shift: floor
-    int cx[3], cy[3];
-#pragma unroll
-    for (int t = 0; t < 3; ++t) {
-        int x = bx + t, y = by + t;
-        if (gc.periodic) {
-            x = x < 0 ? x + gc.nnx : (x >= gc.nnx ? x - gc.nnx : x);
-            y = y < 0 ? y + gc.nny : (y >= gc.nny ? y - gc.nny : y);
-        } else {
-            x = x < 0 ? 0 : (x >= gc.nnx ? gc.nnx - 1 : x);
-            y = y < 0 ? 0 : (y >= gc.nny ? gc.nny - 1 : y);
-        }
-        cx[t] = x;
-        cy[t] = y;
-    }
-    T P[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) P[a][b] = yv(cy[a], cx[b]) - my;
-    // pair (first, second) of patch columns for fine offsets -1, 0, +1:
-    //   even i: (0,1) (1,1) (1,2)      odd i: (0,0) (0,1) (1,1)
-    const bool ox = s.i & 1, oy = s.j & 1;
-    T H[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const T p0 = P[a][0], p1 = P[a][1], p2 = P[a][2];
-        H[a][0] = T(0.5) * p0 + T(0.5) * (ox ? p0 : p1);
-        H[a][1] = T(0.5) * (ox ? p0 : p1) + T(0.5) * p1;
-        H[a][2] = T(0.5) * p1 + T(0.5) * (ox ? p1 : p2);
-    }
-#pragma unroll
-    for (int b = 0; b < 3; ++b) {
-        const T h0 = H[0][b], h1 = H[1][b], h2 = H[2][b];
-        Y[0][b] = T(0.5) * h0 + T(0.5) * (oy ? h0 : h1);
-        Y[1][b] = T(0.5) * (oy ? h0 : h1) + T(0.5) * h1;
-        Y[2][b] = T(0.5) * h1 + T(0.5) * (oy ? h1 : h2);
-    }
-}
-
-template <class T>
-__device__ __forceinline__ void interp3x3(const Grid2D& gc, const T* yc, T my, const NodeNbr& s, T (&Y)[3][3]) {
-    interp3x3_v(gc, [&](int cy, int cx) { return yc[cy * gc.nnx + cx]; }, my, s, Y);
-}
-
 // Pinned-form up-leg value at node n: x + I y_coarse on the stencil (0 at missing and pinned
 // nodes), then one damped Jacobi, 0 at the pins -- Jacobi's pinned path with its operands and
 // operation order.  yv reads the coarse y (see interp3x3_v).
@@ -567,9 +515,13 @@ struct JacobiSZ {
     }
 };
 
-// CG tail handed to the V-cycle (see JacobiSZ); first: the pass before the first iteration
+// CG tail handed to the V-cycle (see JacobiSZ); first: the pass before the first iteration.
+// With cg_p set (tiled V-cycle, see cg_fusable) the iteration's x / r update also rides on the
+// first down leg: r_new = r0 - alpha Ap goes to cg_rout, x += alpha p, and the finalize of the
+// z-step takes r_new . r_new as well.
 struct CgTail {
     double* znew; const double* zold; double* pzero; int flex; int first;
+    const double* cg_p = nullptr; const double* cg_Ap = nullptr; double* cg_x = nullptr; double* cg_rout = nullptr;
 };
 
 // ------------------------------------------------------------------------------ coarse level
@@ -860,11 +812,170 @@ __global__ __launch_bounds__(256) void k_coarse_dense(int nn, const float* __res
     }
 }
 
+// tile shapes of the tiled V-cycle kernels (level nodes per block)
+#ifndef MGT_DOWN_CX
+#define MGT_DOWN_CX 32
+#endif
+#ifndef MGT_DOWN_CY
+#define MGT_DOWN_CY 4
+#endif
+#ifndef MGT_UP_C
+#define MGT_UP_C 64
+#endif
+#ifndef MGT_UP_R
+#define MGT_UP_R 8
+#endif
+
+template <class T>
+static LdTab<T> ld_tab(const Level& L) {
+    LdTab<T> t;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) t.v[a][b] = (T)L.ldc[a][b];
+    return t;
+}
+
+// The tiled V-cycle can carry the PCG's x / r update (first down leg) and z-step (last up leg)
+// when both fine-level legs run as single-level kernels.
+static bool cg_fusable(const Ctx& c) {
+    if (!(c.precond == FCFD_PC_MULTIGRID && c.set_zeroP && !c.g.periodic && c.nlev >= 2 && c.mg_tiled && c.c_r2))
+        return false;
+    const int ld = c.nlev - 1;
+    const bool down_single = !(1 < ld && c.lv[1].g.nny < 200);
+    const bool up_single = c.lv[0].g.nn >= (1 << 17);  // never paired with level 1 (vcycle_tiled's up leg)
+    return down_single && up_single;
+}
+
+// The pinned V-cycle on a non-periodic grid with the tiled kernels (fcfd_mgtile.hpp): two small
+// levels per kernel on each leg below 200 node rows (down) / 2^17 nodes (up), one level per
+// kernel above.  With `tail` the level-0 up leg (when it runs alone) also runs the PCG's z-step;
+// the return value says so.
+template <class T>
+static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTail* tail) {
+    if (tail && tail->cg_p && !cg_fusable(c)) throw Error(FCFD_ERR_STATE, "CG update not fusable here");
+    using B = LvBufs<T>;
+    const int N = c.nlev, ld = N - 1;
+    bool fused = false;
+    int down_nb = 0;  // blocks of the first down leg when it carries the CG's x / r update
+    for (int l = 0; l < ld;) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        if (l + 1 < ld && Lc.g.nny < 200) {
+            Level& L2 = c.lv[l + 2];
+            const dim3 grid((L2.g.nnx + M2_C - 1) / M2_C, (L2.g.nny + M2_R - 1) / M2_R);
+            if (l == 0)
+                k_mgt_down2<T, double><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le,
+                                                                   ld_tab<T>(L), ld_tab<T>(Lc), r0, L.pin, Lc.pin,
+                                                                   B::r(Lc), B::r(L2), active);
+            else
+                k_mgt_down2<T, T><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le,
+                                                              ld_tab<T>(L), ld_tab<T>(Lc), B::r(L), L.pin, Lc.pin,
+                                                              B::r(Lc), B::r(L2), active);
+            l += 2;
+        } else {
+            auto go = [&](auto rl) {
+                using TR = std::remove_const_t<std::remove_pointer_t<decltype(rl)>>;
+                constexpr int CX = MGT_DOWN_CX, CY = MGT_DOWN_CY;
+                const int nbx = (Lc.g.nnx + CX - 1) / CX;
+                k_mgt_down<T, TR, CX, CY><<<dim3(nbx, (Lc.g.nny + CY - 1) / CY), 256, 0, c.stream>>>(
+                    L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), rl, L.pin, B::r(Lc), active);
+            };
+            if (l == 0 && tail && tail->cg_p) {
+                constexpr int CX = MGT_DOWN_CX, CY = MGT_DOWN_CY;
+                const dim3 grid((Lc.g.nnx + CX - 1) / CX, (Lc.g.nny + CY - 1) / CY);
+                down_nb = (int)(grid.x * grid.y);
+                if (down_nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+                k_mgt_down_cg<T, CX, CY><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), r0,
+                                                                   tail->cg_Ap, tail->cg_p, tail->cg_x,
+                                                                   tail->cg_rout, L.pin, B::r(Lc), c.st, c.partials,
+                                                                   c.pstride, 2, active);
+                r0 = tail->cg_rout;
+            } else if (l == 0) {
+                go(r0);
+            } else {
+                go((const T*)B::r(L));
+            }
+            l += 1;
+        }
+        FCFD_CHECK_LAUNCH();
+    }
+    {
+        Level& C = c.lv[N - 1];
+        const bool f32 = std::is_same<T, float>::value;
+        const double ctol = f32 ? (c.tol > 1e-5 ? c.tol : 1e-5) : c.tol;
+        if (f32 && c.coarse_inv32) {
+            const int nb = (C.g.nn + 4 * DENSE_ROWS - 1) / (4 * DENSE_ROWS);
+            k_coarse_dense<T><<<nb, 256, 0, c.stream>>>(C.g.nn, c.coarse_inv32, C.pin, B::r(C), B::y(C), active);
+            FCFD_CHECK_LAUNCH();
+        } else {
+            coarse_solve<T, T>(c, B::r(C), B::y(C), B::cr(c), B::cp(c), B::cAp(c), ctol, active);
+        }
+    }
+    for (int l = ld - 1; l >= 0;) {
+        Level& L = c.lv[l];
+        Level& Lc = c.lv[l + 1];
+        if (l >= 1 && c.lv[l - 1].g.nn < (1 << 17)) {
+            Level& Lf = c.lv[l - 1];
+            const dim3 grid((Lf.g.nnx + U2_C - 1) / U2_C, (Lf.g.nny + U2_R - 1) / U2_R);
+            if (l - 1 == 0)
+                k_mgt_up2<T, double><<<grid, 256, 0, c.stream>>>(Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff,
+                                                                 ld_tab<T>(Lf), ld_tab<T>(L), r0, Lf.pin, B::r(L),
+                                                                 L.pin, B::y(Lc), B::y(Lf), active);
+            else
+                k_mgt_up2<T, T><<<grid, 256, 0, c.stream>>>(Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff,
+                                                            ld_tab<T>(Lf), ld_tab<T>(L), B::r(Lf), Lf.pin, B::r(L),
+                                                            L.pin, B::y(Lc), B::y(Lf), active);
+            FCFD_CHECK_LAUNCH();
+            l -= 2;
+            continue;
+        }
+        // one level: tiles of MGT_UP_C x MGT_UP_R nodes
+        auto go = [&](auto rl, auto zm, double* znew, const double* zold, double* pzero, int flex) {
+            using TR = std::remove_const_t<std::remove_pointer_t<decltype(rl)>>;
+            constexpr int FC = MGT_UP_C, FR = MGT_UP_R, ZM = decltype(zm)::value;
+            const dim3 grid((L.g.nnx + FC - 1) / FC, (L.g.nny + FR - 1) / FR);
+            const int nb = (int)(grid.x * grid.y);
+            if (ZM != 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+            k_mgt_up<T, TR, FC, FR, ZM><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Leoff, ld_tab<T>(L), rl,
+                                                                    L.pin, B::y(Lc), B::y(L), znew, zold, pzero,
+                                                                    flex, c.partials, c.pstride, active);
+            if constexpr (ZM == 1)
+                finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, CgGamma0Fin{c.st}, c.partials,
+                                                                  c.pstride);
+            if constexpr (ZM == 2) {
+                if (down_nb > 0)
+                    finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, down_nb, active, CgK3RFin{c.st, flex},
+                                                                          c.partials, c.pstride);
+                else
+                    finalize_kernel<2><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, CgK3Fin{c.st, flex}, c.partials,
+                                                                      c.pstride);
+            }
+        };
+        using Z0 = std::integral_constant<int, 0>;
+        using Z1 = std::integral_constant<int, 1>;
+        using Z2 = std::integral_constant<int, 2>;
+        if (l == 0 && tail && tail->first) {
+            go(r0, Z1{}, tail->znew, nullptr, tail->pzero, 0);
+            fused = true;
+        } else if (l == 0 && tail) {
+            go(r0, Z2{}, tail->znew, tail->zold, nullptr, tail->flex);
+            fused = true;
+        } else if (l == 0) {
+            go(r0, Z0{}, nullptr, nullptr, nullptr, 0);
+        } else {
+            go((const T*)B::r(L), Z0{}, nullptr, nullptr, nullptr, 0);
+        }
+        FCFD_CHECK_LAUNCH();
+        l -= 1;
+    }
+    return fused;
+}
+
 // ------------------------------------------------------------------------------ V-cycle driver
 // With `tail` (pinned form) the level-0 up leg, when staged, also runs the PCG's z-step and the
 // return value says so (nothing is then stored into the level-0 y).
 template <class T>
 static bool vcycle_t(Ctx& c, const double* r0, const int* active, const CgTail* tail = nullptr) {
+    if (c.set_zeroP && !c.g.periodic && c.nlev >= 2 && c.mg_tiled) return vcycle_tiled<T>(c, r0, active, tail);
     bool fused = false;
     using B = LvBufs<T>;
     const int N = c.nlev;
@@ -1160,6 +1271,11 @@ struct ApproxInv2 {
     }
 };
 
+static bool vcycle_t_mixed_or_64(Ctx& c, const double* r, const int* active, const CgTail* tail) {
+    if (c.precision == FCFD_PREC_MIXED) return vcycle_t<float>(c, r, active, tail);
+    return vcycle_t<double>(c, r, active, tail);
+}
+
 // applies the preconditioner to c.c_r; returns pointer + type of y0.  With `tail`, a V-cycle
 // may also run the CG's z-step (*fused = true; y0 is then not written).
 static const void* precondition(Ctx& c, const int* active, bool& is_f32, const CgTail* tail = nullptr,
@@ -1227,8 +1343,26 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
     FCFD_CHECK_LAUNCH();
     // one chunk = a pair of PCG iterations (parity of the p / z double buffers returns to 0)
     // one PCG iteration on the p / z double buffers of parity o
+    const bool fuse_cg = cg_fusable(c);
     auto one = [&](int o) {
         const int nw = o ^ 1;
+        if (fuse_cg) {
+            // r alternates between c_r (parity 0) and c_r2; CgK1S, then the V-cycle with the x / r
+            // update on its first down leg and the z-step on its last up leg
+            double* rin = o == 0 ? c.c_r : c.c_r2;
+            double* rout = o == 0 ? c.c_r2 : c.c_r;
+            launch_stencil<1>(c, nn, act, CgK1S{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                              CgK1Fin{c.st});
+            CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
+            tail.cg_p = c.c_p[nw];
+            tail.cg_Ap = c.c_Ap;
+            tail.cg_x = c.c_x;
+            tail.cg_rout = rout;
+            const bool f = vcycle_t_mixed_or_64(c, rin, act, &tail);
+            if (!f) throw Error(FCFD_ERR_STATE, "fused PCG iteration without its z-step");
+            FCFD_CHECK_LAUNCH();
+            return;
+        }
         if (g.periodic)
             launch_node<1>(c, nn, act, CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
                            CgK1Fin{c.st});

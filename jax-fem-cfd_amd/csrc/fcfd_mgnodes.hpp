@@ -92,4 +92,57 @@ __global__ __launch_bounds__(1024) void k_coarse_jacobi(int nn, const T* Ld, con
     if (threadIdx.x == 0) st->mg[l][MG_Y] = acc[0] / nn;
 }
 
+// interp_at at all 9 stencil points of a node from one 3x3 patch of coarse values: fine index f
+// interpolates coarse (f >> 1, (f + 1) >> 1), so fine f - 1 .. f + 1 touch coarse
+// floor((f - 1) / 2) + {0, 1, 2}.  Which patch entries pair up depends only on the parity of the
+// node's (i, j): selected per lane, with the same operands and operation order as interp_at, so
+// every value is bitwise the one interp_at returns.  9 coarse loads instead of 36.
+// yv(cy, cx): the coarse value at coarse row cy, column cx (global memory, or a block's LDS copy)
+template <class T, class YV>
+__device__ __forceinline__ void interp3x3_v(const Grid2D& gc, YV yv, T my, const NodeNbr& s, T (&Y)[3][3]) {
+    const int bx = (s.i - 1) >> 1, by = (s.j - 1) >> 1;  // arithmetic shift: floor
+    int cx[3], cy[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        int x = bx + t, y = by + t;
+        if (gc.periodic) {
+            x = x < 0 ? x + gc.nnx : (x >= gc.nnx ? x - gc.nnx : x);
+            y = y < 0 ? y + gc.nny : (y >= gc.nny ? y - gc.nny : y);
+        } else {
+            x = x < 0 ? 0 : (x >= gc.nnx ? gc.nnx - 1 : x);
+            y = y < 0 ? 0 : (y >= gc.nny ? gc.nny - 1 : y);
+        }
+        cx[t] = x;
+        cy[t] = y;
+    }
+    T P[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) P[a][b] = yv(cy[a], cx[b]) - my;
+    // pair (first, second) of patch columns for fine offsets -1, 0, +1:
+    //   even i: (0,1) (1,1) (1,2)      odd i: (0,0) (0,1) (1,1)
+    const bool ox = s.i & 1, oy = s.j & 1;
+    T H[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T p0 = P[a][0], p1 = P[a][1], p2 = P[a][2];
+        H[a][0] = T(0.5) * p0 + T(0.5) * (ox ? p0 : p1);
+        H[a][1] = T(0.5) * (ox ? p0 : p1) + T(0.5) * p1;
+        H[a][2] = T(0.5) * p1 + T(0.5) * (ox ? p1 : p2);
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const T h0 = H[0][b], h1 = H[1][b], h2 = H[2][b];
+        Y[0][b] = T(0.5) * h0 + T(0.5) * (oy ? h0 : h1);
+        Y[1][b] = T(0.5) * (oy ? h0 : h1) + T(0.5) * h1;
+        Y[2][b] = T(0.5) * h1 + T(0.5) * (oy ? h1 : h2);
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void interp3x3(const Grid2D& gc, const T* yc, T my, const NodeNbr& s, T (&Y)[3][3]) {
+    interp3x3_v(gc, [&](int cy, int cx) { return yc[cy * gc.nnx + cx]; }, my, s, Y);
+}
+
 }  // namespace fcfd

@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfcfd.so")
+# FCFD_LIB: another build of the same library (e.g. a variant built elsewhere)
+LIB_PATH = os.environ.get("FCFD_LIB") or os.path.join(_HERE, "libfcfd.so")
 
 FCFD_OK = 0
 PC = {None: 0, "none": 0, "jacobi": 1, "approx_inv": 2, "multigrid": 3}
