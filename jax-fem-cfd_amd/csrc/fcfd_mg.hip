@@ -1351,8 +1351,15 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
             // update on its first down leg and the z-step on its last up leg
             double* rin = o == 0 ? c.c_r : c.c_r2;
             double* rout = o == 0 ? c.c_r2 : c.c_r;
-            launch_stencil<1>(c, nn, act, CgK1S{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                              CgK1Fin{c.st});
+            {
+                constexpr int FC = 64, FR = 8;
+                const dim3 grid((g.nnx + FC - 1) / FC, (g.nny + FR - 1) / FR);
+                const int nb = (int)(grid.x * grid.y);
+                if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+                k_cg_k1t<FC, FR><<<grid, 256, 0, c.stream>>>(g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st,
+                                                            c.partials, c.pstride, act);
+                finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, CgK1Fin{c.st}, c.partials, c.pstride);
+            }
             CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
             tail.cg_p = c.c_p[nw];
             tail.cg_Ap = c.c_Ap;

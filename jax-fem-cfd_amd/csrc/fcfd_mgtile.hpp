@@ -48,6 +48,10 @@ __device__ __forceinline__ T ld_at(const T* lds, const Grid2D& g, int i, int j) 
 template <int H, int W, class V>
 struct Win {
     static constexpr int N = H * W, NIT = (N + BLOCK - 1) / BLOCK;
+    // rows of an LDS array that takes every store of store() (NIT * BLOCK slots, the ones past
+    // the window masked): an unconditional store keeps the compiler from sinking the loads of
+    // the last, partly filled pass into a branch after the ones it could issue first
+    static constexpr int HP = (NIT * BLOCK + W - 1) / W;
     V v[NIT];
     bool in[NIT];
     template <class LD>
@@ -67,10 +71,8 @@ struct Win {
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int t = threadIdx.x + it * BLOCK;
-            if (t < N) {
-                const int a = t / W, b = t - a * W;
-                put(a, b, in[it], v[it]);
-            }
+            const int a = t / W, b = t - a * W;
+            put(a, b, in[it], v[it]);
         }
     }
 };
@@ -171,9 +173,10 @@ __global__ __launch_bounds__(256) void k_mgt_down(Grid2D gf, Grid2D gc, const do
     constexpr int XW = 2 * CX + 3, XR = 2 * CY + 3;
     constexpr int FW = 2 * CX + 1, FR = 2 * CY + 1;
     __shared__ T ldl[9];
-    __shared__ T xs[XR][XW];
-    __shared__ T rs[XR][XW];
-    __shared__ uint8_t ps[XR][XW];
+    constexpr int XP = Win<XR, XW, RPin<TR>>::HP;
+    __shared__ T xs[XP][XW];
+    __shared__ T rs[XP][XW];
+    __shared__ uint8_t ps[XP][XW];
     __shared__ T fs[FR][FW];
     const int nbx = gridDim.x;
     const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
@@ -234,20 +237,21 @@ __global__ __launch_bounds__(256) void k_mgt_down_cg(Grid2D gf, Grid2D gc, const
     constexpr int FW = 2 * CX + 1, FR = 2 * CY + 1;
     constexpr int OW = 2 * CX, OR = 2 * CY, NO = (OW * OR + BLOCK - 1) / BLOCK;  // owned fine nodes
     __shared__ T ldl[9];
-    __shared__ T xs[XR][XW];
-    __shared__ T rs[XR][XW];
+    struct RAP {
+        double r, ap;
+        uint8_t p;
+    };
+    constexpr int XP = Win<XR, XW, RAP>::HP;
+    __shared__ T xs[XP][XW];
+    __shared__ T rs[XP][XW];
     __shared__ double rd[OR][OW];
-    __shared__ uint8_t ps[XR][XW];
+    __shared__ uint8_t ps[XP][XW];
     __shared__ T fs[FR][FW];
     const int nbx = gridDim.x;
     const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
     const int J0 = (lb / nbx) * CY, I0 = (lb - (lb / nbx) * nbx) * CX;
     const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;
     const double alpha = st->c_alpha;
-    struct RAP {
-        double r, ap;
-        uint8_t p;
-    };
     Win<XR, XW, RAP> w;
     w.load(gf, fi0, fj0, [&](int k) { return RAP{r[k], Ap[k], pin[k]}; });
     double po[NO], xo[NO];
@@ -329,13 +333,14 @@ __global__ __launch_bounds__(256) void k_mgt_down2(Grid2D gf, Grid2D gm, Grid2D 
     constexpr int FRl = 2 * XRm + 1, FWl = 2 * XWm + 1;    // l-window of f_l
     constexpr int XRl = FRl + 2, XWl = FWl + 2;            // l-window of x_l
     __shared__ T ldfl[9], ldml[9];
-    __shared__ T xl[XRl][XWl];
-    __shared__ T rl[XRl][XWl];
-    __shared__ uint8_t pl[XRl][XWl];
+    constexpr int XPl = Win<XRl, XWl, RPin<TR>>::HP, XPm = Win<XRm, XWm, uint8_t>::HP;
+    __shared__ T xl[XPl][XWl];
+    __shared__ T rl[XPl][XWl];
+    __shared__ uint8_t pl[XPl][XWl];
     __shared__ T fl[FRl][FWl];
     __shared__ T xms[XRm][XWm];
     __shared__ T rms[XRm][XWm];
-    __shared__ uint8_t pms[XRm][XWm];
+    __shared__ uint8_t pms[XPm][XWm];
     __shared__ T fm[FRm][FWm];
     const int nbx = gridDim.x;
     const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
@@ -437,10 +442,11 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
     constexpr int CW = FC / 2 + 3, CR = FR / 2 + 3;  // coarse window (prolongation sources)
     constexpr int NPT = FC * FR / BLOCK;
     __shared__ T ldl[9];
-    __shared__ T rs[WR][WW];
-    __shared__ uint8_t ps[WR][WW];  // 1 pinned, 2 outside the grid
+    constexpr int WP = Win<WR, WW, RPin<TR>>::HP, CP = Win<CR, CW, T>::HP;
+    __shared__ T rs[WP][WW];
+    __shared__ uint8_t ps[WP][WW];  // 1 pinned, 2 outside the grid
     __shared__ T vs[WR][WW];
-    __shared__ T ys[CR][CW];
+    __shared__ T ys[CP][CW];
     const int nbx = gridDim.x;
     const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
     const int j0 = (lb / nbx) * FR, i0 = (lb - (lb / nbx) * nbx) * FC;
@@ -529,13 +535,14 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
     constexpr int MXR = MR + 2, MXC = MC + 2;              // their stencil window
     constexpr int CRW = MXR / 2 + 3, CCW = MXC / 2 + 3;    // coarse window of the m-window's prolongation
     __shared__ T ldfl[9], ldml[9];
-    __shared__ T rl[LR][LC];
-    __shared__ uint8_t pl[LR][LC];  // 1 pinned, 2 outside the grid
+    constexpr int LP = Win<LR, LC, RPin<TR>>::HP, MP = Win<MXR, MXC, RPin<T>>::HP, CP = Win<CRW, CCW, T>::HP;
+    __shared__ T rl[LP][LC];
+    __shared__ uint8_t pl[LP][LC];  // 1 pinned, 2 outside the grid
     __shared__ T vl[LR][LC];
-    __shared__ T rms[MXR][MXC];
-    __shared__ uint8_t pms[MXR][MXC];
+    __shared__ T rms[MP][MXC];
+    __shared__ uint8_t pms[MP][MXC];
     __shared__ T vm[MXR][MXC];
-    __shared__ T ycs[CRW][CCW];
+    __shared__ T ycs[CP][CCW];
     __shared__ T ym[MR][MC];
     const int nbx = gridDim.x;
     const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
@@ -594,6 +601,59 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
         yf[j * gf.nnx + i] = pinned_jacobi_at(gf, i, j, Leoff_f, rl[bj + 1][bi + 1], ld_at(ldfl, gf, i, j),
                                               pl[bj + 1][bi + 1] != 0,
                                               [&](int p, int q) { return vl[bj + p][bi + q]; });
+}
+
+// ------------------------------------------------------------------------------ PCG K1, tiled
+// p = z + beta p_old (CgK1S's staged value), Ap = L p (pinned rows 0), block partial of p . Ap
+// (jax_iterative_solvers.py:105-107), for a pinned non-periodic grid: a block owns FC x FR
+// nodes, loads z, p_old and the pins on the window one wider in one batch.
+template <int FC, int FR>
+__global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restrict__ Le, const uint8_t* __restrict__ pin,
+                                                const double* __restrict__ z, const double* __restrict__ pold,
+                                                double* __restrict__ pnew, double* __restrict__ Ap,
+                                                const DevState* __restrict__ st, double* __restrict__ partials,
+                                                int pstride, const int* __restrict__ active) {
+    static_assert((FC * FR) % BLOCK == 0, "whole nodes per thread");
+    if (active && *active == 0) return;
+    constexpr int WW = FC + 2, WR = FR + 2, NPT = FC * FR / BLOCK;
+    struct ZP {
+        double z, p;
+        uint8_t pin;
+    };
+    constexpr int WP = Win<WR, WW, ZP>::HP;
+    __shared__ double ps[WP][WW];
+    __shared__ uint8_t pns[WP][WW];
+    const int nbx = gridDim.x;
+    const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    const int j0 = (lb / nbx) * FR, i0 = (lb - (lb / nbx) * nbx) * FC;
+    const double beta = st->c_beta;
+    Win<WR, WW, ZP> w;
+    w.load(g, i0 - 1, j0 - 1, [&](int k) { return ZP{z[k], pold[k], pin[k]}; });
+    w.store([&](int a, int b, bool in, const ZP& v) {
+        ps[a][b] = in ? v.z + beta * v.p : 0.0;
+        pns[a][b] = v.pin;
+    });
+    __syncthreads();
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int m = 0; m < NPT; ++m) {
+        const int t = threadIdx.x + m * BLOCK;
+        const int bi = t % FC, bj = t / FC;
+        const int i = i0 + bi, j = j0 + bj;
+        if (i < g.nnx && j < g.nny) {
+            const int n = j * g.nnx + i;
+            double X[3][3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) X[a][b] = ps[bj + a][bi + b];
+            const double av = pns[bj + 1][bi + 1] ? 0.0 : ebe_has(has_at(g, i, j), Le, X);
+            pnew[n] = X[1][1];
+            Ap[n] = av;
+            acc[0] += X[1][1] * av;
+        }
+    }
+    reduce_epilogue<1>(acc, partials, pstride, lb);
 }
 
 }  // namespace fcfd
