@@ -557,6 +557,12 @@ struct ZeroIfFlag {
     }
 };
 
+}  // namespace fcfd
+
+#include "fcfd_momtile.hpp"
+
+namespace fcfd {
+
 // ------------------------------------------------------------------------------ driver
 // BiCGSTAB iterations after an init kernel has set x, r, rhat, p, q and the state: chunks of
 // iteration pairs (graph-replayed) until the device flag says done.  Returns the reference's
@@ -572,6 +578,25 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
     // one iteration on the double buffers of parity o
     auto one = [&](int o) {
         const int nw = o ^ 1;
+        if constexpr (NC == 2 && !MT && IsMomCoef<LOp>::value) {
+            if (!c.g.periodic && mask) {
+                // the tiled momentum kernels (fcfd_momtile.hpp)
+                const dim3 grid((c.g.nnx + MOMT_C - 1) / MOMT_C, (c.g.nny + MOMT_R - 1) / MOMT_R);
+                const int nb = (int)(grid.x * grid.y);
+                if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+                k_mom_k1t<decltype(lhs.ce), MOMT_C, MOMT_R><<<grid, BLOCK, 0, c.stream>>>(
+                    lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st, c.partials,
+                    c.pstride, act);
+                finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, BK1Fin{c.st}, c.partials, c.pstride);
+                k_mom_k2t<decltype(lhs.ce), MOMT_C, MOMT_R><<<grid, BLOCK, 0, c.stream>>>(
+                    lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st, c.partials, c.pstride, act);
+                finalize_kernel<3><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, BK2Fin{c.st}, c.partials, c.pstride);
+                launch_node<2>(c, nn, act, BK3R<NC>{nn, c.m_p[nw], c.m_q[nw], c.m_t, c.m_rhat, x, c.m_r, c.st},
+                               BK3Fin{c.st});
+                FCFD_CHECK_LAUNCH();
+                return;
+            }
+        }
         if (c.g.periodic) {
             launch_node<1>(c, nn, act,
                            BK1<NC, LOp, MT>{lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw],

@@ -435,7 +435,15 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     c.c_x = dalloc<double>(c, nn);
     c.c_rhs = dalloc<double>(c, nn);
     c.ustar = dalloc<double>(c, 2 * nn);
-    c.pstride = blocks_for(2 * nn) + 1;
+    {
+        // the node kernels reduce over up to 2 nn items; the tiled kernels over their tiles
+        int ps = blocks_for(2 * nn);
+        ps = std::max(ps, tiles_for(g.nnx, g.nny, MOMT_C, MOMT_R));
+        ps = std::max(ps, tiles_for(g.nnx, g.nny, CGT_C, CGT_R));
+        ps = std::max(ps, tiles_for(g.nnx, g.nny, MGT_UP_C, MGT_UP_R));
+        if (c.nlev >= 2) ps = std::max(ps, tiles_for(c.lv[1].g.nnx, c.lv[1].g.nny, MGT_DOWN_CX, MGT_DOWN_CY));
+        c.pstride = ps + 1;
+    }
     c.partials = dalloc<double>(c, (size_t)MAX_RED * c.pstride);
     c.st = dalloc<DevState>(c, 1);
     FCFD_HIP(hipMemset(c.st, 0, sizeof(DevState)));

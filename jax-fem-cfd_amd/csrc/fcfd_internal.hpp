@@ -13,6 +13,11 @@
 namespace fcfd {
 
 constexpr int MAX_LEVELS = 24;
+// tiles (nodes per block) of the tiled kernels on non-periodic quad4 grids: the V-cycle's down
+// legs (coarse nodes) and up legs, the PCG's K1, the momentum BiCGSTAB's K1 / K2
+constexpr int MGT_DOWN_CX = 32, MGT_DOWN_CY = 4, MGT_UP_C = 64, MGT_UP_R = 8;
+constexpr int CGT_C = 64, CGT_R = 8, MOMT_C = 64, MOMT_R = 4;
+inline int tiles_for(int nx, int ny, int tx, int ty) { return ((nx + tx - 1) / tx) * ((ny + ty - 1) / ty); }
 constexpr int MAX_RED = 4;  // reductions per kernel
 
 // Scalars that solver kernels read and the finalize steps of reducing kernels
@@ -131,6 +136,8 @@ struct Ctx {
     // reductions
     double* partials = nullptr;
     int pstride = 0;            // partials per reduction slot (max blocks of a reducing kernel)
+    double* dot_partials = nullptr;  // the generic dot's own partials (any length; never in a graph)
+    int dot_pstride = 0;
     DevState* st = nullptr;
     int* h_pinned = nullptr;   // pinned host mirror for flags / counts
     // iteration predictions for chunked launching

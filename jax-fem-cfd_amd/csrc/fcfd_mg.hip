@@ -812,20 +812,6 @@ __global__ __launch_bounds__(256) void k_coarse_dense(int nn, const float* __res
     }
 }
 
-// tile shapes of the tiled V-cycle kernels (level nodes per block)
-#ifndef MGT_DOWN_CX
-#define MGT_DOWN_CX 32
-#endif
-#ifndef MGT_DOWN_CY
-#define MGT_DOWN_CY 4
-#endif
-#ifndef MGT_UP_C
-#define MGT_UP_C 64
-#endif
-#ifndef MGT_UP_R
-#define MGT_UP_R 8
-#endif
-
 template <class T>
 static LdTab<T> ld_tab(const Level& L) {
     LdTab<T> t;
@@ -1352,7 +1338,7 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
             double* rin = o == 0 ? c.c_r : c.c_r2;
             double* rout = o == 0 ? c.c_r2 : c.c_r;
             {
-                constexpr int FC = 64, FR = 8;
+                constexpr int FC = CGT_C, FR = CGT_R;
                 const dim3 grid((g.nnx + FC - 1) / FC, (g.nny + FR - 1) / FR);
                 const int nb = (int)(grid.x * grid.y);
                 if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");

@@ -187,7 +187,8 @@ int settle_momentum(Ctx& c, int fmt, void* store, float* scale, double rho, doub
 
 // ------------------------------------------------------------------------------ benchmark
 // The momentum operator apply of BiCGSTAB (K2: s = r - alpha q formed on the fly, t = A s stored,
-// 3 dot partials), run back to back `reps` times under its own kernel symbol (TAG = 1) and timed
+// 3 dot partials; the tiled kernel on non-periodic grids with Dirichlet nodes, as in the solve),
+// run back to back `reps` times under its own kernel symbol (TAG = 1) and timed
 // with HIP events on the context stream.  The node kernel alone: its 1-block finalize is not part
 // of the timing.
 template <class S>
@@ -199,11 +200,15 @@ static float bench_t(Ctx& c, S ce, int reps) {
     hipEvent_t e0, e1;
     FCFD_HIP(hipEventCreate(&e0));
     FCFD_HIP(hipEventCreate(&e1));
+    const dim3 gt((c.g.nnx + MOMT_C - 1) / MOMT_C, (c.g.nny + MOMT_R - 1) / MOMT_R);
     auto launch = [&]() {
         if (c.g.periodic)
             node_kernel<3><<<nb, BLOCK, 0, c.stream>>>(
                 c.g.nn, nullptr, BK2<2, MomCoef<S>, 1>{lhs, c.dmask, c.m_r, c.m_q[0], c.m_t, c.st},
                 c.partials, c.pstride);
+        else if (c.dmask)
+            k_mom_k2t<S, MOMT_C, MOMT_R, 1><<<gt, BLOCK, 0, c.stream>>>(lhs, c.dmask, c.m_r, c.m_q[0], c.m_t, c.st,
+                                                                       c.partials, c.pstride, nullptr);
         else
             stencil_kernel<3><<<nbs, BLOCK, 0, c.stream>>>(
                 c.g.nn, nullptr, K{lhs, c.dmask, c.m_r, c.m_q[0], c.m_t, c.st}, c.partials, c.pstride);

@@ -311,17 +311,20 @@ void read_state_ints(Ctx& c, const int* dev, int n, int* host) {
     for (int i = 0; i < n; ++i) host[i] = c.h_pinned[i];
 }
 
+// The generic dot product keeps its own partials buffer, grown to the vector length on demand:
+// the solvers' captured graphs hold c.partials, which therefore never moves.
 double dot(Ctx& c, const double* x, const double* y, int n) {
     const int nb = blocks_for(n);
-    if (nb > c.pstride) {  // generic-vector use on a small grid context: grow the partials buffer
+    if (nb > c.dot_pstride) {
         FCFD_HIP(hipStreamSynchronize(c.stream));
         for (auto& p : c.allocs)
-            if (p == (void*)c.partials) p = nullptr;
-        FCFD_HIP(hipFree(c.partials));
-        c.pstride = nb + 1;
-        c.partials = dalloc<double>(c, (size_t)MAX_RED * c.pstride);
+            if (p == (void*)c.dot_partials) p = nullptr;
+        if (c.dot_partials) FCFD_HIP(hipFree(c.dot_partials));
+        c.dot_pstride = nb + 1;
+        c.dot_partials = dalloc<double>(c, (size_t)c.dot_pstride);
     }
-    launch_node<1>(c, n, nullptr, DotBody{x, y}, DotFin{c.st});
+    node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(n, nullptr, DotBody{x, y}, c.dot_partials, c.dot_pstride);
+    finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, nullptr, DotFin{c.st}, c.dot_partials, c.dot_pstride);
     FCFD_CHECK_LAUNCH();
     double out;
     FCFD_HIP(hipMemcpyAsync(c.h_pinned, &c.st->dot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
