@@ -137,7 +137,19 @@ static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, 
     const Grid2D& g = c.g;
     const MomOps<S> o = mom_ops(c, ce, rho, mu, dt);
     int it;
-    if (!g.periodic) {
+    if (!g.periodic && c.dmask) {
+        const dim3 grid((g.nnx + MOMT_C - 1) / MOMT_C, (g.nny + MOMT_R - 1) / MOMT_R);
+        const int nb = (int)(grid.x * grid.y);
+        if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+        k_mom_initt<S, MOMT_C, MOMT_R><<<grid, BLOCK, 0, c.stream>>>(o.lhs, o.rhs, c.ec_dev, P_n, F, U_n, c.ud_full,
+                                                                    c.dmask, c.m_x, c.m_r, c.m_rhat, c.partials,
+                                                                    c.pstride);
+        finalize_kernel<2><<<1, FIN_BLOCK, 0, c.stream>>>(nb, nullptr, BInitFin{c.st, c.tol, o.maxit}, c.partials,
+                                                          c.pstride);
+        FCFD_CHECK_LAUNCH();
+        it = bicgstab_iterate<2, false>(c, c.g_mom, o.key, ce.c, o.lhs, c.dmask, o.maxit, c.m_x, c.pred_mom,
+                                        deferred);
+    } else if (!g.periodic) {
         launch_stencil<2>(c, g.nn, nullptr,
                           BInitMomS<S>{o.lhs, o.rhs, c.ec_dev, P_n, F, U_n, c.ud_full, c.dmask, c.m_x, c.m_r,
                                        c.m_rhat},

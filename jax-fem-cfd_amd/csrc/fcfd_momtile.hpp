@@ -21,15 +21,37 @@ struct IsMomCoef<MomCoef<S>> : std::true_type {};
 
 // The four rows a node owns of its incident elements (op.row of MomCoef: cM Me + cL Le + cC Ce_e),
 // loaded before any of them is used; missing elements read element 0 and are masked by has.
+__device__ __forceinline__ void inc_elems(const Grid2D& g, int i, int j, int (&e)[4]) {
+    const Has4 hs = has_at(g, i, j);
+    e[0] = hs.h[0] ? (j - 1) * g.nex + (i - 1) : 0;
+    e[1] = hs.h[1] ? (j - 1) * g.nex + i : 0;
+    e[2] = hs.h[2] ? j * g.nex + (i - 1) : 0;
+    e[3] = hs.h[3] ? j * g.nex + i : 0;
+}
 template <class S>
 struct MomRows {
     double v[4][4];
     __device__ __forceinline__ void load(const S& ce, const Grid2D& g, int i, int j) {
-        const Has4 hs = has_at(g, i, j);
-        const int e[4] = {hs.h[0] ? (j - 1) * g.nex + (i - 1) : 0, hs.h[1] ? (j - 1) * g.nex + i : 0,
-                          hs.h[2] ? j * g.nex + (i - 1) : 0, hs.h[3] ? j * g.nex + i : 0};
+        int e[4];
+        inc_elems(g, i, j, e);
 #pragma unroll
         for (int k = 0; k < 4; ++k) ce.ld_row(kRow(k), e[k], v[k]);
+    }
+    __device__ __forceinline__ double at(int k, int b) const { return v[k][b]; }
+};
+// FP32 storage: the rows stay FP32 in registers until they are used (half the registers across
+// the barrier); the value used is the same (double)(float) conversion CeF32::ld_row makes
+template <>
+struct MomRows<CeF32> {
+    float4 v[4];
+    __device__ __forceinline__ void load(const CeF32& ce, const Grid2D& g, int i, int j) {
+        int e[4];
+        inc_elems(g, i, j, e);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = *reinterpret_cast<const float4*>(ce.c + ce_idx(kRow(k) * 4, e[k], ce.ne));
+    }
+    __device__ __forceinline__ double at(int k, int b) const {
+        return (double)(b == 0 ? v[k].x : (b == 1 ? v[k].y : (b == 2 ? v[k].z : v[k].w)));
     }
 };
 
@@ -44,7 +66,7 @@ __device__ __forceinline__ void mom_apply(const MomCoef<S>& op, const Has4& hs, 
         const int a = kRow(k);
         double cf[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) cf[b] = op.cM * op.ec->Me[a * 4 + b] + op.cL * op.ec->Le[a * 4 + b] + op.cC * rw.v[k][b];
+        for (int b = 0; b < 4; ++b) cf[b] = op.cM * op.ec->Me[a * 4 + b] + op.cL * op.ec->Le[a * 4 + b] + op.cC * rw.at(k, b);
         double t[2] = {0.0, 0.0};
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
@@ -194,6 +216,126 @@ __global__ __launch_bounds__(256) void k_mom_k2t(MomCoef<S> op, const uint8_t* _
         }
     }
     reduce_epilogue<3>(acc, partials, pstride, lb);
+}
+
+// ------------------------------------------------------------------------------ init
+// The momentum solve's init (BInitMomS, standard.py:24-39): rhs b = (rho/dt) M_c U_n
+// - (1 - theta)(rho C + mu L) U_n - G P_n + F with the Dirichlet rows set to U_d, x0 = U_n with
+// the Dirichlet values, r0 = rhat = b - A x0; partials b.b, r0.r0.  The window holds U_n, x0
+// (both components) and P_n; each element row feeds both the lhs and the rhs coefficients.
+template <class S, int FC, int FR>
+__global__ __launch_bounds__(256) void k_mom_initt(MomCoef<S> lhs, MomCoef<S> rhs, const ElemConst* __restrict__ ec,
+                                                   const double* __restrict__ Pn, const double* __restrict__ F,
+                                                   const double* __restrict__ xn, const double* __restrict__ ud,
+                                                   const uint8_t* __restrict__ mask, double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ rhat,
+                                                   double* __restrict__ partials, int pstride) {
+    static_assert(FC * FR == BLOCK, "one node per thread");
+    constexpr int WW = FC + 2, WR = FR + 2;
+    struct V7 {
+        double x0, x1, u0, u1, p;
+        uint8_t m0, m1;
+    };
+    constexpr int WP = Win<WR, WW, V7>::HP;
+    __shared__ double xs[2][WP][WW];   // U_n
+    __shared__ double x0s[2][WP][WW];  // x0
+    __shared__ double pps[WP][WW];     // P_n
+    __shared__ double uds[2][WP][WW];  // U_d
+    __shared__ uint8_t ms[2][WP][WW];  // Dirichlet flags
+    const Grid2D& g = lhs.g;
+    const int nn = g.nn;
+    const int nbx = gridDim.x;
+    const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    const int j0 = (lb / nbx) * FR, i0 = (lb - (lb / nbx) * nbx) * FC;
+    const int bi = threadIdx.x % FC, bj = threadIdx.x / FC;
+    const int i = i0 + bi, j = j0 + bj;
+    const bool own = i < g.nnx && j < g.nny;
+    const int n = own ? j * g.nnx + i : 0;
+    Win<WR, WW, V7> w;
+    w.load(g, i0 - 1, j0 - 1, [&](int k) {
+        return V7{xn[k], xn[k + nn], ud[k], ud[k + nn], Pn[k], mask[k], mask[k + nn]};
+    });
+    MomRows<S> rw;
+    rw.load(lhs.ce, g, own ? i : 0, own ? j : 0);
+    double Fo[2] = {0.0, 0.0};
+    if (F) {
+        Fo[0] = F[n];
+        Fo[1] = F[n + nn];
+    }
+    w.store([&](int a, int b, bool in, const V7& v) {
+        xs[0][a][b] = in ? v.x0 : 0.0;
+        xs[1][a][b] = in ? v.x1 : 0.0;
+        x0s[0][a][b] = in ? (v.m0 ? v.u0 : v.x0) : 0.0;
+        x0s[1][a][b] = in ? (v.m1 ? v.u1 : v.x1) : 0.0;
+        pps[a][b] = in ? v.p : 0.0;
+        uds[0][a][b] = v.u0;
+        uds[1][a][b] = v.u1;
+        ms[0][a][b] = v.m0;
+        ms[1][a][b] = v.m1;
+    });
+    __syncthreads();
+    double acc[2] = {0.0, 0.0};
+    if (own) {
+        const Has4 hs = has_at(g, i, j);
+        double b[2] = {0.0, 0.0}, ax[2] = {0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int a = kRow(k);
+            double cl[4], cr[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                cl[q] = lhs.cM * lhs.ec->Me[a * 4 + q] + lhs.cL * lhs.ec->Le[a * 4 + q] + lhs.cC * rw.at(k, q);
+                cr[q] = rhs.cM * rhs.ec->Me[a * 4 + q] + rhs.cL * rhs.ec->Le[a * 4 + q] + rhs.cC * rw.at(k, q);
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                double tl = 0.0, tr = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int dy = kCy(k, q), dx = kCx(k, q);
+                    tl += cl[q] * x0s[c][bj + dy][bi + dx];
+                    tr += cr[q] * xs[c][bj + dy][bi + dx];
+                }
+                ax[c] += hs.h[k] ? tl : 0.0;
+                b[c] += hs.h[k] ? tr : 0.0;
+            }
+        }
+        double gx = 0.0, gy = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int a = kRow(k);
+            double tx = 0.0, ty = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double p = pps[bj + kCy(k, q)][bi + kCx(k, q)];
+                tx += ec->Gx[a * 4 + q] * p;
+                ty += ec->Gy[a * 4 + q] * p;
+            }
+            gx += hs.h[k] ? tx : 0.0;
+            gy += hs.h[k] ? ty : 0.0;
+        }
+        b[0] -= gx;
+        b[1] -= gy;
+        if (F) {
+            b[0] += Fo[0];
+            b[1] += Fo[1];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int ii = n + c * nn;
+            const double x0 = x0s[c][bj + 1][bi + 1];
+            const bool m = ms[c][bj + 1][bi + 1] != 0;
+            const double bc = m ? uds[c][bj + 1][bi + 1] : b[c];
+            const double av = m ? x0 : ax[c];
+            const double rc = bc - av;
+            x[ii] = x0;
+            r[ii] = rc;
+            rhat[ii] = rc;
+            acc[0] += bc * bc;
+            acc[1] += rc * rc;
+        }
+    }
+    reduce_epilogue<2>(acc, partials, pstride, lb);
 }
 
 }  // namespace fcfd
