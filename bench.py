@@ -47,6 +47,10 @@ def _cavity(precision, ce_format, n=N_EL):
 
 
 def run_form(precision, ce_format, steps, warmup):
+    """`warmup` untimed steps through the native step loop (its buffers and graphs are then in
+    place), then `steps` timed steps through the same loop.  The same `steps` steps are then run
+    again from the same start state with one Python `timestep` call each (the reference's
+    timestep_loop shape) and timed beside it."""
     import torch
     compute, timestep, U_d, U0, P0, nn = _cavity(precision, ce_format)
     F_p, F = compute(U_d, None, 0, 0)
@@ -54,24 +58,23 @@ def run_form(precision, ce_format, steps, warmup):
     P = torch.as_tensor(P0, device="cuda")
     Ud = torch.as_tensor(U_d, device="cuda")
     ctx = timestep.context
-    for _ in range(warmup):
-        U, P, ui, pi = timestep(U, P, DT, RHO, MU, Ud, F_p, F)
+    if warmup > 0:
+        U, P, _ = ctx.four_step_rollout(U, P, warmup, DT, RHO, MU, Ud, F_p, F)
+    Uw, Pw = U.clone(), P.clone()
     torch.cuda.synchronize()
-    # the timed steps: one native loop over the step (compute's F_p, F are fixed for the cavity)
     t0 = time.perf_counter()
     U, P, its = ctx.four_step_rollout(U, P, steps, DT, RHO, MU, Ud, F_p, F)
     torch.cuda.synchronize()
     dt_s = time.perf_counter() - t0
-    # the same steps one Python call each (the reference's timestep_loop shape), timed beside it
-    n_py = min(steps, 100)
-    U2, P2 = U, P
+    U2, P2 = Uw, Pw
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for _ in range(n_py):
+    for _ in range(steps):
         U2, P2, ui, pi = timestep(U2, P2, DT, RHO, MU, Ud, F_p, F)
     torch.cuda.synchronize()
-    py_s = (time.perf_counter() - t1) / n_py
-    return dict(sec=dt_s, its=[tuple(map(int, r)) for r in its], U=U, P=P, ctx=ctx, nn=nn, py_step_s=py_s)
+    py_s = (time.perf_counter() - t1) / steps
+    return dict(sec=dt_s, its=[tuple(map(int, r)) for r in its], U=U, P=P, Uw=Uw, Pw=Pw, ctx=ctx, nn=nn,
+                py_step_s=py_s)
 
 
 def poisson_alone(precision, solves=3):
@@ -179,9 +182,18 @@ def gradient_256(steps=200, checkpoint=20):
             "params": 2 * nn}
 
 
-def cpu_baseline(steps=2):
-    """Oracle (NumPy FP64 restatement) on this host, one thread, same workload,
-    bounded sample of `steps` timesteps after set-up."""
+def cpu_baseline(U_start, P_start, first_step, steps=3, threads=None):
+    """The oracle (NumPy FP64 restatement) on this host on a bounded sample of the timed window:
+    it starts from the FP64 GPU state after step `first_step - 1` (the oracle and the FP64 GPU
+    form agree to 1e-10 there), runs step `first_step` untimed (warm-up), then times the next
+    `steps` steps.  OMP / OpenBLAS / MKL threads = `threads` (default: the cores this process
+    may run on); NumPy's gathers and scatters run on one thread whatever that setting."""
+    import tempfile
+    cores = len(os.sched_getaffinity(0))
+    threads = cores if threads is None else threads
+    fd, path = tempfile.mkstemp(suffix=".npz")
+    os.close(fd)
+    np.savez(path, U=U_start, P=P_start)
     code = f"""
 import sys, time, json
 sys.path.insert(0, {ROOT!r})
@@ -189,8 +201,9 @@ import numpy as np
 from oracle import ns
 s, surf, h, U_d = ns.build([{N_EL}, {N_EL}], [1.0, 1.0], False, [0, 1, 2, 3], [], {TOL}, True, "multigrid", [16, 16])
 U_d = ns.cavity_lid(s, surf, U_d)
-U = np.random.default_rng(1).normal(0, 1e-3, 2 * s.nn); U[s.gamma_d] = U_d
-P = np.zeros(s.nn)
+z = np.load({path!r})
+U, P = z["U"], z["P"]
+U, P, ui, pi = ns.four_step_timestep(s, U, P, {DT}, {RHO}, {MU}, U_d, 0, 0)
 t0 = time.perf_counter()
 its = []
 for _ in range({steps}):
@@ -199,12 +212,17 @@ for _ in range({steps}):
 sec = time.perf_counter() - t0
 print(json.dumps(dict(sec=sec, steps={steps}, its=its)))
 """
-    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
-    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
+    t = str(threads)
+    env = dict(os.environ, OMP_NUM_THREADS=t, OPENBLAS_NUM_THREADS=t, MKL_NUM_THREADS=t)
+    try:
+        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
+    finally:
+        os.unlink(path)
     if out.returncode != 0:
         return dict(error=out.stderr[-400:])
     r = json.loads(out.stdout.strip().splitlines()[-1])
-    r["value"] = r["steps"] / r["sec"]
+    r.update(value=r["steps"] / r["sec"], cores=cores, threads=threads,
+             window=[first_step + 1, first_step + steps])
     return r
 
 
@@ -377,7 +395,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU-oracle steps (after one untimed)")
     ap.add_argument("--no-grad", action="store_true", help="skip the 256^2 adjoint-gradient measurement")
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D channel measurement")
     ap.add_argument("--roofline-only", action="store_true",
@@ -397,6 +415,7 @@ def main():
         return
     f64 = run_form("fp64", "f64", args.steps, args.warmup)
     mix = run_form("mixed", "f32", args.steps, args.warmup)
+    cpu_start = (f64["Uw"].cpu().numpy(), f64["Pw"].cpu().numpy())
     nn = mix["nn"]
     U64, P64, Um, Pm = (f64["U"].cpu().numpy(), f64["P"].cpu().numpy(), mix["U"].cpu().numpy(),
                         mix["P"].cpu().numpy())
@@ -420,7 +439,10 @@ def main():
     grad = None if args.no_grad else gradient_256()
     ch3d = None if args.no_3d else channel_3d()
     ch3d_q2 = None if args.no_3d else channel_3d_q2()
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps)
+    cpu = cpu1 = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(*cpu_start, args.warmup + 1, args.cpu_steps)
+        cpu1 = cpu_baseline(*cpu_start, args.warmup + 1, max(1, args.cpu_steps - 1), threads=1)
 
     steps_s_m = args.steps / mix["sec"]
     steps_s_d = args.steps / f64["sec"]
@@ -456,15 +478,20 @@ def main():
         "channel_3d_q2": ch3d_q2,
         "roofline": {"bound": "hbm", "achieved": ach_m, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach_m / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stencil_kernel<3, BK2S<2, MomCoef<CeF32>, 1, false>> (BiCGSTAB s/t: LDS-staged momentum operator apply + 3 dots)",
+                     "kernel": "k_mom_k2t<CeF32, 64, 4, 1> (BiCGSTAB s/t: tiled momentum operator apply + 3 dots)",
                      "bytes_per_launch": bytes_m, "us_per_launch": ms_m * 1e3,
                      "device_copy_gbs": copy_gbs, "frac_of_device_copy": ach_m / copy_gbs,
                      "fp64_ce": {"achieved": ach_d, "bytes_per_launch": bytes_d, "us_per_launch": ms_d * 1e3}},
         "cpu_baseline": None if cpu is None else (
             {"error": cpu["error"]} if "error" in cpu else
-            {"value": cpu["value"], "unit": "timesteps/s", "cores": 1, "kind": "port",
-             "sample": f"{cpu['steps']} timesteps of the same 1024^2 cavity (FP64 oracle, NumPy, 1 thread), "
-                       f"iters {cpu['its']}"}),
+            {"value": cpu["value"], "unit": "timesteps/s", "cores": cpu["threads"], "kind": "port",
+             "cores_available": cpu["cores"],
+             "sample": f"steps {cpu['window'][0]}-{cpu['window'][1]} of the same 1024^2 cavity (inside the timed "
+                       f"window of `value`), FP64 oracle (NumPy), started from the FP64 GPU state after step "
+                       f"{args.warmup} and one untimed oracle step; OMP/OpenBLAS/MKL threads = {cpu['threads']} "
+                       f"(NumPy's gathers and scatters are single-threaded); iters {cpu['its']}",
+             "one_thread": None if cpu1 is None or "error" in cpu1 else
+             {"value": cpu1["value"], "steps": cpu1["steps"], "iters": cpu1["its"]}}),
     }
     print(json.dumps(line))
 

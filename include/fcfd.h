@@ -175,6 +175,24 @@ int fcfd_elem_mult_stored(fcfd_ctx* ctx, const double* K0_16, const double* A1, 
  * quad4, quad9 and (new: the reference has none, simulation.py:75-76) hex8 box contexts */
 int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z);
 
+/* interpolation_operator (multigrid.py:90-140): bilinear prolongation of xH on multigrid level
+ * level + 1 of the context to xh on level `level` (periodic grids wrap).  2D quad4 contexts. */
+int fcfd_interpolate(fcfd_ctx* ctx, int32_t level, const double* xH, double* xh);
+
+/* restriction_operator (multigrid.py:142-212): R = I^T without the 1/4 factor, xh on level
+ * `level` to xH on level + 1; periodic grids restrict the padded grid and drop its wrap row and
+ * column, as the reference does.  2D quad4 contexts. */
+int fcfd_restrict(fcfd_ctx* ctx, int32_t level, const double* xh, double* xH);
+
+/* single_jacobi_iter (multigrid.py:214-228): one damped Jacobi sweep (omega = 2/3) on level
+ * `level`: x_out = omega (rhs - Le_off x) / L_diag + (1 - omega) x.  Le_off16: host double[16];
+ * L_diag: device (nn of the level); gamma_p: device int64[n_gamma_p].  set_zeroP: x zeroed at
+ * gamma_p before the sweep and x_out after it; otherwise the mean of x_out is subtracted.
+ * x_out must not alias x.  2D quad4 contexts. */
+int fcfd_jacobi_sweep(fcfd_ctx* ctx, int32_t level, const double* Le_off16, const double* x, const double* rhs,
+                      const double* L_diag, const int64_t* gamma_p, int64_t n_gamma_p, int32_t set_zeroP,
+                      double* x_out);
+
 /* PCG on the (pinned or Neumann) pressure Laplacian with the context's
  * preconditioner; x holds x0 on entry.  Mirrors the cg call inside
  * solve_pressure (standard.py:114-117) and _cg_solve
@@ -199,7 +217,9 @@ int fcfd_solve_momentum(fcfd_ctx* ctx, const double* U_n, const double* P_n, con
 /* four_step_timestep (standard.py:148-165): Ce(U_n), momentum, U*, pressure,
  * projection, Dirichlet values.  Uses the context's precision policy.
  * U_d: device (n_gamma_d,) or NULL; F_p, F: device or NULL (zero).
- * iters: host int32[2] = (U_iters, P_iters), written after one stream sync. */
+ * iters: host int32[2] = (U_iters, P_iters), written after one stream sync.
+ * U may alias U_n and P may alias P_n (an in-place step): P is written only after both solves
+ * have finished, U only by the final projection. */
 int fcfd_four_step_timestep(fcfd_ctx* ctx, const double* U_n, const double* P_n, double dt,
                             double rho, double mu, const double* U_d, const double* F_p,
                             const double* F, double* U, double* P, int32_t* iters);

@@ -475,16 +475,19 @@ void four_step(Ctx& c, const double* U_n, const double* P_n, double dt, double r
     // The momentum solve's predicted chunk is enqueued without waiting for its iteration count:
     // the pressure solve's own synchronisation also delivers it.  If that chunk had not
     // converged, the momentum solve is completed and U* and the pressure solve are redone.
+    // P is written only by the output pass after the momentum solve has settled, and U only by
+    // the final projection, so either may alias its input (P == P_n, U == U_n).
     int launched = 0;
     solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x, &launched);
     projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
-    it[1] = solve_pressure(c, c.ustar, P_n, F_p, rho, dt, P);
+    it[1] = pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt);
     bool resumed = false;
     it[0] = settle_momentum(c, c.ce_format, c.ce, c.ce_scale, rho, mu, dt, launched, &resumed);
     if (resumed) {
         projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
-        it[1] = solve_pressure(c, c.ustar, P_n, F_p, rho, dt, P);
+        it[1] = pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt);
     }
+    pressure_output(c, P);
     projection_final(c, c.ustar, P, rho, dt, U);
 }
 
@@ -686,6 +689,25 @@ int fcfd_vcycle(fcfd_ctx* ctx, const double* r, double* z) {
     if (ctx->c.dim == 3 && ctx->c.order == 2) return run(ctx, [&] { vcycle_h27_api(ctx->c, r, z); });
     if (ctx->c.dim == 3) return run(ctx, [&] { vcycle3_api(ctx->c, r, z); });
     return run2d(ctx, "fcfd_vcycle", [&] { vcycle(ctx->c, r, z); });
+}
+
+int fcfd_interpolate(fcfd_ctx* ctx, int32_t level, const double* xH, double* xh) {
+    if (!ctx || !xH || !xh) return FCFD_ERR_ARG;
+    return run2d(ctx, "fcfd_interpolate", [&] { interpolate_level(ctx->c, level, xH, xh); });
+}
+
+int fcfd_restrict(fcfd_ctx* ctx, int32_t level, const double* xh, double* xH) {
+    if (!ctx || !xh || !xH) return FCFD_ERR_ARG;
+    return run2d(ctx, "fcfd_restrict", [&] { restrict_level(ctx->c, level, xh, xH); });
+}
+
+int fcfd_jacobi_sweep(fcfd_ctx* ctx, int32_t level, const double* Le_off16, const double* x, const double* rhs,
+                      const double* L_diag, const int64_t* gamma_p, int64_t n_gamma_p, int32_t set_zeroP,
+                      double* x_out) {
+    if (!ctx || !Le_off16 || !x || !rhs || !L_diag || !x_out) return FCFD_ERR_ARG;
+    return run2d(ctx, "fcfd_jacobi_sweep", [&] {
+        jacobi_sweep(ctx->c, level, Le_off16, x, rhs, L_diag, gamma_p, n_gamma_p, set_zeroP, x_out);
+    });
 }
 
 int fcfd_pressure_cg(fcfd_ctx* ctx, const double* b, double* x, double tol, int32_t maxiter, int32_t* iters,

@@ -152,6 +152,7 @@ struct Ctx {
     // scalar part of the U_n cotangent, the U_hat cotangent and the momentum adjoint solution (2nn)
     double *a_sx = nullptr, *a_gut = nullptr, *a_guns = nullptr, *a_gus = nullptr, *a_mx = nullptr;
     double *roll_U = nullptr, *roll_P = nullptr;  // fcfd_four_step_rollout's second state
+    uint8_t* sweep_mask = nullptr;  // fcfd_jacobi_sweep's pin mask (lazy)
     double* a_gel = nullptr;  // SUPG element cotangents, 16 per element (quad4), 36 (quad9)
     std::vector<void*> allocs;
 };
@@ -336,6 +337,10 @@ void supg_api(Ctx& c, const double* U_n, const double* Ce_api, double D, double 
 
 // pressure / multigrid (fcfd_mg.hip)
 void vcycle(Ctx& c, const double* r, double* z);
+void interpolate_level(Ctx& c, int level, const double* xH, double* xh);
+void restrict_level(Ctx& c, int level, const double* xh, double* xH);
+void jacobi_sweep(Ctx& c, int level, const double* Le_off16, const double* x, const double* rhs, const double* Ld,
+                  const int64_t* gamma_p, int64_t n_gamma_p, int set_zeroP, double* y);
 int pressure_cg(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged);
 // The pressure right-hand side c (D U* - F_p), 0 at the pins (standard.py:99-104), for a PCG
 // whose init kernel forms it per node instead of reading a stored b
@@ -349,6 +354,9 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
         bool* converged, bool zero_exit = false, const PrhsSpec* prhs = nullptr);
 int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho,
                    double dt, double* P_out);
+// solve_pressure in two parts: the solve (solution in c.c_x) and the output pass into P_out
+int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt);
+void pressure_output(Ctx& c, double* P_out);
 void projection_ustar(Ctx& c, const double* U_hat, const double* P_n, double rho, double dt, double* U_star);
 void projection_final(Ctx& c, const double* U_star, const double* P, double rho, double dt, double* U);
 

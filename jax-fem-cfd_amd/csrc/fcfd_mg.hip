@@ -1420,8 +1420,9 @@ struct PrhsBody {
 };
 
 
-int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
-                   double* P_out) {
+// The pressure solve proper: the solution stays in c.c_x (pinned form: 0 at the pins; Neumann:
+// its mean is taken by pressure_output).  Reads U_star and P_n (x0) only.
+int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt) {
     const int nn = c.g.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
@@ -1436,12 +1437,104 @@ int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double
                        MeanFin{&c.st->mean_rhs, nullptr, nn});
         it = pcg(c, c.c_rhs, &c.st->mean_rhs, P_n, 0, c.tol, maxit, &conv);
     }
-    if (!c.set_zeroP)
-        launch_node<1>(c, nn, nullptr, SumXBody{c.c_x},
-                                                   MeanFin{&c.st->mean_p, nullptr, nn});
+    return it;
+}
+
+// P = the solution with the pins zeroed, or minus its mean (standard.py:119-123)
+void pressure_output(Ctx& c, double* P_out) {
+    const int nn = c.g.nn;
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    if (!c.set_zeroP) launch_node<1>(c, nn, nullptr, SumXBody{c.c_x}, MeanFin{&c.st->mean_p, nullptr, nn});
     launch_node<0>(c, nn, nullptr, PoutBody{c.c_x, pin, P_out, c.st, !c.set_zeroP}, NoFin{});
     FCFD_CHECK_LAUNCH();
+}
+
+int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                   double* P_out) {
+    const int it = pressure_solve_only(c, U_star, P_n, F_p, rho, dt);
+    pressure_output(c, P_out);
     return it;
+}
+
+// ------------------------------------------------------------------------------ grid operators
+// The reference's multigrid building blocks as standalone operators on the levels of a context
+// (multigrid.py:90-228): prolongation, restriction and one damped Jacobi sweep.
+struct InterpBody {
+    Grid2D gf, gc; const double* yc; double* y;
+    __device__ void operator()(int n, double (&)[1]) const { y[n] = interp_at(gf, gc, yc, 0.0, n); }
+};
+
+void interpolate_level(Ctx& c, int level, const double* xH, double* xh) {
+    FCFD_REQUIRE(level >= 0 && level + 1 < c.nlev, "level out of range (the context needs level + 1 too)");
+    const Level& L = c.lv[level];
+    launch_node<0>(c, L.g.nn, nullptr, InterpBody{L.g, c.lv[level + 1].g, xH, xh}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
+void restrict_level(Ctx& c, int level, const double* xh, double* xH) {
+    FCFD_REQUIRE(level >= 0 && level + 1 < c.nlev, "level out of range (the context needs level + 1 too)");
+    const Level& L = c.lv[level];
+    const Level& Lc = c.lv[level + 1];
+    launch_node<0>(c, Lc.g.nn, nullptr, Restrict<double>{L.g, Lc.g, xh, xH, c.st, level, 0}, NoFin{});
+    FCFD_CHECK_LAUNCH();
+}
+
+// x_out = omega (rhs - Le_off x') / L_diag + (1 - omega) x', x' = x with the pins zeroed
+// (set_zeroP); then the pins zeroed, or (pure Neumann) the mean of x_out subtracted.
+struct ElemMat16 {
+    double v[16];
+};
+struct JacSweepBody {
+    Grid2D g; ElemMat16 Lo; const double* x; const double* rhs; const double* Ld; const uint8_t* pin; double* y;
+    __device__ void operator()(int n, double (&acc)[1]) const {
+        NodeNbr s;
+        node_nbr(g, n, s);
+        double X[3][3];
+        if (pin)
+            load3x3(s, [&](int k) { const double v = x[k]; return pin[k] ? 0.0 : v; }, X);
+        else
+            load3x3(s, [&](int k) { return x[k]; }, X);
+        const double off = ebe_const(s, Lo.v, X);
+        double v = (OMEGA * (rhs[n] - off)) / Ld[n] + (1.0 - OMEGA) * X[1][1];
+        if (pin && pin[n]) v = 0.0;
+        y[n] = v;
+        acc[0] += v;
+    }
+};
+struct SubMeanBody {
+    double* y; const double* mean;
+    __device__ void operator()(int n, double (&)[1]) const { y[n] = y[n] - *mean; }
+};
+__global__ void k_pin_mask(uint8_t* mask, int nn, const int64_t* __restrict__ idx, int64_t n_idx) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nn) mask[k] = 0;
+}
+__global__ void k_pin_set(uint8_t* mask, int nn, const int64_t* __restrict__ idx, int64_t n_idx) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_idx) return;
+    int64_t i = idx[k];
+    if (i < 0) i += nn;  // negative indices count from the end, out-of-range ones are dropped (as .at[].set)
+    if (i >= 0 && i < nn) mask[i] = 1;
+}
+
+void jacobi_sweep(Ctx& c, int level, const double* Le_off16, const double* x, const double* rhs, const double* Ld,
+                  const int64_t* gamma_p, int64_t n_gamma_p, int set_zeroP, double* y) {
+    FCFD_REQUIRE(level >= 0 && level < c.nlev, "level out of range");
+    FCFD_REQUIRE(x != y, "x_out must not alias x (the sweep reads x at the neighbours of every node)");
+    const Grid2D& g = c.lv[level].g;
+    const int nn = g.nn;
+    ElemMat16 Lo;
+    for (int k = 0; k < 16; ++k) Lo.v[k] = Le_off16[k];
+    const uint8_t* pin = nullptr;
+    if (set_zeroP && gamma_p && n_gamma_p > 0) {
+        if (!c.sweep_mask) c.sweep_mask = dalloc<uint8_t>(c, c.lv[0].g.nn);
+        k_pin_mask<<<(nn + 255) / 256, 256, 0, c.stream>>>(c.sweep_mask, nn, gamma_p, n_gamma_p);
+        k_pin_set<<<(int)((n_gamma_p + 255) / 256), 256, 0, c.stream>>>(c.sweep_mask, nn, gamma_p, n_gamma_p);
+        pin = c.sweep_mask;
+    }
+    launch_node<1>(c, nn, nullptr, JacSweepBody{g, Lo, x, rhs, Ld, pin, y}, MeanFin{&c.st->dot, nullptr, nn});
+    if (!set_zeroP) launch_node<0>(c, nn, nullptr, SubMeanBody{y, &c.st->dot}, NoFin{});
+    FCFD_CHECK_LAUNCH();
 }
 
 }  // namespace fcfd

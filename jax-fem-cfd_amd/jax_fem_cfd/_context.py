@@ -198,6 +198,37 @@ class Context:
         nat.check(self._lib.fcfd_vcycle(self._h, nat.ptr(r), nat.ptr(z)), "fcfd_vcycle")
         return z
 
+    def level_nn(self, level):
+        return self.level_shape(level)[0]
+
+    def interpolate(self, xH, level=0):
+        """Prolongation of xH on level + 1 to level (multigrid.py:90-140)."""
+        xH = nat.as_dev(xH)
+        xh = self._empty(self.level_nn(level))
+        self._sync_stream()
+        nat.check(self._lib.fcfd_interpolate(self._h, int(level), nat.ptr(xH), nat.ptr(xh)), "fcfd_interpolate")
+        return xh
+
+    def restrict(self, xh, level=0):
+        """R = I^T of xh on level to level + 1 (multigrid.py:142-212)."""
+        xh = nat.as_dev(xh)
+        xH = self._empty(self.level_nn(level + 1))
+        self._sync_stream()
+        nat.check(self._lib.fcfd_restrict(self._h, int(level), nat.ptr(xh), nat.ptr(xH)), "fcfd_restrict")
+        return xH
+
+    def jacobi_sweep(self, x, rhs, Le_off, L_diag, gamma_p, set_zeroP, level=0):
+        """One damped Jacobi sweep (multigrid.py:214-228) on a level."""
+        x, rhs, Ld = nat.as_dev(x), nat.as_dev(rhs), nat.as_dev(L_diag)
+        gp = None if gamma_p is None else nat.as_index(gamma_p)
+        lo = (ctypes.c_double * 16)(*np.asarray(nat.to_numpy(Le_off), dtype=np.float64).ravel())
+        out = self._empty(self.level_nn(level))
+        self._sync_stream()
+        nat.check(self._lib.fcfd_jacobi_sweep(self._h, int(level), lo, nat.ptr(x), nat.ptr(rhs), nat.ptr(Ld),
+                                              nat.ptr(gp), 0 if gp is None else int(gp.numel()),
+                                              int(bool(set_zeroP)), nat.ptr(out)), "fcfd_jacobi_sweep")
+        return out
+
     def pressure_cg(self, b, x0=None, tol=None, maxiter=1000):
         b = nat.as_dev(b)
         x = self._t.zeros(self.nn, dtype=self._t.float64, device=nat.device()) if x0 is None else nat.as_dev(x0).clone()

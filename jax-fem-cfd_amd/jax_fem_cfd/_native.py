@@ -70,6 +70,9 @@ _SIGS = {
     "fcfd_elem_mult_const": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_d), _vp, _vp, _i32]),
     "fcfd_elem_mult_stored": (ctypes.c_int, [_vp, ctypes.POINTER(_d), _vp, _d, _vp, _d, _i32, _vp, _vp]),
     "fcfd_vcycle": (ctypes.c_int, [_vp, _vp, _vp]),
+    "fcfd_interpolate": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
+    "fcfd_restrict": (ctypes.c_int, [_vp, _i32, _vp, _vp]),
+    "fcfd_jacobi_sweep": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(_d), _vp, _vp, _vp, _vp, _i64, _i32, _vp]),
     "fcfd_pressure_cg": (ctypes.c_int, [_vp, _vp, _vp, _d, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "fcfd_solve_pressure": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _vp, ctypes.POINTER(_i32)]),
     "fcfd_solve_momentum": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
@@ -157,6 +160,12 @@ def as_dev(x, dtype=None):
         y = x.to(device=device(), dtype=dtype)
         return y.contiguous()
     return t.as_tensor(np.asarray(x), dtype=dtype, device=device()).contiguous()
+
+
+def as_index(x):
+    """int64 contiguous CUDA tensor of node / dof indices."""
+    t = require_gpu()
+    return as_dev(np.asarray(to_numpy(x), dtype=np.int64), dtype=t.int64)
 
 
 def ptr(x):

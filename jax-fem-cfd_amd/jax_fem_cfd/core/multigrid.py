@@ -84,11 +84,45 @@ def v_cycle(rhs, lvl_nums, Le_lvls, nconn_lvls, L_diag_lvls, gamma_p_lvls, tol, 
     return ctx.vcycle(rhs)
 
 
+_transfer_cache = {}
+
+
+def _transfer_context(nnh_x, nnh_y, periodic):
+    """A two-level context whose level 0 is the fine node grid of the reference's transfer call
+    (nnh_x x nnh_y nodes; periodic grids count the padded wrap column and row, as the reference's
+    arguments do) and whose level 1 is its coarse grid.  Unit cells: the transfers do not depend
+    on the element size."""
+    nex, ney = int(nnh_x) - 1, int(nnh_y) - 1
+    if nex < 2 or ney < 2 or nex % 2 or ney % 2:
+        raise ValueError("the fine grid must have an even number of elements per direction")
+    key = (nex, ney, bool(periodic))
+    ctx = _transfer_cache.get(key)
+    if ctx is None:
+        ctx = Context(nex, ney, float(nex), float(ney), bool(periodic), precond="multigrid", mg_levels=2)
+        _transfer_cache[key] = ctx
+    return ctx
+
+
 def interpolation_operator(xH, nnH_x, nnH_y, nnh_x, nnh_y, periodic):
-    """multigrid.py:90-140 -- runs inside the fused V-cycle kernels only."""
-    raise NotImplementedError("grid transfers run inside the native V-cycle (fcfd_vcycle)")
+    """multigrid.py:90-140: bilinear interpolation of xH (coarse node grid nnH_x x nnH_y) to the
+    fine grid nnh_x x nnh_y with half the element size, on the GPU (fcfd_interpolate)."""
+    if (int(nnh_x) - 1) != 2 * (int(nnH_x) - 1) or (int(nnh_y) - 1) != 2 * (int(nnH_y) - 1):
+        raise ValueError("the fine grid must have twice the coarse grid's elements per direction")
+    return _transfer_context(nnh_x, nnh_y, periodic).interpolate(xH, 0)
 
 
 def restriction_operator(xh, nnh_x, nnh_y, nnH_x, nnH_y, periodic):
-    """multigrid.py:142-212 -- runs inside the fused V-cycle kernels only."""
-    raise NotImplementedError("grid transfers run inside the native V-cycle (fcfd_vcycle)")
+    """multigrid.py:142-212: R = I^T (no 1/4 full-weighting factor) from the fine grid
+    nnh_x x nnh_y to the coarse grid nnH_x x nnH_y, on the GPU (fcfd_restrict)."""
+    if (int(nnh_x) - 1) != 2 * (int(nnH_x) - 1) or (int(nnh_y) - 1) != 2 * (int(nnH_y) - 1):
+        raise ValueError("the fine grid must have twice the coarse grid's elements per direction")
+    return _transfer_context(nnh_x, nnh_y, periodic).restrict(xh, 0)
+
+
+def single_jacobi_iter(x, rhs, Le_off, L_diag, gamma_p, nconn, nn, set_zeroP):
+    """multigrid.py:214-228: one damped Jacobi sweep (omega = 2/3) with the given Le_off and
+    L_diag on the grid behind nconn; gamma_p entries zeroed before and after when set_zeroP,
+    the mean subtracted otherwise (fcfd_jacobi_sweep)."""
+    from .._context import grid_context
+    ctx = grid_context(nconn, int(nn))
+    return ctx.jacobi_sweep(x, rhs, Le_off, L_diag, gamma_p if set_zeroP else None, set_zeroP)

@@ -189,78 +189,92 @@ def set_face_velocity(U_d, nodes, gamma_d, values, nn, var):
     return out
 
 
+def _n_steps(t_final, dt):
+    """Steps to reach t_final: ceil(t_final / dt), with t_final / dt truncated first (the
+    reference's count at simulation.py:258-260)."""
+    n = int(t_final / dt)
+    return n + 1 if n * dt < t_final else n
+
+
+class _Record:
+    """What timestep_loop keeps: the initial and every saved state, the iteration counts and the
+    step index of every save."""
+
+    def __init__(self, U0, phi0):
+        self.U = [nat.to_numpy(U0)]
+        self.phi = None if phi0 is None else [nat.to_numpy(phi0)]
+        self.its = []
+        self.steps = []
+
+    def save(self, step, U, its, phi=None):
+        self.U.append(nat.to_numpy(U))
+        if self.phi is not None:
+            self.phi.append(nat.to_numpy(phi))
+        self.its.append(its)
+        self.steps.append(step)
+
+    def counts(self, k):
+        return np.array([c[k] for c in self.its])
+
+
 def timestep_loop(t_final, dt, save_interval, print_interval, compute_func, timestep_func, parameters,
                   dirichlet_vals, initial_states, S_func=None, phi_S_func=None, save2file=None, h5name=None):
-    """simulation.py:229-360: per step compute -> timestep, iteration counts and
-    solutions recorded on save steps, progress printed on steps <= 5, every
-    print_interval and the last step.  ``save2file`` needs h5py (not installed
-    in this image) and raises if requested."""
+    """The reference's host time loop (simulation.py:229-360) over the native step.
+
+    One untimed warm-up call of the step first (the reference's compile run; its result is
+    dropped), then ``_n_steps(t_final, dt)`` steps, each calling ``compute_func(U_d, S_func,
+    t_n, t_n + dt)`` and then the step.  The state, its iteration counts and the step index are
+    kept at every ``save_interval``-th step and at the last one (the initial state is the first
+    entry of the state list).  A progress line is printed for the first five steps, every
+    ``print_interval``-th step and the last one.
+
+    Returns (U, P, U_iters, P_iters, steps, U_list), or with a passive scalar (three initial
+    states) (U, P, phi, U_iters, P_iters, phi_iters, steps, U_list, phi_list).  ``save2file``
+    (HDF5 streaming) needs h5py, which this image does not have, and raises if requested."""
     if save2file:
         raise NotImplementedError("HDF5 trajectory streaming needs h5py, which is not available")
     t = nat.torch()
-    has_phi = len(initial_states) == 3
+    passive = len(initial_states) == 3
     U, P = nat.as_dev(initial_states[0]), nat.as_dev(initial_states[1])
-    phi = nat.as_dev(initial_states[2]) if has_phi else None
-    U_d = dirichlet_vals[0]
-    phi_d = dirichlet_vals[1] if has_phi else None
-    U_iter_list, P_iter_list, step_list = [], [], []
-    phi_iter_list, phi_list = [], []
-    F_p, F = compute_func(U_d, S_func, 0, 0)
-    compile_start = time.perf_counter()
-    if has_phi:
-        _ = timestep_func(U, P, phi, dt, *parameters, U_d, phi_d, F_p, F, phi_S_func)
-    else:
-        _ = timestep_func(U, P, dt, *parameters, U_d, F_p, F)
-    compile_time = time.perf_counter() - compile_start
-    print(f"Run to compile took {compile_time:.2f} s\n")
-    num_steps = int(t_final / dt)
-    if num_steps * dt < t_final:
-        num_steps += 1
-    print("num steps:", num_steps)
-    U_list = [nat.to_numpy(U)]
-    if has_phi:
-        phi_list = [nat.to_numpy(phi)]
-    total_start = time.perf_counter()
-    for step in range(1, num_steps + 1):
-        U_n, P_n = U, P
-        phi_n = phi
-        t_n = (step - 1) * dt
-        tt = step * dt
-        start = time.perf_counter()
-        F_p, F = compute_func(U_d, S_func, t_n, tt)
-        if has_phi:
-            U, P, phi, U_iters, P_iters, phi_iters = timestep_func(U_n, P_n, phi_n, dt, *parameters, U_d, phi_d,
-                                                                   F_p, F, phi_S_func)
-        else:
-            U, P, U_iters, P_iters = timestep_func(U_n, P_n, dt, *parameters, U_d, F_p, F)
-        end = time.perf_counter()
-        if step % save_interval == 0 or step == num_steps:
-            U_list.append(nat.to_numpy(U))
-            U_iter_list.append(U_iters)
-            P_iter_list.append(P_iters)
-            step_list.append(step)
-            if has_phi:
-                phi_list.append(nat.to_numpy(phi))
-                phi_iter_list.append(phi_iters)
-        if step <= 5 or step % print_interval == 0 or step == num_steps:
-            diff = float(t.linalg.norm(U - U_n) / t.linalg.norm(U))
-            if has_phi:
-                print(f"End step {step}, time {tt:.2f}, took {((end - start) * 1000):.2f} ms, {U_iters} U iters, "
-                      f"{P_iters} P iters, {phi_iters} phi iters, diff = {diff:.6f}")
-            else:
-                print(f"End step {step}, time {tt:.2f}, took {((end - start) * 1000):.2f} ms, {U_iters} U iters, "
-                      f"{P_iters} P iters, diff = {diff:.6f}")
-    total_time = time.perf_counter() - total_start
-    print(f"\nTotal timestepping time: {total_time:.2f} s")
-    print(f"Avg of {((total_time / num_steps) * 1000):.2f} ms per timestep")
-    print(f"Avg of {(((total_time + compile_time) / num_steps) * 1000):.2f} ms per timestep including compile time\n")
-    U_iter_list = np.array(U_iter_list)
-    P_iter_list = np.array(P_iter_list)
-    step_list = np.array(step_list)
-    U_list = np.stack(U_list, axis=0)
-    if has_phi:
-        return U, P, phi, U_iter_list, P_iter_list, np.array(phi_iter_list), step_list, U_list, np.stack(phi_list)
-    return U, P, U_iter_list, P_iter_list, step_list, U_list
+    phi = nat.as_dev(initial_states[2]) if passive else None
+    U_d, phi_d = dirichlet_vals[0], (dirichlet_vals[1] if passive else None)
+
+    def step_once(U_n, P_n, phi_n, F_p, F):
+        if passive:
+            return timestep_func(U_n, P_n, phi_n, dt, *parameters, U_d, phi_d, F_p, F, phi_S_func)
+        U1, P1, it_u, it_p = timestep_func(U_n, P_n, dt, *parameters, U_d, F_p, F)
+        return U1, P1, None, it_u, it_p, None
+
+    w0 = time.perf_counter()
+    step_once(U, P, phi, *compute_func(U_d, S_func, 0, 0))
+    t_warm = time.perf_counter() - w0
+    n = _n_steps(t_final, dt)
+    print(f"warm-up step (graph capture, first launches): {t_warm:.2f} s; running {n} steps")
+    rec = _Record(U, phi)
+    loop0 = time.perf_counter()
+    for k in range(1, n + 1):
+        s0 = time.perf_counter()
+        U_prev = U
+        F_p, F = compute_func(U_d, S_func, (k - 1) * dt, k * dt)
+        U, P, phi, it_u, it_p, it_phi = step_once(U, P, phi, F_p, F)
+        ms = (time.perf_counter() - s0) * 1e3
+        its = (it_u, it_p) if not passive else (it_u, it_p, it_phi)
+        if k % save_interval == 0 or k == n:
+            rec.save(k, U, its, phi)
+        if k <= 5 or k % print_interval == 0 or k == n:
+            change = float(t.linalg.norm(U - U_prev) / t.linalg.norm(U))
+            extra = f", scalar {it_phi}" if passive else ""
+            print(f"step {k}/{n} (t = {k * dt:.4g}): {ms:.2f} ms; iterations momentum {it_u}, pressure {it_p}"
+                  f"{extra}; |U - U_prev| / |U| = {change:.3e}")
+    total = time.perf_counter() - loop0
+    print(f"{n} steps in {total:.2f} s: {total / n * 1e3:.2f} ms per step "
+          f"({(total + t_warm) / n * 1e3:.2f} ms per step with the warm-up call)")
+    steps = np.array(rec.steps)
+    U_list = np.stack(rec.U, axis=0)
+    if passive:
+        return (U, P, phi, rec.counts(0), rec.counts(1), rec.counts(2), steps, U_list,
+                np.stack(rec.phi, axis=0))
+    return U, P, rec.counts(0), rec.counts(1), steps, U_list
 
 
 def timestep_loop_scan(dt, num_steps, compute_func, timestep_func, rho, mu, D, U_d, phi_d, U0, P0, phi0, S_func,

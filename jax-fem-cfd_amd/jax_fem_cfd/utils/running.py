@@ -17,31 +17,31 @@ def use_single_gpu(DEVICE_ID):
 def print_device_info():
     t = nat.torch()
     n = t.cuda.device_count()
-    print("Visible HIP devices:", [t.cuda.get_device_name(i) for i in range(n)])
+    print(f"{n} HIP device(s):", ", ".join(t.cuda.get_device_name(i) for i in range(n)))
 
 
 def save_sol(U, p, sol_path):
     U, p = nat.to_numpy(U), nat.to_numpy(p)
     sol = np.concatenate((np.array([U.shape[0]]), np.asarray(U), np.asarray(p)))
     np.save(sol_path, sol)
-    print("Saved solution to", sol_path)
+    print(f"solution (U: {U.shape[0]} values, p: {p.shape[0]}) written to {sol_path}")
 
 
 def load_sol(sol_path):
     sol = np.load(sol_path)
     U = sol[1:(1 + int(sol[0]))]
     p = sol[(1 + int(sol[0])):]
-    print("Loaded solution from", sol_path)
+    print(f"solution read from {sol_path}")
     return U, p
 
 
 def save_iter_hist(momentum_list, pressure_list, steps, path):
     hist = np.stack((momentum_list, pressure_list, steps), axis=0)
     np.save(path, hist)
-    print("Saved linear solver iteration counts to", path)
+    print(f"iteration history ({hist.shape[1]} saves) written to {path}")
 
 
 def load_iter_hist(path):
     hist = np.load(path)
-    print("Loaded iteration counts from", path)
+    print(f"iteration history read from {path}")
     return hist[0, :], hist[1, :], hist[2, :]

@@ -407,6 +407,30 @@ def test_step_whose_momentum_chunk_is_too_short():
 # (velocity, pressure): FP32 Ce within the solver tolerance; FP16 + scale (11-bit entries) and
 # BF16 + scale (8-bit entries) at the rounding level of their stored convection matrices, whose
 # pressure error grows by about 4e-6 (FP16) per step over the first steps from rest
+def test_in_place_step_whose_momentum_chunk_is_too_short():
+    """The same settled-and-redone step with U == U_n and P == P_n (one buffer pair updated in
+    place): the redo must build U* and the pressure solve's x0 from P_n, so P may be written
+    only after the momentum solve has settled.  Fields and counts equal the oracle's step."""
+    import torch
+    n = 64
+    compute, timestep, U_d, U0, P0, nn = _cavity(n, 1e-12)
+    s, surf, _, oUd = ns.build([n, n], [1.0, 1.0], False, [0, 1, 2, 3], [], 1e-12, True, "multigrid", [4, 4])
+    oUd = ns.cavity_lid(s, surf, oUd)
+    U0 = U0 + np.random.default_rng(3).normal(0, 0.1, U0.shape)
+    P0 = np.random.default_rng(4).normal(0, 0.1, P0.shape)
+    Uo, Po, uio, pio = ns.four_step_timestep(s, U0.copy(), P0.copy(), 5e-4, 1.0, 1e-3, oUd, 0, 0)
+    ctx = timestep.context
+    U = torch.tensor(U0, dtype=torch.float64, device="cuda")
+    P = torch.tensor(P0, dtype=torch.float64, device="cuda")
+    Ud = torch.tensor(U_d, dtype=torch.float64, device="cuda")
+    U2, P2, ui, pi = ctx.four_step_timestep(U, P, 5e-4, 1.0, 1e-3, Ud, out=(U, P))
+    torch.cuda.synchronize()
+    print(f"\n[case] in-place short momentum chunk: iters {ui},{pi} oracle {uio},{pio}", flush=True)
+    assert U2.data_ptr() == U.data_ptr() and P2.data_ptr() == P.data_ptr()
+    assert uio > 2 and (ui, pi) == (uio, pio)
+    assert _rel(_np(U), Uo) < 1e-10 and _rel(_np(P), Po) < 1e-10
+
+
 MIXED_TOL = {"f32": (1e-8, 1e-8), "f16s": (2e-5, 1e-4), "bf16s": (2e-4, 1e-3)}
 
 

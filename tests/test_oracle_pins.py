@@ -212,6 +212,12 @@ def test_bicgstab_breakdown_codes_and_maxiter():
     with np.errstate(all="ignore"):
         x, k = bicgstab(lambda v: A2 @ v, np.array([1.0, 0.0]), tol=1e-12, maxiter=50)
     assert k == -11
+    # rho breakdown in the second step: b = e1 gives alpha = 1, s = (0, -1, 0), t = (0, -1, -1),
+    # omega = 1/2, r1 = (0, -1/2, 1/2) with rhat.r1 = 0 exactly while r1.r1 = 1/2
+    A3 = np.array([[1.0, 0.0, 0.0], [1.0, 1.0, 0.0], [0.0, 1.0, 1.0]])
+    with np.errstate(all="ignore"):
+        x, k = bicgstab(lambda v: A3 @ v, np.array([1.0, 0.0, 0.0]), tol=1e-12, maxiter=50)
+    assert k == -10
 
 
 # ----------------------------------------------------------------------------- multigrid
