@@ -294,6 +294,18 @@ int fcfd_passive_timestep_vjp(fcfd_ctx* ctx, const double* U_n, const double* P_
 
 /* *out (host) = x . y, FP64, deterministic two-stage reduction */
 int fcfd_dot(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out);
+/* Largest n accepted by fcfd_dot_noalloc: its block partials (one per 256 elements) live in a
+ * fixed device array of the library, 2^16 doubles = 512 KiB per GPU. */
+#define FCFD_DOT_NOALLOC_MAX_N (1 << 24)
+/* *out (a DEVICE pointer) = x . y with the kernels and summation order of fcfd_dot (bitwise the
+ * same value), enqueued on the context's stream; returns without synchronising and allocates
+ * nothing: the block partials live in that fixed array, whose address on each GPU is looked up
+ * once when the first context on that GPU is created.  n > FCFD_DOT_NOALLOC_MAX_N is rejected
+ * with FCFD_ERR_ARG.  Every call on one GPU shares the array: calls that overlap in time on one
+ * GPU (other streams, threads or contexts) race.  No counterpart in the reference: this entry
+ * point is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+int fcfd_dot_noalloc(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out);
 /* y = a x + b y */
 int fcfd_axpby(fcfd_ctx* ctx, double a, const double* x, double b, double* y, int64_t n);
 

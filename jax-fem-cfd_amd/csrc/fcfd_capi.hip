@@ -3,6 +3,8 @@
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
+#include <mutex>
+#include <string>
 #include "fcfd_internal.hpp"
 
 using namespace fcfd;
@@ -28,7 +30,34 @@ void graph_reset(IterGraph& g) {
 
 struct fcfd_ctx {
     Ctx c;
+    double* scratch = nullptr;  // address of `scratch` on this context's GPU
 };
+
+// Block partials of fcfd_dot_noalloc.  Each GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes the array's name to a kernel (in host code it is only a
+// stand-in): it uses the address that hipGetSymbolAddress returns on the current device.
+static __device__ __align__(256) double scratch[FCFD_DOT_NOALLOC_MAX_N / BLOCK];
+static_assert(FCFD_DOT_NOALLOC_MAX_N % BLOCK == 0, "the partials of the largest n fill the array exactly");
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Address of `scratch` on the current device, looked up once per device.
+double* scratch_on_current_device() {
+    int dev = 0;
+    FCFD_HIP(hipGetDevice(&dev));
+    FCFD_REQUIRE(dev >= 0 && dev < kMaxDevices, "device id beyond the scratch address table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        FCFD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
 
 static thread_local std::string g_last_error;
 
@@ -509,7 +538,10 @@ int fcfd_ctx_create(const fcfd_mesh_desc* mesh, const fcfd_solver_desc* solver, 
     }
     *out = nullptr;
     fcfd_ctx* h = new fcfd_ctx();
-    int rc = guard([&] { build_ctx(h->c, mesh, solver, stream); });
+    int rc = guard([&] {
+        build_ctx(h->c, mesh, solver, stream);
+        h->scratch = scratch_on_current_device();
+    });
     if (rc != FCFD_OK) {
         destroy_ctx(h->c);
         delete h;
@@ -924,6 +956,16 @@ int fcfd_supg_elem_calc(fcfd_ctx* ctx, const double* U_n, const double* Ce, doub
 int fcfd_dot(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out) {
     if (!ctx || !x || !y || !out || n < 0) return FCFD_ERR_ARG;
     return run(ctx, [&] { *out = dot(ctx->c, x, y, (int)n); });
+}
+
+int fcfd_dot_noalloc(fcfd_ctx* ctx, const double* x, const double* y, int64_t n, double* out) {
+    if (!ctx || !x || !y || !out || n < 0) return FCFD_ERR_ARG;
+    if (n > FCFD_DOT_NOALLOC_MAX_N) {
+        g_last_error = "fcfd_dot_noalloc: n = " + std::to_string(n) + " exceeds FCFD_DOT_NOALLOC_MAX_N = " +
+                       std::to_string(FCFD_DOT_NOALLOC_MAX_N) + " (the size of its fixed partials array)";
+        return FCFD_ERR_ARG;
+    }
+    return run(ctx, [&] { dot_to_device(ctx->c, x, y, (int)n, ctx->scratch, out); });
 }
 
 int fcfd_axpby(fcfd_ctx* ctx, double a, const double* x, double b, double* y, int64_t n) {

@@ -319,6 +319,7 @@ void boundary_integral(Ctx& c, const double* U_d, double* F_p);
 void mesh_fill(Ctx& c, double* coords, int64_t* nconn);
 void scatter_ud(Ctx& c, const double* U_d);
 double dot(Ctx& c, const double* x, const double* y, int n);
+void dot_to_device(Ctx& c, const double* x, const double* y, int n, double* partials, double* out);
 void axpby(Ctx& c, double a, const double* x, double b, double* y, int n);
 float bench_momentum_apply(Ctx& c, int reps);
 

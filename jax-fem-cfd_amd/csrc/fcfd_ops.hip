@@ -300,15 +300,24 @@ struct DotBody {
     const double *x, *y;
     __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += x[k] * y[k]; }
 };
-struct DotFin {
-    DevState* st;
-    __device__ void operator()(const double (&t)[1]) const { st->dot = t[0]; }
+struct StoreFin {
+    double* out;
+    __device__ void operator()(const double (&t)[1]) const { *out = t[0]; }
 };
 
 void read_state_ints(Ctx& c, const int* dev, int n, int* host) {
     FCFD_HIP(hipMemcpyAsync(c.h_pinned, dev, n * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     FCFD_HIP(hipStreamSynchronize(c.stream));
     for (int i = 0; i < n; ++i) host[i] = c.h_pinned[i];
+}
+
+// *out (device) = x . y: one partial per BLOCK elements into partials[0, blocks_for(n)), then the
+// one-block fixed-order finalize.  Enqueued on c.stream, no synchronisation.
+void dot_to_device(Ctx& c, const double* x, const double* y, int n, double* partials, double* out) {
+    const int nb = blocks_for(n);
+    node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(n, nullptr, DotBody{x, y}, partials, nb);
+    finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, nullptr, StoreFin{out}, partials, nb);
+    FCFD_CHECK_LAUNCH();
 }
 
 // The generic dot product keeps its own partials buffer, grown to the vector length on demand:
@@ -323,9 +332,7 @@ double dot(Ctx& c, const double* x, const double* y, int n) {
         c.dot_pstride = nb + 1;
         c.dot_partials = dalloc<double>(c, (size_t)c.dot_pstride);
     }
-    node_kernel<1><<<nb, BLOCK, 0, c.stream>>>(n, nullptr, DotBody{x, y}, c.dot_partials, c.dot_pstride);
-    finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, nullptr, DotFin{c.st}, c.dot_partials, c.dot_pstride);
-    FCFD_CHECK_LAUNCH();
+    dot_to_device(c, x, y, n, c.dot_partials, &c.st->dot);
     double out;
     FCFD_HIP(hipMemcpyAsync(c.h_pinned, &c.st->dot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
     FCFD_HIP(hipStreamSynchronize(c.stream));

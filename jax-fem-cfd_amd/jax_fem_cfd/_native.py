@@ -89,6 +89,7 @@ _SIGS = {
     "fcfd_passive_timestep_vjp": (ctypes.c_int, [_vp, _vp, _vp, _vp, _d, _d, _d, _d, _vp, _vp, _vp, _vp, _i32, _vp,
                                                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i32)]),
     "fcfd_dot": (ctypes.c_int, [_vp, _vp, _vp, _i64, ctypes.POINTER(_d)]),
+    "fcfd_dot_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp]),
     "fcfd_axpby": (ctypes.c_int, [_vp, _d, _vp, _d, _vp, _i64]),
     "fcfd_bench_momentum_apply": (ctypes.c_int, [_vp, _i32, ctypes.POINTER(ctypes.c_float)]),
 }
