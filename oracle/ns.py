@@ -302,6 +302,14 @@ def passive_timestep(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, F_p, F, gam
     """NS step, then Ce(U_n, beta=1) and Ce(U, beta=1) and the scalar update.
     passive_transport.py:64-97."""
     U, P, U_it, P_it = four_step_timestep(s, U_n, P_n, dt, rho, mu, U_d, F_p, F)
+    phi, phi_it = scalar_step(s, U_n, U, phi_n, dt, D, phi_d, gamma_d_phi, supg)
+    return U, P, phi, U_it, P_it, phi_it
+
+
+def scalar_step(s, U_n, U, phi_n, dt, D, phi_d, gamma_d_phi, supg):
+    """The scalar half of the passive step for given old / new velocities
+    (passive_transport.py:80-95): convection matrices with beta = 1, SUPG tau and
+    Ct from U_n (tau's |Ce| from the new-velocity Ce), then the CN solve."""
     nconn, nn, ndim = s.nconn, s.nn, s.ndim
     Ce_n = el.conservative_convec_elem_calc(nconn, U_n, s.wq, s.N, s.Nderiv, nn, ndim, 1)
     Ce = el.conservative_convec_elem_calc(nconn, U, s.wq, s.N, s.Nderiv, nn, ndim, 1)
@@ -315,7 +323,7 @@ def passive_timestep(s, U_n, P_n, phi_n, dt, rho, mu, D, U_d, phi_d, F_p, F, gam
     else:
         phi, phi_it = update_phi(phi_n, nconn, s.Me, Ce, Ce_n, s.Le, D, dt, nn, gamma_d_phi,
                                  phi_d, s.tol)
-    return U, P, phi, U_it, P_it, phi_it
+    return phi, phi_it
 
 
 def rel_l2(a, b):
