@@ -461,7 +461,8 @@ struct BK3 {
 };
 struct BK3Fin {
     DevState* st;
-    __device__ void operator()(const double (&t)[2]) const {
+    ChunkEnd ce{};  // set on the last iteration of a speculative chunk
+    __device__ void update(const double (&t)[2]) const {
         const double alpha = st->b_alpha, omega = st->b_omega, rho_cur = st->b_rho_new;
         int k = (omega == 0.0 || alpha == 0.0) ? -11 : st->b_k + 1;
         if (rho_cur == 0.0) k = -10;
@@ -470,6 +471,15 @@ struct BK3Fin {
         st->b_rho_new = t[1];
         st->b_beta = t[1] / rho_cur * alpha / omega;
         st->b_active = (t[0] > st->b_atol2) && (k < st->b_maxit) && (k >= 0);
+    }
+    __device__ void operator()(const double (&t)[2]) const {
+        if (!ce.rec) {
+            update(t);
+            return;
+        }
+        if (!st->go) return;
+        if (st->b_active) update(t);
+        chunk_end(st, ce, st->b_active, st->b_k);
     }
 };
 
@@ -567,16 +577,17 @@ namespace fcfd {
 // BiCGSTAB iterations after an init kernel has set x, r, rhat, p, q and the state: chunks of
 // iteration pairs (graph-replayed) until the device flag says done.  Returns the reference's
 // iteration count (or -10 / -11).
-// deferred (non-null): only the predicted chunk is enqueued, followed by an asynchronous copy of
-// (k, active) into h_pinned[8..9]; *deferred = iterations enqueued, and the caller settles the
-// solve after a later synchronisation of the stream (bicgstab_settle).
+// chunk (non-null): only *chunk iterations are enqueued (*chunk <= 0: the predicted count) and no
+// state is read; *chunk = iterations enqueued, and a later call with start = that count (after
+// the device flag said the solve was still active) completes the solve.
 template <int NC, bool MT, class LOp>
 int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
-                     const uint8_t* mask, int maxit, double* x, int& pred, int* deferred = nullptr, int start = 0) {
+                     const uint8_t* mask, int maxit, double* x, int& pred, int* chunk = nullptr, int start = 0,
+                     ChunkEnd cend = ChunkEnd{}) {
     const int nn = c.g.nn;
     const int* act = &c.st->b_active;
-    // one iteration on the double buffers of parity o
-    auto one = [&](int o) {
+    // one iteration on the double buffers of parity o (ce: the last of a speculative chunk)
+    auto one = [&](int o, ChunkEnd ce = ChunkEnd{}) {
         const int nw = o ^ 1;
         if constexpr (NC == 2 && !MT && IsMomCoef<LOp>::value) {
             if (!c.g.periodic && mask) {
@@ -592,7 +603,7 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
                     lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st, c.partials, c.pstride, act);
                 finalize_kernel<3><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, BK2Fin{c.st}, c.partials, c.pstride);
                 launch_node<2>(c, nn, act, BK3R<NC>{nn, c.m_p[nw], c.m_q[nw], c.m_t, c.m_rhat, x, c.m_r, c.st},
-                               BK3Fin{c.st});
+                               BK3Fin{c.st, ce});
                 FCFD_CHECK_LAUNCH();
                 return;
             }
@@ -612,7 +623,7 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
                               BK2Fin{c.st});
         }
         launch_node<2>(c, nn, act, BK3R<NC>{nn, c.m_p[nw], c.m_q[nw], c.m_t, c.m_rhat, x, c.m_r, c.st},
-                       BK3Fin{c.st});
+                       BK3Fin{c.st, ce});
         FCFD_CHECK_LAUNCH();
     };
     auto pair = [&]() {
@@ -623,10 +634,16 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
     int launched = start;
     int n = start > 0 ? 2 : (pred > 0 ? pred : 2);
     int h[2] = {0, 0};
-    if (deferred) {
-        enqueue_iters(c, graph, key, kptr, launched & 1, n, one, pair);
-        FCFD_HIP(hipMemcpyAsync(c.h_pinned + 8, &c.st->b_k, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-        *deferred = n;
+    if (chunk) {
+        if (*chunk > 0) n = *chunk;
+        if (cend.rec) {
+            // the last iteration carries the chunk-end record (launched directly)
+            enqueue_iters(c, graph, key, kptr, launched & 1, n - 1, one, pair);
+            one((launched + n - 1) & 1, cend);
+        } else {
+            enqueue_iters(c, graph, key, kptr, launched & 1, n, one, pair);
+        }
+        *chunk = n;
         return 0;
     }
     while (true) {
@@ -640,34 +657,18 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
     return h[0];
 }
 
-// Settles a deferred solve once the stream has been synchronised past its state copy: returns
-// the iteration count; if the enqueued chunk had not converged, completes the solve here
-// (synchronously) and sets *resumed.
-template <int NC, bool MT, class LOp>
-int bicgstab_settle(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
-                    const uint8_t* mask, int maxit, double* x, int& pred, int launched, bool* resumed) {
-    const int k = c.h_pinned[8], active = c.h_pinned[9];
-    *resumed = false;
-    if (active && launched < maxit) {
-        *resumed = true;
-        return bicgstab_iterate<NC, MT>(c, graph, key, kptr, lhs, mask, maxit, x, pred, nullptr, launched);
-    }
-    if (k > 0) pred = k;
-    return k;
-}
-
 // Solves A x = R x_n + extra (Dirichlet rows: x = ud) from x0 = x_n with Dirichlet values, into
 // `x` (NC * nn, a context buffer: the captured iteration graph keeps its address).  Returns the
 // reference's iteration count (or -10 / -11).
 template <int NC, class LOp, class ROp, class Extra>
 int bicgstab_solve(Ctx& c, IterGraph& graph, const double (&key)[4], const void* kptr, const LOp& lhs,
                    const ROp& rhs, const Extra& extra, const double* xn, const double* ud, const uint8_t* mask,
-                   int maxit, double* x, int& pred, int* deferred = nullptr) {
-    launch_node<2>(c, c.g.nn, nullptr,
+                   int maxit, double* x, int& pred, int* chunk = nullptr, ChunkEnd cend = ChunkEnd{}) {
+    launch_node<2>(c, c.g.nn, c.gate,
                    BInit<NC, LOp, ROp, Extra>{lhs, rhs, extra, xn, ud, mask, x, c.m_r, c.m_rhat},
                    BInitFin{c.st, c.tol, maxit});
     FCFD_CHECK_LAUNCH();
-    return bicgstab_iterate<NC, false>(c, graph, key, kptr, lhs, mask, maxit, x, pred, deferred);
+    return bicgstab_iterate<NC, false>(c, graph, key, kptr, lhs, mask, maxit, x, pred, chunk, 0, cend);
 }
 
 // Transposed solve A^T x = b (adjoint of a masked forward solve), x0 = x_n with Dirichlet values.

@@ -195,11 +195,18 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     FCFD_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     FCFD_HIP(hipEventCreateWithFlags(&c.ev_in, hipEventDisableTiming));
     FCFD_HIP(hipEventCreateWithFlags(&c.ev_out, hipEventDisableTiming));
+    for (auto& e : c.ev_spec) FCFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     {
-        const char* e = std::getenv("FCFD_NO_GRAPHS");
-        c.use_graphs = !(e && e[0] == '1');
+        // solver chunks as captured HIP graphs (FCFD_GRAPHS=1) or as direct launches (default:
+        // measured faster at 1024^2, where each graph launch costs more device time than the
+        // launches it replaces)
+        const char* e = std::getenv("FCFD_GRAPHS");
+        c.use_graphs = e && e[0] == '1';
         const char* t = std::getenv("FCFD_MG_TILED");
         c.mg_tiled = !(t && t[0] == '0');
+        // a fixed chunk length for the solves of speculative steps (default: predicted)
+        const char* k = std::getenv("FCFD_SPEC_CHUNK");
+        c.spec_chunk_fixed = k ? std::max(0, std::atoi(k)) : 0;
     }
     c.tol = s->tol;
     c.set_zeroP = s->set_zeroP;
@@ -476,7 +483,13 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     c.partials = dalloc<double>(c, (size_t)MAX_RED * c.pstride);
     c.st = dalloc<DevState>(c, 1);
     FCFD_HIP(hipMemset(c.st, 0, sizeof(DevState)));
+    const int go = 1;
+    FCFD_HIP(hipMemcpy(&c.st->go, &go, sizeof(int), hipMemcpyHostToDevice));
     FCFD_HIP(hipHostMalloc((void**)&c.h_pinned, 64, hipHostMallocDefault));
+    // speculative-step records: written by kernels, read by the host after an event
+    FCFD_HIP(hipHostMalloc((void**)&c.h_spec, 8 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+    std::memset(c.h_spec, 0, 8 * sizeof(int));
+    FCFD_HIP(hipHostGetDevicePointer((void**)&c.d_spec, c.h_spec, 0));
     FCFD_HIP(hipDeviceSynchronize());
 }
 
@@ -486,6 +499,10 @@ void destroy_ctx(Ctx& c) {
     for (IterGraph* g : {&c.g_mom, &c.g_cg, &c.g_phi, &c.g_amom, &c.g_aphi}) graph_reset(*g);
     if (c.ev_in) (void)hipEventDestroy(c.ev_in);
     if (c.ev_out) (void)hipEventDestroy(c.ev_out);
+    for (auto& e : c.ev_spec) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.ev_in = c.ev_out = nullptr;
     c.stream = nullptr;
@@ -493,31 +510,113 @@ void destroy_ctx(Ctx& c) {
     c.allocs.clear();
     if (c.h_pinned) (void)hipHostFree(c.h_pinned);
     c.h_pinned = nullptr;
+    if (c.h_spec) (void)hipHostFree(c.h_spec);
+    c.h_spec = c.d_spec = nullptr;
 }
 
-// four_step_timestep (standard.py:148-165)
-// scatter = false: the Dirichlet values are already in place (a rollout scatters them once)
+// ------------------------------------------------------------------------------ speculative steps
+// A step is enqueued whole, without a host read: each solve gets a predicted chunk of iterations
+// (the device flags make iterations past convergence return at entry), and the last finalize of
+// each chunk writes the step's record in host-coherent memory -- the iteration count, or, if the
+// solve was still active, that the chunk fell short; it then clears st->go and the solver's flag
+// so that every later kernel of this step and of the steps enqueued behind it returns at entry.
+// The host reads the record once the step's event has passed and, for a step that fell short,
+// resumes the solve where the chunk ended and redoes the rest of the step.
+__global__ void k_spec_open(DevState* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->go = 1;
+}
+__global__ void k_spec_resume(DevState* st, int phase) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    st->go = 1;
+    (phase == 0 ? st->b_active : st->c_active) = 1;
+}
+
+// chunk length: the larger of the last two iteration counts (0 before any solve: the solver's own
+// prediction).  Counts mostly fall or stay level from step to step; a chunk that falls short
+// costs a resume and the no-op replay of the step enqueued behind it, one iteration too many
+// costs only that iteration's launches (its kernels return at entry).
+int spec_chunk(const Ctx& c, const int (&h)[3]) {
+    return c.spec_chunk_fixed > 0 ? c.spec_chunk_fixed : std::max(h[0], h[1]);
+}
+void spec_push(int (&h)[3], int k) {
+    h[2] = h[1];
+    h[1] = h[0];
+    h[0] = k > 0 ? k : 0;
+}
+
+struct StepLaunch {
+    int slot, mom, p;  // record slot, iterations enqueued per solve
+};
+
+// kernels launched while alive take st->go as their gate
+struct GateScope {
+    Ctx& c;
+    explicit GateScope(Ctx& cc) : c(cc) { c.gate = &c.st->go; }
+    ~GateScope() { c.gate = nullptr; }
+};
+
+// four_step_timestep (standard.py:148-165), enqueued as a speculative step.
+// scatter = false: the Dirichlet values are already in place (a rollout scatters them once).
+// P is written only by the output pass and U only by the final projection, after both solves,
+// so either may alias its input (P == P_n, U == U_n).
+StepLaunch four_step_enqueue(Ctx& c, int slot, const double* U_n, const double* P_n, double dt, double rho,
+                             double mu, const double* U_d, const double* F_p, const double* F, double* U, double* P,
+                             bool scatter) {
+    StepLaunch L{slot, spec_chunk(c, c.hist_mom), spec_chunk(c, c.hist_p)};
+    // the slot's previous step has been waited for (or did nothing): its record is free
+    c.h_spec[4 * slot] = 0;
+    int* rec = c.d_spec + 4 * slot;
+    if (scatter) scatter_ud(c, U_d);
+    {
+        GateScope gate(c);
+        build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
+        solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x, &L.mom, ChunkEnd{rec, 0});
+        projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
+        pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt, &L.p, ChunkEnd{rec, 1});
+        pressure_output(c, P);
+        projection_final(c, c.ustar, P, rho, dt, U);
+    }
+    FCFD_CHECK_LAUNCH();
+    FCFD_HIP(hipEventRecord(c.ev_spec[slot], c.stream));
+    return L;
+}
+
+// Waits for the step of L.slot and returns its iteration counts in it[0..1].  If a chunk fell
+// short, the solve is resumed and the rest of the step redone here (with host reads); steps
+// enqueued behind it did nothing and must be enqueued again (the return value is then false).
+bool four_step_finish(Ctx& c, const StepLaunch& L, const double* P_n, double dt, double rho, double mu,
+                      const double* F_p, double* U, double* P, int* it) {
+    FCFD_HIP(hipEventSynchronize(c.ev_spec[L.slot]));
+    const volatile int* rec = c.h_spec + 4 * L.slot;
+    const int status = rec[0], phase = rec[1], mom_it = rec[2], p_it = rec[3];
+    if (status == 1) {
+        it[0] = mom_it;
+        it[1] = p_it;
+    } else {
+        if (status != 2) throw Error(FCFD_ERR_STATE, "speculative step found not run");
+        k_spec_resume<<<1, 64, 0, c.stream>>>(c.st, phase);
+        if (phase == 0) {
+            it[0] = resume_momentum(c, c.ce_format, c.ce, c.ce_scale, rho, mu, dt, L.mom);
+            projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
+            it[1] = pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt);
+        } else {
+            it[0] = mom_it;
+            it[1] = pressure_resume(c, L.p);
+        }
+        pressure_output(c, P);
+        projection_final(c, c.ustar, P, rho, dt, U);
+        FCFD_CHECK_LAUNCH();
+    }
+    spec_push(c.hist_mom, it[0]);
+    spec_push(c.hist_p, it[1]);
+    return status == 1;
+}
+
 void four_step(Ctx& c, const double* U_n, const double* P_n, double dt, double rho, double mu, const double* U_d,
                const double* F_p, const double* F, double* U, double* P, int* it, bool scatter = true) {
-    if (scatter) scatter_ud(c, U_d);
-    build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
-    // The momentum solve's predicted chunk is enqueued without waiting for its iteration count:
-    // the pressure solve's own synchronisation also delivers it.  If that chunk had not
-    // converged, the momentum solve is completed and U* and the pressure solve are redone.
-    // P is written only by the output pass after the momentum solve has settled, and U only by
-    // the final projection, so either may alias its input (P == P_n, U == U_n).
-    int launched = 0;
-    solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x, &launched);
-    projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
-    it[1] = pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt);
-    bool resumed = false;
-    it[0] = settle_momentum(c, c.ce_format, c.ce, c.ce_scale, rho, mu, dt, launched, &resumed);
-    if (resumed) {
-        projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
-        it[1] = pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt);
-    }
-    pressure_output(c, P);
-    projection_final(c, c.ustar, P, rho, dt, U);
+    k_spec_open<<<1, 64, 0, c.stream>>>(c.st);  // no step is in flight at entry
+    const StepLaunch L = four_step_enqueue(c, 0, U_n, P_n, dt, rho, mu, U_d, F_p, F, U, P, scatter);
+    four_step_finish(c, L, P_n, dt, rho, mu, F_p, U, P, it);
 }
 
 double* api_ce_buffer(Ctx& c) {
@@ -836,6 +935,32 @@ int fcfd_four_step_rollout(fcfd_ctx* ctx, const double* U0, const double* P0, in
         }
         // the states alternate between (U, P) and the context's pair so that the last step
         // writes (U, P)
+        if (c.dim == 2 && c.order == 1) {
+            // one step of lookahead: step k + 1 is enqueued before the record of step k is read,
+            // so the device does not wait for the host between steps
+            auto out_U = [&](int k) { return (nsteps - 1 - k) % 2 == 0 ? U : c.roll_U; };
+            auto out_P = [&](int k) { return (nsteps - 1 - k) % 2 == 0 ? P : c.roll_P; };
+            auto in_U = [&](int k) { return k == 0 ? U0 : (const double*)out_U(k - 1); };
+            auto in_P = [&](int k) { return k == 0 ? P0 : (const double*)out_P(k - 1); };
+            auto enqueue = [&](int k) {
+                return four_step_enqueue(c, k & 1, in_U(k), in_P(k), dt, rho, mu, U_d, F_p, F, out_U(k), out_P(k),
+                                         k == 0);
+            };
+            k_spec_open<<<1, 64, 0, c.stream>>>(c.st);  // no step is in flight at entry
+            StepLaunch cur = enqueue(0), next{};
+            for (int k = 0; k < nsteps; ++k) {
+                if (k + 1 < nsteps) next = enqueue(k + 1);
+                int it[2];
+                const bool full = four_step_finish(c, cur, in_P(k), dt, rho, mu, F_p, out_U(k), out_P(k), it);
+                if (!full && k + 1 < nsteps) next = enqueue(k + 1);  // the first one did nothing
+                if (iters) {
+                    iters[2 * k] = it[0];
+                    iters[2 * k + 1] = it[1];
+                }
+                cur = next;
+            }
+            return;
+        }
         const double *Us = U0, *Ps = P0;
         for (int k = 0; k < nsteps; ++k) {
             const bool to_out = (nsteps - 1 - k) % 2 == 0;
@@ -844,8 +969,7 @@ int fcfd_four_step_rollout(fcfd_ctx* ctx, const double* U0, const double* P0, in
             int it[2];
             if (c.dim == 3 && c.order == 2) four_step_h27(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
             else if (c.dim == 3) four_step3(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
-            else if (c.order == 2) four_step_q2(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
-            else four_step(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it, k == 0);
+            else four_step_q2(c, Us, Ps, dt, rho, mu, U_d, F_p, F, Ud, Pd, it);
             if (iters) {
                 iters[2 * k] = it[0];
                 iters[2 * k + 1] = it[1];

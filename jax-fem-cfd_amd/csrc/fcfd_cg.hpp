@@ -88,7 +88,8 @@ struct CgK3 {
 
 struct CgK3Fin {
     DevState* st; int flex;
-    __device__ void operator()(const double (&t)[2]) const {
+    ChunkEnd ce{};  // set on the last iteration of a speculative chunk
+    __device__ void update(const double (&t)[2]) const {
         const double gnew = t[0];
         st->c_beta = flex ? (gnew - t[1]) / st->c_gamma : gnew / st->c_gamma;
         st->c_gamma = gnew;
@@ -96,16 +97,35 @@ struct CgK3Fin {
         st->c_k = k;
         st->c_active = (st->c_rr > st->c_atol2) && (k < st->c_maxit);
     }
+    __device__ void operator()(const double (&t)[2]) const {
+        if (!ce.rec) {
+            update(t);
+            return;
+        }
+        if (!st->go) return;
+        if (st->c_active) update(t);
+        chunk_end(st, ce, st->c_active, st->c_k);
+    }
 };
 
 // CgK3Fin with the iteration's r . r arriving in the same finalize (t[2]: the x / r update fused
 // into the V-cycle's first down leg)
 struct CgK3RFin {
     DevState* st; int flex;
+    ChunkEnd ce{};
     __device__ void operator()(const double (&t)[3]) const {
-        st->c_rr = t[2];
         const double t2[2] = {t[0], t[1]};
-        CgK3Fin{st, flex}(t2);
+        if (!ce.rec) {
+            st->c_rr = t[2];
+            CgK3Fin{st, flex}.update(t2);
+            return;
+        }
+        if (!st->go) return;
+        if (st->c_active) {
+            st->c_rr = t[2];
+            CgK3Fin{st, flex}.update(t2);
+        }
+        chunk_end(st, ce, st->c_active, st->c_k);
     }
 };
 

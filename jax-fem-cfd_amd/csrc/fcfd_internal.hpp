@@ -38,10 +38,46 @@ struct DevState {
     int pad_;
     // generic reduction output
     double dot;
+    // speculative steps: go = 0 once a predicted solver chunk of an enqueued step fell short;
+    // the step kernels given c.gate then return at entry until the host resumes that step
+    int go, go_pad_;
     // multigrid lazy means per level: see MG_* slots
     double mg[MAX_LEVELS][6];
 };
 enum { MG_R = 0, MG_X = 1, MG_F = 2, MG_C = 3, MG_P = 4, MG_Y = 5 };
+
+// The last finalize of a solver chunk of a speculative step (rec null: any other finalize).
+// rec: the step's record in host-coherent memory, {status (0 not run, 1 done, 2 fell short),
+// phase that fell short (0 momentum, 1 pressure), momentum iterations, pressure iterations}.
+struct ChunkEnd {
+    int* rec = nullptr;
+    int phase = 0;
+};
+// Runs after that finalize's own update (active and k are then the solver's flag and count for
+// the next iteration): a solve still active has fallen short -- st->go and its flag are cleared,
+// so every later gated kernel returns at entry; otherwise its count is recorded.
+__device__ inline void chunk_end(DevState* st, const ChunkEnd& ce, int& active, int k) {
+    if (active) {
+        st->go = 0;
+        active = 0;
+        ce.rec[1] = ce.phase;
+        ce.rec[0] = 2;
+        return;
+    }
+    ce.rec[2 + ce.phase] = k;
+    if (ce.phase == 1) ce.rec[0] = 1;
+}
+
+// The finalize of a chunk's last iteration is launched ungated (it must record a solve that
+// converged earlier in the chunk); its functor checks st->go and the solver flag itself.
+template <class Fin>
+inline auto fin_gate(const int* active, const Fin& f, int) -> decltype(f.ce.rec, (const int*)nullptr) {
+    return f.ce.rec ? nullptr : active;
+}
+template <class Fin>
+inline const int* fin_gate(const int* active, const Fin&, long) {
+    return active;
+}
 
 struct Level {
     Grid2D g;
@@ -77,6 +113,12 @@ struct Ctx {
     hipStream_t stream = nullptr;       // private non-blocking stream every kernel runs on
     hipStream_t user_stream = nullptr;  // the caller's stream (joined with events per call)
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipEvent_t ev_spec[2] = {nullptr, nullptr};  // end of the speculative step of each record slot
+    // speculative steps: when non-null, the kernels of a step that are not solver iterations take
+    // it as their `active` flag (st->go)
+    const int* gate = nullptr;
+    int hist_mom[3] = {0, 0, 0}, hist_p[3] = {0, 0, 0};  // the last iteration counts
+    int spec_chunk_fixed = 0;  // FCFD_SPEC_CHUNK: fixed chunk length of the speculative solves
     IterGraph g_mom, g_cg, g_phi, g_amom, g_aphi;
     bool use_graphs = true;
     bool mg_tiled = true;   // pinned non-periodic quad4 V-cycle: the tiled kernels (L_diag by node class)
@@ -140,6 +182,7 @@ struct Ctx {
     int dot_pstride = 0;
     DevState* st = nullptr;
     int* h_pinned = nullptr;   // pinned host mirror for flags / counts
+    int *h_spec = nullptr, *d_spec = nullptr;  // speculative-step records (host-coherent): host / device view
     // iteration predictions for chunked launching
     int pred_mom = 4, pred_p = 8, pred_phi = 4;
     // passive scalar: Dirichlet data and the per-element lhs / rhs matrices (lazy)
@@ -190,7 +233,7 @@ inline void launch_node(Ctx& c, int n, const int* active, const Body& body, cons
     if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
     node_kernel<NR><<<nb, BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
     if constexpr (NR > 0)
-        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, fin, c.partials, c.pstride);
+        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, fin_gate(active, fin, 0), fin, c.partials, c.pstride);
 }
 
 // A non-reducing node kernel in blocks of BS threads: on the small multigrid levels 64-thread
@@ -210,7 +253,7 @@ inline void launch_stencil(Ctx& c, int n, const int* active, const Body& body, c
     if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
     stencil_kernel<NR><<<nb, BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
     if constexpr (NR > 0)
-        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, fin, c.partials, c.pstride);
+        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, fin_gate(active, fin, 0), fin, c.partials, c.pstride);
 }
 
 // join the private stream to the caller's stream on entry / exit of an API call
@@ -324,12 +367,13 @@ void axpby(Ctx& c, double a, const double* x, double b, double* y, int n);
 float bench_momentum_apply(Ctx& c, int reps);
 
 // momentum (fcfd_momentum.hip)
-// deferred (non-null, U_hat = c.m_x): the predicted BiCGSTAB chunk only, settled later by
-// settle_momentum after a synchronisation of the stream (see bicgstab_iterate)
+// chunk (non-null, U_hat = c.m_x): only the init and *chunk BiCGSTAB iterations are enqueued
+// (*chunk <= 0: the predicted count), no host read; *chunk returns the iterations enqueued and
+// resume_momentum completes the solve from there
 int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n, const double* P_n, double rho,
-                   double mu, double dt, const double* F, double* U_hat, int* deferred = nullptr);
-int settle_momentum(Ctx& c, int fmt, void* store, float* scale, double rho, double mu, double dt, int launched,
-                    bool* resumed);
+                   double mu, double dt, const double* F, double* U_hat, int* chunk = nullptr,
+                   ChunkEnd ce = ChunkEnd{});
+int resume_momentum(Ctx& c, int fmt, void* store, float* scale, double rho, double mu, double dt, int launched);
 
 // passive scalar (fcfd_scalar.hip)
 int solve_scalar(Ctx& c, const double* U_n, const double* U, const double* phi_n, double dt, double D,
@@ -351,12 +395,19 @@ struct PrhsSpec {
 // zero_exit: a right-hand side that is exactly zero ends the solve at once with c_zero set
 // (the solution is 0; adjoint solves only).  prhs (pinned form only): b is formed by the init
 // kernel from prhs (b and bmean are then unused).
+// chunk (non-null): only the init and *chunk iterations are enqueued (*chunk <= 0: the predicted
+// count), no host read; *chunk returns the iterations enqueued and pcg_resume completes the solve
 int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
-        bool* converged, bool zero_exit = false, const PrhsSpec* prhs = nullptr);
+        bool* converged, bool zero_exit = false, const PrhsSpec* prhs = nullptr, int* chunk = nullptr,
+        ChunkEnd ce = ChunkEnd{});
+int pcg_resume(Ctx& c, int launched, int maxit, bool* converged);
 int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho,
                    double dt, double* P_out);
 // solve_pressure in two parts: the solve (solution in c.c_x) and the output pass into P_out
-int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt);
+// chunk: as for pcg; pressure_resume then completes it
+int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                        int* chunk = nullptr, ChunkEnd ce = ChunkEnd{});
+int pressure_resume(Ctx& c, int launched);
 void pressure_output(Ctx& c, double* P_out);
 void projection_ustar(Ctx& c, const double* U_hat, const double* P_n, double rho, double dt, double* U_star);
 void projection_final(Ctx& c, const double* U_star, const double* P, double rho, double dt, double* U);

@@ -522,6 +522,7 @@ struct JacobiSZ {
 struct CgTail {
     double* znew; const double* zold; double* pzero; int flex; int first;
     const double* cg_p = nullptr; const double* cg_Ap = nullptr; double* cg_x = nullptr; double* cg_rout = nullptr;
+    ChunkEnd ce{};  // the iteration ends a speculative chunk: its last finalize records it
 };
 
 // ------------------------------------------------------------------------------ coarse level
@@ -928,11 +929,13 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
                 finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, CgGamma0Fin{c.st}, c.partials,
                                                                   c.pstride);
             if constexpr (ZM == 2) {
+                const ChunkEnd ce = tail->ce;
+                const int* fa = ce.rec ? nullptr : active;
                 if (down_nb > 0)
-                    finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, down_nb, active, CgK3RFin{c.st, flex},
+                    finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, down_nb, fa, CgK3RFin{c.st, flex, ce},
                                                                           c.partials, c.pstride);
                 else
-                    finalize_kernel<2><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, CgK3Fin{c.st, flex}, c.partials,
+                    finalize_kernel<2><<<1, FIN_BLOCK, 0, c.stream>>>(nb, fa, CgK3Fin{c.st, flex, ce}, c.partials,
                                                                       c.pstride);
             }
         };
@@ -1060,7 +1063,7 @@ static bool vcycle_t(Ctx& c, const double* r0, const int* active, const CgTail* 
                 fused = true;
             } else if (l == 0 && tail) {
                 launch_stencil<2>(c, n, active, JacobiSZ<T, 2>{j0, tail->znew, tail->zold, tail->flex, nullptr},
-                                  CgK3Fin{c.st, tail->flex});
+                                  CgK3Fin{c.st, tail->flex, tail->ce});
                 fused = true;
             } else if (l == 0)
                 launch_stencil<0>(c, n, active, j0, NoFin{});
@@ -1302,19 +1305,106 @@ static const void* precondition(Ctx& c, const int* active, bool& is_f32, const C
 }
 
 // PCG: returns iterations; x0 in x0in (pins zeroed if zero_pins), solution in c.c_x
-int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
-        bool* converged, bool zero_exit, const PrhsSpec* prhs) {
+// one PCG iteration on the p / z double buffers of parity o (the solution accumulates in c.c_x)
+static void pcg_one(Ctx& c, int o, ChunkEnd ce = ChunkEnd{}) {
     const Grid2D& g = c.g;
     const int nn = g.nn;
     const int neu = (c.precond == FCFD_PC_MULTIGRID) && !c.set_zeroP;
     const int flex = c.precision == FCFD_PREC_MIXED && c.precond == FCFD_PC_MULTIGRID;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* Le = c.lv[0].ec_dev->Le;
+    const int* act = &c.st->c_active;
+    const int nw = o ^ 1;
+    if (cg_fusable(c)) {
+        // r alternates between c_r (parity 0) and c_r2; the tiled K1, then the V-cycle with the
+        // x / r update on its first down leg and the z-step on its last up leg
+        double* rin = o == 0 ? c.c_r : c.c_r2;
+        double* rout = o == 0 ? c.c_r2 : c.c_r;
+        {
+            constexpr int FC = CGT_C, FR = CGT_R;
+            const dim3 grid((g.nnx + FC - 1) / FC, (g.nny + FR - 1) / FR);
+            const int nb = (int)(grid.x * grid.y);
+            if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+            k_cg_k1t<FC, FR><<<grid, 256, 0, c.stream>>>(g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st,
+                                                        c.partials, c.pstride, act);
+            finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, CgK1Fin{c.st}, c.partials, c.pstride);
+        }
+        CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
+        tail.cg_p = c.c_p[nw];
+        tail.cg_Ap = c.c_Ap;
+        tail.cg_x = c.c_x;
+        tail.cg_rout = rout;
+        tail.ce = ce;
+        const bool f = vcycle_t_mixed_or_64(c, rin, act, &tail);
+        if (!f) throw Error(FCFD_ERR_STATE, "fused PCG iteration without its z-step");
+        FCFD_CHECK_LAUNCH();
+        return;
+    }
+    if (g.periodic)
+        launch_node<1>(c, nn, act, CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st}, CgK1Fin{c.st});
+    else
+        launch_stencil<1>(c, nn, act, CgK1S{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                          CgK1Fin{c.st});
+    launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
+    bool f, fused;
+    CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
+    tail.ce = ce;
+    const void* y = precondition(c, act, f, neu ? nullptr : &tail, &fused);
+    if (!fused && f)
+        launch_node<2>(c, nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                       CgK3Fin{c.st, flex, ce});
+    else if (!fused)
+        launch_node<2>(c, nn, act, CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                       CgK3Fin{c.st, flex, ce});
+    FCFD_CHECK_LAUNCH();
+}
+
+// n more PCG iterations, the first of parity `launched` & 1 (graph-replayed); with ce.rec the
+// last one is launched directly and records the end of a speculative chunk
+static void pcg_enqueue(Ctx& c, int launched, int n, ChunkEnd ce = ChunkEnd{}) {
+    const double key[4] = {0.0, 0.0, 0.0, 0.0};
+    auto one = [&](int o) { pcg_one(c, o); };
+    auto pair = [&]() {
+        pcg_one(c, 0);
+        pcg_one(c, 1);
+    };
+    if (ce.rec) {
+        enqueue_iters(c, c.g_cg, key, nullptr, launched & 1, n - 1, one, pair);
+        pcg_one(c, (launched + n - 1) & 1, ce);
+    } else {
+        enqueue_iters(c, c.g_cg, key, nullptr, launched & 1, n, one, pair);
+    }
+}
+
+// Chunks of PCG iterations until the device flag says done: the first chunk (launched = 0) is as
+// long as the previous solve took, later ones add two iterations.
+int pcg_resume(Ctx& c, int launched, int maxit, bool* converged) {
+    int n = launched > 0 ? 2 : (c.pred_p > 0 ? c.pred_p : 2);
+    int h[2] = {0, 0};
+    while (true) {
+        pcg_enqueue(c, launched, n);
+        launched += n;
+        n = 2;
+        read_state_ints(c, &c.st->c_k, 2, h);  // c_k, c_active
+        if (!h[1] || launched >= maxit) break;
+    }
+    if (converged) *converged = !h[1] && h[0] <= maxit;
+    if (h[0] > 0) c.pred_p = h[0];
+    return h[0];
+}
+
+int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
+        bool* converged, bool zero_exit, const PrhsSpec* prhs, int* chunk, ChunkEnd ce) {
+    const Grid2D& g = c.g;
+    const int nn = g.nn;
+    const int neu = (c.precond == FCFD_PC_MULTIGRID) && !c.set_zeroP;
+    const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
+    const double* Le = c.lv[0].ec_dev->Le;
     if (prhs)
-        launch_node<3>(c, nn, nullptr, CgInitD{g, Le, pin, *prhs, x0in, zero_pins, c.c_x, c.c_r},
+        launch_node<3>(c, nn, c.gate, CgInitD{g, Le, pin, *prhs, x0in, zero_pins, c.c_x, c.c_r},
                        CgInitFin{c.st, tol, maxit, nn, zero_exit});
     else
-        launch_node<3>(c, nn, nullptr, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
+        launch_node<3>(c, nn, c.gate, CgInit{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
                        CgInitFin{c.st, tol, maxit, nn, zero_exit});
     const int* act = &c.st->c_active;
     bool f32, fused0;
@@ -1327,72 +1417,13 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
         launch_node<1>(c, nn, act, CgGamma0<double>{c.c_r, (const double*)y0, c.c_z[0], c.c_p[0], c.st, neu},
                        CgGamma0Fin{c.st});
     FCFD_CHECK_LAUNCH();
-    // one chunk = a pair of PCG iterations (parity of the p / z double buffers returns to 0)
-    // one PCG iteration on the p / z double buffers of parity o
-    const bool fuse_cg = cg_fusable(c);
-    auto one = [&](int o) {
-        const int nw = o ^ 1;
-        if (fuse_cg) {
-            // r alternates between c_r (parity 0) and c_r2; CgK1S, then the V-cycle with the x / r
-            // update on its first down leg and the z-step on its last up leg
-            double* rin = o == 0 ? c.c_r : c.c_r2;
-            double* rout = o == 0 ? c.c_r2 : c.c_r;
-            {
-                constexpr int FC = CGT_C, FR = CGT_R;
-                const dim3 grid((g.nnx + FC - 1) / FC, (g.nny + FR - 1) / FR);
-                const int nb = (int)(grid.x * grid.y);
-                if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-                k_cg_k1t<FC, FR><<<grid, 256, 0, c.stream>>>(g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st,
-                                                            c.partials, c.pstride, act);
-                finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, CgK1Fin{c.st}, c.partials, c.pstride);
-            }
-            CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
-            tail.cg_p = c.c_p[nw];
-            tail.cg_Ap = c.c_Ap;
-            tail.cg_x = c.c_x;
-            tail.cg_rout = rout;
-            const bool f = vcycle_t_mixed_or_64(c, rin, act, &tail);
-            if (!f) throw Error(FCFD_ERR_STATE, "fused PCG iteration without its z-step");
-            FCFD_CHECK_LAUNCH();
-            return;
-        }
-        if (g.periodic)
-            launch_node<1>(c, nn, act, CgK1{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                           CgK1Fin{c.st});
-        else
-            launch_stencil<1>(c, nn, act, CgK1S{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                              CgK1Fin{c.st});
-        launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
-        bool f, fused;
-        const CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
-        const void* y = precondition(c, act, f, neu ? nullptr : &tail, &fused);
-        if (!fused && f)
-            launch_node<2>(c, nn, act, CgK3<float>{c.c_r, (const float*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
-                           CgK3Fin{c.st, flex});
-        else if (!fused)
-            launch_node<2>(c, nn, act, CgK3<double>{c.c_r, (const double*)y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
-                           CgK3Fin{c.st, flex});
-        FCFD_CHECK_LAUNCH();
-    };
-    auto pair = [&]() {
-        one(0);
-        one(1);
-    };
-    const double key[4] = {0.0, 0.0, 0.0, 0.0};
-    // the first chunk is as long as the previous solve took; later chunks add two iterations
-    int launched = 0;
-    int n = c.pred_p > 0 ? c.pred_p : 2;
-    int h[2] = {0, 0};
-    while (true) {
-        enqueue_iters(c, c.g_cg, key, nullptr, launched & 1, n, one, pair);
-        launched += n;
-        n = 2;
-        read_state_ints(c, &c.st->c_k, 2, h);  // c_k, c_active
-        if (!h[1] || launched >= maxit) break;
+    if (chunk) {
+        const int n = *chunk > 0 ? *chunk : (c.pred_p > 0 ? c.pred_p : 2);
+        pcg_enqueue(c, 0, n, ce);
+        *chunk = n;
+        return 0;
     }
-    if (converged) *converged = !h[1] && h[0] <= maxit;
-    if (h[0] > 0) c.pred_p = h[0];
-    return h[0];
+    return pcg_resume(c, 0, maxit, converged);
 }
 
 int pressure_cg(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged) {
@@ -1422,7 +1453,8 @@ struct PrhsBody {
 
 // The pressure solve proper: the solution stays in c.c_x (pinned form: 0 at the pins; Neumann:
 // its mean is taken by pressure_output).  Reads U_star and P_n (x0) only.
-int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt) {
+int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
+                        int* chunk, ChunkEnd ce) {
     const int nn = c.g.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
@@ -1431,21 +1463,26 @@ int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const d
     if (c.set_zeroP) {
         // pinned: no mean to subtract, so the PCG's init kernel forms the rhs itself
         const PrhsSpec ps{c.ec_dev, U_star, F_p, rho / dt};
-        it = pcg(c, nullptr, nullptr, P_n, 1, c.tol, maxit, &conv, false, &ps);
+        it = pcg(c, nullptr, nullptr, P_n, 1, c.tol, maxit, &conv, false, &ps, chunk, ce);
     } else {
-        launch_node<1>(c, nn, nullptr, PrhsBody{c.g, c.ec_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
+        launch_node<1>(c, nn, c.gate, PrhsBody{c.g, c.ec_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
                        MeanFin{&c.st->mean_rhs, nullptr, nn});
-        it = pcg(c, c.c_rhs, &c.st->mean_rhs, P_n, 0, c.tol, maxit, &conv);
+        it = pcg(c, c.c_rhs, &c.st->mean_rhs, P_n, 0, c.tol, maxit, &conv, false, nullptr, chunk, ce);
     }
     return it;
+}
+
+int pressure_resume(Ctx& c, int launched) {
+    bool conv;
+    return pcg_resume(c, launched, c.maxit_p > 0 ? c.maxit_p : 1000, &conv);
 }
 
 // P = the solution with the pins zeroed, or minus its mean (standard.py:119-123)
 void pressure_output(Ctx& c, double* P_out) {
     const int nn = c.g.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
-    if (!c.set_zeroP) launch_node<1>(c, nn, nullptr, SumXBody{c.c_x}, MeanFin{&c.st->mean_p, nullptr, nn});
-    launch_node<0>(c, nn, nullptr, PoutBody{c.c_x, pin, P_out, c.st, !c.set_zeroP}, NoFin{});
+    if (!c.set_zeroP) launch_node<1>(c, nn, c.gate, SumXBody{c.c_x}, MeanFin{&c.st->mean_p, nullptr, nn});
+    launch_node<0>(c, nn, c.gate, PoutBody{c.c_x, pin, P_out, c.st, !c.set_zeroP}, NoFin{});
     FCFD_CHECK_LAUNCH();
 }
 

@@ -130,10 +130,10 @@ static MomOps<S> mom_ops(Ctx& c, S ce, double rho, double mu, double dt) {
                      c.maxit_mom > 0 ? c.maxit_mom : 10 * 2 * g.nn, {rho, mu, dt, 0.0}};
 }
 
-// deferred: see bicgstab_iterate (U_hat must then be c.m_x)
+// chunk: see bicgstab_iterate (U_hat must then be c.m_x)
 template <class S>
 static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, double rho, double mu, double dt,
-                            const double* F, double* U_hat, int* deferred) {
+                            const double* F, double* U_hat, int* chunk, ChunkEnd cend) {
     const Grid2D& g = c.g;
     const MomOps<S> o = mom_ops(c, ce, rho, mu, dt);
     int it;
@@ -143,23 +143,23 @@ static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, 
         if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
         k_mom_initt<S, MOMT_C, MOMT_R><<<grid, BLOCK, 0, c.stream>>>(o.lhs, o.rhs, c.ec_dev, P_n, F, U_n, c.ud_full,
                                                                     c.dmask, c.m_x, c.m_r, c.m_rhat, c.partials,
-                                                                    c.pstride);
-        finalize_kernel<2><<<1, FIN_BLOCK, 0, c.stream>>>(nb, nullptr, BInitFin{c.st, c.tol, o.maxit}, c.partials,
+                                                                    c.pstride, c.gate);
+        finalize_kernel<2><<<1, FIN_BLOCK, 0, c.stream>>>(nb, c.gate, BInitFin{c.st, c.tol, o.maxit}, c.partials,
                                                           c.pstride);
         FCFD_CHECK_LAUNCH();
         it = bicgstab_iterate<2, false>(c, c.g_mom, o.key, ce.c, o.lhs, c.dmask, o.maxit, c.m_x, c.pred_mom,
-                                        deferred);
+                                        chunk, 0, cend);
     } else if (!g.periodic) {
-        launch_stencil<2>(c, g.nn, nullptr,
+        launch_stencil<2>(c, g.nn, c.gate,
                           BInitMomS<S>{o.lhs, o.rhs, c.ec_dev, P_n, F, U_n, c.ud_full, c.dmask, c.m_x, c.m_r,
                                        c.m_rhat},
                           BInitFin{c.st, c.tol, o.maxit});
         FCFD_CHECK_LAUNCH();
         it = bicgstab_iterate<2, false>(c, c.g_mom, o.key, ce.c, o.lhs, c.dmask, o.maxit, c.m_x, c.pred_mom,
-                                        deferred);
+                                        chunk, 0, cend);
     } else {
         it = bicgstab_solve<2>(c, c.g_mom, o.key, ce.c, o.lhs, o.rhs, MomExtra{g, c.ec_dev, P_n, F}, U_n, c.ud_full,
-                               c.dmask, o.maxit, c.m_x, c.pred_mom, deferred);
+                               c.dmask, o.maxit, c.m_x, c.pred_mom, chunk, cend);
     }
     if (U_hat != c.m_x)
         FCFD_HIP(hipMemcpyAsync(U_hat, c.m_x, sizeof(double) * 2 * g.nn, hipMemcpyDeviceToDevice, c.stream));
@@ -167,10 +167,10 @@ static int solve_momentum_t(Ctx& c, S ce, const double* U_n, const double* P_n, 
 }
 
 template <class S>
-static int settle_momentum_t(Ctx& c, S ce, double rho, double mu, double dt, int launched, bool* resumed) {
+static int resume_momentum_t(Ctx& c, S ce, double rho, double mu, double dt, int launched) {
     const MomOps<S> o = mom_ops(c, ce, rho, mu, dt);
-    return bicgstab_settle<2, false>(c, c.g_mom, o.key, ce.c, o.lhs, c.dmask, o.maxit, c.m_x, c.pred_mom, launched,
-                                     resumed);
+    return bicgstab_iterate<2, false>(c, c.g_mom, o.key, ce.c, o.lhs, c.dmask, o.maxit, c.m_x, c.pred_mom, nullptr,
+                                      launched);
 }
 
 template <class F>
@@ -185,16 +185,14 @@ static int with_ce(Ctx& c, int fmt, void* store, float* scale, F&& f) {
 }
 
 int solve_momentum(Ctx& c, int fmt, void* store, float* scale, const double* U_n, const double* P_n, double rho,
-                   double mu, double dt, const double* F, double* U_hat, int* deferred) {
-    if (deferred && U_hat != c.m_x) throw Error(FCFD_ERR_STATE, "a deferred momentum solve works in place");
+                   double mu, double dt, const double* F, double* U_hat, int* chunk, ChunkEnd cend) {
+    if (chunk && U_hat != c.m_x) throw Error(FCFD_ERR_STATE, "a chunked momentum solve works in place");
     return with_ce(c, fmt, store, scale,
-                   [&](auto ce) { return solve_momentum_t(c, ce, U_n, P_n, rho, mu, dt, F, U_hat, deferred); });
+                   [&](auto ce) { return solve_momentum_t(c, ce, U_n, P_n, rho, mu, dt, F, U_hat, chunk, cend); });
 }
 
-int settle_momentum(Ctx& c, int fmt, void* store, float* scale, double rho, double mu, double dt, int launched,
-                    bool* resumed) {
-    return with_ce(c, fmt, store, scale,
-                   [&](auto ce) { return settle_momentum_t(c, ce, rho, mu, dt, launched, resumed); });
+int resume_momentum(Ctx& c, int fmt, void* store, float* scale, double rho, double mu, double dt, int launched) {
+    return with_ce(c, fmt, store, scale, [&](auto ce) { return resume_momentum_t(c, ce, rho, mu, dt, launched); });
 }
 
 // ------------------------------------------------------------------------------ benchmark
@@ -267,7 +265,7 @@ struct UstarBody {
     }
 };
 void projection_ustar(Ctx& c, const double* U_hat, const double* P_n, double rho, double dt, double* U_star) {
-    launch_node<0>(c, c.g.nn, nullptr, UstarBody{c.g, c.ec_dev, U_hat, P_n, c.Mlump, U_star, dt / rho}, NoFin{});
+    launch_node<0>(c, c.g.nn, c.gate, UstarBody{c.g, c.ec_dev, U_hat, P_n, c.Mlump, U_star, dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
 }
 
@@ -294,7 +292,8 @@ struct FinalBody {
     }
 };
 void projection_final(Ctx& c, const double* U_star, const double* P, double rho, double dt, double* U) {
-    launch_node<0>(c, c.g.nn, nullptr, FinalBody{c.g, c.ec_dev, U_star, P, c.Mlump, c.ud_full, c.dmask, U, dt / rho}, NoFin{});
+    launch_node<0>(c, c.g.nn, c.gate, FinalBody{c.g, c.ec_dev, U_star, P, c.Mlump, c.ud_full, c.dmask, U, dt / rho},
+                   NoFin{});
     FCFD_CHECK_LAUNCH();
 }
 

@@ -229,8 +229,10 @@ __global__ __launch_bounds__(256) void k_mom_initt(MomCoef<S> lhs, MomCoef<S> rh
                                                    const double* __restrict__ xn, const double* __restrict__ ud,
                                                    const uint8_t* __restrict__ mask, double* __restrict__ x,
                                                    double* __restrict__ r, double* __restrict__ rhat,
-                                                   double* __restrict__ partials, int pstride) {
+                                                   double* __restrict__ partials, int pstride,
+                                                   const int* __restrict__ gate) {
     static_assert(FC * FR == BLOCK, "one node per thread");
+    if (gate && *gate == 0) return;
     constexpr int WW = FC + 2, WR = FR + 2;
     struct V7 {
         double x0, x1, u0, u1, p;

@@ -72,7 +72,9 @@ void launch_fill32(Ctx& c, float* x, float v, int n) {
 // ---------------------------------------------------------------------------------------------
 template <class S>
 __global__ __launch_bounds__(BLOCK) void k_build_ce(Grid2D g, const ElemConst* __restrict__ ec,
-                                                   const double* __restrict__ U, double beta, S store) {
+                                                   const double* __restrict__ U, double beta, S store,
+                                                   const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;
     const int e = xcd_block(blockIdx.x, gridDim.x) * BLOCK + threadIdx.x;
     if (e < g.ne) {
         int cn[4];
@@ -90,17 +92,17 @@ void build_ce(Ctx& c, const double* U, double beta, int fmt, void* store, float*
     int nb = blocks_for(c.g.ne);
     switch (fmt) {
         case FCFD_CE_F64:
-            k_build_ce<<<nb, BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeF64{(double*)store, c.g.ne});
+            k_build_ce<<<nb, BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeF64{(double*)store, c.g.ne}, c.gate);
             break;
         case FCFD_CE_F32:
-            k_build_ce<<<nb, BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeF32{(float*)store, c.g.ne});
+            k_build_ce<<<nb, BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeF32{(float*)store, c.g.ne}, c.gate);
             break;
         case FCFD_CE_F16S:
-            k_build_ce<<<nb, BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeF16S{(__half*)store, scale, c.g.ne});
+            k_build_ce<<<nb, BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeF16S{(__half*)store, scale, c.g.ne}, c.gate);
             break;
         case FCFD_CE_BF16S:
             k_build_ce<<<nb, BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta,
-                                                   CeBF16S{(__hip_bfloat16*)store, scale, c.g.ne});
+                                                   CeBF16S{(__hip_bfloat16*)store, scale, c.g.ne}, c.gate);
             break;
         default:
             throw Error(FCFD_ERR_ARG, "unknown Ce format");
@@ -116,7 +118,7 @@ struct CeAPIStore {
     }
 };
 void build_ce_api(Ctx& c, const double* U, double beta, double* Ce) {
-    k_build_ce<<<blocks_for(c.g.ne), BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeAPIStore{Ce});
+    k_build_ce<<<blocks_for(c.g.ne), BLOCK, 0, c.stream>>>(c.g, c.ec_dev, U, beta, CeAPIStore{Ce}, nullptr);
     FCFD_CHECK_LAUNCH();
 }
 

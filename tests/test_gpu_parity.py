@@ -451,6 +451,44 @@ def test_four_step_rollout_equals_single_steps(precision):
     assert [tuple(map(int, r)) for r in itr] == its
 
 
+def test_rollout_with_graphs_equals_direct_launches(monkeypatch):
+    """FCFD_GRAPHS=1 captures the solver chunks into HIP graphs and replays them; the fields
+    (bitwise) and per-step counts equal those of the default direct launches."""
+    n = 64
+    c, t, U_d, U0, P0, nn = _cavity(n, 1e-8, precision="mixed")
+    Ur, Pr, itr = t.context.four_step_rollout(U0, P0, 6, 5e-4, 1.0, 1e-3, U_d, None, None)
+    monkeypatch.setenv("FCFD_GRAPHS", "1")
+    c1, t1, *_ = _cavity(n, 1e-8, precision="mixed")
+    monkeypatch.delenv("FCFD_GRAPHS")
+    print(f"\n[case] rollout {n} mixed with captured graphs", flush=True)
+    Ug, Pg, itg = t1.context.four_step_rollout(U0, P0, 6, 5e-4, 1.0, 1e-3, U_d, None, None)
+    assert [tuple(map(int, r)) for r in itg] == [tuple(map(int, r)) for r in itr]
+    assert np.array_equal(_np(Ug), _np(Ur)) and np.array_equal(_np(Pg), _np(Pr))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_rollout_whose_chunks_all_fall_short(precision, monkeypatch):
+    """FCFD_SPEC_CHUNK=1 gives every solve of every speculative step a one-iteration chunk, so each
+    step of the native loop falls short in its momentum or its pressure solve: the steps enqueued
+    behind it do nothing, the step is resumed on the host and the next one enqueued again.  Fields
+    (bitwise) and per-step counts must equal those of single steps on a context with predicted
+    chunks, and the counts must exceed one iteration in both solves (both fallbacks taken)."""
+    n = 64
+    c, t, U_d, U0, P0, nn = _cavity(n, 1e-10, precision=precision)
+    U, P, its = U0, P0, []
+    for _ in range(6):
+        U, P, ui, pi = t(U, P, 5e-4, 1.0, 1e-3, U_d, 0, 0)
+        its.append((ui, pi))
+    monkeypatch.setenv("FCFD_SPEC_CHUNK", "1")
+    c1, t1, *_ = _cavity(n, 1e-10, precision=precision)
+    monkeypatch.delenv("FCFD_SPEC_CHUNK")
+    print(f"\n[case] rollout with one-iteration chunks {n} {precision}: counts {its}", flush=True)
+    Ur, Pr, itr = t1.context.four_step_rollout(U0, P0, 6, 5e-4, 1.0, 1e-3, U_d, None, None)
+    assert [tuple(map(int, r)) for r in itr] == its
+    assert np.array_equal(_np(Ur), _np(U)) and np.array_equal(_np(Pr), _np(P))
+    assert any(ui > 1 for ui, _ in its) and all(pi > 1 for _, pi in its)
+
+
 @pytest.mark.parametrize("ce_format", ["f32", "f16s", "bf16s"])
 def test_mixed_precision_cavity(ce_format):
     """Mixed form (FP32 V-cycle in FP64 flexible CG, reduced-precision stored Ce, FP64 Krylov
