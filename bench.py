@@ -307,7 +307,7 @@ def channel_3d(steps=200, warmup=3):
     mg["multigrid_vs_approx_inv_fp64_rel_l2"] = compare(fields["fp64"], mgf["fp64"])
     out["multigrid"] = mg
     out["config"] = {"workload": "3D channel 128x64x64 Q1 hex8, approx_inv PCG (and box-grid multigrid PCG), "
-                                 "dt=2e-3, 200 steps from rest",
+                                 f"dt=2e-3, {steps} steps from rest",
                      "nodes": int(nn), "elements": 128 * 64 * 64, "steps": steps}
     return out
 
@@ -400,6 +400,9 @@ def main():
     ap.add_argument("--no-3d", action="store_true", help="skip the 3D channel measurement")
     ap.add_argument("--roofline-only", action="store_true",
                     help="run only the dominant kernel and a calibration copy (for counter runs)")
+    ap.add_argument("--only-3d", action="store_true",
+                    help="run only the 3D channel measurements (hex8 and hex27) and print them")
+    ap.add_argument("--steps-3d", type=int, default=200, help="timed steps of each hex8 channel run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the timestep is a single-GPU path; "
@@ -412,6 +415,9 @@ def main():
 
     if args.roofline_only:
         roofline_only(20)
+        return
+    if args.only_3d:
+        print(json.dumps({"channel_3d": channel_3d(args.steps_3d), "channel_3d_q2": channel_3d_q2()}))
         return
     f64 = run_form("fp64", "f64", args.steps, args.warmup)
     mix = run_form("mixed", "f32", args.steps, args.warmup)
@@ -437,7 +443,7 @@ def main():
     pois64 = poisson_alone("fp64")
     poism = poisson_alone("mixed")
     grad = None if args.no_grad else gradient_256()
-    ch3d = None if args.no_3d else channel_3d()
+    ch3d = None if args.no_3d else channel_3d(args.steps_3d)
     ch3d_q2 = None if args.no_3d else channel_3d_q2()
     cpu = cpu1 = None
     if not args.no_cpu:
