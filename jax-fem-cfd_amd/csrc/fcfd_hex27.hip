@@ -152,9 +152,38 @@ __device__ __forceinline__ void cand_axis(int i, int ne, int q, int& el, int& lo
     v = q ? (!o && (i >> 1) < ne) : (el >= 0);
 }
 
-// sum_q has_q sum_b A(row_q, b, e_q) X(node_b of e_q) for NC components; A from rowf(a, e, cf)
-template <int NC, class RowF, class ValF>
-__device__ __forceinline__ void h27_apply(const Grid3D& g, const Inc27& s, RowF rowf, ValF val, double (&y)[NC]) {
+// Where element (ex, ey, ez)'s local node b is read: base(ex, ey, ez) + off(b).  GIdx27: global
+// node ids; TIdx27: ids in a staged tile window (k_tile27).
+struct GIdx27 {
+    int nnx, nxy;
+    __device__ __forceinline__ int base(int ex, int ey, int ez) const { return 2 * ez * nxy + 2 * ey * nnx + 2 * ex; }
+    __device__ __forceinline__ int off(int b) const { return h27z(b) * nxy + h27y(b) * nnx + h27x(b); }
+};
+
+// Staged tiles: a block owns T27X x T27Y x T27Z nodes with an even origin in every direction, so
+// every element incident to them lies in the window of (T27X + 3) x (T27Y + 3) x (T27Z + 3) nodes
+// from the origin - 2 (an even node reaches two nodes either way, an odd one one).
+constexpr int T27X = 16, T27Y = 4, T27Z = 4;
+constexpr int W27X = T27X + 3, W27Y = T27Y + 3, W27Z = T27Z + 3, W27N = W27X * W27Y * W27Z;
+constexpr int W27IT = (W27N + BLOCK - 1) / BLOCK, W27P = W27IT * BLOCK;
+static_assert(T27X * T27Y * T27Z == BLOCK && T27X % 2 == 0 && T27Y % 2 == 0 && T27Z % 2 == 0,
+              "one node per thread, even tile origins");
+struct TIdx27 {
+    int wi0, wj0, wk0;  // window origin
+    __device__ __forceinline__ int base(int ex, int ey, int ez) const {
+        return ((2 * ez - wk0) * W27Y + (2 * ey - wj0)) * W27X + (2 * ex - wi0);
+    }
+    __device__ __forceinline__ int off(int b) const { return (h27z(b) * W27Y + h27y(b)) * W27X + h27x(b); }
+    __device__ __forceinline__ int at(int i, int j, int k) const {
+        return ((k - wk0) * W27Y + (j - wj0)) * W27X + (i - wi0);
+    }
+};
+
+// sum_q has_q sum_b A(row_q, b, e_q) X(node_b of e_q) for NC components; A from rowf(a, e, cf),
+// X from val(c, ix.base(e) + ix.off(b))
+template <int NC, class RowF, class ValF, class Idx>
+__device__ __forceinline__ void h27_apply_ix(const Grid3D& g, const Inc27& s, RowF rowf, ValF val, double (&y)[NC],
+                                             const Idx& ix) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) y[c] = 0.0;
 #pragma unroll 1
@@ -166,7 +195,7 @@ __device__ __forceinline__ void h27_apply(const Grid3D& g, const Inc27& s, RowF 
         cand_axis(s.k, g.nez, q >> 2, ez, cz, vz);
         if (vx && vy && vz) {
             const int e = (ez * g.ney + ey) * g.nex + ex;
-            const int base = 2 * ez * g.nxy + 2 * ey * g.nnx + 2 * ex;
+            const int base = ix.base(ex, ey, ez);
             double cf[27];
             rowf((cz * 3 + cy) * 3 + cx, e, cf);
             double t[NC];
@@ -174,7 +203,7 @@ __device__ __forceinline__ void h27_apply(const Grid3D& g, const Inc27& s, RowF 
             for (int c = 0; c < NC; ++c) t[c] = 0.0;
 #pragma unroll
             for (int b = 0; b < 27; ++b) {
-                const int id = base + h27z(b) * g.nxy + h27y(b) * g.nnx + h27x(b);
+                const int id = base + ix.off(b);
 #pragma unroll
                 for (int c = 0; c < NC; ++c) t[c] += cf[b] * val(c, id);
             }
@@ -183,75 +212,152 @@ __device__ __forceinline__ void h27_apply(const Grid3D& g, const Inc27& s, RowF 
         }
     }
 }
+template <int NC, class RowF, class ValF>
+__device__ __forceinline__ void h27_apply(const Grid3D& g, const Inc27& s, RowF rowf, ValF val, double (&y)[NC]) {
+    h27_apply_ix<NC>(g, s, rowf, val, y, GIdx27{g.nnx, g.nxy});
+}
 
 // one shared 27 x 27 matrix (TR: its transpose)
+template <bool TR = false, class ValF, class Idx>
+__device__ __forceinline__ double h27_const_ix(const Grid3D& g, const Inc27& s, const double* __restrict__ A, ValF val,
+                                               const Idx& ix) {
+    double y[1];
+    h27_apply_ix<1>(g, s,
+                    [&](int a, int, double (&cf)[27]) {
+#pragma unroll
+                        for (int b = 0; b < 27; ++b) cf[b] = TR ? A[b * 27 + a] : A[a * 27 + b];
+                    },
+                    [&](int, int id) { return val(id); }, y, ix);
+    return y[0];
+}
 template <bool TR = false, class ValF>
 __device__ __forceinline__ double h27_const(const Grid3D& g, const Inc27& s, const double* __restrict__ A, ValF val) {
-    double y[1];
-    h27_apply<1>(g, s,
-                 [&](int a, int, double (&cf)[27]) {
+    return h27_const_ix<TR>(g, s, A, val, GIdx27{g.nnx, g.nxy});
+}
+
+// The staged tile kernel.  Body::stage(id, v) forms the Body::NV values of node id that the
+// element products read (e.g. p = r + beta (p_old - omega q_old)), once per window node; every
+// load of the block is issued before the first LDS store.  Window nodes outside the grid hold 0
+// (no existing element reads them).  The body then runs per owned node with the window in LDS
+// (sm[v][ix.base(e) + ix.off(b)]), in the operand order of the node-kernel form, so the results
+// are bitwise those of direct gathers.
+template <int NR, class Body>
+__global__ __launch_bounds__(256) void k_tile27(Grid3D g, int ntx, int nty, const int* __restrict__ active, Body body,
+                                                double* __restrict__ partials, int pstride) {
+    constexpr int NV = Body::NV;
+    if (active && *active == 0) return;
+    __shared__ double sm[NV][W27P];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int tz = lb / (ntx * nty), rem = lb - tz * (ntx * nty), ty = rem / ntx, tx = rem - ty * ntx;
+    const int i0 = tx * T27X, j0 = ty * T27Y, k0 = tz * T27Z;
+    const TIdx27 ix{i0 - 2, j0 - 2, k0 - 2};
+    double v[W27IT][NV];
+    bool in[W27IT];
 #pragma unroll
-                     for (int b = 0; b < 27; ++b) cf[b] = TR ? A[b * 27 + a] : A[a * 27 + b];
-                 },
-                 [&](int, int id) { return val(id); }, y);
-    return y[0];
+    for (int it = 0; it < W27IT; ++it) {
+        const int w = threadIdx.x + it * BLOCK;
+        const int wz = w / (W27X * W27Y), wr = w - wz * (W27X * W27Y), wy = wr / W27X, wx = wr - wy * W27X;
+        const int i = ix.wi0 + wx, j = ix.wj0 + wy, k = ix.wk0 + wz;
+        const bool ok = w < W27N && i >= 0 && i < g.nnx && j >= 0 && j < g.nny && k >= 0 && k < g.nnz;
+        in[it] = ok;
+        body.stage(ok ? (k * g.nny + j) * g.nnx + i : 0, v[it]);
+    }
+#pragma unroll
+    for (int it = 0; it < W27IT; ++it)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) sm[c][threadIdx.x + it * BLOCK] = in[it] ? v[it][c] : 0.0;
+    __syncthreads();
+    double acc[NR > 0 ? NR : 1];
+#pragma unroll
+    for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
+    const int t = threadIdx.x;
+    const int i = i0 + t % T27X, j = j0 + (t / T27X) % T27Y, k = k0 + t / (T27X * T27Y);
+    if (i < g.nnx && j < g.nny && k < g.nnz) body((k * g.nny + j) * g.nnx + i, Inc27{i, j, k}, ix, sm, acc);
+    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
+}
+
+// Launch a staged tile kernel over grid g and, when it reduces, its finalize kernel.
+template <int NR, class Body, class Fin>
+static void launch_tile27(Ctx& c, const Grid3D& g, const int* active, const Body& body, const Fin& fin) {
+    const int ntx = (g.nnx + T27X - 1) / T27X, nty = (g.nny + T27Y - 1) / T27Y, ntz = (g.nnz + T27Z - 1) / T27Z;
+    const int nb = ntx * nty * ntz;
+    if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+    k_tile27<NR><<<nb, BLOCK, 0, c.stream>>>(g, ntx, nty, active, body, c.partials, c.pstride);
+    if constexpr (NR > 0)
+        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, fin_gate(active, fin, 0), fin, c.partials, c.pstride);
 }
 
 // ------------------------------------------------------------------------------ Ce
 // Ce[a][b] = sum_q w_q N_qa (U_q . grad N_qb + beta div U_q N_qb)   (element_calc.py:165-185)
-// One element per thread: the quadrature values of U and div U first (loops unrolled), then per
-// column b the advection term at the 27 points and the 27 entries of the column.
+// A block builds CE27_E consecutive elements with 27 threads each: thread (el, b) stages the
+// element's nodal U, thread (el, q) forms U, V, W and div U at Gauss point q (sums over b in order),
+// then thread (el, b) forms the advection term of column b at the 27 points and the 27 entries of
+// column b.  For each row a the block's stores cover CE27_E * 27 consecutive values of the
+// row-blocked layout, so they coalesce.
+constexpr int CE27_E = 9;  // 243 of 256 threads
 template <class S>
-__global__ __launch_bounds__(64) void k_build_ce_h27(Grid3D g, const ElemConstH27* __restrict__ ec,
-                                                    const double* __restrict__ U, double beta, S store) {
-    const int e = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-    if (e >= g.ne) return;
-    const int ke = e / (g.nex * g.ney), r = e - ke * g.nex * g.ney;
-    const int je = r / g.nex, ie = r - je * g.nex;
-    const int base = 2 * ke * g.nxy + 2 * je * g.nnx + 2 * ie;
-    double uq[27], vq[27], wq[27], dq[27];
-#pragma unroll
-    for (int q = 0; q < 27; ++q) uq[q] = vq[q] = wq[q] = dq[q] = 0.0;
-#pragma unroll
-    for (int b = 0; b < 27; ++b) {
-        const int id = base + h27z(b) * g.nxy + h27y(b) * g.nnx + h27x(b);
-        const double u = U[id], v = U[id + g.nn], w = U[id + 2 * (size_t)g.nn];
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
-            const double Nb = ec->N[q * 27 + b];
-            uq[q] += Nb * u;
-            vq[q] += Nb * v;
-            wq[q] += Nb * w;
-            dq[q] += ec->dN[0][q * 27 + b] * u + ec->dN[1][q * 27 + b] * v + ec->dN[2][q * 27 + b] * w;
-        }
+__global__ __launch_bounds__(256) void k_build_ce_h27(Grid3D g, const ElemConstH27* __restrict__ ec,
+                                                     const double* __restrict__ U, double beta, S store) {
+    __shared__ double un[CE27_E][3][27];
+    __shared__ double qv[CE27_E][4][27];
+    const int el = threadIdx.x / 27, b = threadIdx.x - el * 27;
+    const int e = xcd_block(blockIdx.x, gridDim.x) * CE27_E + el;
+    const bool live = el < CE27_E && e < g.ne;
+    if (live) {
+        const int ke = e / (g.nex * g.ney), r = e - ke * g.nex * g.ney;
+        const int je = r / g.nex, ie = r - je * g.nex;
+        const int id = 2 * ke * g.nxy + 2 * je * g.nnx + 2 * ie + h27z(b) * g.nxy + h27y(b) * g.nnx + h27x(b);
+        un[el][0][b] = U[id];
+        un[el][1][b] = U[id + g.nn];
+        un[el][2][b] = U[id + 2 * (size_t)g.nn];
     }
-    for (int b = 0; b < 27; ++b) {
-        double adv[27];
+    __syncthreads();
+    if (live) {
+        const int q = b;
+        double uq = 0.0, vq = 0.0, wq = 0.0, dq = 0.0;
 #pragma unroll
-        for (int q = 0; q < 27; ++q)
-            adv[q] = uq[q] * ec->dN[0][q * 27 + b] + vq[q] * ec->dN[1][q * 27 + b] + wq[q] * ec->dN[2][q * 27 + b] +
-                     beta * dq[q] * ec->N[q * 27 + b];
-        for (int a = 0; a < 27; ++a) {
-            double t = 0.0;
-#pragma unroll
-            for (int q = 0; q < 27; ++q) t += ec->WN[a * 27 + q] * adv[q];
-            store.st(a * 27 + b, e, t);
+        for (int bb = 0; bb < 27; ++bb) {
+            const double u = un[el][0][bb], v = un[el][1][bb], w = un[el][2][bb];
+            const double Nb = ec->N[q * 27 + bb];
+            uq += Nb * u;
+            vq += Nb * v;
+            wq += Nb * w;
+            dq += ec->dN[0][q * 27 + bb] * u + ec->dN[1][q * 27 + bb] * v + ec->dN[2][q * 27 + bb] * w;
         }
+        qv[el][0][q] = uq;
+        qv[el][1][q] = vq;
+        qv[el][2][q] = wq;
+        qv[el][3][q] = dq;
     }
+    __syncthreads();
+    if (!live) return;
+    double adv[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+        adv[q] = qv[el][0][q] * ec->dN[0][q * 27 + b] + qv[el][1][q] * ec->dN[1][q * 27 + b] +
+                 qv[el][2][q] * ec->dN[2][q * 27 + b] + beta * qv[el][3][q] * ec->N[q * 27 + b];
+    for (int a = 0; a < 27; ++a) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < 27; ++q) t += ec->WN[a * 27 + q] * adv[q];
+        store.st(a * 27 + b, e, t);
+    }
+}
+
+template <class S>
+static void launch_build_ce_h27(Ctx& c, const Grid3D& g, const double* U, double beta, S store) {
+    const int nb = (g.ne + CE27_E - 1) / CE27_E;
+    k_build_ce_h27<<<nb, 256, 0, c.stream>>>(g, c.eh27_dev, U, beta, store);
+    FCFD_CHECK_LAUNCH();
 }
 
 static void build_ce_h27(Ctx& c, const double* U, double beta) {
-    const int nb = (c.g3.ne + 63) / 64;
-    if (c.ce_format == FCFD_CE_F64)
-        k_build_ce_h27<<<nb, 64, 0, c.stream>>>(c.g3, c.eh27_dev, U, beta, Ce27F64{(double*)c.ce, c.g3.ne});
-    else
-        k_build_ce_h27<<<nb, 64, 0, c.stream>>>(c.g3, c.eh27_dev, U, beta, Ce27F32{(float*)c.ce, c.g3.ne});
-    FCFD_CHECK_LAUNCH();
+    if (c.ce_format == FCFD_CE_F64) launch_build_ce_h27(c, c.g3, U, beta, Ce27F64{(double*)c.ce, c.g3.ne});
+    else launch_build_ce_h27(c, c.g3, U, beta, Ce27F32{(float*)c.ce, c.g3.ne});
 }
 
 void build_ce_h27_api(Ctx& c, const double* U, double beta, double* Ce) {
-    k_build_ce_h27<<<(c.g3.ne + 63) / 64, 64, 0, c.stream>>>(c.g3, c.eh27_dev, U, beta, Ce27Api{Ce});
-    FCFD_CHECK_LAUNCH();
+    launch_build_ce_h27(c, c.g3, U, beta, Ce27Api{Ce});
 }
 
 // ------------------------------------------------------------------------------ momentum
@@ -270,33 +376,40 @@ struct MomCoef27 {
 };
 
 // rhs = R U_n - G P_n + F (Dirichlet rows U_d), x0 = U_n with Dirichlet values, r0 = rhs - A x0,
-// rhat = p = q = r0; reductions b.b, r0.r0 (the quad4 BInit on hex27 gathers)
+// rhat = p = q = r0; reductions b.b, r0.r0 (the quad4 BInit on hex27 gathers).  Staged: U_n (3
+// components), x0 (3) and P_n.
 template <class S>
 struct BInit27 {
+    static constexpr int NV = 7;
     MomCoef27<S> lhs, rhs;
     const double *xn, *ud, *Pn, *F;
     const uint8_t* mask;
     double *x, *r, *rhat, *p, *q;
-    __device__ void operator()(int n, double (&acc)[2]) const {
-        const Grid3D& g = lhs.g;
-        const int nn = g.nn;
-        Inc27 s;
-        node_inc_h27(g, n, s);
-        double b[3], ax[3];
-        h27_apply<3>(g, s, [&](int a, int e, double (&cf)[27]) { rhs.row(a, e, cf); },
-                     [&](int c, int id) { return xn[id + c * nn]; }, b);
+    __device__ void stage(int id, double (&v)[NV]) const {
+        const int nn = lhs.g.nn;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            b[c] -= h27_const(g, s, lhs.ec->G[c], [&](int id) { return Pn[id]; });
+            const int i = id + c * nn;
+            const double xv = xn[i];
+            v[c] = xv;
+            v[3 + c] = (mask && mask[i]) ? ud[i] : xv;
+        }
+        v[6] = Pn[id];
+    }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[2]) const {
+        const Grid3D& g = lhs.g;
+        const int nn = g.nn;
+        double b[3], ax[3];
+        h27_apply_ix<3>(g, s, [&](int a, int e, double (&cf)[27]) { rhs.row(a, e, cf); },
+                        [&](int c, int id) { return sm[c][id]; }, b, ix);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            b[c] -= h27_const_ix(g, s, lhs.ec->G[c], [&](int id) { return sm[6][id]; }, ix);
             if (F) b[c] += F[n + c * nn];
         }
-        h27_apply<3>(g, s, [&](int a, int e, double (&cf)[27]) { lhs.row(a, e, cf); },
-                     [&](int c, int id) {
-                         const int i = id + c * nn;
-                         const double xv = xn[i];
-                         return (mask && mask[i]) ? ud[i] : xv;
-                     },
-                     ax);
+        h27_apply_ix<3>(g, s, [&](int a, int e, double (&cf)[27]) { lhs.row(a, e, cf); },
+                        [&](int c, int id) { return sm[3 + c][id]; }, ax, ix);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int i = n + c * nn;
@@ -313,29 +426,36 @@ struct BInit27 {
     }
 };
 
+// p = r + beta (p_old - omega q_old) (staged), q = A p, rhat . q
 template <class Op>
 struct BK1_27 {
+    static constexpr int NV = 3;
     Op op;
     const uint8_t* mask;
     const double *r, *pold, *qold, *rhat;
     double *pnew, *qnew;
     const DevState* st;
-    __device__ void operator()(int n, double (&acc)[1]) const {
+    __device__ void stage(int id, double (&v)[NV]) const {
+        const double beta = st->b_beta, omega = st->b_omega;
+        const int nn = op.g.nn;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int i = id + c * nn;
+            v[c] = r[i] + beta * (pold[i] - omega * qold[i]);
+        }
+    }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
         const Grid3D& g = op.g;
         const int nn = g.nn;
-        const double beta = st->b_beta, omega = st->b_omega;
-        auto pv = [&](int c, int id) {
-            const int i = id + c * nn;
-            return r[i] + beta * (pold[i] - omega * qold[i]);
-        };
-        Inc27 s;
-        node_inc_h27(g, n, s);
         double y[3];
-        h27_apply<3>(g, s, [&](int a, int e, double (&cf)[27]) { op.row(a, e, cf); }, pv, y);
+        h27_apply_ix<3>(g, s, [&](int a, int e, double (&cf)[27]) { op.row(a, e, cf); },
+                        [&](int c, int id) { return sm[c][id]; }, y, ix);
+        const int own = ix.at(s.i, s.j, s.k);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int i = n + c * nn;
-            const double pc = pv(c, n);
+            const double pc = sm[c][own];
             const double yc = (mask && mask[i]) ? pc : y[c];
             pnew[i] = pc;
             qnew[i] = yc;
@@ -344,26 +464,33 @@ struct BK1_27 {
     }
 };
 
+// s = r - alpha q (staged), t = A s, s.s, t.s, t.t
 template <class Op>
 struct BK2_27 {
+    static constexpr int NV = 3;
     Op op;
     const uint8_t* mask;
     const double *r, *q;
     double *sv, *tv;
     const DevState* st;
-    __device__ void operator()(int n, double (&acc)[3]) const {
+    __device__ void stage(int id, double (&v)[NV]) const {
+        const double alpha = st->b_alpha;
+        const int nn = op.g.nn;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = r[id + c * nn] - alpha * q[id + c * nn];
+    }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[3]) const {
         const Grid3D& g = op.g;
         const int nn = g.nn;
-        const double alpha = st->b_alpha;
-        auto svv = [&](int c, int id) { return r[id + c * nn] - alpha * q[id + c * nn]; };
-        Inc27 s;
-        node_inc_h27(g, n, s);
         double y[3];
-        h27_apply<3>(g, s, [&](int a, int e, double (&cf)[27]) { op.row(a, e, cf); }, svv, y);
+        h27_apply_ix<3>(g, s, [&](int a, int e, double (&cf)[27]) { op.row(a, e, cf); },
+                        [&](int c, int id) { return sm[c][id]; }, y, ix);
+        const int own = ix.at(s.i, s.j, s.k);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int i = n + c * nn;
-            const double sc = svv(c, n);
+            const double sc = sm[c][own];
             const double yc = (mask && mask[i]) ? sc : y[c];
             sv[i] = sc;
             tv[i] = yc;
@@ -384,18 +511,19 @@ static int solve_momentum_h27_t(Ctx& c, S ce, const double* U_n, const double* P
     Op lhs{g, c.eh27_dev, ce, rho / dt, theta * mu, theta * rho};
     Op rhs{g, c.eh27_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
     const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 3 * nn;
-    launch_node<2>(c, nn, nullptr,
-                   BInit27<S>{lhs, rhs, U_n, c.ud_full, P_n, F, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
-                   BInitFin{c.st, c.tol, maxit});
+    launch_tile27<2>(c, g, nullptr,
+                     BInit27<S>{lhs, rhs, U_n, c.ud_full, P_n, F, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                     BInitFin{c.st, c.tol, maxit});
     FCFD_CHECK_LAUNCH();
     const int* act = &c.st->b_active;
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_node<1>(c, nn, act,
-                           BK1_27<Op>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
-                           BK1Fin{c.st});
-            launch_node<3>(c, nn, act, BK2_27<Op>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st}, BK2Fin{c.st});
+            launch_tile27<1>(c, g, act,
+                             BK1_27<Op>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
+                             BK1Fin{c.st});
+            launch_tile27<3>(c, g, act, BK2_27<Op>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+                             BK2Fin{c.st});
             launch_node<2>(c, nn, act, BK3<3>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
                            BK3Fin{c.st});
         }
@@ -429,18 +557,19 @@ int solve_momentum_h27(Ctx& c, const double* U_n, const double* P_n, double rho,
 // ------------------------------------------------------------------------------ projection
 // Uout = Uin + c G P / M (c = dt/rho: U*; c = -dt/rho with Dirichlet values: the final U)
 struct Proj27Body {
+    static constexpr int NV = 1;
     Grid3D g;
     const ElemConstH27* ec;
     const double *Uin, *P, *M, *ud;
     const uint8_t* mask;
     double* Uout;
     double c;
-    __device__ void operator()(int n, double (&)[1]) const {
-        Inc27 s;
-        node_inc_h27(g, n, s);
+    __device__ void stage(int id, double (&v)[NV]) const { v[0] = P[id]; }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&)[1]) const {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const double gd = h27_const(g, s, ec->G[d], [&](int id) { return P[id]; });
+            const double gd = h27_const_ix(g, s, ec->G[d], [&](int id) { return sm[0][id]; }, ix);
             const size_t i = n + (size_t)d * g.nn;
             double u = Uin[i] + (c * gd) / M[i];
             if (mask && mask[i]) u = ud[i];
@@ -451,16 +580,17 @@ struct Proj27Body {
 
 // ------------------------------------------------------------------------------ pressure
 struct Prhs27Body {
+    static constexpr int NV = 3;
     Grid3D g; const ElemConstH27* ec; const double* Us; const double* Fp; const uint8_t* pin; double* rhs; double c;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        Inc27 s;
-        node_inc_h27(g, n, s);
+    __device__ void stage(int id, double (&v)[NV]) const {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] = Us[id + (size_t)d * g.nn];
+    }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
         double div = 0.0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double* u = Us + (size_t)d * g.nn;
-            div += h27_const<true>(g, s, ec->G[d], [&](int id) { return u[id]; });
-        }
+        for (int d = 0; d < 3; ++d) div += h27_const_ix<true>(g, s, ec->G[d], [&](int id) { return sm[d][id]; }, ix);
         double v = c * (div - (Fp ? Fp[n] : 0.0));
         if (pin && pin[n]) v = 0.0;
         rhs[n] = v;
@@ -469,20 +599,20 @@ struct Prhs27Body {
 };
 
 struct CgInit27 {
+    static constexpr int NV = 1;
     Grid3D g; const double* Le; const uint8_t* pin; const double* b; const double* bmean; const double* x0in;
     int zero_pins; double* x; double* r;
-    __device__ void operator()(int n, double (&acc)[3]) const {
-        Inc27 s;
-        node_inc_h27(g, n, s);
-        auto xv = [&](int id) {
-            const double v = x0in[id];
-            return (zero_pins && pin && pin[id]) ? 0.0 : v;
-        };
+    __device__ void stage(int id, double (&v)[NV]) const {
+        const double xv = x0in[id];
+        v[0] = (zero_pins && pin && pin[id]) ? 0.0 : xv;
+    }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[3]) const {
         const bool pn = pin && pin[n];
-        const double ax = pn ? 0.0 : h27_const(g, s, Le, xv);
+        const double ax = pn ? 0.0 : h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix);
         const double bn = b[n] - (bmean ? *bmean : 0.0);
         const double rn = bn - ax;
-        x[n] = xv(n);
+        x[n] = sm[0][ix.at(s.i, s.j, s.k)];
         r[n] = rn;
         acc[0] += bn * bn;
         acc[1] += rn * rn;
@@ -491,15 +621,15 @@ struct CgInit27 {
 };
 
 struct CgK1_27 {
+    static constexpr int NV = 1;
     Grid3D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
     const DevState* st;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        const double beta = st->c_beta;
-        auto pv = [&](int id) { return z[id] + beta * pold[id]; };
-        Inc27 s;
-        node_inc_h27(g, n, s);
-        const double a = (pin && pin[n]) ? 0.0 : h27_const(g, s, Le, pv);
-        const double pn = pv(n);
+    __device__ void stage(int id, double (&v)[NV]) const { v[0] = z[id] + st->c_beta * pold[id]; }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
+        const double a =
+            (pin && pin[n]) ? 0.0 : h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix);
+        const double pn = sm[0][ix.at(s.i, s.j, s.k)];
         pnew[n] = pn;
         Ap[n] = a;
         acc[0] += pn * a;
@@ -508,20 +638,22 @@ struct CgK1_27 {
 
 // two-term Neumann-series approximate inverse (standard.py:225-234)
 struct ApproxInv27a {
+    static constexpr int NV = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
-    __device__ void operator()(int n, double (&)[1]) const {
-        Inc27 s;
-        node_inc_h27(g, n, s);
-        const double v = h27_const(g, s, Le, [&](int id) { return r[id] / Ld[id]; }) / Ld[n];
+    __device__ void stage(int id, double (&v)[NV]) const { v[0] = r[id] / Ld[id]; }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&)[1]) const {
+        const double v = h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix) / Ld[n];
         d1l[n] = (pin && pin[n]) ? 0.0 : v;
     }
 };
 struct ApproxInv27b {
+    static constexpr int NV = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l; double* z;
-    __device__ void operator()(int n, double (&)[1]) const {
-        Inc27 s;
-        node_inc_h27(g, n, s);
-        double l2 = h27_const(g, s, Le, [&](int id) { return d1l[id]; });
+    __device__ void stage(int id, double (&v)[NV]) const { v[0] = d1l[id]; }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&)[1]) const {
+        double l2 = h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix);
         if (pin && pin[n]) l2 = 0.0;
         const double d1x = r[n] / Ld[n];
         z[n] = (3.0 * d1x - 3.0 * d1l[n]) + l2 / Ld[n];
@@ -538,24 +670,30 @@ struct ApproxInv27b {
 // f = r - L x: pinned (x = 2/3 r / L_diag with the pins 0, pinned rows of L x 0) or Neumann (x
 // and r mean-projected)
 struct DownRes27 {
+    static constexpr int NV = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* x; double* f;
     const DevState* st; int l; int neumann;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        Inc27 s;
-        node_inc_h27(g, n, s);
+    // the pre-smoothed x (pinned: 2/3 r / L_diag, 0 at the pins) or the mean-projected x (Neumann)
+    __device__ void stage(int id, double (&v)[NV]) const {
+        if (neumann) {
+            v[0] = x[id] - st->mg[l][MG_X];
+        } else {
+            const double xv = (2.0 / 3.0 * r[id]) / Ld[id];
+            v[0] = (pin && pin[id]) ? 0.0 : xv;
+        }
+    }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
+        auto xv = [&](int id) { return sm[0][id]; };
         double fv;
         if (neumann) {
-            const double mx = st->mg[l][MG_X], mr = st->mg[l][MG_R];
-            fv = (r[n] - mr) - h27_const(g, s, Le, [&](int id) { return x[id] - mx; });
+            const double mr = st->mg[l][MG_R];
+            fv = (r[n] - mr) - h27_const_ix(g, s, Le, xv, ix);
         } else {
-            auto xv = [&](int id) {
-                const double v = (2.0 / 3.0 * r[id]) / Ld[id];
-                return (pin && pin[id]) ? 0.0 : v;
-            };
-            x[n] = xv(n);
+            x[n] = xv(ix.at(s.i, s.j, s.k));
             const bool pn = pin && pin[n];
             const double rh = pn ? 0.0 : r[n];
-            const double lx = pn ? 0.0 : h27_const(g, s, Le, xv);
+            const double lx = pn ? 0.0 : h27_const_ix(g, s, Le, xv, ix);
             fv = rh - lx;
         }
         f[n] = fv;
@@ -565,20 +703,25 @@ struct DownRes27 {
 
 // x <- x + p (p = P y of the coarser level); one damped Jacobi (multigrid.py:214-228, :306-326)
 struct Jacobi27 {
+    static constexpr int NV = 1;
     Grid3D g; const double* Leoff; const double* Ld; const uint8_t* pin; const double* r; const double* x;
     const double* p; double* y; const DevState* st; int l; int neumann;
-    __device__ void operator()(int n, double (&acc)[1]) const {
-        Inc27 s;
-        node_inc_h27(g, n, s);
-        const double mx = neumann ? st->mg[l][MG_X] : 0.0, mp = neumann ? st->mg[l][MG_P] : 0.0;
-        auto xh = [&](int id) {
-            if (neumann) return (x[id] - mx) + (p[id] - mp);
-            const double v = x[id] + p[id];
-            return (pin && pin[id]) ? 0.0 : v;
-        };
+    // x + P y (0 at the pins), or with the means projected out (Neumann)
+    __device__ void stage(int id, double (&v)[NV]) const {
+        if (neumann) {
+            const double mx = st->mg[l][MG_X], mp = st->mg[l][MG_P];
+            v[0] = (x[id] - mx) + (p[id] - mp);
+        } else {
+            const double xv = x[id] + p[id];
+            v[0] = (pin && pin[id]) ? 0.0 : xv;
+        }
+    }
+    template <class SM>
+    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
+        auto xh = [&](int id) { return sm[0][id]; };
         const double rh = neumann ? r[n] - (l == 0 ? 0.0 : st->mg[l][MG_R]) : r[n];
-        const double off = h27_const(g, s, Leoff, xh);
-        double v = (OMEGA * (rh - off)) / Ld[n] + (1.0 - OMEGA) * xh(n);
+        const double off = h27_const_ix(g, s, Leoff, xh, ix);
+        double v = (OMEGA * (rh - off)) / Ld[n] + (1.0 - OMEGA) * xh(ix.at(s.i, s.j, s.k));
         if (!neumann && pin && pin[n]) v = 0.0;
         y[n] = v;
         acc[0] += v;
@@ -644,6 +787,10 @@ static const double* vcycle_h27(Ctx& c, const double* r0, const int* active) {
         if (neu) launch_node<1>(c, n, active, body, fin);
         else launch_node<0>(c, n, active, body, fin);
     };
+    auto run_tile = [&](const Grid3D& g, const auto& body, const MeanFin& fin) {
+        if (neu) launch_tile27<1>(c, g, active, body, fin);
+        else launch_tile27<0>(c, g, active, body, fin);
+    };
     auto rl = [&](int l) -> const double* { return l == 0 ? r0 : c.lv[l].r; };
     for (int l = 0; l < ld; ++l) {
         Level& L = c.lv[l];
@@ -652,8 +799,8 @@ static const double* vcycle_h27(Ctx& c, const double* r0, const int* active) {
         if (neu)
             launch_node<1>(c, n, active, DownX<double, double>{rl(l), L.Ld, L.x, c.st, l},
                            MeanFin{&c.st->mg[l][MG_X], nullptr, n});
-        run(n, DownRes27{L.g3, L.eh27_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f, c.st, l, neu},
-            MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+        run_tile(L.g3, DownRes27{L.g3, L.eh27_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f, c.st, l, neu},
+                 MeanFin{&c.st->mg[l][MG_F], nullptr, n});
         run(Lc.g3.nn, Restrict3{L.g3, Lc.g3, L.f, Lc.r, c.st, l, neu},
             MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g3.nn});
     }
@@ -673,8 +820,8 @@ static const double* vcycle_h27(Ctx& c, const double* r0, const int* active) {
         Level& Lc = c.lv[l + 1];
         const int n = L.g3.nn;
         run(n, Prolong3{L.g3, Lc.g3, Lc.y, L.f, c.st, l, neu}, MeanFin{&c.st->mg[l][MG_P], nullptr, n});
-        run(n, Jacobi27{L.g3, L.eh27_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, L.y, c.st, l, neu},
-            MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
+        run_tile(L.g3, Jacobi27{L.g3, L.eh27_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, L.y, c.st, l, neu},
+                 MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
     }
     FCFD_CHECK_LAUNCH();
     return c.lv[0].y;
@@ -707,9 +854,10 @@ static const double* precondition_h27(Ctx& c, const int* active) {
             launch_node<0>(c, nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y}, NoFin{});
             return c.lv[0].y;
         case FCFD_PC_APPROX_INV:
-            launch_node<0>(c, nn, active, ApproxInv27a{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x}, NoFin{});
-            launch_node<0>(c, nn, active, ApproxInv27b{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y},
-                           NoFin{});
+            launch_tile27<0>(c, c.g3, active, ApproxInv27a{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x},
+                             NoFin{});
+            launch_tile27<0>(c, c.g3, active,
+                             ApproxInv27b{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y}, NoFin{});
             return c.lv[0].y;
         case FCFD_PC_MULTIGRID: return vcycle_h27(c, c.c_r, active);
     }
@@ -723,8 +871,8 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
     const int nn = g.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* Le = c.eh27_dev->Le;
-    launch_node<3>(c, nn, nullptr, CgInit27{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
-                   CgInitFin{c.st, tol, maxit, nn, 0});
+    launch_tile27<3>(c, g, nullptr, CgInit27{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
+                     CgInitFin{c.st, tol, maxit, nn, 0});
     const int* act = &c.st->c_active;
     const int neu = c.precond == FCFD_PC_MULTIGRID && !c.set_zeroP;
     const double* y0 = precondition_h27(c, act);
@@ -734,8 +882,8 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_node<1>(c, nn, act, CgK1_27{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
-                           CgK1Fin{c.st});
+            launch_tile27<1>(c, g, act, CgK1_27{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+                             CgK1Fin{c.st});
             if (c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_NONE) {
                 launch_node<2>(c, nn, act,
                                CgK23{c.c_p[nw], c.c_Ap, c.precond == FCFD_PC_JACOBI ? Ld : nullptr, c.c_x, c.c_r,
@@ -776,8 +924,8 @@ int solve_pressure_h27(Ctx& c, const double* U_star, const double* P_n, const do
                        double* P_out) {
     const int nn = c.g3.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
-    launch_node<1>(c, nn, nullptr, Prhs27Body{c.g3, c.eh27_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
-                   MeanFin{&c.st->mean_rhs, nullptr, nn});
+    launch_tile27<1>(c, c.g3, nullptr, Prhs27Body{c.g3, c.eh27_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
+                     MeanFin{&c.st->mean_rhs, nullptr, nn});
     const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
     bool conv;
     const int it = pcg_h27(c, c.c_rhs, c.set_zeroP ? nullptr : &c.st->mean_rhs, P_n, c.set_zeroP, c.tol, maxit, &conv);
@@ -1074,13 +1222,12 @@ void four_step_h27(Ctx& c, const double* U_n, const double* P_n, double dt, doub
                    const double* F_p, const double* F, double* U, double* P, int* it) {
     scatter_ud(c, U_d);
     it[0] = solve_momentum_h27(c, U_n, P_n, rho, mu, dt, F, c.m_x);
-    const int nn = c.g3.nn;
-    launch_node<0>(c, nn, nullptr,
-                   Proj27Body{c.g3, c.eh27_dev, c.m_x, P_n, c.Mlump, nullptr, nullptr, c.ustar, dt / rho}, NoFin{});
+    launch_tile27<0>(c, c.g3, nullptr,
+                     Proj27Body{c.g3, c.eh27_dev, c.m_x, P_n, c.Mlump, nullptr, nullptr, c.ustar, dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
     it[1] = solve_pressure_h27(c, c.ustar, P_n, F_p, rho, dt, P);
-    launch_node<0>(c, nn, nullptr,
-                   Proj27Body{c.g3, c.eh27_dev, c.ustar, P, c.Mlump, c.ud_full, c.dmask, U, -dt / rho}, NoFin{});
+    launch_tile27<0>(c, c.g3, nullptr,
+                     Proj27Body{c.g3, c.eh27_dev, c.ustar, P, c.Mlump, c.ud_full, c.dmask, U, -dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
 }
 
