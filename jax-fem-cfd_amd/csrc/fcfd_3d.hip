@@ -807,8 +807,9 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
             if (c.precond == FCFD_PC_MULTIGRID) {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
                 const double* y = precondition3(c, act);
-                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, 0},
-                               CgK3Fin{c.st, 0});
+                const int flex = c.coarse_inv32 != nullptr;
+                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                               CgK3Fin{c.st, flex});
             } else if (c.precond == FCFD_PC_APPROX_INV) {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
                 launch_stencil3<0>(c, nn, act, ApproxInv3aS{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x}, NoFin{});

@@ -103,6 +103,63 @@ static int run2d(fcfd_ctx* ctx, const char* what, F&& f) {
     return run(ctx, std::forward<F>(f));
 }
 
+namespace fcfd {
+
+// Inverse of an SPD matrix A (n x n, row-major, entries beyond half-bandwidth w from the diagonal
+// zero): banded Cholesky A = R^T R in place, then one forward / back substitution per unit
+// vector.  Returns false if A is not numerically SPD.  With `symmetrise`, inv is replaced by
+// (X + X^T) / 2, so that its FP32 rounding is symmetric too.
+bool spd_inverse_banded(std::vector<double>& A, int n, int w, std::vector<double>& inv, bool symmetrise) {
+    auto lo = [&](int i) { return i - w > 0 ? i - w : 0; };
+    for (int j = 0; j < n; ++j) {
+        double d = A[(size_t)j * n + j];
+        for (int k = lo(j); k < j; ++k) d -= A[(size_t)k * n + j] * A[(size_t)k * n + j];
+        if (!(d > 0.0)) return false;
+        d = std::sqrt(d);
+        A[(size_t)j * n + j] = d;
+        const int ie = j + w < n ? j + w + 1 : n;
+        for (int i = j + 1; i < ie; ++i) {
+            double v = A[(size_t)j * n + i];
+            for (int k = lo(i); k < j; ++k) v -= A[(size_t)k * n + j] * A[(size_t)k * n + i];
+            A[(size_t)j * n + i] = v / d;
+        }
+    }
+    inv.assign((size_t)n * n, 0.0);
+    std::vector<double> z(n);
+    for (int c = 0; c < n; ++c) {
+        // R^T z = e_c (z = 0 above c), then R x = z
+        for (int i = 0; i < c; ++i) z[i] = 0.0;
+        for (int i = c; i < n; ++i) {
+            double v = i == c ? 1.0 : 0.0;
+            for (int k = lo(i) > c ? lo(i) : c; k < i; ++k) v -= A[(size_t)k * n + i] * z[k];
+            z[i] = v / A[(size_t)i * n + i];
+        }
+        double* x = &inv[(size_t)c * n];
+        for (int i = n - 1; i >= 0; --i) {
+            double v = z[i];
+            const int ke = i + w < n ? i + w + 1 : n;
+            for (int k = i + 1; k < ke; ++k) v -= A[(size_t)i * n + k] * x[k];
+            x[i] = v / A[(size_t)i * n + i];
+        }
+    }
+    for (int i = 0; i < n && symmetrise; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            const double v = 0.5 * (inv[(size_t)i * n + j] + inv[(size_t)j * n + i]);
+            inv[(size_t)i * n + j] = inv[(size_t)j * n + i] = v;
+        }
+    return true;
+}
+
+// Replace the rows and columns of the pinned nodes by those of the identity.
+void pin_identity(std::vector<double>& A, int n, const std::vector<int64_t>& pins) {
+    for (int64_t p : pins) {
+        for (int k = 0; k < n; ++k) A[(size_t)p * n + k] = A[(size_t)k * n + p] = 0.0;
+        A[(size_t)p * n + p] = 1.0;
+    }
+}
+
+}  // namespace fcfd
+
 namespace {
 
 // L_diag and lumped mass on a level: per node, the sum over its incident elements (in element
@@ -147,44 +204,8 @@ bool coarse_inverse(const Level& L, std::vector<double>& inv) {
         for (int a = 0; a < 4; ++a)
             for (int b = 0; b < 4; ++b) A[(size_t)cn[a] * n + cn[b]] += L.ec.Le[a * 4 + b];
     }
-    for (int64_t p : L.pins) {
-        for (int k = 0; k < n; ++k) A[(size_t)p * n + k] = A[(size_t)k * n + p] = 0.0;
-        A[(size_t)p * n + p] = 1.0;
-    }
-    auto lo = [&](int i) { return i - w > 0 ? i - w : 0; };
-    // A = R^T R (upper R in place of the upper triangle, within the band)
-    for (int j = 0; j < n; ++j) {
-        double d = A[(size_t)j * n + j];
-        for (int k = lo(j); k < j; ++k) d -= A[(size_t)k * n + j] * A[(size_t)k * n + j];
-        if (!(d > 0.0)) return false;
-        d = std::sqrt(d);
-        A[(size_t)j * n + j] = d;
-        const int ie = j + w < n ? j + w + 1 : n;
-        for (int i = j + 1; i < ie; ++i) {
-            double v = A[(size_t)j * n + i];
-            for (int k = lo(i); k < j; ++k) v -= A[(size_t)k * n + j] * A[(size_t)k * n + i];
-            A[(size_t)j * n + i] = v / d;
-        }
-    }
-    inv.assign((size_t)n * n, 0.0);
-    std::vector<double> z(n);
-    for (int c = 0; c < n; ++c) {
-        // R^T z = e_c (z = 0 above c), then R x = z
-        for (int i = 0; i < c; ++i) z[i] = 0.0;
-        for (int i = c; i < n; ++i) {
-            double v = i == c ? 1.0 : 0.0;
-            for (int k = lo(i) > c ? lo(i) : c; k < i; ++k) v -= A[(size_t)k * n + i] * z[k];
-            z[i] = v / A[(size_t)i * n + i];
-        }
-        double* x = &inv[(size_t)c * n];
-        for (int i = n - 1; i >= 0; --i) {
-            double v = z[i];
-            const int ke = i + w < n ? i + w + 1 : n;
-            for (int k = i + 1; k < ke; ++k) v -= A[(size_t)i * n + k] * x[k];
-            x[i] = v / A[(size_t)i * n + i];
-        }
-    }
-    return true;
+    pin_identity(A, n, L.pins);
+    return spd_inverse_banded(A, n, w, inv, false);
 }
 
 void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void* stream) {

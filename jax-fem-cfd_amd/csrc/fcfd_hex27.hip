@@ -808,6 +808,8 @@ static const double* vcycle_h27(Ctx& c, const double* r0, const int* active) {
         Level& C = c.lv[ld];
         if (neu) {
             k_coarse_jacobi<double, double><<<1, 1024, 0, c.stream>>>(C.g3.nn, C.Ld, rl(ld), C.y, c.st, ld, active);
+        } else if (c.coarse_inv32) {
+            coarse_dense64(c, C.g3.nn, C.pin, rl(ld), C.y, active);
         } else {
             const size_t shm = sizeof(double) * (size_t)C.g3.nn;
             k_coarse_cg27<<<1, 1024, shm, c.stream>>>(C.g3, C.eh27_dev, C.pin, rl(ld), C.y, c.cc_r, c.cc_Ap, c.st,
@@ -892,8 +894,9 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
             } else {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
                 const double* y = precondition_h27(c, act);
-                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, 0},
-                               CgK3Fin{c.st, 0});
+                const int flex = c.coarse_inv32 != nullptr;
+                launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
+                               CgK3Fin{c.st, flex});
             }
         }
         FCFD_CHECK_LAUNCH();
@@ -1112,6 +1115,34 @@ void build_ctx_h27(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
                      "3D multigrid: the pinned coarse grid must have at most 8192 nodes");
         c.cc_r = dalloc<double>(c, ncoarse);
         c.cc_Ap = dalloc<double>(c, ncoarse);
+        // mixed form, pinned pressure: exact inverse of the pinned coarse hex27 operator (FP32)
+        if (c.precision == FCFD_PREC_MIXED && c.set_zeroP && nlev > 1 && ncoarse <= COARSE3_DENSE_MAX) {
+            const Grid3D& gc = c.lv[nlev - 1].g3;
+            ElemConstH27 ecc;
+            hex27_constants(c.lx / gc.nex, c.ly / gc.ney, c.lz / gc.nez, ecc);
+            std::vector<double> A((size_t)ncoarse * ncoarse, 0.0), inv;
+            for (int ke = 0; ke < gc.nez; ++ke)
+                for (int je = 0; je < gc.ney; ++je)
+                    for (int ie = 0; ie < gc.nex; ++ie) {
+                        const int n0 = 2 * ke * gc.nxy + 2 * je * gc.nnx + 2 * ie;
+                        for (int a = 0; a < 27; ++a) {
+                            const size_t ra = (size_t)(n0 + h27z(a) * gc.nxy + h27y(a) * gc.nnx + h27x(a)) * ncoarse;
+                            for (int b = 0; b < 27; ++b)
+                                A[ra + n0 + h27z(b) * gc.nxy + h27y(b) * gc.nnx + h27x(b)] += ecc.Le[a * 27 + b];
+                        }
+                    }
+            std::vector<uint8_t> pin(ncoarse);
+            FCFD_HIP(hipMemcpy(pin.data(), c.lv[nlev - 1].pin, ncoarse, hipMemcpyDeviceToHost));
+            std::vector<int64_t> pins;
+            for (int k = 0; k < ncoarse; ++k)
+                if (pin[k]) pins.push_back(k);
+            pin_identity(A, ncoarse, pins);
+            if (spd_inverse_banded(A, ncoarse, 2 * gc.nxy + 2 * gc.nnx + 2, inv, true)) {
+                std::vector<float> inv32(inv.begin(), inv.end());
+                c.coarse_inv32 = dalloc<float>(c, inv32.size());
+                upload27(inv32, c.coarse_inv32);
+            }
+        }
     }
     if (c.set_zeroP || c.precond == FCFD_PC_APPROX_INV) {
         std::vector<uint8_t> pin(nn, 0);

@@ -366,6 +366,14 @@ void dot_to_device(Ctx& c, const double* x, const double* y, int n, double* part
 void axpby(Ctx& c, double a, const double* x, double b, double* y, int n);
 float bench_momentum_apply(Ctx& c, int reps);
 
+// dense coarse solves (fcfd_capi.hip, fcfd_mg.hip): the inverse of an SPD matrix with
+// half-bandwidth w, the pinned rows / columns replaced by the identity, and y = Ainv r (FP32
+// entries, FP64 sums; r zeroed at the pins, y 0 there) on an FP64 level
+bool spd_inverse_banded(std::vector<double>& A, int n, int w, std::vector<double>& inv, bool symmetrise);
+constexpr int COARSE3_DENSE_MAX = 2048;  // 3D coarse levels: at most 16 MB of FP32 inverse
+void pin_identity(std::vector<double>& A, int n, const std::vector<int64_t>& pins);
+void coarse_dense64(Ctx& c, int nn, const uint8_t* pin, const double* r, double* y, const int* active);
+
 // momentum (fcfd_momentum.hip)
 // chunk (non-null, U_hat = c.m_x): only the init and *chunk BiCGSTAB iterations are enqueued
 // (*chunk <= 0: the predicted count), no host read; *chunk returns the iterations enqueued and

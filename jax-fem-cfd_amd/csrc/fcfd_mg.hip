@@ -813,6 +813,12 @@ __global__ __launch_bounds__(256) void k_coarse_dense(int nn, const float* __res
     }
 }
 
+void coarse_dense64(Ctx& c, int nn, const uint8_t* pin, const double* r, double* y, const int* active) {
+    const int nb = (nn + 4 * DENSE_ROWS - 1) / (4 * DENSE_ROWS);
+    k_coarse_dense<double><<<nb, 256, 0, c.stream>>>(nn, c.coarse_inv32, pin, r, y, active);
+    FCFD_CHECK_LAUNCH();
+}
+
 template <class T>
 static LdTab<T> ld_tab(const Level& L) {
     LdTab<T> t;

@@ -193,6 +193,34 @@ void build_mg3(Ctx& c, const fcfd_solver_desc* s) {
                  "3D multigrid: the pinned coarse grid must have at most 8192 nodes");
     c.cc_r = dalloc<double>(c, ncoarse);
     c.cc_Ap = dalloc<double>(c, ncoarse);
+    // mixed form, pinned pressure: the coarse solve is the exact inverse of the pinned coarse
+    // operator, stored FP32 (the FP64 form keeps CG to tol)
+    if (c.precision == FCFD_PREC_MIXED && c.set_zeroP && nlev > 1 && ncoarse <= COARSE3_DENSE_MAX) {
+        const Grid3D& g = c.lv[nlev - 1].g3;
+        ElemConst3 ec;
+        hex8_constants(c.lx / g.nex, c.ly / g.ney, c.lz / g.nez, ec);
+        std::vector<double> A((size_t)ncoarse * ncoarse, 0.0), inv;
+        for (int ke = 0; ke < g.nez; ++ke)
+            for (int je = 0; je < g.ney; ++je)
+                for (int ie = 0; ie < g.nex; ++ie) {
+                    const int n0 = ke * g.nxy + je * g.nnx + ie;
+                    for (int a = 0; a < 8; ++a)
+                        for (int b = 0; b < 8; ++b)
+                            A[(size_t)(n0 + hbz(a) * g.nxy + hby(a) * g.nnx + hbx(a)) * ncoarse + n0 +
+                              hbz(b) * g.nxy + hby(b) * g.nnx + hbx(b)] += ec.Le[a * 8 + b];
+                }
+        std::vector<uint8_t> pin(ncoarse);
+        FCFD_HIP(hipMemcpy(pin.data(), c.lv[nlev - 1].pin, ncoarse, hipMemcpyDeviceToHost));
+        std::vector<int64_t> pins;
+        for (int k = 0; k < ncoarse; ++k)
+            if (pin[k]) pins.push_back(k);
+        pin_identity(A, ncoarse, pins);
+        if (spd_inverse_banded(A, ncoarse, g.nxy + g.nnx + 1, inv, true)) {
+            std::vector<float> inv32(inv.begin(), inv.end());
+            c.coarse_inv32 = dalloc<float>(c, inv32.size());
+            FCFD_HIP(hipMemcpy(c.coarse_inv32, inv32.data(), sizeof(float) * inv32.size(), hipMemcpyHostToDevice));
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------ V-cycle driver
@@ -223,6 +251,8 @@ const double* vcycle3(Ctx& c, const double* r0, const int* active) {
         Level& C = c.lv[ld];
         if (neu) {
             k_coarse_jacobi<double, double><<<1, 1024, 0, c.stream>>>(C.g3.nn, C.Ld, rl(ld), C.y, c.st, ld, active);
+        } else if (c.coarse_inv32) {
+            coarse_dense64(c, C.g3.nn, C.pin, rl(ld), C.y, active);
         } else {
             const size_t shm = sizeof(double) * (size_t)C.g3.nn;
             k_coarse_cg3<<<1, 1024, shm, c.stream>>>(C.g3, C.ec3_dev, C.pin, rl(ld), C.y, c.cc_r, c.cc_Ap, c.st,

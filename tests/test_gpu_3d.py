@@ -224,6 +224,51 @@ def test_channel_multigrid_steps_match_oracle():
         assert eu < 1e-10 and ep < 1e-10
 
 
+def test_vcycle3_mixed_dense_coarse():
+    """Mixed form, pinned: the coarse solve is the exact inverse of the pinned coarse operator
+    (stored FP32).  One V-cycle equals the oracle's (coarse CG to 1e-12) to the FP32 rounding of
+    that inverse (tolerance 1e-6), and far from a V-cycle whose coarse solve is skipped."""
+    from jax_fem_cfd._context import Context
+    n, L, nf = [16, 8, 8], [2.0, 1.0, 1.0], [2, 1, 1]
+    print(f"\n[case] hex8 V-cycle {n} mixed, dense coarse inverse", flush=True)
+    s, *_ = ns.build(n, L, False, [0, 2, 3, 4, 5], [1], 1e-12, True, "multigrid", nf)
+    c = Context(n[0], n[1], L[0], L[1], False, tol=1e-12, set_zeroP=True, precond="multigrid",
+                mg_levels=len(s.levels), gamma_p=s.gamma_p, mg_gamma_p=[Lv["gamma_p"] for Lv in s.levels],
+                nez=n[2], lz=L[2], precision="mixed")
+    r = RNG.standard_normal(s.nn)
+    r[s.gamma_p] = 0
+    ref = s.M_op(r)
+    got = _np(c.vcycle(r))
+    print(f"  rel {_rel(got, ref):.2e}", flush=True)
+    assert _rel(got, ref) < 1e-6
+
+
+def test_channel_mixed_multigrid_steps():
+    """Mixed form with the box-grid V-cycle (FP32 Ce, dense FP32 coarse inverse, flexible PCG):
+    5 channel steps from rest against the FP64 oracle, per field (u, v, w, p) within 1e-6, the
+    momentum iteration counts equal and the pressure counts within one."""
+    n, dt, rho, mu = (16, 8, 8), 2e-3, 1.0, 1e-3
+    print(f"\n[case] hex8 channel {n} multigrid mixed", flush=True)
+    compute, timestep, s, surf, U_d = _channel(n, precond="multigrid", final=[2, 1, 1], precision="mixed")
+    assert timestep.context.ce_format == "f32"
+    U = np.zeros(3 * s.nn)
+    U[s.gamma_d] = U_d
+    P = np.zeros(s.nn)
+    Ug, Pg = U.copy(), P.copy()
+    for k in range(5):
+        F_p, F = compute(U_d, None, 0, 0)
+        Ug, Pg, ui, pi = timestep(Ug, Pg, dt, rho, mu, U_d, F_p, F)
+        oF_p, oF = ns.compute(s, U_d, None, 0, 0)
+        U, P, oui, opi = ns.four_step_timestep(s, U, P, dt, rho, mu, U_d, oF_p, oF)
+        print(f"  step {k}: iters {ui},{pi} oracle {oui},{opi}", flush=True)
+        assert ui == oui and abs(pi - opi) <= 1
+    Ug = _np(Ug)
+    errs = [_rel(Ug[d * s.nn:(d + 1) * s.nn], U[d * s.nn:(d + 1) * s.nn]) for d in range(3)]
+    ep = _rel(_np(Pg), P)
+    print(f"  rel u {errs[0]:.2e} v {errs[1]:.2e} w {errs[2]:.2e} p {ep:.2e}", flush=True)
+    assert max(errs) < 1e-6 and ep < 1e-6
+
+
 def test_lid_box_neumann_multigrid_steps_match_oracle():
     """Closed box, lid u = 1 on the top face, pure-Neumann pressure (no pins) with the box-grid
     V-cycle: 4 steps, U and P (mean-projected) to 1e-10, equal iteration counts."""
