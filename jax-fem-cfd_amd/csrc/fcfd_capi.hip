@@ -592,8 +592,7 @@ StepLaunch four_step_enqueue(Ctx& c, int slot, const double* U_n, const double* 
         GateScope gate(c);
         build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
         solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x, &L.mom, ChunkEnd{rec, 0});
-        projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
-        pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt, &L.p, ChunkEnd{rec, 1});
+        ustar_pressure_solve(c, c.m_x, P_n, F_p, rho, dt, &L.p, ChunkEnd{rec, 1});
         pressure_output(c, P);
         projection_final(c, c.ustar, P, rho, dt, U);
     }
@@ -618,8 +617,7 @@ bool four_step_finish(Ctx& c, const StepLaunch& L, const double* P_n, double dt,
         k_spec_resume<<<1, 64, 0, c.stream>>>(c.st, phase);
         if (phase == 0) {
             it[0] = resume_momentum(c, c.ce_format, c.ce, c.ce_scale, rho, mu, dt, L.mom);
-            projection_ustar(c, c.m_x, P_n, rho, dt, c.ustar);
-            it[1] = pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt);
+            it[1] = ustar_pressure_solve(c, c.m_x, P_n, F_p, rho, dt);
         } else {
             it[0] = mom_it;
             it[1] = pressure_resume(c, L.p);

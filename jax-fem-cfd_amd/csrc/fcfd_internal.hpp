@@ -413,6 +413,10 @@ int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double
                    double dt, double* P_out);
 // solve_pressure in two parts: the solve (solution in c.c_x) and the output pass into P_out
 // chunk: as for pcg; pressure_resume then completes it
+// U* = U_hat + (dt/rho) G P_n / M into c.ustar, then pressure_solve_only on it; the pinned form on a
+// non-periodic grid forms U* inside the PCG's init kernel (same values)
+int ustar_pressure_solve(Ctx& c, const double* U_hat, const double* P_n, const double* F_p, double rho, double dt,
+                         int* chunk = nullptr, ChunkEnd ce = ChunkEnd{});
 int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
                         int* chunk = nullptr, ChunkEnd ce = ChunkEnd{});
 int pressure_resume(Ctx& c, int launched);

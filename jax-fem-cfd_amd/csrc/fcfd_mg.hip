@@ -1399,14 +1399,33 @@ int pcg_resume(Ctx& c, int launched, int maxit, bool* converged) {
     return h[0];
 }
 
-int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
-        bool* converged, bool zero_exit, const PrhsSpec* prhs, int* chunk, ChunkEnd ce) {
+// U_hat, the lumped mass and dt / rho of a projection U* = U_hat + (dt/rho) G P_n / M that the
+// pinned PCG init forms itself (k_cginit_us), storing U* into Us
+struct UstarSpec {
+    const double* Uh; const double* M; double cus; double* Us;
+};
+
+static int pcg_impl(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol,
+                    int maxit, bool* converged, bool zero_exit, const PrhsSpec* prhs, int* chunk, ChunkEnd ce,
+                    const UstarSpec* us) {
     const Grid2D& g = c.g;
     const int nn = g.nn;
     const int neu = (c.precond == FCFD_PC_MULTIGRID) && !c.set_zeroP;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* Le = c.lv[0].ec_dev->Le;
-    if (prhs)
+    if (us) {
+        if (!prhs || !pin || !zero_pins || g.periodic) throw Error(FCFD_ERR_STATE, "fused U* init: pinned form only");
+        constexpr int FC = MOMT_C, FR = MOMT_R;
+        const dim3 grid((g.nnx + FC - 1) / FC, (g.nny + FR - 1) / FR);
+        const int nb = (int)(grid.x * grid.y);
+        if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+        k_cginit_us<FC, FR><<<grid, 256, 0, c.stream>>>(g, prhs->ec, Le, us->Uh, x0in, us->M, prhs->Fp, pin, us->cus,
+                                                        prhs->c, us->Us, c.c_x, c.c_r, c.partials, c.pstride,
+                                                        c.gate);
+        const CgInitFin fin{c.st, tol, maxit, nn, zero_exit};
+        finalize_kernel<3><<<1, FIN_BLOCK, 0, c.stream>>>(nb, fin_gate(c.gate, fin, 0), fin, c.partials,
+                                                          c.pstride);
+    } else if (prhs)
         launch_node<3>(c, nn, c.gate, CgInitD{g, Le, pin, *prhs, x0in, zero_pins, c.c_x, c.c_r},
                        CgInitFin{c.st, tol, maxit, nn, zero_exit});
     else
@@ -1430,6 +1449,11 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
         return 0;
     }
     return pcg_resume(c, 0, maxit, converged);
+}
+
+int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int zero_pins, double tol, int maxit,
+        bool* converged, bool zero_exit, const PrhsSpec* prhs, int* chunk, ChunkEnd ce) {
+    return pcg_impl(c, b, bmean, x0in, zero_pins, tol, maxit, converged, zero_exit, prhs, chunk, ce, nullptr);
 }
 
 int pressure_cg(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged) {
@@ -1476,6 +1500,19 @@ int pressure_solve_only(Ctx& c, const double* U_star, const double* P_n, const d
         it = pcg(c, c.c_rhs, &c.st->mean_rhs, P_n, 0, c.tol, maxit, &conv, false, nullptr, chunk, ce);
     }
     return it;
+}
+
+int ustar_pressure_solve(Ctx& c, const double* U_hat, const double* P_n, const double* F_p, double rho, double dt,
+                         int* chunk, ChunkEnd ce) {
+    if (c.set_zeroP && !c.g.periodic) {
+        const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
+        bool conv;
+        const PrhsSpec ps{c.ec_dev, c.ustar, F_p, rho / dt};
+        const UstarSpec us{U_hat, c.Mlump, dt / rho, c.ustar};
+        return pcg_impl(c, nullptr, nullptr, P_n, 1, c.tol, maxit, &conv, false, &ps, chunk, ce, &us);
+    }
+    projection_ustar(c, U_hat, P_n, rho, dt, c.ustar);
+    return pressure_solve_only(c, c.ustar, P_n, F_p, rho, dt, chunk, ce);
 }
 
 int pressure_resume(Ctx& c, int launched) {

@@ -114,6 +114,21 @@ __device__ __forceinline__ T ebe_has(const Has4& hs, const double* __restrict__ 
     return acc;
 }
 
+// the same with A^T (the divergence uses G_i^T; ebe_const_T's order)
+template <class T>
+__device__ __forceinline__ T ebe_has_T(const Has4& hs, const double* __restrict__ A, const T (&X)[3][3]) {
+    T acc = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int a = kRow(k);
+        T t = T(0);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) t += T(A[b * 4 + a]) * X[kCy(k, b)][kCx(k, b)];
+        acc += hs.h[k] ? t : T(0);
+    }
+    return acc;
+}
+
 // f = r - L x at node (i, j) of level grid g (pinned rows 0) from the x stencil X
 template <class T>
 __device__ __forceinline__ T pinned_residual(const Grid2D& g, int i, int j, const double* Le, T r, bool pinned,
@@ -654,6 +669,107 @@ __global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restri
         }
     }
     reduce_epilogue<1>(acc, partials, pstride, lb);
+}
+
+// ------------------------------------------------------------------------------ U* + PCG init, tiled
+// The projection U* = U_hat + (dt/rho) G P_n / M (UstarBody, standard.py:155) and the pinned PCG
+// init with the pressure rhs formed in it (CgInitD: b = (rho/dt)(D U* - F_p), 0 at the pins;
+// x0 = P_n with the pins zeroed; r = b - L x0; partials b.b, r.r, sum r) in one pass, for a
+// non-periodic grid.  A block owns FC x FR nodes: it loads P_n on the window two wider (G P_n at
+// every node around its own) and U_hat, the lumped mass (one value per node: both components
+// carry the same) and the pins on the window one wider, forms U* there in LDS (stored for its own
+// nodes: the final projection reads it), then the divergence and the Laplacian of its nodes.
+// The operands and order are those of UstarBody and CgInitD.
+template <int FC, int FR>
+__global__ __launch_bounds__(256) void k_cginit_us(Grid2D g, const ElemConst* __restrict__ ec,
+                                                   const double* __restrict__ Le, const double* __restrict__ Uh,
+                                                   const double* __restrict__ Pn, const double* __restrict__ M,
+                                                   const double* __restrict__ Fp, const uint8_t* __restrict__ pin,
+                                                   double cus, double cb, double* __restrict__ Us,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ partials, int pstride,
+                                                   const int* __restrict__ gate) {
+    static_assert(FC * FR == BLOCK, "one node per thread");
+    if (gate && *gate == 0) return;
+    constexpr int PW = FC + 4, PR = FR + 4;  // P_n window: two wider
+    constexpr int UW = FC + 2, UR = FR + 2;  // U_hat / M / pins / U* window: one wider
+    struct UMP {
+        double u0, u1, m;
+        uint8_t p;
+    };
+    constexpr int PP = Win<PR, PW, double>::HP, UP = Win<UR, UW, UMP>::HP;
+    __shared__ double ps[PP][PW];
+    __shared__ double uhs[2][UP][UW];
+    __shared__ double mss[UP][UW];
+    __shared__ uint8_t pns[UP][UW];
+    __shared__ double uss[2][UR][UW];
+    const int nn = g.nn;
+    const int nbx = gridDim.x;
+    const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    const int j0 = (lb / nbx) * FR, i0 = (lb - (lb / nbx) * nbx) * FC;
+    const int bi = threadIdx.x % FC, bj = threadIdx.x / FC;
+    const int i = i0 + bi, j = j0 + bj;
+    const bool own = i < g.nnx && j < g.nny;
+    const int n = own ? j * g.nnx + i : 0;
+    Win<PR, PW, double> wp;
+    wp.load(g, i0 - 2, j0 - 2, [&](int k) { return Pn[k]; });
+    Win<UR, UW, UMP> wu;
+    wu.load(g, i0 - 1, j0 - 1, [&](int k) { return UMP{Uh[k], Uh[k + nn], M[k], pin[k]}; });
+    const double fpo = Fp ? Fp[n] : 0.0;
+    wp.store([&](int a, int b, bool in, double v) { ps[a][b] = in ? v : 0.0; });
+    wu.store([&](int a, int b, bool in, const UMP& v) {
+        uhs[0][a][b] = v.u0;
+        uhs[1][a][b] = v.u1;
+        mss[a][b] = v.m;
+        pns[a][b] = in && v.p != 0;
+    });
+    __syncthreads();
+    for (int t = threadIdx.x; t < UR * UW; t += BLOCK) {
+        const int a = t / UW, b = t - a * UW;
+        const int ui = i0 - 1 + b, uj = j0 - 1 + a;
+        double v0 = 0.0, v1 = 0.0;
+        if (ui >= 0 && ui < g.nnx && uj >= 0 && uj < g.nny) {
+            double X[3][3];
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) X[dy][dx] = ps[a + dy][b + dx];
+            const Has4 hs = has_at(g, ui, uj);
+            const double gx = ebe_has(hs, ec->Gx, X), gy = ebe_has(hs, ec->Gy, X);
+            v0 = uhs[0][a][b] + (cus * gx) / mss[a][b];
+            v1 = uhs[1][a][b] + (cus * gy) / mss[a][b];
+        }
+        uss[0][a][b] = v0;
+        uss[1][a][b] = v1;
+    }
+    __syncthreads();
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (own) {
+        double Xu[3][3], Xv[3][3], X0[3][3];
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                Xu[dy][dx] = uss[0][bj + dy][bi + dx];
+                Xv[dy][dx] = uss[1][bj + dy][bi + dx];
+                X0[dy][dx] = pns[bj + dy][bi + dx] ? 0.0 : ps[bj + 1 + dy][bi + 1 + dx];
+            }
+        Us[n] = Xu[1][1];
+        Us[n + nn] = Xv[1][1];
+        const Has4 hs = has_at(g, i, j);
+        const bool pn = pns[bj + 1][bi + 1] != 0;
+        const double div = ebe_has_T(hs, ec->Gx, Xu) + ebe_has_T(hs, ec->Gy, Xv);
+        double bn = cb * (div - fpo);
+        if (pn) bn = 0.0;
+        const double ax = pn ? 0.0 : ebe_has(hs, Le, X0);
+        const double rn = bn - ax;
+        x[n] = X0[1][1];
+        r[n] = rn;
+        acc[0] += bn * bn;
+        acc[1] += rn * rn;
+        acc[2] += rn;
+    }
+    reduce_epilogue<3>(acc, partials, pstride, lb);
 }
 
 }  // namespace fcfd
