@@ -89,6 +89,56 @@ struct Jacobi3 {
     }
 };
 
+// LDS-staged forms of the pinned DownRes3 / Jacobi3 (stencil3_kernel): the pre-smoothed x, and
+// x + P yc, are formed once per window entry with the node kernels' operands (interp3_at is
+// bitwise interp27), and every node reads its 27 values from LDS.
+struct DownRes3S {
+    static constexpr int NV = 1;
+    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* x; double* f;
+    __device__ __forceinline__ int nnx() const { return g.nnx; }
+    __device__ __forceinline__ int nxy() const { return g.nxy; }
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const {
+        const double xv = (2.0 / 3.0 * r[id]) / Ld[id];
+        v[0] = pin[id] ? 0.0 : xv;
+    }
+    __device__ void operator()(int n, const StageView3<1>& sv, double (&)[1]) const {
+        Nbr3 s;
+        node_elems3(g, n, s);
+        double X[27];
+        view27(sv, 0, n, X);
+        x[n] = X[S3C];
+        const bool pn = pin[n];
+        const double rh = pn ? 0.0 : r[n];
+        const double lx = pn ? 0.0 : ebe3(s, Le, X);
+        f[n] = rh - lx;
+    }
+};
+struct Jacobi3S {
+    static constexpr int NV = 1;
+    Grid3D gf, gc; const double* Leoff; const double* Ld; const uint8_t* pin; const double* r; const double* x;
+    const double* yc; double* y;
+    __device__ __forceinline__ int nnx() const { return gf.nnx; }
+    __device__ __forceinline__ int nxy() const { return gf.nxy; }
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const {
+        const double xv = x[id] + interp3_at(gf, gc, yc, 0.0, id);
+        v[0] = pin[id] ? 0.0 : xv;
+    }
+    __device__ void operator()(int n, const StageView3<1>& sv, double (&)[1]) const {
+        Nbr3 s;
+        node_elems3(gf, n, s);
+        double X[27];
+        view27(sv, 0, n, X);
+        const double off = ebe3(s, Leoff, X);
+        double v = (OMEGA * (r[n] - off)) / Ld[n] + (1.0 - OMEGA) * X[S3C];
+        if (pin[n]) v = 0.0;
+        y[n] = v;
+    }
+};
+template <class Body>
+static void launch_stencil3_nored(Ctx& c, int n, const int* active, const Body& body) {
+    stencil3_kernel<0><<<blocks_for(n), BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
+}
+
 // ------------------------------------------------------------------------------ coarse level
 // Pinned: CG (M = identity) to tol, maxiter 1000, on L with the pinned rows zeroed
 // (multigrid.py:278-287), in one workgroup; the search direction lives in LDS, x / r / Ap in
@@ -242,8 +292,11 @@ const double* vcycle3(Ctx& c, const double* r0, const int* active) {
         if (neu)
             launch_node<1>(c, n, active, DownX<double, double>{rl(l), L.Ld, L.x, c.st, l},
                            MeanFin{&c.st->mg[l][MG_X], nullptr, n});
-        run(n, DownRes3{L.g3, L.ec3_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f, c.st, l, neu},
-            MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+        if (neu)
+            run(n, DownRes3{L.g3, L.ec3_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f, c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_F], nullptr, n});
+        else
+            launch_stencil3_nored(c, n, active, DownRes3S{L.g3, L.ec3_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f});
         run(Lc.g3.nn, Restrict3{L.g3, Lc.g3, L.f, Lc.r, c.st, l, neu},
             MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g3.nn});
     }
@@ -267,8 +320,12 @@ const double* vcycle3(Ctx& c, const double* r0, const int* active) {
         if (neu)
             launch_node<1>(c, n, active, Prolong3{L.g3, Lc.g3, Lc.y, L.f, c.st, l, 1},
                            MeanFin{&c.st->mg[l][MG_P], nullptr, n});
-        run(n, Jacobi3{L.g3, Lc.g3, L.ec3_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, Lc.y, L.y, c.st, l, neu},
-            MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
+        if (neu)
+            run(n, Jacobi3{L.g3, Lc.g3, L.ec3_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, Lc.y, L.y, c.st, l, neu},
+                MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
+        else
+            launch_stencil3_nored(c, n, active,
+                                  Jacobi3S{L.g3, Lc.g3, L.ec3_dev->Leoff, L.Ld, L.pin, rl(l), L.x, Lc.y, L.y});
     }
     FCFD_CHECK_LAUNCH();
     return c.lv[0].y;
