@@ -155,6 +155,7 @@ __device__ __forceinline__ void cand_axis(int i, int ne, int q, int& el, int& lo
 // Where element (ex, ey, ez)'s local node b is read: base(ex, ey, ez) + off(b).  GIdx27: global
 // node ids; TIdx27: ids in a staged tile window (k_tile27).
 struct GIdx27 {
+    static constexpr bool kClass = false;
     int nnx, nxy;
     __device__ __forceinline__ int base(int ex, int ey, int ez) const { return 2 * ez * nxy + 2 * ey * nnx + 2 * ex; }
     __device__ __forceinline__ int off(int b) const { return h27z(b) * nxy + h27y(b) * nnx + h27x(b); }
@@ -169,6 +170,7 @@ constexpr int W27IT = (W27N + BLOCK - 1) / BLOCK, W27P = W27IT * BLOCK;
 static_assert(T27X * T27Y * T27Z == BLOCK && T27X % 2 == 0 && T27Y % 2 == 0 && T27Z % 2 == 0,
               "one node per thread, even tile origins");
 struct TIdx27 {
+    static constexpr bool kClass = false;
     int wi0, wj0, wk0;  // window origin
     __device__ __forceinline__ int base(int ex, int ey, int ez) const {
         return ((2 * ez - wk0) * W27Y + (2 * ey - wj0)) * W27X + (2 * ex - wi0);
@@ -179,6 +181,34 @@ struct TIdx27 {
     }
 };
 
+// Class-uniform staged tiles (k_ctile27): a block owns C27X x C27Y x C27Z nodes with an even
+// origin and one wave per parity class of (i, j, k): the 64 nodes of a class have the same
+// incident-element count and local rows, so the candidate loop does not diverge and the rows of
+// the shared element matrices are uniform loads.  The window's x direction is stored even
+// entries first, then odd ones, so the lanes of a wave (consecutive elements along x) read
+// consecutive LDS words.
+constexpr int C27X = 16, C27Y = 8, C27Z = 4, C27T = 512;
+constexpr int V27X = C27X + 3, V27Y = C27Y + 3, V27Z = C27Z + 3, V27N = V27X * V27Y * V27Z;
+constexpr int V27E = (V27X + 1) / 2;  // even-x entries of a window row
+constexpr int V27IT = (V27N + C27T - 1) / C27T, V27P = V27IT * C27T;
+static_assert(C27X * C27Y * C27Z == C27T && C27T == 8 * 64 && C27X == 16 && C27Y == 8 && C27Z == 4,
+              "eight classes of 8 x 4 x 2 nodes, one wave each");
+struct CIdx27 {
+    static constexpr bool kClass = true;
+    int wi0, wj0, wk0;  // window origin (even)
+    int px, py, pz;     // the parities of the wave's nodes (uniform)
+    __device__ __forceinline__ int base(int ex, int ey, int ez) const {
+        return ((2 * ez - wk0) * V27Y + (2 * ey - wj0)) * V27X + (ex - (wi0 >> 1));
+    }
+    __device__ __forceinline__ int off(int b) const {
+        return (h27z(b) * V27Y + h27y(b)) * V27X + (h27x(b) == 1 ? V27E : (h27x(b) >> 1));
+    }
+    __device__ __forceinline__ static int pos(int wx, int wy, int wz) {
+        return (wz * V27Y + wy) * V27X + ((wx & 1) ? V27E + (wx >> 1) : (wx >> 1));
+    }
+    __device__ __forceinline__ int at(int i, int j, int k) const { return pos(i - wi0, j - wj0, k - wk0); }
+};
+
 // sum_q has_q sum_b A(row_q, b, e_q) X(node_b of e_q) for NC components; A from rowf(a, e, cf),
 // X from val(c, ix.base(e) + ix.off(b))
 template <int NC, class RowF, class ValF, class Idx>
@@ -186,6 +216,42 @@ __device__ __forceinline__ void h27_apply_ix(const Grid3D& g, const Inc27& s, Ro
                                              const Idx& ix) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) y[c] = 0.0;
+    if constexpr (Idx::kClass) {
+        // the candidates of the full loop below that exist for the wave's parities, in its order
+        // (an odd index has only q = 0 along its axis); counts and local rows are uniform
+        const int nqx = ix.px ? 1 : 2, nqy = ix.py ? 1 : 2, nqz = ix.pz ? 1 : 2;
+#pragma unroll 1
+        for (int qz = 0; qz < nqz; ++qz)
+#pragma unroll 1
+            for (int qy = 0; qy < nqy; ++qy)
+#pragma unroll 1
+                for (int qx = 0; qx < nqx; ++qx) {
+                    int ex, cx, ey, cy, ez, cz;
+                    bool vx, vy, vz;
+                    cand_axis(s.i, g.nex, qx, ex, cx, vx);
+                    cand_axis(s.j, g.ney, qy, ey, cy, vy);
+                    cand_axis(s.k, g.nez, qz, ez, cz, vz);
+                    const int a = ((ix.pz ? 1 : (qz ? 0 : 2)) * 3 + (ix.py ? 1 : (qy ? 0 : 2))) * 3 +
+                                  (ix.px ? 1 : (qx ? 0 : 2));
+                    const bool v = vx && vy && vz;
+                    const int e = v ? (ez * g.ney + ey) * g.nex + ex : 0;
+                    const int base = v ? ix.base(ex, ey, ez) : 0;
+                    double cf[27];
+                    rowf(a, e, cf);
+                    double t[NC];
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) t[c] = 0.0;
+#pragma unroll
+                    for (int b = 0; b < 27; ++b) {
+                        const int id = base + ix.off(b);
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) t[c] += cf[b] * val(c, id);
+                    }
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) y[c] += v ? t[c] : 0.0;
+                }
+        return;
+    }
 #pragma unroll 1
     for (int q = 0; q < 8; ++q) {
         int ex, cx, ey, cy, ez, cz;
@@ -274,6 +340,58 @@ __global__ __launch_bounds__(256) void k_tile27(Grid3D g, int ntx, int nty, cons
     const int i = i0 + t % T27X, j = j0 + (t / T27X) % T27Y, k = k0 + t / (T27X * T27Y);
     if (i < g.nnx && j < g.nny && k < g.nnz) body((k * g.nny + j) * g.nnx + i, Inc27{i, j, k}, ix, sm, acc);
     if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
+}
+
+// The class-uniform staged tile kernel (CIdx27): the same staging and bodies as k_tile27, 512
+// threads, wave w = the nodes (i0 + 2a + px, j0 + 2b + py, k0 + 2d + pz) of class w = px + 2 py + 4 pz,
+// lane = a + 8 b + 32 d.
+template <int NR, class Body>
+__global__ __launch_bounds__(C27T) void k_ctile27(Grid3D g, int ntx, int nty, const int* __restrict__ active,
+                                                  Body body, double* __restrict__ partials, int pstride) {
+    constexpr int NV = Body::NV;
+    if (active && *active == 0) return;
+    __shared__ double sm[NV][V27P];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int tz = lb / (ntx * nty), rem = lb - tz * (ntx * nty), ty = rem / ntx, tx = rem - ty * ntx;
+    const int i0 = tx * C27X, j0 = ty * C27Y, k0 = tz * C27Z;
+    const int cls = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const CIdx27 ix{i0 - 2, j0 - 2, k0 - 2, cls & 1, (cls >> 1) & 1, cls >> 2};
+    double v[V27IT][NV];
+    bool in[V27IT];
+    int at[V27IT];
+#pragma unroll
+    for (int it = 0; it < V27IT; ++it) {
+        const int w = threadIdx.x + it * C27T;
+        const int wz = w / (V27X * V27Y), wr = w - wz * (V27X * V27Y), wy = wr / V27X, wx = wr - wy * V27X;
+        const int i = ix.wi0 + wx, j = ix.wj0 + wy, k = ix.wk0 + wz;
+        const bool ok = w < V27N && i >= 0 && i < g.nnx && j >= 0 && j < g.nny && k >= 0 && k < g.nnz;
+        in[it] = ok;
+        at[it] = w < V27N ? CIdx27::pos(wx, wy, wz) : w;
+        body.stage(ok ? (k * g.nny + j) * g.nnx + i : 0, v[it]);
+    }
+#pragma unroll
+    for (int it = 0; it < V27IT; ++it)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) sm[c][at[it]] = in[it] ? v[it][c] : 0.0;
+    __syncthreads();
+    double acc[NR > 0 ? NR : 1];
+#pragma unroll
+    for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
+    const int lane = threadIdx.x & 63;
+    const int i = i0 + 2 * (lane & 7) + ix.px, j = j0 + 2 * ((lane >> 3) & 3) + ix.py,
+              k = k0 + 2 * (lane >> 5) + ix.pz;
+    if (i < g.nnx && j < g.nny && k < g.nnz) body((k * g.nny + j) * g.nnx + i, Inc27{i, j, k}, ix, sm, acc);
+    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
+}
+
+template <int NR, class Body, class Fin>
+static void launch_ctile27(Ctx& c, const Grid3D& g, const int* active, const Body& body, const Fin& fin) {
+    const int ntx = (g.nnx + C27X - 1) / C27X, nty = (g.nny + C27Y - 1) / C27Y, ntz = (g.nnz + C27Z - 1) / C27Z;
+    const int nb = ntx * nty * ntz;
+    if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+    k_ctile27<NR><<<nb, C27T, 0, c.stream>>>(g, ntx, nty, active, body, c.partials, c.pstride);
+    if constexpr (NR > 0)
+        finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, fin_gate(active, fin, 0), fin, c.partials, c.pstride);
 }
 
 // Launch a staged tile kernel over grid g and, when it reduces, its finalize kernel.
@@ -396,8 +514,8 @@ struct BInit27 {
         }
         v[6] = Pn[id];
     }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[2]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[2]) const {
         const Grid3D& g = lhs.g;
         const int nn = g.nn;
         double b[3], ax[3];
@@ -444,8 +562,8 @@ struct BK1_27 {
             v[c] = r[i] + beta * (pold[i] - omega * qold[i]);
         }
     }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
         const Grid3D& g = op.g;
         const int nn = g.nn;
         double y[3];
@@ -479,8 +597,8 @@ struct BK2_27 {
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[c] = r[id + c * nn] - alpha * q[id + c * nn];
     }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[3]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[3]) const {
         const Grid3D& g = op.g;
         const int nn = g.nn;
         double y[3];
@@ -519,10 +637,10 @@ static int solve_momentum_h27_t(Ctx& c, S ce, const double* U_n, const double* P
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_tile27<1>(c, g, act,
+            launch_ctile27<1>(c, g, act,
                              BK1_27<Op>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
                              BK1Fin{c.st});
-            launch_tile27<3>(c, g, act, BK2_27<Op>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+            launch_ctile27<3>(c, g, act, BK2_27<Op>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
                              BK2Fin{c.st});
             launch_node<2>(c, nn, act, BK3<3>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
                            BK3Fin{c.st});
@@ -565,8 +683,8 @@ struct Proj27Body {
     double* Uout;
     double c;
     __device__ void stage(int id, double (&v)[NV]) const { v[0] = P[id]; }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&)[1]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&)[1]) const {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double gd = h27_const_ix(g, s, ec->G[d], [&](int id) { return sm[0][id]; }, ix);
@@ -586,8 +704,8 @@ struct Prhs27Body {
 #pragma unroll
         for (int d = 0; d < 3; ++d) v[d] = Us[id + (size_t)d * g.nn];
     }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
         double div = 0.0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) div += h27_const_ix<true>(g, s, ec->G[d], [&](int id) { return sm[d][id]; }, ix);
@@ -606,8 +724,8 @@ struct CgInit27 {
         const double xv = x0in[id];
         v[0] = (zero_pins && pin && pin[id]) ? 0.0 : xv;
     }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[3]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[3]) const {
         const bool pn = pin && pin[n];
         const double ax = pn ? 0.0 : h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix);
         const double bn = b[n] - (bmean ? *bmean : 0.0);
@@ -625,8 +743,8 @@ struct CgK1_27 {
     Grid3D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
     const DevState* st;
     __device__ void stage(int id, double (&v)[NV]) const { v[0] = z[id] + st->c_beta * pold[id]; }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
         const double a =
             (pin && pin[n]) ? 0.0 : h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix);
         const double pn = sm[0][ix.at(s.i, s.j, s.k)];
@@ -641,8 +759,8 @@ struct ApproxInv27a {
     static constexpr int NV = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
     __device__ void stage(int id, double (&v)[NV]) const { v[0] = r[id] / Ld[id]; }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&)[1]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&)[1]) const {
         const double v = h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix) / Ld[n];
         d1l[n] = (pin && pin[n]) ? 0.0 : v;
     }
@@ -651,8 +769,8 @@ struct ApproxInv27b {
     static constexpr int NV = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l; double* z;
     __device__ void stage(int id, double (&v)[NV]) const { v[0] = d1l[id]; }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&)[1]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&)[1]) const {
         double l2 = h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix);
         if (pin && pin[n]) l2 = 0.0;
         const double d1x = r[n] / Ld[n];
@@ -682,8 +800,8 @@ struct DownRes27 {
             v[0] = (pin && pin[id]) ? 0.0 : xv;
         }
     }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
         auto xv = [&](int id) { return sm[0][id]; };
         double fv;
         if (neumann) {
@@ -716,8 +834,8 @@ struct Jacobi27 {
             v[0] = (pin && pin[id]) ? 0.0 : xv;
         }
     }
-    template <class SM>
-    __device__ void operator()(int n, const Inc27& s, const TIdx27& ix, SM sm, double (&acc)[1]) const {
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
         auto xh = [&](int id) { return sm[0][id]; };
         const double rh = neumann ? r[n] - (l == 0 ? 0.0 : st->mg[l][MG_R]) : r[n];
         const double off = h27_const_ix(g, s, Leoff, xh, ix);
@@ -788,8 +906,8 @@ static const double* vcycle_h27(Ctx& c, const double* r0, const int* active) {
         else launch_node<0>(c, n, active, body, fin);
     };
     auto run_tile = [&](const Grid3D& g, const auto& body, const MeanFin& fin) {
-        if (neu) launch_tile27<1>(c, g, active, body, fin);
-        else launch_tile27<0>(c, g, active, body, fin);
+        if (neu) launch_ctile27<1>(c, g, active, body, fin);
+        else launch_ctile27<0>(c, g, active, body, fin);
     };
     auto rl = [&](int l) -> const double* { return l == 0 ? r0 : c.lv[l].r; };
     for (int l = 0; l < ld; ++l) {
@@ -856,9 +974,9 @@ static const double* precondition_h27(Ctx& c, const int* active) {
             launch_node<0>(c, nn, active, JacobiPrec{c.c_r, c.lv[0].Ld, c.lv[0].y}, NoFin{});
             return c.lv[0].y;
         case FCFD_PC_APPROX_INV:
-            launch_tile27<0>(c, c.g3, active, ApproxInv27a{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x},
+            launch_ctile27<0>(c, c.g3, active, ApproxInv27a{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x},
                              NoFin{});
-            launch_tile27<0>(c, c.g3, active,
+            launch_ctile27<0>(c, c.g3, active,
                              ApproxInv27b{c.g3, Le, c.c_r, c.lv[0].Ld, c.ppin, c.lv[0].x, c.lv[0].y}, NoFin{});
             return c.lv[0].y;
         case FCFD_PC_MULTIGRID: return vcycle_h27(c, c.c_r, active);
@@ -873,7 +991,7 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
     const int nn = g.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
     const double* Le = c.eh27_dev->Le;
-    launch_tile27<3>(c, g, nullptr, CgInit27{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
+    launch_ctile27<3>(c, g, nullptr, CgInit27{g, Le, pin, b, bmean, x0in, zero_pins, c.c_x, c.c_r},
                      CgInitFin{c.st, tol, maxit, nn, 0});
     const int* act = &c.st->c_active;
     const int neu = c.precond == FCFD_PC_MULTIGRID && !c.set_zeroP;
@@ -884,7 +1002,7 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_tile27<1>(c, g, act, CgK1_27{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+            launch_ctile27<1>(c, g, act, CgK1_27{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
                              CgK1Fin{c.st});
             if (c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_NONE) {
                 launch_node<2>(c, nn, act,
@@ -927,7 +1045,7 @@ int solve_pressure_h27(Ctx& c, const double* U_star, const double* P_n, const do
                        double* P_out) {
     const int nn = c.g3.nn;
     const uint8_t* pin = c.set_zeroP ? c.ppin : nullptr;
-    launch_tile27<1>(c, c.g3, nullptr, Prhs27Body{c.g3, c.eh27_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
+    launch_ctile27<1>(c, c.g3, nullptr, Prhs27Body{c.g3, c.eh27_dev, U_star, F_p, pin, c.c_rhs, rho / dt},
                      MeanFin{&c.st->mean_rhs, nullptr, nn});
     const int maxit = c.maxit_p > 0 ? c.maxit_p : 1000;
     bool conv;
@@ -1253,11 +1371,11 @@ void four_step_h27(Ctx& c, const double* U_n, const double* P_n, double dt, doub
                    const double* F_p, const double* F, double* U, double* P, int* it) {
     scatter_ud(c, U_d);
     it[0] = solve_momentum_h27(c, U_n, P_n, rho, mu, dt, F, c.m_x);
-    launch_tile27<0>(c, c.g3, nullptr,
+    launch_ctile27<0>(c, c.g3, nullptr,
                      Proj27Body{c.g3, c.eh27_dev, c.m_x, P_n, c.Mlump, nullptr, nullptr, c.ustar, dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
     it[1] = solve_pressure_h27(c, c.ustar, P_n, F_p, rho, dt, P);
-    launch_tile27<0>(c, c.g3, nullptr,
+    launch_ctile27<0>(c, c.g3, nullptr,
                      Proj27Body{c.g3, c.eh27_dev, c.ustar, P, c.Mlump, c.ud_full, c.dmask, U, -dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
 }
