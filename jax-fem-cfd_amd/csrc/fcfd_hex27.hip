@@ -30,25 +30,43 @@ __host__ __device__ constexpr int h27x(int b) { return b % 3; }
 __host__ __device__ constexpr int h27y(int b) { return (b / 3) % 3; }
 __host__ __device__ constexpr int h27z(int b) { return b / 9; }
 
-// Row-blocked 27 x 27 storage: entry (a, b) of element e at ((a * ne + e) * 27 + b)
+// Row-blocked 27 x 27 storage with rows padded to 28 entries: entry (a, b) of element e at
+// ((a * ne + e) * 28 + b).  Every row starts on a 16-byte boundary, so a node reads its row of an
+// element with 7 float4 (FP32) or 14 double2 (FP64) loads instead of 27 scalar ones.
+constexpr int H27_ROW = 28;
 __host__ __device__ __forceinline__ size_t h27_idx(int k, int e, int ne) {
-    return ((size_t)(k / 27) * ne + e) * 27 + k % 27;
+    return ((size_t)(k / 27) * ne + e) * H27_ROW + k % 27;
 }
 struct Ce27F64 {
     double* c; int ne;
     __device__ __forceinline__ void ld_row(int a, int e, double (&v)[27]) const {
-        const double* p = c + h27_idx(a * 27, e, ne);
+        const double2* p = reinterpret_cast<const double2*>(c + h27_idx(a * 27, e, ne));
 #pragma unroll
-        for (int b = 0; b < 27; ++b) v[b] = p[b];
+        for (int q = 0; q < 13; ++q) {
+            const double2 t = p[q];
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+        v[26] = reinterpret_cast<const double*>(p)[26];
     }
     __device__ __forceinline__ void st(int k, int e, double v) const { c[h27_idx(k, e, ne)] = v; }
 };
 struct Ce27F32 {
     float* c; int ne;
     __device__ __forceinline__ void ld_row(int a, int e, double (&v)[27]) const {
-        const float* p = c + h27_idx(a * 27, e, ne);
+        const float4* p = reinterpret_cast<const float4*>(c + h27_idx(a * 27, e, ne));
 #pragma unroll
-        for (int b = 0; b < 27; ++b) v[b] = (double)p[b];
+        for (int q = 0; q < 6; ++q) {
+            const float4 t = p[q];
+            v[4 * q] = (double)t.x;
+            v[4 * q + 1] = (double)t.y;
+            v[4 * q + 2] = (double)t.z;
+            v[4 * q + 3] = (double)t.w;
+        }
+        const float4 t = p[6];
+        v[24] = (double)t.x;
+        v[25] = (double)t.y;
+        v[26] = (double)t.z;
     }
     __device__ __forceinline__ void st(int k, int e, double v) const { c[h27_idx(k, e, ne)] = (float)v; }
 };
@@ -153,32 +171,13 @@ __device__ __forceinline__ void cand_axis(int i, int ne, int q, int& el, int& lo
 }
 
 // Where element (ex, ey, ez)'s local node b is read: base(ex, ey, ez) + off(b).  GIdx27: global
-// node ids; TIdx27: ids in a staged tile window (k_tile27).
+// node ids; CIdx27: ids in a staged tile window (k_ctile27, k_btile27); RIdx27: the element
+// products already formed (k_btile27).
 struct GIdx27 {
-    static constexpr bool kClass = false;
+    static constexpr bool kClass = false, kRes = false;
     int nnx, nxy;
     __device__ __forceinline__ int base(int ex, int ey, int ez) const { return 2 * ez * nxy + 2 * ey * nnx + 2 * ex; }
     __device__ __forceinline__ int off(int b) const { return h27z(b) * nxy + h27y(b) * nnx + h27x(b); }
-};
-
-// Staged tiles: a block owns T27X x T27Y x T27Z nodes with an even origin in every direction, so
-// every element incident to them lies in the window of (T27X + 3) x (T27Y + 3) x (T27Z + 3) nodes
-// from the origin - 2 (an even node reaches two nodes either way, an odd one one).
-constexpr int T27X = 16, T27Y = 4, T27Z = 4;
-constexpr int W27X = T27X + 3, W27Y = T27Y + 3, W27Z = T27Z + 3, W27N = W27X * W27Y * W27Z;
-constexpr int W27IT = (W27N + BLOCK - 1) / BLOCK, W27P = W27IT * BLOCK;
-static_assert(T27X * T27Y * T27Z == BLOCK && T27X % 2 == 0 && T27Y % 2 == 0 && T27Z % 2 == 0,
-              "one node per thread, even tile origins");
-struct TIdx27 {
-    static constexpr bool kClass = false;
-    int wi0, wj0, wk0;  // window origin
-    __device__ __forceinline__ int base(int ex, int ey, int ez) const {
-        return ((2 * ez - wk0) * W27Y + (2 * ey - wj0)) * W27X + (2 * ex - wi0);
-    }
-    __device__ __forceinline__ int off(int b) const { return (h27z(b) * W27Y + h27y(b)) * W27X + h27x(b); }
-    __device__ __forceinline__ int at(int i, int j, int k) const {
-        return ((k - wk0) * W27Y + (j - wj0)) * W27X + (i - wi0);
-    }
 };
 
 // Class-uniform staged tiles (k_ctile27): a block owns C27X x C27Y x C27Z nodes with an even
@@ -194,7 +193,7 @@ constexpr int V27IT = (V27N + C27T - 1) / C27T, V27P = V27IT * C27T;
 static_assert(C27X * C27Y * C27Z == C27T && C27T == 8 * 64 && C27X == 16 && C27Y == 8 && C27Z == 4,
               "eight classes of 8 x 4 x 2 nodes, one wave each");
 struct CIdx27 {
-    static constexpr bool kClass = true;
+    static constexpr bool kClass = true, kRes = false;
     int wi0, wj0, wk0;  // window origin (even)
     int px, py, pz;     // the parities of the wave's nodes (uniform)
     __device__ __forceinline__ int base(int ex, int ey, int ez) const {
@@ -209,6 +208,43 @@ struct CIdx27 {
     __device__ __forceinline__ int at(int i, int j, int k) const { return pos(i - wi0, j - wj0, k - wk0); }
 };
 
+// Balanced class tiles (k_btile27): the 27 (class, candidate) pairs of a 16 x 8 x 4 tile -- class
+// c = px + 2 py + 4 pz has (2 - px)(2 - py)(2 - pz) incident-element candidates -- in the order of
+// the candidate loop, classes ascending.  A wave forms the element products of pairs w, w + 8,
+// w + 16, w + 24 for the 64 nodes of the pair's class; then the wave of class c sums its pairs in
+// order.  RIdx27 hands those sums to the bodies.
+struct Combo27 {
+    int8_t cls[27], qx[27], qy[27], qz[27];
+    int8_t m0[8], nm[8];
+};
+__host__ __device__ constexpr Combo27 make_combo27() {
+    Combo27 t{};
+    int m = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int px = c & 1, py = (c >> 1) & 1, pz = c >> 2;
+        t.m0[c] = (int8_t)m;
+        for (int qz = 0; qz < 2 - pz; ++qz)
+            for (int qy = 0; qy < 2 - py; ++qy)
+                for (int qx = 0; qx < 2 - px; ++qx) {
+                    t.cls[m] = (int8_t)c;
+                    t.qx[m] = (int8_t)qx;
+                    t.qy[m] = (int8_t)qy;
+                    t.qz[m] = (int8_t)qz;
+                    ++m;
+                }
+        t.nm[c] = (int8_t)(m - t.m0[c]);
+    }
+    return t;
+}
+__device__ constexpr Combo27 kCombo27 = make_combo27();
+struct RIdx27 {
+    static constexpr bool kClass = false, kRes = true;
+    CIdx27 w;             // the window of the block (own values)
+    const double* res;    // res[(c * 27 + m) * 64 + lane]: the element products of pair m, component c
+    int m0, nm, lane;     // the pairs of the node's class
+    __device__ __forceinline__ int at(int i, int j, int k) const { return w.at(i, j, k); }
+};
+
 // sum_q has_q sum_b A(row_q, b, e_q) X(node_b of e_q) for NC components; A from rowf(a, e, cf),
 // X from val(c, ix.base(e) + ix.off(b))
 template <int NC, class RowF, class ValF, class Idx>
@@ -216,7 +252,11 @@ __device__ __forceinline__ void h27_apply_ix(const Grid3D& g, const Inc27& s, Ro
                                              const Idx& ix) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) y[c] = 0.0;
-    if constexpr (Idx::kClass) {
+    if constexpr (Idx::kRes) {
+        for (int m = ix.m0; m < ix.m0 + ix.nm; ++m)
+#pragma unroll
+            for (int c = 0; c < NC; ++c) y[c] += ix.res[(c * 27 + m) * 64 + ix.lane];
+    } else if constexpr (Idx::kClass) {
         // the candidates of the full loop below that exist for the wave's parities, in its order
         // (an odd index has only q = 0 along its axis); counts and local rows are uniform
         const int nqx = ix.px ? 1 : 2, nqy = ix.py ? 1 : 2, nqz = ix.pz ? 1 : 2;
@@ -250,34 +290,35 @@ __device__ __forceinline__ void h27_apply_ix(const Grid3D& g, const Inc27& s, Ro
 #pragma unroll
                     for (int c = 0; c < NC; ++c) y[c] += v ? t[c] : 0.0;
                 }
-        return;
-    }
+    } else {
 #pragma unroll 1
-    for (int q = 0; q < 8; ++q) {
-        int ex, cx, ey, cy, ez, cz;
-        bool vx, vy, vz;
-        cand_axis(s.i, g.nex, q & 1, ex, cx, vx);
-        cand_axis(s.j, g.ney, (q >> 1) & 1, ey, cy, vy);
-        cand_axis(s.k, g.nez, q >> 2, ez, cz, vz);
-        if (vx && vy && vz) {
-            const int e = (ez * g.ney + ey) * g.nex + ex;
-            const int base = ix.base(ex, ey, ez);
-            double cf[27];
-            rowf((cz * 3 + cy) * 3 + cx, e, cf);
-            double t[NC];
+        for (int q = 0; q < 8; ++q) {
+            int ex, cx, ey, cy, ez, cz;
+            bool vx, vy, vz;
+            cand_axis(s.i, g.nex, q & 1, ex, cx, vx);
+            cand_axis(s.j, g.ney, (q >> 1) & 1, ey, cy, vy);
+            cand_axis(s.k, g.nez, q >> 2, ez, cz, vz);
+            if (vx && vy && vz) {
+                const int e = (ez * g.ney + ey) * g.nex + ex;
+                const int base = ix.base(ex, ey, ez);
+                double cf[27];
+                rowf((cz * 3 + cy) * 3 + cx, e, cf);
+                double t[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) t[c] = 0.0;
+                for (int c = 0; c < NC; ++c) t[c] = 0.0;
 #pragma unroll
-            for (int b = 0; b < 27; ++b) {
-                const int id = base + ix.off(b);
+                for (int b = 0; b < 27; ++b) {
+                    const int id = base + ix.off(b);
 #pragma unroll
-                for (int c = 0; c < NC; ++c) t[c] += cf[b] * val(c, id);
+                    for (int c = 0; c < NC; ++c) t[c] += cf[b] * val(c, id);
+                }
+#pragma unroll
+                for (int c = 0; c < NC; ++c) y[c] += t[c];
             }
-#pragma unroll
-            for (int c = 0; c < NC; ++c) y[c] += t[c];
         }
     }
 }
+
 template <int NC, class RowF, class ValF>
 __device__ __forceinline__ void h27_apply(const Grid3D& g, const Inc27& s, RowF rowf, ValF val, double (&y)[NC]) {
     h27_apply_ix<NC>(g, s, rowf, val, y, GIdx27{g.nnx, g.nxy});
@@ -301,50 +342,13 @@ __device__ __forceinline__ double h27_const(const Grid3D& g, const Inc27& s, con
     return h27_const_ix<TR>(g, s, A, val, GIdx27{g.nnx, g.nxy});
 }
 
-// The staged tile kernel.  Body::stage(id, v) forms the Body::NV values of node id that the
-// element products read (e.g. p = r + beta (p_old - omega q_old)), once per window node; every
-// load of the block is issued before the first LDS store.  Window nodes outside the grid hold 0
-// (no existing element reads them).  The body then runs per owned node with the window in LDS
-// (sm[v][ix.base(e) + ix.off(b)]), in the operand order of the node-kernel form, so the results
-// are bitwise those of direct gathers.
-template <int NR, class Body>
-__global__ __launch_bounds__(256) void k_tile27(Grid3D g, int ntx, int nty, const int* __restrict__ active, Body body,
-                                                double* __restrict__ partials, int pstride) {
-    constexpr int NV = Body::NV;
-    if (active && *active == 0) return;
-    __shared__ double sm[NV][W27P];
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int tz = lb / (ntx * nty), rem = lb - tz * (ntx * nty), ty = rem / ntx, tx = rem - ty * ntx;
-    const int i0 = tx * T27X, j0 = ty * T27Y, k0 = tz * T27Z;
-    const TIdx27 ix{i0 - 2, j0 - 2, k0 - 2};
-    double v[W27IT][NV];
-    bool in[W27IT];
-#pragma unroll
-    for (int it = 0; it < W27IT; ++it) {
-        const int w = threadIdx.x + it * BLOCK;
-        const int wz = w / (W27X * W27Y), wr = w - wz * (W27X * W27Y), wy = wr / W27X, wx = wr - wy * W27X;
-        const int i = ix.wi0 + wx, j = ix.wj0 + wy, k = ix.wk0 + wz;
-        const bool ok = w < W27N && i >= 0 && i < g.nnx && j >= 0 && j < g.nny && k >= 0 && k < g.nnz;
-        in[it] = ok;
-        body.stage(ok ? (k * g.nny + j) * g.nnx + i : 0, v[it]);
-    }
-#pragma unroll
-    for (int it = 0; it < W27IT; ++it)
-#pragma unroll
-        for (int c = 0; c < NV; ++c) sm[c][threadIdx.x + it * BLOCK] = in[it] ? v[it][c] : 0.0;
-    __syncthreads();
-    double acc[NR > 0 ? NR : 1];
-#pragma unroll
-    for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
-    const int t = threadIdx.x;
-    const int i = i0 + t % T27X, j = j0 + (t / T27X) % T27Y, k = k0 + t / (T27X * T27Y);
-    if (i < g.nnx && j < g.nny && k < g.nnz) body((k * g.nny + j) * g.nnx + i, Inc27{i, j, k}, ix, sm, acc);
-    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
-}
-
-// The class-uniform staged tile kernel (CIdx27): the same staging and bodies as k_tile27, 512
-// threads, wave w = the nodes (i0 + 2a + px, j0 + 2b + py, k0 + 2d + pz) of class w = px + 2 py + 4 pz,
-// lane = a + 8 b + 32 d.
+// The class-uniform staged tile kernel (CIdx27), 512 threads.  Body::stage(id, v) forms the
+// Body::NV values of node id that the element products read (e.g. p = r + beta (p_old - omega
+// q_old)), once per window node; every load of the block is issued before the first LDS store,
+// and window nodes outside the grid hold 0 (no existing element reads them).  The body then runs
+// per owned node with the window in LDS, in the operand order of a direct gather (bitwise its
+// results).  Wave w = the nodes (i0 + 2a + px, j0 + 2b + py, k0 + 2d + pz) of class
+// w = px + 2 py + 4 pz, lane = a + 8 b + 32 d.
 template <int NR, class Body>
 __global__ __launch_bounds__(C27T) void k_ctile27(Grid3D g, int ntx, int nty, const int* __restrict__ active,
                                                   Body body, double* __restrict__ partials, int pstride) {
@@ -394,16 +398,96 @@ static void launch_ctile27(Ctx& c, const Grid3D& g, const int* active, const Bod
         finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, fin_gate(active, fin, 0), fin, c.partials, c.pstride);
 }
 
-// Launch a staged tile kernel over grid g and, when it reduces, its finalize kernel.
+// The balanced class tile kernel: k_ctile27's tile, window and staging; the element products of
+// the tile's 27 (class, candidate) pairs spread over the 8 waves (3-4 pairs each instead of 1-8),
+// kept in LDS and summed per node in candidate order (RIdx27), so bitwise the k_ctile27 results.
+// Bodies with one element product of NC components read from the first NC staged vectors.
+template <int NR, class Body>
+__global__ __launch_bounds__(C27T) void k_btile27(Grid3D g, int ntx, int nty, const int* __restrict__ active,
+                                                  Body body, double* __restrict__ partials, int pstride) {
+    constexpr int NV = Body::NV, NC = Body::NC;
+    static_assert(NC <= NV, "the products read the first NC staged vectors");
+    if (active && *active == 0) return;
+    __shared__ double sm[NV][V27P];
+    __shared__ double res[NC * 27 * 64];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int tz = lb / (ntx * nty), rem = lb - tz * (ntx * nty), ty = rem / ntx, tx = rem - ty * ntx;
+    const int i0 = tx * C27X, j0 = ty * C27Y, k0 = tz * C27Z;
+    const CIdx27 ix{i0 - 2, j0 - 2, k0 - 2, 0, 0, 0};
+    double v[V27IT][NV];
+    bool in[V27IT];
+    int at[V27IT];
+#pragma unroll
+    for (int it = 0; it < V27IT; ++it) {
+        const int w = threadIdx.x + it * C27T;
+        const int wz = w / (V27X * V27Y), wr = w - wz * (V27X * V27Y), wy = wr / V27X, wx = wr - wy * V27X;
+        const int i = ix.wi0 + wx, j = ix.wj0 + wy, k = ix.wk0 + wz;
+        const bool ok = w < V27N && i >= 0 && i < g.nnx && j >= 0 && j < g.nny && k >= 0 && k < g.nnz;
+        in[it] = ok;
+        at[it] = w < V27N ? CIdx27::pos(wx, wy, wz) : w;
+        body.stage(ok ? (k * g.nny + j) * g.nnx + i : 0, v[it]);
+    }
+#pragma unroll
+    for (int it = 0; it < V27IT; ++it)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) sm[c][at[it]] = in[it] ? v[it][c] : 0.0;
+    __syncthreads();
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int la = lane & 7, lm = (lane >> 3) & 3, lh = lane >> 5;
+    for (int m = wv; m < 27; m += 8) {
+        const int c = kCombo27.cls[m];
+        const int px = c & 1, py = (c >> 1) & 1, pz = c >> 2;
+        const int qx = kCombo27.qx[m], qy = kCombo27.qy[m], qz = kCombo27.qz[m];
+        const int i = i0 + 2 * la + px, j = j0 + 2 * lm + py, k = k0 + 2 * lh + pz;
+        int ex, cx, ey, cy, ez, cz;
+        bool vx, vy, vz;
+        cand_axis(i, g.nex, qx, ex, cx, vx);
+        cand_axis(j, g.ney, qy, ey, cy, vy);
+        cand_axis(k, g.nez, qz, ez, cz, vz);
+        const int a = ((pz ? 1 : (qz ? 0 : 2)) * 3 + (py ? 1 : (qy ? 0 : 2))) * 3 + (px ? 1 : (qx ? 0 : 2));
+        // cand_axis assumes a node of the grid: a lane past the grid's far edge (ragged last tile)
+        // would get element index ne along that axis, so such lanes are masked here
+        const bool ok = vx && vy && vz && i < g.nnx && j < g.nny && k < g.nnz;
+        const int e = ok ? (ez * g.ney + ey) * g.nex + ex : 0;
+        const int base = ok ? ix.base(ex, ey, ez) : 0;
+        double cf[27];
+        body.row(a, e, cf);
+        double t[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) t[q] = 0.0;
+#pragma unroll
+        for (int b = 0; b < 27; ++b) {
+            const int id = base + ix.off(b);
+#pragma unroll
+            for (int q = 0; q < NC; ++q) t[q] += cf[b] * sm[q][id];
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) res[(q * 27 + m) * 64 + lane] = ok ? t[q] : 0.0;
+    }
+    __syncthreads();
+    double acc[NR > 0 ? NR : 1];
+#pragma unroll
+    for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
+    {
+        const int px = wv & 1, py = (wv >> 1) & 1, pz = wv >> 2;
+        const int i = i0 + 2 * la + px, j = j0 + 2 * lm + py, k = k0 + 2 * lh + pz;
+        const RIdx27 rx{ix, res, kCombo27.m0[wv], kCombo27.nm[wv], lane};
+        if (i < g.nnx && j < g.nny && k < g.nnz) body((k * g.nny + j) * g.nnx + i, Inc27{i, j, k}, rx, sm, acc);
+    }
+    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
+}
+
 template <int NR, class Body, class Fin>
-static void launch_tile27(Ctx& c, const Grid3D& g, const int* active, const Body& body, const Fin& fin) {
-    const int ntx = (g.nnx + T27X - 1) / T27X, nty = (g.nny + T27Y - 1) / T27Y, ntz = (g.nnz + T27Z - 1) / T27Z;
+static void launch_btile27(Ctx& c, const Grid3D& g, const int* active, const Body& body, const Fin& fin) {
+    const int ntx = (g.nnx + C27X - 1) / C27X, nty = (g.nny + C27Y - 1) / C27Y, ntz = (g.nnz + C27Z - 1) / C27Z;
     const int nb = ntx * nty * ntz;
     if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-    k_tile27<NR><<<nb, BLOCK, 0, c.stream>>>(g, ntx, nty, active, body, c.partials, c.pstride);
+    k_btile27<NR><<<nb, C27T, 0, c.stream>>>(g, ntx, nty, active, body, c.partials, c.pstride);
     if constexpr (NR > 0)
         finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, fin_gate(active, fin, 0), fin, c.partials, c.pstride);
 }
+
 
 // ------------------------------------------------------------------------------ Ce
 // Ce[a][b] = sum_q w_q N_qa (U_q . grad N_qb + beta div U_q N_qb)   (element_calc.py:165-185)
@@ -547,7 +631,7 @@ struct BInit27 {
 // p = r + beta (p_old - omega q_old) (staged), q = A p, rhat . q
 template <class Op>
 struct BK1_27 {
-    static constexpr int NV = 3;
+    static constexpr int NV = 3, NC = 3;
     Op op;
     const uint8_t* mask;
     const double *r, *pold, *qold, *rhat;
@@ -562,6 +646,7 @@ struct BK1_27 {
             v[c] = r[i] + beta * (pold[i] - omega * qold[i]);
         }
     }
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[27]) const { op.row(a, e, cf); }
     template <class SM, class IX>
     __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
         const Grid3D& g = op.g;
@@ -585,7 +670,7 @@ struct BK1_27 {
 // s = r - alpha q (staged), t = A s, s.s, t.s, t.t
 template <class Op>
 struct BK2_27 {
-    static constexpr int NV = 3;
+    static constexpr int NV = 3, NC = 3;
     Op op;
     const uint8_t* mask;
     const double *r, *q;
@@ -597,6 +682,7 @@ struct BK2_27 {
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[c] = r[id + c * nn] - alpha * q[id + c * nn];
     }
+    __device__ __forceinline__ void row(int a, int e, double (&cf)[27]) const { op.row(a, e, cf); }
     template <class SM, class IX>
     __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[3]) const {
         const Grid3D& g = op.g;
@@ -629,18 +715,18 @@ static int solve_momentum_h27_t(Ctx& c, S ce, const double* U_n, const double* P
     Op lhs{g, c.eh27_dev, ce, rho / dt, theta * mu, theta * rho};
     Op rhs{g, c.eh27_dev, ce, rho / dt, -(1.0 - theta) * mu, -(1.0 - theta) * rho};
     const int maxit = c.maxit_mom > 0 ? c.maxit_mom : 10 * 3 * nn;
-    launch_tile27<2>(c, g, nullptr,
-                     BInit27<S>{lhs, rhs, U_n, c.ud_full, P_n, F, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
-                     BInitFin{c.st, c.tol, maxit});
+    launch_ctile27<2>(c, g, nullptr,
+                      BInit27<S>{lhs, rhs, U_n, c.ud_full, P_n, F, c.dmask, c.m_x, c.m_r, c.m_rhat, c.m_p[0], c.m_q[0]},
+                      BInitFin{c.st, c.tol, maxit});
     FCFD_CHECK_LAUNCH();
     const int* act = &c.st->b_active;
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_ctile27<1>(c, g, act,
+            launch_btile27<1>(c, g, act,
                              BK1_27<Op>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st},
                              BK1Fin{c.st});
-            launch_ctile27<3>(c, g, act, BK2_27<Op>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+            launch_btile27<3>(c, g, act, BK2_27<Op>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
                              BK2Fin{c.st});
             launch_node<2>(c, nn, act, BK3<3>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
                            BK3Fin{c.st});
@@ -739,10 +825,14 @@ struct CgInit27 {
 };
 
 struct CgK1_27 {
-    static constexpr int NV = 1;
+    static constexpr int NV = 1, NC = 1;
     Grid3D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
     const DevState* st;
     __device__ void stage(int id, double (&v)[NV]) const { v[0] = z[id] + st->c_beta * pold[id]; }
+    __device__ __forceinline__ void row(int a, int, double (&cf)[27]) const {
+#pragma unroll
+        for (int b = 0; b < 27; ++b) cf[b] = Le[a * 27 + b];
+    }
     template <class SM, class IX>
     __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
         const double a =
@@ -756,9 +846,13 @@ struct CgK1_27 {
 
 // two-term Neumann-series approximate inverse (standard.py:225-234)
 struct ApproxInv27a {
-    static constexpr int NV = 1;
+    static constexpr int NV = 1, NC = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
     __device__ void stage(int id, double (&v)[NV]) const { v[0] = r[id] / Ld[id]; }
+    __device__ __forceinline__ void row(int a, int, double (&cf)[27]) const {
+#pragma unroll
+        for (int b = 0; b < 27; ++b) cf[b] = Le[a * 27 + b];
+    }
     template <class SM, class IX>
     __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&)[1]) const {
         const double v = h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix) / Ld[n];
@@ -766,9 +860,13 @@ struct ApproxInv27a {
     }
 };
 struct ApproxInv27b {
-    static constexpr int NV = 1;
+    static constexpr int NV = 1, NC = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l; double* z;
     __device__ void stage(int id, double (&v)[NV]) const { v[0] = d1l[id]; }
+    __device__ __forceinline__ void row(int a, int, double (&cf)[27]) const {
+#pragma unroll
+        for (int b = 0; b < 27; ++b) cf[b] = Le[a * 27 + b];
+    }
     template <class SM, class IX>
     __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&)[1]) const {
         double l2 = h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix);
@@ -788,7 +886,7 @@ struct ApproxInv27b {
 // f = r - L x: pinned (x = 2/3 r / L_diag with the pins 0, pinned rows of L x 0) or Neumann (x
 // and r mean-projected)
 struct DownRes27 {
-    static constexpr int NV = 1;
+    static constexpr int NV = 1, NC = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* x; double* f;
     const DevState* st; int l; int neumann;
     // the pre-smoothed x (pinned: 2/3 r / L_diag, 0 at the pins) or the mean-projected x (Neumann)
@@ -799,6 +897,10 @@ struct DownRes27 {
             const double xv = (2.0 / 3.0 * r[id]) / Ld[id];
             v[0] = (pin && pin[id]) ? 0.0 : xv;
         }
+    }
+    __device__ __forceinline__ void row(int a, int, double (&cf)[27]) const {
+#pragma unroll
+        for (int b = 0; b < 27; ++b) cf[b] = Le[a * 27 + b];
     }
     template <class SM, class IX>
     __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
@@ -821,7 +923,7 @@ struct DownRes27 {
 
 // x <- x + p (p = P y of the coarser level); one damped Jacobi (multigrid.py:214-228, :306-326)
 struct Jacobi27 {
-    static constexpr int NV = 1;
+    static constexpr int NV = 1, NC = 1;
     Grid3D g; const double* Leoff; const double* Ld; const uint8_t* pin; const double* r; const double* x;
     const double* p; double* y; const DevState* st; int l; int neumann;
     // x + P y (0 at the pins), or with the means projected out (Neumann)
@@ -833,6 +935,10 @@ struct Jacobi27 {
             const double xv = x[id] + p[id];
             v[0] = (pin && pin[id]) ? 0.0 : xv;
         }
+    }
+    __device__ __forceinline__ void row(int a, int, double (&cf)[27]) const {
+#pragma unroll
+        for (int b = 0; b < 27; ++b) cf[b] = Leoff[a * 27 + b];
     }
     template <class SM, class IX>
     __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
@@ -905,7 +1011,12 @@ static const double* vcycle_h27(Ctx& c, const double* r0, const int* active) {
         if (neu) launch_node<1>(c, n, active, body, fin);
         else launch_node<0>(c, n, active, body, fin);
     };
+    // the down leg measured faster on the balanced tiles, the up leg on the class tiles
     auto run_tile = [&](const Grid3D& g, const auto& body, const MeanFin& fin) {
+        if (neu) launch_btile27<1>(c, g, active, body, fin);
+        else launch_btile27<0>(c, g, active, body, fin);
+    };
+    auto run_ctile = [&](const Grid3D& g, const auto& body, const MeanFin& fin) {
         if (neu) launch_ctile27<1>(c, g, active, body, fin);
         else launch_ctile27<0>(c, g, active, body, fin);
     };
@@ -940,7 +1051,7 @@ static const double* vcycle_h27(Ctx& c, const double* r0, const int* active) {
         Level& Lc = c.lv[l + 1];
         const int n = L.g3.nn;
         run(n, Prolong3{L.g3, Lc.g3, Lc.y, L.f, c.st, l, neu}, MeanFin{&c.st->mg[l][MG_P], nullptr, n});
-        run_tile(L.g3, Jacobi27{L.g3, L.eh27_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, L.y, c.st, l, neu},
+        run_ctile(L.g3, Jacobi27{L.g3, L.eh27_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, L.y, c.st, l, neu},
                  MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
     }
     FCFD_CHECK_LAUNCH();
@@ -1002,7 +1113,7 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_ctile27<1>(c, g, act, CgK1_27{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+            launch_btile27<1>(c, g, act, CgK1_27{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
                              CgK1Fin{c.st});
             if (c.precond == FCFD_PC_JACOBI || c.precond == FCFD_PC_NONE) {
                 launch_node<2>(c, nn, act,
@@ -1339,8 +1450,8 @@ void build_ctx_h27(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
         upload27(ui, c.fb_ui);
     }
     const size_t ne = g.ne;
-    if (c.ce_format == FCFD_CE_F64) c.ce = dalloc<double>(c, 729 * ne);
-    else c.ce = dalloc<float>(c, 729 * ne);
+    if (c.ce_format == FCFD_CE_F64) c.ce = dalloc<double>(c, 27 * H27_ROW * ne);
+    else c.ce = dalloc<float>(c, 27 * H27_ROW * ne);
     const size_t n3 = 3 * (size_t)nn;
     c.m_r = dalloc<double>(c, n3);
     c.m_rhat = dalloc<double>(c, n3);
