@@ -487,7 +487,14 @@ void build_ctx(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s, void*
     c.m_t = dalloc<double>(c, 2 * nn);
     c.m_x = dalloc<double>(c, 2 * nn);
     c.c_r = dalloc<double>(c, nn);
-    if (c.precond == FCFD_PC_MULTIGRID && c.set_zeroP) c.c_r2 = dalloc<double>(c, nn);
+    // the tiled mixed pure-Neumann V-cycle (see mg_neu_tiled)
+    c.mg_neu_tiled = c.precond == FCFD_PC_MULTIGRID && !c.set_zeroP && c.precision == FCFD_PREC_MIXED &&
+                     !c.g.periodic && c.nlev >= 2 && c.mg_tiled;
+    if (c.mg_neu_tiled) {
+        c.zero_pin = dalloc<uint8_t>(c, nn);
+        FCFD_HIP(hipMemset(c.zero_pin, 0, nn));
+    }
+    if (c.precond == FCFD_PC_MULTIGRID && (c.set_zeroP || c.mg_neu_tiled)) c.c_r2 = dalloc<double>(c, nn);
     c.c_Ap = dalloc<double>(c, nn);
     c.c_x = dalloc<double>(c, nn);
     c.c_rhs = dalloc<double>(c, nn);

@@ -122,6 +122,10 @@ struct Ctx {
     IterGraph g_mom, g_cg, g_phi, g_amom, g_aphi;
     bool use_graphs = true;
     bool mg_tiled = true;   // pinned non-periodic quad4 V-cycle: the tiled kernels (L_diag by node class)
+    // mixed form, pure Neumann, non-periodic: the FP32 V-cycle runs on the tiled kernels with no
+    // pins (zero_pin: nn zeros) and no intermediate mean projections
+    bool mg_neu_tiled = false;
+    uint8_t* zero_pin = nullptr;
     Grid2D g;
     double lx, ly;
     ElemConst ec;

@@ -829,8 +829,18 @@ static LdTab<T> ld_tab(const Level& L) {
 
 // The tiled V-cycle can carry the PCG's x / r update (first down leg) and z-step (last up leg)
 // when both fine-level legs run as single-level kernels.
+// y = 2/3 r / L_diag: one damped Jacobi from zero with no pinned rows
+template <class T>
+__global__ __launch_bounds__(256) void k_coarse_jacobi0(int nn, const T* __restrict__ Ld, const T* __restrict__ r,
+                                                        T* __restrict__ y, const int* __restrict__ active) {
+    if (active && *active == 0) return;
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n < nn) y[n] = (T(OMEGA) * r[n]) / Ld[n];
+}
+
 static bool cg_fusable(const Ctx& c) {
-    if (!(c.precond == FCFD_PC_MULTIGRID && c.set_zeroP && !c.g.periodic && c.nlev >= 2 && c.mg_tiled && c.c_r2))
+    if (!(c.precond == FCFD_PC_MULTIGRID && (c.set_zeroP || c.mg_neu_tiled) && !c.g.periodic && c.nlev >= 2 &&
+          c.mg_tiled && c.c_r2))
         return false;
     const int ld = c.nlev - 1;
     const bool down_single = !(1 < ld && c.lv[1].g.nny < 200);
@@ -847,6 +857,8 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
     if (tail && tail->cg_p && !cg_fusable(c)) throw Error(FCFD_ERR_STATE, "CG update not fusable here");
     using B = LvBufs<T>;
     const int N = c.nlev, ld = N - 1;
+    // pure Neumann (mixed form): no pinned rows anywhere
+    auto pn = [&](const Level& X) -> const uint8_t* { return X.pin ? X.pin : c.zero_pin; };
     bool fused = false;
     int down_nb = 0;  // blocks of the first down leg when it carries the CG's x / r update
     for (int l = 0; l < ld;) {
@@ -857,11 +869,11 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
             const dim3 grid((L2.g.nnx + M2_C - 1) / M2_C, (L2.g.nny + M2_R - 1) / M2_R);
             if (l == 0)
                 k_mgt_down2<T, double><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le,
-                                                                   ld_tab<T>(L), ld_tab<T>(Lc), r0, L.pin, Lc.pin,
+                                                                   ld_tab<T>(L), ld_tab<T>(Lc), r0, pn(L), pn(Lc),
                                                                    B::r(Lc), B::r(L2), active);
             else
                 k_mgt_down2<T, T><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le,
-                                                              ld_tab<T>(L), ld_tab<T>(Lc), B::r(L), L.pin, Lc.pin,
+                                                              ld_tab<T>(L), ld_tab<T>(Lc), B::r(L), pn(L), pn(Lc),
                                                               B::r(Lc), B::r(L2), active);
             l += 2;
         } else {
@@ -870,7 +882,7 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
                 constexpr int CX = MGT_DOWN_CX, CY = MGT_DOWN_CY;
                 const int nbx = (Lc.g.nnx + CX - 1) / CX;
                 k_mgt_down<T, TR, CX, CY><<<dim3(nbx, (Lc.g.nny + CY - 1) / CY), 256, 0, c.stream>>>(
-                    L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), rl, L.pin, B::r(Lc), active);
+                    L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), rl, pn(L), B::r(Lc), active);
             };
             if (l == 0 && tail && tail->cg_p) {
                 constexpr int CX = MGT_DOWN_CX, CY = MGT_DOWN_CY;
@@ -879,7 +891,7 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
                 if (down_nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
                 k_mgt_down_cg<T, CX, CY><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), r0,
                                                                    tail->cg_Ap, tail->cg_p, tail->cg_x,
-                                                                   tail->cg_rout, L.pin, B::r(Lc), c.st, c.partials,
+                                                                   tail->cg_rout, pn(L), B::r(Lc), c.st, c.partials,
                                                                    c.pstride, 2, active);
                 r0 = tail->cg_rout;
             } else if (l == 0) {
@@ -895,9 +907,15 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
         Level& C = c.lv[N - 1];
         const bool f32 = std::is_same<T, float>::value;
         const double ctol = f32 ? (c.tol > 1e-5 ? c.tol : 1e-5) : c.tol;
-        if (f32 && c.coarse_inv32) {
+        if (!c.set_zeroP) {
+            // the reference's pure-Neumann coarse solve: one Jacobi from zero (without the mean
+            // projections, which this V-cycle leaves out)
+            k_coarse_jacobi0<T><<<(C.g.nn + 255) / 256, 256, 0, c.stream>>>(C.g.nn, B::Ld(C), B::r(C), B::y(C),
+                                                                              active);
+            FCFD_CHECK_LAUNCH();
+        } else if (f32 && c.coarse_inv32) {
             const int nb = (C.g.nn + 4 * DENSE_ROWS - 1) / (4 * DENSE_ROWS);
-            k_coarse_dense<T><<<nb, 256, 0, c.stream>>>(C.g.nn, c.coarse_inv32, C.pin, B::r(C), B::y(C), active);
+            k_coarse_dense<T><<<nb, 256, 0, c.stream>>>(C.g.nn, c.coarse_inv32, pn(C), B::r(C), B::y(C), active);
             FCFD_CHECK_LAUNCH();
         } else {
             coarse_solve<T, T>(c, B::r(C), B::y(C), B::cr(c), B::cp(c), B::cAp(c), ctol, active);
@@ -911,12 +929,12 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
             const dim3 grid((Lf.g.nnx + U2_C - 1) / U2_C, (Lf.g.nny + U2_R - 1) / U2_R);
             if (l - 1 == 0)
                 k_mgt_up2<T, double><<<grid, 256, 0, c.stream>>>(Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff,
-                                                                 ld_tab<T>(Lf), ld_tab<T>(L), r0, Lf.pin, B::r(L),
-                                                                 L.pin, B::y(Lc), B::y(Lf), active);
+                                                                 ld_tab<T>(Lf), ld_tab<T>(L), r0, pn(Lf), B::r(L),
+                                                                 pn(L), B::y(Lc), B::y(Lf), active);
             else
                 k_mgt_up2<T, T><<<grid, 256, 0, c.stream>>>(Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff,
-                                                            ld_tab<T>(Lf), ld_tab<T>(L), B::r(Lf), Lf.pin, B::r(L),
-                                                            L.pin, B::y(Lc), B::y(Lf), active);
+                                                            ld_tab<T>(Lf), ld_tab<T>(L), B::r(Lf), pn(Lf), B::r(L),
+                                                            pn(L), B::y(Lc), B::y(Lf), active);
             FCFD_CHECK_LAUNCH();
             l -= 2;
             continue;
@@ -929,7 +947,7 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
             const int nb = (int)(grid.x * grid.y);
             if (ZM != 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
             k_mgt_up<T, TR, FC, FR, ZM><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Leoff, ld_tab<T>(L), rl,
-                                                                    L.pin, B::y(Lc), B::y(L), znew, zold, pzero,
+                                                                    pn(L), B::y(Lc), B::y(L), znew, zold, pzero,
                                                                     flex, c.partials, c.pstride, active);
             if constexpr (ZM == 1)
                 finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, CgGamma0Fin{c.st}, c.partials,
@@ -970,7 +988,9 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
 // return value says so (nothing is then stored into the level-0 y).
 template <class T>
 static bool vcycle_t(Ctx& c, const double* r0, const int* active, const CgTail* tail = nullptr) {
-    if (c.set_zeroP && !c.g.periodic && c.nlev >= 2 && c.mg_tiled) return vcycle_tiled<T>(c, r0, active, tail);
+    if ((c.set_zeroP || (std::is_same<T, float>::value && c.mg_neu_tiled)) && !c.g.periodic && c.nlev >= 2 &&
+        c.mg_tiled)
+        return vcycle_tiled<T>(c, r0, active, tail);
     bool fused = false;
     using B = LvBufs<T>;
     const int N = c.nlev;
@@ -1130,6 +1150,14 @@ void vcycle(Ctx& c, const double* r, double* z) {
                                                          MeanFin{&c.st->mg[0][MG_R], nullptr, nn});
     if (c.precision == FCFD_PREC_MIXED) {
         vcycle_t<float>(c, r, nullptr);
+        if (c.mg_neu_tiled) {
+            // the tiled pure-Neumann V-cycle leaves the projections out: take the mean of y here
+            struct SumY {
+                const float* y;
+                __device__ void operator()(int k, double (&acc)[1]) const { acc[0] += (double)y[k]; }
+            };
+            launch_node<1>(c, nn, nullptr, SumY{c.lv[0].y32}, MeanFin{&c.st->mg[0][MG_Y], nullptr, nn});
+        }
         launch_node<0>(c, nn, nullptr,
                                                              ZOut<float>{c.lv[0].y32, z, c.st, !c.set_zeroP},
                                                              NoFin{});
@@ -1322,6 +1350,7 @@ static void pcg_one(Ctx& c, int o, ChunkEnd ce = ChunkEnd{}) {
     const int* act = &c.st->c_active;
     const int nw = o ^ 1;
     if (cg_fusable(c)) {
+        if (!pin) pin = c.zero_pin;  // pure Neumann: no pinned rows
         // r alternates between c_r (parity 0) and c_r2; the tiled K1, then the V-cycle with the
         // x / r update on its first down leg and the z-step on its last up leg
         double* rin = o == 0 ? c.c_r : c.c_r2;
@@ -1434,7 +1463,7 @@ static int pcg_impl(Ctx& c, const double* b, const double* bmean, const double* 
     const int* act = &c.st->c_active;
     bool f32, fused0;
     const CgTail tail0{c.c_z[0], nullptr, c.c_p[0], 0, 1};
-    const void* y0 = precondition(c, act, f32, neu ? nullptr : &tail0, &fused0);
+    const void* y0 = precondition(c, act, f32, (neu && !c.mg_neu_tiled) ? nullptr : &tail0, &fused0);
     if (!fused0 && f32)
         launch_node<1>(c, nn, act, CgGamma0<float>{c.c_r, (const float*)y0, c.c_z[0], c.c_p[0], c.st, neu},
                        CgGamma0Fin{c.st});
