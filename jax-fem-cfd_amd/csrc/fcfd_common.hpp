@@ -42,6 +42,28 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * per + (x < rem ? x : rem) + local;
 }
 
+// The gate flag of a kernel enqueued ahead of its solver's convergence test (null: always
+// open), read at entry but tested late: a test at entry would put the flag's memory round trip in
+// front of every other load of the block.  The kernels that use this test it once the block's
+// first loads are issued -- after their first barrier, or before their first global store --
+// and until then only read, at valid (clamped) addresses.  The value is the same for the whole
+// grid, so the test is block-uniform and may sit between barriers.  The flag is read with a
+// vector-memory load: scalar loads share one wait counter with the kernel argument loads, so the
+// first use of any argument would wait for the flag too.  The lane-varying index (threadIdx.y,
+// 0 in the 1-D blocks that call this) keeps the load off the scalar path without making it an
+// atomic, which would count as a write and push the kernel's uniform loads of constant data
+// (element matrices) off the scalar path as well.  A null gate reads an always-open flag
+// instead of branching.
+namespace {
+__device__ int kGateAlwaysOpen = 1;
+}
+__device__ __forceinline__ int gate_read(const int* active) {
+    const int* p = active ? active : &kGateAlwaysOpen;
+    return p[threadIdx.y];
+}
+// the test, at the site where the block needs it
+__device__ __forceinline__ bool gate_open(int g) { return __builtin_amdgcn_readfirstlane(g) != 0; }
+
 // Stencil context of one node: the four incident elements (element to the lower-left,
 // lower-right, upper-left, upper-right of the node) with the node's local index in each
 // (2, 3, 1, 0), and the 3x3 neighbour node ids.  Ids of missing neighbours / elements (edges of a
@@ -401,7 +423,7 @@ __global__ __launch_bounds__(BLOCK) void stencil_kernel(int n, const int* __rest
 template <int NR, class Fin>
 __global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(int nb, const int* __restrict__ active, Fin fin,
                                                             const double* __restrict__ partials, int pstride) {
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     // The partials were written by blocks on every XCD, so each read is a far-memory round trip:
     // a thread issues FIN_U loads per reduction before it adds any of them (fixed order, so the
     // sums stay bitwise reproducible), and 4096 partials take one round trip instead of four.
@@ -424,7 +446,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(int nb, const int* 
             for (int u = 0; u < FIN_U; ++u) tot[r] += v[r][u];
     }
     block_sum<NR>(tot);
-    if (threadIdx.x == 0) fin(tot);
+    if (gate_open(go) && threadIdx.x == 0) fin(tot);
 }
 
 // Two groups of partials with different block counts: slots [0, NR1) hold nb1 partials each,
@@ -433,7 +455,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(int nb, const int* 
 template <int NR1, int NR2, class Fin>
 __global__ __launch_bounds__(FIN_BLOCK) void finalize2_kernel(int nb1, int nb2, const int* __restrict__ active, Fin fin,
                                                              const double* __restrict__ partials, int pstride) {
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int NR = NR1 + NR2, FIN_U = 8;
     double tot[NR];
 #pragma unroll
@@ -454,7 +476,7 @@ __global__ __launch_bounds__(FIN_BLOCK) void finalize2_kernel(int nb1, int nb2, 
             for (int u = 0; u < FIN_U; ++u) tot[r] += v[r][u];
     }
     block_sum<NR>(tot);
-    if (threadIdx.x == 0) fin(tot);
+    if (gate_open(go) && threadIdx.x == 0) fin(tot);
 }
 
 }  // namespace fcfd

@@ -790,7 +790,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_coarse_dense(int nn, const float* __restrict__ Ainv,
                                                       const uint8_t* __restrict__ pin, const T* __restrict__ r,
                                                       T* __restrict__ y, const int* active) {
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     const int lane = threadIdx.x & 63;
     const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * DENSE_ROWS;
     double acc[DENSE_ROWS];
@@ -809,7 +809,7 @@ __global__ __launch_bounds__(256) void k_coarse_dense(int nn, const float* __res
     for (int m = 0; m < DENSE_ROWS; ++m) {
         const double v = wave_sum_dpp(acc[m]);
         const int i = row0 + m;
-        if (lane == 0 && i < nn) y[i] = (pin && pin[i]) ? T(0) : (T)v;
+        if (gate_open(go) && lane == 0 && i < nn) y[i] = (pin && pin[i]) ? T(0) : (T)v;
     }
 }
 

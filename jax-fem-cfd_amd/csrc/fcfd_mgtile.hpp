@@ -184,7 +184,7 @@ template <class T, class TR, int CX, int CY>
 __global__ __launch_bounds__(256) void k_mgt_down(Grid2D gf, Grid2D gc, const double* __restrict__ Le, LdTab<T> ldt_,
                                                   const TR* __restrict__ r, const uint8_t* __restrict__ pin,
                                                   T* __restrict__ rc, const int* __restrict__ active) {
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int XW = 2 * CX + 3, XR = 2 * CY + 3;
     constexpr int FW = 2 * CX + 1, FR = 2 * CY + 1;
     __shared__ T ldl[9];
@@ -201,6 +201,7 @@ __global__ __launch_bounds__(256) void k_mgt_down(Grid2D gf, Grid2D gc, const do
     w.load(gf, fi0, fj0, [&](int k) { return RPin<TR>{r[k], pin[k]}; });
     ld_tab_lds(ldt_, ldl);
     __syncthreads();
+    if (!gate_open(go)) return;
     w.store([&](int a, int b, bool in, const RPin<TR>& v) {
         const T rv = (T)v.r;
         const bool pn = v.p != 0;
@@ -247,7 +248,7 @@ __global__ __launch_bounds__(256) void k_mgt_down_cg(Grid2D gf, Grid2D gc, const
                                                      const uint8_t* __restrict__ pin, T* __restrict__ rc,
                                                      const DevState* __restrict__ st, double* __restrict__ partials,
                                                      int pstride, int slot, const int* __restrict__ active) {
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int XW = 2 * CX + 3, XR = 2 * CY + 3;
     constexpr int FW = 2 * CX + 1, FR = 2 * CY + 1;
     constexpr int OW = 2 * CX, OR = 2 * CY, NO = (OW * OR + BLOCK - 1) / BLOCK;  // owned fine nodes
@@ -280,6 +281,7 @@ __global__ __launch_bounds__(256) void k_mgt_down_cg(Grid2D gf, Grid2D gc, const
     }
     ld_tab_lds(ldt_, ldl);
     __syncthreads();
+    if (!gate_open(go)) return;
     w.store([&](int a, int b, bool in, const RAP& v) {
         const double rk = v.r - alpha * v.ap;
         const T rv = (T)rk;
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(256) void k_mgt_down2(Grid2D gf, Grid2D gm, Grid2D 
                                                    const TR* __restrict__ rf, const uint8_t* __restrict__ pinf,
                                                    const uint8_t* __restrict__ pinm, T* __restrict__ rm,
                                                    T* __restrict__ rc, const int* __restrict__ active) {
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int XRm = 2 * M2_R + 3, XWm = 2 * M2_C + 3;  // m-window of x_m, r_m
     constexpr int FRm = 2 * M2_R + 1, FWm = 2 * M2_C + 1;  // m-window of f_m
     constexpr int FRl = 2 * XRm + 1, FWl = 2 * XWm + 1;    // l-window of f_l
@@ -369,6 +371,7 @@ __global__ __launch_bounds__(256) void k_mgt_down2(Grid2D gf, Grid2D gm, Grid2D 
     ld_tab_lds(ldf_, ldfl);
     ld_tab_lds(ldm_, ldml);
     __syncthreads();
+    if (!gate_open(go)) return;
     w.store([&](int a, int b, bool in, const RPin<TR>& v) {
         const T rv = (T)v.r;
         const bool pn = v.p != 0;
@@ -452,7 +455,7 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
                                                 const int* __restrict__ active) {
     static_assert(FC % 2 == 0 && FR % 2 == 0 && (FC * FR) % BLOCK == 0,
                   "even tile sides (the coarse window starts at tile origin / 2 - 1), whole nodes per thread");
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int WW = FC + 2, WR = FR + 2;          // fine window (stencil of the owned nodes)
     constexpr int CW = FC / 2 + 3, CR = FR / 2 + 3;  // coarse window (prolongation sources)
     constexpr int NPT = FC * FR / BLOCK;
@@ -490,6 +493,7 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
     });
     wc.store([&](int a, int b, bool, T v) { ys[a][b] = v; });
     __syncthreads();
+    if (!gate_open(go)) return;
     for (int t = threadIdx.x; t < WR * WW; t += BLOCK) {
         const int a = t / WW, b = t - a * WW;
         const int fi = wi0 + b, fj = wj0 + a;
@@ -544,7 +548,7 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
                                                  const T* __restrict__ rm, const uint8_t* __restrict__ pinm,
                                                  const T* __restrict__ yc, T* __restrict__ yf,
                                                  const int* __restrict__ active) {
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int LR = U2_R + 2, LC = U2_C + 2;            // level-l window (tile stencil)
     constexpr int MR = U2_R / 2 + 3, MC = U2_C / 2 + 3;    // m-nodes the l-window's prolongation reads
     constexpr int MXR = MR + 2, MXC = MC + 2;              // their stencil window
@@ -584,6 +588,7 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
     });
     wc.store([&](int a, int b, bool, T v) { ycs[a][b] = v; });
     __syncthreads();
+    if (!gate_open(go)) return;
     for (int t = threadIdx.x; t < MXR * MXC; t += BLOCK) {
         const int a = t / MXC, b = t - a * MXC;
         const int mi = xmi0 + b, mj = xmj0 + a;
@@ -629,7 +634,7 @@ __global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restri
                                                 const DevState* __restrict__ st, double* __restrict__ partials,
                                                 int pstride, const int* __restrict__ active) {
     static_assert((FC * FR) % BLOCK == 0, "whole nodes per thread");
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int WW = FC + 2, WR = FR + 2, NPT = FC * FR / BLOCK;
     struct ZP {
         double z, p;
@@ -649,6 +654,7 @@ __global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restri
         pns[a][b] = v.pin;
     });
     __syncthreads();
+    if (!gate_open(go)) return;
     double acc[1] = {0.0};
 #pragma unroll
     for (int m = 0; m < NPT; ++m) {
@@ -690,7 +696,7 @@ __global__ __launch_bounds__(256) void k_cginit_us(Grid2D g, const ElemConst* __
                                                    double* __restrict__ partials, int pstride,
                                                    const int* __restrict__ gate) {
     static_assert(FC * FR == BLOCK, "one node per thread");
-    if (gate && *gate == 0) return;
+    const int go = gate_read(gate);
     constexpr int PW = FC + 4, PR = FR + 4;  // P_n window: two wider
     constexpr int UW = FC + 2, UR = FR + 2;  // U_hat / M / pins / U* window: one wider
     struct UMP {
@@ -724,6 +730,7 @@ __global__ __launch_bounds__(256) void k_cginit_us(Grid2D g, const ElemConst* __
         pns[a][b] = in && v.p != 0;
     });
     __syncthreads();
+    if (!gate_open(go)) return;
     for (int t = threadIdx.x; t < UR * UW; t += BLOCK) {
         const int a = t / UW, b = t - a * UW;
         const int ui = i0 - 1 + b, uj = j0 - 1 + a;

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void k_mom_k1t(MomCoef<S> op, const uint8_t* _
                                                  const DevState* __restrict__ st, double* __restrict__ partials,
                                                  int pstride, const int* __restrict__ active) {
     static_assert(FC * FR == BLOCK, "one node per thread");
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int WW = FC + 2, WR = FR + 2;
     constexpr int WP = Win<WR, WW, Val2M>::HP;
     __shared__ double xs[2][WP][WW];
@@ -131,6 +131,7 @@ __global__ __launch_bounds__(256) void k_mom_k1t(MomCoef<S> op, const uint8_t* _
         });
     }
     __syncthreads();
+    if (!gate_open(go)) return;
     double acc[1] = {0.0};
     if (own) {
         double X[2][3][3];
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(256) void k_mom_k2t(MomCoef<S> op, const uint8_t* _
                                                  double* __restrict__ partials, int pstride,
                                                  const int* __restrict__ active) {
     static_assert(FC * FR == BLOCK, "one node per thread");
-    if (active && *active == 0) return;
+    const int go = gate_read(active);
     constexpr int WW = FC + 2, WR = FR + 2;
     struct V4 {
         double r0, q0, r1, q1;
@@ -193,6 +194,7 @@ __global__ __launch_bounds__(256) void k_mom_k2t(MomCoef<S> op, const uint8_t* _
         xs[1][a][b] = in ? bicg_s(v.r1, alpha, v.q1) : 0.0;
     });
     __syncthreads();
+    if (!gate_open(go)) return;
     double acc[3] = {0.0, 0.0, 0.0};
     if (own) {
         double X[2][3][3];
