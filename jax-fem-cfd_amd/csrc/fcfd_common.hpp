@@ -49,17 +49,24 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // and until then only read, at valid (clamped) addresses.  The value is the same for the whole
 // grid, so the test is block-uniform and may sit between barriers.  The flag is read with a
 // vector-memory load: scalar loads share one wait counter with the kernel argument loads, so the
-// first use of any argument would wait for the flag too.  The lane-varying index (threadIdx.y,
-// 0 in the 1-D blocks that call this) keeps the load off the scalar path without making it an
+// first use of any argument would wait for the flag too.  gate_read issues the load at entry
+// and gate_open tests the value where the block needs it.  The load is a raw buffer load with the
+// agent-scope coherence bit (sc1) -- what an agent-scope atomic load compiles to, since other
+// kernels on other CUs and XCDs write the flag -- issued through the builtin rather than as an
 // atomic, which would count as a write and push the kernel's uniform loads of constant data
 // (element matrices) off the scalar path as well.  A null gate reads an always-open flag
 // instead of branching.
 namespace {
 __device__ int kGateAlwaysOpen = 1;
 }
+// Buffer resource word 3 for a raw 32-bit buffer load on CDNA (DATA_FORMAT = 32 bits), and the
+// cache-policy operand with sc1 set (agent-scope coherence on gfx94x / gfx950)
+constexpr int kBufRsrcWord3 = 0x00020000;
+constexpr int kCachePolicySc1 = 16;
 __device__ __forceinline__ int gate_read(const int* active) {
     const int* p = active ? active : &kGateAlwaysOpen;
-    return p[threadIdx.y];
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, 4, kBufRsrcWord3);
+    return __builtin_amdgcn_raw_buffer_load_b32(rs, 0, 0, kCachePolicySc1);
 }
 // the test, at the site where the block needs it
 __device__ __forceinline__ bool gate_open(int g) { return __builtin_amdgcn_readfirstlane(g) != 0; }

@@ -523,6 +523,14 @@ struct CgTail {
     double* znew; const double* zold; double* pzero; int flex; int first;
     const double* cg_p = nullptr; const double* cg_Ap = nullptr; double* cg_x = nullptr; double* cg_rout = nullptr;
     ChunkEnd ce{};  // the iteration ends a speculative chunk: its last finalize records it
+    // z is stored in FP32 (the tiled FP32 V-cycle's z-step; the z buffers then hold floats): set
+    // only where the tiled K1 reads it (z_f32), FP64 everywhere else
+    int z32 = 0;
+};
+
+template <class X>
+struct type_tag {
+    using type = X;
 };
 
 // ------------------------------------------------------------------------------ coarse level
@@ -848,6 +856,11 @@ static bool cg_fusable(const Ctx& c) {
     return down_single && up_single;
 }
 
+// Whether the PCG's z buffers hold FP32 values: the mixed form on the fused tiled iteration,
+// whose z-step (the tiled FP32 V-cycle's last up leg) stores z as the FP32 y it is and whose K1
+// widens it on load -- the FP64 values an FP64 store of the widened y would hold.
+static bool z_f32(const Ctx& c) { return c.precision == FCFD_PREC_MIXED && cg_fusable(c); }
+
 // The pinned V-cycle on a non-periodic grid with the tiled kernels (fcfd_mgtile.hpp): two small
 // levels per kernel on each leg below 200 node rows (down) / 2^17 nodes (up), one level per
 // kernel above.  With `tail` the level-0 up leg (when it runs alone) also runs the PCG's z-step;
@@ -940,15 +953,16 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
             continue;
         }
         // one level: tiles of MGT_UP_C x MGT_UP_R nodes
-        auto go = [&](auto rl, auto zm, double* znew, const double* zold, double* pzero, int flex) {
+        auto go = [&](auto rl, auto zm, auto zt, double* znew, const double* zold, double* pzero, int flex) {
             using TR = std::remove_const_t<std::remove_pointer_t<decltype(rl)>>;
+            using ZT = typename decltype(zt)::type;
             constexpr int FC = MGT_UP_C, FR = MGT_UP_R, ZM = decltype(zm)::value;
             const dim3 grid((L.g.nnx + FC - 1) / FC, (L.g.nny + FR - 1) / FR);
             const int nb = (int)(grid.x * grid.y);
             if (ZM != 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-            k_mgt_up<T, TR, FC, FR, ZM><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Leoff, ld_tab<T>(L), rl,
-                                                                    pn(L), B::y(Lc), B::y(L), znew, zold, pzero,
-                                                                    flex, c.partials, c.pstride, active);
+            k_mgt_up<T, TR, FC, FR, ZM, ZT><<<grid, 256, 0, c.stream>>>(
+                L.g, Lc.g, L.ec_dev->Leoff, ld_tab<T>(L), rl, pn(L), B::y(Lc), B::y(L), (ZT*)znew, (const ZT*)zold,
+                pzero, flex, c.partials, c.pstride, active);
             if constexpr (ZM == 1)
                 finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, CgGamma0Fin{c.st}, c.partials,
                                                                   c.pstride);
@@ -966,16 +980,21 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
         using Z0 = std::integral_constant<int, 0>;
         using Z1 = std::integral_constant<int, 1>;
         using Z2 = std::integral_constant<int, 2>;
+        using ZF = type_tag<float>;
+        using ZD = type_tag<double>;
+        const bool z32 = std::is_same<T, float>::value && tail && tail->z32;
         if (l == 0 && tail && tail->first) {
-            go(r0, Z1{}, tail->znew, nullptr, tail->pzero, 0);
+            if (z32) go(r0, Z1{}, ZF{}, tail->znew, nullptr, tail->pzero, 0);
+            else go(r0, Z1{}, ZD{}, tail->znew, nullptr, tail->pzero, 0);
             fused = true;
         } else if (l == 0 && tail) {
-            go(r0, Z2{}, tail->znew, tail->zold, nullptr, tail->flex);
+            if (z32) go(r0, Z2{}, ZF{}, tail->znew, tail->zold, nullptr, tail->flex);
+            else go(r0, Z2{}, ZD{}, tail->znew, tail->zold, nullptr, tail->flex);
             fused = true;
         } else if (l == 0) {
-            go(r0, Z0{}, nullptr, nullptr, nullptr, 0);
+            go(r0, Z0{}, type_tag<T>{}, nullptr, nullptr, nullptr, 0);
         } else {
-            go((const T*)B::r(L), Z0{}, nullptr, nullptr, nullptr, 0);
+            go((const T*)B::r(L), Z0{}, type_tag<T>{}, nullptr, nullptr, nullptr, 0);
         }
         FCFD_CHECK_LAUNCH();
         l -= 1;
@@ -1360,8 +1379,13 @@ static void pcg_one(Ctx& c, int o, ChunkEnd ce = ChunkEnd{}) {
             const dim3 grid((g.nnx + FC - 1) / FC, (g.nny + FR - 1) / FR);
             const int nb = (int)(grid.x * grid.y);
             if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-            k_cg_k1t<FC, FR><<<grid, 256, 0, c.stream>>>(g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st,
-                                                        c.partials, c.pstride, act);
+            if (z_f32(c))
+                k_cg_k1t<FC, FR, float><<<grid, 256, 0, c.stream>>>(g, Le, pin, (const float*)c.c_z[o], c.c_p[o],
+                                                                    c.c_p[nw], c.c_Ap, c.st, c.partials, c.pstride,
+                                                                    act);
+            else
+                k_cg_k1t<FC, FR, double><<<grid, 256, 0, c.stream>>>(g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw],
+                                                                     c.c_Ap, c.st, c.partials, c.pstride, act);
             finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, CgK1Fin{c.st}, c.partials, c.pstride);
         }
         CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
@@ -1370,6 +1394,7 @@ static void pcg_one(Ctx& c, int o, ChunkEnd ce = ChunkEnd{}) {
         tail.cg_x = c.c_x;
         tail.cg_rout = rout;
         tail.ce = ce;
+        tail.z32 = z_f32(c);
         const bool f = vcycle_t_mixed_or_64(c, rin, act, &tail);
         if (!f) throw Error(FCFD_ERR_STATE, "fused PCG iteration without its z-step");
         FCFD_CHECK_LAUNCH();
@@ -1462,7 +1487,8 @@ static int pcg_impl(Ctx& c, const double* b, const double* bmean, const double* 
                        CgInitFin{c.st, tol, maxit, nn, zero_exit});
     const int* act = &c.st->c_active;
     bool f32, fused0;
-    const CgTail tail0{c.c_z[0], nullptr, c.c_p[0], 0, 1};
+    CgTail tail0{c.c_z[0], nullptr, c.c_p[0], 0, 1};
+    tail0.z32 = z_f32(c);
     const void* y0 = precondition(c, act, f32, (neu && !c.mg_neu_tiled) ? nullptr : &tail0, &fused0);
     if (!fused0 && f32)
         launch_node<1>(c, nn, act, CgGamma0<float>{c.c_r, (const float*)y0, c.c_z[0], c.c_p[0], c.st, neu},

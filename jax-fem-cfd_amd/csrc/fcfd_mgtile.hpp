@@ -444,13 +444,15 @@ __global__ __launch_bounds__(256) void k_mgt_down2(Grid2D gf, Grid2D gm, Grid2D 
 // the coarse correction y_{l+1} on the window its prolongation reads, forms v = x + I y at every
 // window node once (0 where missing or pinned), then one damped Jacobi per owned node from v.
 // ZM = 0: y_l stored (T).  ZM = 1 (level 0, before the first PCG iteration: CgGamma0's pass):
-// z = y stored as FP64 into znew, p_old cleared, block partial of r.z.  ZM = 2 (level 0 inside an
-// iteration: CgK3's pass): z stored, partials of r.z and (flexible form) r.z_old.
-template <class T, class TR, int FC, int FR, int ZM>
+// z = y stored into znew, p_old cleared, block partial of r.z.  ZM = 2 (level 0 inside an
+// iteration: CgK3's pass): z stored, partials of r.z and (flexible form) r.z_old.  ZT: the type
+// z is stored in -- FP64, or the FP32 V-cycle's own type where the CG's reader widens it on load
+// (the FP64 value an FP64 store of the widened y would hold, at half the bytes).
+template <class T, class TR, int FC, int FR, int ZM, class ZT = T>
 __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const double* __restrict__ Leoff, LdTab<T> ldt_,
                                                 const TR* __restrict__ r, const uint8_t* __restrict__ pin,
-                                                const T* __restrict__ yc, T* __restrict__ y, double* __restrict__ znew,
-                                                const double* __restrict__ zold, double* __restrict__ pzero, int flex,
+                                                const T* __restrict__ yc, T* __restrict__ y, ZT* __restrict__ znew,
+                                                const ZT* __restrict__ zold, double* __restrict__ pzero, int flex,
                                                 double* __restrict__ partials, int pstride,
                                                 const int* __restrict__ active) {
     static_assert(FC % 2 == 0 && FR % 2 == 0 && (FC * FR) % BLOCK == 0,
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
             const int i = i0 + t % FC, j = j0 + t / FC;
             const int n = (i < gf.nnx && j < gf.nny) ? j * gf.nnx + i : 0;
             rown[m] = (double)r[n];
-            zown[m] = (ZM == 2 && flex) ? zold[n] : 0.0;
+            zown[m] = (ZM == 2 && flex) ? (double)zold[n] : 0.0;
         }
     }
     ld_tab_lds(ldt_, ldl);
@@ -516,7 +518,7 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
                 y[n] = v;
             } else {
                 const double zk = (double)v;
-                znew[n] = zk;
+                znew[n] = (ZT)v;
                 if constexpr (ZM == 1) pzero[n] = 0.0;
                 acc[0] += rown[m] * zk;
                 if constexpr (ZM == 2) {
@@ -626,10 +628,11 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
 // ------------------------------------------------------------------------------ PCG K1, tiled
 // p = z + beta p_old (CgK1S's staged value), Ap = L p (pinned rows 0), block partial of p . Ap
 // (jax_iterative_solvers.py:105-107), for a pinned non-periodic grid: a block owns FC x FR
-// nodes, loads z, p_old and the pins on the window one wider in one batch.
-template <int FC, int FR>
+// nodes, loads z, p_old and the pins on the window one wider in one batch.  ZT: the type z is
+// stored in (the V-cycle's), widened to FP64 on load.
+template <int FC, int FR, class ZT>
 __global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restrict__ Le, const uint8_t* __restrict__ pin,
-                                                const double* __restrict__ z, const double* __restrict__ pold,
+                                                const ZT* __restrict__ z, const double* __restrict__ pold,
                                                 double* __restrict__ pnew, double* __restrict__ Ap,
                                                 const DevState* __restrict__ st, double* __restrict__ partials,
                                                 int pstride, const int* __restrict__ active) {
@@ -639,7 +642,7 @@ __global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restri
     struct ZP {
         double z, p;
         uint8_t pin;
-    };
+    };  // z widened on load
     constexpr int WP = Win<WR, WW, ZP>::HP;
     __shared__ double ps[WP][WW];
     __shared__ uint8_t pns[WP][WW];
@@ -648,7 +651,7 @@ __global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restri
     const int j0 = (lb / nbx) * FR, i0 = (lb - (lb / nbx) * nbx) * FC;
     const double beta = st->c_beta;
     Win<WR, WW, ZP> w;
-    w.load(g, i0 - 1, j0 - 1, [&](int k) { return ZP{z[k], pold[k], pin[k]}; });
+    w.load(g, i0 - 1, j0 - 1, [&](int k) { return ZP{(double)z[k], pold[k], pin[k]}; });
     w.store([&](int a, int b, bool in, const ZP& v) {
         ps[a][b] = in ? v.z + beta * v.p : 0.0;
         pns[a][b] = v.pin;
