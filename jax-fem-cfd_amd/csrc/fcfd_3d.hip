@@ -278,7 +278,8 @@ void build_ctx3(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
     c.c_x = dalloc<double>(c, nn);
     c.c_rhs = dalloc<double>(c, nn);
     c.ustar = dalloc<double>(c, n3);
-    c.pstride = blocks_for((int)n3) + 1;
+    const dim3 tiles = s3t_tiles(c.g3);  // blocks of the box-tiled stencil kernels
+    c.pstride = std::max(blocks_for((int)n3), (int)(tiles.x * tiles.y * tiles.z)) + 1;
     c.partials = dalloc<double>(c, (size_t)MAX_RED * c.pstride);
     c.st = dalloc<DevState>(c, 1);
     FCFD_HIP(hipMemset(c.st, 0, sizeof(DevState)));
@@ -443,8 +444,8 @@ struct BInit3 {
 // rows are applied to the staged stencil directly, one element at a time (a rolled loop keeps the
 // live set to one row; the element id and validity are recomputed per element so that no private
 // array is indexed at run time)
-template <class Op>
-__device__ __forceinline__ void op_apply3s(const Op& op, const StageView3<3>& sv, int n, double (&y)[3]) {
+template <class Op, class SV>
+__device__ __forceinline__ void op_apply3s(const Op& op, const SV& sv, int n, double (&y)[3]) {
     const Grid3D& g = op.g;
     const int kz = n / g.nxy, rr = n - kz * g.nxy;
     const int jy = rr / g.nnx, ix = rr - jy * g.nnx;
@@ -479,8 +480,6 @@ struct BK1S3 {
     const double *r, *pold, *qold, *rhat;
     double *pnew, *qnew;
     const DevState* st;
-    __device__ __forceinline__ int nnx() const { return op.g.nnx; }
-    __device__ __forceinline__ int nxy() const { return op.g.nxy; }
     __device__ __forceinline__ void stage(int id, double (&v)[3]) const {
         const double beta = st->b_beta, omega = st->b_omega;
         const int nn = op.g.nn;
@@ -490,7 +489,8 @@ struct BK1S3 {
             v[c] = r[i] + beta * (pold[i] - omega * qold[i]);
         }
     }
-    __device__ void operator()(int n, const StageView3<3>& sv, double (&acc)[1]) const {
+    template <class SV>
+    __device__ void operator()(int n, const SV& sv, double (&acc)[1]) const {
         const int nn = op.g.nn;
         double y[3];
         op_apply3s(op, sv, n, y);
@@ -514,15 +514,14 @@ struct BK2S3 {
     const double *r, *q;
     double *sv_, *tv;
     const DevState* st;
-    __device__ __forceinline__ int nnx() const { return op.g.nnx; }
-    __device__ __forceinline__ int nxy() const { return op.g.nxy; }
     __device__ __forceinline__ void stage(int id, double (&v)[3]) const {
         const double alpha = st->b_alpha;
         const int nn = op.g.nn;
 #pragma unroll
         for (int c = 0; c < 3; ++c) v[c] = r[id + c * nn] - alpha * q[id + c * nn];
     }
-    __device__ void operator()(int n, const StageView3<3>& sv, double (&acc)[3]) const {
+    template <class SV>
+    __device__ void operator()(int n, const SV& sv, double (&acc)[3]) const {
         const int nn = op.g.nn;
         double y[3];
         op_apply3s(op, sv, n, y);
@@ -541,10 +540,11 @@ struct BK2S3 {
 };
 
 template <int NR, class Body, class Fin>
-static void launch_stencil3(Ctx& c, int n, const int* active, const Body& body, const Fin& fin) {
-    const int nb = blocks_for(n);
+static void launch_stencil3(Ctx& c, const Grid3D& g, const int* active, const Body& body, const Fin& fin) {
+    const dim3 t = s3t_tiles(g);
+    const int nb = (int)(t.x * t.y * t.z);
     if (NR > 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-    stencil3_kernel<NR><<<nb, BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
+    stencil3t_kernel<NR><<<nb, BLOCK, 0, c.stream>>>(g, (int)t.x, (int)t.y, active, body, c.partials, c.pstride);
     if constexpr (NR > 0)
         finalize_kernel<NR><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, fin, c.partials, c.pstride);
 }
@@ -567,11 +567,11 @@ static int solve_momentum3_t(Ctx& c, S ce, const double* U_n, const double* P_n,
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_stencil3<1>(c, nn, act,
+            launch_stencil3<1>(c, g, act,
                                BK1S3<MomCoef3<S>>{lhs, c.dmask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw],
                                                   c.m_q[nw], c.st},
                                BK1Fin{c.st});
-            launch_stencil3<3>(c, nn, act, BK2S3<MomCoef3<S>>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
+            launch_stencil3<3>(c, g, act, BK2S3<MomCoef3<S>>{lhs, c.dmask, c.m_r, c.m_q[nw], c.m_s, c.m_t, c.st},
                                BK2Fin{c.st});
             launch_node<2>(c, nn, act, BK3<3>{nn, c.m_p[nw], c.m_s, c.m_t, c.m_rhat, c.m_x, c.m_r, c.st},
                            BK3Fin{c.st});
@@ -619,10 +619,9 @@ struct Proj3S {
     const uint8_t* mask;
     double* Uout;
     double c;
-    __device__ __forceinline__ int nnx() const { return g.nnx; }
-    __device__ __forceinline__ int nxy() const { return g.nxy; }
     __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = P[id]; }
-    __device__ void operator()(int n, const StageView3<1>& sv, double (&)[1]) const {
+    template <class SV>
+    __device__ void operator()(int n, const SV& sv, double (&)[1]) const {
         Nbr3 s;
         node_elems3(g, n, s);
         double X[27];
@@ -715,10 +714,9 @@ struct CgK1S3 {
     static constexpr int NV = 1;
     Grid3D g; const double* Le; const uint8_t* pin; const double* z; const double* pold; double* pnew; double* Ap;
     const DevState* st;
-    __device__ __forceinline__ int nnx() const { return g.nnx; }
-    __device__ __forceinline__ int nxy() const { return g.nxy; }
     __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = z[id] + st->c_beta * pold[id]; }
-    __device__ void operator()(int n, const StageView3<1>& sv, double (&acc)[1]) const {
+    template <class SV>
+    __device__ void operator()(int n, const SV& sv, double (&acc)[1]) const {
         Nbr3 s;
         node_elems3(g, n, s);
         double X[27];
@@ -732,10 +730,9 @@ struct CgK1S3 {
 struct ApproxInv3aS {
     static constexpr int NV = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
-    __device__ __forceinline__ int nnx() const { return g.nnx; }
-    __device__ __forceinline__ int nxy() const { return g.nxy; }
     __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = r[id] / Ld[id]; }
-    __device__ void operator()(int n, const StageView3<1>& sv, double (&)[1]) const {
+    template <class SV>
+    __device__ void operator()(int n, const SV& sv, double (&)[1]) const {
         Nbr3 s;
         node_elems3(g, n, s);
         double X[27];
@@ -748,10 +745,9 @@ struct ApproxInv3bK3S {
     static constexpr int NV = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l;
     double* z;
-    __device__ __forceinline__ int nnx() const { return g.nnx; }
-    __device__ __forceinline__ int nxy() const { return g.nxy; }
     __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = d1l[id]; }
-    __device__ void operator()(int n, const StageView3<1>& sv, double (&acc)[2]) const {
+    template <class SV>
+    __device__ void operator()(int n, const SV& sv, double (&acc)[2]) const {
         Nbr3 s;
         node_elems3(g, n, s);
         double X[27];
@@ -802,7 +798,7 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
     auto pair = [&]() {
         for (int o = 0; o < 2; ++o) {
             const int nw = o ^ 1;
-            launch_stencil3<1>(c, nn, act, CgK1S3{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
+            launch_stencil3<1>(c, g, act, CgK1S3{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
                                CgK1Fin{c.st});
             if (c.precond == FCFD_PC_MULTIGRID) {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
@@ -812,8 +808,8 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
                                CgK3Fin{c.st, flex});
             } else if (c.precond == FCFD_PC_APPROX_INV) {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
-                launch_stencil3<0>(c, nn, act, ApproxInv3aS{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x}, NoFin{});
-                launch_stencil3<2>(c, nn, act, ApproxInv3bK3S{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x, c.c_z[nw]},
+                launch_stencil3<0>(c, g, act, ApproxInv3aS{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x}, NoFin{});
+                launch_stencil3<2>(c, g, act, ApproxInv3bK3S{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x, c.c_z[nw]},
                                    CgK3Fin{c.st, 0});
             } else {
                 launch_node<2>(c, nn, act,
@@ -921,11 +917,11 @@ void four_step3(Ctx& c, const double* U_n, const double* P_n, double dt, double 
     scatter_ud(c, U_d);
     build_ce3(c, U_n, 0.5);
     it[0] = solve_momentum3(c, U_n, P_n, rho, mu, dt, F, c.m_x);
-    launch_stencil3<0>(c, c.g3.nn, nullptr,
+    launch_stencil3<0>(c, c.g3, nullptr,
                        Proj3S{c.g3, c.ec3_dev, c.m_x, P_n, c.Mlump, nullptr, nullptr, c.ustar, dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
     it[1] = solve_pressure3(c, c.ustar, P_n, F_p, rho, dt, P);
-    launch_stencil3<0>(c, c.g3.nn, nullptr,
+    launch_stencil3<0>(c, c.g3, nullptr,
                        Proj3S{c.g3, c.ec3_dev, c.ustar, P, c.Mlump, c.ud_full, c.dmask, U, -dt / rho}, NoFin{});
     FCFD_CHECK_LAUNCH();
 }

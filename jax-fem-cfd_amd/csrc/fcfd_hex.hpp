@@ -118,67 +118,6 @@ __device__ __forceinline__ double ebe3(const Nbr3& s, const double* __restrict__
     return acc;
 }
 
-// ---------------------------------------------------------------------------------------------
-// LDS-staged 27-point kernels.  The neighbours of the nodes [n0, n0 + BLOCK) of a block are the
-// linear ids n + dz*nxy + dy*nnx + dx, which lie in nine contiguous windows
-// [n0 + dz*nxy + dy*nnx - 1, ... + BLOCK + 1).  The block stages those windows once (coalesced
-// loads; the body forms the staged value, e.g. z + beta p, once per entry) and every node reads
-// its stencil from LDS.  Ids that wrap across a row or layer end belong only to missing
-// elements (masked by has[k]); ids outside [0, nn) are staged as 0.
-// ---------------------------------------------------------------------------------------------
-constexpr int W3 = BLOCK + 2;
-
-template <int NV>
-struct StageView3 {
-    const double (*s)[9][W3];
-    int n0;
-    // stencil slot t = ((dz+1)*3 + (dy+1))*3 + (dx+1) of node n
-    __device__ __forceinline__ double at(int v, int n, int t) const { return s[v][t / 3][n - n0 + (t % 3)]; }
-};
-
-// Body: static constexpr int NV; int nnx() const; int nxy() const;
-//       void stage(int id, double (&v)[NV]) const;
-//       void operator()(int n, const StageView3<NV>&, double (&acc)[NR]) const.
-template <int NR, class Body>
-__global__ __launch_bounds__(BLOCK) void stencil3_kernel(int n, const int* __restrict__ active, Body body,
-                                                        double* __restrict__ partials, int pstride) {
-    if (active && *active == 0) return;
-    constexpr int NV = Body::NV;
-    __shared__ double sm[NV][9][W3];
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int n0 = lb * BLOCK;
-    const int nnx = body.nnx(), nxy = body.nxy();
-    // all of a thread's staging loads are issued before the first LDS store (a fixed trip count,
-    // unrolled), so their latencies overlap instead of adding up
-    constexpr int NIT = (9 * W3 + BLOCK - 1) / BLOCK;
-    double v[NIT][NV];
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int idx = threadIdx.x + it * BLOCK;
-        const int w = idx / W3, off = idx - w * W3;
-        const int id = n0 + (w / 3 - 1) * nxy + (w % 3 - 1) * nnx - 1 + off;
-        const bool in = idx < 9 * W3 && id >= 0 && id < n;
-        body.stage(in ? id : 0, v[it]);
-#pragma unroll
-        for (int q = 0; q < NV; ++q) v[it][q] = in ? v[it][q] : 0.0;
-    }
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int idx = threadIdx.x + it * BLOCK;
-        if (idx < 9 * W3) {
-            const int w = idx / W3, off = idx - w * W3;
-#pragma unroll
-            for (int q = 0; q < NV; ++q) sm[q][w][off] = v[it][q];
-        }
-    }
-    __syncthreads();
-    double acc[NR > 0 ? NR : 1];
-#pragma unroll
-    for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
-    const int k = n0 + threadIdx.x;
-    if (k < n) body(k, StageView3<NV>{sm, n0}, acc);
-    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
-}
 
 // incident elements of node n (ids clamped to 0 where missing)
 __device__ __forceinline__ void node_elems3(const Grid3D& g, int n, Nbr3& s) {
@@ -193,10 +132,79 @@ __device__ __forceinline__ void node_elems3(const Grid3D& g, int n, Nbr3& s) {
     }
 }
 
-template <int NV>
-__device__ __forceinline__ void view27(const StageView3<NV>& sv, int v, int n, double (&X)[27]) {
+template <class SV>
+__device__ __forceinline__ void view27(const SV& sv, int v, int n, double (&X)[27]) {
 #pragma unroll
     for (int t = 0; t < 27; ++t) X[t] = sv.at(v, n, t);
+}
+
+// ---------------------------------------------------------------------------------------------
+// LDS-staged 27-point kernels over box tiles.  A block owns S3T_X x S3T_Y x S3T_Z nodes (one per
+// thread) and stages the box one node wider (18 x 6 x 6 = 2.5 staged values per node; the body
+// forms the staged value, e.g. z + beta p, once per window node): every node then reads its
+// stencil from LDS.  The 3 x 3 (dy, dz) neighbour rows of 256 consecutive nodes would stage 9
+// values per node and measured 6-15% slower per 3D step; flatter boxes (64 x 2 x 2, 32 x 8 x 1,
+// 32 x 4 x 2) were slower than 16 x 4 x 4 too.  Window nodes outside the grid stage 0: the
+// bodies mask the elements a node lacks with selects, so the values at such stencil points never
+// enter a result.  Each thread's view holds the LDS address of its stencil's first point, so the
+// 27 reads are compile-time offsets.
+// Body: static constexpr int NV; void stage(int id, double (&v)[NV]) const;
+//       template <class SV> void operator()(int n, const SV& view, double (&acc)[NR]) const,
+//       reading stencil slot t = ((dz+1)*3 + (dy+1))*3 + (dx+1) of value v as view.at(v, n, t).
+constexpr int S3T_X = 16, S3T_Y = 4, S3T_Z = 4;
+template <int NV, int WX, int WY, int WN>
+struct StageView3T {
+    const double* b;  // this thread's stencil point (dx, dy, dz) = (-1, -1, -1) in the window of value 0
+    __device__ __forceinline__ double at(int v, int, int t) const {
+        return b[v * WN + (t / 9) * (WX * WY) + ((t / 3) % 3) * WX + (t % 3)];
+    }
+};
+template <int NR, class Body>
+__global__ __launch_bounds__(BLOCK) void stencil3t_kernel(Grid3D g, int ntx, int nty, const int* __restrict__ active,
+                                                         Body body, double* __restrict__ partials, int pstride) {
+    constexpr int TX = S3T_X, TY = S3T_Y, TZ = S3T_Z;
+    static_assert(TX * TY * TZ == BLOCK, "one node per thread");
+    constexpr int NV = Body::NV, WX = TX + 2, WY = TY + 2, WZ = TZ + 2, WN = WX * WY * WZ;
+    constexpr int NIT = (WN + BLOCK - 1) / BLOCK;
+    const int go = gate_read(active);
+    __shared__ double sm[NV * WN];
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int tz = lb / (ntx * nty), trem = lb - tz * (ntx * nty), ty = trem / ntx, tx = trem - ty * ntx;
+    const int i0 = tx * TX - 1, j0 = ty * TY - 1, k0 = tz * TZ - 1;  // window origin
+    // every staging load of the thread is issued before the first LDS store
+    double v[NIT][NV];
+    int at[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int w = threadIdx.x + it * BLOCK;
+        const int wz = w / (WX * WY), wr = w - wz * (WX * WY), wy = wr / WX, wx = wr - wy * WX;
+        const int i = i0 + wx, j = j0 + wy, k = k0 + wz;
+        const bool in = w < WN && i >= 0 && i < g.nnx && j >= 0 && j < g.nny && k >= 0 && k < g.nnz;
+        at[it] = w < WN ? w : -1;
+        body.stage(in ? (k * g.nny + j) * g.nnx + i : 0, v[it]);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[it][q] = in ? v[it][q] : 0.0;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it)
+        if (at[it] >= 0) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sm[q * WN + at[it]] = v[it][q];
+        }
+    __syncthreads();
+    if (!gate_open(go)) return;
+    double acc[NR > 0 ? NR : 1];
+#pragma unroll
+    for (int r = 0; r < (NR > 0 ? NR : 1); ++r) acc[r] = 0.0;
+    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
+    const int i = i0 + 1 + lx, j = j0 + 1 + ly, k = k0 + 1 + lz;
+    if (i < g.nnx && j < g.nny && k < g.nnz)
+        body((k * g.nny + j) * g.nnx + i, StageView3T<NV, WX, WY, WN>{sm + (lz * WY + ly) * WX + lx}, acc);
+    if constexpr (NR > 0) reduce_epilogue<NR>(acc, partials, pstride, lb);
+}
+// the grid of box tiles of a level
+inline dim3 s3t_tiles(const Grid3D& g) {
+    return dim3((g.nnx + S3T_X - 1) / S3T_X, (g.nny + S3T_Y - 1) / S3T_Y, (g.nnz + S3T_Z - 1) / S3T_Z);
 }
 
 // Row-blocked 8 x 8 storage: entry (a, b) of element e at ((a * ne + e) * 8 + b); a node reads

@@ -95,13 +95,12 @@ struct Jacobi3 {
 struct DownRes3S {
     static constexpr int NV = 1;
     Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* x; double* f;
-    __device__ __forceinline__ int nnx() const { return g.nnx; }
-    __device__ __forceinline__ int nxy() const { return g.nxy; }
     __device__ __forceinline__ void stage(int id, double (&v)[1]) const {
         const double xv = (2.0 / 3.0 * r[id]) / Ld[id];
         v[0] = pin[id] ? 0.0 : xv;
     }
-    __device__ void operator()(int n, const StageView3<1>& sv, double (&)[1]) const {
+    template <class SV>
+    __device__ void operator()(int n, const SV& sv, double (&)[1]) const {
         Nbr3 s;
         node_elems3(g, n, s);
         double X[27];
@@ -117,13 +116,12 @@ struct Jacobi3S {
     static constexpr int NV = 1;
     Grid3D gf, gc; const double* Leoff; const double* Ld; const uint8_t* pin; const double* r; const double* x;
     const double* yc; double* y;
-    __device__ __forceinline__ int nnx() const { return gf.nnx; }
-    __device__ __forceinline__ int nxy() const { return gf.nxy; }
     __device__ __forceinline__ void stage(int id, double (&v)[1]) const {
         const double xv = x[id] + interp3_at(gf, gc, yc, 0.0, id);
         v[0] = pin[id] ? 0.0 : xv;
     }
-    __device__ void operator()(int n, const StageView3<1>& sv, double (&)[1]) const {
+    template <class SV>
+    __device__ void operator()(int n, const SV& sv, double (&)[1]) const {
         Nbr3 s;
         node_elems3(gf, n, s);
         double X[27];
@@ -135,8 +133,10 @@ struct Jacobi3S {
     }
 };
 template <class Body>
-static void launch_stencil3_nored(Ctx& c, int n, const int* active, const Body& body) {
-    stencil3_kernel<0><<<blocks_for(n), BLOCK, 0, c.stream>>>(n, active, body, c.partials, c.pstride);
+static void launch_stencil3_nored(Ctx& c, const Grid3D& g, const int* active, const Body& body) {
+    const dim3 t = s3t_tiles(g);
+    stencil3t_kernel<0><<<t.x * t.y * t.z, BLOCK, 0, c.stream>>>(g, (int)t.x, (int)t.y, active, body, c.partials,
+                                                                 c.pstride);
 }
 
 // ------------------------------------------------------------------------------ coarse level
@@ -296,7 +296,7 @@ const double* vcycle3(Ctx& c, const double* r0, const int* active) {
             run(n, DownRes3{L.g3, L.ec3_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f, c.st, l, neu},
                 MeanFin{&c.st->mg[l][MG_F], nullptr, n});
         else
-            launch_stencil3_nored(c, n, active, DownRes3S{L.g3, L.ec3_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f});
+            launch_stencil3_nored(c, L.g3, active, DownRes3S{L.g3, L.ec3_dev->Le, rl(l), L.Ld, L.pin, L.x, L.f});
         run(Lc.g3.nn, Restrict3{L.g3, Lc.g3, L.f, Lc.r, c.st, l, neu},
             MeanFin{&c.st->mg[l][MG_C], &c.st->mg[l + 1][MG_R], Lc.g3.nn});
     }
@@ -324,7 +324,7 @@ const double* vcycle3(Ctx& c, const double* r0, const int* active) {
             run(n, Jacobi3{L.g3, Lc.g3, L.ec3_dev->Leoff, L.Ld, L.pin, rl(l), L.x, L.f, Lc.y, L.y, c.st, l, neu},
                 MeanFin{&c.st->mg[l][MG_Y], nullptr, n});
         else
-            launch_stencil3_nored(c, n, active,
+            launch_stencil3_nored(c, L.g3, active,
                                   Jacobi3S{L.g3, Lc.g3, L.ec3_dev->Leoff, L.Ld, L.pin, rl(l), L.x, Lc.y, L.y});
     }
     FCFD_CHECK_LAUNCH();
