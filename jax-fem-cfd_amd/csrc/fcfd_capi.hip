@@ -600,8 +600,7 @@ StepLaunch four_step_enqueue(Ctx& c, int slot, const double* U_n, const double* 
         build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
         solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x, &L.mom, ChunkEnd{rec, 0});
         ustar_pressure_solve(c, c.m_x, P_n, F_p, rho, dt, &L.p, ChunkEnd{rec, 1});
-        pressure_output(c, P);
-        projection_final(c, c.ustar, P, rho, dt, U);
+        output_and_project(c, P, rho, dt, U);
     }
     FCFD_CHECK_LAUNCH();
     FCFD_HIP(hipEventRecord(c.ev_spec[slot], c.stream));
@@ -629,8 +628,7 @@ bool four_step_finish(Ctx& c, const StepLaunch& L, const double* P_n, double dt,
             it[0] = mom_it;
             it[1] = pressure_resume(c, L.p);
         }
-        pressure_output(c, P);
-        projection_final(c, c.ustar, P, rho, dt, U);
+        output_and_project(c, P, rho, dt, U);
         FCFD_CHECK_LAUNCH();
     }
     spec_push(c.hist_mom, it[0]);

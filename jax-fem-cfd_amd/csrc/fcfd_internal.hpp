@@ -427,6 +427,8 @@ int pressure_resume(Ctx& c, int launched);
 void pressure_output(Ctx& c, double* P_out);
 void projection_ustar(Ctx& c, const double* U_hat, const double* P_n, double rho, double dt, double* U_star);
 void projection_final(Ctx& c, const double* U_star, const double* P, double rho, double dt, double* U);
+// pressure_output then projection_final from c.ustar (one tiled pass for the pinned form)
+void output_and_project(Ctx& c, double* P_out, double rho, double dt, double* U);
 
 
 // 3D (hex8) path (fcfd_3d.hip)

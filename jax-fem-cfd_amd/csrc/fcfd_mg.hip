@@ -1584,6 +1584,22 @@ void pressure_output(Ctx& c, double* P_out) {
     FCFD_CHECK_LAUNCH();
 }
 
+// The step's last two passes: the pressure output and the final projection from c.ustar, as one
+// tiled kernel for the pinned form on non-periodic grids
+void output_and_project(Ctx& c, double* P_out, double rho, double dt, double* U) {
+    if (c.set_zeroP && !c.g.periodic && c.ppin && c.dmask) {
+        const Grid2D& g = c.g;
+        constexpr int FC = MOMT_C, FR = MOMT_R;
+        const dim3 grid((g.nnx + FC - 1) / FC, (g.nny + FR - 1) / FR);
+        k_final_t<FC, FR><<<grid, 256, 0, c.stream>>>(g, c.ec_dev, c.c_x, c.ppin, c.ustar, c.Mlump, c.ud_full,
+                                                      c.dmask, dt / rho, P_out, U, c.gate);
+        FCFD_CHECK_LAUNCH();
+        return;
+    }
+    pressure_output(c, P_out);
+    projection_final(c, c.ustar, P_out, rho, dt, U);
+}
+
 int solve_pressure(Ctx& c, const double* U_star, const double* P_n, const double* F_p, double rho, double dt,
                    double* P_out) {
     const int it = pressure_solve_only(c, U_star, P_n, F_p, rho, dt);

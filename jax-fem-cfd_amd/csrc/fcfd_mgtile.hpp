@@ -782,4 +782,60 @@ __global__ __launch_bounds__(256) void k_cginit_us(Grid2D g, const ElemConst* __
     reduce_epilogue<3>(acc, partials, pstride, lb);
 }
 
+// ------------------------------------------------------------------------------ output + projection, tiled
+// The pinned form's pressure output (PoutBody: P = the solution with the pins zeroed) and the
+// final projection U = U* - (dt/rho) G P / M with the Dirichlet values restored (FinalBody,
+// standard.py:159-163) in one pass, for a non-periodic grid: a block owns FC x FR nodes, stages
+// the solution x with the pins zeroed on the window one wider -- that is P -- stores P for its
+// own nodes and forms G P from the window.  The operands and order are those of PoutBody and
+// FinalBody, so P and U are bitwise theirs; P is written only here, so it may alias P_n.
+template <int FC, int FR>
+__global__ __launch_bounds__(256) void k_final_t(Grid2D g, const ElemConst* __restrict__ ec,
+                                                 const double* __restrict__ x, const uint8_t* __restrict__ pin,
+                                                 const double* __restrict__ Us, const double* __restrict__ M,
+                                                 const double* __restrict__ ud, const uint8_t* __restrict__ mask,
+                                                 double c, double* __restrict__ P, double* __restrict__ U,
+                                                 const int* __restrict__ gate) {
+    static_assert(FC * FR == BLOCK, "one node per thread");
+    if (gate && *gate == 0) return;
+    constexpr int WW = FC + 2, WR = FR + 2;
+    struct XP {
+        double x;
+        uint8_t p;
+    };
+    constexpr int WP = Win<WR, WW, XP>::HP;
+    __shared__ double ps[WP][WW];
+    const int nn = g.nn;
+    const int nbx = gridDim.x;
+    const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    const int j0 = (lb / nbx) * FR, i0 = (lb - (lb / nbx) * nbx) * FC;
+    const int bi = threadIdx.x % FC, bj = threadIdx.x / FC;
+    const int i = i0 + bi, j = j0 + bj;
+    const bool own = i < g.nnx && j < g.nny;
+    const int n = own ? j * g.nnx + i : 0;
+    Win<WR, WW, XP> w;
+    w.load(g, i0 - 1, j0 - 1, [&](int k) { return XP{x[k], pin[k]}; });
+    const double us0 = Us[n], us1 = Us[n + nn], m0 = M[n], m1 = M[n + nn];
+    const double ud0 = ud[n], ud1 = ud[n + nn];
+    const uint8_t k0 = mask[n], k1 = mask[n + nn];
+    w.store([&](int a, int b, bool in, const XP& v) { ps[a][b] = in ? (v.p ? 0.0 : v.x) : 0.0; });
+    __syncthreads();
+    if (own) {
+        double X[3][3];
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) X[dy][dx] = ps[bj + dy][bi + dx];
+        P[n] = X[1][1];
+        const Has4 hs = has_at(g, i, j);
+        const double gx = ebe_has(hs, ec->Gx, X), gy = ebe_has(hs, ec->Gy, X);
+        double u = us0 - (c * gx) / m0;
+        double v = us1 - (c * gy) / m1;
+        if (k0) u = ud0;
+        if (k1) v = ud1;
+        U[n] = u;
+        U[n + nn] = v;
+    }
+}
+
 }  // namespace fcfd
