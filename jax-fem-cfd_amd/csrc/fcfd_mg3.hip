@@ -89,7 +89,7 @@ struct Jacobi3 {
     }
 };
 
-// LDS-staged forms of the pinned DownRes3 / Jacobi3 (stencil3_kernel): the pre-smoothed x, and
+// LDS-staged forms of the pinned DownRes3 / Jacobi3 (stencil3t_kernel): the pre-smoothed x, and
 // x + P yc, are formed once per window entry with the node kernels' operands (interp3_at is
 // bitwise interp27), and every node reads its 27 values from LDS.
 struct DownRes3S {
