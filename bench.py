@@ -226,6 +226,92 @@ print(json.dumps(dict(sec=sec, steps={steps}, its=its)))
     return r
 
 
+def taylor_green_64(with_cpu):
+    """Config 1: the 2D periodic Taylor-Green vortex on [0, 2 pi]^2, 64x64 Q1 (65^2 nodes, periodic),
+    nu = 0.01, dt = 1e-3, 100 steps of the 4-step projection from t = 0, FP64, MG-PCG and BiCGSTAB
+    tol 1e-10 (examples/taylor_green.py).  GPU: 100 steps through the native step loop after an
+    untimed 10-step warm-up from the same state, with the error of the fields against the exact
+    decaying vortex.  CPU (the cpu_baseline leg): the oracle runs the same 100 steps from the same
+    state in a subprocess, and the GPU fields are compared with its fields."""
+    import tempfile
+    import torch
+    import jax_fem_cfd.setup.simulation as sim
+    n, L, rho, mu, dt, steps = 64, 2 * np.pi, 1.0, 0.01, 1e-3, 100
+    cfg = sim.Config(mesh="simple", solver="standard",
+                     mesh_config=sim.SimpleMeshConfig([n, n], [L, L], 1, True),
+                     solver_config=sim.StandardSolverConfig([], [], [], False, 1e-10, False, "multigrid", [2, 2]))
+    compute, timestep, coords, surf, h, nn, gamma_d, U_d = sim.setup(cfg)
+
+    def exact(t):
+        x, y = coords[:, 0], coords[:, 1]
+        f = np.exp((-2 * mu / rho) * t)
+        u, v = np.sin(x) * np.cos(y) * f, -np.cos(x) * np.sin(y) * f
+        p = (rho / 4) * (np.cos(2 * x) + np.cos(2 * y)) * f * f
+        return np.concatenate([u, v]), p - p.mean()
+
+    U0, P0 = exact(0.0)
+    F_p, F = compute(U_d, None, 0, 0)
+    ctx = timestep.context
+    U = torch.as_tensor(U0, device="cuda")
+    P = torch.as_tensor(P0, device="cuda")
+    ctx.four_step_rollout(U, P, 10, dt, rho, mu, None, F_p, F)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Ug, Pg, its = ctx.four_step_rollout(U, P, steps, dt, rho, mu, None, F_p, F)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    Ug, Pg = Ug.cpu().numpy(), Pg.cpu().numpy()
+    Ue, Pe = exact(steps * dt)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    out = {"workload": "2D periodic Taylor-Green 64x64 Q1, nu=0.01, dt=1e-3, 100 steps, FP64, tol 1e-10",
+           "timesteps_per_s": steps / sec, "ms_per_step": 1e3 * sec / steps,
+           "mean_iters": [float(np.mean([i[0] for i in its])), float(np.mean([i[1] for i in its]))],
+           "rel_l2_vs_exact": {"u": rel(Ug[:nn], Ue[:nn]), "v": rel(Ug[nn:], Ue[nn:]),
+                               "p": rel(Pg - Pg.mean(), Pe)}}
+    if not with_cpu:
+        return out
+    fd, path = tempfile.mkstemp(suffix=".npz")
+    os.close(fd)
+    np.savez(path, U=U0, P=P0)
+    code = f"""
+import sys, time, json
+sys.path.insert(0, {ROOT!r})
+import numpy as np
+from oracle import ns
+s, *_ = ns.build([{n}, {n}], [{L!r}, {L!r}], True, [], [], 1e-10, False, "multigrid", [2, 2])
+z = np.load({path!r})
+U, P = z["U"], z["P"]
+t0 = time.perf_counter()
+its = []
+for _ in range({steps}):
+    U, P, ui, pi = ns.four_step_timestep(s, U, P, {dt}, {rho}, {mu}, None, 0, 0)
+    its.append((ui, pi))
+sec = time.perf_counter() - t0
+np.savez({path!r}, U=U, P=P)
+print(json.dumps(dict(sec=sec, steps={steps}, its=its)))
+"""
+    cores = len(os.sched_getaffinity(0))
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OPENBLAS_NUM_THREADS=str(cores), MKL_NUM_THREADS=str(cores))
+    try:
+        res = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=600)
+        if res.returncode != 0:
+            out["cpu_baseline"] = dict(error=res.stderr[-400:])
+            return out
+        r = json.loads(res.stdout.strip().splitlines()[-1])
+        z = np.load(path)
+        Uo, Po = z["U"], z["P"]
+    finally:
+        os.unlink(path)
+    out["cpu_baseline"] = {"kind": "port", "timesteps_per_s": r["steps"] / r["sec"], "cores": cores,
+                           "iters_equal": [tuple(map(int, a)) for a in its] == [tuple(a) for a in r["its"]],
+                           "gpu_vs_oracle_rel_l2": {"u": rel(Ug[:nn], Uo[:nn]), "v": rel(Ug[nn:], Uo[nn:]),
+                                                    "p": rel(Pg, Po)}}
+    return out
+
+
 def roofline_only(reps):
     """The dominant kernel alone (for counter runs): one mixed-form timestep to build the FP32
     Ce, `reps` momentum operator applies, then a plain device copy of known size (the
@@ -442,6 +528,7 @@ def main():
     copy_gbs = copy_bandwidth()
     pois64 = poisson_alone("fp64")
     poism = poisson_alone("mixed")
+    tg64 = taylor_green_64(not args.no_cpu)
     grad = None if args.no_grad else gradient_256()
     ch3d = None if args.no_3d else channel_3d(args.steps_3d)
     ch3d_q2 = None if args.no_3d else channel_3d_q2()
@@ -479,6 +566,7 @@ def main():
         "speedup_mixed_vs_fp64": steps_s_m / steps_s_d,
         "mixed_vs_fp64_rel_l2": {"u": rel(Um[:nn], U64[:nn]), "v": rel(Um[nn:], U64[nn:]), "p": rel(Pm, P64)},
         "poisson_1024_neumann": {"fp64": pois64, "mixed": poism},
+        "taylor_green_64": tg64,
         "gradient_256": grad,
         "channel_3d": ch3d,
         "channel_3d_q2": ch3d_q2,
