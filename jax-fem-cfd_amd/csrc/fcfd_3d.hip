@@ -800,17 +800,38 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
             const int nw = o ^ 1;
             launch_stencil3<1>(c, g, act, CgK1S3{g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw], c.c_Ap, c.st},
                                CgK1Fin{c.st});
-            if (c.precond == FCFD_PC_MULTIGRID) {
+            // CgK2's partials of r.r and sum r go to slots 2, 3: without the pure-Neumann V-cycle
+            // (which reads the mean of r) nothing reads r.r before the z-step, whose finalize
+            // takes it (CgK3RFin: CgK2Fin's c_rr, then CgK3Fin; the same sums in the same order)
+            const int nb2 = blocks_for(nn);
+            auto k2 = [&]() {
+                node_kernel<2><<<nb2, BLOCK, 0, c.stream>>>(nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st},
+                                                             c.partials + 2 * (size_t)c.pstride, c.pstride);
+            };
+            if (c.precond == FCFD_PC_MULTIGRID && neu) {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
                 const double* y = precondition3(c, act);
                 const int flex = c.coarse_inv32 != nullptr;
                 launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
                                CgK3Fin{c.st, flex});
+            } else if (c.precond == FCFD_PC_MULTIGRID) {
+                k2();
+                const double* y = precondition3(c, act);
+                const int flex = c.coarse_inv32 != nullptr;
+                node_kernel<2><<<nb2, BLOCK, 0, c.stream>>>(
+                    nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, 0, flex}, c.partials, c.pstride);
+                finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb2, nb2, act, CgK3RFin{c.st, flex},
+                                                                      c.partials, c.pstride);
             } else if (c.precond == FCFD_PC_APPROX_INV) {
-                launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
+                k2();
                 launch_stencil3<0>(c, g, act, ApproxInv3aS{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x}, NoFin{});
-                launch_stencil3<2>(c, g, act, ApproxInv3bK3S{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x, c.c_z[nw]},
-                                   CgK3Fin{c.st, 0});
+                const dim3 t = s3t_tiles(g);
+                const int nb3 = (int)(t.x * t.y * t.z);
+                stencil3t_kernel<2><<<nb3, BLOCK, 0, c.stream>>>(
+                    g, (int)t.x, (int)t.y, act, ApproxInv3bK3S{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x, c.c_z[nw]},
+                    c.partials, c.pstride);
+                finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb3, nb2, act, CgK3RFin{c.st, 0}, c.partials,
+                                                                      c.pstride);
             } else {
                 launch_node<2>(c, nn, act,
                                CgK23{c.c_p[nw], c.c_Ap, c.precond == FCFD_PC_JACOBI ? Ld : nullptr, c.c_x, c.c_r,

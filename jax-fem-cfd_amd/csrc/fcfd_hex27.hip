@@ -1120,12 +1120,25 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
                                CgK23{c.c_p[nw], c.c_Ap, c.precond == FCFD_PC_JACOBI ? Ld : nullptr, c.c_x, c.c_r,
                                      c.c_z[nw], c.st},
                                CgK23Fin{c.st});
-            } else {
+            } else if (neu) {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
                 const double* y = precondition_h27(c, act);
                 const int flex = c.coarse_inv32 != nullptr;
                 launch_node<2>(c, nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, neu, flex},
                                CgK3Fin{c.st, flex});
+            } else {
+                // CgK2's partials of r.r and sum r in slots 2, 3, taken by the z-step's finalize
+                // (CgK3RFin: CgK2Fin's c_rr, then CgK3Fin); without the pure-Neumann V-cycle
+                // nothing reads them before
+                const int nb2 = blocks_for(nn);
+                node_kernel<2><<<nb2, BLOCK, 0, c.stream>>>(nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st},
+                                                             c.partials + 2 * (size_t)c.pstride, c.pstride);
+                const double* y = precondition_h27(c, act);
+                const int flex = c.coarse_inv32 != nullptr;
+                node_kernel<2><<<nb2, BLOCK, 0, c.stream>>>(
+                    nn, act, CgK3<double>{c.c_r, y, c.c_z[nw], c.c_z[o], c.st, 0, flex}, c.partials, c.pstride);
+                finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb2, nb2, act, CgK3RFin{c.st, flex},
+                                                                      c.partials, c.pstride);
             }
         }
         FCFD_CHECK_LAUNCH();
