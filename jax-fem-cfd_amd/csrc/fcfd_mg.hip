@@ -1511,9 +1511,28 @@ int pcg(Ctx& c, const double* b, const double* bmean, const double* x0in, int ze
     return pcg_impl(c, b, bmean, x0in, zero_pins, tol, maxit, converged, zero_exit, prhs, chunk, ce, nullptr);
 }
 
+// x_out = x - mean(x) + st->mean_rhs (the mean of x0, recorded before the solve)
+struct RestoreMeanBody {
+    const double* x; double* out; const DevState* st;
+    __device__ void operator()(int k, double (&)[1]) const { out[k] = (x[k] - st->mean_p) + st->mean_rhs; }
+};
+
 int pressure_cg(Ctx& c, const double* b, double* x, double tol, int maxit, bool* converged) {
+    // The tiled pure-Neumann V-cycle (mg_neu_tiled) leaves out the intermediate mean projections,
+    // so its z carries a constant and the CG iterate drifts by one.  The reference's iterate keeps
+    // the mean of x0 (every search direction has mean zero): the solve records mean(x0) and
+    // restores it.  The timestep's pressure output subtracts the mean anyway.
+    const int nn = c.g.nn;
+    if (c.mg_neu_tiled)
+        launch_node<1>(c, nn, nullptr, SumXBody{x}, MeanFin{&c.st->mean_rhs, nullptr, nn});
     int it = pcg(c, b, nullptr, x, 0, tol, maxit, converged);
-    FCFD_HIP(hipMemcpyAsync(x, c.c_x, sizeof(double) * c.g.nn, hipMemcpyDeviceToDevice, c.stream));
+    if (c.mg_neu_tiled) {
+        launch_node<1>(c, nn, nullptr, SumXBody{c.c_x}, MeanFin{&c.st->mean_p, nullptr, nn});
+        launch_node<0>(c, nn, nullptr, RestoreMeanBody{c.c_x, x, c.st}, NoFin{});
+        FCFD_CHECK_LAUNCH();
+        return it;
+    }
+    FCFD_HIP(hipMemcpyAsync(x, c.c_x, sizeof(double) * nn, hipMemcpyDeviceToDevice, c.stream));
     return it;
 }
 

@@ -207,6 +207,31 @@ def test_pressure_cg_multigrid(set_zeroP):
     assert _rel(_np(x), xr) < 1e-10
 
 
+def test_neumann_pressure_cg_mixed_keeps_the_mean_of_x0():
+    """Pure Neumann, mixed form (the tiled V-cycle without the intermediate mean projections,
+    whose z carries a constant): the solve still returns the reference's iterate, which keeps the
+    mean of x0 -- x0 with a nonzero mean, the raw solution against the oracle's CG from the same
+    x0."""
+    n = 64
+    s, *_ = ns.build([n, n], [1.0, 1.0], False, [], [], 1e-10, False, "multigrid", [8, 8])
+    c = _ctx(n, n, 1.0, 1.0, False, tol=1e-10, set_zeroP=False, precond="multigrid", mg_levels=len(s.levels),
+             precision="mixed", mg_gamma_p=[L["gamma_p"] for L in s.levels])
+    b = RNG.standard_normal(s.nn)
+    b -= b.mean()
+    x0 = RNG.standard_normal(s.nn) + 0.75
+
+    def A(x):
+        return oem.laplacian_element_mult(x, s.Le, s.nconn, s.nn)
+
+    from oracle.solvers import cg
+    xr, kr = cg(A, b, x0=x0.copy(), tol=1e-10, maxiter=1000, M=s.M_op)
+    x, k, conv = c.pressure_cg(b, x0=x0, tol=1e-10, maxiter=1000)
+    xg = _np(x)
+    assert conv
+    assert abs(xg.mean() - x0.mean()) < 1e-12 * abs(x0.mean())
+    assert _rel(xg, xr) < 1e-8, _rel(xg, xr)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_vcycle_and_pressure_cg_at_512(precision):
     """512x512 elements (263,169 nodes), pinned: the sizes at which the V-cycle runs its staged
