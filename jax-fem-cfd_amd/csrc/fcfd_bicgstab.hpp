@@ -595,12 +595,16 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
                 const dim3 grid((c.g.nnx + MOMT_C - 1) / MOMT_C, (c.g.nny + MOMT_R - 1) / MOMT_R);
                 const int nb = (int)(grid.x * grid.y);
                 if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-                k_mom_k1t<decltype(lhs.ce), MOMT_C, MOMT_R><<<grid, BLOCK, 0, c.stream>>>(
-                    lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st, c.partials,
-                    c.pstride, act);
-                finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, BK1Fin{c.st}, c.partials, c.pstride);
-                k_mom_k2t<decltype(lhs.ce), MOMT_C, MOMT_R><<<grid, BLOCK, 0, c.stream>>>(
-                    lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st, c.partials, c.pstride, act);
+                using S = decltype(lhs.ce);
+                with_gate_mode(c, act, [&](auto e) {
+                    constexpr bool E = decltype(e)::value;
+                    k_mom_k1t<S, MOMT_C, MOMT_R, E><<<grid, BLOCK, 0, c.stream>>>(
+                        lhs, mask, c.m_r, c.m_p[o], c.m_q[o], c.m_rhat, c.m_p[nw], c.m_q[nw], c.st, c.partials,
+                        c.pstride, act);
+                    finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, BK1Fin{c.st}, c.partials, c.pstride);
+                    k_mom_k2t<S, MOMT_C, MOMT_R, E ? 2 : 0><<<grid, BLOCK, 0, c.stream>>>(
+                        lhs, mask, c.m_r, c.m_q[nw], c.m_t, c.st, c.partials, c.pstride, act);
+                });
                 finalize_kernel<3><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, BK2Fin{c.st}, c.partials, c.pstride);
                 launch_node<2>(c, nn, act, BK3R<NC>{nn, c.m_p[nw], c.m_q[nw], c.m_t, c.m_rhat, x, c.m_r, c.st},
                                BK3Fin{c.st, ce});
@@ -637,8 +641,15 @@ int bicgstab_iterate(Ctx& c, IterGraph& graph, const double (&key)[4], const voi
     if (chunk) {
         if (*chunk > 0) n = *chunk;
         if (cend.rec) {
-            // the last iteration carries the chunk-end record (launched directly)
-            enqueue_iters(c, graph, key, kptr, launched & 1, n - 1, one, pair);
+            // the last iteration carries the chunk-end record (launched directly); iterations
+            // past the count the host is sure of test their gates at entry as well
+            const int sure = chunk_sure_of(c, n), a = sure < n - 1 ? sure : n - 1;
+            enqueue_iters(c, graph, key, kptr, launched & 1, a, one, pair);
+            {
+                EarlyGates eg(c, true);
+                enqueue_iters(c, graph, key, kptr, (launched + a) & 1, n - 1 - a, one, pair);
+            }
+            EarlyGates eg(c, sure < n);
             one((launched + n - 1) & 1, cend);
         } else {
             enqueue_iters(c, graph, key, kptr, launched & 1, n, one, pair);

@@ -566,6 +566,12 @@ __global__ void k_spec_resume(DevState* st, int phase) {
 int spec_chunk(const Ctx& c, const int (&h)[3]) {
     return c.spec_chunk_fixed > 0 ? c.spec_chunk_fixed : std::max(h[0], h[1]);
 }
+// of such a chunk, the iterations the host is confident of: the smaller of the last two counts
+// (-1, all of them, before two solves or with a fixed chunk length)
+int spec_sure(const Ctx& c, const int (&h)[3]) {
+    if (c.spec_chunk_fixed > 0 || h[0] <= 0 || h[1] <= 0) return -1;
+    return std::min(h[0], h[1]);
+}
 void spec_push(int (&h)[3], int k) {
     h[2] = h[1];
     h[1] = h[0];
@@ -580,7 +586,10 @@ struct StepLaunch {
 struct GateScope {
     Ctx& c;
     explicit GateScope(Ctx& cc) : c(cc) { c.gate = &c.st->go; }
-    ~GateScope() { c.gate = nullptr; }
+    ~GateScope() {
+        c.gate = nullptr;
+        c.chunk_sure = -1;
+    }
 };
 
 // four_step_timestep (standard.py:148-165), enqueued as a speculative step.
@@ -598,7 +607,9 @@ StepLaunch four_step_enqueue(Ctx& c, int slot, const double* U_n, const double* 
     {
         GateScope gate(c);
         build_ce(c, U_n, 0.5, c.ce_format, c.ce, c.ce_scale);
+        c.chunk_sure = spec_sure(c, c.hist_mom);
         solve_momentum(c, c.ce_format, c.ce, c.ce_scale, U_n, P_n, rho, mu, dt, F, c.m_x, &L.mom, ChunkEnd{rec, 0});
+        c.chunk_sure = spec_sure(c, c.hist_p);
         ustar_pressure_solve(c, c.m_x, P_n, F_p, rho, dt, &L.p, ChunkEnd{rec, 1});
         output_and_project(c, P, rho, dt, U);
     }

@@ -166,7 +166,6 @@ __global__ __launch_bounds__(BLOCK) void stencil3t_kernel(Grid3D g, int ntx, int
     static_assert(TX * TY * TZ == BLOCK, "one node per thread");
     constexpr int NV = Body::NV, WX = TX + 2, WY = TY + 2, WZ = TZ + 2, WN = WX * WY * WZ;
     constexpr int NIT = (WN + BLOCK - 1) / BLOCK;
-    const int go = gate_read(active);
     __shared__ double sm[NV * WN];
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int tz = lb / (ntx * nty), trem = lb - tz * (ntx * nty), ty = trem / ntx, tx = trem - ty * ntx;
@@ -174,6 +173,7 @@ __global__ __launch_bounds__(BLOCK) void stencil3t_kernel(Grid3D g, int ntx, int
     // every staging load of the thread is issued before the first LDS store
     double v[NIT][NV];
     int at[NIT];
+    bool inw[NIT];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
         const int w = threadIdx.x + it * BLOCK;
@@ -181,15 +181,16 @@ __global__ __launch_bounds__(BLOCK) void stencil3t_kernel(Grid3D g, int ntx, int
         const int i = i0 + wx, j = j0 + wy, k = k0 + wz;
         const bool in = w < WN && i >= 0 && i < g.nnx && j >= 0 && j < g.nny && k >= 0 && k < g.nnz;
         at[it] = w < WN ? w : -1;
+        inw[it] = in;
         body.stage(in ? (k * g.nny + j) * g.nnx + i : 0, v[it]);
-#pragma unroll
-        for (int q = 0; q < NV; ++q) v[it][q] = in ? v[it][q] : 0.0;
     }
+    // the gate's load goes out behind the staging loads (its kernel-argument load then overlaps them)
+    const int go = gate_read(active);
 #pragma unroll
     for (int it = 0; it < NIT; ++it)
         if (at[it] >= 0) {
 #pragma unroll
-            for (int q = 0; q < NV; ++q) sm[q * WN + at[it]] = v[it][q];
+            for (int q = 0; q < NV; ++q) sm[q * WN + at[it]] = inw[it] ? v[it][q] : 0.0;
         }
     __syncthreads();
     if (!gate_open(go)) return;

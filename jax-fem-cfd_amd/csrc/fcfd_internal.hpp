@@ -117,6 +117,12 @@ struct Ctx {
     // speculative steps: when non-null, the kernels of a step that are not solver iterations take
     // it as their `active` flag (st->go)
     const int* gate = nullptr;
+    // set while the iterations of a chunk past the count the host is confident of are enqueued:
+    // their kernels with the late gate test take the EARLY instantiations (entry test as well)
+    int gate_early = 0;
+    // of the iterations of the next speculative chunk, how many the host is confident of (the
+    // rest are enqueued with early gates); negative: all of them
+    int chunk_sure = -1;
     int hist_mom[3] = {0, 0, 0}, hist_p[3] = {0, 0, 0};  // the last iteration counts
     int spec_chunk_fixed = 0;  // FCFD_SPEC_CHUNK: fixed chunk length of the speculative solves
     IterGraph g_mom, g_cg, g_phi, g_amom, g_aphi;
@@ -228,6 +234,31 @@ T* dalloc(Ctx& c, size_t n) {
     FCFD_HIP(hipMalloc(&p, n * sizeof(T)));
     c.allocs.push_back(p);
     return (T*)p;
+}
+
+// f(std::true_type{}) for a solver-iteration launch on flag `a` that takes the EARLY instantiation
+// (c.gate_early, and a flag), else f(std::false_type{})
+template <class F>
+inline void with_gate_mode(const Ctx& c, const int* a, F&& f) {
+    if (a && c.gate_early)
+        f(std::true_type{});
+    else
+        f(std::false_type{});
+}
+
+// sets c.gate_early for its lifetime
+struct EarlyGates {
+    Ctx& c;
+    int old;
+    EarlyGates(Ctx& cc, bool on) : c(cc), old(cc.gate_early) { c.gate_early = on ? 1 : 0; }
+    ~EarlyGates() { c.gate_early = old; }
+};
+
+// Of a speculative chunk of n iterations, how many to enqueue with the late gate only: the first
+// c.chunk_sure, or all of them when the host has no count to be sure of or replays captured
+// graphs (a captured launch keeps the gate it was captured with).
+inline int chunk_sure_of(const Ctx& c, int n) {
+    return (c.use_graphs || c.chunk_sure < 0) ? n : (c.chunk_sure < n ? c.chunk_sure : n);
 }
 
 // Launch a node kernel over n items and, when it reduces (NR > 0), its finalize kernel.

@@ -794,10 +794,13 @@ static void coarse_solve(Ctx& c, const TR* r, T* y, T* cr, T* cp, T* cAp, double
 // DENSE_ROWS rows: its lanes stride the columns (independent coalesced loads), then a DPP wave
 // sum per row.  ceil(nn / (4 DENSE_ROWS)) blocks of 4 waves.
 constexpr int DENSE_ROWS = 2;
-template <class T>
+template <class T, bool EARLY = false>
 __global__ __launch_bounds__(256) void k_coarse_dense(int nn, const float* __restrict__ Ainv,
                                                       const uint8_t* __restrict__ pin, const T* __restrict__ r,
                                                       T* __restrict__ y, const int* active) {
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
     const int go = gate_read(active);
     const int lane = threadIdx.x & 63;
     const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * DENSE_ROWS;
@@ -880,32 +883,39 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
         if (l + 1 < ld && Lc.g.nny < 200) {
             Level& L2 = c.lv[l + 2];
             const dim3 grid((L2.g.nnx + M2_C - 1) / M2_C, (L2.g.nny + M2_R - 1) / M2_R);
-            if (l == 0)
-                k_mgt_down2<T, double><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le,
-                                                                   ld_tab<T>(L), ld_tab<T>(Lc), r0, pn(L), pn(Lc),
-                                                                   B::r(Lc), B::r(L2), active);
-            else
-                k_mgt_down2<T, T><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le,
-                                                              ld_tab<T>(L), ld_tab<T>(Lc), B::r(L), pn(L), pn(Lc),
-                                                              B::r(Lc), B::r(L2), active);
+            with_gate_mode(c, active, [&](auto e) {
+                constexpr bool E = decltype(e)::value;
+                if (l == 0)
+                    k_mgt_down2<T, double, E><<<grid, 256, 0, c.stream>>>(
+                        L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le, ld_tab<T>(L), ld_tab<T>(Lc), r0, pn(L), pn(Lc),
+                        B::r(Lc), B::r(L2), active);
+                else
+                    k_mgt_down2<T, T, E><<<grid, 256, 0, c.stream>>>(
+                        L.g, Lc.g, L2.g, L.ec_dev->Le, Lc.ec_dev->Le, ld_tab<T>(L), ld_tab<T>(Lc), B::r(L), pn(L),
+                        pn(Lc), B::r(Lc), B::r(L2), active);
+            });
             l += 2;
         } else {
             auto go = [&](auto rl) {
                 using TR = std::remove_const_t<std::remove_pointer_t<decltype(rl)>>;
                 constexpr int CX = MGT_DOWN_CX, CY = MGT_DOWN_CY;
                 const int nbx = (Lc.g.nnx + CX - 1) / CX;
-                k_mgt_down<T, TR, CX, CY><<<dim3(nbx, (Lc.g.nny + CY - 1) / CY), 256, 0, c.stream>>>(
-                    L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), rl, pn(L), B::r(Lc), active);
+                with_gate_mode(c, active, [&](auto e) {
+                    k_mgt_down<T, TR, CX, CY, decltype(e)::value>
+                        <<<dim3(nbx, (Lc.g.nny + CY - 1) / CY), 256, 0, c.stream>>>(
+                            L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), rl, pn(L), B::r(Lc), active);
+                });
             };
             if (l == 0 && tail && tail->cg_p) {
                 constexpr int CX = MGT_DOWN_CX, CY = MGT_DOWN_CY;
                 const dim3 grid((Lc.g.nnx + CX - 1) / CX, (Lc.g.nny + CY - 1) / CY);
                 down_nb = (int)(grid.x * grid.y);
                 if (down_nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-                k_mgt_down_cg<T, CX, CY><<<grid, 256, 0, c.stream>>>(L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), r0,
-                                                                   tail->cg_Ap, tail->cg_p, tail->cg_x,
-                                                                   tail->cg_rout, pn(L), B::r(Lc), c.st, c.partials,
-                                                                   c.pstride, 2, active);
+                with_gate_mode(c, active, [&](auto e) {
+                    k_mgt_down_cg<T, CX, CY, decltype(e)::value><<<grid, 256, 0, c.stream>>>(
+                        L.g, Lc.g, L.ec_dev->Le, ld_tab<T>(L), r0, tail->cg_Ap, tail->cg_p, tail->cg_x,
+                        tail->cg_rout, pn(L), B::r(Lc), c.st, c.partials, c.pstride, 2, active);
+                });
                 r0 = tail->cg_rout;
             } else if (l == 0) {
                 go(r0);
@@ -928,7 +938,10 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
             FCFD_CHECK_LAUNCH();
         } else if (f32 && c.coarse_inv32) {
             const int nb = (C.g.nn + 4 * DENSE_ROWS - 1) / (4 * DENSE_ROWS);
-            k_coarse_dense<T><<<nb, 256, 0, c.stream>>>(C.g.nn, c.coarse_inv32, pn(C), B::r(C), B::y(C), active);
+            with_gate_mode(c, active, [&](auto e) {
+                k_coarse_dense<T, decltype(e)::value><<<nb, 256, 0, c.stream>>>(C.g.nn, c.coarse_inv32, pn(C),
+                                                                                 B::r(C), B::y(C), active);
+            });
             FCFD_CHECK_LAUNCH();
         } else {
             coarse_solve<T, T>(c, B::r(C), B::y(C), B::cr(c), B::cp(c), B::cAp(c), ctol, active);
@@ -940,14 +953,17 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
         if (l >= 1 && c.lv[l - 1].g.nn < (1 << 17)) {
             Level& Lf = c.lv[l - 1];
             const dim3 grid((Lf.g.nnx + U2_C - 1) / U2_C, (Lf.g.nny + U2_R - 1) / U2_R);
-            if (l - 1 == 0)
-                k_mgt_up2<T, double><<<grid, 256, 0, c.stream>>>(Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff,
-                                                                 ld_tab<T>(Lf), ld_tab<T>(L), r0, pn(Lf), B::r(L),
-                                                                 pn(L), B::y(Lc), B::y(Lf), active);
-            else
-                k_mgt_up2<T, T><<<grid, 256, 0, c.stream>>>(Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff,
-                                                            ld_tab<T>(Lf), ld_tab<T>(L), B::r(Lf), pn(Lf), B::r(L),
-                                                            pn(L), B::y(Lc), B::y(Lf), active);
+            with_gate_mode(c, active, [&](auto e) {
+                constexpr bool E = decltype(e)::value;
+                if (l - 1 == 0)
+                    k_mgt_up2<T, double, E><<<grid, 256, 0, c.stream>>>(
+                        Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff, ld_tab<T>(Lf), ld_tab<T>(L), r0, pn(Lf),
+                        B::r(L), pn(L), B::y(Lc), B::y(Lf), active);
+                else
+                    k_mgt_up2<T, T, E><<<grid, 256, 0, c.stream>>>(
+                        Lf.g, L.g, Lc.g, Lf.ec_dev->Leoff, L.ec_dev->Leoff, ld_tab<T>(Lf), ld_tab<T>(L), B::r(Lf),
+                        pn(Lf), B::r(L), pn(L), B::y(Lc), B::y(Lf), active);
+            });
             FCFD_CHECK_LAUNCH();
             l -= 2;
             continue;
@@ -960,9 +976,11 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
             const dim3 grid((L.g.nnx + FC - 1) / FC, (L.g.nny + FR - 1) / FR);
             const int nb = (int)(grid.x * grid.y);
             if (ZM != 0 && nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-            k_mgt_up<T, TR, FC, FR, ZM, ZT><<<grid, 256, 0, c.stream>>>(
-                L.g, Lc.g, L.ec_dev->Leoff, ld_tab<T>(L), rl, pn(L), B::y(Lc), B::y(L), (ZT*)znew, (const ZT*)zold,
-                pzero, flex, c.partials, c.pstride, active);
+            with_gate_mode(c, active, [&](auto e) {
+                k_mgt_up<T, TR, FC, FR, ZM, ZT, decltype(e)::value><<<grid, 256, 0, c.stream>>>(
+                    L.g, Lc.g, L.ec_dev->Leoff, ld_tab<T>(L), rl, pn(L), B::y(Lc), B::y(L), (ZT*)znew,
+                    (const ZT*)zold, pzero, flex, c.partials, c.pstride, active);
+            });
             if constexpr (ZM == 1)
                 finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, active, CgGamma0Fin{c.st}, c.partials,
                                                                   c.pstride);
@@ -1379,13 +1397,17 @@ static void pcg_one(Ctx& c, int o, ChunkEnd ce = ChunkEnd{}) {
             const dim3 grid((g.nnx + FC - 1) / FC, (g.nny + FR - 1) / FR);
             const int nb = (int)(grid.x * grid.y);
             if (nb > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
-            if (z_f32(c))
-                k_cg_k1t<FC, FR, float><<<grid, 256, 0, c.stream>>>(g, Le, pin, (const float*)c.c_z[o], c.c_p[o],
-                                                                    c.c_p[nw], c.c_Ap, c.st, c.partials, c.pstride,
-                                                                    act);
-            else
-                k_cg_k1t<FC, FR, double><<<grid, 256, 0, c.stream>>>(g, Le, pin, c.c_z[o], c.c_p[o], c.c_p[nw],
-                                                                     c.c_Ap, c.st, c.partials, c.pstride, act);
+            with_gate_mode(c, act, [&](auto e) {
+                constexpr bool E = decltype(e)::value;
+                if (z_f32(c))
+                    k_cg_k1t<FC, FR, float, E><<<grid, 256, 0, c.stream>>>(g, Le, pin, (const float*)c.c_z[o],
+                                                                           c.c_p[o], c.c_p[nw], c.c_Ap, c.st,
+                                                                           c.partials, c.pstride, act);
+                else
+                    k_cg_k1t<FC, FR, double, E><<<grid, 256, 0, c.stream>>>(g, Le, pin, c.c_z[o], c.c_p[o],
+                                                                            c.c_p[nw], c.c_Ap, c.st, c.partials,
+                                                                            c.pstride, act);
+            });
             finalize_kernel<1><<<1, FIN_BLOCK, 0, c.stream>>>(nb, act, CgK1Fin{c.st}, c.partials, c.pstride);
         }
         CgTail tail{c.c_z[nw], c.c_z[o], nullptr, flex, 0};
@@ -1429,7 +1451,14 @@ static void pcg_enqueue(Ctx& c, int launched, int n, ChunkEnd ce = ChunkEnd{}) {
         pcg_one(c, 1);
     };
     if (ce.rec) {
-        enqueue_iters(c, c.g_cg, key, nullptr, launched & 1, n - 1, one, pair);
+        // iterations past the count the host is sure of test their gates at entry as well
+        const int sure = chunk_sure_of(c, n), a = sure < n - 1 ? sure : n - 1;
+        enqueue_iters(c, c.g_cg, key, nullptr, launched & 1, a, one, pair);
+        {
+            EarlyGates eg(c, true);
+            enqueue_iters(c, c.g_cg, key, nullptr, (launched + a) & 1, n - 1 - a, one, pair);
+        }
+        EarlyGates eg(c, sure < n);
         pcg_one(c, (launched + n - 1) & 1, ce);
     } else {
         enqueue_iters(c, c.g_cg, key, nullptr, launched & 1, n, one, pair);

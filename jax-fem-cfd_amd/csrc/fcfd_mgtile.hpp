@@ -180,11 +180,13 @@ __device__ __forceinline__ T pinned_jacobi_at(const Grid2D& g, int i, int j, con
 // A block owns CX x CY coarse nodes (2 CX x 2 CY fine ones).  It loads r and the pins of the
 // fine window rows 2 J0 - 2 .. 2 J0 + 2 CY, forms x there, f = r - L x on the window one in, and
 // restricts f into its coarse nodes.
-template <class T, class TR, int CX, int CY>
+template <class T, class TR, int CX, int CY, bool EARLY>
 __global__ __launch_bounds__(256) void k_mgt_down(Grid2D gf, Grid2D gc, const double* __restrict__ Le, LdTab<T> ldt_,
                                                   const TR* __restrict__ r, const uint8_t* __restrict__ pin,
                                                   T* __restrict__ rc, const int* __restrict__ active) {
-    const int go = gate_read(active);
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
     constexpr int XW = 2 * CX + 3, XR = 2 * CY + 3;
     constexpr int FW = 2 * CX + 1, FR = 2 * CY + 1;
     __shared__ T ldl[9];
@@ -199,6 +201,7 @@ __global__ __launch_bounds__(256) void k_mgt_down(Grid2D gf, Grid2D gc, const do
     const int fi0 = 2 * I0 - 2, fj0 = 2 * J0 - 2;
     Win<XR, XW, RPin<TR>> w;
     w.load(gf, fi0, fj0, [&](int k) { return RPin<TR>{r[k], pin[k]}; });
+    const int go = gate_read(active);
     ld_tab_lds(ldt_, ldl);
     __syncthreads();
     if (!gate_open(go)) return;
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(256) void k_mgt_down(Grid2D gf, Grid2D gc, const do
 // (CgK2's operation, alpha from the device state), the block stores r_new of its own fine nodes
 // into r_out (r is double-buffered: neighbouring blocks still read r at their halo) and
 // x += alpha p there, and writes its partial of r_new . r_new into partials[slot * pstride + lb].
-template <class T, int CX, int CY>
+template <class T, int CX, int CY, bool EARLY>
 __global__ __launch_bounds__(256) void k_mgt_down_cg(Grid2D gf, Grid2D gc, const double* __restrict__ Le,
                                                      LdTab<T> ldt_, const double* __restrict__ r,
                                                      const double* __restrict__ Ap, const double* __restrict__ p,
@@ -248,7 +251,9 @@ __global__ __launch_bounds__(256) void k_mgt_down_cg(Grid2D gf, Grid2D gc, const
                                                      const uint8_t* __restrict__ pin, T* __restrict__ rc,
                                                      const DevState* __restrict__ st, double* __restrict__ partials,
                                                      int pstride, int slot, const int* __restrict__ active) {
-    const int go = gate_read(active);
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
     constexpr int XW = 2 * CX + 3, XR = 2 * CY + 3;
     constexpr int FW = 2 * CX + 1, FR = 2 * CY + 1;
     constexpr int OW = 2 * CX, OR = 2 * CY, NO = (OW * OR + BLOCK - 1) / BLOCK;  // owned fine nodes
@@ -279,6 +284,7 @@ __global__ __launch_bounds__(256) void k_mgt_down_cg(Grid2D gf, Grid2D gc, const
         po[m] = p[n];
         xo[m] = xcg[n];
     }
+    const int go = gate_read(active);
     ld_tab_lds(ldt_, ldl);
     __syncthreads();
     if (!gate_open(go)) return;
@@ -338,13 +344,15 @@ __global__ __launch_bounds__(256) void k_mgt_down_cg(Grid2D gf, Grid2D gc, const
 // r_m and x_m on a one-wider m-window, r_m being the restriction of f_l on the (2 (2 M2_R + 3) + 1)-
 // row l-window, hence x_l on a one-wider l-window.  It stores r_m and r_c of the nodes it owns.
 constexpr int M2_R = 2, M2_C = 16;
-template <class T, class TR>
+template <class T, class TR, bool EARLY>
 __global__ __launch_bounds__(256) void k_mgt_down2(Grid2D gf, Grid2D gm, Grid2D gc, const double* __restrict__ Lef,
                                                    const double* __restrict__ Lem, LdTab<T> ldf_, LdTab<T> ldm_,
                                                    const TR* __restrict__ rf, const uint8_t* __restrict__ pinf,
                                                    const uint8_t* __restrict__ pinm, T* __restrict__ rm,
                                                    T* __restrict__ rc, const int* __restrict__ active) {
-    const int go = gate_read(active);
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
     constexpr int XRm = 2 * M2_R + 3, XWm = 2 * M2_C + 3;  // m-window of x_m, r_m
     constexpr int FRm = 2 * M2_R + 1, FWm = 2 * M2_C + 1;  // m-window of f_m
     constexpr int FRl = 2 * XRm + 1, FWl = 2 * XWm + 1;    // l-window of f_l
@@ -368,6 +376,7 @@ __global__ __launch_bounds__(256) void k_mgt_down2(Grid2D gf, Grid2D gm, Grid2D 
     w.load(gf, li0, lj0, [&](int k) { return RPin<TR>{rf[k], pinf[k]}; });
     Win<XRm, XWm, uint8_t> wp;
     wp.load(gm, mi0, mj0, [&](int k) { return pinm[k]; });
+    const int go = gate_read(active);
     ld_tab_lds(ldf_, ldfl);
     ld_tab_lds(ldm_, ldml);
     __syncthreads();
@@ -448,7 +457,7 @@ __global__ __launch_bounds__(256) void k_mgt_down2(Grid2D gf, Grid2D gm, Grid2D 
 // iteration: CgK3's pass): z stored, partials of r.z and (flexible form) r.z_old.  ZT: the type
 // z is stored in -- FP64, or the FP32 V-cycle's own type where the CG's reader widens it on load
 // (the FP64 value an FP64 store of the widened y would hold, at half the bytes).
-template <class T, class TR, int FC, int FR, int ZM, class ZT = T>
+template <class T, class TR, int FC, int FR, int ZM, class ZT, bool EARLY>
 __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const double* __restrict__ Leoff, LdTab<T> ldt_,
                                                 const TR* __restrict__ r, const uint8_t* __restrict__ pin,
                                                 const T* __restrict__ yc, T* __restrict__ y, ZT* __restrict__ znew,
@@ -457,7 +466,9 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
                                                 const int* __restrict__ active) {
     static_assert(FC % 2 == 0 && FR % 2 == 0 && (FC * FR) % BLOCK == 0,
                   "even tile sides (the coarse window starts at tile origin / 2 - 1), whole nodes per thread");
-    const int go = gate_read(active);
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
     constexpr int WW = FC + 2, WR = FR + 2;          // fine window (stencil of the owned nodes)
     constexpr int CW = FC / 2 + 3, CR = FR / 2 + 3;  // coarse window (prolongation sources)
     constexpr int NPT = FC * FR / BLOCK;
@@ -488,6 +499,7 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
             zown[m] = (ZM == 2 && flex) ? (double)zold[n] : 0.0;
         }
     }
+    const int go = gate_read(active);
     ld_tab_lds(ldt_, ldl);
     w.store([&](int a, int b, bool in, const RPin<TR>& v) {
         rs[a][b] = (T)v.r;
@@ -543,14 +555,16 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
 // then, in LDS: v_m = x_m + I y_c on the m-window, the level-m Jacobi y_m (not stored),
 // v_l = x_l + I y_m on the l-window and the level-l Jacobi of its nodes.
 constexpr int U2_R = 4, U2_C = 64;
-template <class T, class TR>
+template <class T, class TR, bool EARLY>
 __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc, const double* __restrict__ Leoff_f,
                                                  const double* __restrict__ Leoff_m, LdTab<T> ldf_, LdTab<T> ldm_,
                                                  const TR* __restrict__ rf, const uint8_t* __restrict__ pinf,
                                                  const T* __restrict__ rm, const uint8_t* __restrict__ pinm,
                                                  const T* __restrict__ yc, T* __restrict__ yf,
                                                  const int* __restrict__ active) {
-    const int go = gate_read(active);
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
     constexpr int LR = U2_R + 2, LC = U2_C + 2;            // level-l window (tile stencil)
     constexpr int MR = U2_R / 2 + 3, MC = U2_C / 2 + 3;    // m-nodes the l-window's prolongation reads
     constexpr int MXR = MR + 2, MXC = MC + 2;              // their stencil window
@@ -578,6 +592,7 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
     wm.load(gm, xmi0, xmj0, [&](int k) { return RPin<T>{rm[k], pinm[k]}; });
     Win<CRW, CCW, T> wc;
     wc.load(gc, ci0, cj0, [&](int k) { return yc[k]; });
+    const int go = gate_read(active);
     ld_tab_lds(ldf_, ldfl);
     ld_tab_lds(ldm_, ldml);
     wl.store([&](int a, int b, bool in, const RPin<TR>& v) {
@@ -630,14 +645,16 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
 // (jax_iterative_solvers.py:105-107), for a pinned non-periodic grid: a block owns FC x FR
 // nodes, loads z, p_old and the pins on the window one wider in one batch.  ZT: the type z is
 // stored in (the V-cycle's), widened to FP64 on load.
-template <int FC, int FR, class ZT>
+template <int FC, int FR, class ZT, bool EARLY>
 __global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restrict__ Le, const uint8_t* __restrict__ pin,
                                                 const ZT* __restrict__ z, const double* __restrict__ pold,
                                                 double* __restrict__ pnew, double* __restrict__ Ap,
                                                 const DevState* __restrict__ st, double* __restrict__ partials,
                                                 int pstride, const int* __restrict__ active) {
     static_assert((FC * FR) % BLOCK == 0, "whole nodes per thread");
-    const int go = gate_read(active);
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
     constexpr int WW = FC + 2, WR = FR + 2, NPT = FC * FR / BLOCK;
     struct ZP {
         double z, p;
@@ -652,6 +669,7 @@ __global__ __launch_bounds__(256) void k_cg_k1t(Grid2D g, const double* __restri
     const double beta = st->c_beta;
     Win<WR, WW, ZP> w;
     w.load(g, i0 - 1, j0 - 1, [&](int k) { return ZP{(double)z[k], pold[k], pin[k]}; });
+    const int go = gate_read(active);
     w.store([&](int a, int b, bool in, const ZP& v) {
         ps[a][b] = in ? v.z + beta * v.p : 0.0;
         pns[a][b] = v.pin;
@@ -699,7 +717,6 @@ __global__ __launch_bounds__(256) void k_cginit_us(Grid2D g, const ElemConst* __
                                                    double* __restrict__ partials, int pstride,
                                                    const int* __restrict__ gate) {
     static_assert(FC * FR == BLOCK, "one node per thread");
-    const int go = gate_read(gate);
     constexpr int PW = FC + 4, PR = FR + 4;  // P_n window: two wider
     constexpr int UW = FC + 2, UR = FR + 2;  // U_hat / M / pins / U* window: one wider
     struct UMP {
@@ -725,6 +742,7 @@ __global__ __launch_bounds__(256) void k_cginit_us(Grid2D g, const ElemConst* __
     Win<UR, UW, UMP> wu;
     wu.load(g, i0 - 1, j0 - 1, [&](int k) { return UMP{Uh[k], Uh[k + nn], M[k], pin[k]}; });
     const double fpo = Fp ? Fp[n] : 0.0;
+    const int go = gate_read(gate);
     wp.store([&](int a, int b, bool in, double v) { ps[a][b] = in ? v : 0.0; });
     wu.store([&](int a, int b, bool in, const UMP& v) {
         uhs[0][a][b] = v.u0;

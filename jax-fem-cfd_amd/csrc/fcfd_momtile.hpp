@@ -85,7 +85,7 @@ struct Val2M {
 // ------------------------------------------------------------------------------ K1
 // p = r + beta (p_old - omega q_old) (p = r at the first iteration), q = A p with the Dirichlet
 // rows replaced by p, block partial of rhat . q (BK1S, MT = false).
-template <class S, int FC, int FR>
+template <class S, int FC, int FR, bool EARLY>
 __global__ __launch_bounds__(256) void k_mom_k1t(MomCoef<S> op, const uint8_t* __restrict__ mask,
                                                  const double* __restrict__ r, const double* __restrict__ pold,
                                                  const double* __restrict__ qold, const double* __restrict__ rhat,
@@ -93,6 +93,9 @@ __global__ __launch_bounds__(256) void k_mom_k1t(MomCoef<S> op, const uint8_t* _
                                                  const DevState* __restrict__ st, double* __restrict__ partials,
                                                  int pstride, const int* __restrict__ active) {
     static_assert(FC * FR == BLOCK, "one node per thread");
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
     const int go = gate_read(active);
     constexpr int WW = FC + 2, WR = FR + 2;
     constexpr int WP = Win<WR, WW, Val2M>::HP;
@@ -159,7 +162,8 @@ __global__ __launch_bounds__(256) void k_mom_k1t(MomCoef<S> op, const uint8_t* _
 
 // ------------------------------------------------------------------------------ K2
 // s = r - alpha q (bicg_s), t = A s with the Dirichlet rows replaced by s, block partials of
-// s.s, t.s, t.t (BK2S, MT = false).  s is not stored.  TAG only separates the standalone timing.
+// s.s, t.s, t.t (BK2S, MT = false).  s is not stored.  TAG = 1: the standalone
+// timing copy; TAG = 2: an in-solver launch that also tests its gate at entry (EARLY).
 template <class S, int FC, int FR, int TAG = 0>
 __global__ __launch_bounds__(256) void k_mom_k2t(MomCoef<S> op, const uint8_t* __restrict__ mask,
                                                  const double* __restrict__ r, const double* __restrict__ q,
@@ -167,6 +171,9 @@ __global__ __launch_bounds__(256) void k_mom_k2t(MomCoef<S> op, const uint8_t* _
                                                  double* __restrict__ partials, int pstride,
                                                  const int* __restrict__ active) {
     static_assert(FC * FR == BLOCK, "one node per thread");
+    if constexpr (TAG == 2) {
+        if (*active == 0) return;
+    }
     const int go = gate_read(active);
     constexpr int WW = FC + 2, WR = FR + 2;
     struct V4 {
