@@ -274,6 +274,7 @@ void build_ctx3(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
     c.m_t = dalloc<double>(c, n3);
     c.m_x = dalloc<double>(c, n3);
     c.c_r = dalloc<double>(c, nn);
+    if (c.precond == FCFD_PC_APPROX_INV) c.c_r2 = dalloc<double>(c, nn);  // the fused x / r update's r_out
     c.c_Ap = dalloc<double>(c, nn);
     c.c_x = dalloc<double>(c, nn);
     c.c_rhs = dalloc<double>(c, nn);
@@ -727,17 +728,33 @@ struct CgK1S3 {
         acc[0] += X[S3C] * a;
     }
 };
-struct ApproxInv3aS {
+// ApproxInv3a, LDS-staged, with CgK2's x / r update (jax_iterative_solvers.py:108-109) fused in:
+// the window stages r_new / L_diag with r_new = r - alpha Ap formed as CgK2 forms it, and the block stores
+// x + alpha p and r_new of its own nodes (into rout: rin stays intact, the neighbouring blocks
+// stage it) with the partial of r_new.r_new
+struct ApproxInv3aK2S {
     static constexpr int NV = 1;
-    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; double* d1l;
-    __device__ __forceinline__ void stage(int id, double (&v)[1]) const { v[0] = r[id] / Ld[id]; }
+    Grid3D g; const double* Le; const double* rin; const double* Ap; const double* p; double* x; double* rout;
+    const double* Ld; const uint8_t* pin; double* d1l; const DevState* st;
+    __device__ __forceinline__ void stage(int id, double (&v)[1]) const {
+        const double rk = rin[id] - st->c_alpha * Ap[id];
+        v[0] = rk / Ld[id];
+    }
     template <class SV>
-    __device__ void operator()(int n, const SV& sv, double (&)[1]) const {
+    __device__ void operator()(int n, const SV& sv, double (&acc)[1]) const {
+        // the stencil before any store: a store ahead of it would force Le's uniform loads off the
+        // scalar path (the compiler cannot rule out that x or rout alias Le)
         Nbr3 s;
         node_elems3(g, n, s);
         double X[27];
         view27(sv, 0, n, X);
         const double v = ebe3(s, Le, X) / Ld[n];
+        const double a = st->c_alpha;
+        const double xn = x[n] + a * p[n];
+        const double rk = rin[n] - a * Ap[n];
+        x[n] = xn;
+        rout[n] = rk;
+        acc[0] += rk * rk;
         d1l[n] = (pin && pin[n]) ? 0.0 : v;
     }
 };
@@ -823,14 +840,21 @@ static int pcg3(Ctx& c, const double* b, const double* bmean, const double* x0in
                 finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb2, nb2, act, CgK3RFin{c.st, flex},
                                                                       c.partials, c.pstride);
             } else if (c.precond == FCFD_PC_APPROX_INV) {
-                k2();
-                launch_stencil3<0>(c, g, act, ApproxInv3aS{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x}, NoFin{});
+                // the x / r update rides on the first stage (its r.r partials in slot 2); r
+                // alternates between c_r (parity 0) and c_r2
+                const double* rin = o == 0 ? c.c_r : c.c_r2;
+                double* rout = o == 0 ? c.c_r2 : c.c_r;
                 const dim3 t = s3t_tiles(g);
                 const int nb3 = (int)(t.x * t.y * t.z);
+                if (nb3 > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+                stencil3t_kernel<1><<<nb3, BLOCK, 0, c.stream>>>(
+                    g, (int)t.x, (int)t.y, act,
+                    ApproxInv3aK2S{g, Le, rin, c.c_Ap, c.c_p[nw], c.c_x, rout, Ld, c.ppin, c.lv[0].x, c.st},
+                    c.partials + 2 * (size_t)c.pstride, c.pstride);
                 stencil3t_kernel<2><<<nb3, BLOCK, 0, c.stream>>>(
-                    g, (int)t.x, (int)t.y, act, ApproxInv3bK3S{g, Le, c.c_r, Ld, c.ppin, c.lv[0].x, c.c_z[nw]},
+                    g, (int)t.x, (int)t.y, act, ApproxInv3bK3S{g, Le, rout, Ld, c.ppin, c.lv[0].x, c.c_z[nw]},
                     c.partials, c.pstride);
-                finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb3, nb2, act, CgK3RFin{c.st, 0}, c.partials,
+                finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nb3, nb3, act, CgK3RFin{c.st, 0}, c.partials,
                                                                       c.pstride);
             } else {
                 launch_node<2>(c, nn, act,
