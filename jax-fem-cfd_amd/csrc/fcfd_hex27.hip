@@ -876,6 +876,57 @@ struct ApproxInv27b {
     }
 };
 
+// The PCG iteration's forms of the two stages (pinned form, no flexible term).  Stage 1 with
+// CgK2's x / r update (jax_iterative_solvers.py:108-109) fused in: the window stages r_new /
+// L_diag with r_new = r - alpha Ap formed as CgK2 forms it, and the block stores x + alpha p and
+// r_new of its own nodes (into rout: rin stays intact, the neighbouring blocks stage it) with the
+// partial of r_new.r_new.  The stencil is formed before any store: a store ahead of it would force
+// Le's uniform loads off the scalar path (the compiler cannot rule out that x or rout alias Le).
+struct ApproxInv27aK2 {
+    static constexpr int NV = 1, NC = 1;
+    Grid3D g; const double* Le; const double* rin; const double* Ap; const double* p; double* x; double* rout;
+    const double* Ld; const uint8_t* pin; double* d1l; const DevState* st;
+    __device__ void stage(int id, double (&v)[NV]) const {
+        const double rk = rin[id] - st->c_alpha * Ap[id];
+        v[0] = rk / Ld[id];
+    }
+    __device__ __forceinline__ void row(int a, int, double (&cf)[27]) const {
+#pragma unroll
+        for (int b = 0; b < 27; ++b) cf[b] = Le[a * 27 + b];
+    }
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[1]) const {
+        const double v = h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix) / Ld[n];
+        const double a = st->c_alpha;
+        const double xn = x[n] + a * p[n];
+        const double rk = rin[n] - a * Ap[n];
+        x[n] = xn;
+        rout[n] = rk;
+        acc[0] += rk * rk;
+        d1l[n] = (pin && pin[n]) ? 0.0 : v;
+    }
+};
+// Stage 2 with the CG update that follows it (CgK3 with no mean and no flexible term): z_new = M r
+// stored, partial r.z_new
+struct ApproxInv27bK3 {
+    static constexpr int NV = 1, NC = 1;
+    Grid3D g; const double* Le; const double* r; const double* Ld; const uint8_t* pin; const double* d1l; double* z;
+    __device__ void stage(int id, double (&v)[NV]) const { v[0] = d1l[id]; }
+    __device__ __forceinline__ void row(int a, int, double (&cf)[27]) const {
+#pragma unroll
+        for (int b = 0; b < 27; ++b) cf[b] = Le[a * 27 + b];
+    }
+    template <class SM, class IX>
+    __device__ void operator()(int n, const Inc27& s, const IX& ix, SM sm, double (&acc)[2]) const {
+        double l2 = h27_const_ix(g, s, Le, [&](int id) { return sm[0][id]; }, ix);
+        if (pin && pin[n]) l2 = 0.0;
+        const double d1x = r[n] / Ld[n];
+        const double zk = (3.0 * d1x - 3.0 * d1l[n]) + l2 / Ld[n];
+        z[n] = zk;
+        acc[0] += r[n] * zk;
+    }
+};
+
 // ------------------------------------------------------------------------------ multigrid
 // The box-grid V-cycle (multigrid.py:231-330 carried to 3D) on hex27 levels of nex >> l elements:
 // the level operators are the 27 x 27 hex27 Laplacians, the transfers the node-grid trilinear P
@@ -1120,6 +1171,24 @@ static int pcg_h27(Ctx& c, const double* b, const double* bmean, const double* x
                                CgK23{c.c_p[nw], c.c_Ap, c.precond == FCFD_PC_JACOBI ? Ld : nullptr, c.c_x, c.c_r,
                                      c.c_z[nw], c.st},
                                CgK23Fin{c.st});
+            } else if (c.precond == FCFD_PC_APPROX_INV && c.coarse_inv32 == nullptr) {
+                // the x / r update rides on stage 1 (its r.r partials in slot 2) and the z-step on
+                // stage 2; r alternates between c_r (parity 0) and c_r2
+                const double* rin = o == 0 ? c.c_r : c.c_r2;
+                double* rout = o == 0 ? c.c_r2 : c.c_r;
+                const int ntx = (g.nnx + C27X - 1) / C27X, nty = (g.nny + C27Y - 1) / C27Y,
+                          ntz = (g.nnz + C27Z - 1) / C27Z;
+                const int nbt = ntx * nty * ntz;
+                if (nbt > c.pstride) throw Error(FCFD_ERR_STATE, "reduction partials buffer too small");
+                k_ctile27<1><<<nbt, C27T, 0, c.stream>>>(
+                    g, ntx, nty, act,
+                    ApproxInv27aK2{g, Le, rin, c.c_Ap, c.c_p[nw], c.c_x, rout, Ld, c.ppin, c.lv[0].x, c.st},
+                    c.partials + 2 * (size_t)c.pstride, c.pstride);
+                k_ctile27<2><<<nbt, C27T, 0, c.stream>>>(
+                    g, ntx, nty, act, ApproxInv27bK3{g, Le, rout, Ld, c.ppin, c.lv[0].x, c.c_z[nw]}, c.partials,
+                    c.pstride);
+                finalize2_kernel<2, 1><<<1, FIN_BLOCK, 0, c.stream>>>(nbt, nbt, act, CgK3RFin{c.st, 0}, c.partials,
+                                                                      c.pstride);
             } else if (neu) {
                 launch_node<2>(c, nn, act, CgK2{c.c_p[nw], c.c_Ap, c.c_x, c.c_r, c.st}, CgK2Fin{c.st, nn});
                 const double* y = precondition_h27(c, act);
@@ -1478,6 +1547,7 @@ void build_ctx_h27(Ctx& c, const fcfd_mesh_desc* m, const fcfd_solver_desc* s) {
     c.m_t = dalloc<double>(c, n3);
     c.m_x = dalloc<double>(c, n3);
     c.c_r = dalloc<double>(c, nn);
+    if (c.precond == FCFD_PC_APPROX_INV) c.c_r2 = dalloc<double>(c, nn);  // the fused x / r update's r_out
     c.c_Ap = dalloc<double>(c, nn);
     c.c_x = dalloc<double>(c, nn);
     c.c_rhs = dalloc<double>(c, nn);
