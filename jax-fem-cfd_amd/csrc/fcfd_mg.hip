@@ -950,6 +950,22 @@ static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTa
     for (int l = ld - 1; l >= 0;) {
         Level& L = c.lv[l];
         Level& Lc = c.lv[l + 1];
+        if (l >= 3 && c.lv[l - 1].g.nn < (1 << 17) && c.lv[l - 2].g.nn >= (1 << 17)) {
+            // levels l, l - 1 and l - 2 in one kernel, where a pair (l, l - 1) would leave level
+            // l - 2 to a one-level kernel of its own (1024^2: levels 3, 2, 1)
+            Level& Lm = c.lv[l - 1];
+            Level& Lf = c.lv[l - 2];
+            const dim3 grid((Lf.g.nnx + U3_C - 1) / U3_C, (Lf.g.nny + U3_R - 1) / U3_R);
+            with_gate_mode(c, active, [&](auto e) {
+                k_mgt_up3<T, decltype(e)::value><<<grid, 256, 0, c.stream>>>(
+                    Lf.g, Lm.g, L.g, Lc.g, Lf.ec_dev->Leoff, Lm.ec_dev->Leoff, L.ec_dev->Leoff, ld_tab<T>(Lf),
+                    ld_tab<T>(Lm), ld_tab<T>(L), B::r(Lf), pn(Lf), B::r(Lm), pn(Lm), B::r(L), pn(L), B::y(Lc),
+                    B::y(Lf), active);
+            });
+            FCFD_CHECK_LAUNCH();
+            l -= 3;
+            continue;
+        }
         if (l >= 1 && c.lv[l - 1].g.nn < (1 << 17)) {
             Level& Lf = c.lv[l - 1];
             const dim3 grid((Lf.g.nnx + U2_C - 1) / U2_C, (Lf.g.nny + U2_R - 1) / U2_R);

@@ -640,6 +640,138 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
                                               [&](int p, int q) { return vl[bj + p][bi + q]; });
 }
 
+// ------------------------------------------------------------------------------ up leg, three levels
+// Levels c = l + 3 -> n = l + 2 -> m = l + 1 -> l in one kernel, for an l too large for the
+// two-level kernel's next pairing (the level below the two-level pairs): a block owns a U3_R x U3_C
+// tile of level-l nodes and recomputes, in LDS, everything its Jacobi depends on -- v_n = x_n +
+// I y_c on the n-window and the level-n Jacobi y_n on the n-nodes the m-window's prolongation
+// reads, then v_m and y_m the same way, then v_l and the level-l Jacobi of its own nodes.  Every
+// value is formed by the operands and order of the one-level kernel, so y_l is bitwise the chain
+// of three one-level up legs.
+constexpr int U3_R = 4, U3_C = 64;
+template <class T, bool EARLY>
+__global__ __launch_bounds__(256) void k_mgt_up3(Grid2D gf, Grid2D gm, Grid2D gn, Grid2D gc,
+                                                 const double* __restrict__ Leoff_f, const double* __restrict__ Leoff_m,
+                                                 const double* __restrict__ Leoff_n, LdTab<T> ldf_, LdTab<T> ldm_,
+                                                 LdTab<T> ldn_, const T* __restrict__ rf, const uint8_t* __restrict__ pinf,
+                                                 const T* __restrict__ rm, const uint8_t* __restrict__ pinm,
+                                                 const T* __restrict__ rn, const uint8_t* __restrict__ pinn,
+                                                 const T* __restrict__ yc, T* __restrict__ yf,
+                                                 const int* __restrict__ active) {
+    static_assert(U3_R * U3_C == BLOCK && U3_R % 2 == 0 && U3_C % 2 == 0,
+                  "one node per thread; even tile sides (the windows start at tile origin / 2 - 1)");
+    if constexpr (EARLY) {
+        if (*active == 0) return;
+    }
+    constexpr int LR = U3_R + 2, LC = U3_C + 2;          // level-l window (tile stencil)
+    constexpr int MR = U3_R / 2 + 3, MC = U3_C / 2 + 3;  // m-nodes the l-window's prolongation reads
+    constexpr int MXR = MR + 2, MXC = MC + 2;            // their stencil window
+    constexpr int NR = MXR / 2 + 3, NC = MXC / 2 + 3;    // n-nodes the m-window's prolongation reads
+    constexpr int NXR = NR + 2, NXC = NC + 2;            // their stencil window
+    constexpr int CRW = NXR / 2 + 3, CCW = NXC / 2 + 3;  // coarse window of the n-window's prolongation
+    __shared__ T ldfl[9], ldml[9], ldnl[9];
+    constexpr int LP = Win<LR, LC, RPin<T>>::HP, MP = Win<MXR, MXC, RPin<T>>::HP;
+    constexpr int NP = Win<NXR, NXC, RPin<T>>::HP, CP = Win<CRW, CCW, T>::HP;
+    __shared__ T rl[LP][LC];
+    __shared__ uint8_t pl[LP][LC];  // 1 pinned, 2 outside the grid
+    __shared__ T vl[LR][LC];
+    __shared__ T rms[MP][MXC];
+    __shared__ uint8_t pms[MP][MXC];
+    __shared__ T vm[MXR][MXC];
+    __shared__ T ym[MR][MC];
+    __shared__ T rns[NP][NXC];
+    __shared__ uint8_t pns[NP][NXC];
+    __shared__ T vn[NXR][NXC];
+    __shared__ T yn[NR][NC];
+    __shared__ T ycs[CP][CCW];
+    const int nbx = gridDim.x;
+    const int lb = xcd_block(blockIdx.y * nbx + blockIdx.x, nbx * gridDim.y);
+    const int j0 = (lb / nbx) * U3_R, i0 = (lb - (lb / nbx) * nbx) * U3_C;
+    const int li0 = i0 - 1, lj0 = j0 - 1;
+    const int mj0 = (j0 - 1) >> 1, mi0 = (i0 - 1) >> 1;  // first m-node of the l-window's prolongation
+    const int xmj0 = mj0 - 1, xmi0 = mi0 - 1;              // its stencil window
+    const int nj0 = (xmj0 >> 1) - 1, ni0 = (xmi0 >> 1) - 1;  // first n-node of the m-window's prolongation
+    const int xnj0 = nj0 - 1, xni0 = ni0 - 1;
+    const int cj0 = (xnj0 >> 1) - 1, ci0 = (xni0 >> 1) - 1;
+    Win<LR, LC, RPin<T>> wl;
+    wl.load(gf, li0, lj0, [&](int k) { return RPin<T>{rf[k], pinf[k]}; });
+    Win<MXR, MXC, RPin<T>> wm;
+    wm.load(gm, xmi0, xmj0, [&](int k) { return RPin<T>{rm[k], pinm[k]}; });
+    Win<NXR, NXC, RPin<T>> wn;
+    wn.load(gn, xni0, xnj0, [&](int k) { return RPin<T>{rn[k], pinn[k]}; });
+    Win<CRW, CCW, T> wc;
+    wc.load(gc, ci0, cj0, [&](int k) { return yc[k]; });
+    const int go = gate_read(active);
+    ld_tab_lds(ldf_, ldfl);
+    ld_tab_lds(ldm_, ldml);
+    ld_tab_lds(ldn_, ldnl);
+    wl.store([&](int a, int b, bool in, const RPin<T>& v) {
+        rl[a][b] = v.r;
+        pl[a][b] = in ? (v.p != 0 ? 1 : 0) : 2;
+    });
+    wm.store([&](int a, int b, bool in, const RPin<T>& v) {
+        rms[a][b] = v.r;
+        pms[a][b] = in ? (v.p != 0 ? 1 : 0) : 2;
+    });
+    wn.store([&](int a, int b, bool in, const RPin<T>& v) {
+        rns[a][b] = v.r;
+        pns[a][b] = in ? (v.p != 0 ? 1 : 0) : 2;
+    });
+    wc.store([&](int a, int b, bool, T v) { ycs[a][b] = v; });
+    __syncthreads();
+    if (!gate_open(go)) return;
+    for (int t = threadIdx.x; t < NXR * NXC; t += BLOCK) {
+        const int a = t / NXC, b = t - a * NXC;
+        const int ni = xni0 + b, nj = xnj0 + a;
+        const T x = (T(2.0 / 3.0) * rns[a][b]) / ld_at(ldnl, gn, ni, nj);
+        const T iy = interp_pt<T>(ni, nj, [&](int J, int I) { return ycs[J - cj0][I - ci0]; });
+        vn[a][b] = pns[a][b] == 0 ? x + iy : T(0);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NR * NC; t += BLOCK) {
+        const int a = t / NC, b = t - a * NC;
+        const int ni = ni0 + b, nj = nj0 + a;
+        T v = T(0);
+        if (pns[a + 1][b + 1] != 2)
+            v = pinned_jacobi_at(gn, ni, nj, Leoff_n, rns[a + 1][b + 1], ld_at(ldnl, gn, ni, nj),
+                                 pns[a + 1][b + 1] != 0, [&](int p, int q) { return vn[a + p][b + q]; });
+        yn[a][b] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < MXR * MXC; t += BLOCK) {
+        const int a = t / MXC, b = t - a * MXC;
+        const int mi = xmi0 + b, mj = xmj0 + a;
+        const T x = (T(2.0 / 3.0) * rms[a][b]) / ld_at(ldml, gm, mi, mj);
+        const T iy = interp_pt<T>(mi, mj, [&](int J, int I) { return yn[J - nj0][I - ni0]; });
+        vm[a][b] = pms[a][b] == 0 ? x + iy : T(0);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < MR * MC; t += BLOCK) {
+        const int a = t / MC, b = t - a * MC;
+        const int mi = mi0 + b, mj = mj0 + a;
+        T v = T(0);
+        if (pms[a + 1][b + 1] != 2)
+            v = pinned_jacobi_at(gm, mi, mj, Leoff_m, rms[a + 1][b + 1], ld_at(ldml, gm, mi, mj),
+                                 pms[a + 1][b + 1] != 0, [&](int p, int q) { return vm[a + p][b + q]; });
+        ym[a][b] = v;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < LR * LC; t += BLOCK) {
+        const int a = t / LC, b = t - a * LC;
+        const int fi = li0 + b, fj = lj0 + a;
+        const T x = (T(2.0 / 3.0) * rl[a][b]) / ld_at(ldfl, gf, fi, fj);
+        const T iy = interp_pt<T>(fi, fj, [&](int J, int I) { return ym[J - mj0][I - mi0]; });
+        vl[a][b] = pl[a][b] == 0 ? x + iy : T(0);
+    }
+    __syncthreads();
+    const int bj = threadIdx.x / U3_C, bi = threadIdx.x % U3_C;
+    const int j = j0 + bj, i = i0 + bi;
+    if (i < gf.nnx && j < gf.nny)
+        yf[j * gf.nnx + i] = pinned_jacobi_at(gf, i, j, Leoff_f, rl[bj + 1][bi + 1], ld_at(ldfl, gf, i, j),
+                                              pl[bj + 1][bi + 1] != 0,
+                                              [&](int p, int q) { return vl[bj + p][bi + q]; });
+}
+
 // ------------------------------------------------------------------------------ PCG K1, tiled
 // p = z + beta p_old (CgK1S's staged value), Ap = L p (pinned rows 0), block partial of p . Ap
 // (jax_iterative_solvers.py:105-107), for a pinned non-periodic grid: a block owns FC x FR
