@@ -554,7 +554,7 @@ __global__ __launch_bounds__(256) void k_mgt_up(Grid2D gf, Grid2D gc, const doub
 // the m-nodes the tile's prolongation reads, and y_c on the window their own prolongation reads;
 // then, in LDS: v_m = x_m + I y_c on the m-window, the level-m Jacobi y_m (not stored),
 // v_l = x_l + I y_m on the l-window and the level-l Jacobi of its nodes.
-constexpr int U2_R = 4, U2_C = 64;
+constexpr int U2_R = 8, U2_C = 32;
 template <class T, class TR, bool EARLY>
 __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc, const double* __restrict__ Leoff_f,
                                                  const double* __restrict__ Leoff_m, LdTab<T> ldf_, LdTab<T> ldm_,
@@ -648,7 +648,7 @@ __global__ __launch_bounds__(256) void k_mgt_up2(Grid2D gf, Grid2D gm, Grid2D gc
 // reads, then v_m and y_m the same way, then v_l and the level-l Jacobi of its own nodes.  Every
 // value is formed by the operands and order of the one-level kernel, so y_l is bitwise the chain
 // of three one-level up legs.
-constexpr int U3_R = 4, U3_C = 64;
+constexpr int U3_R = 8, U3_C = 32;
 template <class T, bool EARLY>
 __global__ __launch_bounds__(256) void k_mgt_up3(Grid2D gf, Grid2D gm, Grid2D gn, Grid2D gc,
                                                  const double* __restrict__ Leoff_f, const double* __restrict__ Leoff_m,
