@@ -866,7 +866,8 @@ static bool z_f32(const Ctx& c) { return c.precision == FCFD_PREC_MIXED && cg_fu
 
 // The pinned V-cycle on a non-periodic grid with the tiled kernels (fcfd_mgtile.hpp): two small
 // levels per kernel on each leg below 200 node rows (down) / 2^17 nodes (up), one level per
-// kernel above.  With `tail` the level-0 up leg (when it runs alone) also runs the PCG's z-step;
+// kernel above; on the up leg, three levels in one kernel where a pair would leave the level
+// below it (at least 2^17 nodes) to a kernel of its own.  With `tail` the level-0 up leg (when it runs alone) also runs the PCG's z-step;
 // the return value says so.
 template <class T>
 static bool vcycle_tiled(Ctx& c, const double* r0, const int* active, const CgTail* tail) {
