@@ -125,3 +125,22 @@ def test_neumann_poisson_1024(precision):
         assert k == kr and ns.rel_l2(xg, xr) < 1e-10
     else:
         assert err < MIXED_FIELD_TOL and ns.rel_l2(xg, xr) < MIXED_FIELD_TOL
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-5)])
+def test_vcycle_1024_pinned(precision, tol):
+    """One pinned V-cycle on the 1024^2 hierarchy (7 levels to 17^2) against the oracle's: the
+    launch shapes of the benchmark's preconditioner, including the up leg that runs levels 3, 2
+    and 1 in one kernel.  FP64: 1e-12 relative L2 (the same bound as the small-grid V-cycle test);
+    mixed (FP32 levels, exact FP32 coarse inverse against the oracle's coarse CG): 2e-5."""
+    from jax_fem_cfd._context import Context
+    s, *_ = ns.build([N, N], [1.0, 1.0], False, [], [], 1e-12, True, "multigrid", [16, 16])
+    print(f"\n[case] 1024^2 pinned V-cycle, {precision}", flush=True)
+    c = Context(N, N, 1.0, 1.0, False, tol=1e-12, set_zeroP=True, precond="multigrid", mg_levels=len(s.levels),
+                precision=precision, gamma_p=[0], mg_gamma_p=[L["gamma_p"] for L in s.levels])
+    r = np.random.default_rng(7).standard_normal(s.nn)
+    r[0] = 0
+    ref = s.M_op(r)
+    got = _np(c.vcycle(r))
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err < tol, err
